@@ -1,0 +1,430 @@
+// ebem_kernels.hpp -- gfx950 kernels of the single-layer hot path.
+//
+// Assembly (replaces SAc/BEMenerg_coreSA_calcMatrixBlock.m and the kernelV
+// launches of dir/BEMenerg_dir_timeMarchingD.m):
+//   k_windows      per (test i, field j) pair: the contiguous lag window that
+//                  passes the light-cone test of calcMatrixBlock.m:57-61;
+//   k_rowcount     per test row: number of blocks per (row, lag) and of
+//                  (pair, level) items;
+//   k_pair_items   per row: item offsets of every pair and the item -> pair map;
+//   k_levels       per (pair, level m) item: W_m = sum_q w_q * integral at
+//                  t_m = m dt over the field triangle (the time-level term of
+//                  calcMatrixSubBlock.m:259-337), rotated to the global frame.
+//                  Each W_m serves the three lags m-1, m, m+1, so the
+//                  analytic work is a third of the reference's;
+//   k_emit         per row: E_l = (W_{l+1} - 2 W_l + W_{l-1}) / (4 pi rho) with
+//                  the 1e-14 cleanup (calcMatrixBlock.m:77), written into the
+//                  row-lag-segmented block store (symmetric 3x3 -> 6 values).
+// MOT (SA/BEMenerg_coreSA_timeMarching.m:70-113):
+//   k_history      r_n = beta_n - sum_{l>=1} E_l alpha_{n-l}, one workgroup per
+//                  test row, streaming the row's contiguous lag segments;
+//   k_gemv_rows    alpha_n = E_0^{-1} r_n with a row-major inverse, one wave
+//                  per row.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ebem_sa.hpp"
+
+namespace ebem {
+
+// Block store: six value arrays (w00,w01,w02,w11,w12,w22) of nnzb entries,
+// column (field triangle) index per block, and segment offsets
+// seg[i*L + l] .. seg[i*L + l + 1] of the blocks of test row i at lag l.
+struct Store {
+    double* v[6];
+    int32_t* col;
+    const int64_t* seg;
+};
+
+struct AsmParams {
+    int N, L;
+    double dt, cP, cS, rho;
+    int nGH;
+};
+
+// (lag-1)*cS*dt < dmax && (lag+1)*cP*dt >= dmin  (calcMatrixBlock.m:61), same
+// operation order, no contraction.
+__device__ inline bool cull_pass(int lag, double cS, double cP, double dt, double dmin, double dmax) {
+#pragma clang fp contract(off)
+    return ((double)(lag - 1) * cS * dt < dmax) && ((double)(lag + 1) * cP * dt >= dmin);
+}
+
+__device__ inline void pair_dist(const FieldTri& S, const FieldTri& F, double& dmin, double& dmax) {
+#pragma clang fp contract(off)
+    double v0 = F.center[0] - S.center[0], v1 = F.center[1] - S.center[1], v2 = F.center[2] - S.center[2];
+    double d = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    dmin = d - F.maxL - S.maxL;
+    dmax = d + F.maxL + S.maxL;
+}
+
+// Lag window [lo, hi] (clipped to [0, L-1]); lo > hi when empty.
+__device__ inline void lag_window(const AsmParams& P, double dmin, double dmax, int& lo, int& hi) {
+    // upper end: largest lag with (lag-1) cS dt < dmax
+    double uh = dmax / (P.cS * P.dt) + 1.0;
+    int h = (int)fmin(uh, (double)(P.L + 2));
+    if (h > P.L - 1) h = P.L - 1;
+    while (h >= 0 && !((double)(h - 1) * P.cS * P.dt < dmax)) h--;
+    while (h + 1 <= P.L - 1 && ((double)(h) * P.cS * P.dt < dmax)) h++;
+    // lower end: smallest lag with (lag+1) cP dt >= dmin
+    double ul = dmin / (P.cP * P.dt) - 1.0;
+    int l = (int)fmax(0.0, fmin(ul, (double)(P.L + 2)));
+    while (l > 0 && ((double)(l) * P.cP * P.dt >= dmin)) l--;
+    while (l <= P.L - 1 && !((double)(l + 1) * P.cP * P.dt >= dmin)) l++;
+    lo = l;
+    hi = h;
+}
+
+__device__ inline int64_t pair_items(int lo, int hi) {
+    if (lo > hi) return 0;
+    int mlo = lo - 1 < 1 ? 1 : lo - 1;
+    return (int64_t)(hi + 1 - mlo + 1);
+}
+
+// grid: ceil(rows*N/256); pairs p = (i - r0)*N + j for rows [r0, r1).
+__global__ void __launch_bounds__(256) k_windows(const FieldTri* __restrict__ tri, AsmParams P, int r0, int r1,
+                                                 int16_t* __restrict__ lo, int16_t* __restrict__ hi) {
+    int64_t np = (int64_t)(r1 - r0) * P.N;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int i = r0 + (int)(p / P.N), j = (int)(p % P.N);
+        double dmin, dmax;
+        pair_dist(tri[i], tri[j], dmin, dmax);
+        int l, h;
+        lag_window(P, dmin, dmax, l, h);
+        lo[p] = (int16_t)l;
+        hi[p] = (int16_t)h;
+    }
+}
+
+// One workgroup per row (rows [r0, r1)): cnt[(i-r0)*L + l] = blocks of row i at lag l,
+// items[i - r0] = (pair, level) items of the row.
+__global__ void __launch_bounds__(256) k_rowcount(const int16_t* __restrict__ lo, const int16_t* __restrict__ hi,
+                                                  int N, int L, int r0, int r1, int64_t* __restrict__ cnt,
+                                                  int64_t* __restrict__ items) {
+    extern __shared__ int sdiff[];  // L + 1 ints
+    __shared__ long long sitems;
+    int row = r0 + blockIdx.x;
+    if (row >= r1) return;
+    for (int l = threadIdx.x; l <= L; l += blockDim.x) sdiff[l] = 0;
+    if (threadIdx.x == 0) sitems = 0;
+    __syncthreads();
+    long long my = 0;
+    const int16_t* plo = lo + (int64_t)blockIdx.x * N;
+    const int16_t* phi = hi + (int64_t)blockIdx.x * N;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        int a = plo[j], b = phi[j];
+        if (a <= b) {
+            atomicAdd(&sdiff[a], 1);
+            atomicAdd(&sdiff[b + 1], -1);
+            my += pair_items(a, b);
+        }
+    }
+    atomicAdd((unsigned long long*)&sitems, (unsigned long long)my);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int l = 0; l < L; l++) {
+            run += sdiff[l];
+            cnt[(int64_t)blockIdx.x * L + l] = run;
+        }
+        items[blockIdx.x] = sitems;
+    }
+}
+
+// One workgroup (256 threads) per row of the chunk: item offsets per pair
+// (base[row] + exclusive prefix within the row) and the item -> pair map.
+__global__ void __launch_bounds__(256) k_pair_items(const int16_t* __restrict__ lo, const int16_t* __restrict__ hi,
+                                                    int N, const int64_t* __restrict__ rowbase,
+                                                    int64_t* __restrict__ off, int32_t* __restrict__ item_pair) {
+    __shared__ long long part[256];
+    const int r = blockIdx.x;
+    const int nper = (N + blockDim.x - 1) / blockDim.x;
+    const int j0 = threadIdx.x * nper;
+    const int64_t pbase = (int64_t)r * N;
+    long long s = 0;
+    for (int k = 0; k < nper; k++) {
+        int j = j0 + k;
+        if (j < N) s += pair_items(lo[pbase + j], hi[pbase + j]);
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    // inclusive scan over 256 partials (Hillis-Steele)
+    for (int d = 1; d < 256; d <<= 1) {
+        long long v = (threadIdx.x >= d) ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    long long run = rowbase[r] + (threadIdx.x ? part[threadIdx.x - 1] : 0);
+    for (int k = 0; k < nper; k++) {
+        int j = j0 + k;
+        if (j >= N) break;
+        int64_t p = pbase + j;
+        int64_t n = pair_items(lo[p], hi[p]);
+        off[p] = run;
+        for (int64_t q = 0; q < n; q++) item_pair[run + q] = (int32_t)p;
+        run += n;
+    }
+}
+
+// Persistent grid-stride kernel over (pair, level) items of a row chunk.
+// W[item*6 + q]: symmetric 3x3 in the global frame, NOT yet divided by 4 pi rho.
+template <int NGH>
+__global__ void __launch_bounds__(256) k_levels(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
+                                                AsmParams P, int r0, int64_t nitems,
+                                                const int32_t* __restrict__ item_pair,
+                                                const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
+                                                double* __restrict__ W) {
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < nitems;
+         it += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t p = item_pair[it];
+        const int i = r0 + p / P.N, j = p % P.N;
+        const int l0 = lo[p];
+        const int mlo = l0 - 1 < 1 ? 1 : l0 - 1;
+        const int m = mlo + (int)(it - off[p]);
+        const FieldTri& F = tri[j];
+        const double* g = gh + (int64_t)i * 4 * 32;  // 28 nodes max: x,y,z rows then weights
+        Level K = make_level((double)m * P.dt, P.cP, P.cS);
+        NodeAcc acc;
+        acc.s = 0.0;
+        for (int q = 0; q < 6; q++) acc.m[q] = 0.0;
+#pragma unroll 1
+        for (int q = 0; q < NGH; q++) {
+            double x[3] = {g[q], g[32 + q], g[64 + q]};
+            NodeGeo G = node_geo(x, F);
+            node_eval(G, K, g[96 + q], acc);
+        }
+        double Wv[6];
+        finish_rotation(acc, F.C, Wv);
+        double* o = W + it * 6;
+#pragma unroll
+        for (int q = 0; q < 6; q++) o[q] = Wv[q];
+    }
+}
+
+// One wave (64 threads) per row of the chunk.  Writes the blocks of lags
+// lo..hi of every pair of the row into the segmented store.
+__global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, const int16_t* __restrict__ hi,
+                                             const int64_t* __restrict__ off, const double* __restrict__ W,
+                                             int N, int L, int r0, double inv4pirho, Store S) {
+    extern __shared__ int64_t run[];  // L entries
+    const int r = blockIdx.x;
+    const int i = r0 + r;
+    const int lane = threadIdx.x;
+    for (int l = lane; l < L; l += 64) run[l] = 0;
+    __syncthreads();
+    const int64_t pbase = (int64_t)r * N;
+    for (int j0 = 0; j0 < N; j0 += 64) {
+        int j = j0 + lane;
+        int a = 1 << 30, b = -1;
+        int64_t o = 0;
+        if (j < N) {
+            a = lo[pbase + j];
+            b = hi[pbase + j];
+            o = off[pbase + j];
+        }
+        int amin = a, bmax = b;
+        for (int d = 32; d >= 1; d >>= 1) {
+            amin = min(amin, __shfl_xor(amin, d));
+            bmax = max(bmax, __shfl_xor(bmax, d));
+        }
+        const int mlo = a - 1 < 1 ? 1 : a - 1;
+        for (int l = amin; l <= bmax; l++) {
+            bool in = (a <= l && l <= b);
+            unsigned long long mask = __ballot(in);
+            int64_t base = run[l];
+            if (in) {
+                int rank = __popcll(mask & ((1ull << lane) - 1ull));
+                int64_t pos = S.seg[(int64_t)i * L + l] + base + rank;
+                const double* w1 = W + (o + (l + 1 - mlo)) * 6;
+                double e[6];
+#pragma unroll
+                for (int q = 0; q < 6; q++) e[q] = 1.0 * w1[q];
+                if (l >= 1) {
+                    const double* w0 = W + (o + (l - mlo)) * 6;
+#pragma unroll
+                    for (int q = 0; q < 6; q++) e[q] = e[q] + (-2.0) * w0[q];
+                }
+                if (l >= 2) {
+                    const double* wm = W + (o + (l - 1 - mlo)) * 6;
+#pragma unroll
+                    for (int q = 0; q < 6; q++) e[q] = e[q] + wm[q];
+                }
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    double v = e[q] * inv4pirho;
+                    S.v[q][pos] = (fabs(v) < 1.0e-14) ? 0.0 : v;
+                }
+                S.col[pos] = j;
+            }
+            __syncthreads();
+            if (lane == 0) run[l] = base + __popcll(mask);
+            __syncthreads();
+        }
+    }
+}
+
+// Dense export of lag l: out (3N x 3N col-major), one thread per block of the segment.
+__global__ void k_dense_lag(Store S, int N, int L, int lag, double* __restrict__ out) {
+    int i = blockIdx.y;
+    int64_t b0 = S.seg[(int64_t)i * L + lag], b1 = S.seg[(int64_t)i * L + lag + 1];
+    int64_t n3 = 3 * (int64_t)N;
+    for (int64_t b = b0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < b1; b += (int64_t)gridDim.x * blockDim.x) {
+        int j = S.col[b];
+        const int ii[9] = {0, 0, 0, 1, 1, 1, 2, 2, 2}, jj[9] = {0, 1, 2, 0, 1, 2, 0, 1, 2};
+        const int qq[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+        for (int k = 0; k < 9; k++) out[(3 * (int64_t)i + ii[k]) + n3 * (3 * (int64_t)j + jj[k])] = S.v[qq[k]][b];
+    }
+}
+
+// ---------------------------------------------------------------- MOT --
+
+// r = beta_n - sum_{l=1}^{lmax} E_l alpha_{n-l}; one workgroup of 256 per test row.
+// alpha: 3N x nT column-major (column n contiguous).
+__global__ void __launch_bounds__(256) k_history(Store S, int N, int L, int n, const double* __restrict__ beta,
+                                                 const double* __restrict__ alpha, double* __restrict__ r) {
+    const int i = blockIdx.x;
+    const int64_t n3 = 3 * (int64_t)N;
+    const int lmax = min(n, L - 1);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int l = 1; l <= lmax; l++) {
+        const int64_t b0 = S.seg[(int64_t)i * L + l], b1 = S.seg[(int64_t)i * L + l + 1];
+        const double* al = alpha + (int64_t)(n - l) * n3;
+        for (int64_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+            const int j = S.col[b];
+            const double x0 = al[3 * j], x1 = al[3 * j + 1], x2 = al[3 * j + 2];
+            const double w00 = S.v[0][b], w01 = S.v[1][b], w02 = S.v[2][b];
+            const double w11 = S.v[3][b], w12 = S.v[4][b], w22 = S.v[5][b];
+            a0 += w00 * x0 + w01 * x1 + w02 * x2;
+            a1 += w01 * x0 + w11 * x1 + w12 * x2;
+            a2 += w02 * x0 + w12 * x1 + w22 * x2;
+        }
+    }
+    // workgroup reduction (4 waves)
+    for (int d = 32; d >= 1; d >>= 1) {
+        a0 += __shfl_xor(a0, d);
+        a1 += __shfl_xor(a1, d);
+        a2 += __shfl_xor(a2, d);
+    }
+    __shared__ double red[4][3];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[w][0] = a0;
+        red[w][1] = a1;
+        red[w][2] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int c = threadIdx.x;
+        double s = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+        r[3 * i + c] = beta[(int64_t)n * n3 + 3 * i + c] - s;
+    }
+}
+
+// y[row] = sum_k A[row][k] x[k]; A row-major n x n; one wave per row, 4 rows per block.
+__global__ void __launch_bounds__(256) k_gemv_rows(const double* __restrict__ A, const double* __restrict__ x,
+                                                   double* __restrict__ y, int n) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double* a = A + (int64_t)row * n;
+    double s0 = 0.0, s1 = 0.0;
+    int k = lane;
+    for (; k + 64 < n; k += 128) {
+        s0 += a[k] * x[k];
+        s1 += a[k + 64] * x[k + 64];
+    }
+    if (k < n) s0 += a[k] * x[k];
+    double s = s0 + s1;
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
+    if (lane == 0) y[row] = s;
+}
+
+// E_0 dense (col-major) from the lag-0 segments; one thread per block.
+__global__ void k_e0_dense(Store S, int N, int L, double* __restrict__ A) {
+    int i = blockIdx.y;
+    int64_t b0 = S.seg[(int64_t)i * L + 0], b1 = S.seg[(int64_t)i * L + 1];
+    int64_t n3 = 3 * (int64_t)N;
+    for (int64_t b = b0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < b1; b += (int64_t)gridDim.x * blockDim.x) {
+        int j = S.col[b];
+        const int ii[9] = {0, 0, 0, 1, 1, 1, 2, 2, 2}, jj[9] = {0, 1, 2, 0, 1, 2, 0, 1, 2};
+        const int qq[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+        for (int k = 0; k < 9; k++) A[(3 * (int64_t)i + ii[k]) + n3 * (3 * (int64_t)j + jj[k])] = S.v[qq[k]][b];
+    }
+}
+
+// out (row-major) = in (col-major) ; n x n, 32x32 tiles through LDS.
+__global__ void k_transpose(const double* __restrict__ in, double* __restrict__ out, int n) {
+    __shared__ double tile[32][33];
+    int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        int r = by + k, c = bx + tx;  // in: element (row=c? ) -- read in[c_row + n * col]
+        if (r < n && c < n) tile[k][tx] = in[(int64_t)r * n + c];  // column r, row c of the col-major matrix
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        int r = bx + k, c = by + tx;
+        if (r < n && c < n) out[(int64_t)r * n + c] = tile[tx][k];
+    }
+}
+
+// Single point / single triangle: M(x, t) of ppA/BEMenerg_postProcA_calcMatrixBlock.m
+// (one outer point, weight 1, no cleanup), symmetric 6 values scaled by 1/(4 pi rho).
+__global__ void k_point_block(const FieldTri* __restrict__ tri, int j, const double* __restrict__ x, double t,
+                              double cP, double cS, double inv4pirho, double* __restrict__ out6) {
+    if (threadIdx.x != 0) return;
+    const FieldTri& F = tri[j];
+    Level K = make_level(t, cP, cS);
+    NodeAcc acc;
+    acc.s = 0.0;
+    for (int q = 0; q < 6; q++) acc.m[q] = 0.0;
+    double xx[3] = {x[0], x[1], x[2]};
+    NodeGeo G = node_geo(xx, F);
+    node_eval(G, K, 1.0, acc);
+    double W[6];
+    finish_rotation(acc, F.C, W);
+    for (int q = 0; q < 6; q++) out6[q] = W[q] * inv4pirho;
+}
+
+// ------------------------------------------------------------------ RHS --
+
+__device__ inline void datum(int kind, const double* x, const double* nrm, double t, double* g) {
+    g[0] = g[1] = g[2] = 0.0;
+    if (kind == 0) {
+        double s = sin(kPi * t / 2.0);
+        g[2] = t > 0.0 ? s * s : 0.0;
+    } else if (kind == 1) {
+        double f = t > 0.0 ? t * t * exp(-4.0 * t) : 0.0;
+        g[0] = nrm[0] * f;
+        g[1] = nrm[1] * f;
+        g[2] = nrm[2] * f;
+    } else if (kind == 2) {
+        g[2] = 0.5 * t;
+    } else if (kind == 3) {
+        double s = sin(t);
+        double a = (t < kPi / 2.0) ? s * s * s * s * s : 1.0;
+        g[2] = a * (x[0] * x[0]);
+    }
+}
+
+// beta (3N x nT col-major): Tn/BEMenerg_core_calcTnBlock.m:27-58.
+__global__ void k_rhs(const FieldTri* __restrict__ tri, const double* __restrict__ gh, int N, int nT, int nGH,
+                      double dt, int kind, double* __restrict__ beta) {
+    int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (id >= (int64_t)N * nT) return;
+    int T = (int)(id % N), n = (int)(id / N);
+    double t0 = dt * n, t1 = dt * (n + 1);
+    const double* g = gh + (int64_t)T * 4 * 32;
+    double ris[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < nGH; q++) {
+        double x[3] = {g[q], g[32 + q], g[64 + q]};
+        double a[3], b[3];
+        datum(kind, x, tri[T].n, t1, a);
+        datum(kind, x, tri[T].n, t0, b);
+        for (int k = 0; k < 3; k++) ris[k] = ris[k] + (a[k] - b[k]) * g[96 + q];
+    }
+    for (int k = 0; k < 3; k++) beta[(int64_t)n * 3 * N + 3 * T + k] = fabs(ris[k]) < 1.0e-14 ? 0.0 : ris[k];
+}
+
+}  // namespace ebem

@@ -1,0 +1,605 @@
+// ebem_sa.hpp -- device-side semi-analytic (SA) inner integration of the
+// single-layer retarded potential over one flat field triangle, for one outer
+// point and one time level.
+//
+// Mathematical content follows the reference's SA path
+// (functions/BEMenerg_indir/BEMenerg_core/BEMenerg_coreSA/calcoloMatrice/ and
+// functions/BEMenerg_analiticIntg/); the evaluation is reorganised for the GPU:
+//
+//  * the six kernel integrals ris1..ris6 of calcMatrixSubBlock.m:267-272 are
+//    never formed.  ris1/ris3/ris5 are multiples of the identity and ris2/4/6
+//    are linear in the six basis matrices B_1..B_6 of coeffB.m:150-160, so each
+//    region routine folds its contribution straight into one scalar (identity
+//    part) and six basis coefficients, already weighted by the level
+//    coefficients of calcMatrixSubBlock.m:334-337;
+//  * B_1..B_6 are sparse; the child matrix is built once per child from the six
+//    coefficients (6 symmetric entries, not 6 dense 3x3 products) and rotated
+//    once by the child angle theta0 (fieldTriangle.m:191-204);
+//  * the inverse Euler rotation (rotazioneInversaEulero.m) depends only on the
+//    field triangle: it is applied once per (pair, level) by the caller, after
+//    the Gauss-Hammer sum over outer nodes, instead of 3x per node.
+//
+// Branch structure, thresholds and the region decomposition E1..E9 are those of
+// calcIntgInt_childTriangle.m:62-234 (kept exactly: they define the result).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ebem {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// Per-triangle record (256 B, one cache-line pair), built on the host.
+struct FieldTri {
+    double v0[3];               // first vertex
+    double L1, L3, cosd, sind;  // calcoloSistemaRiferimento.m: |P2-P1|, |P3-P1|, cos/sin(angle e1,e3)
+    double e1[3], e2[3], n[3];  // in-plane frame and unit normal
+    double C[9];                // inverse Euler rotation, row-major, entries < 1e-14 zeroed
+    double center[3];
+    double maxL;                // longest edge
+    double area;
+    double pad[2];
+};
+static_assert(sizeof(FieldTri) == 256, "FieldTri must stay 256 B");
+
+// Level (time) constants.  s = k1*r1 + k3*r3 + k5*r5 and
+// M = k2*ris2 + k4*ris4 + k6*ris6 reproduce calcMatrixSubBlock.m:334-337.
+struct Level {
+    double t, rP, rS;
+    double k1, k3, k5, k2, k4, k6;
+};
+
+__host__ __device__ inline Level make_level(double t, double cP, double cS) {
+    Level K;
+    double cP2 = cP * cP, cS2 = cS * cS, den = (cP * cS) * (cP * cS);
+    K.t = t;
+    K.rP = cP * t;
+    K.rS = cS * t;
+    K.k1 = 0.5 / cP2;
+    K.k2 = -0.5 / cP2;
+    K.k3 = -0.5 * t * t;
+    K.k4 = 1.5 * t * t;
+    K.k5 = 0.5 * (cP2 + cS2) / den;
+    K.k6 = 0.5 * (cP2 - cS2) / den;
+    return K;
+}
+
+// Geometry of one child triangle (calcoloDatiTriangolo.m), angles at the
+// origin vertex (gamma), second vertex (beta), third vertex (alpha).
+struct ChildGeo {
+    double a, b, c;
+    double ca, cb, cg, sa, sb, sg;
+    double al, be, ga;
+};
+
+// Angular integrals Phi(1..5) of the region routines (e.g. CC.m:34-38).
+struct Phi {
+    double p1, p2, p3, p4, p5;
+};
+
+__host__ __device__ inline double clamp1(double x) { return fmin(1.0, fmax(-1.0, x)); }
+__host__ __device__ inline double sqrt0(double x) { return x < 0.0 ? 0.0 : sqrt(x); }
+__host__ __device__ inline double msign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+
+__host__ __device__ inline Phi make_phi(double th1, double th2, const ChildGeo& g) {
+    double T1t = th2 - th1, T2t = th2 + th1;
+    double s1t = sin(T1t);
+    double sh1 = sin(T1t * 0.5);
+    Phi P;
+    P.p1 = 2.0 * sin(T2t * 0.5) * sh1;
+    P.p2 = 2.0 * sin(g.ga - T2t * 0.5) * sh1;
+    P.p3 = 0.5 * (T1t - cos(T2t) * s1t);
+    P.p4 = -0.5 * (-(s1t * cos(g.ga - T2t)) + T1t * g.cg);
+    P.p5 = 0.5 * (T1t - cos(2.0 * g.ga - T2t) * s1t);
+    return P;
+}
+
+// Accumulator of one child: identity part and coefficients of B_1..B_6.
+struct ChildAcc {
+    double s;
+    double b[6];
+};
+
+// ---- sector / annulus routines -------------------------------------------
+
+// BEMenerg_analiticIntg_CC.m (P annulus between the S and P circles, inner
+// radius RS/rhoS) and BEMenerg_analiticIntg_SC_P.m (inner radius collapsed to
+// the plane distance: RS = zeta, rhoS = 0), folded with the level weights.
+__host__ __device__ inline void region_annulus(bool cc, double RS, double rhoS, double RP, double rhoP,
+                                      double zeta, double th1, double th2, const ChildGeo& g,
+                                      const Level& K, ChildAcc& A) {
+    double T1t = th2 - th1;
+    Phi P = make_phi(th1, th2, g);
+    if (cc && !(zeta > 1.0e-6)) {
+        // CC.m:59-91, coplanar
+        double I1 = RP - RS;
+        double I4 = -(1.0 / RP) + (1.0 / RS);
+        A.s += K.k1 * (T1t * I1) + K.k3 * (T1t * I4);
+        double w = K.k2 * I1 + K.k4 * I4;
+        A.b[3] += w * P.p5;
+        A.b[4] += w * P.p4;
+        A.b[5] += w * P.p3;
+        return;
+    }
+    double I1, I2, I3, I4, I5, I6, I7;
+    double iRP = 1.0 / RP, iRP3 = iRP * iRP * iRP, z2 = zeta * zeta;
+    double qP = rhoP * iRP;
+    if (cc) {  // CC.m:25-31
+        double iRS = 1.0 / RS, iRS3 = iRS * iRS * iRS;
+        double qS = rhoS * iRS;
+        I1 = RP - RS;
+        I2 = RP - RS + z2 * (iRP - iRS);
+        I3 = log((rhoP + RP) / (rhoS + RS)) - qP + qS;
+        I4 = -iRP + iRS;
+        I5 = -iRP + iRS + (z2 / 3.0) * (iRP3 - iRS3);
+        I6 = (1.0 / (3.0 * z2)) * (qP * qP * qP - qS * qS * qS);
+        I7 = -1.0 / 3.0 * (iRP3 - iRS3);
+    } else {  // SC_P.m:114-120
+        I1 = RP - zeta;
+        I2 = RP + z2 * iRP - 2.0 * zeta;
+        I3 = log((rhoP + RP) / zeta) - qP;
+        I4 = -iRP + 1.0 / zeta;
+        I5 = -iRP + z2 / (3.0 * RP * RP * RP) + 2.0 / (3.0 * zeta);
+        I6 = (1.0 / (3.0 * z2)) * (qP * qP * qP);
+        I7 = -(1.0 / (3.0 * RP * RP * RP)) + 1.0 / (3.0 * zeta * z2);
+    }
+    A.s += K.k1 * (T1t * I1) + K.k3 * (T1t * I4);
+    A.b[0] += K.k2 * (T1t * I4) + K.k4 * (T1t * I7);
+    double w3 = K.k2 * I3 + K.k4 * I6;
+    A.b[1] += w3 * P.p2;
+    A.b[2] += w3 * P.p1;
+    double w2 = K.k2 * I2 + K.k4 * I5;
+    A.b[3] += w2 * P.p5;
+    A.b[4] += w2 * P.p4;
+    A.b[5] += w2 * P.p3;
+}
+
+// BEMenerg_analiticIntg_SC_SP.m: S disk sector, kernels 1/r and r_i r_j/r^3.
+__host__ __device__ inline void region_sdisk(double RS, double rhoS, double zeta, double th1, double th2,
+                                    const ChildGeo& g, const Level& K, ChildAcc& A) {
+    double T1t = th2 - th1;
+    Phi P = make_phi(th1, th2, g);
+    if (zeta > 1.0e-6) {
+        double I1 = RS - zeta;
+        double I2 = RS + zeta * zeta / RS - 2.0 * zeta;
+        double I3 = log((rhoS + RS) / zeta) - rhoS / RS;
+        double I4 = -(1.0 / RS) + 1.0 / zeta;
+        A.s += K.k5 * (T1t * I1);
+        A.b[0] += K.k6 * (T1t * I4);
+        A.b[1] += K.k6 * I3 * P.p2;
+        A.b[2] += K.k6 * I3 * P.p1;
+        A.b[3] += K.k6 * I2 * P.p5;
+        A.b[4] += K.k6 * I2 * P.p4;
+        A.b[5] += K.k6 * I2 * P.p3;
+    } else {
+        A.s += K.k5 * (T1t * RS);
+        A.b[3] += K.k6 * RS * P.p5;
+        A.b[4] += K.k6 * RS * P.p4;
+        A.b[5] += K.k6 * RS * P.p3;
+    }
+}
+
+// ---- regions bounded by the child's far edge (Milroy-type G_i) ------------
+
+// Edge-bounded terms shared by coeffG_TC / coeffG_TR_P / coeffG_TR_SP for
+// zeta > 1e-6: g1..g8 and the angle functions of TH1, TH2.
+struct EdgeG {
+    double F, g1, g2, g3, g4, g5, g7, g8, ADD;
+    double c1b, c2b, s1b, s2b;
+};
+
+__host__ __device__ inline EdgeG make_edge(double zeta, double th1, double th2, const ChildGeo& g) {
+    EdgeG E;
+    E.F = g.a * g.sb;
+    E.s1b = sin(th1 + g.be);
+    E.s2b = sin(th2 + g.be);
+    E.c1b = cos(th1 + g.be);
+    E.c2b = cos(th2 + g.be);
+    double R1 = E.F / E.s1b, R2 = E.F / E.s2b;
+    double fz1 = zeta / R1, fz2 = zeta / R2;
+    E.ADD = zeta * zeta + E.F * E.F;
+    E.g1 = sqrt(1.0 + fz1 * fz1);
+    E.g2 = sqrt(1.0 + fz2 * fz2);
+    E.g3 = 0.5 * (log((E.g1 + E.c1b) * (E.g2 - E.c2b)) - log((E.g1 - E.c1b) * (E.g2 + E.c2b)));
+    double i1 = 1.0 / fz1, i2 = 1.0 / fz2;
+    E.g4 = log(i1 + sqrt(1.0 + i1 * i1));
+    E.g5 = log(i2 + sqrt(1.0 + i2 * i2));
+    double sq = sqrt(E.ADD);
+    E.g7 = kPi - acos(clamp1((-zeta * E.c2b) / sq)) - acos(clamp1((zeta * E.c1b) / sq));
+    E.g8 = E.F * E.F / E.ADD;
+    return E;
+}
+
+// TC (P region between the S circle and the edge, coeffG_TC.m) when tc is
+// true; TR_P (P region between the plane foot and the edge, coeffG_TR_P.m)
+// otherwise.  zeta > 1e-6 branch of both.
+__host__ __device__ inline void region_edgeP(bool tc, double RS, double rhoS, double zeta, double th1, double th2,
+                                   const ChildGeo& g, const Level& K, ChildAcc& A) {
+    double T1t = th2 - th1;
+    Phi P = make_phi(th1, th2, g);
+    EdgeG E = make_edge(zeta, th1, th2, g);
+    const double F = E.F;
+    double ca_g_1 = cos(g.al - g.ga + th1), ca_g_2 = cos(g.al - g.ga + th2);
+    double cta1 = cos(th1 + g.al), cta2 = cos(th2 + g.al);
+    double ctb1 = cos(th1 - g.be), ctb2 = cos(th2 - g.be);
+    double cgt1 = cos(g.ga - th1), cgt2 = cos(g.ga - th2);
+    double ct1 = cos(th1), ct2 = cos(th2);
+    double G1, G4, G5, G6, G7, G8, G9, G14, G15, G16, G17, G18, G19;
+    double z2 = zeta * zeta, iz = 1.0 / zeta;
+    double h5 = cgt2 * E.g5 - cgt1 * E.g4 - g.ca * E.g3;
+    double h6 = -ct2 * E.g5 + ct1 * E.g4 - g.cb * E.g3;
+    double f7 = F * (-E.g1 * ca_g_1 + E.g2 * ca_g_2 + E.g3 * g.sa * g.sa);
+    double f8 = F * (-g.sa * g.sb * E.g3 + E.g1 * cta1 - E.g2 * cta2);
+    double f9 = F * (g.sb * g.sb * E.g3 + E.g1 * ctb1 - E.g2 * ctb2);
+    double u15 = (g.sa * E.s2b - E.g8 * g.ca * E.c2b) / E.g2 + (-g.sa * E.s1b + E.g8 * g.ca * E.c1b) / E.g1;
+    double u16 = (g.sb * E.s1b + E.g8 * g.cb * E.c1b) / E.g1 + (-g.sb * E.s2b - E.g8 * g.cb * E.c2b) / E.g2;
+    double v17 = (-(E.s2b * E.s2b) * ca_g_2 + E.g8 * (g.ca * g.ca) * E.c2b) / E.g2 -
+                 (-(E.s1b * E.s1b) * ca_g_1 + E.g8 * (g.ca * g.ca) * E.c1b) / E.g1;
+    double v18 = ((E.s2b * E.s2b) * cta2 + E.g8 * (g.ca * g.cb * E.c2b)) / E.g2 -
+                 ((E.s1b * E.s1b) * cta1 + E.g8 * (g.ca * g.cb * E.c1b)) / E.g1;
+    double v19 = ((E.s2b * E.s2b) * ctb2 + E.g8 * (g.cb * g.cb * E.c2b)) / E.g2 -
+                 ((E.s1b * E.s1b) * ctb1 + E.g8 * (g.cb * g.cb * E.c1b)) / E.g1;
+    double w14 = F / E.ADD * (E.c1b / E.g1 - E.c2b / E.g2);
+    if (tc) {  // coeffG_TC.m:44-136
+        double lnS = (rhoS / RS) - log((rhoS + RS) * iz);
+        double q = (RS * RS + z2) / RS;
+        double iRS3 = 1.0 / (3.0 * RS * RS * RS);
+        double rS3 = rhoS * rhoS * rhoS / (3.0 * z2 * RS * RS * RS);
+        double tail = (1.0 / RS) - z2 * iRS3;
+        G1 = -(RS * T1t) + zeta * E.g7 + F * E.g3;
+        G4 = (1.0 / RS) * T1t - iz * E.g7;
+        G5 = h5 + lnS * (cgt2 - cgt1);
+        G6 = h6 - lnS * (ct2 - ct1);
+        G7 = zeta * E.g7 - q * P.p5 + f7;
+        G8 = -zeta * g.cg * E.g7 - q * P.p4 + f8;
+        G9 = zeta * E.g7 - q * P.p3 + f9;
+        G14 = iRS3 * T1t + 1.0 / (3.0 * z2) * (w14 - iz * E.g7);
+        G15 = -rS3 * P.p2 + 1.0 / (3.0 * z2) * u15;
+        G16 = -rS3 * P.p1 + 1.0 / (3.0 * z2) * u16;
+        G17 = 1.0 / 3.0 * v17 / F - E.g7 / (3.0 * zeta) + tail * P.p5;
+        G18 = 1.0 / 3.0 * v18 / F + E.g7 / (3.0 * zeta) * g.cg + tail * P.p4;
+        G19 = 1.0 / 3.0 * v19 / F - E.g7 / (3.0 * zeta) + tail * P.p3;
+    } else {  // coeffG_TR_P.m:63-119
+        G1 = zeta * (-T1t + E.g7) + F * E.g3;
+        G4 = iz * (T1t - E.g7);
+        G5 = h5;
+        G6 = h6;
+        G7 = zeta * (E.g7 - 2.0 * P.p5) + f7;
+        G8 = -zeta * (g.cg * E.g7 + 2.0 * P.p4) + f8;
+        G9 = zeta * (E.g7 - 2.0 * P.p3) + f9;
+        G14 = 1.0 / (3.0 * z2) * (iz * (T1t - E.g7) + w14);
+        G15 = 1.0 / (3.0 * z2) * u15;
+        G16 = 1.0 / (3.0 * z2) * u16;
+        G17 = 1.0 / (3.0 * F) * v17 - 1.0 / (3.0 * zeta) * (E.g7 - 2.0 * P.p5);
+        G18 = 1.0 / (3.0 * F) * v18 + 1.0 / (3.0 * zeta) * (E.g7 * g.cg + 2.0 * P.p4);
+        G19 = 1.0 / (3.0 * F) * v19 - 1.0 / (3.0 * zeta) * (E.g7 - 2.0 * P.p3);
+    }
+    // TC.m:18-29 / TR_P.m:65-76
+    A.s += K.k1 * G1 + K.k3 * G4;
+    A.b[0] += K.k2 * G4 + K.k4 * G14;
+    A.b[1] += K.k2 * G5 + K.k4 * G15;
+    A.b[2] += K.k2 * G6 + K.k4 * G16;
+    A.b[3] += K.k2 * G7 + K.k4 * G17;
+    A.b[4] += K.k2 * G8 + K.k4 * G18;
+    A.b[5] += K.k2 * G9 + K.k4 * G19;
+}
+
+// Coplanar (zeta <= 1e-6) branch of coeffG_TC.m:138-181 (TC.m:31-49).
+__host__ __device__ inline void region_tc_coplanar(double RS, double th1, double th2, const ChildGeo& g,
+                                          const Level& K, ChildAcc& A) {
+    double T1t = th2 - th1;
+    Phi P = make_phi(th1, th2, g);
+    const double F = g.a * g.sb;
+    double c1b = cos(th1 + g.be), c2b = cos(th2 + g.be);
+    double s1b = sin(th1 + g.be), s2b = sin(th2 + g.be);
+    double g_3 = 0.5 * (log((1.0 + c1b) * (1.0 - c2b)) - log((1.0 - c1b) * (1.0 + c2b)));
+    double ca_g_1 = cos(g.al - g.ga + th1), ca_g_2 = cos(g.al - g.ga + th2);
+    double cta1 = cos(th1 + g.al), cta2 = cos(th2 + g.al);
+    double ctb1 = cos(g.be - th1), ctb2 = cos(g.be - th2);
+    double iRS = 1.0 / RS;
+    double G1 = F * g_3 - RS * T1t;
+    double G4 = 1.0 / F * (c2b - c1b) + iRS * T1t;
+    double G7 = -RS * P.p5 + F * (ca_g_2 - ca_g_1 + g.sa * g.sa * g_3);
+    double G8 = -RS * P.p4 + F * (cta1 - cta2 - g.sa * g.sb * g_3);
+    double G9 = -RS * P.p3 + F * (ctb1 - ctb2 + g.sb * g.sb * g_3);
+    double i3F = 1.0 / (3.0 * F);
+    double G17 = i3F * (((1.0 + g.ca * g.ca) * c2b - (s2b * s2b) * ca_g_2) -
+                        ((1.0 + g.ca * g.ca) * c1b - (s1b * s1b) * ca_g_1)) +
+                 iRS * P.p5;
+    double cab = g.ca * g.cb + cos(g.al + g.be);
+    double G18 = i3F * ((c2b * cab + (s2b * s2b) * cta2) - (c1b * cab + (s1b * s1b) * cta1)) + iRS * P.p4;
+    double G19 = i3F * (((1.0 + g.cb * g.cb) * c2b + (s2b * s2b) * ctb2) -
+                        ((1.0 + g.cb * g.cb) * c1b + (s1b * s1b) * ctb1)) +
+                 iRS * P.p3;
+    A.s += K.k1 * G1 + K.k3 * G4;
+    A.b[3] += K.k2 * G7 + K.k4 * G17;
+    A.b[4] += K.k2 * G8 + K.k4 * G18;
+    A.b[5] += K.k2 * G9 + K.k4 * G19;
+}
+
+// TR_SP (S disk part bounded by the edge, coeffG_TR_SP.m, TR_SP.m).
+__host__ __device__ inline void region_edgeS(double zeta, double th1, double th2, const ChildGeo& g, const Level& K,
+                                    ChildAcc& A) {
+    double T1t = th2 - th1;
+    if (zeta > 1.0e-6) {
+        Phi P = make_phi(th1, th2, g);
+        EdgeG E = make_edge(zeta, th1, th2, g);
+        const double F = E.F;
+        double G1 = zeta * (-T1t + E.g7) + F * E.g3;
+        double G4 = 1.0 / zeta * (T1t - E.g7);
+        double G5 = cos(g.ga - th2) * E.g5 - cos(g.ga - th1) * E.g4 - g.ca * E.g3;
+        double G6 = -cos(th2) * E.g5 + cos(th1) * E.g4 - g.cb * E.g3;
+        double G7 = zeta * (E.g7 - 2.0 * P.p5) +
+                    F * (-E.g1 * cos(g.al - g.ga + th1) + E.g2 * cos(g.al - g.ga + th2) + E.g3 * g.sa * g.sa);
+        double G8 = -zeta * (g.cg * E.g7 + 2.0 * P.p4) +
+                    F * (-g.sa * g.sb * E.g3 + E.g1 * cos(th1 + g.al) - E.g2 * cos(th2 + g.al));
+        double G9 = zeta * (E.g7 - 2.0 * P.p3) +
+                    F * (g.sb * g.sb * E.g3 + E.g1 * cos(th1 - g.be) - E.g2 * cos(th2 - g.be));
+        A.s += K.k5 * G1;
+        A.b[0] += K.k6 * G4;
+        A.b[1] += K.k6 * G5;
+        A.b[2] += K.k6 * G6;
+        A.b[3] += K.k6 * G7;
+        A.b[4] += K.k6 * G8;
+        A.b[5] += K.k6 * G9;
+    } else {
+        const double F = g.a * g.sb;
+        double c1b = cos(th1 + g.be), c2b = cos(th2 + g.be);
+        double g_3 = 0.5 * (log((1.0 + c1b) * (1.0 - c2b)) - log((1.0 - c1b) * (1.0 + c2b)));
+        double G1 = F * g_3;
+        double G7 = F * (-cos(g.al - g.ga + th1) + cos(g.al - g.ga + th2) + g.sa * g.sa * g_3);
+        double G8 = F * (-(g.sa * g.sb * g_3) + cos(th1 + g.al) - cos(th2 + g.al));
+        double G9 = F * (g.sb * g.sb * g_3 + cos(th1 - g.be) - cos(th2 - g.be));
+        A.s += K.k5 * G1;
+        A.b[3] += K.k6 * G7;
+        A.b[4] += K.k6 * G8;
+        A.b[5] += K.k6 * G9;
+    }
+}
+
+enum : int { kNone = 0, kSPSP = 1, kSPP = 2, kP = 3 };
+
+// calcoloDatiTriangolo.m:1-39 for the child (O, V2, V3), O at the origin.
+__host__ __device__ inline ChildGeo child_geo(double v2x, double v2y, double v3x, double v3y) {
+    ChildGeo g;
+    g.a = sqrt(v2x * v2x + v2y * v2y);
+    g.b = sqrt(v3x * v3x + v3y * v3y);
+    double dx = v3x - v2x, dy = v3y - v2y;
+    g.c = sqrt(dx * dx + dy * dy);
+    const double a = g.a, b = g.b, c = g.c;
+    g.ca = (b * b + c * c - a * a) / (2.0 * b * c);
+    g.cb = (a * a + c * c - b * b) / (2.0 * a * c);
+    g.cg = (a * a + b * b - c * c) / (2.0 * a * b);
+    g.sa = sqrt0(1.0 - g.ca * g.ca);
+    g.sb = sqrt0(1.0 - g.cb * g.cb);
+    g.sg = sqrt0(1.0 - g.cg * g.cg);
+    g.al = acos(clamp1(g.ca));
+    g.be = acos(clamp1(g.cb));
+    g.ga = acos(clamp1(g.cg));
+    return g;
+}
+
+// calcIntgInt_childTriangle.m:9-234 for one child, result folded into A.
+__host__ __device__ inline void child_regions(const ChildGeo& g, double zeta, double RP, double rhoP, double RS,
+                                     double rhoS, int type, const Level& K, ChildAcc& A) {
+    const double a = g.a;
+    // childTriangle.m:27-50.  Rotated child: V2 = (a,0), V3 = (b cg, b sg).
+    double vt0 = (g.b * g.cg - a) / g.c;
+    double Bm = a * vt0;
+    double DeltaP = Bm * Bm - (a * a - rhoP * rhoP);
+    double DeltaS = Bm * Bm - (a * a - rhoS * rhoS);
+    double rP = a * g.sb / rhoP;
+    double rS = a * g.sb / rhoS;
+    if (type == kP) {
+        rS = 0.0;
+        DeltaS = 0.0;
+    }
+    const double gamma = g.ga, beta = g.be;
+    double TH1 = 0.0, TH2, TH3, TH4, TH5, TH6 = gamma;
+    if (DeltaP <= 1.0e-6) {
+        TH2 = TH3 = TH4 = TH5 = gamma;
+    } else if (DeltaS <= 1.0e-6) {
+        double as = asin(clamp1(rP));
+        double O1 = as - beta, O2 = kPi - as - beta;
+        double th1 = fmin(O1, O2), th2 = fmax(O1, O2);
+        TH2 = fmin(fmax(0.0, th1), gamma);
+        TH3 = fmax(0.0, fmin(th2, gamma));
+        TH4 = TH3;
+        TH5 = TH3;
+    } else {
+        double as = asin(clamp1(rP)), bs = asin(clamp1(rS));
+        double O1 = as - beta, O2 = kPi - as - beta, O3 = bs - beta, O4 = kPi - bs - beta;
+        TH2 = fmin(fmax(0.0, fmin(O1, O2)), gamma);
+        TH3 = fmin(fmax(0.0, fmin(O3, O4)), gamma);
+        TH4 = fmax(0.0, fmin(fmax(O3, O4), gamma));
+        TH5 = fmax(0.0, fmin(fmax(O1, O2), gamma));
+    }
+    const bool sp = (type == kSPSP);
+    const bool coplanar = !(zeta > 1.0e-6);
+    if (TH1 < TH2) {  // E1 (S sector) + E6 (P annulus)
+        if (sp) region_sdisk(RS, rhoS, zeta, TH1, TH2, g, K, A);
+        region_annulus(type != kP, RS, rhoS, RP, rhoP, zeta, TH1, TH2, g, K, A);
+    }
+    if (TH2 < TH3) {  // E2 (S sector) + E7 (P region bounded by the edge)
+        if (sp) region_sdisk(RS, rhoS, zeta, TH2, TH3, g, K, A);
+        if (type == kP) region_edgeP(false, RS, rhoS, zeta, TH2, TH5, g, K, A);
+        else if (coplanar) region_tc_coplanar(RS, TH2, TH3, g, K, A);
+        else region_edgeP(true, RS, rhoS, zeta, TH2, TH3, g, K, A);
+    }
+    if (TH3 < TH4) {  // E3 (S region bounded by the edge)
+        if (sp) region_edgeS(zeta, TH3, TH4, g, K, A);
+    }
+    if (TH4 < TH5) {  // E4 (S sector) + E8 (P region between S circle and edge)
+        if (sp) region_sdisk(RS, rhoS, zeta, TH4, TH5, g, K, A);
+        if (coplanar) region_tc_coplanar(RS, TH4, TH5, g, K, A);
+        else region_edgeP(true, RS, rhoS, zeta, TH4, TH5, g, K, A);
+    }
+    if (TH5 < TH6) {  // E5 (S sector) + E9 (P annulus)
+        if (sp) region_sdisk(RS, rhoS, zeta, TH5, TH6, g, K, A);
+        region_annulus(type != kP, RS, rhoS, RP, rhoP, zeta, TH5, TH6, g, K, A);
+    }
+}
+
+// Result of one outer point: identity coefficient and symmetric local-frame
+// matrix (m00, m01, m02, m11, m12, m22), before the inverse Euler rotation.
+struct NodeAcc {
+    double s;
+    double m[6];
+};
+
+// geo/calcoloDistanzaTriangolo.m:1-52 (MATLAB min ignores NaN).
+__host__ __device__ inline double dist_tri(const double t[3][2]) {
+#pragma clang fp contract(off)
+    double D[3];
+    for (int i = 0; i < 3; i++) D[i] = sqrt(t[i][0] * t[i][0] + t[i][1] * t[i][1]);
+    double best = __builtin_nan("");
+    const int ea[3] = {0, 1, 0}, eb[3] = {1, 2, 2};
+    for (int i = 0; i < 3; i++) {
+        const double* P1 = t[ea[i]];
+        const double* P2 = t[eb[i]];
+        double d1 = D[ea[i]], d2 = D[eb[i]];
+        double vx = P2[0] - P1[0], vy = P2[1] - P1[1];
+        double L = sqrt(vx * vx + vy * vy);
+        double dm;
+        if (-(P1[0] * vx + P1[1] * vy) < -1.0e-6) {
+            dm = d1;
+        } else if (-(P2[0] * (-vx) + P2[1] * (-vy)) < -1.0e-6) {
+            dm = d2;
+        } else {
+            double ca = (-P1[0] / d1) * (vx / L) + (-P1[1] / d1) * (vy / L);
+            dm = d1 * sqrt0(1.0 - ca * ca);
+        }
+        if (!__builtin_isnan(dm) && (__builtin_isnan(best) || dm < best)) best = dm;
+    }
+    return best;
+}
+
+// Projection of the outer point on the field plane and the fan of three child
+// triangles around it (geo/preparazioneTriangolo.m:1-115).
+struct NodeGeo {
+    double zeta, sgn, dMIN;
+    double t[3][2];  // field triangle vertices in the plane frame, origin at the projection
+    double c[3];     // signed (doubled) child areas
+};
+
+__host__ __device__ inline NodeGeo node_geo(const double* x, const FieldTri& F) {
+#pragma clang fp contract(off)
+    NodeGeo G;
+    double d0 = x[0] - F.v0[0], d1 = x[1] - F.v0[1], d2 = x[2] - F.v0[2];
+    double ps = F.n[0] * d0 + F.n[1] * d1 + F.n[2] * d2;
+    double sp0 = x[0] - ps * F.n[0], sp1 = x[1] - ps * F.n[1], sp2 = x[2] - ps * F.n[2];
+    double sgp = (x[0] - sp0) * F.n[0] + (x[1] - sp1) * F.n[1] + (x[2] - sp2) * F.n[2];
+    if (fabs(sgp) <= 1.0e-6) sgp = 0.0;
+    G.sgn = msign(sgp);
+    double z = fabs(ps);
+    G.zeta = (z < 1.0e-6) ? 0.0 : z;
+    double f0 = F.v0[0] - sp0, f1 = F.v0[1] - sp1, f2 = F.v0[2] - sp2;
+    G.t[0][0] = f0 * F.e1[0] + f1 * F.e1[1] + f2 * F.e1[2];
+    G.t[0][1] = f0 * F.e2[0] + f1 * F.e2[1] + f2 * F.e2[2];
+    G.t[1][0] = G.t[0][0] + F.L1;
+    G.t[1][1] = G.t[0][1] + 0.0;
+    G.t[2][0] = G.t[0][0] + F.L3 * F.cosd;
+    G.t[2][1] = G.t[0][1] + F.L3 * F.sind;
+    G.c[0] = (G.t[0][0] * G.t[1][1]) - (G.t[0][1] * G.t[1][0]);
+    G.c[1] = (G.t[1][0] * G.t[2][1]) - (G.t[1][1] * G.t[2][0]);
+    G.c[2] = (G.t[2][0] * G.t[0][1]) - (G.t[2][1] * G.t[0][0]);
+    double A2 = (G.t[1][0] - G.t[0][0]) * (G.t[2][1] - G.t[0][1]);
+    double e2 = G.c[2] / A2, e3 = G.c[0] / A2;
+    double e1 = 1.0 - e2 - e3;
+    bool in = (-1.0e-6 <= e1 && e1 <= 1.0) && (-1.0e-6 <= e2 && e2 <= 1.0) && (-1.0e-6 <= e3 && e3 <= 1.0);
+    G.dMIN = in ? 0.0 : dist_tri(G.t);
+    return G;
+}
+
+// calcMatrixSubBlock.m:290-314: in-plane front radii and integration class.
+__host__ __device__ inline int classify(double zeta, double rP, double rS, double dMIN, double& rhoP, double& rhoS) {
+#pragma clang fp contract(off)
+    if (zeta == 0.0) {
+        rhoP = rP;
+        rhoS = rS;
+    } else if (zeta < rS) {
+        rhoP = sqrt(rP * rP - zeta * zeta);
+        rhoS = sqrt(rS * rS - zeta * zeta);
+    } else if (rS < zeta && zeta < rP) {
+        rhoP = sqrt(rP * rP - zeta * zeta);
+        rhoS = -1.0;
+    } else {
+        rhoP = -1.0;
+        rhoS = -1.0;
+    }
+    if (dMIN < rhoS) return kSPSP;
+    if (0.0 < rhoS && rhoS <= dMIN && dMIN < rhoP) return kSPP;
+    if (rhoS <= 0.0 && dMIN < rhoP) return kP;
+    return kNone;
+}
+
+// One outer point at one level: adds w * (s, M_local) into acc.
+// Returns the integration class (0 when the point is skipped).
+__host__ __device__ inline int node_eval(const NodeGeo& G, const Level& K, double w, NodeAcc& acc) {
+    double rhoP, rhoS;
+    int type = classify(G.zeta, K.rP, K.rS, G.dMIN, rhoP, rhoS);
+    if (type == kNone) return kNone;
+    const double zeta = G.zeta;
+    const double sgn = G.sgn;
+    const int nb[3] = {1, 2, 0};
+    for (int k = 0; k < 3; k++) {
+        double C = G.c[k];
+        if (fabs(C) < 1.0e-06) continue;
+        double v2x = G.t[k][0], v2y = G.t[k][1];
+        double v3x = G.t[nb[k]][0], v3y = G.t[nb[k]][1];
+        if (C < 0.0) {  // fieldTriangle.m:163-167: counter-clockwise order
+            double tx = v2x, ty = v2y;
+            v2x = v3x; v2y = v3y;
+            v3x = tx; v3y = ty;
+        }
+        double th0 = atan2(v2y, v2x);
+        th0 = (th0 < 0.0) ? fabs(th0) : 2.0 * kPi - th0;
+        ChildGeo g = child_geo(v2x, v2y, v3x, v3y);
+        ChildAcc A;
+        A.s = 0.0;
+        for (int i = 0; i < 6; i++) A.b[i] = 0.0;
+        child_regions(g, zeta, K.rP, rhoP, K.rS, rhoS, type, K, A);
+        // Child matrix sum_i b_i B_i (coeffB.m:136-160), symmetric, child frame.
+        double isg = 1.0 / g.sg;
+        double isg2 = isg * isg;
+        double m00 = (A.b[3] + 2.0 * g.cg * A.b[4] + g.cg * g.cg * A.b[5]) * isg2;
+        double m01 = (A.b[4] + g.cg * A.b[5]) * isg;
+        double m11 = A.b[5];
+        double zs = zeta * sgn;
+        double m02 = -zs * (A.b[1] + g.cg * A.b[2]) * isg;
+        double m12 = -zs * A.b[2];
+        double m22 = zs * zs * A.b[0];
+        // A(theta0) * M * A(theta0)' (fieldTriangle.m:191-204)
+        double ct = cos(th0), st = sin(th0);
+        double r00 = ct * ct * m00 + 2.0 * ct * st * m01 + st * st * m11;
+        double r01 = -ct * st * m00 + (ct * ct - st * st) * m01 + ct * st * m11;
+        double r11 = st * st * m00 - 2.0 * ct * st * m01 + ct * ct * m11;
+        double r02 = ct * m02 + st * m12;
+        double r12 = -st * m02 + ct * m12;
+        double ws = w * msign(C);
+        acc.s += ws * A.s;
+        acc.m[0] += ws * r00;
+        acc.m[1] += ws * r01;
+        acc.m[2] += ws * r02;
+        acc.m[3] += ws * r11;
+        acc.m[4] += ws * r12;
+        acc.m[5] += ws * m22;
+    }
+    return type;
+}
+
+// W = s I + C' M C  (rotazioneInversaEulero.m:133), symmetric output
+// (w00, w01, w02, w11, w12, w22).
+__host__ __device__ inline void finish_rotation(const NodeAcc& acc, const double* Cm, double* W) {
+    double M[3][3] = {{acc.m[0], acc.m[1], acc.m[2]}, {acc.m[1], acc.m[3], acc.m[4]}, {acc.m[2], acc.m[4], acc.m[5]}};
+    double T[3][3];  // T = M C
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) T[i][j] = M[i][0] * Cm[0 * 3 + j] + M[i][1] * Cm[1 * 3 + j] + M[i][2] * Cm[2 * 3 + j];
+    const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
+    for (int q = 0; q < 6; q++) {
+        int i = ii[q], j = jj[q];
+        double v = Cm[0 * 3 + i] * T[0][j] + Cm[1 * 3 + i] * T[1][j] + Cm[2 * 3 + i] * T[2][j];
+        W[q] = v + (i == j ? acc.s : 0.0);
+    }
+}
+
+}  // namespace ebem

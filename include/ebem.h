@@ -1,0 +1,124 @@
+/*
+ * ebem.h -- C ABI of the MI355X (gfx950) energetic space-time Galerkin BEM
+ * single-layer hot path: E_lag assembly, marching-on-in-time (MOT) solve and
+ * field post-processing.
+ *
+ * The reference reaches its device code only through MATLAB
+ * (parallel.gpu.CUDAKernel + feval, functions/BEMenerg_dir/BEMenerg_dir_launchKernelV.m:7-14)
+ * and through the MATLAB functions named below; each entry point cites the
+ * reference interface it replaces.  A MEX gateway (elastodynamicebem_amd/mex/)
+ * and a Python ctypes layer (elastodynamicebem_amd/api.py) bind these symbols.
+ *
+ * Conventions (all match the MATLAB layouts of the reference):
+ *   - coordinates: Nv x 3, row-major (vertex v at coords[3v..3v+2]);
+ *   - triangles:   N x 3 vertex indices, 0- or 1-based (flag);
+ *   - E_lag, beta, density: column-major, DOF index 3*t + component, i.e. the
+ *     3N x 3N blocks of BEMenerg_coreSA_calcMatrixBlock.m and the 3N x nT
+ *     arrays of BEMenerg_coreSA_timeMarching.m;
+ *   - every function returns 0 on success or a negative EBEM_E* code; the
+ *     message is available from ebem_last_error() on the calling thread.
+ * A handle serialises its own calls with a mutex (the reference calls the GPU
+ * from parfor workers, dirPP/BEMenerg_ppD_calc.m:8); distinct handles may be
+ * used from distinct threads.
+ */
+#ifndef EBEM_H
+#define EBEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EBEM_OK 0
+#define EBEM_EINVAL -1   /* bad argument / state */
+#define EBEM_EHIP -2     /* HIP runtime failure */
+#define EBEM_ENOMEM -3   /* device allocation failed (dir/BEMenerg_dir_timeMarchingD.m:20-22) */
+#define EBEM_ESINGULAR -4 /* E_0 factorisation hit a zero pivot */
+#define EBEM_ESOLVER -5  /* rocSOLVER/rocBLAS failure */
+
+/* Integration precision of ebem_assemble. */
+#define EBEM_PREC_FP64 0   /* plain form: everything FP64 */
+#define EBEM_PREC_MIXED 1  /* far-field pairs (centroid distance > 3h) integrated in FP32 */
+
+/* Dirichlet data of ebem_rhs (Tn/BEMenerg_core_calcNucleoTn.m:133-183). */
+#define EBEM_DATUM_SCREEN_SIN2 0    /* (0,0,sin^2(pi t/2) H(t)) */
+#define EBEM_DATUM_SPHERE_T2EXP 1   /* n(x) t^2 exp(-4t) H(t), n = triangle normal */
+#define EBEM_DATUM_REF_SPHERE 2     /* reference 'sphereUniform': (0,0,0.5 t) */
+#define EBEM_DATUM_REF_SCREEN 3     /* reference 'screenUniform': (0,0,[sin^5 t | 1] x1^2) */
+
+typedef struct ebem_handle ebem_handle;
+
+/* Thread-local message of the last failing call on this thread. */
+const char *ebem_last_error(void);
+
+/* Library/device set-up.  Replaces gpuDevice/reset (dir/BEMenerg_dir_timeMarchingD.m:5-8). */
+int ebem_create(ebem_handle **h, int device);
+int ebem_destroy(ebem_handle *h);
+
+/* Mesh: replaces readSpaceMesh's derived data (util/fileRead/readSpaceMesh.m:93-139);
+ * normals, areas, centres and longest edges are derived here with the same
+ * expressions.  coords Nv x 3 row-major; tri N x 3. */
+int ebem_set_mesh(ebem_handle *h, int N, int Nv, const double *coords, const int32_t *tri, int one_based);
+
+/* Problem: pbParam of util/fileRead/readInputFile.m:36-70
+ * (c_P = sqrt((lambda+2mu)/rho), c_S = sqrt(mu/rho), dt = Tfin/nT). */
+int ebem_set_problem(ebem_handle *h, double rho, double mu, double lambda, double Tfin, int nT);
+
+/* Outer Gauss-Hammer rule of methodInfo "SA nn" (util/setup/BEMenerg_setupCore.m:9-21,
+ * GaussHammerComposite(1, nn)); nn in {3, 7, 12, 19}. */
+int ebem_set_rule(ebem_handle *h, int nGH);
+
+/* Number of lag blocks L = min(nT, ceil(maxVertexDist/(c_S dt)) + 1)
+ * (dir/BEMenerg_dir_calcNumBlocks.m:1-17). */
+int ebem_num_blocks(ebem_handle *h, int *L);
+
+/* Assemble E_0..E_{L-1} into the device-resident block store.  Replaces the
+ * SA block loop (SA/BEMenerg_coreSA_timeMarching.m:39,89 ->
+ * SAc/BEMenerg_coreSA_calcMatrixBlock.m) and the kernelV launches of
+ * dir/BEMenerg_dir_timeMarchingD.m:97-161.  precision: EBEM_PREC_*. */
+int ebem_assemble(ebem_handle *h, int precision);
+
+/* Block-store statistics: non-zero 3x3 blocks (all lags), culled pair.lags,
+ * (pair, level) integral items evaluated, bytes of the store. */
+int ebem_store_info(ebem_handle *h, int64_t *nnzb, int64_t *pair_lags, int64_t *items, int64_t *bytes);
+
+/* Export E_lag as a dense 3N x 3N column-major host matrix (the `matrix` of
+ * calcMatrixBlock.m / matrixSavedV{lag+1} of timeMarchingD.m:141-158). */
+int ebem_get_block(ebem_handle *h, int lag, double *out);
+
+/* Right-hand side beta (3N x nT col-major) on the device and, if beta_out is
+ * non-NULL, copied to the host (Tn/BEMenerg_core_calcTnBlock.m:27-58). */
+int ebem_rhs(ebem_handle *h, int datum, double *beta_out);
+
+/* MOT: factorise E_0 in FP64 on the GPU and march
+ * alpha_n = E_0^{-1}(beta_n - sum_{l=1}^{min(n,L)-1} E_l alpha_{n-l})
+ * (SA/BEMenerg_coreSA_timeMarching.m:45-113, FN/BEMenerg_coreFN_timeMarching.m:152-171).
+ * beta: host 3N x nT col-major, or NULL to use the device RHS of ebem_rhs.
+ * alpha_out: host 3N x nT col-major, or NULL to keep the density on the device. */
+int ebem_mot_solve(ebem_handle *h, const double *beta, double *alpha_out);
+
+/* Post-processing u(x_p, t_k) for npts points and nt times (analytic, ppA/
+ * BEMenerg_postProcA_singlePoint.m; signature role of BEMenerg_ppD_setup,
+ * dirPP/BEMenerg_ppD_setup.m:1).  density: host 3N x nT col-major or NULL for
+ * the device density of ebem_mot_solve.  x: npts x 3 row-major; t: nt times.
+ * u_out: 3 x npts x nt (u_out[3*(p + npts*k) + c]). */
+int ebem_pp_eval(ebem_handle *h, const double *density, int npts, const double *x, int nt, const double *t,
+                 double *u_out);
+
+/* Unit hook for the analytic inner integration (functions/BEMenerg_analiticIntg via calcMatrixSubBlock.m):
+ * the kernel-combined, /(4 pi rho)-scaled 3x3 integral (column-major) over
+ * triangle `tri` of the mesh for outer point x at time t. */
+int ebem_node_integral(ebem_handle *h, const double *x, int tri, double t, double *out9);
+
+/* Timing of the last ebem_assemble / ebem_mot_solve / ebem_pp_eval call, in
+ * milliseconds, from HIP events recorded on the library's stream:
+ * out[0] assembly integral kernel, out[1] assembly total, out[2] E_0 factorisation,
+ * out[3] history-sum kernel total, out[4] E_0 apply total, out[5] MOT total,
+ * out[6] post-processing total, out[7] history-sum launches. */
+int ebem_timings(ebem_handle *h, double *out8);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

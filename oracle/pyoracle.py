@@ -1,0 +1,172 @@
+"""ctypes wrapper over oracle/libebem_oracle.so (test infrastructure only)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libebem_oracle.so")
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class _Param(C.Structure):
+    _fields_ = [("rho", C.c_double), ("mu", C.c_double), ("lam", C.c_double),
+                ("velP", C.c_double), ("velS", C.c_double), ("Tfin", C.c_double),
+                ("nT", C.c_int)]
+
+
+class _Mesh(C.Structure):
+    _fields_ = [("N", C.c_int), ("Nv", C.c_int), ("coords", _dp), ("tri", _ip),
+                ("normal", _dp), ("area", _dp), ("center", _dp), ("maxL", _dp)]
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    """Problem-bound oracle: material, time grid, mesh and outer GH rule."""
+
+    def __init__(self, coords, tri, rho=1.0, mu=1.0, lam=1.0, Tfin=1.0, nT=16, nGH=7):
+        if not os.path.exists(_LIB):
+            build()
+        self.lib = C.CDLL(_LIB)
+        self._sig()
+        self.coords = np.ascontiguousarray(coords, np.float64)
+        self.tri = np.ascontiguousarray(tri, np.int32)
+        N = self.tri.shape[0]
+        self.N = N
+        self.normal = np.zeros((N, 3))
+        self.area = np.zeros(N)
+        self.center = np.zeros((N, 3))
+        self.maxL = np.zeros(N)
+        self.lib.orc_mesh_derive(N, _p(self.coords), _p(self.tri, _ip), _p(self.normal), _p(self.area),
+                                 _p(self.center), _p(self.maxL))
+        velP = np.sqrt((lam + 2 * mu) / rho)
+        velS = np.sqrt(mu / rho)
+        self.param = _Param(rho, mu, lam, velP, velS, Tfin, nT)
+        self.mesh = _Mesh(N, self.coords.shape[0], _p(self.coords), _p(self.tri, _ip), _p(self.normal),
+                          _p(self.area), _p(self.center), _p(self.maxL))
+        self.nGH = nGH
+        self.ghn = np.zeros((nGH, 3))
+        self.ghw = np.zeros(nGH)
+        if self.lib.orc_gauss_hammer(nGH, _p(self.ghn), _p(self.ghw)) != 0:
+            raise ValueError("unsupported Gauss-Hammer rule")
+        self.velP, self.velS, self.rho, self.nT, self.Tfin = velP, velS, rho, nT, Tfin
+        self.dt = Tfin / nT
+
+    def _sig(self):
+        L = self.lib
+        L.orc_mesh_derive.argtypes = [C.c_int, _dp, _ip, _dp, _dp, _dp, _dp]
+        L.orc_gauss_hammer.argtypes = [C.c_int, _dp, _dp]
+        L.orc_gauss_hammer.restype = C.c_int
+        L.orc_num_blocks.argtypes = [C.POINTER(_Param), C.c_int, _dp]
+        L.orc_num_blocks.restype = C.c_int
+        L.orc_node_ris.argtypes = [C.POINTER(_Param), _dp, _dp, _dp, C.c_double, _dp]
+        L.orc_node_ris.restype = C.c_int
+        L.orc_pair_block.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, _dp, C.c_int,
+                                     C.c_int, C.c_int, _dp]
+        L.orc_pair_block.restype = C.c_int
+        L.orc_matrix_block_rows.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, _dp,
+                                            C.c_int, C.c_int, C.c_int, _dp, C.c_int]
+        L.orc_matrix_block_rows.restype = C.c_int
+        L.orc_rhs.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, _dp, C.c_int, _dp]
+        L.orc_datum.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp]
+        L.orc_lu.argtypes = [C.c_int, _dp, _ip]
+        L.orc_lu.restype = C.c_int
+        L.orc_lu_solve.argtypes = [C.c_int, _dp, _ip, _dp]
+        L.orc_mot.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_mot.restype = C.c_int
+        L.orc_pp_block.argtypes = [C.POINTER(_Param), _dp, _dp, _dp, C.c_double, _dp]
+        L.orc_pp_point.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), _dp, _dp, C.c_double, _dp]
+        L.orc_counters.argtypes = [C.POINTER(C.c_longlong)]
+        L.orc_field_record.argtypes = [_dp, _dp, _dp]
+
+    # -- pieces -------------------------------------------------------------
+    def num_blocks(self) -> int:
+        return int(self.lib.orc_num_blocks(C.byref(self.param), self.coords.shape[0], _p(self.coords)))
+
+    def node_ris(self, x, T, n, t):
+        """ris1..ris6 (6 x 3 x 3) for outer point x, field triangle T (3x3 rows=vertices), time t."""
+        out = np.zeros(54)
+        x = np.ascontiguousarray(x, np.float64)
+        T = np.ascontiguousarray(T, np.float64)
+        n = np.ascontiguousarray(n, np.float64)
+        cls = self.lib.orc_node_ris(C.byref(self.param), _p(x), _p(T), _p(n), float(t), _p(out))
+        return out.reshape(6, 3, 3).transpose(0, 2, 1).copy(), cls
+
+    def pair_block(self, lag, S, F):
+        out = np.zeros(9)
+        ok = self.lib.orc_pair_block(C.byref(self.param), C.byref(self.mesh), self.nGH, _p(self.ghn),
+                                     _p(self.ghw), lag, S, F, _p(out))
+        return out.reshape(3, 3).T.copy(), ok
+
+    def matrix_block(self, lag, rows=None, nthreads=None):
+        """Dense E_lag (3N x 3N) as a numpy array (row index = test DOF)."""
+        N = self.N
+        out = np.zeros((3 * N, 3 * N), order="F")
+        r0, r1 = (0, N) if rows is None else rows
+        nt = nthreads or os.cpu_count() or 1
+        any_ = self.lib.orc_matrix_block_rows(C.byref(self.param), C.byref(self.mesh), self.nGH, _p(self.ghn),
+                                              _p(self.ghw), lag, r0, r1, out.ctypes.data_as(_dp), nt)
+        return np.ascontiguousarray(out), bool(any_)
+
+    def rhs(self, datum):
+        out = np.zeros((3 * self.N, self.nT), order="F")
+        self.lib.orc_rhs(C.byref(self.param), C.byref(self.mesh), self.nGH, _p(self.ghn), _p(self.ghw),
+                         int(datum), out.ctypes.data_as(_dp))
+        return np.ascontiguousarray(out)
+
+    def datum(self, kind, x, nrm, t):
+        g = np.zeros(3)
+        self.lib.orc_datum(int(kind), _p(np.ascontiguousarray(x, np.float64)),
+                           _p(np.ascontiguousarray(nrm, np.float64)), float(t), _p(g))
+        return g
+
+    def mot(self, E, beta):
+        """E: (L, 3N, 3N) numpy (row = test DOF), beta (3N, nT).  Returns alpha (3N, nT)."""
+        L, n3, _ = E.shape
+        Ef = np.ascontiguousarray(E.transpose(0, 2, 1))  # column-major per lag
+        b = np.asfortranarray(beta)
+        a = np.zeros_like(b, order="F")
+        rc = self.lib.orc_mot(n3, L, b.shape[1], _p(Ef), b.ctypes.data_as(_dp), a.ctypes.data_as(_dp))
+        if rc != 0:
+            raise np.linalg.LinAlgError("singular E0")
+        return np.ascontiguousarray(a)
+
+    def pp_block(self, x, j, tau):
+        T = np.ascontiguousarray(self.coords[self.tri[j]])
+        out = np.zeros(9)
+        self.lib.orc_pp_block(C.byref(self.param), _p(np.ascontiguousarray(x, np.float64)), _p(T),
+                              _p(np.ascontiguousarray(self.normal[j])), float(tau), _p(out))
+        return out.reshape(3, 3).T.copy()
+
+    def pp_point(self, density, x, t):
+        d = np.asfortranarray(density)
+        u = np.zeros(3)
+        self.lib.orc_pp_point(C.byref(self.param), C.byref(self.mesh), d.ctypes.data_as(_dp),
+                              _p(np.ascontiguousarray(x, np.float64)), float(t), _p(u))
+        return u
+
+    def field_record(self, T, n):
+        rec = np.zeros(32)
+        self.lib.orc_field_record(_p(np.ascontiguousarray(T, np.float64)), _p(np.ascontiguousarray(n, np.float64)),
+                                  _p(rec))
+        return rec
+
+    def counters(self):
+        out = (C.c_longlong * 12)()
+        self.lib.orc_counters(out)
+        return list(out)
+
+    def counters_reset(self):
+        self.lib.orc_counters_reset()

@@ -1,0 +1,32 @@
+// Test-only host build of the device SA math (ebem_sa.hpp is __host__ __device__):
+// lets the CPU test-suite compare the GPU formulation with the oracle without a GPU.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../elastodynamicebem_amd/csrc/ebem_sa.hpp"
+
+using namespace ebem;
+
+extern "C" {
+
+// Field triangle record from three vertices, frame and rotation as built by the
+// library (the host copy of build_tri is not exported, so the harness takes the
+// record fields from the caller): rec = 32 doubles in FieldTri layout.
+int sa_point_block(const double* rec, const double* x, double t, double cP, double cS, double rho, double* out9) {
+    FieldTri F;
+    std::memcpy(&F, rec, sizeof(FieldTri));
+    Level K = make_level(t, cP, cS);
+    NodeAcc acc;
+    acc.s = 0.0;
+    for (int q = 0; q < 6; q++) acc.m[q] = 0.0;
+    double xx[3] = {x[0], x[1], x[2]};
+    NodeGeo G = node_geo(xx, F);
+    int type = node_eval(G, K, 1.0, acc);
+    double W[6];
+    finish_rotation(acc, F.C, W);
+    const int qq[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    for (int k = 0; k < 9; k++) out9[k] = W[qq[k]] / (4 * kPi * rho);
+    return type;
+}
+}
