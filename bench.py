@@ -1,0 +1,158 @@
+"""Headline benchmark: energetic space-time Galerkin BEM single-layer hot path on one MI355X.
+
+Workload (the FP64 plain method at scale): unit icosphere of level 3
+(sphereUniform_3.mesh, 1280 triangles, 3840 DOF), lambda = mu = rho = 1
+(c_P = sqrt 3, c_S = 1), t in [0, 4] with 128 steps, L = 66 lag blocks,
+7-point Gauss-Hammer outer rule, Dirichlet datum n(x) t^2 e^{-4t}.
+
+One step = the whole hot path on device-resident inputs: assembly of
+E_0..E_65 (5.61e7 culled element-pair.lag integrals), right-hand side, E_0
+factorisation and the 128-step marching-on-in-time solve.
+value = culled pair.lag integrals / step time.
+
+Also reported: a roofline object for the dominant kernel (the assembly
+integral kernel k_levels) and the history-sum kernel, and the CPU oracle
+(FP64 C/OpenMP restatement of the reference SA path) timed on the host on a
+bounded sample of the same assembly.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic FP64 operations per (pair, level) item of k_levels on this
+# workload (7 outer nodes): mean over a uniform sample of 200000 of the
+# 59284540 items, counted by the op-counting host build of the device math
+# (python tests/opcount.py 200000 -> 8298.5 +- 8.5); each + - * / (a fused
+# multiply-add as 2), sqrt and libm call counts once.
+ALG_FLOP_PER_ITEM = 8298.5
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def workload():
+    from elastodynamicebem_amd import api
+    from elastodynamicebem_amd.mesh import read_mesh
+    m = read_mesh(os.path.join(ROOT, "tests", "golden", "meshes", "sphereUniform_3.mesh"))
+    pb = api.PbParam(rho=1.0, mu=1.0, lambda_=1.0, Tfin=4.0, nT=128, domainType="sphereUniform", lev=3)
+    return m, pb
+
+
+def history_bytes(m, pb, L):
+    """Algorithmic bytes the 128 history sums stream: 48 B of block values + 4 B
+    column index per block with 1 <= lag <= min(n, L-1), summed over steps."""
+    c, mx = m.center, m.maxL
+    dt, cS, cP = pb.Tfin / pb.nT, pb.velS, pb.velP
+    d = np.sqrt(((c[None, :, :] - c[:, None, :]) ** 2).sum(-1))
+    dmin = d - mx[None, :] - mx[:, None]
+    dmax = d + mx[None, :] + mx[:, None]
+    per_lag = np.array([np.count_nonzero(((l - 1) * cS * dt < dmax) & ((l + 1) * cP * dt >= dmin))
+                        for l in range(L)], dtype=np.int64)
+    blocks = sum(int(per_lag[1:min(n, L - 1) + 1].sum()) for n in range(pb.nT))
+    return blocks * 52, per_lag
+
+
+def cpu_baseline(m, pb, seconds_hint=True):
+    """Oracle assembly on the first rows of every lag; pair.lags / s on the host."""
+    from oracle import Oracle
+    threads = min(os.cpu_count() or 1, 16)
+    o = Oracle(m.coordinates, m.triangles, Tfin=pb.Tfin, nT=pb.nT)
+    L = o.num_blocks()
+    rows = (0, 96)
+    c, mx = m.center, m.maxL
+    dt = pb.Tfin / pb.nT
+    d = np.sqrt(((c[rows[0]:rows[1], None, :] - c[None, :, :]) ** 2).sum(-1))
+    dmin = d - mx[None, :] - mx[rows[0]:rows[1], None]
+    dmax = d + mx[None, :] + mx[rows[0]:rows[1], None]
+    units = sum(np.count_nonzero(((l - 1) * pb.velS * dt < dmax) & ((l + 1) * pb.velP * dt >= dmin))
+                for l in range(L))
+    t0 = time.perf_counter()
+    for l in range(L):
+        o.matrix_block(l, rows=rows, nthreads=threads)
+    el = time.perf_counter() - t0
+    return {"value": units / el, "unit": "pair·lag integrals/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C/OpenMP FP64 restatement of the reference SA path) assembling test rows "
+                      f"{rows[0]}-{rows[1] - 1} of E_0..E_{L - 1} ({units} culled pair.lags, {el:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch  # noqa: F401  (HIP runtime present)
+    from elastodynamicebem_amd import api
+
+    m, pb = workload()
+    s = api.Solver(m, pb, 7, device=0)
+    L = s.num_blocks()
+
+    def step():
+        s.assemble(api.PREC_FP64)
+        s.rhs(api.DATUM_SPHERE_T2EXP)
+        s.lib.ebem_mot_solve(s.h, None, None)
+
+    for _ in range(args.warmup):
+        step()
+    times, tims = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - t0)
+        tims.append(s.timings())
+    info = s.store_info()
+    ms_step = 1e3 * float(np.mean(times))
+    units = info["pair_lags"]
+    lev_ms = float(np.mean([t["levels_ms"] for t in tims]))
+    asm_ms = float(np.mean([t["assembly_ms"] for t in tims]))
+    mot_ms = float(np.mean([t["mot_ms"] for t in tims]))
+    hist_ms = float(np.mean([t["history_ms"] for t in tims]))
+    fac_ms = float(np.mean([t["factor_ms"] for t in tims]))
+    hb, _ = history_bytes(m, pb, L)
+
+    roof = None
+    if ALG_FLOP_PER_ITEM:
+        ach = ALG_FLOP_PER_ITEM * info["items"] / (lev_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_levels", "achieved": ach,
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
+    hist_gbs = hb / (hist_ms * 1e-3) / 1e9
+    out = {
+        "metric": "element-pair·lag integrals/s + solve time at ≤1e-6 rel err vs FP64, one MI355X",
+        "value": units / (ms_step * 1e-3),
+        "unit": "pair·lag integrals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "sphereUniform_3 icosphere (1280 triangles, 3840 DOF), T=4, nT=128, L=66, "
+                               "SA 7-point outer rule, plain FP64: assembly + RHS + E0 factorisation + MOT",
+                   "pair_lags": units, "items": info["items"], "store_bytes": info["bytes"]},
+        "breakdown_ms": {"assembly": asm_ms, "assembly_integral_kernel": lev_ms, "mot_total": mot_ms,
+                         "e0_factor": fac_ms, "history_sum": hist_ms},
+        "history_sum": {"bytes": hb, "achieved_GBs": hist_gbs, "peak_GBs": PEAK_HBM_GBS,
+                        "frac": hist_gbs / PEAK_HBM_GBS},
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+    s.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(m, pb)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,76 @@
+"""ctypes binding of the native library libebem_hip.so (C ABI of include/ebem.h).
+
+The library is the only compute path: if it is missing or cannot be loaded this
+module raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libebem_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_vp = C.c_void_p
+
+# symbol -> (restype, argtypes); every symbol declared in include/ebem.h
+SIGNATURES = {
+    "ebem_last_error": (C.c_char_p, []),
+    "ebem_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+    "ebem_destroy": (C.c_int, [_vp]),
+    "ebem_set_mesh": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _ip, C.c_int]),
+    "ebem_set_problem": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
+    "ebem_set_rule": (C.c_int, [_vp, C.c_int]),
+    "ebem_num_blocks": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "ebem_assemble": (C.c_int, [_vp, C.c_int]),
+    "ebem_store_info": (C.c_int, [_vp, _i64p, _i64p, _i64p, _i64p]),
+    "ebem_get_block": (C.c_int, [_vp, C.c_int, _dp]),
+    "ebem_rhs": (C.c_int, [_vp, C.c_int, _dp]),
+    "ebem_mot_solve": (C.c_int, [_vp, _dp, _dp]),
+    "ebem_pp_eval": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
+    "ebem_node_integral": (C.c_int, [_vp, _dp, C.c_int, C.c_double, _dp]),
+    "ebem_timings": (C.c_int, [_vp, _dp]),
+}
+
+_lib = None
+
+
+class EbemError(RuntimeError):
+    """A native call returned a negative EBEM_E* code."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def load() -> C.CDLL:
+    """Load libebem_hip.so (built by elastodynamicebem_amd.build) or raise."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"native library missing: {LIB_PATH} (run __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().ebem_last_error().decode(errors="replace")
+        raise EbemError(rc, msg)
+
+
+def dptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)

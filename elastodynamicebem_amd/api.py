@@ -1,0 +1,259 @@
+"""Python mirror of the reference's MATLAB entry points on the single-layer path.
+
+Same names, argument meaning and error behaviour as the reference functions
+cited below; the computation runs in the native HIP library (libebem_hip.so).
+
+  readInputFile                 util/fileRead/readInputFile.m
+  readSpaceMesh                 util/fileRead/readSpaceMesh.m
+  BEMenerg_setupCore            util/setup/BEMenerg_setupCore.m   ("SA nn" only)
+  BEMenerg_dir_calcNumBlocks    functions/BEMenerg_dir/BEMenerg_dir_calcNumBlocks.m
+  BEMenerg_coreSA_timeMarching  functions/BEMenerg_indir/BEMenerg_core/BEMenerg_coreSA/BEMenerg_coreSA_timeMarching.m
+  BEMenerg_dir_timeMarchingD    functions/BEMenerg_dir/BEMenerg_dir_timeMarchingD.m (single-layer system)
+  BEMenerg_postProcA_setup      functions/BEMenerg_indir/BEMenerg_postProc/BEMenerg_postProcA/BEMenerg_postProcA_setup.m
+  BEMenerg_ppD_setup            functions/BEMenerg_dirPP/BEMenerg_ppD_setup.m (single-layer part)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .mesh import SpaceMesh, read_mesh
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+DATUM_SCREEN_SIN2 = 0
+DATUM_SPHERE_T2EXP = 1
+DATUM_REF_SPHERE = 2
+DATUM_REF_SCREEN = 3
+
+# datum hard-coded per domainType in Tn/BEMenerg_core_calcNucleoTn.m:133-150
+_DOMAIN_DATUM = {"sphereUniform": DATUM_REF_SPHERE, "sphereNotUniform": DATUM_REF_SPHERE,
+                 "screenUniform": DATUM_REF_SCREEN, "screenGraded": DATUM_REF_SCREEN}
+
+
+@dataclass
+class PbParam:
+    """pbParam of readInputFile.m:36-93."""
+    rho: float = 1.0
+    mu: float = 1.0
+    nu: float = 0.25
+    lambda_: float = 1.0
+    Tfin: float = 1.0
+    nT: int = 16
+    domainType: str = "screenUniform"
+    lev: int = 2
+    BIE: str = "EFIE"
+    BOU: str = "DIR"
+    MTDTN: str | None = None
+
+    @property
+    def velP(self) -> float:
+        return math.sqrt((self.lambda_ + 2 * self.mu) / self.rho)
+
+    @property
+    def velS(self) -> float:
+        return math.sqrt(self.mu / self.rho)
+
+
+@dataclass
+class MethodInfo:
+    typeIntg: str = "SA"
+    numNodiExt: int = 7
+    extra: dict = field(default_factory=dict)
+
+
+def readInputFile(path: str) -> PbParam:
+    """Ten '%header'/value line pairs (readInputFile.m:38-93)."""
+    if not os.path.exists(path):
+        raise FileNotFoundError("Impossibile aprire file coi dati del problema")
+    with open(path, "r", errors="replace") as fh:
+        lines = [ln.rstrip("\n") for ln in fh]
+    vals = [lines[k] for k in range(1, len(lines), 2)]
+    if len(vals) < 10:
+        raise ValueError("problem file has fewer than 10 header/value pairs")
+    p = PbParam(rho=float(vals[0]), mu=float(vals[1]), nu=float(vals[2]), lambda_=float(vals[3]),
+                Tfin=float(vals[4]), nT=int(float(vals[5])), domainType=vals[6].strip(), lev=int(float(vals[7])),
+                BIE=vals[8].strip(), BOU=vals[9].strip())
+    if p.BOU == "TEST" and len(vals) > 10:
+        p.MTDTN = vals[10].strip()
+    return p
+
+
+def readSpaceMesh(path: str) -> SpaceMesh:
+    """MEDIT mesh with derived data (readSpaceMesh.m:13-151)."""
+    return read_mesh(path)
+
+
+def GaussHammerComposite_1(n: int):
+    """Outer rule of GaussHammerComposite(1, n) (same table as the native library)."""
+    from . import quad
+    return quad.gauss_hammer(n)
+
+
+def BEMenerg_setupCore(methodSelected: str):
+    """Parse "SA nn" (BEMenerg_setupCore.m:3-21).  Other method families of the
+    reference (MX.*, GPU, FN) select numerical inner quadratures that this
+    library does not provide; they raise like an unknown method does."""
+    toks = methodSelected.split()
+    if not toks:
+        raise ValueError("Metodo non disponibile")
+    mi = MethodInfo(typeIntg=toks[0])
+    if mi.typeIntg != "SA":
+        raise NotImplementedError(f"method family {mi.typeIntg!r} is not on the analytic SA path")
+    if len(toks) < 2:
+        raise ValueError("Valori nodi esterni non validi")
+    mi.numNodiExt = int(toks[1])
+    if mi.numNodiExt not in (3, 7, 12, 19):
+        raise ValueError("Valori nodi esterni non validi")
+    EXTn, EXTw = GaussHammerComposite_1(mi.numNodiExt)
+    return mi, EXTn, EXTw, None, None, None, None
+
+
+class Solver:
+    """One native handle: mesh + problem + outer rule, device-resident store."""
+
+    def __init__(self, mesh: SpaceMesh, pb: PbParam, nGH: int = 7, device: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self.lib.ebem_create(C.byref(h), device))
+        self.h = h
+        self.mesh = mesh
+        self.pb = pb
+        self.N = mesh.numberTriangles
+        self.nT = pb.nT
+        coords = np.ascontiguousarray(mesh.coordinates, np.float64)
+        tri = np.ascontiguousarray(mesh.triangles, np.int32)
+        _lib.check(self.lib.ebem_set_mesh(h, self.N, coords.shape[0], _lib.dptr(coords),
+                                          tri.ctypes.data_as(_lib._ip), 0))
+        _lib.check(self.lib.ebem_set_problem(h, pb.rho, pb.mu, pb.lambda_, pb.Tfin, pb.nT))
+        _lib.check(self.lib.ebem_set_rule(h, nGH))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ebem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- pieces --------------------------------------------------------------
+    def num_blocks(self) -> int:
+        L = C.c_int()
+        _lib.check(self.lib.ebem_num_blocks(self.h, C.byref(L)))
+        return L.value
+
+    def assemble(self, precision: int = PREC_FP64) -> None:
+        _lib.check(self.lib.ebem_assemble(self.h, precision))
+
+    def store_info(self) -> dict:
+        v = [C.c_int64() for _ in range(4)]
+        _lib.check(self.lib.ebem_store_info(self.h, *[C.byref(x) for x in v]))
+        return dict(nnzb=v[0].value, pair_lags=v[1].value, items=v[2].value, bytes=v[3].value)
+
+    def get_block(self, lag: int) -> np.ndarray:
+        """Dense E_lag (3N x 3N), row index = test DOF."""
+        n3 = 3 * self.N
+        out = np.zeros((n3, n3), order="F")
+        _lib.check(self.lib.ebem_get_block(self.h, lag, out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def rhs(self, datum: int) -> np.ndarray:
+        out = np.zeros((3 * self.N, self.nT), order="F")
+        _lib.check(self.lib.ebem_rhs(self.h, datum, out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def mot_solve(self, beta: np.ndarray | None = None) -> np.ndarray:
+        out = np.zeros((3 * self.N, self.nT), order="F")
+        b = None if beta is None else np.asfortranarray(beta, dtype=np.float64)
+        _lib.check(self.lib.ebem_mot_solve(self.h, None if b is None else b.ctypes.data_as(_lib._dp),
+                                           out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def pp_eval(self, x: np.ndarray, t: np.ndarray, density: np.ndarray | None = None) -> np.ndarray:
+        """u (3, npts, nt)."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(t, np.float64).ravel()
+        out = np.zeros((t.size, x.shape[0], 3))
+        d = None if density is None else np.asfortranarray(density, dtype=np.float64)
+        _lib.check(self.lib.ebem_pp_eval(self.h, None if d is None else d.ctypes.data_as(_lib._dp), x.shape[0],
+                                         _lib.dptr(x), t.size, _lib.dptr(t), _lib.dptr(out)))
+        return out.transpose(2, 1, 0).copy()
+
+    def node_integral(self, x, tri: int, t: float) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros(9)
+        _lib.check(self.lib.ebem_node_integral(self.h, _lib.dptr(x), int(tri), float(t), _lib.dptr(out)))
+        return out.reshape(3, 3).T.copy()
+
+    def timings(self) -> dict:
+        out = np.zeros(8)
+        _lib.check(self.lib.ebem_timings(self.h, _lib.dptr(out)))
+        keys = ["levels_ms", "assembly_ms", "factor_ms", "history_ms", "apply_ms", "mot_ms", "pp_ms",
+                "history_launches"]
+        return dict(zip(keys, out.tolist()))
+
+
+def _datum_for(pb: PbParam, datum):
+    if datum is not None:
+        return datum
+    if pb.domainType in _DOMAIN_DATUM:
+        return _DOMAIN_DATUM[pb.domainType]
+    raise ValueError("Caso ancora non riportato/Errore nei dati")
+
+
+def BEMenerg_dir_calcNumBlocks(pbParam: PbParam, domainMesh: SpaceMesh) -> int:
+    with Solver(domainMesh, pbParam) as s:
+        return s.num_blocks()
+
+
+def BEMenerg_coreSA_timeMarching(pbParam: PbParam, domainMesh: SpaceMesh, methodInfo: MethodInfo, GHn=None,
+                                 GHw=None, *, datum=None, precision: int = PREC_FP64, device: int = 0):
+    """density (3N x nT): E_0..E_{L-1} assembly, RHS and MOT on the GPU."""
+    with Solver(domainMesh, pbParam, methodInfo.numNodiExt, device) as s:
+        s.assemble(precision)
+        s.rhs(_datum_for(pbParam, datum))
+        return s.mot_solve(None)
+
+
+def BEMenerg_dir_timeMarchingD(pbParam, domainMesh, methodInfo, GHn=None, GHw=None, GHCn=None, GHCw=None,
+                               DIAGn=None, DIAGw=None, *, datum=None, precision: int = PREC_FP64, device: int = 0):
+    """Single-layer (V) system of the direct Dirichlet driver with SA inner
+    integration.  The double-layer right-hand side of timeMarchingD.m:287 is
+    not part of this library (raises if a DD problem with lambda+mu == 0 is
+    given, like main.m:61-63)."""
+    if pbParam.lambda_ + pbParam.mu == 0:
+        raise ValueError("Problema non adatto a questa formulazione")
+    return BEMenerg_coreSA_timeMarching(pbParam, domainMesh, methodInfo, GHn, GHw, datum=datum,
+                                        precision=precision, device=device)
+
+
+def BEMenerg_postProcA_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, *, device: int = 0):
+    """campoVett: xSize x tSize array of 3-vectors (postProcA_setup.m:1-26)."""
+    xVal = np.asarray(xVal, float).reshape(-1, 3)
+    tVal = np.asarray(tVal, float).ravel()
+    if numPoints != xVal.shape[0] * tVal.size:
+        raise ValueError("numPoints does not match xVal x tVal")
+    with Solver(domainMesh, pbParam, 7, device) as s:
+        u = s.pp_eval(xVal, tVal, density)
+    return u.transpose(1, 2, 0)  # (xSize, tSize, 3)
+
+
+def BEMenerg_ppD_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, methodInfo=None, PPn=None, PPw=None,
+                       *, device: int = 0):
+    """Single-layer part of the direct post-processing (BEMenerg_ppD_setup.m:1)."""
+    return BEMenerg_postProcA_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, device=device)

@@ -24,7 +24,7 @@ def build(verbose: bool = False) -> str:
     """Compile every HIP source for gfx950 into elastodynamicebem_amd/libebem_hip.so."""
     out_tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=fast-honor-pragmas",
            "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
            "-o", out_tmp]
     cmd += [os.path.join(CSRC, s) for s in SOURCES]

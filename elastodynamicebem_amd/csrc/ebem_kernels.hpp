@@ -387,6 +387,173 @@ __global__ void k_point_block(const FieldTri* __restrict__ tri, int j, const dou
     for (int q = 0; q < 6; q++) out6[q] = W[q] * inv4pirho;
 }
 
+// ---------------------------------------------------------- post-process --
+//
+// ppA/BEMenerg_postProcA_singlePoint.m:
+//   u(x,t) = sum_{c=0}^{nhat} M(x, t - c dt) (alpha_c - alpha_{c-1}),  nhat = ceil(t/dt) - 1,
+// rewritten with tau_m = delta + m dt (delta = t - nhat dt, m = nhat - c) as
+//   u = sum_m D_m alpha_{nhat - m},  D_m = M_m - M_{m-1},  M_m = M(x, tau_m),
+// where D_m vanishes before the P front reaches the triangle and once the S
+// front has passed it.  Times sharing delta share every M_m.
+
+struct PPParams {
+    int N, nT;
+    double dt, cP, cS, delta, inv4pirho;
+    int mmax;
+    double trep;  // representative time of the group (largest nhat)
+    int nrep;     // its nhat: tau_m = trep - (nrep - m) dt, as singlePoint.m:37
+};
+
+// Window [ms, me] of m per (point, triangle); count = me - ms + 1 (0 if empty).
+__global__ void __launch_bounds__(256) k_pp_windows(const FieldTri* __restrict__ tri, const double* __restrict__ xs,
+                                                    int p0, int np, PPParams P, int32_t* __restrict__ ms,
+                                                    int32_t* __restrict__ me) {
+    int64_t tot = (int64_t)np * P.N;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        int p = p0 + (int)(q / P.N), j = (int)(q % P.N);
+        const FieldTri& F = tri[j];
+        double d0 = xs[3 * p] - F.center[0], d1 = xs[3 * p + 1] - F.center[1], d2 = xs[3 * p + 2] - F.center[2];
+        double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        double dmin = d - F.maxL, dmax = d + F.maxL;
+        // M_{ms-1} = 0: c_P tau_{ms-1} <= dmin (one step of margin)
+        double a = floor((dmin / P.cP - P.delta) / P.dt);
+        int lo = (int)fmax(0.0, fmin(a, (double)P.mmax + 1.0));
+        // D_m = 0 for m > me: c_S tau_{m-1} >= dmax (one step of margin)
+        double b = ceil((dmax / P.cS - P.delta) / P.dt) + 2.0;
+        int hi = (int)fmin(b, (double)P.mmax);
+        ms[q] = lo;
+        me[q] = hi;
+    }
+}
+
+// One workgroup per point of the chunk: exclusive prefix of the window sizes
+// (item offsets) and the item -> (point, triangle) map.
+__global__ void __launch_bounds__(256) k_pp_items(const int32_t* __restrict__ ms, const int32_t* __restrict__ me, int N,
+                                                  const int64_t* __restrict__ rowbase, int64_t* __restrict__ off,
+                                                  int32_t* __restrict__ item_q) {
+    __shared__ long long part[256];
+    const int r = blockIdx.x;
+    const int nper = (N + blockDim.x - 1) / blockDim.x;
+    const int j0 = threadIdx.x * nper;
+    const int64_t base = (int64_t)r * N;
+    long long s = 0;
+    for (int k = 0; k < nper; k++) {
+        int j = j0 + k;
+        if (j < N) s += max(0, me[base + j] - ms[base + j] + 1);
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        long long v = (threadIdx.x >= d) ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    long long run = rowbase[r] + (threadIdx.x ? part[threadIdx.x - 1] : 0);
+    for (int k = 0; k < nper; k++) {
+        int j = j0 + k;
+        if (j >= N) break;
+        int64_t q = base + j;
+        int n = max(0, me[q] - ms[q] + 1);
+        off[q] = run;
+        for (int c = 0; c < n; c++) item_q[run + c] = (int32_t)q;
+        run += n;
+    }
+}
+
+// M_m(x_p, tri j) for every item; 6 symmetric values, /(4 pi rho), entries
+// below 1e-14 zeroed (ppA/BEMenerg_postProcA_calcMatrix.m:20).
+__global__ void __launch_bounds__(256) k_pp_M(const FieldTri* __restrict__ tri, const double* __restrict__ xs, int p0,
+                                              PPParams P, int64_t nitems, const int32_t* __restrict__ item_q,
+                                              const int64_t* __restrict__ off, const int32_t* __restrict__ ms,
+                                              double* __restrict__ M) {
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < nitems;
+         it += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t q = item_q[it];
+        const int p = p0 + q / P.N, j = q % P.N;
+        const int m = ms[q] + (int)(it - off[q]);
+        double tau;
+        {
+#pragma clang fp contract(off)
+            tau = (m == P.nrep) ? P.trep : P.trep - (double)(P.nrep - m) * P.dt;
+        }
+        double W[6] = {0, 0, 0, 0, 0, 0};
+        if (tau > 0.0) {
+            const FieldTri& F = tri[j];
+            Level K = make_level(tau, P.cP, P.cS);
+            NodeAcc acc;
+            acc.s = 0.0;
+            for (int k = 0; k < 6; k++) acc.m[k] = 0.0;
+            double x[3] = {xs[3 * p], xs[3 * p + 1], xs[3 * p + 2]};
+            NodeGeo G = node_geo(x, F);
+            if (node_eval(G, K, 1.0, acc) != kNone) finish_rotation(acc, F.C, W);
+        }
+        double* o = M + it * 6;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            double v = W[k] * P.inv4pirho;
+            o[k] = fabs(v) < 1.0e-14 ? 0.0 : v;
+        }
+    }
+}
+
+// One workgroup per point; thread k handles requested time tk[k] (nhat[k]).
+// aT: density transposed to [triangle][step][3].
+__global__ void __launch_bounds__(256) k_pp_conv(const double* __restrict__ M, const int64_t* __restrict__ off,
+                                                 const int32_t* __restrict__ ms, const int32_t* __restrict__ me,
+                                                 const double* __restrict__ aT, int N, int nT, int p0,
+                                                 const int32_t* __restrict__ nhat, const int32_t* __restrict__ kidx,
+                                                 int nk, int npts, double* __restrict__ u) {
+    const int r = blockIdx.x;
+    const int p = p0 + r;
+    for (int kb = 0; kb < nk; kb += blockDim.x) {
+        const int kk = kb + threadIdx.x;
+        const bool act = kk < nk;
+        const int nh = act ? nhat[kk] : -1;
+        double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+        for (int j = 0; j < N; j++) {
+            const int64_t q = (int64_t)r * N + j;
+            const int a = ms[q], b = me[q];
+            if (a > b) continue;
+            const int64_t o = off[q];
+            const double* aj = aT + (int64_t)j * nT * 3;
+            double pv[6] = {0, 0, 0, 0, 0, 0};
+            for (int m = a; m <= b; m++) {
+                if (m > nh) break;
+                const double* Mm = M + (o + (m - a)) * 6;
+                double d[6];
+#pragma unroll
+                for (int k = 0; k < 6; k++) {
+                    d[k] = Mm[k] - pv[k];
+                    pv[k] = Mm[k];
+                }
+                const double* al = aj + (int64_t)(nh - m) * 3;
+                const double x0 = al[0], x1 = al[1], x2 = al[2];
+                u0 += d[0] * x0 + d[1] * x1 + d[2] * x2;
+                u1 += d[1] * x0 + d[3] * x1 + d[4] * x2;
+                u2 += d[2] * x0 + d[4] * x1 + d[5] * x2;
+            }
+        }
+        if (act) {
+            int64_t k = kidx[kk];
+            u[3 * ((int64_t)p + (int64_t)npts * k) + 0] = u0;
+            u[3 * ((int64_t)p + (int64_t)npts * k) + 1] = u1;
+            u[3 * ((int64_t)p + (int64_t)npts * k) + 2] = u2;
+        }
+    }
+}
+
+// aT[(j*nT + n)*3 + c] = alpha[n*3N + 3j + c]
+__global__ void k_alpha_T(const double* __restrict__ alpha, int N, int nT, double* __restrict__ aT) {
+    int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t tot = (int64_t)N * nT * 3;
+    if (id >= tot) return;
+    int c = (int)(id % 3);
+    int64_t r = id / 3;
+    int n = (int)(r % nT), j = (int)(r / nT);
+    aT[id] = alpha[(int64_t)n * 3 * N + 3 * j + c];
+}
+
 // ------------------------------------------------------------------ RHS --
 
 __device__ inline void datum(int kind, const double* x, const double* nrm, double t, double* g) {

@@ -353,7 +353,7 @@ int assemble(ebem_handle* h, int precision) {
     if (precision != EBEM_PREC_FP64) return fail(EBEM_EINVAL, "precision %d not available in this build", precision);
     compute_L(h);
     const int N = h->N, L = h->L;
-    if (L > 32000) return fail(EBEM_EINVAL, "too many lag blocks (%d)", L);
+    if (L > 4096) return fail(EBEM_EINVAL, "too many lag blocks (%d > 4096)", L);
     AsmParams P;
     P.N = N;
     P.L = L;
@@ -544,6 +544,133 @@ int mot(ebem_handle* h, const double* beta, double* alpha_out) {
         HIPC(hipMemcpyAsync(alpha_out, h->d_alpha.p, sizeof(double) * (size_t)n3 * nT, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
     h->have_alpha = true;
+    return 0;
+}
+
+// Post-processing: times grouped by delta = t - nhat dt (shared M_m), points
+// chunked so that the M scratch stays within budget.
+int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, int nt, const double* t,
+            double* u_out) {
+#pragma clang fp contract(off)
+    if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
+    if (npts <= 0 || nt <= 0 || !x || !t || !u_out) return fail(EBEM_EINVAL, "bad post-processing arguments");
+    if (!density && !h->have_alpha) return fail(EBEM_EINVAL, "no density: pass one or run ebem_mot_solve first");
+    const int N = h->N, nT = h->nT;
+    const double dt = h->dt;
+    std::vector<int> nhat(nt);
+    std::vector<double> delta(nt);
+    for (int k = 0; k < nt; k++) {
+        if (!(t[k] > 0)) {
+            nhat[k] = -1;
+            delta[k] = 0;
+            continue;
+        }
+        nhat[k] = (int)std::ceil(t[k] / dt) - 1;  // singlePoint.m:21
+        if (nhat[k] > nT - 1) return fail(EBEM_EINVAL, "time %g beyond the density's last step", t[k]);
+        delta[k] = t[k] - nhat[k] * dt;
+    }
+    Timer tall;
+    tall.start(h->stream);
+    DevBuf<double> d_x, d_aT, d_u, d_dens, d_M;
+    HIPC(d_x.alloc(3 * (size_t)npts));
+    HIPC(hipMemcpyAsync(d_x.p, x, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, h->stream));
+    const double* dens = h->d_alpha.p;
+    if (density) {
+        HIPC(d_dens.alloc((size_t)3 * N * nT));
+        HIPC(hipMemcpyAsync(d_dens.p, density, sizeof(double) * 3 * N * nT, hipMemcpyHostToDevice, h->stream));
+        dens = d_dens.p;
+    }
+    HIPC(d_aT.alloc((size_t)3 * N * nT));
+    {
+        int64_t tot = (int64_t)3 * N * nT;
+        hipLaunchKernelGGL(k_alpha_T, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, dens, N, nT,
+                           d_aT.p);
+        HIPC(hipGetLastError());
+    }
+    HIPC(d_u.alloc((size_t)3 * npts * nt));
+    HIPC(hipMemsetAsync(d_u.p, 0, sizeof(double) * 3 * npts * nt, h->stream));
+    // group times by delta
+    std::vector<int> order;
+    for (int k = 0; k < nt; k++)
+        if (nhat[k] >= 0) order.push_back(k);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return delta[a] < delta[b]; });
+    const int P0 = std::min(npts, 256);  // points per chunk
+    DevBuf<int32_t> d_ms, d_me, d_iq, d_nh, d_kidx;
+    DevBuf<int64_t> d_off, d_rb;
+    HIPC(d_ms.alloc((size_t)P0 * N));
+    HIPC(d_me.alloc((size_t)P0 * N));
+    HIPC(d_off.alloc((size_t)P0 * N));
+    HIPC(d_rb.alloc(P0));
+    size_t g0 = 0;
+    while (g0 < order.size()) {
+        size_t g1 = g0 + 1;
+        while (g1 < order.size() && delta[order[g1]] - delta[order[g0]] <= 1e-12 * dt) g1++;
+        std::vector<int32_t> nh, kid;
+        int mmax = -1, krep = order[g0];
+        for (size_t q = g0; q < g1; q++) {
+            nh.push_back(nhat[order[q]]);
+            kid.push_back(order[q]);
+            if (nhat[order[q]] > mmax) {
+                mmax = nhat[order[q]];
+                krep = order[q];
+            }
+        }
+        PPParams P;
+        P.N = N;
+        P.nT = nT;
+        P.dt = dt;
+        P.cP = h->cP;
+        P.cS = h->cS;
+        P.delta = delta[krep];
+        P.inv4pirho = 1.0 / (4 * kPi * h->rho);
+        P.mmax = mmax;
+        P.trep = t[krep];
+        P.nrep = mmax;
+        HIPC(d_nh.alloc(nh.size()));
+        HIPC(d_kidx.alloc(kid.size()));
+        HIPC(hipMemcpyAsync(d_nh.p, nh.data(), sizeof(int32_t) * nh.size(), hipMemcpyHostToDevice, h->stream));
+        HIPC(hipMemcpyAsync(d_kidx.p, kid.data(), sizeof(int32_t) * kid.size(), hipMemcpyHostToDevice, h->stream));
+        for (int p0 = 0; p0 < npts; p0 += P0) {
+            const int np = std::min(P0, npts - p0);
+            int64_t tot = (int64_t)np * N;
+            int64_t blocks = std::min<int64_t>((tot + 255) / 256, (int64_t)h->CUs * 64);
+            hipLaunchKernelGGL(k_pp_windows, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0,
+                               np, P, d_ms.p, d_me.p);
+            HIPC(hipGetLastError());
+            // per-point item counts on the host (np*N ints)
+            std::vector<int32_t> ms((size_t)np * N), me((size_t)np * N);
+            HIPC(hipMemcpyAsync(ms.data(), d_ms.p, sizeof(int32_t) * ms.size(), hipMemcpyDeviceToHost, h->stream));
+            HIPC(hipMemcpyAsync(me.data(), d_me.p, sizeof(int32_t) * me.size(), hipMemcpyDeviceToHost, h->stream));
+            HIPC(hipStreamSynchronize(h->stream));
+            std::vector<int64_t> rb(np);
+            int64_t run = 0;
+            for (int r = 0; r < np; r++) {
+                rb[r] = run;
+                for (int j = 0; j < N; j++) run += std::max(0, me[(size_t)r * N + j] - ms[(size_t)r * N + j] + 1);
+            }
+            const int64_t nitems = run;
+            HIPC(hipMemcpyAsync(d_rb.p, rb.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice, h->stream));
+            if (nitems == 0) continue;
+            HIPC(d_iq.alloc((size_t)nitems));
+            HIPC(d_M.alloc((size_t)nitems * 6));
+            hipLaunchKernelGGL(k_pp_items, dim3(np), dim3(256), 0, h->stream, d_ms.p, d_me.p, N, d_rb.p, d_off.p,
+                               d_iq.p);
+            HIPC(hipGetLastError());
+            int64_t mb = std::min<int64_t>((nitems + 255) / 256, (int64_t)h->CUs * 16);
+            hipLaunchKernelGGL(k_pp_M, dim3((unsigned)mb), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P, nitems,
+                               d_iq.p, d_off.p, d_ms.p, d_M.p);
+            HIPC(hipGetLastError());
+            int bs = (int)std::min<size_t>(256, ((nh.size() + 63) / 64) * 64);
+            hipLaunchKernelGGL(k_pp_conv, dim3(np), dim3(bs), 0, h->stream, d_M.p, d_off.p, d_ms.p, d_me.p, d_aT.p,
+                               N, nT, p0, d_nh.p, d_kidx.p, (int)nh.size(), npts, d_u.p);
+            HIPC(hipGetLastError());
+        }
+        g0 = g1;
+    }
+    tall.stop(h->stream);
+    HIPC(hipMemcpyAsync(u_out, d_u.p, sizeof(double) * 3 * npts * nt, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    h->tim[6] = tall.ms();
     return 0;
 }
 
@@ -756,8 +883,8 @@ int ebem_pp_eval(ebem_handle* h, const double* density, int npts, const double* 
                  double* u_out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
-    (void)density; (void)npts; (void)x; (void)nt; (void)t; (void)u_out;
-    return fail(EBEM_EINVAL, "post-processing not built yet");
+    HIPC(hipSetDevice(h->device));
+    return pp_eval(h, density, npts, x, nt, t, u_out);
 }
 
 int ebem_timings(ebem_handle* h, double* out8) {

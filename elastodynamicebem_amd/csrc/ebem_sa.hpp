@@ -24,6 +24,13 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// The reference evaluates every expression unfused; several closed forms here
+// are differences of nearly equal terms (thin child fans, 1/(3F) prefactors of
+// coeffG_TC.m:164-180), where a fused multiply-add changes the result by up to
+// 1e-8 relative.  Contraction is therefore off for the whole header (restored
+// at its end); libm-level calls keep their own internal FMAs.
+#pragma clang fp contract(off)
+
 namespace ebem {
 
 constexpr double kPi = 3.141592653589793238462643383279502884;
@@ -360,6 +367,10 @@ enum : int { kNone = 0, kSPSP = 1, kSPP = 2, kP = 3 };
 
 // calcoloDatiTriangolo.m:1-39 for the child (O, V2, V3), O at the origin.
 __host__ __device__ inline ChildGeo child_geo(double v2x, double v2y, double v3x, double v3y) {
+    // No contraction: cos(gamma) of a thin child (distant outer point) comes from
+    // the law of cosines with heavy cancellation; an FMA here changes the
+    // result by ~1e-8 relative where the unfused form matches the reference.
+#pragma clang fp contract(off)
     ChildGeo g;
     g.a = sqrt(v2x * v2x + v2y * v2y);
     g.b = sqrt(v3x * v3x + v3y * v3y);
@@ -603,3 +614,5 @@ __host__ __device__ inline void finish_rotation(const NodeAcc& acc, const double
 }
 
 }  // namespace ebem
+
+#pragma clang fp contract(fast)

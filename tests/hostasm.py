@@ -1,0 +1,76 @@
+"""Host replay of the GPU assembly (items W_m -> second differences -> blocks)
+using a host build of the device math; used to study numerics on the CPU."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_dp = C.POINTER(C.c_double)
+
+
+def load(fma=False):
+    lib = C.CDLL(os.path.join(HERE, "native", "libsa_host_fma.so" if fma else "libsa_host.so"))
+    lib.sa_item.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double, _dp]
+    return lib
+
+
+def windows(o, L):
+    c, mx, dt = o.center, o.maxL, o.dt
+    d = np.sqrt(((c[:, None, :] - c[None, :, :]) ** 2).sum(-1))
+    dmin = d - mx[None, :] - mx[:, None]
+    dmax = d + mx[None, :] + mx[:, None]
+    lo = np.full(d.shape, L, dtype=np.int64)
+    hi = np.full(d.shape, -1, dtype=np.int64)
+    for l in range(L):
+        ok = ((l - 1) * o.velS * dt < dmax) & ((l + 1) * o.velP * dt >= dmin)
+        lo = np.where(ok & (lo == L), l, lo)
+        hi = np.where(ok, l, hi)
+    return lo, hi
+
+
+def assemble(o, lib, L, rows=None, lags=None):
+    """Dict lag -> dense E (3N x 3N) restricted to `rows`."""
+    N = o.N
+    lo, hi = windows(o, L)
+    rows = range(N) if rows is None else rows
+    lags = range(L) if lags is None else lags
+    E = {l: np.zeros((3 * N, 3 * N)) for l in lags}
+    recs = [o.field_record(o.coords[o.tri[j]], o.normal[j]) for j in range(N)]
+    qq = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    inv = 1.0 / (4 * np.pi * o.rho)
+    for i in rows:
+        V = o.coords[o.tri[i]]
+        gh = np.zeros(128)
+        # same operation order as the library's host set-up (and the oracle)
+        nodes = (o.ghn[:, 0:1] * V[0][None, :] + o.ghn[:, 1:2] * V[1][None, :]) + o.ghn[:, 2:3] * V[2][None, :]
+        n = o.nGH
+        gh[0:n], gh[32:32 + n], gh[64:64 + n] = nodes[:, 0], nodes[:, 1], nodes[:, 2]
+        gh[96:96 + n] = o.area[i] * o.ghw
+        for j in range(N):
+            a, b = lo[i, j], hi[i, j]
+            if a > b:
+                continue
+            W = {}
+            for m in range(max(1, a - 1), b + 2):
+                w = np.zeros(6)
+                lib.sa_item(recs[j].ctypes.data_as(_dp), gh.ctypes.data_as(_dp), n, m * o.dt, o.velP, o.velS,
+                            w.ctypes.data_as(_dp))
+                W[m] = w
+            for l in lags:
+                if not (a <= l <= b):
+                    continue
+                e = 1.0 * W[l + 1]
+                if l >= 1:
+                    e = e + (-2.0) * W[l]
+                if l >= 2:
+                    e = e + W[l - 1]
+                e = e * inv
+                e[np.abs(e) < 1e-14] = 0
+                blk = np.zeros((3, 3))
+                for q, (r, c) in enumerate(qq):
+                    blk[r, c] = blk[c, r] = e[q]
+                E[l][3 * i:3 * i + 3, 3 * j:3 * j + 3] = blk
+    return E
