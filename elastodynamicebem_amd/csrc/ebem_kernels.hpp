@@ -167,6 +167,67 @@ __global__ void __launch_bounds__(256) k_pair_items(const int16_t* __restrict__ 
     }
 }
 
+// Front-state key of an item: where the P and the S front of level m sit
+// relative to the span [d - e, d + e] of the pair (d centroid distance, e sum of
+// longest edges), each in 5 classes (not arrived, three thirds of the span,
+// passed).  Items with equal keys take the same region branches in most
+// outer-node evaluations, so sorting by key keeps wavefronts convergent.
+constexpr int kNumKeys = 25;
+
+__device__ inline int front_class(double r, double lo, double span) {
+    double u = (r - lo) / span;
+    int q = (int)floor(u * 3.0);
+    return q < -1 ? -1 : (q > 3 ? 3 : q);
+}
+
+// Pass 1 of the counting sort: block b keys its contiguous item range
+// [b*per, (b+1)*per) and writes its histogram to bh[key*G + b].
+__global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ tri, AsmParams P, int r0,
+                                                   int64_t nitems, int64_t per, const int32_t* __restrict__ item_pair,
+                                                   const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
+                                                   uint8_t* __restrict__ key, unsigned int* __restrict__ bh) {
+    __shared__ unsigned int sh[kNumKeys];
+    for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) sh[k] = 0;
+    __syncthreads();
+    const int64_t a = blockIdx.x * per;
+    const int64_t e = a + per < nitems ? a + per : nitems;
+    for (int64_t it = a + threadIdx.x; it < e; it += blockDim.x) {
+        const int32_t p = item_pair[it];
+        const int i = r0 + p / P.N, j = p % P.N;
+        const int l0 = lo[p];
+        const int m = (l0 - 1 < 1 ? 1 : l0 - 1) + (int)(it - off[p]);
+        const FieldTri& S = tri[i];
+        const FieldTri& F = tri[j];
+        double v0 = F.center[0] - S.center[0], v1 = F.center[1] - S.center[1], v2 = F.center[2] - S.center[2];
+        double d = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+        double ex = F.maxL + S.maxL;
+        double t = (double)m * P.dt;
+        int qp = front_class(P.cP * t, d - ex, 2.0 * ex) + 1;
+        int qs = front_class(P.cS * t, d - ex, 2.0 * ex) + 1;
+        int k = qp * 5 + qs;
+        key[it] = (uint8_t)k;
+        atomicAdd(&sh[k], 1u);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) bh[(int64_t)k * gridDim.x + blockIdx.x] = sh[k];
+}
+
+// Pass 2: block b scatters its range starting from the exclusive offsets
+// bo[key*G + b] (ranks inside a block come from LDS atomics).
+__global__ void __launch_bounds__(256) k_item_scatter(int64_t nitems, int64_t per, const uint8_t* __restrict__ key,
+                                                      const unsigned long long* __restrict__ bo,
+                                                      int32_t* __restrict__ perm) {
+    __shared__ unsigned long long cur[kNumKeys];
+    for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) cur[k] = bo[(int64_t)k * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const int64_t a = blockIdx.x * per;
+    const int64_t e = a + per < nitems ? a + per : nitems;
+    for (int64_t it = a + threadIdx.x; it < e; it += blockDim.x) {
+        unsigned long long pos = atomicAdd(&cur[key[it]], 1ull);
+        perm[pos] = (int32_t)it;
+    }
+}
+
 // Persistent grid-stride kernel over (pair, level) items of a row chunk.
 // W[item*6 + q]: symmetric 3x3 in the global frame, NOT yet divided by 4 pi rho.
 template <int NGH>
@@ -174,9 +235,10 @@ __global__ void __launch_bounds__(256) k_levels(const FieldTri* __restrict__ tri
                                                 AsmParams P, int r0, int64_t nitems,
                                                 const int32_t* __restrict__ item_pair,
                                                 const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
-                                                double* __restrict__ W) {
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < nitems;
-         it += (int64_t)gridDim.x * blockDim.x) {
+                                                const int32_t* __restrict__ perm, double* __restrict__ W) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nitems;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t it = perm ? (int64_t)perm[k] : k;
         const int32_t p = item_pair[it];
         const int i = r0 + p / P.N, j = p % P.N;
         const int l0 = lo[p];

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -294,6 +295,7 @@ struct ebem_handle {
     DevBuf<rocblas_int> d_ipiv, d_info;
     bool have_beta = false, have_alpha = false;
     double tim[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool sort_items = true;
 
     Store store() {
         Store S;
@@ -339,13 +341,13 @@ int compute_L(ebem_handle* h) {
 
 template <int NGH>
 void launch_levels(ebem_handle* h, AsmParams P, int r0, int64_t nitems, const int32_t* item_pair, const int64_t* off,
-                   const int16_t* lo, double* W) {
+                   const int16_t* lo, const int32_t* perm, double* W) {
     int64_t blocks = (nitems + 255) / 256;
     int64_t cap = (int64_t)h->CUs * 16;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_levels<NGH>, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, r0,
-                       nitems, item_pair, off, lo, W);
+                       nitems, item_pair, off, lo, perm, W);
 }
 
 int assemble(ebem_handle* h, int precision) {
@@ -412,6 +414,17 @@ int assemble(ebem_handle* h, int precision) {
     int64_t chunk_cap = std::min<int64_t>(budget_items, std::max<int64_t>(h->items, 1));
     HIPC(d_W.alloc((size_t)chunk_cap * 6));
     HIPC(d_ip.alloc((size_t)chunk_cap));
+    DevBuf<int32_t> d_perm;
+    DevBuf<uint8_t> d_key;
+    DevBuf<unsigned int> d_hist;
+    DevBuf<unsigned long long> d_cur;
+    const int G = h->CUs * 8;  // sort blocks
+    HIPC(d_perm.alloc((size_t)chunk_cap));
+    HIPC(d_key.alloc((size_t)chunk_cap));
+    HIPC(d_hist.alloc((size_t)kNumKeys * G));
+    HIPC(d_cur.alloc((size_t)kNumKeys * G));
+    std::vector<unsigned int> hist((size_t)kNumKeys * G);
+    std::vector<unsigned long long> cur((size_t)kNumKeys * G);
     const double inv4pirho = 1.0 / (4 * kPi * h->rho);
     int r0 = 0;
     while (r0 < N) {
@@ -434,12 +447,31 @@ int assemble(ebem_handle* h, int precision) {
         hipLaunchKernelGGL(k_pair_items, dim3(rows), dim3(256), 0, h->stream, lo, hi, N, d_rowbase.p, d_off.p, d_ip.p);
         HIPC(hipGetLastError());
         if (acc > 0) {
+            // bucket items by front-state key (counting sort) for convergent wavefronts
+            const int64_t per = (acc + G - 1) / G;
+            hipLaunchKernelGGL(k_item_keys, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, P, r0, acc, per, d_ip.p,
+                               d_off.p, lo, d_key.p, d_hist.p);
+            HIPC(hipGetLastError());
+            HIPC(hipMemcpyAsync(hist.data(), d_hist.p, sizeof(unsigned int) * hist.size(), hipMemcpyDeviceToHost,
+                                h->stream));
+            HIPC(hipStreamSynchronize(h->stream));
+            unsigned long long runk = 0;
+            for (size_t k = 0; k < hist.size(); k++) {  // key-major, block-minor
+                cur[k] = runk;
+                runk += hist[k];
+            }
+            HIPC(hipMemcpyAsync(d_cur.p, cur.data(), sizeof(unsigned long long) * cur.size(), hipMemcpyHostToDevice,
+                                h->stream));
+            hipLaunchKernelGGL(k_item_scatter, dim3(G), dim3(256), 0, h->stream, acc, per, d_key.p, d_cur.p,
+                               d_perm.p);
+            HIPC(hipGetLastError());
+            const int32_t* perm = h->sort_items ? d_perm.p : nullptr;
             tlev.start(h->stream);
             switch (h->nGH) {
-                case 3: launch_levels<3>(h, P, r0, acc, d_ip.p, d_off.p, lo, d_W.p); break;
-                case 7: launch_levels<7>(h, P, r0, acc, d_ip.p, d_off.p, lo, d_W.p); break;
-                case 12: launch_levels<12>(h, P, r0, acc, d_ip.p, d_off.p, lo, d_W.p); break;
-                default: launch_levels<19>(h, P, r0, acc, d_ip.p, d_off.p, lo, d_W.p); break;
+                case 3: launch_levels<3>(h, P, r0, acc, d_ip.p, d_off.p, lo, perm, d_W.p); break;
+                case 7: launch_levels<7>(h, P, r0, acc, d_ip.p, d_off.p, lo, perm, d_W.p); break;
+                case 12: launch_levels<12>(h, P, r0, acc, d_ip.p, d_off.p, lo, perm, d_W.p); break;
+                default: launch_levels<19>(h, P, r0, acc, d_ip.p, d_off.p, lo, perm, d_W.p); break;
             }
             HIPC(hipGetLastError());
             tlev.stop(h->stream);
@@ -698,6 +730,10 @@ int ebem_create(ebem_handle** out, int device) {
     }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) h->CUs = cus;
+    // EBEM_ITEM_ORDER=pair keeps the assembly items in pair-major order instead
+    // of bucketing them by front-state key (same results, for A/B timing).
+    const char* order = std::getenv("EBEM_ITEM_ORDER");
+    h->sort_items = !(order && std::strcmp(order, "pair") == 0);
     *out = h;
     return 0;
 }

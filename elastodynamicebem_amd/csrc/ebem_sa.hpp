@@ -112,10 +112,9 @@ struct ChildAcc {
 // radius RS/rhoS) and BEMenerg_analiticIntg_SC_P.m (inner radius collapsed to
 // the plane distance: RS = zeta, rhoS = 0), folded with the level weights.
 __host__ __device__ inline void region_annulus(bool cc, double RS, double rhoS, double RP, double rhoP,
-                                      double zeta, double th1, double th2, const ChildGeo& g,
+                                      double zeta, double th1, double th2, const Phi& P,
                                       const Level& K, ChildAcc& A) {
     double T1t = th2 - th1;
-    Phi P = make_phi(th1, th2, g);
     if (cc && !(zeta > 1.0e-6)) {
         // CC.m:59-91, coplanar
         double I1 = RP - RS;
@@ -162,9 +161,8 @@ __host__ __device__ inline void region_annulus(bool cc, double RS, double rhoS, 
 
 // BEMenerg_analiticIntg_SC_SP.m: S disk sector, kernels 1/r and r_i r_j/r^3.
 __host__ __device__ inline void region_sdisk(double RS, double rhoS, double zeta, double th1, double th2,
-                                    const ChildGeo& g, const Level& K, ChildAcc& A) {
+                                    const Phi& P, const Level& K, ChildAcc& A) {
     double T1t = th2 - th1;
-    Phi P = make_phi(th1, th2, g);
     if (zeta > 1.0e-6) {
         double I1 = RS - zeta;
         double I2 = RS + zeta * zeta / RS - 2.0 * zeta;
@@ -220,9 +218,8 @@ __host__ __device__ inline EdgeG make_edge(double zeta, double th1, double th2, 
 // true; TR_P (P region between the plane foot and the edge, coeffG_TR_P.m)
 // otherwise.  zeta > 1e-6 branch of both.
 __host__ __device__ inline void region_edgeP(bool tc, double RS, double rhoS, double zeta, double th1, double th2,
-                                   const ChildGeo& g, const Level& K, ChildAcc& A) {
+                                   const ChildGeo& g, const Phi& P, const Level& K, ChildAcc& A) {
     double T1t = th2 - th1;
-    Phi P = make_phi(th1, th2, g);
     EdgeG E = make_edge(zeta, th1, th2, g);
     const double F = E.F;
     double ca_g_1 = cos(g.al - g.ga + th1), ca_g_2 = cos(g.al - g.ga + th2);
@@ -292,9 +289,8 @@ __host__ __device__ inline void region_edgeP(bool tc, double RS, double rhoS, do
 
 // Coplanar (zeta <= 1e-6) branch of coeffG_TC.m:138-181 (TC.m:31-49).
 __host__ __device__ inline void region_tc_coplanar(double RS, double th1, double th2, const ChildGeo& g,
-                                          const Level& K, ChildAcc& A) {
+                                          const Phi& P, const Level& K, ChildAcc& A) {
     double T1t = th2 - th1;
-    Phi P = make_phi(th1, th2, g);
     const double F = g.a * g.sb;
     double c1b = cos(th1 + g.be), c2b = cos(th2 + g.be);
     double s1b = sin(th1 + g.be), s2b = sin(th2 + g.be);
@@ -324,11 +320,10 @@ __host__ __device__ inline void region_tc_coplanar(double RS, double th1, double
 }
 
 // TR_SP (S disk part bounded by the edge, coeffG_TR_SP.m, TR_SP.m).
-__host__ __device__ inline void region_edgeS(double zeta, double th1, double th2, const ChildGeo& g, const Level& K,
-                                    ChildAcc& A) {
+__host__ __device__ inline void region_edgeS(double zeta, double th1, double th2, const ChildGeo& g, const Phi& P,
+                                    const Level& K, ChildAcc& A) {
     double T1t = th2 - th1;
     if (zeta > 1.0e-6) {
-        Phi P = make_phi(th1, th2, g);
         EdgeG E = make_edge(zeta, th1, th2, g);
         const double F = E.F;
         double G1 = zeta * (-T1t + E.g7) + F * E.g3;
@@ -426,27 +421,36 @@ __host__ __device__ inline void child_regions(const ChildGeo& g, double zeta, do
     }
     const bool sp = (type == kSPSP);
     const bool coplanar = !(zeta > 1.0e-6);
+    // One Phi per angular interval, shared by the S-sector and P routines on it
+    // (identical arguments, identical values).
     if (TH1 < TH2) {  // E1 (S sector) + E6 (P annulus)
-        if (sp) region_sdisk(RS, rhoS, zeta, TH1, TH2, g, K, A);
-        region_annulus(type != kP, RS, rhoS, RP, rhoP, zeta, TH1, TH2, g, K, A);
+        Phi P = make_phi(TH1, TH2, g);
+        if (sp) region_sdisk(RS, rhoS, zeta, TH1, TH2, P, K, A);
+        region_annulus(type != kP, RS, rhoS, RP, rhoP, zeta, TH1, TH2, P, K, A);
     }
-    if (TH2 < TH3) {  // E2 (S sector) + E7 (P region bounded by the edge)
-        if (sp) region_sdisk(RS, rhoS, zeta, TH2, TH3, g, K, A);
-        if (type == kP) region_edgeP(false, RS, rhoS, zeta, TH2, TH5, g, K, A);
-        else if (coplanar) region_tc_coplanar(RS, TH2, TH3, g, K, A);
-        else region_edgeP(true, RS, rhoS, zeta, TH2, TH3, g, K, A);
+    if (TH2 < TH3) {  // E2 (S sector) + E7 (P region bounded by the edge); TH5 == TH3 for class P
+        Phi P = make_phi(TH2, TH3, g);
+        if (sp) region_sdisk(RS, rhoS, zeta, TH2, TH3, P, K, A);
+        if (type == kP) region_edgeP(false, RS, rhoS, zeta, TH2, TH5, g, P, K, A);
+        else if (coplanar) region_tc_coplanar(RS, TH2, TH3, g, P, K, A);
+        else region_edgeP(true, RS, rhoS, zeta, TH2, TH3, g, P, K, A);
     }
     if (TH3 < TH4) {  // E3 (S region bounded by the edge)
-        if (sp) region_edgeS(zeta, TH3, TH4, g, K, A);
+        if (sp) {
+            Phi P = make_phi(TH3, TH4, g);
+            region_edgeS(zeta, TH3, TH4, g, P, K, A);
+        }
     }
     if (TH4 < TH5) {  // E4 (S sector) + E8 (P region between S circle and edge)
-        if (sp) region_sdisk(RS, rhoS, zeta, TH4, TH5, g, K, A);
-        if (coplanar) region_tc_coplanar(RS, TH4, TH5, g, K, A);
-        else region_edgeP(true, RS, rhoS, zeta, TH4, TH5, g, K, A);
+        Phi P = make_phi(TH4, TH5, g);
+        if (sp) region_sdisk(RS, rhoS, zeta, TH4, TH5, P, K, A);
+        if (coplanar) region_tc_coplanar(RS, TH4, TH5, g, P, K, A);
+        else region_edgeP(true, RS, rhoS, zeta, TH4, TH5, g, P, K, A);
     }
     if (TH5 < TH6) {  // E5 (S sector) + E9 (P annulus)
-        if (sp) region_sdisk(RS, rhoS, zeta, TH5, TH6, g, K, A);
-        region_annulus(type != kP, RS, rhoS, RP, rhoP, zeta, TH5, TH6, g, K, A);
+        Phi P = make_phi(TH5, TH6, g);
+        if (sp) region_sdisk(RS, rhoS, zeta, TH5, TH6, P, K, A);
+        region_annulus(type != kP, RS, rhoS, RP, rhoP, zeta, TH5, TH6, P, K, A);
     }
 }
 
@@ -551,19 +555,20 @@ __host__ __device__ inline int node_eval(const NodeGeo& G, const Level& K, doubl
     if (type == kNone) return kNone;
     const double zeta = G.zeta;
     const double sgn = G.sgn;
-    const int nb[3] = {1, 2, 0};
+#pragma unroll 1
     for (int k = 0; k < 3; k++) {
-        double C = G.c[k];
+        // child k = (O, V_k, V_{k+1 mod 3}); selects keep the fan in registers
+        const double C = (k == 0) ? G.c[0] : ((k == 1) ? G.c[1] : G.c[2]);
         if (fabs(C) < 1.0e-06) continue;
-        double v2x = G.t[k][0], v2y = G.t[k][1];
-        double v3x = G.t[nb[k]][0], v3y = G.t[nb[k]][1];
+        double v2x = (k == 0) ? G.t[0][0] : ((k == 1) ? G.t[1][0] : G.t[2][0]);
+        double v2y = (k == 0) ? G.t[0][1] : ((k == 1) ? G.t[1][1] : G.t[2][1]);
+        double v3x = (k == 0) ? G.t[1][0] : ((k == 1) ? G.t[2][0] : G.t[0][0]);
+        double v3y = (k == 0) ? G.t[1][1] : ((k == 1) ? G.t[2][1] : G.t[0][1]);
         if (C < 0.0) {  // fieldTriangle.m:163-167: counter-clockwise order
             double tx = v2x, ty = v2y;
             v2x = v3x; v2y = v3y;
             v3x = tx; v3y = ty;
         }
-        double th0 = atan2(v2y, v2x);
-        th0 = (th0 < 0.0) ? fabs(th0) : 2.0 * kPi - th0;
         ChildGeo g = child_geo(v2x, v2y, v3x, v3y);
         ChildAcc A;
         A.s = 0.0;
@@ -579,8 +584,9 @@ __host__ __device__ inline int node_eval(const NodeGeo& G, const Level& K, doubl
         double m02 = -zs * (A.b[1] + g.cg * A.b[2]) * isg;
         double m12 = -zs * A.b[2];
         double m22 = zs * zs * A.b[0];
-        // A(theta0) * M * A(theta0)' (fieldTriangle.m:191-204)
-        double ct = cos(th0), st = sin(th0);
+        // A(theta0) * M * A(theta0)' (fieldTriangle.m:171-204) with theta0 the
+        // clockwise angle of V2: cos(theta0) = V2x/|V2|, sin(theta0) = -V2y/|V2|.
+        double ct = v2x / g.a, st = -v2y / g.a;
         double r00 = ct * ct * m00 + 2.0 * ct * st * m01 + st * st * m11;
         double r01 = -ct * st * m00 + (ct * ct - st * st) * m01 + ct * st * m11;
         double r11 = st * st * m00 - 2.0 * ct * st * m01 + ct * ct * m11;
