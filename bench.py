@@ -7,7 +7,8 @@ Workload (the FP64 plain method at scale): unit icosphere of level 3
 
 One step = the whole hot path on device-resident inputs: assembly of
 E_0..E_65 (5.61e7 culled element-pair.lag integrals), right-hand side, E_0
-factorisation and the 128-step marching-on-in-time solve.
+factorisation, the 128-step marching-on-in-time solve and the post-processed
+displacement at 4096 points of the shell 1.2 <= |x| <= 2 (seeded) at every step.
 value = culled pair.lag integrals / step time.
 
 Also reported: a roofline object for the dominant kernel (the assembly
@@ -44,6 +45,15 @@ def workload():
     m = read_mesh(os.path.join(ROOT, "tests", "golden", "meshes", "sphereUniform_3.mesh"))
     pb = api.PbParam(rho=1.0, mu=1.0, lambda_=1.0, Tfin=4.0, nT=128, domainType="sphereUniform", lev=3)
     return m, pb
+
+
+def shell_points(n=4096, seed=0):
+    """Points uniform in volume in the shell 1.2 <= |x| <= 2.0 (numpy PCG64, seed 0)."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r = (1.2 ** 3 + rng.random(n) * (2.0 ** 3 - 1.2 ** 3)) ** (1 / 3)
+    return d * r[:, None]
 
 
 def history_bytes(m, pb, L):
@@ -89,6 +99,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy kernel of MIB MiB (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -97,23 +109,40 @@ def main():
     import torch  # noqa: F401  (HIP runtime present)
     from elastodynamicebem_amd import api
 
+    if args.calib_copy:
+        import torch
+        a = torch.ones(args.calib_copy * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
     m, pb = workload()
     s = api.Solver(m, pb, 7, device=0)
     L = s.num_blocks()
+
+    xpts = shell_points()
+    tpts = pb.Tfin / pb.nT * np.arange(1, pb.nT + 1)
+    upts = np.zeros((pb.nT, xpts.shape[0], 3))
+    from elastodynamicebem_amd import _lib
 
     def step():
         s.assemble(api.PREC_FP64)
         s.rhs(api.DATUM_SPHERE_T2EXP)
         s.lib.ebem_mot_solve(s.h, None, None)
+        _lib.check(s.lib.ebem_pp_eval(s.h, None, xpts.shape[0], _lib.dptr(xpts), tpts.size, _lib.dptr(tpts),
+                                      _lib.dptr(upts)))
 
-    for _ in range(args.warmup):
+    for w in range(args.warmup):
         step()
+        print(f"bench: warmup step {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
     times, tims = [], []
-    for _ in range(args.steps):
+    for k in range(args.steps):
         t0 = time.perf_counter()
         step()
         times.append(time.perf_counter() - t0)
         tims.append(s.timings())
+        print(f"bench: timed step {k + 1}/{args.steps}: {1e3 * times[-1]:.1f} ms", file=sys.stderr, flush=True)
     info = s.store_info()
     ms_step = 1e3 * float(np.mean(times))
     units = info["pair_lags"]
@@ -122,6 +151,7 @@ def main():
     mot_ms = float(np.mean([t["mot_ms"] for t in tims]))
     hist_ms = float(np.mean([t["history_ms"] for t in tims]))
     fac_ms = float(np.mean([t["factor_ms"] for t in tims]))
+    pp_ms = float(np.mean([t["pp_ms"] for t in tims]))
     hb, _ = history_bytes(m, pb, L)
 
     roof = None
@@ -139,10 +169,11 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "sphereUniform_3 icosphere (1280 triangles, 3840 DOF), T=4, nT=128, L=66, "
-                               "SA 7-point outer rule, plain FP64: assembly + RHS + E0 factorisation + MOT",
+                               "SA 7-point outer rule, plain FP64: assembly + RHS + E0 factorisation + MOT "
+                               "+ post-processing at 4096 shell points x 128 steps",
                    "pair_lags": units, "items": info["items"], "store_bytes": info["bytes"]},
         "breakdown_ms": {"assembly": asm_ms, "assembly_integral_kernel": lev_ms, "mot_total": mot_ms,
-                         "e0_factor": fac_ms, "history_sum": hist_ms},
+                         "e0_factor": fac_ms, "history_sum": hist_ms, "post_processing": pp_ms},
         "history_sum": {"bytes": hb, "achieved_GBs": hist_gbs, "peak_GBs": PEAK_HBM_GBS,
                         "frac": hist_gbs / PEAK_HBM_GBS},
         "roofline": roof,

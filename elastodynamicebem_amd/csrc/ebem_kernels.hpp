@@ -229,9 +229,13 @@ __global__ void __launch_bounds__(256) k_item_scatter(int64_t nitems, int64_t pe
 }
 
 // Persistent grid-stride kernel over (pair, level) items of a row chunk.
+// The whole node evaluation is inlined (an out-of-line call spills ~1 KB of
+// stack per lane); a 256-VGPR budget (2 waves per SIMD) is the measured optimum
+// on gfx950: 209 ms vs 427 ms at 128 VGPRs and 605 ms out of line (1280-triangle
+// sphere, 5.9e7 items).
 // W[item*6 + q]: symmetric 3x3 in the global frame, NOT yet divided by 4 pi rho.
 template <int NGH>
-__global__ void __launch_bounds__(256) k_levels(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_levels(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
                                                 AsmParams P, int r0, int64_t nitems,
                                                 const int32_t* __restrict__ item_pair,
                                                 const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
@@ -525,7 +529,7 @@ __global__ void __launch_bounds__(256) k_pp_items(const int32_t* __restrict__ ms
 
 // M_m(x_p, tri j) for every item; 6 symmetric values, /(4 pi rho), entries
 // below 1e-14 zeroed (ppA/BEMenerg_postProcA_calcMatrix.m:20).
-__global__ void __launch_bounds__(256) k_pp_M(const FieldTri* __restrict__ tri, const double* __restrict__ xs, int p0,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_pp_M(const FieldTri* __restrict__ tri, const double* __restrict__ xs, int p0,
                                               PPParams P, int64_t nitems, const int32_t* __restrict__ item_q,
                                               const int64_t* __restrict__ off, const int32_t* __restrict__ ms,
                                               double* __restrict__ M) {

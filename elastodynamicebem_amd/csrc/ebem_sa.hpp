@@ -87,7 +87,7 @@ __host__ __device__ inline double clamp1(double x) { return fmin(1.0, fmax(-1.0,
 __host__ __device__ inline double sqrt0(double x) { return x < 0.0 ? 0.0 : sqrt(x); }
 __host__ __device__ inline double msign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
 
-__host__ __device__ inline Phi make_phi(double th1, double th2, const ChildGeo& g) {
+__host__ __device__ __attribute__((always_inline)) inline Phi make_phi(double th1, double th2, const ChildGeo& g) {
     double T1t = th2 - th1, T2t = th2 + th1;
     double s1t = sin(T1t);
     double sh1 = sin(T1t * 0.5);
@@ -549,7 +549,7 @@ __host__ __device__ inline int classify(double zeta, double rP, double rS, doubl
 
 // One outer point at one level: adds w * (s, M_local) into acc.
 // Returns the integration class (0 when the point is skipped).
-__host__ __device__ inline int node_eval(const NodeGeo& G, const Level& K, double w, NodeAcc& acc) {
+__host__ __device__ __attribute__((always_inline)) inline int node_eval(const NodeGeo& G, const Level& K, double w, NodeAcc& acc) {
     double rhoP, rhoS;
     int type = classify(G.zeta, K.rP, K.rS, G.dMIN, rhoP, rhoS);
     if (type == kNone) return kNone;
