@@ -532,9 +532,10 @@ __global__ void __launch_bounds__(256) k_pp_items(const int32_t* __restrict__ ms
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_pp_M(const FieldTri* __restrict__ tri, const double* __restrict__ xs, int p0,
                                               PPParams P, int64_t nitems, const int32_t* __restrict__ item_q,
                                               const int64_t* __restrict__ off, const int32_t* __restrict__ ms,
-                                              double* __restrict__ M) {
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < nitems;
-         it += (int64_t)gridDim.x * blockDim.x) {
+                                              const int32_t* __restrict__ perm, double* __restrict__ M) {
+    for (int64_t kx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; kx < nitems;
+         kx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t it = perm ? (int64_t)perm[kx] : kx;
         const int32_t q = item_q[it];
         const int p = p0 + q / P.N, j = q % P.N;
         const int m = ms[q] + (int)(it - off[q]);
@@ -563,29 +564,35 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     }
 }
 
-// One workgroup per point; thread k handles requested time tk[k] (nhat[k]).
-// aT: density transposed to [triangle][step][3].
-__global__ void __launch_bounds__(256) k_pp_conv(const double* __restrict__ M, const int64_t* __restrict__ off,
-                                                 const int32_t* __restrict__ ms, const int32_t* __restrict__ me,
-                                                 const double* __restrict__ aT, int N, int nT, int p0,
-                                                 const int32_t* __restrict__ nhat, const int32_t* __restrict__ kidx,
-                                                 int nk, int npts, double* __restrict__ u) {
+// One workgroup per point: thread (slice s, time k) sums the triangles
+// j = s, s + S, s + 2S, ... for requested time k (nhat[k]); the S slice
+// partials are reduced through LDS.  aT: density as [triangle][step][3].
+constexpr int kConvSlices = 8;
+
+__global__ void __launch_bounds__(1024) k_pp_conv(const double* __restrict__ M, const int64_t* __restrict__ off,
+                                                  const int32_t* __restrict__ ms, const int32_t* __restrict__ me,
+                                                  const double* __restrict__ aT, int N, int nT, int p0,
+                                                  const int32_t* __restrict__ nhat, const int32_t* __restrict__ kidx,
+                                                  int nk, int npts, double* __restrict__ u) {
+    __shared__ double red[kConvSlices][128][3];
     const int r = blockIdx.x;
     const int p = p0 + r;
-    for (int kb = 0; kb < nk; kb += blockDim.x) {
-        const int kk = kb + threadIdx.x;
+    const int lane_k = threadIdx.x % 128;
+    const int slice = threadIdx.x / 128;  // blockDim = 128 * kConvSlices
+    for (int kb = 0; kb < nk; kb += 128) {
+        const int kk = kb + lane_k;
         const bool act = kk < nk;
         const int nh = act ? nhat[kk] : -1;
         double u0 = 0.0, u1 = 0.0, u2 = 0.0;
-        for (int j = 0; j < N; j++) {
+        for (int j = slice; j < N; j += kConvSlices) {
             const int64_t q = (int64_t)r * N + j;
             const int a = ms[q], b = me[q];
-            if (a > b) continue;
+            if (a > b || a > nh) continue;
             const int64_t o = off[q];
             const double* aj = aT + (int64_t)j * nT * 3;
             double pv[6] = {0, 0, 0, 0, 0, 0};
-            for (int m = a; m <= b; m++) {
-                if (m > nh) break;
+            const int mend = b < nh ? b : nh;
+            for (int m = a; m <= mend; m++) {
                 const double* Mm = M + (o + (m - a)) * 6;
                 double d[6];
 #pragma unroll
@@ -600,13 +607,71 @@ __global__ void __launch_bounds__(256) k_pp_conv(const double* __restrict__ M, c
                 u2 += d[2] * x0 + d[4] * x1 + d[5] * x2;
             }
         }
-        if (act) {
+        red[slice][lane_k][0] = u0;
+        red[slice][lane_k][1] = u1;
+        red[slice][lane_k][2] = u2;
+        __syncthreads();
+        if (slice == 0 && act) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            for (int q = 0; q < kConvSlices; q++) {
+                s0 += red[q][lane_k][0];
+                s1 += red[q][lane_k][1];
+                s2 += red[q][lane_k][2];
+            }
             int64_t k = kidx[kk];
-            u[3 * ((int64_t)p + (int64_t)npts * k) + 0] = u0;
-            u[3 * ((int64_t)p + (int64_t)npts * k) + 1] = u1;
-            u[3 * ((int64_t)p + (int64_t)npts * k) + 2] = u2;
+            u[3 * ((int64_t)p + (int64_t)npts * k) + 0] = s0;
+            u[3 * ((int64_t)p + (int64_t)npts * k) + 1] = s1;
+            u[3 * ((int64_t)p + (int64_t)npts * k) + 2] = s2;
         }
+        __syncthreads();
     }
+}
+
+// Per point of the chunk: total window size (number of M items).
+__global__ void __launch_bounds__(256) k_pp_counts(const int32_t* __restrict__ ms, const int32_t* __restrict__ me,
+                                                   int N, int64_t* __restrict__ cnt) {
+    __shared__ long long part[256];
+    const int r = blockIdx.x;
+    long long sum = 0;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) sum += max(0, me[(int64_t)r * N + j] - ms[(int64_t)r * N + j] + 1);
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int d = 128; d >= 1; d >>= 1) {
+        if (threadIdx.x < d) part[threadIdx.x] += part[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cnt[r] = part[0];
+}
+
+// Front-state keys of post-processing items (same 5 x 5 classes as the
+// assembly: P and S front against the span [d - maxL, d + maxL] of the
+// point-triangle distance), pass 1 of the counting sort.
+__global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tri, const double* __restrict__ xs,
+                                                 int p0, PPParams P, int64_t nitems, int64_t per,
+                                                 const int32_t* __restrict__ item_q, const int64_t* __restrict__ off,
+                                                 const int32_t* __restrict__ ms, uint8_t* __restrict__ key,
+                                                 unsigned int* __restrict__ bh) {
+    __shared__ unsigned int sh[kNumKeys];
+    for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) sh[k] = 0;
+    __syncthreads();
+    const int64_t a = blockIdx.x * per;
+    const int64_t e = a + per < nitems ? a + per : nitems;
+    for (int64_t it = a + threadIdx.x; it < e; it += blockDim.x) {
+        const int32_t q = item_q[it];
+        const int p = p0 + q / P.N, j = q % P.N;
+        const int m = ms[q] + (int)(it - off[q]);
+        const FieldTri& F = tri[j];
+        double d0 = xs[3 * p] - F.center[0], d1 = xs[3 * p + 1] - F.center[1], d2 = xs[3 * p + 2] - F.center[2];
+        double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        double tau = P.delta + (double)m * P.dt;
+        int qp = front_class(P.cP * tau, d - F.maxL, 2.0 * F.maxL) + 1;
+        int qs = front_class(P.cS * tau, d - F.maxL, 2.0 * F.maxL) + 1;
+        int k = qp * 5 + qs;
+        key[it] = (uint8_t)k;
+        atomicAdd(&sh[k], 1u);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) bh[(int64_t)k * gridDim.x + blockIdx.x] = sh[k];
 }
 
 // aT[(j*nT + n)*3 + c] = alpha[n*3N + 3j + c]

@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <climits>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -626,13 +628,23 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     for (int k = 0; k < nt; k++)
         if (nhat[k] >= 0) order.push_back(k);
     std::sort(order.begin(), order.end(), [&](int a, int b) { return delta[a] < delta[b]; });
-    const int P0 = std::min(npts, 256);  // points per chunk
-    DevBuf<int32_t> d_ms, d_me, d_iq, d_nh, d_kidx;
-    DevBuf<int64_t> d_off, d_rb;
+    const int P0 = std::min(npts, 1024);  // points per chunk
+    const int G = h->CUs * 8;             // counting-sort blocks
+    DevBuf<int32_t> d_ms, d_me, d_iq, d_nh, d_kidx, d_perm;
+    DevBuf<int64_t> d_off, d_rb, d_cnt;
+    DevBuf<uint8_t> d_key;
+    DevBuf<unsigned int> d_bh;
+    DevBuf<unsigned long long> d_bo;
     HIPC(d_ms.alloc((size_t)P0 * N));
     HIPC(d_me.alloc((size_t)P0 * N));
     HIPC(d_off.alloc((size_t)P0 * N));
     HIPC(d_rb.alloc(P0));
+    HIPC(d_cnt.alloc(P0));
+    HIPC(d_bh.alloc((size_t)kNumKeys * G));
+    HIPC(d_bo.alloc((size_t)kNumKeys * G));
+    std::vector<unsigned int> bh((size_t)kNumKeys * G);
+    std::vector<unsigned long long> bo((size_t)kNumKeys * G);
+    std::vector<int64_t> cnt(P0), rb(P0);
     size_t g0 = 0;
     while (g0 < order.size()) {
         size_t g1 = g0 + 1;
@@ -669,32 +681,50 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             hipLaunchKernelGGL(k_pp_windows, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0,
                                np, P, d_ms.p, d_me.p);
             HIPC(hipGetLastError());
-            // per-point item counts on the host (np*N ints)
-            std::vector<int32_t> ms((size_t)np * N), me((size_t)np * N);
-            HIPC(hipMemcpyAsync(ms.data(), d_ms.p, sizeof(int32_t) * ms.size(), hipMemcpyDeviceToHost, h->stream));
-            HIPC(hipMemcpyAsync(me.data(), d_me.p, sizeof(int32_t) * me.size(), hipMemcpyDeviceToHost, h->stream));
+            hipLaunchKernelGGL(k_pp_counts, dim3(np), dim3(256), 0, h->stream, d_ms.p, d_me.p, N, d_cnt.p);
+            HIPC(hipGetLastError());
+            HIPC(hipMemcpyAsync(cnt.data(), d_cnt.p, sizeof(int64_t) * np, hipMemcpyDeviceToHost, h->stream));
             HIPC(hipStreamSynchronize(h->stream));
-            std::vector<int64_t> rb(np);
             int64_t run = 0;
             for (int r = 0; r < np; r++) {
                 rb[r] = run;
-                for (int j = 0; j < N; j++) run += std::max(0, me[(size_t)r * N + j] - ms[(size_t)r * N + j] + 1);
+                run += cnt[r];
             }
             const int64_t nitems = run;
-            HIPC(hipMemcpyAsync(d_rb.p, rb.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice, h->stream));
             if (nitems == 0) continue;
+            if (nitems > INT32_MAX) return fail(EBEM_EINVAL, "too many post-processing items in one chunk");
+            HIPC(hipMemcpyAsync(d_rb.p, rb.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice, h->stream));
             HIPC(d_iq.alloc((size_t)nitems));
             HIPC(d_M.alloc((size_t)nitems * 6));
+            HIPC(d_key.alloc((size_t)nitems));
+            HIPC(d_perm.alloc((size_t)nitems));
             hipLaunchKernelGGL(k_pp_items, dim3(np), dim3(256), 0, h->stream, d_ms.p, d_me.p, N, d_rb.p, d_off.p,
                                d_iq.p);
             HIPC(hipGetLastError());
+            // bucket items by front-state key (two-pass counting sort)
+            const int64_t per = (nitems + G - 1) / G;
+            hipLaunchKernelGGL(k_pp_keys, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P, nitems, per,
+                               d_iq.p, d_off.p, d_ms.p, d_key.p, d_bh.p);
+            HIPC(hipGetLastError());
+            HIPC(hipMemcpyAsync(bh.data(), d_bh.p, sizeof(unsigned int) * bh.size(), hipMemcpyDeviceToHost,
+                                h->stream));
+            HIPC(hipStreamSynchronize(h->stream));
+            unsigned long long runk = 0;
+            for (size_t k = 0; k < bh.size(); k++) {
+                bo[k] = runk;
+                runk += bh[k];
+            }
+            HIPC(hipMemcpyAsync(d_bo.p, bo.data(), sizeof(unsigned long long) * bo.size(), hipMemcpyHostToDevice,
+                                h->stream));
+            hipLaunchKernelGGL(k_item_scatter, dim3(G), dim3(256), 0, h->stream, nitems, per, d_key.p, d_bo.p,
+                               d_perm.p);
+            HIPC(hipGetLastError());
             int64_t mb = std::min<int64_t>((nitems + 255) / 256, (int64_t)h->CUs * 16);
             hipLaunchKernelGGL(k_pp_M, dim3((unsigned)mb), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P, nitems,
-                               d_iq.p, d_off.p, d_ms.p, d_M.p);
+                               d_iq.p, d_off.p, d_ms.p, h->sort_items ? d_perm.p : nullptr, d_M.p);
             HIPC(hipGetLastError());
-            int bs = (int)std::min<size_t>(256, ((nh.size() + 63) / 64) * 64);
-            hipLaunchKernelGGL(k_pp_conv, dim3(np), dim3(bs), 0, h->stream, d_M.p, d_off.p, d_ms.p, d_me.p, d_aT.p,
-                               N, nT, p0, d_nh.p, d_kidx.p, (int)nh.size(), npts, d_u.p);
+            hipLaunchKernelGGL(k_pp_conv, dim3(np), dim3(128 * kConvSlices), 0, h->stream, d_M.p, d_off.p, d_ms.p,
+                               d_me.p, d_aT.p, N, nT, p0, d_nh.p, d_kidx.p, (int)nh.size(), npts, d_u.p);
             HIPC(hipGetLastError());
         }
         g0 = g1;
