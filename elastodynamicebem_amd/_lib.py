@@ -29,6 +29,7 @@ SIGNATURES = {
     "ebem_num_blocks": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "ebem_assemble": (C.c_int, [_vp, C.c_int]),
     "ebem_store_info": (C.c_int, [_vp, _i64p, _i64p, _i64p, _i64p]),
+    "ebem_assembly_stats": (C.c_int, [_vp, _i64p, _i64p]),
     "ebem_get_block": (C.c_int, [_vp, C.c_int, _dp]),
     "ebem_rhs": (C.c_int, [_vp, C.c_int, _dp]),
     "ebem_mot_solve": (C.c_int, [_vp, _dp, _dp]),

@@ -165,6 +165,11 @@ class Solver:
         _lib.check(self.lib.ebem_store_info(self.h, *[C.byref(x) for x in v]))
         return dict(nnzb=v[0].value, pair_lags=v[1].value, items=v[2].value, bytes=v[3].value)
 
+    def assembly_stats(self) -> dict:
+        a, b = C.c_int64(), C.c_int64()
+        _lib.check(self.lib.ebem_assembly_stats(self.h, C.byref(a), C.byref(b)))
+        return dict(far_items=a.value, fp64_items=b.value)
+
     def get_block(self, lag: int) -> np.ndarray:
         """Dense E_lag (3N x 3N), row index = test DOF."""
         n3 = 3 * self.N

@@ -41,6 +41,7 @@ struct AsmParams {
     int N, L;
     double dt, cP, cS, rho;
     int nGH;
+    double far2;  // squared centroid distance above which a pair is far field (mixed precision); <0: none
 };
 
 // (lag-1)*cS*dt < dmax && (lag+1)*cP*dt >= dmin  (calcMatrixBlock.m:61), same
@@ -172,7 +173,8 @@ __global__ void __launch_bounds__(256) k_pair_items(const int16_t* __restrict__ 
 // longest edges), each in 5 classes (not arrived, three thirds of the span,
 // passed).  Items with equal keys take the same region branches in most
 // outer-node evaluations, so sorting by key keeps wavefronts convergent.
-constexpr int kNumKeys = 25;
+constexpr int kFrontKeys = 25;
+constexpr int kNumKeys = 2 * kFrontKeys;  // far-field pairs (mixed precision) take keys 0..24
 
 __device__ inline int front_class(double r, double lo, double span) {
     double u = (r - lo) / span;
@@ -199,12 +201,13 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
         const FieldTri& S = tri[i];
         const FieldTri& F = tri[j];
         double v0 = F.center[0] - S.center[0], v1 = F.center[1] - S.center[1], v2 = F.center[2] - S.center[2];
-        double d = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+        double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        double d = sqrt(d2);
         double ex = F.maxL + S.maxL;
         double t = (double)m * P.dt;
         int qp = front_class(P.cP * t, d - ex, 2.0 * ex) + 1;
         int qs = front_class(P.cS * t, d - ex, 2.0 * ex) + 1;
-        int k = qp * 5 + qs;
+        int k = qp * 5 + qs + ((P.far2 >= 0.0 && d2 > P.far2) ? 0 : kFrontKeys);
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
     }
@@ -234,7 +237,7 @@ __global__ void __launch_bounds__(256) k_item_scatter(int64_t nitems, int64_t pe
 // on gfx950: 209 ms vs 427 ms at 128 VGPRs and 605 ms out of line (1280-triangle
 // sphere, 5.9e7 items).
 // W[item*6 + q]: symmetric 3x3 in the global frame, NOT yet divided by 4 pi rho.
-template <int NGH>
+template <int NGH, typename R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_levels(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
                                                 AsmParams P, int r0, int64_t nitems,
                                                 const int32_t* __restrict__ item_pair,
@@ -258,7 +261,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         for (int q = 0; q < NGH; q++) {
             double x[3] = {g[q], g[32 + q], g[64 + q]};
             NodeGeo G = node_geo(x, F);
-            node_eval(G, K, g[96 + q], acc);
+            node_eval<R>(G, K, g[96 + q], acc);
         }
         double Wv[6];
         finish_rotation(acc, F.C, Wv);
@@ -666,7 +669,7 @@ __global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tr
         double tau = P.delta + (double)m * P.dt;
         int qp = front_class(P.cP * tau, d - F.maxL, 2.0 * F.maxL) + 1;
         int qs = front_class(P.cS * tau, d - F.maxL, 2.0 * F.maxL) + 1;
-        int k = qp * 5 + qs;
+        int k = qp * 5 + qs + kFrontKeys;
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
     }

@@ -298,6 +298,8 @@ struct ebem_handle {
     bool have_beta = false, have_alpha = false;
     double tim[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool sort_items = true;
+    int64_t far_items = 0;
+    int precision = 0;
 
     Store store() {
         Store S;
@@ -341,20 +343,35 @@ int compute_L(ebem_handle* h) {
     return 0;
 }
 
-template <int NGH>
+template <int NGH, typename R>
 void launch_levels(ebem_handle* h, AsmParams P, int r0, int64_t nitems, const int32_t* item_pair, const int64_t* off,
                    const int16_t* lo, const int32_t* perm, double* W) {
+    if (nitems <= 0) return;
     int64_t blocks = (nitems + 255) / 256;
     int64_t cap = (int64_t)h->CUs * 16;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_levels<NGH>, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, r0,
-                       nitems, item_pair, off, lo, perm, W);
+    hipLaunchKernelGGL((k_levels<NGH, R>), dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P,
+                       r0, nitems, item_pair, off, lo, perm, W);
+}
+
+// Far-field items (keys below kFrontKeys, a prefix of the sorted permutation)
+// in R_far, the rest in FP64.
+template <int NGH>
+void launch_levels_split(ebem_handle* h, AsmParams P, int r0, int64_t nitems, int64_t nfar, const int32_t* item_pair,
+                         const int64_t* off, const int16_t* lo, const int32_t* perm, double* W) {
+    if (perm && nfar > 0) {
+        launch_levels<NGH, float>(h, P, r0, nfar, item_pair, off, lo, perm, W);
+        launch_levels<NGH, double>(h, P, r0, nitems - nfar, item_pair, off, lo, perm + nfar, W);
+    } else {
+        launch_levels<NGH, double>(h, P, r0, nitems, item_pair, off, lo, perm, W);
+    }
 }
 
 int assemble(ebem_handle* h, int precision) {
     if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set before ebem_assemble");
-    if (precision != EBEM_PREC_FP64) return fail(EBEM_EINVAL, "precision %d not available in this build", precision);
+    if (precision != EBEM_PREC_FP64 && precision != EBEM_PREC_MIXED)
+        return fail(EBEM_EINVAL, "unknown precision %d", precision);
     compute_L(h);
     const int N = h->N, L = h->L;
     if (L > 4096) return fail(EBEM_EINVAL, "too many lag blocks (%d > 4096)", L);
@@ -366,10 +383,18 @@ int assemble(ebem_handle* h, int precision) {
     P.cS = h->cS;
     P.rho = h->rho;
     P.nGH = h->nGH;
+    {
+        // far field: centroid distance > 3h, h = longest edge of the mesh
+        double hmax = 0;
+        for (int t = 0; t < N; t++) hmax = std::max(hmax, h->tri[t].maxL);
+        P.far2 = (precision == EBEM_PREC_MIXED) ? (3.0 * hmax) * (3.0 * hmax) : -1.0;
+    }
     Timer tall, tlev;
     double lev_ms = 0;
     tall.start(h->stream);
     const int64_t np = (int64_t)N * N;
+    h->far_items = 0;
+    h->precision = precision;
     DevBuf<int16_t> d_lo, d_hi;
     HIPC(d_lo.alloc(np));
     HIPC(d_hi.alloc(np));
@@ -406,7 +431,7 @@ int assemble(ebem_handle* h, int precision) {
     // row chunks bounded by the W scratch budget
     size_t freeb = 0, totb = 0;
     HIPC(hipMemGetInfo(&freeb, &totb));
-    int64_t budget_items = (int64_t)std::min<double>(256.0e6, 0.35 * (double)freeb / (6 * 8 + 4 + 8.0 / 1));
+    int64_t budget_items = (int64_t)std::min<double>(256.0e6, 0.35 * (double)freeb / (6 * 8 + 4 + 4 + 1));
     if (budget_items < 1) return fail(EBEM_ENOMEM, "not enough device memory for assembly scratch");
     int64_t maxrow_items = *std::max_element(rowitems.begin(), rowitems.end());
     if (maxrow_items > budget_items) budget_items = maxrow_items;
@@ -467,13 +492,16 @@ int assemble(ebem_handle* h, int precision) {
             hipLaunchKernelGGL(k_item_scatter, dim3(G), dim3(256), 0, h->stream, acc, per, d_key.p, d_cur.p,
                                d_perm.p);
             HIPC(hipGetLastError());
-            const int32_t* perm = h->sort_items ? d_perm.p : nullptr;
+            const int32_t* perm = (h->sort_items || precision == EBEM_PREC_MIXED) ? d_perm.p : nullptr;
+            int64_t nfar = 0;
+            for (size_t k = 0; k < (size_t)kFrontKeys * G; k++) nfar += hist[k];
+            h->far_items += nfar;
             tlev.start(h->stream);
             switch (h->nGH) {
-                case 3: launch_levels<3>(h, P, r0, acc, d_ip.p, d_off.p, lo, perm, d_W.p); break;
-                case 7: launch_levels<7>(h, P, r0, acc, d_ip.p, d_off.p, lo, perm, d_W.p); break;
-                case 12: launch_levels<12>(h, P, r0, acc, d_ip.p, d_off.p, lo, perm, d_W.p); break;
-                default: launch_levels<19>(h, P, r0, acc, d_ip.p, d_off.p, lo, perm, d_W.p); break;
+                case 3: launch_levels_split<3>(h, P, r0, acc, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
+                case 7: launch_levels_split<7>(h, P, r0, acc, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
+                case 12: launch_levels_split<12>(h, P, r0, acc, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
+                default: launch_levels_split<19>(h, P, r0, acc, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
             }
             HIPC(hipGetLastError());
             tlev.stop(h->stream);
@@ -872,6 +900,15 @@ int ebem_store_info(ebem_handle* h, int64_t* nnzb, int64_t* pair_lags, int64_t* 
     if (pair_lags) *pair_lags = h->pair_lags;
     if (items) *items = h->items;
     if (bytes) *bytes = h->nnzb * (6 * 8 + 4) + (int64_t)h->seg.size() * 8;
+    return 0;
+}
+
+int ebem_assembly_stats(ebem_handle* h, int64_t* far_items, int64_t* fp64_items) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
+    if (far_items) *far_items = h->far_items;
+    if (fp64_items) *fp64_items = h->items - h->far_items;
     return 0;
 }
 

@@ -50,10 +50,12 @@ static_assert(sizeof(FieldTri) == 256, "FieldTri must stay 256 B");
 
 // Level (time) constants.  s = k1*r1 + k3*r3 + k5*r5 and
 // M = k2*ris2 + k4*ris4 + k6*ris6 reproduce calcMatrixSubBlock.m:334-337.
-struct Level {
-    double t, rP, rS;
-    double k1, k3, k5, k2, k4, k6;
+template <typename R>
+struct LevelT {
+    R t, rP, rS;
+    R k1, k3, k5, k2, k4, k6;
 };
+using Level = LevelT<double>;
 
 __host__ __device__ inline Level make_level(double t, double cP, double cS) {
     Level K;
@@ -72,31 +74,44 @@ __host__ __device__ inline Level make_level(double t, double cP, double cS) {
 
 // Geometry of one child triangle (calcoloDatiTriangolo.m), angles at the
 // origin vertex (gamma), second vertex (beta), third vertex (alpha).
-struct ChildGeo {
-    double a, b, c;
-    double ca, cb, cg, sa, sb, sg;
-    double al, be, ga;
+template <typename R>
+struct ChildGeoT {
+    R a, b, c;
+    R ca, cb, cg, sa, sb, sg;
+    R al, be, ga;
 };
+using ChildGeo = ChildGeoT<double>;
 
-// Angular integrals Phi(1..5) of the region routines (e.g. CC.m:34-38).
-struct Phi {
-    double p1, p2, p3, p4, p5;
-};
+template <typename R>
+__host__ __device__ inline ChildGeoT<R> cast_geo(const ChildGeo& g) {
+    return ChildGeoT<R>{R(g.a), R(g.b), R(g.c), R(g.ca), R(g.cb), R(g.cg), R(g.sa), R(g.sb), R(g.sg),
+                        R(g.al), R(g.be), R(g.ga)};
+}
 
 __host__ __device__ inline double clamp1(double x) { return fmin(1.0, fmax(-1.0, x)); }
 __host__ __device__ inline double sqrt0(double x) { return x < 0.0 ? 0.0 : sqrt(x); }
 __host__ __device__ inline double msign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+template <typename R>
+__host__ __device__ inline R clampR(R x) { return fmin(R(1), fmax(R(-1), x)); }
 
-__host__ __device__ __attribute__((always_inline)) inline Phi make_phi(double th1, double th2, const ChildGeo& g) {
-    double T1t = th2 - th1, T2t = th2 + th1;
-    double s1t = sin(T1t);
-    double sh1 = sin(T1t * 0.5);
-    Phi P;
-    P.p1 = 2.0 * sin(T2t * 0.5) * sh1;
-    P.p2 = 2.0 * sin(g.ga - T2t * 0.5) * sh1;
-    P.p3 = 0.5 * (T1t - cos(T2t) * s1t);
-    P.p4 = -0.5 * (-(s1t * cos(g.ga - T2t)) + T1t * g.cg);
-    P.p5 = 0.5 * (T1t - cos(2.0 * g.ga - T2t) * s1t);
+// Angular integrals Phi(1..5) of the region routines (e.g. CC.m:34-38).
+template <typename R>
+struct Phi {
+    R p1, p2, p3, p4, p5;
+};
+
+
+template <typename R>
+__host__ __device__ __attribute__((always_inline)) inline Phi<R> make_phi(R th1, R th2, const ChildGeoT<R>& g) {
+    R T1t = th2 - th1, T2t = th2 + th1;
+    R s1t = sin(T1t);
+    R sh1 = sin(T1t * R(0.5));
+    Phi<R> P;
+    P.p1 = R(2.0) * sin(T2t * R(0.5)) * sh1;
+    P.p2 = R(2.0) * sin(g.ga - T2t * R(0.5)) * sh1;
+    P.p3 = R(0.5) * (T1t - cos(T2t) * s1t);
+    P.p4 = -R(0.5) * (-(s1t * cos(g.ga - T2t)) + T1t * g.cg);
+    P.p5 = R(0.5) * (T1t - cos(R(2.0) * g.ga - T2t) * s1t);
     return P;
 }
 
@@ -111,63 +126,65 @@ struct ChildAcc {
 // BEMenerg_analiticIntg_CC.m (P annulus between the S and P circles, inner
 // radius RS/rhoS) and BEMenerg_analiticIntg_SC_P.m (inner radius collapsed to
 // the plane distance: RS = zeta, rhoS = 0), folded with the level weights.
-__host__ __device__ inline void region_annulus(bool cc, double RS, double rhoS, double RP, double rhoP,
-                                      double zeta, double th1, double th2, const Phi& P,
-                                      const Level& K, ChildAcc& A) {
-    double T1t = th2 - th1;
-    if (cc && !(zeta > 1.0e-6)) {
+template <typename R>
+__host__ __device__ inline void region_annulus(bool cc, bool zpos, R RS, R rhoS, R RP, R rhoP,
+                                      R zeta, R th1, R th2, const Phi<R>& P,
+                                      const LevelT<R>& K, ChildAcc& A) {
+    R T1t = th2 - th1;
+    if (cc && !zpos) {
         // CC.m:59-91, coplanar
-        double I1 = RP - RS;
-        double I4 = -(1.0 / RP) + (1.0 / RS);
+        R I1 = RP - RS;
+        R I4 = -(R(1.0) / RP) + (R(1.0) / RS);
         A.s += K.k1 * (T1t * I1) + K.k3 * (T1t * I4);
-        double w = K.k2 * I1 + K.k4 * I4;
+        R w = K.k2 * I1 + K.k4 * I4;
         A.b[3] += w * P.p5;
         A.b[4] += w * P.p4;
         A.b[5] += w * P.p3;
         return;
     }
-    double I1, I2, I3, I4, I5, I6, I7;
-    double iRP = 1.0 / RP, iRP3 = iRP * iRP * iRP, z2 = zeta * zeta;
-    double qP = rhoP * iRP;
+    R I1, I2, I3, I4, I5, I6, I7;
+    R iRP = R(1.0) / RP, iRP3 = iRP * iRP * iRP, z2 = zeta * zeta;
+    R qP = rhoP * iRP;
     if (cc) {  // CC.m:25-31
-        double iRS = 1.0 / RS, iRS3 = iRS * iRS * iRS;
-        double qS = rhoS * iRS;
+        R iRS = R(1.0) / RS, iRS3 = iRS * iRS * iRS;
+        R qS = rhoS * iRS;
         I1 = RP - RS;
         I2 = RP - RS + z2 * (iRP - iRS);
         I3 = log((rhoP + RP) / (rhoS + RS)) - qP + qS;
         I4 = -iRP + iRS;
-        I5 = -iRP + iRS + (z2 / 3.0) * (iRP3 - iRS3);
-        I6 = (1.0 / (3.0 * z2)) * (qP * qP * qP - qS * qS * qS);
-        I7 = -1.0 / 3.0 * (iRP3 - iRS3);
+        I5 = -iRP + iRS + (z2 / R(3.0)) * (iRP3 - iRS3);
+        I6 = (R(1.0) / (R(3.0) * z2)) * (qP * qP * qP - qS * qS * qS);
+        I7 = -R(1.0) / R(3.0) * (iRP3 - iRS3);
     } else {  // SC_P.m:114-120
         I1 = RP - zeta;
-        I2 = RP + z2 * iRP - 2.0 * zeta;
+        I2 = RP + z2 * iRP - R(2.0) * zeta;
         I3 = log((rhoP + RP) / zeta) - qP;
-        I4 = -iRP + 1.0 / zeta;
-        I5 = -iRP + z2 / (3.0 * RP * RP * RP) + 2.0 / (3.0 * zeta);
-        I6 = (1.0 / (3.0 * z2)) * (qP * qP * qP);
-        I7 = -(1.0 / (3.0 * RP * RP * RP)) + 1.0 / (3.0 * zeta * z2);
+        I4 = -iRP + R(1.0) / zeta;
+        I5 = -iRP + z2 / (R(3.0) * RP * RP * RP) + R(2.0) / (R(3.0) * zeta);
+        I6 = (R(1.0) / (R(3.0) * z2)) * (qP * qP * qP);
+        I7 = -(R(1.0) / (R(3.0) * RP * RP * RP)) + R(1.0) / (R(3.0) * zeta * z2);
     }
     A.s += K.k1 * (T1t * I1) + K.k3 * (T1t * I4);
     A.b[0] += K.k2 * (T1t * I4) + K.k4 * (T1t * I7);
-    double w3 = K.k2 * I3 + K.k4 * I6;
+    R w3 = K.k2 * I3 + K.k4 * I6;
     A.b[1] += w3 * P.p2;
     A.b[2] += w3 * P.p1;
-    double w2 = K.k2 * I2 + K.k4 * I5;
+    R w2 = K.k2 * I2 + K.k4 * I5;
     A.b[3] += w2 * P.p5;
     A.b[4] += w2 * P.p4;
     A.b[5] += w2 * P.p3;
 }
 
 // BEMenerg_analiticIntg_SC_SP.m: S disk sector, kernels 1/r and r_i r_j/r^3.
-__host__ __device__ inline void region_sdisk(double RS, double rhoS, double zeta, double th1, double th2,
-                                    const Phi& P, const Level& K, ChildAcc& A) {
-    double T1t = th2 - th1;
-    if (zeta > 1.0e-6) {
-        double I1 = RS - zeta;
-        double I2 = RS + zeta * zeta / RS - 2.0 * zeta;
-        double I3 = log((rhoS + RS) / zeta) - rhoS / RS;
-        double I4 = -(1.0 / RS) + 1.0 / zeta;
+template <typename R>
+__host__ __device__ inline void region_sdisk(bool zpos, R RS, R rhoS, R zeta, R th1, R th2,
+                                    const Phi<R>& P, const LevelT<R>& K, ChildAcc& A) {
+    R T1t = th2 - th1;
+    if (zpos) {
+        R I1 = RS - zeta;
+        R I2 = RS + zeta * zeta / RS - R(2.0) * zeta;
+        R I3 = log((rhoS + RS) / zeta) - rhoS / RS;
+        R I4 = -(R(1.0) / RS) + R(1.0) / zeta;
         A.s += K.k5 * (T1t * I1);
         A.b[0] += K.k6 * (T1t * I4);
         A.b[1] += K.k6 * I3 * P.p2;
@@ -187,29 +204,31 @@ __host__ __device__ inline void region_sdisk(double RS, double rhoS, double zeta
 
 // Edge-bounded terms shared by coeffG_TC / coeffG_TR_P / coeffG_TR_SP for
 // zeta > 1e-6: g1..g8 and the angle functions of TH1, TH2.
+template <typename R>
 struct EdgeG {
-    double F, g1, g2, g3, g4, g5, g7, g8, ADD;
-    double c1b, c2b, s1b, s2b;
+    R F, g1, g2, g3, g4, g5, g7, g8, ADD;
+    R c1b, c2b, s1b, s2b;
 };
 
-__host__ __device__ inline EdgeG make_edge(double zeta, double th1, double th2, const ChildGeo& g) {
-    EdgeG E;
+template <typename R>
+__host__ __device__ inline EdgeG<R> make_edge(R zeta, R th1, R th2, const ChildGeoT<R>& g) {
+    EdgeG<R> E;
     E.F = g.a * g.sb;
     E.s1b = sin(th1 + g.be);
     E.s2b = sin(th2 + g.be);
     E.c1b = cos(th1 + g.be);
     E.c2b = cos(th2 + g.be);
-    double R1 = E.F / E.s1b, R2 = E.F / E.s2b;
-    double fz1 = zeta / R1, fz2 = zeta / R2;
+    R R1 = E.F / E.s1b, R2 = E.F / E.s2b;
+    R fz1 = zeta / R1, fz2 = zeta / R2;
     E.ADD = zeta * zeta + E.F * E.F;
-    E.g1 = sqrt(1.0 + fz1 * fz1);
-    E.g2 = sqrt(1.0 + fz2 * fz2);
-    E.g3 = 0.5 * (log((E.g1 + E.c1b) * (E.g2 - E.c2b)) - log((E.g1 - E.c1b) * (E.g2 + E.c2b)));
-    double i1 = 1.0 / fz1, i2 = 1.0 / fz2;
-    E.g4 = log(i1 + sqrt(1.0 + i1 * i1));
-    E.g5 = log(i2 + sqrt(1.0 + i2 * i2));
-    double sq = sqrt(E.ADD);
-    E.g7 = kPi - acos(clamp1((-zeta * E.c2b) / sq)) - acos(clamp1((zeta * E.c1b) / sq));
+    E.g1 = sqrt(R(1.0) + fz1 * fz1);
+    E.g2 = sqrt(R(1.0) + fz2 * fz2);
+    E.g3 = R(0.5) * (log((E.g1 + E.c1b) * (E.g2 - E.c2b)) - log((E.g1 - E.c1b) * (E.g2 + E.c2b)));
+    R i1 = R(1.0) / fz1, i2 = R(1.0) / fz2;
+    E.g4 = log(i1 + sqrt(R(1.0) + i1 * i1));
+    E.g5 = log(i2 + sqrt(R(1.0) + i2 * i2));
+    R sq = sqrt(E.ADD);
+    E.g7 = R(kPi) - acos(clampR((-zeta * E.c2b) / sq)) - acos(clampR((zeta * E.c1b) / sq));
     E.g8 = E.F * E.F / E.ADD;
     return E;
 }
@@ -217,65 +236,66 @@ __host__ __device__ inline EdgeG make_edge(double zeta, double th1, double th2, 
 // TC (P region between the S circle and the edge, coeffG_TC.m) when tc is
 // true; TR_P (P region between the plane foot and the edge, coeffG_TR_P.m)
 // otherwise.  zeta > 1e-6 branch of both.
-__host__ __device__ inline void region_edgeP(bool tc, double RS, double rhoS, double zeta, double th1, double th2,
-                                   const ChildGeo& g, const Phi& P, const Level& K, ChildAcc& A) {
-    double T1t = th2 - th1;
-    EdgeG E = make_edge(zeta, th1, th2, g);
-    const double F = E.F;
-    double ca_g_1 = cos(g.al - g.ga + th1), ca_g_2 = cos(g.al - g.ga + th2);
-    double cta1 = cos(th1 + g.al), cta2 = cos(th2 + g.al);
-    double ctb1 = cos(th1 - g.be), ctb2 = cos(th2 - g.be);
-    double cgt1 = cos(g.ga - th1), cgt2 = cos(g.ga - th2);
-    double ct1 = cos(th1), ct2 = cos(th2);
-    double G1, G4, G5, G6, G7, G8, G9, G14, G15, G16, G17, G18, G19;
-    double z2 = zeta * zeta, iz = 1.0 / zeta;
-    double h5 = cgt2 * E.g5 - cgt1 * E.g4 - g.ca * E.g3;
-    double h6 = -ct2 * E.g5 + ct1 * E.g4 - g.cb * E.g3;
-    double f7 = F * (-E.g1 * ca_g_1 + E.g2 * ca_g_2 + E.g3 * g.sa * g.sa);
-    double f8 = F * (-g.sa * g.sb * E.g3 + E.g1 * cta1 - E.g2 * cta2);
-    double f9 = F * (g.sb * g.sb * E.g3 + E.g1 * ctb1 - E.g2 * ctb2);
-    double u15 = (g.sa * E.s2b - E.g8 * g.ca * E.c2b) / E.g2 + (-g.sa * E.s1b + E.g8 * g.ca * E.c1b) / E.g1;
-    double u16 = (g.sb * E.s1b + E.g8 * g.cb * E.c1b) / E.g1 + (-g.sb * E.s2b - E.g8 * g.cb * E.c2b) / E.g2;
-    double v17 = (-(E.s2b * E.s2b) * ca_g_2 + E.g8 * (g.ca * g.ca) * E.c2b) / E.g2 -
+template <typename R>
+__host__ __device__ inline void region_edgeP(bool tc, R RS, R rhoS, R zeta, R th1, R th2,
+                                   const ChildGeoT<R>& g, const Phi<R>& P, const LevelT<R>& K, ChildAcc& A) {
+    R T1t = th2 - th1;
+    EdgeG<R> E = make_edge<R>(zeta, th1, th2, g);
+    const R F = E.F;
+    R ca_g_1 = cos(g.al - g.ga + th1), ca_g_2 = cos(g.al - g.ga + th2);
+    R cta1 = cos(th1 + g.al), cta2 = cos(th2 + g.al);
+    R ctb1 = cos(th1 - g.be), ctb2 = cos(th2 - g.be);
+    R cgt1 = cos(g.ga - th1), cgt2 = cos(g.ga - th2);
+    R ct1 = cos(th1), ct2 = cos(th2);
+    R G1, G4, G5, G6, G7, G8, G9, G14, G15, G16, G17, G18, G19;
+    R z2 = zeta * zeta, iz = R(1.0) / zeta;
+    R h5 = cgt2 * E.g5 - cgt1 * E.g4 - g.ca * E.g3;
+    R h6 = -ct2 * E.g5 + ct1 * E.g4 - g.cb * E.g3;
+    R f7 = F * (-E.g1 * ca_g_1 + E.g2 * ca_g_2 + E.g3 * g.sa * g.sa);
+    R f8 = F * (-g.sa * g.sb * E.g3 + E.g1 * cta1 - E.g2 * cta2);
+    R f9 = F * (g.sb * g.sb * E.g3 + E.g1 * ctb1 - E.g2 * ctb2);
+    R u15 = (g.sa * E.s2b - E.g8 * g.ca * E.c2b) / E.g2 + (-g.sa * E.s1b + E.g8 * g.ca * E.c1b) / E.g1;
+    R u16 = (g.sb * E.s1b + E.g8 * g.cb * E.c1b) / E.g1 + (-g.sb * E.s2b - E.g8 * g.cb * E.c2b) / E.g2;
+    R v17 = (-(E.s2b * E.s2b) * ca_g_2 + E.g8 * (g.ca * g.ca) * E.c2b) / E.g2 -
                  (-(E.s1b * E.s1b) * ca_g_1 + E.g8 * (g.ca * g.ca) * E.c1b) / E.g1;
-    double v18 = ((E.s2b * E.s2b) * cta2 + E.g8 * (g.ca * g.cb * E.c2b)) / E.g2 -
+    R v18 = ((E.s2b * E.s2b) * cta2 + E.g8 * (g.ca * g.cb * E.c2b)) / E.g2 -
                  ((E.s1b * E.s1b) * cta1 + E.g8 * (g.ca * g.cb * E.c1b)) / E.g1;
-    double v19 = ((E.s2b * E.s2b) * ctb2 + E.g8 * (g.cb * g.cb * E.c2b)) / E.g2 -
+    R v19 = ((E.s2b * E.s2b) * ctb2 + E.g8 * (g.cb * g.cb * E.c2b)) / E.g2 -
                  ((E.s1b * E.s1b) * ctb1 + E.g8 * (g.cb * g.cb * E.c1b)) / E.g1;
-    double w14 = F / E.ADD * (E.c1b / E.g1 - E.c2b / E.g2);
+    R w14 = F / E.ADD * (E.c1b / E.g1 - E.c2b / E.g2);
     if (tc) {  // coeffG_TC.m:44-136
-        double lnS = (rhoS / RS) - log((rhoS + RS) * iz);
-        double q = (RS * RS + z2) / RS;
-        double iRS3 = 1.0 / (3.0 * RS * RS * RS);
-        double rS3 = rhoS * rhoS * rhoS / (3.0 * z2 * RS * RS * RS);
-        double tail = (1.0 / RS) - z2 * iRS3;
+        R lnS = (rhoS / RS) - log((rhoS + RS) * iz);
+        R q = (RS * RS + z2) / RS;
+        R iRS3 = R(1.0) / (R(3.0) * RS * RS * RS);
+        R rS3 = rhoS * rhoS * rhoS / (R(3.0) * z2 * RS * RS * RS);
+        R tail = (R(1.0) / RS) - z2 * iRS3;
         G1 = -(RS * T1t) + zeta * E.g7 + F * E.g3;
-        G4 = (1.0 / RS) * T1t - iz * E.g7;
+        G4 = (R(1.0) / RS) * T1t - iz * E.g7;
         G5 = h5 + lnS * (cgt2 - cgt1);
         G6 = h6 - lnS * (ct2 - ct1);
         G7 = zeta * E.g7 - q * P.p5 + f7;
         G8 = -zeta * g.cg * E.g7 - q * P.p4 + f8;
         G9 = zeta * E.g7 - q * P.p3 + f9;
-        G14 = iRS3 * T1t + 1.0 / (3.0 * z2) * (w14 - iz * E.g7);
-        G15 = -rS3 * P.p2 + 1.0 / (3.0 * z2) * u15;
-        G16 = -rS3 * P.p1 + 1.0 / (3.0 * z2) * u16;
-        G17 = 1.0 / 3.0 * v17 / F - E.g7 / (3.0 * zeta) + tail * P.p5;
-        G18 = 1.0 / 3.0 * v18 / F + E.g7 / (3.0 * zeta) * g.cg + tail * P.p4;
-        G19 = 1.0 / 3.0 * v19 / F - E.g7 / (3.0 * zeta) + tail * P.p3;
+        G14 = iRS3 * T1t + R(1.0) / (R(3.0) * z2) * (w14 - iz * E.g7);
+        G15 = -rS3 * P.p2 + R(1.0) / (R(3.0) * z2) * u15;
+        G16 = -rS3 * P.p1 + R(1.0) / (R(3.0) * z2) * u16;
+        G17 = R(1.0) / R(3.0) * v17 / F - E.g7 / (R(3.0) * zeta) + tail * P.p5;
+        G18 = R(1.0) / R(3.0) * v18 / F + E.g7 / (R(3.0) * zeta) * g.cg + tail * P.p4;
+        G19 = R(1.0) / R(3.0) * v19 / F - E.g7 / (R(3.0) * zeta) + tail * P.p3;
     } else {  // coeffG_TR_P.m:63-119
         G1 = zeta * (-T1t + E.g7) + F * E.g3;
         G4 = iz * (T1t - E.g7);
         G5 = h5;
         G6 = h6;
-        G7 = zeta * (E.g7 - 2.0 * P.p5) + f7;
-        G8 = -zeta * (g.cg * E.g7 + 2.0 * P.p4) + f8;
-        G9 = zeta * (E.g7 - 2.0 * P.p3) + f9;
-        G14 = 1.0 / (3.0 * z2) * (iz * (T1t - E.g7) + w14);
-        G15 = 1.0 / (3.0 * z2) * u15;
-        G16 = 1.0 / (3.0 * z2) * u16;
-        G17 = 1.0 / (3.0 * F) * v17 - 1.0 / (3.0 * zeta) * (E.g7 - 2.0 * P.p5);
-        G18 = 1.0 / (3.0 * F) * v18 + 1.0 / (3.0 * zeta) * (E.g7 * g.cg + 2.0 * P.p4);
-        G19 = 1.0 / (3.0 * F) * v19 - 1.0 / (3.0 * zeta) * (E.g7 - 2.0 * P.p3);
+        G7 = zeta * (E.g7 - R(2.0) * P.p5) + f7;
+        G8 = -zeta * (g.cg * E.g7 + R(2.0) * P.p4) + f8;
+        G9 = zeta * (E.g7 - R(2.0) * P.p3) + f9;
+        G14 = R(1.0) / (R(3.0) * z2) * (iz * (T1t - E.g7) + w14);
+        G15 = R(1.0) / (R(3.0) * z2) * u15;
+        G16 = R(1.0) / (R(3.0) * z2) * u16;
+        G17 = R(1.0) / (R(3.0) * F) * v17 - R(1.0) / (R(3.0) * zeta) * (E.g7 - R(2.0) * P.p5);
+        G18 = R(1.0) / (R(3.0) * F) * v18 + R(1.0) / (R(3.0) * zeta) * (E.g7 * g.cg + R(2.0) * P.p4);
+        G19 = R(1.0) / (R(3.0) * F) * v19 - R(1.0) / (R(3.0) * zeta) * (E.g7 - R(2.0) * P.p3);
     }
     // TC.m:18-29 / TR_P.m:65-76
     A.s += K.k1 * G1 + K.k3 * G4;
@@ -288,30 +308,31 @@ __host__ __device__ inline void region_edgeP(bool tc, double RS, double rhoS, do
 }
 
 // Coplanar (zeta <= 1e-6) branch of coeffG_TC.m:138-181 (TC.m:31-49).
-__host__ __device__ inline void region_tc_coplanar(double RS, double th1, double th2, const ChildGeo& g,
-                                          const Phi& P, const Level& K, ChildAcc& A) {
-    double T1t = th2 - th1;
-    const double F = g.a * g.sb;
-    double c1b = cos(th1 + g.be), c2b = cos(th2 + g.be);
-    double s1b = sin(th1 + g.be), s2b = sin(th2 + g.be);
-    double g_3 = 0.5 * (log((1.0 + c1b) * (1.0 - c2b)) - log((1.0 - c1b) * (1.0 + c2b)));
-    double ca_g_1 = cos(g.al - g.ga + th1), ca_g_2 = cos(g.al - g.ga + th2);
-    double cta1 = cos(th1 + g.al), cta2 = cos(th2 + g.al);
-    double ctb1 = cos(g.be - th1), ctb2 = cos(g.be - th2);
-    double iRS = 1.0 / RS;
-    double G1 = F * g_3 - RS * T1t;
-    double G4 = 1.0 / F * (c2b - c1b) + iRS * T1t;
-    double G7 = -RS * P.p5 + F * (ca_g_2 - ca_g_1 + g.sa * g.sa * g_3);
-    double G8 = -RS * P.p4 + F * (cta1 - cta2 - g.sa * g.sb * g_3);
-    double G9 = -RS * P.p3 + F * (ctb1 - ctb2 + g.sb * g.sb * g_3);
-    double i3F = 1.0 / (3.0 * F);
-    double G17 = i3F * (((1.0 + g.ca * g.ca) * c2b - (s2b * s2b) * ca_g_2) -
-                        ((1.0 + g.ca * g.ca) * c1b - (s1b * s1b) * ca_g_1)) +
+template <typename R>
+__host__ __device__ inline void region_tc_coplanar(R RS, R th1, R th2, const ChildGeoT<R>& g,
+                                          const Phi<R>& P, const LevelT<R>& K, ChildAcc& A) {
+    R T1t = th2 - th1;
+    const R F = g.a * g.sb;
+    R c1b = cos(th1 + g.be), c2b = cos(th2 + g.be);
+    R s1b = sin(th1 + g.be), s2b = sin(th2 + g.be);
+    R g_3 = R(0.5) * (log((R(1.0) + c1b) * (R(1.0) - c2b)) - log((R(1.0) - c1b) * (R(1.0) + c2b)));
+    R ca_g_1 = cos(g.al - g.ga + th1), ca_g_2 = cos(g.al - g.ga + th2);
+    R cta1 = cos(th1 + g.al), cta2 = cos(th2 + g.al);
+    R ctb1 = cos(g.be - th1), ctb2 = cos(g.be - th2);
+    R iRS = R(1.0) / RS;
+    R G1 = F * g_3 - RS * T1t;
+    R G4 = R(1.0) / F * (c2b - c1b) + iRS * T1t;
+    R G7 = -RS * P.p5 + F * (ca_g_2 - ca_g_1 + g.sa * g.sa * g_3);
+    R G8 = -RS * P.p4 + F * (cta1 - cta2 - g.sa * g.sb * g_3);
+    R G9 = -RS * P.p3 + F * (ctb1 - ctb2 + g.sb * g.sb * g_3);
+    R i3F = R(1.0) / (R(3.0) * F);
+    R G17 = i3F * (((R(1.0) + g.ca * g.ca) * c2b - (s2b * s2b) * ca_g_2) -
+                        ((R(1.0) + g.ca * g.ca) * c1b - (s1b * s1b) * ca_g_1)) +
                  iRS * P.p5;
-    double cab = g.ca * g.cb + cos(g.al + g.be);
-    double G18 = i3F * ((c2b * cab + (s2b * s2b) * cta2) - (c1b * cab + (s1b * s1b) * cta1)) + iRS * P.p4;
-    double G19 = i3F * (((1.0 + g.cb * g.cb) * c2b + (s2b * s2b) * ctb2) -
-                        ((1.0 + g.cb * g.cb) * c1b + (s1b * s1b) * ctb1)) +
+    R cab = g.ca * g.cb + cos(g.al + g.be);
+    R G18 = i3F * ((c2b * cab + (s2b * s2b) * cta2) - (c1b * cab + (s1b * s1b) * cta1)) + iRS * P.p4;
+    R G19 = i3F * (((R(1.0) + g.cb * g.cb) * c2b + (s2b * s2b) * ctb2) -
+                        ((R(1.0) + g.cb * g.cb) * c1b + (s1b * s1b) * ctb1)) +
                  iRS * P.p3;
     A.s += K.k1 * G1 + K.k3 * G4;
     A.b[3] += K.k2 * G7 + K.k4 * G17;
@@ -320,21 +341,22 @@ __host__ __device__ inline void region_tc_coplanar(double RS, double th1, double
 }
 
 // TR_SP (S disk part bounded by the edge, coeffG_TR_SP.m, TR_SP.m).
-__host__ __device__ inline void region_edgeS(double zeta, double th1, double th2, const ChildGeo& g, const Phi& P,
-                                    const Level& K, ChildAcc& A) {
-    double T1t = th2 - th1;
-    if (zeta > 1.0e-6) {
-        EdgeG E = make_edge(zeta, th1, th2, g);
-        const double F = E.F;
-        double G1 = zeta * (-T1t + E.g7) + F * E.g3;
-        double G4 = 1.0 / zeta * (T1t - E.g7);
-        double G5 = cos(g.ga - th2) * E.g5 - cos(g.ga - th1) * E.g4 - g.ca * E.g3;
-        double G6 = -cos(th2) * E.g5 + cos(th1) * E.g4 - g.cb * E.g3;
-        double G7 = zeta * (E.g7 - 2.0 * P.p5) +
+template <typename R>
+__host__ __device__ inline void region_edgeS(bool zpos, R zeta, R th1, R th2, const ChildGeoT<R>& g, const Phi<R>& P,
+                                    const LevelT<R>& K, ChildAcc& A) {
+    R T1t = th2 - th1;
+    if (zpos) {
+        EdgeG<R> E = make_edge<R>(zeta, th1, th2, g);
+        const R F = E.F;
+        R G1 = zeta * (-T1t + E.g7) + F * E.g3;
+        R G4 = R(1.0) / zeta * (T1t - E.g7);
+        R G5 = cos(g.ga - th2) * E.g5 - cos(g.ga - th1) * E.g4 - g.ca * E.g3;
+        R G6 = -cos(th2) * E.g5 + cos(th1) * E.g4 - g.cb * E.g3;
+        R G7 = zeta * (E.g7 - R(2.0) * P.p5) +
                     F * (-E.g1 * cos(g.al - g.ga + th1) + E.g2 * cos(g.al - g.ga + th2) + E.g3 * g.sa * g.sa);
-        double G8 = -zeta * (g.cg * E.g7 + 2.0 * P.p4) +
+        R G8 = -zeta * (g.cg * E.g7 + R(2.0) * P.p4) +
                     F * (-g.sa * g.sb * E.g3 + E.g1 * cos(th1 + g.al) - E.g2 * cos(th2 + g.al));
-        double G9 = zeta * (E.g7 - 2.0 * P.p3) +
+        R G9 = zeta * (E.g7 - R(2.0) * P.p3) +
                     F * (g.sb * g.sb * E.g3 + E.g1 * cos(th1 - g.be) - E.g2 * cos(th2 - g.be));
         A.s += K.k5 * G1;
         A.b[0] += K.k6 * G4;
@@ -344,13 +366,13 @@ __host__ __device__ inline void region_edgeS(double zeta, double th1, double th2
         A.b[4] += K.k6 * G8;
         A.b[5] += K.k6 * G9;
     } else {
-        const double F = g.a * g.sb;
-        double c1b = cos(th1 + g.be), c2b = cos(th2 + g.be);
-        double g_3 = 0.5 * (log((1.0 + c1b) * (1.0 - c2b)) - log((1.0 - c1b) * (1.0 + c2b)));
-        double G1 = F * g_3;
-        double G7 = F * (-cos(g.al - g.ga + th1) + cos(g.al - g.ga + th2) + g.sa * g.sa * g_3);
-        double G8 = F * (-(g.sa * g.sb * g_3) + cos(th1 + g.al) - cos(th2 + g.al));
-        double G9 = F * (g.sb * g.sb * g_3 + cos(th1 - g.be) - cos(th2 - g.be));
+        const R F = g.a * g.sb;
+        R c1b = cos(th1 + g.be), c2b = cos(th2 + g.be);
+        R g_3 = R(0.5) * (log((R(1.0) + c1b) * (R(1.0) - c2b)) - log((R(1.0) - c1b) * (R(1.0) + c2b)));
+        R G1 = F * g_3;
+        R G7 = F * (-cos(g.al - g.ga + th1) + cos(g.al - g.ga + th2) + g.sa * g.sa * g_3);
+        R G8 = F * (-(g.sa * g.sb * g_3) + cos(th1 + g.al) - cos(th2 + g.al));
+        R G9 = F * (g.sb * g.sb * g_3 + cos(th1 - g.be) - cos(th2 - g.be));
         A.s += K.k5 * G1;
         A.b[3] += K.k6 * G7;
         A.b[4] += K.k6 * G8;
@@ -385,10 +407,12 @@ __host__ __device__ inline ChildGeo child_geo(double v2x, double v2y, double v3x
 }
 
 // calcIntgInt_childTriangle.m:9-234 for one child, result folded into A.
+template <typename R>
 __host__ __device__ inline void child_regions(const ChildGeo& g, double zeta, double RP, double rhoP, double RS,
-                                     double rhoS, int type, const Level& K, ChildAcc& A) {
+                                     double rhoS, int type, const Level& Kd, ChildAcc& A) {
     const double a = g.a;
-    // childTriangle.m:27-50.  Rotated child: V2 = (a,0), V3 = (b cg, b sg).
+    // childTriangle.m:27-50 (front cutting, always FP64).  Rotated child:
+    // V2 = (a,0), V3 = (b cg, b sg).
     double vt0 = (g.b * g.cg - a) / g.c;
     double Bm = a * vt0;
     double DeltaP = Bm * Bm - (a * a - rhoP * rhoP);
@@ -420,37 +444,42 @@ __host__ __device__ inline void child_regions(const ChildGeo& g, double zeta, do
         TH5 = fmax(0.0, fmin(fmax(O1, O2), gamma));
     }
     const bool sp = (type == kSPSP);
-    const bool coplanar = !(zeta > 1.0e-6);
+    const bool zpos = zeta > 1.0e-6;
+    // Closed-form region integrals in R (FP64 for the plain form, FP32 for the
+    // far field of the mixed-precision form).
+    const ChildGeoT<R> gr = cast_geo<R>(g);
+    const LevelT<R> K{R(Kd.t), R(Kd.rP), R(Kd.rS), R(Kd.k1), R(Kd.k3), R(Kd.k5), R(Kd.k2), R(Kd.k4), R(Kd.k6)};
+    const R rs = R(RS), rhs = R(rhoS), rp = R(RP), rhp = R(rhoP), z = R(zeta);
     // One Phi per angular interval, shared by the S-sector and P routines on it
     // (identical arguments, identical values).
     if (TH1 < TH2) {  // E1 (S sector) + E6 (P annulus)
-        Phi P = make_phi(TH1, TH2, g);
-        if (sp) region_sdisk(RS, rhoS, zeta, TH1, TH2, P, K, A);
-        region_annulus(type != kP, RS, rhoS, RP, rhoP, zeta, TH1, TH2, P, K, A);
+        Phi<R> P = make_phi<R>(R(TH1), R(TH2), gr);
+        if (sp) region_sdisk<R>(zpos, rs, rhs, z, R(TH1), R(TH2), P, K, A);
+        region_annulus<R>(type != kP, zpos, rs, rhs, rp, rhp, z, R(TH1), R(TH2), P, K, A);
     }
     if (TH2 < TH3) {  // E2 (S sector) + E7 (P region bounded by the edge); TH5 == TH3 for class P
-        Phi P = make_phi(TH2, TH3, g);
-        if (sp) region_sdisk(RS, rhoS, zeta, TH2, TH3, P, K, A);
-        if (type == kP) region_edgeP(false, RS, rhoS, zeta, TH2, TH5, g, P, K, A);
-        else if (coplanar) region_tc_coplanar(RS, TH2, TH3, g, P, K, A);
-        else region_edgeP(true, RS, rhoS, zeta, TH2, TH3, g, P, K, A);
+        Phi<R> P = make_phi<R>(R(TH2), R(TH3), gr);
+        if (sp) region_sdisk<R>(zpos, rs, rhs, z, R(TH2), R(TH3), P, K, A);
+        if (type == kP) region_edgeP<R>(false, rs, rhs, z, R(TH2), R(TH5), gr, P, K, A);
+        else if (!zpos) region_tc_coplanar<R>(rs, R(TH2), R(TH3), gr, P, K, A);
+        else region_edgeP<R>(true, rs, rhs, z, R(TH2), R(TH3), gr, P, K, A);
     }
     if (TH3 < TH4) {  // E3 (S region bounded by the edge)
         if (sp) {
-            Phi P = make_phi(TH3, TH4, g);
-            region_edgeS(zeta, TH3, TH4, g, P, K, A);
+            Phi<R> P = make_phi<R>(R(TH3), R(TH4), gr);
+            region_edgeS<R>(zpos, z, R(TH3), R(TH4), gr, P, K, A);
         }
     }
     if (TH4 < TH5) {  // E4 (S sector) + E8 (P region between S circle and edge)
-        Phi P = make_phi(TH4, TH5, g);
-        if (sp) region_sdisk(RS, rhoS, zeta, TH4, TH5, P, K, A);
-        if (coplanar) region_tc_coplanar(RS, TH4, TH5, g, P, K, A);
-        else region_edgeP(true, RS, rhoS, zeta, TH4, TH5, g, P, K, A);
+        Phi<R> P = make_phi<R>(R(TH4), R(TH5), gr);
+        if (sp) region_sdisk<R>(zpos, rs, rhs, z, R(TH4), R(TH5), P, K, A);
+        if (!zpos) region_tc_coplanar<R>(rs, R(TH4), R(TH5), gr, P, K, A);
+        else region_edgeP<R>(true, rs, rhs, z, R(TH4), R(TH5), gr, P, K, A);
     }
     if (TH5 < TH6) {  // E5 (S sector) + E9 (P annulus)
-        Phi P = make_phi(TH5, TH6, g);
-        if (sp) region_sdisk(RS, rhoS, zeta, TH5, TH6, P, K, A);
-        region_annulus(type != kP, RS, rhoS, RP, rhoP, zeta, TH5, TH6, P, K, A);
+        Phi<R> P = make_phi<R>(R(TH5), R(TH6), gr);
+        if (sp) region_sdisk<R>(zpos, rs, rhs, z, R(TH5), R(TH6), P, K, A);
+        region_annulus<R>(type != kP, zpos, rs, rhs, rp, rhp, z, R(TH5), R(TH6), P, K, A);
     }
 }
 
@@ -549,6 +578,7 @@ __host__ __device__ inline int classify(double zeta, double rP, double rS, doubl
 
 // One outer point at one level: adds w * (s, M_local) into acc.
 // Returns the integration class (0 when the point is skipped).
+template <typename R = double>
 __host__ __device__ __attribute__((always_inline)) inline int node_eval(const NodeGeo& G, const Level& K, double w, NodeAcc& acc) {
     double rhoP, rhoS;
     int type = classify(G.zeta, K.rP, K.rS, G.dMIN, rhoP, rhoS);
@@ -573,7 +603,7 @@ __host__ __device__ __attribute__((always_inline)) inline int node_eval(const No
         ChildAcc A;
         A.s = 0.0;
         for (int i = 0; i < 6; i++) A.b[i] = 0.0;
-        child_regions(g, zeta, K.rP, rhoP, K.rS, rhoS, type, K, A);
+        child_regions<R>(g, zeta, K.rP, rhoP, K.rS, rhoS, type, K, A);
         // Child matrix sum_i b_i B_i (coeffB.m:136-160), symmetric, child frame.
         double isg = 1.0 / g.sg;
         double isg2 = isg * isg;

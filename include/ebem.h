@@ -83,6 +83,11 @@ int ebem_assemble(ebem_handle *h, int precision);
  * (pair, level) integral items evaluated, bytes of the store. */
 int ebem_store_info(ebem_handle *h, int64_t *nnzb, int64_t *pair_lags, int64_t *items, int64_t *bytes);
 
+/* Items of the last ebem_assemble: (pair, level) integral items evaluated in
+ * FP32 (far field, EBEM_PREC_MIXED) and in FP64.  No reference counterpart
+ * (the reference has no mixed precision); reports the split of the work. */
+int ebem_assembly_stats(ebem_handle *h, int64_t *far_items, int64_t *fp64_items);
+
 /* Export E_lag as a dense 3N x 3N column-major host matrix (the `matrix` of
  * calcMatrixBlock.m / matrixSavedV{lag+1} of timeMarchingD.m:141-158). */
 int ebem_get_block(ebem_handle *h, int lag, double *out);

@@ -13,7 +13,7 @@ _dp = C.POINTER(C.c_double)
 
 def load(fma=False):
     lib = C.CDLL(os.path.join(HERE, "native", "libsa_host_fma.so" if fma else "libsa_host.so"))
-    lib.sa_item.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double, _dp]
+    lib.sa_item.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double, _dp, C.c_int]
     return lib
 
 
@@ -31,7 +31,7 @@ def windows(o, L):
     return lo, hi
 
 
-def assemble(o, lib, L, rows=None, lags=None):
+def assemble(o, lib, L, rows=None, lags=None, fp32_far=None):
     """Dict lag -> dense E (3N x 3N) restricted to `rows`."""
     N = o.N
     lo, hi = windows(o, L)
@@ -54,10 +54,11 @@ def assemble(o, lib, L, rows=None, lags=None):
             if a > b:
                 continue
             W = {}
+            f32 = int(fp32_far is not None and np.linalg.norm(o.center[i] - o.center[j]) > fp32_far)
             for m in range(max(1, a - 1), b + 2):
                 w = np.zeros(6)
                 lib.sa_item(recs[j].ctypes.data_as(_dp), gh.ctypes.data_as(_dp), n, m * o.dt, o.velP, o.velS,
-                            w.ctypes.data_as(_dp))
+                            w.ctypes.data_as(_dp), f32)
                 W[m] = w
             for l in lags:
                 if not (a <= l <= b):

@@ -34,7 +34,7 @@ int sa_point_block(const double* rec, const double* x, double t, double cP, doub
 extern "C" {
 // One (pair, level) item exactly as k_levels computes it: W (6 symmetric values,
 // global frame, not divided by 4 pi rho).  gh: x,y,z,w rows of 32.
-void sa_item(const double* rec, const double* gh, int nGH, double t, double cP, double cS, double* W6) {
+void sa_item(const double* rec, const double* gh, int nGH, double t, double cP, double cS, double* W6, int fp32) {
     FieldTri F;
     std::memcpy(&F, rec, sizeof(FieldTri));
     Level K = make_level(t, cP, cS);
@@ -44,7 +44,8 @@ void sa_item(const double* rec, const double* gh, int nGH, double t, double cP, 
     for (int q = 0; q < nGH; q++) {
         double x[3] = {gh[q], gh[32 + q], gh[64 + q]};
         NodeGeo G = node_geo(x, F);
-        node_eval(G, K, gh[96 + q], acc);
+        if (fp32) node_eval<float>(G, K, gh[96 + q], acc);
+        else node_eval<double>(G, K, gh[96 + q], acc);
     }
     finish_rotation(acc, F.C, W6);
 }
