@@ -56,18 +56,13 @@ def shell_points(n=4096, seed=0):
     return d * r[:, None]
 
 
-def history_bytes(m, pb, L):
-    """Algorithmic bytes the 128 history sums stream: 48 B of block values + 4 B
-    column index per block with 1 <= lag <= min(n, L-1), summed over steps."""
-    c, mx = m.center, m.maxL
-    dt, cS, cP = pb.Tfin / pb.nT, pb.velS, pb.velP
-    d = np.sqrt(((c[None, :, :] - c[:, None, :]) ** 2).sum(-1))
-    dmin = d - mx[None, :] - mx[:, None]
-    dmax = d + mx[None, :] + mx[:, None]
-    per_lag = np.array([np.count_nonzero(((l - 1) * cS * dt < dmax) & ((l + 1) * cP * dt >= dmin))
-                        for l in range(L)], dtype=np.int64)
-    blocks = sum(int(per_lag[1:min(n, L - 1) + 1].sum()) for n in range(pb.nT))
-    return blocks * 52, per_lag
+def history_bytes(per_lag, nT):
+    """Algorithmic bytes the nT history sums stream: 48 B of block values + 4 B
+    column index per stored block with 1 <= lag <= min(n, L-1), summed over steps
+    (per_lag: stored blocks per lag)."""
+    L = len(per_lag)
+    blocks = sum(int(per_lag[1:min(n, L - 1) + 1].sum()) for n in range(nT))
+    return blocks * 52
 
 
 def cpu_baseline(m, pb, seconds_hint=True):
@@ -152,7 +147,7 @@ def main():
     hist_ms = float(np.mean([t["history_ms"] for t in tims]))
     fac_ms = float(np.mean([t["factor_ms"] for t in tims]))
     pp_ms = float(np.mean([t["pp_ms"] for t in tims]))
-    hb, _ = history_bytes(m, pb, L)
+    hb = history_bytes(s.lag_blocks(), pb.nT)
 
     roof = None
     if ALG_FLOP_PER_ITEM:

@@ -17,6 +17,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
 _vp = C.c_void_p
+_fp = C.POINTER(C.c_float)
 
 # symbol -> (restype, argtypes); every symbol declared in include/ebem.h
 SIGNATURES = {
@@ -30,12 +31,26 @@ SIGNATURES = {
     "ebem_assemble": (C.c_int, [_vp, C.c_int]),
     "ebem_store_info": (C.c_int, [_vp, _i64p, _i64p, _i64p, _i64p]),
     "ebem_assembly_stats": (C.c_int, [_vp, _i64p, _i64p]),
+    "ebem_store_lag_blocks": (C.c_int, [_vp, _i64p]),
     "ebem_get_block": (C.c_int, [_vp, C.c_int, _dp]),
     "ebem_rhs": (C.c_int, [_vp, C.c_int, _dp]),
     "ebem_mot_solve": (C.c_int, [_vp, _dp, _dp]),
     "ebem_pp_eval": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
     "ebem_node_integral": (C.c_int, [_vp, _dp, C.c_int, C.c_double, _dp]),
     "ebem_timings": (C.c_int, [_vp, _dp]),
+    "ebem_sur_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+    "ebem_sur_destroy": (C.c_int, [_vp]),
+    "ebem_sur_add_samples": (C.c_int, [_vp, _vp, C.c_int64, _ip, _ip, _ip]),
+    "ebem_sur_num_samples": (C.c_int, [_vp, _i64p]),
+    "ebem_sur_get_samples": (C.c_int, [_vp, C.c_int64, C.c_int64, _fp, _fp]),
+    "ebem_sur_init": (C.c_int, [_vp, C.c_int64, C.c_uint64]),
+    "ebem_sur_train": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]),
+    "ebem_sur_loss": (C.c_int, [_vp, C.c_int64, C.c_int64, _dp]),
+    "ebem_sur_grad": (C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _dp]),
+    "ebem_sur_predict": (C.c_int, [_vp, C.c_int64, _fp, _fp]),
+    "ebem_sur_params": (C.c_int, [_vp, C.c_int, _fp, _fp]),
+    "ebem_attach_surrogate": (C.c_int, [_vp, _vp]),
+    "ebem_surrogate_stats": (C.c_int, [_vp, _i64p, _i64p, _dp]),
 }
 
 _lib = None
@@ -75,3 +90,9 @@ def dptr(a: np.ndarray | None):
     if a is None:
         return None
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_fp)

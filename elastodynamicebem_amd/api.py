@@ -26,6 +26,7 @@ from .mesh import SpaceMesh, read_mesh
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_SURROGATE = 2  # flag: near-field pairs from the attached surrogate (surrogate.py)
 
 DATUM_SCREEN_SIN2 = 0
 DATUM_SPHERE_T2EXP = 1
@@ -165,10 +166,21 @@ class Solver:
         _lib.check(self.lib.ebem_store_info(self.h, *[C.byref(x) for x in v]))
         return dict(nnzb=v[0].value, pair_lags=v[1].value, items=v[2].value, bytes=v[3].value)
 
+    def lag_blocks(self) -> np.ndarray:
+        """Stored 3x3 blocks per lag (length L)."""
+        out = np.zeros(self.num_blocks(), np.int64)
+        _lib.check(self.lib.ebem_store_lag_blocks(self.h, out.ctypes.data_as(_lib._i64p)))
+        return out
+
     def assembly_stats(self) -> dict:
         a, b = C.c_int64(), C.c_int64()
         _lib.check(self.lib.ebem_assembly_stats(self.h, C.byref(a), C.byref(b)))
         return dict(far_items=a.value, fp64_items=b.value)
+
+    def surrogate_stats(self) -> dict:
+        a, b, ms = C.c_int64(), C.c_int64(), C.c_double()
+        _lib.check(self.lib.ebem_surrogate_stats(self.h, C.byref(a), C.byref(b), C.byref(ms)))
+        return dict(items=a.value, blocks=b.value, ms=ms.value)
 
     def get_block(self, lag: int) -> np.ndarray:
         """Dense E_lag (3N x 3N), row index = test DOF."""

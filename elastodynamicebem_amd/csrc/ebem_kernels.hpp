@@ -83,8 +83,54 @@ __device__ inline int64_t pair_items(int lo, int hi) {
 }
 
 // grid: ceil(rows*N/256); pairs p = (i - r0)*N + j for rows [r0, r1).
-__global__ void __launch_bounds__(256) k_windows(const FieldTri* __restrict__ tri, AsmParams P, int r0, int r1,
-                                                 int16_t* __restrict__ lo, int16_t* __restrict__ hi) {
+// Lags whose block is zero in exact arithmetic although the centroid test of
+// calcMatrixBlock.m:57-61 lets the pair through:
+//  * after the S front: W_m = sum_q w_q I(x_q, t_m) is the same for every
+//    level m with c_S t_m > Dmax = max over outer nodes x_q and field vertices
+//    v of |x_q - v| (every node sees the whole field triangle inside its S disk,
+//    the P annulus misses it, and the disk integrals no longer depend on t), so
+//    E_l = W_{l+1} - 2 W_l + W_{l-1} = 0 once level l-1 is such a level;
+//  * before the P front: W_m = 0 while c_P t_m < Dlo = min_q |x_q - c_F| - R_F
+//    (R_F the field triangle's circumradius about its centroid), so E_l = 0
+//    while W_{l+1} = 0.
+// The analytic form reproduces these zeros to rounding, far below the 1e-14
+// cleanup of calcMatrixBlock.m:77, so dropping the lags leaves the matrices
+// unchanged; the relative margins 1e-12 keep ties on the computed side.
+__device__ inline void exact_window(const AsmParams& P, const double* __restrict__ gS, const double* __restrict__ vF,
+                                    const double* cF, int& l, int& h) {
+    double rf2 = 0.0;
+    for (int v = 0; v < 3; v++) {
+        double a = vF[3 * v] - cF[0], b = vF[3 * v + 1] - cF[1], c = vF[3 * v + 2] - cF[2];
+        rf2 = fmax(rf2, a * a + b * b + c * c);
+    }
+    double dmax2 = 0.0, dlo = 1e300;
+    for (int q = 0; q < P.nGH; q++) {
+        const double x = gS[q], y = gS[32 + q], z = gS[64 + q];
+        for (int v = 0; v < 3; v++) {
+            double a = x - vF[3 * v], b = y - vF[3 * v + 1], c = z - vF[3 * v + 2];
+            dmax2 = fmax(dmax2, a * a + b * b + c * c);
+        }
+        double a = x - cF[0], b = y - cF[1], c = z - cF[2];
+        dlo = fmin(dlo, sqrt(a * a + b * b + c * c));
+    }
+    dlo = fmax(0.0, dlo - sqrt(rf2));
+    // last lag with a non-static level l-1: m* = floor(Dmax/(c_S dt)) + 1 is the
+    // first static level, so E_l = 0 for l >= m* + 1
+    const double ms = floor(sqrt(dmax2) * (1.0 + 1e-12) / (P.cS * P.dt)) + 1.0;
+    if (ms < (double)h) h = (int)ms;
+    // first lag whose level l+1 can reach the field triangle
+    const double ml = ceil(dlo * (1.0 - 1e-12) / (P.cP * P.dt) - 1.0);
+    if (ml > (double)l) l = (int)fmin(ml, 32000.0);
+}
+
+__global__ void __launch_bounds__(256) k_windows(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
+                                                 const double* __restrict__ verts, AsmParams P, int r0, int r1,
+                                                 int16_t* __restrict__ lo, int16_t* __restrict__ hi,
+                                                 unsigned long long* __restrict__ culled) {
+    __shared__ unsigned long long part;
+    if (threadIdx.x == 0) part = 0;
+    __syncthreads();
+    unsigned long long my = 0;
     int64_t np = (int64_t)(r1 - r0) * P.N;
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
         int i = r0 + (int)(p / P.N), j = (int)(p % P.N);
@@ -92,9 +138,16 @@ __global__ void __launch_bounds__(256) k_windows(const FieldTri* __restrict__ tr
         pair_dist(tri[i], tri[j], dmin, dmax);
         int l, h;
         lag_window(P, dmin, dmax, l, h);
+        if (l <= h) {
+            my += (unsigned long long)(h - l + 1);  // pair.lags of the reference's test
+            exact_window(P, gh + (int64_t)i * 128, verts + (int64_t)j * 9, tri[j].center, l, h);
+        }
         lo[p] = (int16_t)l;
         hi[p] = (int16_t)h;
     }
+    atomicAdd(&part, my);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(culled, part);
 }
 
 // One workgroup per row (rows [r0, r1)): cnt[(i-r0)*L + l] = blocks of row i at lag l,
@@ -174,7 +227,8 @@ __global__ void __launch_bounds__(256) k_pair_items(const int16_t* __restrict__ 
 // passed).  Items with equal keys take the same region branches in most
 // outer-node evaluations, so sorting by key keeps wavefronts convergent.
 constexpr int kFrontKeys = 25;
-constexpr int kNumKeys = 2 * kFrontKeys;  // far-field pairs (mixed precision) take keys 0..24
+constexpr int kNumKeys = 2 * kFrontKeys + 1;  // far-field pairs (mixed precision) take keys 0..24
+constexpr int kSurKey = kNumKeys - 1;         // pairs predicted by the surrogate: the last key
 
 __device__ inline int front_class(double r, double lo, double span) {
     double u = (r - lo) / span;
@@ -187,7 +241,8 @@ __device__ inline int front_class(double r, double lo, double span) {
 __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ tri, AsmParams P, int r0,
                                                    int64_t nitems, int64_t per, const int32_t* __restrict__ item_pair,
                                                    const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
-                                                   uint8_t* __restrict__ key, unsigned int* __restrict__ bh) {
+                                                   const uint8_t* __restrict__ sflag, uint8_t* __restrict__ key,
+                                                   unsigned int* __restrict__ bh) {
     __shared__ unsigned int sh[kNumKeys];
     for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) sh[k] = 0;
     __syncthreads();
@@ -208,6 +263,7 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
         int qp = front_class(P.cP * t, d - ex, 2.0 * ex) + 1;
         int qs = front_class(P.cS * t, d - ex, 2.0 * ex) + 1;
         int k = qp * 5 + qs + ((P.far2 >= 0.0 && d2 > P.far2) ? 0 : kFrontKeys);
+        if (sflag && sflag[p]) k = kSurKey;
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
     }
@@ -275,7 +331,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 // lo..hi of every pair of the row into the segmented store.
 __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, const int16_t* __restrict__ hi,
                                              const int64_t* __restrict__ off, const double* __restrict__ W,
-                                             int N, int L, int r0, double inv4pirho, Store S) {
+                                             const uint8_t* __restrict__ sflag, int N, int L, int r0,
+                                             double inv4pirho, Store S) {
     extern __shared__ int64_t run[];  // L entries
     const int r = blockIdx.x;
     const int i = r0 + r;
@@ -287,10 +344,12 @@ __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, con
         int j = j0 + lane;
         int a = 1 << 30, b = -1;
         int64_t o = 0;
+        bool sur = false;
         if (j < N) {
             a = lo[pbase + j];
             b = hi[pbase + j];
             o = off[pbase + j];
+            sur = sflag && sflag[pbase + j];
         }
         int amin = a, bmax = b;
         for (int d = 32; d >= 1; d >>= 1) {
@@ -309,12 +368,14 @@ __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, con
                 double e[6];
 #pragma unroll
                 for (int q = 0; q < 6; q++) e[q] = 1.0 * w1[q];
-                if (l >= 1) {
+                if (sur) {
+                    // surrogate pair: the slot already holds 4 pi rho E_l
+                } else if (l >= 1) {
                     const double* w0 = W + (o + (l - mlo)) * 6;
 #pragma unroll
                     for (int q = 0; q < 6; q++) e[q] = e[q] + (-2.0) * w0[q];
                 }
-                if (l >= 2) {
+                if (!sur && l >= 2) {
                     const double* wm = W + (o + (l - 1 - mlo)) * 6;
 #pragma unroll
                     for (int q = 0; q < 6; q++) e[q] = e[q] + wm[q];

@@ -21,6 +21,7 @@
 
 #include "../../include/ebem.h"
 #include "ebem_kernels.hpp"
+#include "ebem_sur.hpp"
 
 using namespace ebem;
 
@@ -284,6 +285,12 @@ struct ebem_handle {
     int L = 0;
     DevBuf<FieldTri> d_tri;
     DevBuf<double> d_gh;
+    DevBuf<double> d_verts;  // N x 9: the three vertices of every triangle, mesh order
+    double hmax = 0;         // longest edge of the mesh
+    // surrogate attached by ebem_attach_surrogate, and what the last assembly gave it
+    ebem_sur* sur = nullptr;
+    int64_t sur_items = 0, sur_pairs_blocks = 0;
+    double sur_ms = 0;
     // store
     bool assembled = false;
     int store_L = 0;
@@ -310,6 +317,188 @@ struct ebem_handle {
     }
 };
 
+struct ebem_sur {
+    std::mutex lock;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int users = 0;  // handles it is attached to
+    // samples: raw features (n x 24) and labels (n x 18)
+    int64_t n = 0, cap = 0;
+    DevBuf<float> d_X, d_Y;
+    // parameters (FP32 masters, BF16 copy), gradients, Adam moments, and the
+    // normalisation block: x mean, x 1/std, y mean, y std, box lo, box hi
+    DevBuf<float> d_p, d_g, d_m, d_v, d_norm, d_part;
+    DevBuf<uint16_t> d_pb;
+    std::vector<float> norm;
+    bool ready = false;
+    int64_t adam_t = 0;
+    // batch workspace
+    int B = 0;
+    bool wsZ = false;
+    DevBuf<uint16_t> d_A0, d_A[4], d_dYb, d_dZb[2];
+    DevBuf<float> d_Z[4], d_Yo, d_T, d_dY, d_dZ[2];
+    DevBuf<int32_t> d_idx;
+    DevBuf<double> d_loss;
+    DevBuf<int64_t> d_slot;
+};
+
+namespace {
+
+constexpr int kSurNorm = 2 * kSurIn + 2 * kSurOut + 2 * kSurIn;  // 132 floats
+static_assert(kSurNorm == EBEM_SUR_NORM, "normalisation block size");
+static_assert(kSurParams == EBEM_SUR_PARAMS, "parameter count");
+
+int sur_workspace(ebem_sur* s, int B, bool train) {
+    if (B == s->B && (!train || s->wsZ)) return 0;
+    s->B = B;
+    s->wsZ = train;
+    HIPC(s->d_A0.alloc((size_t)B * kSurInPad));
+    for (int l = 0; l < 4; l++) {
+        HIPC(s->d_A[l].alloc((size_t)B * kSurHid));
+        if (train) HIPC(s->d_Z[l].alloc((size_t)B * kSurHid));
+        else s->d_Z[l].reset();
+    }
+    HIPC(s->d_Yo.alloc((size_t)B * kSurOutPad));
+    HIPC(s->d_T.alloc((size_t)B * kSurOut));
+    HIPC(s->d_loss.alloc(1));
+    if (train) {
+        HIPC(s->d_dY.alloc((size_t)B * kSurOutPad));
+        HIPC(s->d_dYb.alloc((size_t)B * kSurOutPad));
+        for (int q = 0; q < 2; q++) {
+            HIPC(s->d_dZ[q].alloc((size_t)B * kSurHid));
+            HIPC(s->d_dZb[q].alloc((size_t)B * kSurHid));
+        }
+    }
+    return 0;
+}
+
+template <bool TA, bool TB, int EPI>
+int gemm(hipStream_t st, const uint16_t* A, int lda, const uint16_t* B, int ldb, int M, int N, int K, int splits,
+         const EpiArgs& e) {
+    if (M % 64 || N % 64 || K % 32 || lda % 8 || ldb % 8 || splits < 1)
+        return fail(EBEM_EINVAL, "gemm shape M=%d N=%d K=%d not tileable", M, N, K);
+    int ksplit = ((K / splits + 31) / 32) * 32;
+    if (ksplit < 32) ksplit = 32;
+    const int z = (K + ksplit - 1) / ksplit;
+    hipLaunchKernelGGL((k_gemm_bf16<TA, TB, EPI>), dim3(N / 64, M / 64, z), dim3(256), 0, st, A, lda, B, ldb, K, ksplit,
+                       e);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+// Forward pass of B rows from s->d_A0; keepZ stores the pre-activations.
+int sur_forward(ebem_sur* s, hipStream_t st, int B, bool keepZ) {
+    for (int l = 0; l < kSurLayers; l++) {
+        const uint16_t* in = l == 0 ? s->d_A0.p : s->d_A[l - 1].p;
+        const int K = sur_K(l), N = sur_N(l);
+        EpiArgs e{};
+        e.bias = s->d_p.p + sur_boff(l);
+        if (l < kSurLayers - 1) {
+            e.C = keepZ ? s->d_Z[l].p : nullptr;
+            e.ldc = N;
+            e.Cb = s->d_A[l].p;
+            e.ldcb = N;
+            if (int r = gemm<false, false, kEpiBiasGelu>(st, in, K, s->d_pb.p + sur_woff(l), N, B, N, K, 1, e)) return r;
+        } else {
+            e.C = s->d_Yo.p;
+            e.ldc = N;
+            if (int r = gemm<false, false, kEpiBias>(st, in, K, s->d_pb.p + sur_woff(l), N, B, N, K, 1, e)) return r;
+        }
+    }
+    return 0;
+}
+
+// Backward pass from dY (s->d_dY, s->d_dYb) into the gradient s->d_g.
+int sur_backward(ebem_sur* s, hipStream_t st, int B) {
+    const float* cur_f = s->d_dY.p;
+    const uint16_t* cur_b = s->d_dYb.p;
+    int q = 0;
+    for (int l = kSurLayers - 1; l >= 0; l--) {
+        const uint16_t* in = l == 0 ? s->d_A0.p : s->d_A[l - 1].p;
+        const int K = sur_K(l), N = sur_N(l);
+        // dW_l = in^T cur (split over the batch), db_l = column sums of cur
+        const int tiles = (K / 64) * (N / 64);
+        int splits = std::max(1, std::min(B / 32, 256 / tiles));
+        EpiArgs e{};
+        e.C = s->d_part.p;
+        e.ldc = N;
+        e.cz = (int64_t)K * N;
+        int ksplit = ((B / splits + 31) / 32) * 32;
+        const int z = (B + ksplit - 1) / ksplit;
+        if (int r = gemm<true, false, kEpiStore>(st, in, K, cur_b, N, K, N, B, splits, e)) return r;
+        const int64_t kn = (int64_t)K * N;
+        hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((kn + 255) / 256)), dim3(256), 0, st, s->d_part.p, z, kn,
+                           s->d_g.p + sur_woff(l));
+        hipLaunchKernelGGL(k_colsum, dim3(N / 64), dim3(256), 0, st, cur_f, B, N, s->d_g.p + sur_boff(l));
+        HIPC(hipGetLastError());
+        if (l > 0) {
+            EpiArgs g{};
+            g.C = s->d_dZ[q].p;
+            g.ldc = K;
+            g.Cb = s->d_dZb[q].p;
+            g.ldcb = K;
+            g.Zin = s->d_Z[l - 1].p;
+            g.ldz = K;
+            if (int r = gemm<false, true, kEpiGeluBwd>(st, cur_b, N, s->d_pb.p + sur_woff(l), N, B, K, N, 1, g)) return r;
+            cur_f = s->d_dZ[q].p;
+            cur_b = s->d_dZb[q].p;
+            q ^= 1;
+        }
+    }
+    return 0;
+}
+
+// Forward (and loss) of nb rows of samples idx (device), padded to B rows.
+int sur_batch_loss(ebem_sur* s, hipStream_t st, const float* X, const float* Y, const int32_t* idx, int nb, int B,
+                   bool backward) {
+    hipLaunchKernelGGL(k_sur_gather, dim3((B + 3) / 4), dim3(64, 4), 0, st, X, Y, idx, nb, B, s->d_norm.p, s->d_A0.p,
+                       s->d_T.p);
+    HIPC(hipGetLastError());
+    if (int r = sur_forward(s, st, B, backward)) return r;
+    hipLaunchKernelGGL(k_sur_loss, dim3((unsigned)(((int64_t)B * kSurOutPad + 255) / 256)), dim3(256), 0, st, s->d_Yo.p,
+                       s->d_T.p, nb, B, backward ? s->d_dY.p : nullptr, backward ? s->d_dYb.p : nullptr,
+                       s->d_loss.p);
+    HIPC(hipGetLastError());
+    if (backward) return sur_backward(s, st, B);
+    return 0;
+}
+
+int sur_alloc_params(ebem_sur* s) {
+    HIPC(s->d_p.alloc(kSurParams));
+    HIPC(s->d_pb.alloc(kSurParams));
+    HIPC(s->d_g.alloc(kSurParams));
+    HIPC(s->d_m.alloc(kSurParams));
+    HIPC(s->d_v.alloc(kSurParams));
+    HIPC(s->d_norm.alloc(kSurNorm));
+    HIPC(s->d_part.alloc((size_t)256 * 64 * 64));  // largest splits * K * N of the weight gradients
+    return 0;
+}
+
+// Surrogate prediction of the items perm[0..n) of a chunk into their level
+// slots of W (k_sur_feats -> MLP -> k_sur_scatter), in batches.
+int sur_assemble_items(ebem_handle* h, ebem_sur* s, int r0, int64_t n, const int32_t* perm, const int32_t* item_pair,
+                       const int64_t* off, const int16_t* lo, const int16_t* hi, double* W) {
+    const int Bi = 65536;
+    if (int r = sur_workspace(s, Bi, false)) return r;
+    HIPC(s->d_slot.alloc(Bi));
+    hipStream_t st = h->stream;
+    for (int64_t k0 = 0; k0 < n; k0 += Bi) {
+        const int nb = (int)std::min<int64_t>(Bi, n - k0);
+        const int B = ((nb + 63) / 64) * 64;
+        hipLaunchKernelGGL(k_sur_feats, dim3((B + 255) / 256), dim3(256), 0, st, h->d_tri.p, h->d_verts.p, h->N, r0,
+                           (int64_t)nb, B, perm + k0, item_pair, off, lo, hi, h->dt, h->cP, h->cS, h->hmax,
+                           s->d_norm.p, s->d_A0.p, s->d_slot.p);
+        HIPC(hipGetLastError());
+        if (int r = sur_forward(s, st, B, false)) return r;
+        hipLaunchKernelGGL(k_sur_scatter, dim3((B + 255) / 256), dim3(256), 0, st, h->d_tri.p, h->N, r0, B,
+                           s->d_slot.p, item_pair, s->d_Yo.p, s->d_norm.p, h->rho, h->hmax, W);
+        HIPC(hipGetLastError());
+    }
+    return 0;
+}
+
+}  // namespace
+
 namespace {
 
 int check_handle(ebem_handle* h) {
@@ -320,6 +509,15 @@ int check_handle(ebem_handle* h) {
 int upload_geometry(ebem_handle* h) {
     std::vector<double> gh;
     build_gh(h->X.data(), h->T.data(), h->tri, h->nGH, gh);
+    std::vector<double> verts((size_t)h->N * 9);
+    h->hmax = 0;
+    for (int t = 0; t < h->N; t++) {
+        for (int v = 0; v < 3; v++)
+            for (int k = 0; k < 3; k++) verts[(size_t)t * 9 + 3 * v + k] = h->X[3 * (size_t)h->T[3 * t + v] + k];
+        h->hmax = std::max(h->hmax, h->tri[t].maxL);
+    }
+    HIPC(h->d_verts.alloc(verts.size()));
+    HIPC(hipMemcpyAsync(h->d_verts.p, verts.data(), sizeof(double) * verts.size(), hipMemcpyHostToDevice, h->stream));
     HIPC(h->d_tri.alloc(h->N));
     HIPC(hipMemcpyAsync(h->d_tri.p, h->tri.data(), sizeof(FieldTri) * h->N, hipMemcpyHostToDevice, h->stream));
     HIPC(h->d_gh.alloc(gh.size()));
@@ -370,8 +568,17 @@ void launch_levels_split(ebem_handle* h, AsmParams P, int r0, int64_t nitems, in
 
 int assemble(ebem_handle* h, int precision) {
     if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set before ebem_assemble");
-    if (precision != EBEM_PREC_FP64 && precision != EBEM_PREC_MIXED)
+    if (precision < 0 || precision > (EBEM_PREC_MIXED | EBEM_PREC_SURROGATE))
         return fail(EBEM_EINVAL, "unknown precision %d", precision);
+    const bool mixed = (precision & EBEM_PREC_MIXED) != 0;
+    ebem_sur* sur = (precision & EBEM_PREC_SURROGATE) ? h->sur : nullptr;
+    if ((precision & EBEM_PREC_SURROGATE) && !sur)
+        return fail(EBEM_EINVAL, "EBEM_PREC_SURROGATE needs a surrogate attached with ebem_attach_surrogate");
+    std::unique_lock<std::mutex> slk;
+    if (sur) {
+        slk = std::unique_lock<std::mutex>(sur->lock);
+        if (!sur->ready) return fail(EBEM_EINVAL, "the attached surrogate has no parameters (init, train or set them)");
+    }
     compute_L(h);
     const int N = h->N, L = h->L;
     if (L > 4096) return fail(EBEM_EINVAL, "too many lag blocks (%d > 4096)", L);
@@ -387,23 +594,39 @@ int assemble(ebem_handle* h, int precision) {
         // far field: centroid distance > 3h, h = longest edge of the mesh
         double hmax = 0;
         for (int t = 0; t < N; t++) hmax = std::max(hmax, h->tri[t].maxL);
-        P.far2 = (precision == EBEM_PREC_MIXED) ? (3.0 * hmax) * (3.0 * hmax) : -1.0;
+        P.far2 = mixed ? (3.0 * hmax) * (3.0 * hmax) : -1.0;
     }
+    const double near2 = (3.0 * h->hmax) * (3.0 * h->hmax);  // surrogate pairs: centroid distance <= 3h
     Timer tall, tlev;
     double lev_ms = 0;
     tall.start(h->stream);
     const int64_t np = (int64_t)N * N;
     h->far_items = 0;
+    h->sur_items = 0;
+    h->sur_pairs_blocks = 0;
+    h->sur_ms = 0;
     h->precision = precision;
+    DevBuf<uint8_t> d_flag;
+    DevBuf<unsigned long long> d_nsb;
+    if (sur) {
+        HIPC(d_nsb.alloc(1));
+        HIPC(hipMemsetAsync(d_nsb.p, 0, sizeof(unsigned long long), h->stream));
+    }
+    Timer tsur;
     DevBuf<int16_t> d_lo, d_hi;
     HIPC(d_lo.alloc(np));
     HIPC(d_hi.alloc(np));
+    DevBuf<unsigned long long> d_culled;
+    HIPC(d_culled.alloc(1));
+    HIPC(hipMemsetAsync(d_culled.p, 0, sizeof(unsigned long long), h->stream));
     {
         int64_t blocks = std::min<int64_t>((np + 255) / 256, (int64_t)h->CUs * 64);
-        hipLaunchKernelGGL(k_windows, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, P, 0, N, d_lo.p,
-                           d_hi.p);
+        hipLaunchKernelGGL(k_windows, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p,
+                           h->d_verts.p, P, 0, N, d_lo.p, d_hi.p, d_culled.p);
         HIPC(hipGetLastError());
     }
+    unsigned long long culled = 0;
+    HIPC(hipMemcpyAsync(&culled, d_culled.p, sizeof culled, hipMemcpyDeviceToHost, h->stream));
     DevBuf<int64_t> d_cnt, d_items;
     HIPC(d_cnt.alloc((size_t)N * L));
     HIPC(d_items.alloc(N));
@@ -417,7 +640,7 @@ int assemble(ebem_handle* h, int precision) {
     h->seg.assign((size_t)N * L + 1, 0);
     for (size_t k = 0; k < cnt.size(); k++) h->seg[k + 1] = h->seg[k] + cnt[k];
     h->nnzb = h->seg.back();
-    h->pair_lags = h->nnzb;
+    h->pair_lags = (int64_t)culled;  // the reference's culled pair.lags; nnzb leaves out exact zeros
     h->items = 0;
     for (int i = 0; i < N; i++) h->items += rowitems[i];
     HIPC(h->d_seg.alloc(h->seg.size()));
@@ -473,11 +696,21 @@ int assemble(ebem_handle* h, int precision) {
         const int16_t* hi = d_hi.p + (int64_t)r0 * N;
         hipLaunchKernelGGL(k_pair_items, dim3(rows), dim3(256), 0, h->stream, lo, hi, N, d_rowbase.p, d_off.p, d_ip.p);
         HIPC(hipGetLastError());
+        const uint8_t* sflag = nullptr;
+        if (sur) {
+            HIPC(d_flag.alloc((size_t)rows * N));
+            const int64_t npr = (int64_t)rows * N;
+            hipLaunchKernelGGL(k_sur_flag, dim3((unsigned)((npr + 255) / 256)), dim3(256), 0, h->stream, h->d_tri.p,
+                               h->d_verts.p, N, r0, rows, lo, hi, h->dt, h->cP, h->cS, h->hmax, near2,
+                               sur->d_norm.p + 2 * kSurIn + 2 * kSurOut, d_flag.p, d_nsb.p);
+            HIPC(hipGetLastError());
+            sflag = d_flag.p;
+        }
         if (acc > 0) {
             // bucket items by front-state key (counting sort) for convergent wavefronts
             const int64_t per = (acc + G - 1) / G;
             hipLaunchKernelGGL(k_item_keys, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, P, r0, acc, per, d_ip.p,
-                               d_off.p, lo, d_key.p, d_hist.p);
+                               d_off.p, lo, sflag, d_key.p, d_hist.p);
             HIPC(hipGetLastError());
             HIPC(hipMemcpyAsync(hist.data(), d_hist.p, sizeof(unsigned int) * hist.size(), hipMemcpyDeviceToHost,
                                 h->stream));
@@ -492,29 +725,45 @@ int assemble(ebem_handle* h, int precision) {
             hipLaunchKernelGGL(k_item_scatter, dim3(G), dim3(256), 0, h->stream, acc, per, d_key.p, d_cur.p,
                                d_perm.p);
             HIPC(hipGetLastError());
-            const int32_t* perm = (h->sort_items || precision == EBEM_PREC_MIXED) ? d_perm.p : nullptr;
-            int64_t nfar = 0;
+            const int32_t* perm = (h->sort_items || mixed || sur) ? d_perm.p : nullptr;
+            int64_t nfar = 0, nsur = 0;
             for (size_t k = 0; k < (size_t)kFrontKeys * G; k++) nfar += hist[k];
+            for (int b = 0; b < G; b++) nsur += hist[(size_t)kSurKey * G + b];
             h->far_items += nfar;
+            h->sur_items += nsur;
+            const int64_t nlev = acc - nsur;  // the surrogate items are the suffix of perm
             tlev.start(h->stream);
             switch (h->nGH) {
-                case 3: launch_levels_split<3>(h, P, r0, acc, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
-                case 7: launch_levels_split<7>(h, P, r0, acc, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
-                case 12: launch_levels_split<12>(h, P, r0, acc, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
-                default: launch_levels_split<19>(h, P, r0, acc, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
+                case 3: launch_levels_split<3>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
+                case 7: launch_levels_split<7>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
+                case 12: launch_levels_split<12>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
+                default: launch_levels_split<19>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
             }
             HIPC(hipGetLastError());
             tlev.stop(h->stream);
             lev_ms += tlev.ms();
+            if (nsur > 0) {
+                tsur.start(h->stream);
+                if (int r = sur_assemble_items(h, sur, r0, nsur, d_perm.p + nlev, d_ip.p, d_off.p, lo, hi, d_W.p))
+                    return r;
+                tsur.stop(h->stream);
+                h->sur_ms += tsur.ms();
+            }
         }
-        hipLaunchKernelGGL(k_emit, dim3(rows), dim3(64), L * sizeof(int64_t), h->stream, lo, hi, d_off.p, d_W.p, N, L,
-                           r0, inv4pirho, S);
+        hipLaunchKernelGGL(k_emit, dim3(rows), dim3(64), L * sizeof(int64_t), h->stream, lo, hi, d_off.p, d_W.p, sflag,
+                           N, L, r0, inv4pirho, S);
         HIPC(hipGetLastError());
         r0 = r1;
     }
     tall.stop(h->stream);
     h->tim[1] = tall.ms();
     h->tim[0] = lev_ms;
+    if (sur) {
+        unsigned long long nb = 0;
+        HIPC(hipMemcpyAsync(&nb, d_nsb.p, sizeof nb, hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        h->sur_pairs_blocks = (int64_t)nb;
+    }
     HIPC(hipStreamSynchronize(h->stream));
     h->assembled = true;
     h->store_L = L;
@@ -803,6 +1052,11 @@ int ebem_destroy(ebem_handle* h) {
         (void)hipSetDevice(h->device);
         (void)hipStreamSynchronize(h->stream);
         if (h->blas) rocblas_destroy_handle(h->blas);
+        if (h->sur) {
+            std::lock_guard<std::mutex> lk2(h->sur->lock);
+            h->sur->users--;
+            h->sur = nullptr;
+        }
         h->d_tri.reset();
         h->d_gh.reset();
         h->d_seg.reset();
@@ -903,12 +1157,24 @@ int ebem_store_info(ebem_handle* h, int64_t* nnzb, int64_t* pair_lags, int64_t* 
     return 0;
 }
 
+int ebem_store_lag_blocks(ebem_handle* h, int64_t* counts) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
+    if (!counts) return fail(EBEM_EINVAL, "null output");
+    const int L = h->store_L;
+    for (int l = 0; l < L; l++) counts[l] = 0;
+    for (int i = 0; i < h->N; i++)
+        for (int l = 0; l < L; l++) counts[l] += h->seg[(size_t)i * L + l + 1] - h->seg[(size_t)i * L + l];
+    return 0;
+}
+
 int ebem_assembly_stats(ebem_handle* h, int64_t* far_items, int64_t* fp64_items) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
     if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
     if (far_items) *far_items = h->far_items;
-    if (fp64_items) *fp64_items = h->items - h->far_items;
+    if (fp64_items) *fp64_items = h->items - h->far_items - h->sur_items;
     return 0;
 }
 
@@ -995,6 +1261,386 @@ int ebem_timings(ebem_handle* h, double* out8) {
     std::lock_guard<std::mutex> lk(h->lock);
     if (!out8) return fail(EBEM_EINVAL, "null output");
     for (int k = 0; k < 8; k++) out8[k] = h->tim[k];
+    return 0;
+}
+
+
+// ------------------------------------------------------------ surrogate --
+
+int ebem_sur_create(ebem_sur** out, int device) {
+    if (!out) return fail(EBEM_EINVAL, "null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    HIPC(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(EBEM_EINVAL, "device %d not available (%d devices)", device, ndev);
+    HIPC(hipSetDevice(device));
+    ebem_sur* s = new ebem_sur();
+    s->device = device;
+    hipError_t e = hipStreamCreate(&s->stream);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(EBEM_EHIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    if (int r = sur_alloc_params(s)) {
+        (void)hipStreamDestroy(s->stream);
+        delete s;
+        return r;
+    }
+    s->norm.assign(kSurNorm, 0.f);
+    *out = s;
+    return 0;
+}
+
+int ebem_sur_destroy(ebem_sur* s) {
+    if (!s) return 0;
+    {
+        std::lock_guard<std::mutex> lk(s->lock);
+        if (s->users > 0) return fail(EBEM_EINVAL, "surrogate still attached to %d handle(s)", s->users);
+        (void)hipSetDevice(s->device);
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipStreamDestroy(s->stream);
+    }
+    delete s;
+    return 0;
+}
+
+int ebem_sur_add_samples(ebem_sur* s, ebem_handle* h, int64_t n, const int32_t* test, const int32_t* field,
+                         const int32_t* lag) {
+    if (!s || !h) return fail(EBEM_EINVAL, "null surrogate or handle");
+    std::lock_guard<std::mutex> lk(h->lock);
+    std::lock_guard<std::mutex> lk2(s->lock);
+    if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
+    if (h->device != s->device) return fail(EBEM_EINVAL, "surrogate and handle live on different devices");
+    if (n < 0 || (n > 0 && (!test || !field || !lag))) return fail(EBEM_EINVAL, "bad sample arrays");
+    for (int64_t k = 0; k < n; k++)
+        if (test[k] < 0 || test[k] >= h->N || field[k] < 0 || field[k] >= h->N || lag[k] < 0 || lag[k] > 4096)
+            return fail(EBEM_EINVAL, "sample %lld out of range (test %d, field %d, lag %d)", (long long)k, test[k],
+                        field[k], lag[k]);
+    if (n == 0) return 0;
+    HIPC(hipSetDevice(h->device));
+    if (s->n + n > s->cap) {
+        int64_t cap = std::max<int64_t>(s->n + n, 2 * s->cap);
+        DevBuf<float> X, Y;
+        HIPC(X.alloc((size_t)cap * kSurIn));
+        HIPC(Y.alloc((size_t)cap * kSurOut));
+        if (s->n) {
+            HIPC(hipMemcpyAsync(X.p, s->d_X.p, sizeof(float) * s->n * kSurIn, hipMemcpyDeviceToDevice, h->stream));
+            HIPC(hipMemcpyAsync(Y.p, s->d_Y.p, sizeof(float) * s->n * kSurOut, hipMemcpyDeviceToDevice, h->stream));
+        }
+        HIPC(hipStreamSynchronize(h->stream));
+        std::swap(s->d_X.p, X.p);
+        std::swap(s->d_X.n, X.n);
+        std::swap(s->d_Y.p, Y.p);
+        std::swap(s->d_Y.n, Y.n);
+        s->cap = cap;
+    }
+    DevBuf<int32_t> idx;
+    HIPC(idx.alloc((size_t)3 * n));
+    HIPC(hipMemcpyAsync(idx.p, test, sizeof(int32_t) * n, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(idx.p + n, field, sizeof(int32_t) * n, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(idx.p + 2 * n, lag, sizeof(int32_t) * n, hipMemcpyHostToDevice, h->stream));
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    float* X = s->d_X.p + s->n * kSurIn;
+    float* Y = s->d_Y.p + s->n * kSurOut;
+#define SUR_SAMPLES(NG)                                                                                               \
+    hipLaunchKernelGGL(k_sur_samples<NG>, dim3(blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, h->d_verts.p, \
+                       n, idx.p, idx.p + n, idx.p + 2 * n, h->dt, h->cP, h->cS, h->rho, h->hmax, X, Y)
+    switch (h->nGH) {
+        case 3: SUR_SAMPLES(3); break;
+        case 7: SUR_SAMPLES(7); break;
+        case 12: SUR_SAMPLES(12); break;
+        default: SUR_SAMPLES(19); break;
+    }
+#undef SUR_SAMPLES
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(h->stream));
+    s->n += n;
+    return 0;
+}
+
+int ebem_sur_num_samples(ebem_sur* s, int64_t* n) {
+    if (!s || !n) return fail(EBEM_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->lock);
+    *n = s->n;
+    return 0;
+}
+
+int ebem_sur_get_samples(ebem_sur* s, int64_t first, int64_t n, float* x, float* y) {
+    if (!s) return fail(EBEM_EINVAL, "null surrogate");
+    std::lock_guard<std::mutex> lk(s->lock);
+    if (first < 0 || n < 0 || first + n > s->n) return fail(EBEM_EINVAL, "sample range out of bounds");
+    HIPC(hipSetDevice(s->device));
+    if (x) HIPC(hipMemcpyAsync(x, s->d_X.p + first * kSurIn, sizeof(float) * n * kSurIn, hipMemcpyDeviceToHost, s->stream));
+    if (y) HIPC(hipMemcpyAsync(y, s->d_Y.p + first * kSurOut, sizeof(float) * n * kSurOut, hipMemcpyDeviceToHost, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int ebem_sur_init(ebem_sur* s, int64_t ntrain, uint64_t seed) {
+    if (!s) return fail(EBEM_EINVAL, "null surrogate");
+    std::lock_guard<std::mutex> lk(s->lock);
+    if (ntrain <= 1 || ntrain > s->n) return fail(EBEM_EINVAL, "ntrain %lld outside [2, %lld]", (long long)ntrain, (long long)s->n);
+    HIPC(hipSetDevice(s->device));
+    std::vector<float> X((size_t)ntrain * kSurIn), Y((size_t)ntrain * kSurOut);
+    HIPC(hipMemcpyAsync(X.data(), s->d_X.p, sizeof(float) * X.size(), hipMemcpyDeviceToHost, s->stream));
+    HIPC(hipMemcpyAsync(Y.data(), s->d_Y.p, sizeof(float) * Y.size(), hipMemcpyDeviceToHost, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
+    std::vector<float>& nm = s->norm;
+    nm.assign(kSurNorm, 0.f);
+    float* box = nm.data() + 2 * kSurIn + 2 * kSurOut;
+    for (int c = 0; c < kSurIn; c++) {
+        double m = 0, q = 0;
+        float lo = X[c], hi = X[c];
+        for (int64_t k = 0; k < ntrain; k++) {
+            const double v = X[(size_t)k * kSurIn + c];
+            m += v;
+            q += v * v;
+            lo = std::min(lo, X[(size_t)k * kSurIn + c]);
+            hi = std::max(hi, X[(size_t)k * kSurIn + c]);
+        }
+        m /= (double)ntrain;
+        const double sd = std::sqrt(std::max(0.0, q / (double)ntrain - m * m));
+        nm[c] = (float)m;
+        nm[kSurIn + c] = sd > 1e-6 ? (float)(1.0 / sd) : 1.0f;
+        // a small margin: features of congruent meshes agree to rounding only
+        const float pad = 1e-4f * (hi - lo) + 1e-6f * std::max(std::fabs(lo), std::fabs(hi));
+        box[c] = lo - pad;
+        box[kSurIn + c] = hi + pad;
+    }
+    for (int c = 0; c < kSurOut; c++) {
+        double m = 0, q = 0;
+        for (int64_t k = 0; k < ntrain; k++) {
+            const double v = Y[(size_t)k * kSurOut + c];
+            m += v;
+            q += v * v;
+        }
+        m /= (double)ntrain;
+        const double sd = std::sqrt(std::max(0.0, q / (double)ntrain - m * m));
+        nm[2 * kSurIn + c] = (float)m;
+        nm[2 * kSurIn + kSurOut + c] = sd > 1e-12 ? (float)sd : 1.0f;
+    }
+    // He-normal weights over the real fan-in, zero biases and zero padding
+    std::vector<float> p(kSurParams, 0.f);
+    uint64_t st = seed ^ 0x9e3779b97f4a7c15ull;
+    auto next = [&]() {  // splitmix64 -> uniform (0, 1)
+        uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        return ((double)(z >> 11) + 0.5) / 9007199254740992.0;
+    };
+    for (int l = 0; l < kSurLayers; l++) {
+        const int K = sur_K(l), N = sur_N(l);
+        const int kin = l == 0 ? kSurIn : K, nout = l == kSurLayers - 1 ? kSurOut : N;
+        const double sd = (l == kSurLayers - 1) ? std::sqrt(1.0 / kin) : std::sqrt(2.0 / kin);
+        for (int a = 0; a < kin; a++)
+            for (int b = 0; b < nout; b++) {
+                const double u1 = next(), u2 = next();
+                p[sur_woff(l) + (int64_t)a * N + b] =
+                    (float)(sd * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kPi * u2));
+            }
+    }
+    HIPC(hipMemcpyAsync(s->d_p.p, p.data(), sizeof(float) * kSurParams, hipMemcpyHostToDevice, s->stream));
+    HIPC(hipMemcpyAsync(s->d_norm.p, nm.data(), sizeof(float) * kSurNorm, hipMemcpyHostToDevice, s->stream));
+    HIPC(hipMemsetAsync(s->d_m.p, 0, sizeof(float) * kSurParams, s->stream));
+    HIPC(hipMemsetAsync(s->d_v.p, 0, sizeof(float) * kSurParams, s->stream));
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((kSurParams + 255) / 256)), dim3(256), 0, s->stream, s->d_p.p,
+                       s->d_pb.p, kSurParams);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(s->stream));
+    s->adam_t = 0;
+    s->ready = true;
+    return 0;
+}
+
+int ebem_sur_train(ebem_sur* s, int64_t ntrain, int epochs, int batch, double lr, uint64_t seed, double* loss_out) {
+    if (!s) return fail(EBEM_EINVAL, "null surrogate");
+    std::lock_guard<std::mutex> lk(s->lock);
+    if (!s->ready) return fail(EBEM_EINVAL, "surrogate parameters not initialised (ebem_sur_init)");
+    if (ntrain <= 0 || ntrain > s->n || epochs <= 0 || batch < 64 || batch % 64 || batch > 65536 || !(lr > 0))
+        return fail(EBEM_EINVAL, "bad training arguments");
+    HIPC(hipSetDevice(s->device));
+    if (int r = sur_workspace(s, batch, true)) return r;
+    const int64_t steps_per = (ntrain + batch - 1) / batch;
+    const int64_t total = steps_per * epochs;
+    std::vector<int32_t> perm(ntrain);
+    DevBuf<int32_t> d_perm;
+    HIPC(d_perm.alloc(ntrain));
+    uint64_t st = seed;
+    auto next = [&]() {
+        uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    };
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    int64_t step = 0;
+    for (int ep = 0; ep < epochs; ep++) {
+        for (int64_t k = 0; k < ntrain; k++) perm[k] = (int32_t)k;
+        for (int64_t k = ntrain - 1; k > 0; k--) std::swap(perm[k], perm[next() % (uint64_t)(k + 1)]);
+        HIPC(hipMemcpyAsync(d_perm.p, perm.data(), sizeof(int32_t) * ntrain, hipMemcpyHostToDevice, s->stream));
+        HIPC(hipMemsetAsync(s->d_loss.p, 0, sizeof(double), s->stream));
+        for (int64_t b0 = 0; b0 < ntrain; b0 += batch, step++) {
+            const int nb = (int)std::min<int64_t>(batch, ntrain - b0);
+            if (int r = sur_batch_loss(s, s->stream, s->d_X.p, s->d_Y.p, d_perm.p + b0, nb, batch, true)) return r;
+            // cosine decay from lr to lr / 100
+            const double frac = (double)step / (double)std::max<int64_t>(1, total - 1);
+            const double lrt = lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(kPi * frac)));
+            s->adam_t++;
+            const float bc1 = (float)(1.0 - std::pow((double)b1, (double)s->adam_t));
+            const float bc2 = (float)(1.0 - std::pow((double)b2, (double)s->adam_t));
+            hipLaunchKernelGGL(k_adam, dim3((unsigned)((kSurParams + 255) / 256)), dim3(256), 0, s->stream, s->d_p.p,
+                               s->d_g.p, s->d_m.p, s->d_v.p, s->d_pb.p, kSurParams, (float)lrt, b1, b2, eps, bc1, bc2);
+            HIPC(hipGetLastError());
+        }
+        double l = 0;
+        HIPC(hipMemcpyAsync(&l, s->d_loss.p, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIPC(hipStreamSynchronize(s->stream));
+        // the loss kernel averages per batch; report the mean over the epoch's batches
+        if (loss_out) loss_out[ep] = l / (double)steps_per;
+    }
+    return 0;
+}
+
+int ebem_sur_loss(ebem_sur* s, int64_t first, int64_t n, double* loss) {
+    if (!s || !loss) return fail(EBEM_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->lock);
+    if (!s->ready) return fail(EBEM_EINVAL, "surrogate parameters not set");
+    if (first < 0 || n <= 0 || first + n > s->n) return fail(EBEM_EINVAL, "sample range out of bounds");
+    HIPC(hipSetDevice(s->device));
+    const int B = 16384;
+    if (int r = sur_workspace(s, B, false)) return r;
+    std::vector<int32_t> idx(B);
+    DevBuf<int32_t> d_idx;
+    HIPC(d_idx.alloc(B));
+    double tot = 0;
+    for (int64_t k0 = 0; k0 < n; k0 += B) {
+        const int nb = (int)std::min<int64_t>(B, n - k0);
+        for (int k = 0; k < nb; k++) idx[k] = (int32_t)(first + k0 + k);
+        HIPC(hipMemcpyAsync(d_idx.p, idx.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, s->stream));
+        HIPC(hipMemsetAsync(s->d_loss.p, 0, sizeof(double), s->stream));
+        if (int r = sur_batch_loss(s, s->stream, s->d_X.p, s->d_Y.p, d_idx.p, nb, ((nb + 63) / 64) * 64, false)) return r;
+        double l = 0;
+        HIPC(hipMemcpyAsync(&l, s->d_loss.p, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIPC(hipStreamSynchronize(s->stream));
+        tot += l * (double)nb;
+    }
+    *loss = tot / (double)n;
+    return 0;
+}
+
+int ebem_sur_grad(ebem_sur* s, int n, const float* x, const float* y, float* grads, double* loss) {
+    if (!s || !x || !y) return fail(EBEM_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->lock);
+    if (!s->ready) return fail(EBEM_EINVAL, "surrogate parameters not set");
+    if (n <= 0 || n > 65536) return fail(EBEM_EINVAL, "n must be in [1, 65536]");
+    HIPC(hipSetDevice(s->device));
+    const int B = ((n + 63) / 64) * 64;
+    if (int r = sur_workspace(s, B, true)) return r;
+    DevBuf<float> dx, dy;
+    DevBuf<int32_t> di;
+    HIPC(dx.alloc((size_t)n * kSurIn));
+    HIPC(dy.alloc((size_t)n * kSurOut));
+    HIPC(di.alloc(n));
+    std::vector<int32_t> idx(n);
+    for (int k = 0; k < n; k++) idx[k] = k;
+    HIPC(hipMemcpyAsync(dx.p, x, sizeof(float) * n * kSurIn, hipMemcpyHostToDevice, s->stream));
+    HIPC(hipMemcpyAsync(dy.p, y, sizeof(float) * n * kSurOut, hipMemcpyHostToDevice, s->stream));
+    HIPC(hipMemcpyAsync(di.p, idx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream));
+    HIPC(hipMemsetAsync(s->d_loss.p, 0, sizeof(double), s->stream));
+    if (int r = sur_batch_loss(s, s->stream, dx.p, dy.p, di.p, n, B, true)) return r;
+    if (grads) HIPC(hipMemcpyAsync(grads, s->d_g.p, sizeof(float) * kSurParams, hipMemcpyDeviceToHost, s->stream));
+    double l = 0;
+    HIPC(hipMemcpyAsync(&l, s->d_loss.p, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
+    if (loss) *loss = l;
+    return 0;
+}
+
+int ebem_sur_predict(ebem_sur* s, int64_t n, const float* x, float* y) {
+    if (!s || (n > 0 && (!x || !y))) return fail(EBEM_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->lock);
+    if (!s->ready) return fail(EBEM_EINVAL, "surrogate parameters not set");
+    if (n < 0) return fail(EBEM_EINVAL, "negative count");
+    HIPC(hipSetDevice(s->device));
+    const int B = 16384;
+    if (int r = sur_workspace(s, B, false)) return r;
+    DevBuf<float> dx, dyz;
+    DevBuf<int32_t> di;
+    HIPC(dx.alloc((size_t)B * kSurIn));
+    HIPC(dyz.alloc((size_t)B * kSurOut));
+    HIPC(di.alloc(B));
+    std::vector<int32_t> idx(B);
+    for (int k = 0; k < B; k++) idx[k] = k;
+    HIPC(hipMemcpyAsync(di.p, idx.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, s->stream));
+    HIPC(hipMemsetAsync(dyz.p, 0, sizeof(float) * B * kSurOut, s->stream));
+    std::vector<float> Yo((size_t)B * kSurOutPad);
+    const float* ym = s->norm.data() + 2 * kSurIn;
+    const float* ys = ym + kSurOut;
+    for (int64_t k0 = 0; k0 < n; k0 += B) {
+        const int nb = (int)std::min<int64_t>(B, n - k0);
+        const int Bp = ((nb + 63) / 64) * 64;
+        HIPC(hipMemcpyAsync(dx.p, x + k0 * kSurIn, sizeof(float) * nb * kSurIn, hipMemcpyHostToDevice, s->stream));
+        if (int r = sur_batch_loss(s, s->stream, dx.p, dyz.p, di.p, nb, Bp, false)) return r;
+        HIPC(hipMemcpyAsync(Yo.data(), s->d_Yo.p, sizeof(float) * (size_t)Bp * kSurOutPad, hipMemcpyDeviceToHost,
+                            s->stream));
+        HIPC(hipStreamSynchronize(s->stream));
+        for (int k = 0; k < nb; k++)
+            for (int c = 0; c < kSurOut; c++) y[(k0 + k) * kSurOut + c] = Yo[(size_t)k * kSurOutPad + c] * ys[c] + ym[c];
+    }
+    return 0;
+}
+
+int ebem_sur_params(ebem_sur* s, int set, float* params, float* norm) {
+    if (!s) return fail(EBEM_EINVAL, "null surrogate");
+    std::lock_guard<std::mutex> lk(s->lock);
+    HIPC(hipSetDevice(s->device));
+    if (set) {
+        if (!params || !norm) return fail(EBEM_EINVAL, "set needs params and norm");
+        s->norm.assign(norm, norm + kSurNorm);
+        HIPC(hipMemcpyAsync(s->d_p.p, params, sizeof(float) * kSurParams, hipMemcpyHostToDevice, s->stream));
+        HIPC(hipMemcpyAsync(s->d_norm.p, norm, sizeof(float) * kSurNorm, hipMemcpyHostToDevice, s->stream));
+        HIPC(hipMemsetAsync(s->d_m.p, 0, sizeof(float) * kSurParams, s->stream));
+        HIPC(hipMemsetAsync(s->d_v.p, 0, sizeof(float) * kSurParams, s->stream));
+        hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((kSurParams + 255) / 256)), dim3(256), 0, s->stream, s->d_p.p,
+                           s->d_pb.p, kSurParams);
+        HIPC(hipGetLastError());
+        HIPC(hipStreamSynchronize(s->stream));
+        s->adam_t = 0;
+        s->ready = true;
+        return 0;
+    }
+    if (!s->ready) return fail(EBEM_EINVAL, "surrogate parameters not set");
+    if (params)
+        HIPC(hipMemcpyAsync(params, s->d_p.p, sizeof(float) * kSurParams, hipMemcpyDeviceToHost, s->stream));
+    if (norm) std::memcpy(norm, s->norm.data(), sizeof(float) * kSurNorm);
+    HIPC(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int ebem_attach_surrogate(ebem_handle* h, ebem_sur* s) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (s && s->device != h->device) return fail(EBEM_EINVAL, "surrogate and handle live on different devices");
+    if (h->sur) {
+        std::lock_guard<std::mutex> lk2(h->sur->lock);
+        h->sur->users--;
+    }
+    h->sur = s;
+    if (s) {
+        std::lock_guard<std::mutex> lk2(s->lock);
+        s->users++;
+    }
+    return 0;
+}
+
+int ebem_surrogate_stats(ebem_handle* h, int64_t* items, int64_t* blocks, double* ms) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
+    if (items) *items = h->sur_items;
+    if (blocks) *blocks = h->sur_pairs_blocks;
+    if (ms) *ms = h->sur_ms;
     return 0;
 }
 

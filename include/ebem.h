@@ -40,6 +40,7 @@ extern "C" {
 /* Integration precision of ebem_assemble. */
 #define EBEM_PREC_FP64 0   /* plain form: everything FP64 */
 #define EBEM_PREC_MIXED 1  /* far-field pairs (centroid distance > 3h) integrated in FP32 */
+#define EBEM_PREC_SURROGATE 2 /* flag: near-field pairs predicted by the attached surrogate MLP */
 
 /* Dirichlet data of ebem_rhs (Tn/BEMenerg_core_calcNucleoTn.m:133-183). */
 #define EBEM_DATUM_SCREEN_SIN2 0    /* (0,0,sin^2(pi t/2) H(t)) */
@@ -48,6 +49,7 @@ extern "C" {
 #define EBEM_DATUM_REF_SCREEN 3     /* reference 'screenUniform': (0,0,[sin^5 t | 1] x1^2) */
 
 typedef struct ebem_handle ebem_handle;
+typedef struct ebem_sur ebem_sur;
 
 /* Thread-local message of the last failing call on this thread. */
 const char *ebem_last_error(void);
@@ -79,9 +81,16 @@ int ebem_num_blocks(ebem_handle *h, int *L);
  * dir/BEMenerg_dir_timeMarchingD.m:97-161.  precision: EBEM_PREC_*. */
 int ebem_assemble(ebem_handle *h, int precision);
 
-/* Block-store statistics: non-zero 3x3 blocks (all lags), culled pair.lags,
- * (pair, level) integral items evaluated, bytes of the store. */
+/* Block-store statistics: stored 3x3 blocks (all lags), the pair.lags that
+ * pass the reference's light-cone test (calcMatrixBlock.m:57-61), (pair,
+ * level) integral items evaluated, bytes of the store.  The store leaves out
+ * the lags of a pair whose block is zero in exact arithmetic (the field
+ * triangle inside every outer node's S disk at the lag's earliest level, or
+ * out of reach of the P front at its latest), so nnzb < pair_lags. */
 int ebem_store_info(ebem_handle *h, int64_t *nnzb, int64_t *pair_lags, int64_t *items, int64_t *bytes);
+
+/* Stored blocks per lag: counts[0..L-1] (L of ebem_num_blocks). */
+int ebem_store_lag_blocks(ebem_handle *h, int64_t *counts);
 
 /* Items of the last ebem_assemble: (pair, level) integral items evaluated in
  * FP32 (far field, EBEM_PREC_MIXED) and in FP64.  No reference counterpart
@@ -122,6 +131,68 @@ int ebem_node_integral(ebem_handle *h, const double *x, int tri, double t, doubl
  * out[3] history-sum kernel total, out[4] E_0 apply total, out[5] MOT total,
  * out[6] post-processing total, out[7] history-sum launches. */
 int ebem_timings(ebem_handle *h, double *out8);
+
+/* ------------------------------------------------------------------------
+ * Learned near-field surrogate.  No reference counterpart: the reference
+ * evaluates every pair analytically (functions/BEMenerg_analiticIntg/ via
+ * SAc/BEMenerg_coreSA_calcMatrixSubBlock.m).  A surrogate is an MLP
+ * 24 -> 256 -> 256 -> 256 -> 256 -> 18 (GELU, BF16 weights and activations,
+ * FP32 accumulation, FP32 master weights) that predicts, for a test/field
+ * pair and a lag, the P-front and S-front parts of the 3x3 block E_lag in the
+ * field triangle's frame, divided by A_S A_F / (4 pi rho h).
+ * Features (all lengths / h, h = the mesh's longest edge, frame of the field
+ * triangle: origin at its centroid, e1 along its first edge, e3 its normal):
+ * the test triangle's vertices (9), the field triangle's vertices (9),
+ * dt c_P / h, dt c_S / h, lag, (lag dt c_P - d) / h and (lag dt c_S - d) / h
+ * (d the centroid distance), cosine between the normals.
+ * Labels come from the library's FP64 analytic routine; training, inference
+ * and gradients run on the device with hand-written BF16 MFMA kernels.
+ * ------------------------------------------------------------------------ */
+
+/* Number of FP32 parameters (padded layout: per layer W (K x N row-major)
+ * then b (N), K = 64, 256, 256, 256, 256 and N = 256, 256, 256, 256, 64;
+ * rows/columns beyond 24 inputs and 18 outputs stay zero) and of the
+ * normalisation block (x mean 24, x 1/std 24, y mean 18, y std 18,
+ * training box lo 24, hi 24). */
+#define EBEM_SUR_PARAMS 230464
+#define EBEM_SUR_NORM 132
+
+int ebem_sur_create(ebem_sur **s, int device);
+/* Fails with EBEM_EINVAL while the surrogate is attached to a handle. */
+int ebem_sur_destroy(ebem_sur *s);
+
+/* Append n samples (test triangle, field triangle, lag) of h's mesh and
+ * problem: features and FP64 analytic labels computed on the device. */
+int ebem_sur_add_samples(ebem_sur *s, ebem_handle *h, int64_t n, const int32_t *test, const int32_t *field,
+                         const int32_t *lag);
+int ebem_sur_num_samples(ebem_sur *s, int64_t *n);
+/* Copy samples [first, first+n): x n x 24, y n x 18 (either may be NULL). */
+int ebem_sur_get_samples(ebem_sur *s, int64_t first, int64_t n, float *x, float *y);
+
+/* Normalisation and training box from samples [0, ntrain), He-normal weights. */
+int ebem_sur_init(ebem_sur *s, int64_t ntrain, uint64_t seed);
+/* Adam (0.9, 0.999), cosine learning-rate decay lr -> lr/100, MSE on
+ * standardised labels over samples [0, ntrain), shuffled per epoch (seed);
+ * batch a multiple of 64.  loss_out[epochs]: mean training loss per epoch. */
+int ebem_sur_train(ebem_sur *s, int64_t ntrain, int epochs, int batch, double lr, uint64_t seed, double *loss_out);
+/* Mean standardised MSE over samples [first, first+n). */
+int ebem_sur_loss(ebem_sur *s, int64_t first, int64_t n, double *loss);
+/* One forward + backward pass on n <= 65536 raw samples (x n x 24, y n x 18):
+ * gradient of the MSE w.r.t. every parameter (EBEM_SUR_PARAMS floats). */
+int ebem_sur_grad(ebem_sur *s, int n, const float *x, const float *y, float *grads, double *loss);
+/* Predicted labels (n x 18, unstandardised) of raw features x (n x 24). */
+int ebem_sur_predict(ebem_sur *s, int64_t n, const float *x, float *y);
+/* set = 0: copy parameters / normalisation out (either may be NULL);
+ * set = 1: load both (resets Adam). */
+int ebem_sur_params(ebem_sur *s, int set, float *params, float *norm);
+
+/* Attach (s) or detach (NULL) the surrogate used by ebem_assemble with
+ * EBEM_PREC_SURROGATE: pairs i != j with centroid distance <= 3h whose features
+ * over their whole lag window lie inside the training box are predicted; all
+ * other pairs fall back to the analytic routine. */
+int ebem_attach_surrogate(ebem_handle *h, ebem_sur *s);
+/* Last assembly: surrogate items, predicted blocks, milliseconds. */
+int ebem_surrogate_stats(ebem_handle *h, int64_t *items, int64_t *blocks, double *ms);
 
 #ifdef __cplusplus
 }

@@ -75,3 +75,27 @@ def assemble(o, lib, L, rows=None, lags=None, fp32_far=None):
                     blk[r, c] = blk[c, r] = e[q]
                 E[l][3 * i:3 * i + 3, 3 * j:3 * j + 3] = blk
     return E
+
+
+def exact_windows(o, L):
+    """The library's pruned lag windows: the reference's cull window of
+    calcMatrixBlock.m:57-61 cut to the lags whose block is not zero in exact
+    arithmetic (after the S front: every outer node of the test triangle sees
+    the field triangle inside its S disk at level l-1; before the P front:
+    level l+1 cannot reach the field triangle from any node)."""
+    lo, hi = windows(o, L)
+    N, dt = o.N, o.dt
+    V = o.coords[o.tri]
+    X = (o.ghn[None, :, 0:1] * V[:, None, 0, :] + o.ghn[None, :, 1:2] * V[:, None, 1, :]) + \
+        o.ghn[None, :, 2:3] * V[:, None, 2, :]                      # N, nGH, 3
+    RF = np.sqrt(((V - o.center[:, None, :]) ** 2).sum(-1)).max(1)
+    lo2, hi2 = lo.copy(), hi.copy()
+    for i in range(N):
+        dmax = np.sqrt(((X[i][None, :, None, :] - V[:, None, :, :]) ** 2).sum(-1)).max(axis=(1, 2))
+        dlo = np.maximum(0.0, np.sqrt(((X[i][None, :, :] - o.center[:, None, :]) ** 2).sum(-1)).min(1) - RF)
+        ms = np.floor(dmax * (1 + 1e-12) / (o.velS * dt)) + 1
+        ml = np.ceil(dlo * (1 - 1e-12) / (o.velP * dt) - 1)
+        ok = lo[i] <= hi[i]
+        hi2[i] = np.where(ok, np.minimum(hi[i], ms), hi[i]).astype(np.int64)
+        lo2[i] = np.where(ok, np.maximum(lo[i], ml), lo[i]).astype(np.int64)
+    return lo2, hi2

@@ -17,19 +17,11 @@ sys.path.insert(0, os.path.dirname(HERE))
 _dp = C.POINTER(C.c_double)
 
 
-def items_of(mesh, Tfin, nT, L):
-    c, mx = mesh.center, mesh.maxL
-    dt, cS, cP = Tfin / nT, 1.0, 3 ** 0.5
-    d = np.sqrt(((c[:, None, :] - c[None, :, :]) ** 2).sum(-1))
-    dmin = d - mx[None, :] - mx[:, None]
-    dmax = d + mx[None, :] + mx[:, None]
-    lags = np.arange(L)
-    lo = np.full(d.shape, L, dtype=np.int64)
-    hi = np.full(d.shape, -1, dtype=np.int64)
-    for l in lags:
-        ok = ((l - 1) * cS * dt < dmax) & ((l + 1) * cP * dt >= dmin)
-        lo = np.where(ok & (lo == L), l, lo)
-        hi = np.where(ok, l, hi)
+def items_of(o, L):
+    """First level and item count of every pair under the library's pruned
+    lag windows (the items the assembly kernel evaluates)."""
+    from hostasm import exact_windows
+    lo, hi = exact_windows(o, L)
     mlo = np.maximum(1, lo - 1)
     n = np.where(hi >= lo, hi + 1 - mlo + 1, 0)
     return mlo, n
@@ -42,7 +34,7 @@ def count(mesh, Tfin, nT, L, nsample, seed=0, nGH=7):
     lib.opcount_item.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double]
     lib.opcount_item.restype = C.c_longlong
     o = Oracle(mesh.coordinates, mesh.triangles, Tfin=Tfin, nT=nT, nGH=nGH)
-    mlo, n = items_of(mesh, Tfin, nT, L)
+    mlo, n = items_of(o, L)
     flat = n.ravel()
     off = np.concatenate([[0], np.cumsum(flat)])
     total = int(off[-1])
