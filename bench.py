@@ -31,10 +31,10 @@ sys.path.insert(0, ROOT)
 
 # Algorithmic FP64 operations per (pair, level) item of k_levels on this
 # workload (7 outer nodes): mean over a uniform sample of 200000 of the
-# 59284540 items, counted by the op-counting host build of the device math
-# (python tests/opcount.py 200000 -> 8238.8 +- 8.4); each + - * / (a fused
-# multiply-add as 2), sqrt and libm call counts once.
-ALG_FLOP_PER_ITEM = 8238.8
+# 43938820 items of the pruned lag windows, counted by the op-counting host
+# build of the device math (python tests/opcount.py 200000 -> 9325.5 +- 7.4);
+# each + - * / (a fused multiply-add as 2), sqrt and libm call counts once.
+ALG_FLOP_PER_ITEM = 9325.5
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -88,12 +88,72 @@ def cpu_baseline(m, pb, seconds_hint=True):
                       f"{rows[0]}-{rows[1] - 1} of E_0..E_{L - 1} ({units} culled pair.lags, {el:.1f} s)"}
 
 
+def _solve(s, api, xpts, tpts, precision):
+    """Assemble (timed by the library's events), solve, post-process; host copies."""
+    s.assemble(precision)
+    t = s.timings()
+    s.rhs(api.DATUM_SPHERE_T2EXP)
+    a = s.mot_solve(None)
+    u = s.pp_eval(xpts, tpts)
+    return t, a, u
+
+
+def chosen_forms(s, m, pb, xpts, tpts, train_samples=2_000_000, epochs=20):
+    """The two chosen forms on the same problem, each against the plain FP64
+    form of the same run: (1) far field in FP32, (2) near field from the learned
+    surrogate (trained here, on the device, on seeded near-field pairs of
+    generated icospheres and screens), and both together."""
+    from elastodynamicebem_amd import api
+    from elastodynamicebem_amd import surrogate as SG
+    rel = lambda p, q: float(np.linalg.norm(p - q) / np.linalg.norm(q))
+    t64, a64, u64 = _solve(s, api, xpts, tpts, api.PREC_FP64)
+    out = {"fp64": {"assembly_ms": t64["assembly_ms"], "integral_kernel_ms": t64["levels_ms"]}}
+    tm, am, um = _solve(s, api, xpts, tpts, api.PREC_MIXED)
+    st = s.assembly_stats()
+    out["mixed"] = {"assembly_ms": tm["assembly_ms"], "integral_kernel_ms": tm["levels_ms"],
+                    "speedup_vs_fp64": t64["assembly_ms"] / tm["assembly_ms"],
+                    "fp32_items": st["far_items"], "fp64_items": st["fp64_items"],
+                    "density_rel_l2_vs_fp64": rel(am, a64), "field_rel_l2_vs_fp64": rel(um, u64)}
+    print("bench: forms: mixed done; generating surrogate training samples", file=sys.stderr, flush=True)
+    sur = SG.Surrogate()
+    t0 = time.perf_counter()
+    ntr, nhe = SG.build_training_set(sur, train_samples, seed=0, held_frac=0.02)
+    tgen = time.perf_counter() - t0
+    sur.init(ntr, seed=0)
+    print(f"bench: forms: {ntr} samples in {tgen:.1f} s; training", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    losses = sur.train(ntr, epochs, batch=2048, lr=2e-3, seed=0)
+    ttrain = time.perf_counter() - t0
+    print(f"bench: forms: trained in {ttrain:.1f} s", file=sys.stderr, flush=True)
+    held = sur.loss(ntr, nhe)
+    x, y = sur.get_samples(ntr, nhe)
+    lab_err = rel(sur.predict(x), y)
+    sur.attach(s)
+    res = {}
+    for name, prec in (("surrogate", api.PREC_SURROGATE), ("mixed_surrogate", api.PREC_MIXED | api.PREC_SURROGATE)):
+        ts, as_, us = _solve(s, api, xpts, tpts, prec)
+        ss = s.surrogate_stats()
+        res[name] = {"assembly_ms": ts["assembly_ms"], "integral_kernel_ms": ts["levels_ms"],
+                     "surrogate_ms": ss["ms"], "surrogate_blocks": ss["blocks"],
+                     "speedup_vs_fp64": t64["assembly_ms"] / ts["assembly_ms"],
+                     "density_rel_l2_vs_fp64": rel(as_, a64), "field_rel_l2_vs_fp64": rel(us, u64)}
+    sur.detach(s)
+    sur.close()
+    out.update(res)
+    out["surrogate_training"] = {"samples": ntr, "held_out": nhe, "sample_generation_s": tgen, "epochs": epochs,
+                                 "train_s": ttrain, "final_train_loss": float(losses[-1]),
+                                 "held_out_loss": held, "held_out_label_rel_l2": lab_err}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the mixed-precision and surrogate forms (their timing and error vs FP64)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy kernel of MIB MiB (traffic-counter calibration)")
     args = ap.parse_args()
@@ -124,7 +184,7 @@ def main():
     def step():
         s.assemble(api.PREC_FP64)
         s.rhs(api.DATUM_SPHERE_T2EXP)
-        s.lib.ebem_mot_solve(s.h, None, None)
+        _lib.check(s.lib.ebem_mot_solve(s.h, None, None))
         _lib.check(s.lib.ebem_pp_eval(s.h, None, xpts.shape[0], _lib.dptr(xpts), tpts.size, _lib.dptr(tpts),
                                       _lib.dptr(upts)))
 
@@ -174,6 +234,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": None,
     }
+    if not args.no_forms:
+        out["forms"] = chosen_forms(s, m, pb, xpts, tpts)
     s.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, pb)

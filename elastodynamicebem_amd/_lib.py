@@ -33,6 +33,7 @@ SIGNATURES = {
     "ebem_assembly_stats": (C.c_int, [_vp, _i64p, _i64p]),
     "ebem_store_lag_blocks": (C.c_int, [_vp, _i64p]),
     "ebem_get_block": (C.c_int, [_vp, C.c_int, _dp]),
+    "ebem_get_block_rows": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
     "ebem_rhs": (C.c_int, [_vp, C.c_int, _dp]),
     "ebem_mot_solve": (C.c_int, [_vp, _dp, _dp]),
     "ebem_pp_eval": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),

@@ -189,6 +189,12 @@ class Solver:
         _lib.check(self.lib.ebem_get_block(self.h, lag, out.ctypes.data_as(_lib._dp)))
         return out
 
+    def get_block_rows(self, lag: int, r0: int, r1: int) -> np.ndarray:
+        """Rows of test triangles [r0, r1) of E_lag: (3 (r1 - r0)) x 3N."""
+        out = np.zeros((3 * (r1 - r0), 3 * self.N), order="F")
+        _lib.check(self.lib.ebem_get_block_rows(self.h, lag, r0, r1, out.ctypes.data_as(_lib._dp)))
+        return out
+
     def rhs(self, datum: int) -> np.ndarray:
         out = np.zeros((3 * self.N, self.nT), order="F")
         _lib.check(self.lib.ebem_rhs(self.h, datum, out.ctypes.data_as(_lib._dp)))

@@ -395,15 +395,17 @@ __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, con
 }
 
 // Dense export of lag l: out (3N x 3N col-major), one thread per block of the segment.
-__global__ void k_dense_lag(Store S, int N, int L, int lag, double* __restrict__ out) {
-    int i = blockIdx.y;
+// Rows [r0, r0 + gridDim.y) of E_lag into a dense column-major array with
+// leading dimension ld (3 x rows by 3N).
+__global__ void k_dense_lag(Store S, int N, int L, int lag, int r0, int64_t ld, double* __restrict__ out) {
+    int i = r0 + blockIdx.y;
     int64_t b0 = S.seg[(int64_t)i * L + lag], b1 = S.seg[(int64_t)i * L + lag + 1];
-    int64_t n3 = 3 * (int64_t)N;
     for (int64_t b = b0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < b1; b += (int64_t)gridDim.x * blockDim.x) {
         int j = S.col[b];
         const int ii[9] = {0, 0, 0, 1, 1, 1, 2, 2, 2}, jj[9] = {0, 1, 2, 0, 1, 2, 0, 1, 2};
         const int qq[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
-        for (int k = 0; k < 9; k++) out[(3 * (int64_t)i + ii[k]) + n3 * (3 * (int64_t)j + jj[k])] = S.v[qq[k]][b];
+        for (int k = 0; k < 9; k++)
+            out[(3 * (int64_t)blockIdx.y + ii[k]) + ld * (3 * (int64_t)j + jj[k])] = S.v[qq[k]][b];
     }
 }
 

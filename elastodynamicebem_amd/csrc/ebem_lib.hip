@@ -1178,28 +1178,38 @@ int ebem_assembly_stats(ebem_handle* h, int64_t* far_items, int64_t* fp64_items)
     return 0;
 }
 
-int ebem_get_block(ebem_handle* h, int lag, double* out) {
+int ebem_get_block_rows(ebem_handle* h, int lag, int r0, int r1, double* out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
     if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
     if (!out) return fail(EBEM_EINVAL, "null output");
+    if (lag < 0) return fail(EBEM_EINVAL, "negative lag");
+    if (r0 < 0 || r1 > h->N || r0 >= r1) return fail(EBEM_EINVAL, "row range [%d, %d) outside [0, %d)", r0, r1, h->N);
     HIPC(hipSetDevice(h->device));
-    const size_t n3 = 3 * (size_t)h->N;
-    if (lag < 0 || lag >= h->store_L) {
-        if (lag >= 0) {
-            std::memset(out, 0, sizeof(double) * n3 * n3);
-            return 0;  // beyond L: exactly zero
-        }
-        return fail(EBEM_EINVAL, "negative lag");
+    const size_t ld = 3 * (size_t)(r1 - r0), n3 = 3 * (size_t)h->N;
+    if (lag >= h->store_L) {
+        std::memset(out, 0, sizeof(double) * ld * n3);
+        return 0;  // beyond L: exactly zero
     }
     DevBuf<double> d;
-    HIPC(d.alloc(n3 * n3));
-    HIPC(hipMemsetAsync(d.p, 0, sizeof(double) * n3 * n3, h->stream));
-    hipLaunchKernelGGL(k_dense_lag, dim3(4, h->N), dim3(64), 0, h->stream, h->store(), h->N, h->store_L, lag, d.p);
+    HIPC(d.alloc(ld * n3));
+    HIPC(hipMemsetAsync(d.p, 0, sizeof(double) * ld * n3, h->stream));
+    hipLaunchKernelGGL(k_dense_lag, dim3(4, r1 - r0), dim3(64), 0, h->stream, h->store(), h->N, h->store_L, lag, r0,
+                       (int64_t)ld, d.p);
     HIPC(hipGetLastError());
-    HIPC(hipMemcpyAsync(out, d.p, sizeof(double) * n3 * n3, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipMemcpyAsync(out, d.p, sizeof(double) * ld * n3, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
     return 0;
+}
+
+int ebem_get_block(ebem_handle* h, int lag, double* out) {
+    int N = 0;
+    {
+        if (int e = check_handle(h)) return e;
+        std::lock_guard<std::mutex> lk(h->lock);
+        N = h->N;
+    }
+    return ebem_get_block_rows(h, lag, 0, N, out);
 }
 
 int ebem_rhs(ebem_handle* h, int datum_kind, double* beta_out) {

@@ -101,6 +101,10 @@ int ebem_assembly_stats(ebem_handle *h, int64_t *far_items, int64_t *fp64_items)
  * calcMatrixBlock.m / matrixSavedV{lag+1} of timeMarchingD.m:141-158). */
 int ebem_get_block(ebem_handle *h, int lag, double *out);
 
+/* Test rows [r0, r1) of E_lag: dense 3(r1-r0) x 3N column-major (the rows of
+ * the `matrix` of calcMatrixBlock.m that belong to those test triangles). */
+int ebem_get_block_rows(ebem_handle *h, int lag, int r0, int r1, double *out);
+
 /* Right-hand side beta (3N x nT col-major) on the device and, if beta_out is
  * non-NULL, copied to the host (Tn/BEMenerg_core_calcTnBlock.m:27-58). */
 int ebem_rhs(ebem_handle *h, int datum, double *beta_out);

@@ -40,3 +40,13 @@ def sphere3():
 def frob_rel(a, b):
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a - b)
+
+
+def ulp_oracle(mesh, Tfin, nT, nGH=7, seed=0):
+    """Oracle on coordinates moved by one ulp (random direction per entry): its
+    distance to the plain oracle is the reference algorithm's own sensitivity."""
+    from oracle import Oracle
+    rng = np.random.default_rng(seed)
+    X = mesh.coordinates
+    X2 = np.where(rng.random(X.shape) < 0.5, np.nextafter(X, np.inf), np.nextafter(X, -np.inf))
+    return Oracle(X2, mesh.triangles, Tfin=Tfin, nT=nT, nGH=nGH)

@@ -1,0 +1,81 @@
+"""Full-size flat screen: [-0.5, 0.5]^2 in 64 x 64 squares (8192 triangles,
+24576 DOF), t in [0, 2] with 64 steps, datum (0, 0, sin^2(pi t / 2)), plain
+FP64 and mixed precision, on the device.
+
+Sampled test rows of E_lag against the oracle (FP64 bar 1e-10, or 3x the
+oracle's own change under a one-ulp perturbation of the coordinates where the
+reference algorithm is less reproducible than that: late lags of the corner
+rows move by 1e-4 .. 1 under such a perturbation), the store size
+against the reference's light-cone count, and size-independent properties of
+the solved density: the mesh and the datum are invariant under the rotation by
+pi about the screen's normal, so the density is too (x, y components change
+sign, z does not).  The mixed-precision error against the FP64 run is
+printed."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import frob_rel, ulp_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def screen64():
+    from elastodynamicebem_amd.mesh import screen_uniform
+    return screen_uniform(64)
+
+
+def _rot_pairs(mesh):
+    """Index of the triangle each triangle maps to under (x, y) -> (-x, -y)."""
+    c = mesh.center
+    key = {tuple(np.round(p[:2], 9)): k for k, p in enumerate(c)}
+    return np.array([key[tuple(np.round(-p[:2], 9))] for p in c])
+
+
+def test_screen64_rows_store_and_density(screen64):
+    from elastodynamicebem_amd import api
+    from oracle import Oracle
+    pb = api.PbParam(rho=1.0, mu=1.0, lambda_=1.0, Tfin=2.0, nT=64)
+    with api.Solver(screen64, pb, 7) as s:
+        assert s.num_blocks() == 47
+        t0 = time.perf_counter()
+        s.assemble(api.PREC_FP64)
+        t_asm = time.perf_counter() - t0
+        info = s.store_info()
+        assert abs(info["pair_lags"] - 7.565e8) / 7.565e8 < 5e-3, info
+        o = Oracle(screen64.coordinates, screen64.triangles, Tfin=2.0, nT=64)
+        o2 = ulp_oracle(screen64, 2.0, 64)
+        for rows in ((0, 2), (4131, 4133)):
+            for lag in (0, 1, 7, 30, 46):
+                G = s.get_block_rows(lag, *rows)
+                R = o.matrix_block(lag, rows=rows)[0][3 * rows[0]:3 * rows[1]]
+                R2 = o2.matrix_block(lag, rows=rows)[0][3 * rows[0]:3 * rows[1]]
+                if np.linalg.norm(R) > 0:
+                    e, noise = frob_rel(G, R), frob_rel(R2, R)
+                else:
+                    e, noise = np.linalg.norm(G), np.linalg.norm(R2)
+                assert e < max(1e-10, 3 * noise), (rows, lag, e, noise)
+        s.rhs(api.DATUM_SCREEN_SIN2)
+        a = s.mot_solve(None)
+        tm = s.timings()
+        per_lag = s.lag_blocks()
+        hist_bytes = 52 * sum(int(per_lag[1:min(n, 46) + 1].sum()) for n in range(64))
+        assert np.all(np.isfinite(a)) and np.linalg.norm(a) > 0
+        P = _rot_pairs(screen64)
+        A = a.reshape(-1, 3, a.shape[1])
+        sc = np.abs(A).max()
+        assert np.abs(A[:, 2] - A[P, 2]).max() < 1e-9 * sc
+        assert np.abs(A[:, :2] + A[P, :2]).max() < 1e-9 * sc
+        s.assemble(api.PREC_MIXED)
+        st = s.assembly_stats()
+        am = s.mot_solve(None)
+        tmm = s.timings()
+    err = frob_rel(am, a)
+    print(f"screen64: assembly {t_asm:.2f} s wall ({tm['levels_ms']:.0f} ms integrals), "
+          f"{info['pair_lags'] / (tm['assembly_ms'] * 1e-3):.3e} pair.lags/s; "
+          f"MOT {tm['mot_ms']:.0f} ms (history {tm['history_ms']:.0f} ms, "
+          f"{hist_bytes / (tm['history_ms'] * 1e-3) / 1e9:.0f} GB/s); mixed assembly "
+          f"{tmm['assembly_ms']:.0f} ms, {st['far_items']} FP32 items, density rel L2 vs FP64 {err:.3e}")
+    assert np.isfinite(err)
