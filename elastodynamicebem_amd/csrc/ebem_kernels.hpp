@@ -16,8 +16,10 @@
 //                  the 1e-14 cleanup (calcMatrixBlock.m:77), written into the
 //                  row-lag-segmented block store (symmetric 3x3 -> 6 values).
 // MOT (SA/BEMenerg_coreSA_timeMarching.m:70-113):
-//   k_history      r_n = beta_n - sum_{l>=1} E_l alpha_{n-l}, one workgroup per
-//                  test row, streaming the row's contiguous lag segments;
+//   k_history_far  the lags of the history sum whose densities precede a block of
+//                  kHistBlock steps, for all steps of the block in one pass;
+//   k_history      r_n = beta_n - (far part) - the remaining lags, one workgroup
+//                  per test row, streaming the row's contiguous lag segments;
 //   k_gemv_rows    alpha_n = E_0^{-1} r_n with a row-major inverse, one wave
 //                  per row.
 #pragma once
@@ -413,11 +415,70 @@ __global__ void k_dense_lag(Store S, int N, int L, int lag, int r0, int64_t ld, 
 
 // r = beta_n - sum_{l=1}^{lmax} E_l alpha_{n-l}; one workgroup of 256 per test row.
 // alpha: 3N x nT column-major (column n contiguous).
-__global__ void __launch_bounds__(256) k_history(Store S, int N, int L, int n, const double* __restrict__ beta,
-                                                 const double* __restrict__ alpha, double* __restrict__ r) {
+// Workgroup sum of v (all 256 threads) returned on every thread of wave 0.
+__device__ inline double wg_sum(double v, double* red4) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red4[w] = v;
+    __syncthreads();
+    return (red4[0] + red4[1]) + (red4[2] + red4[3]);
+}
+
+// The history sum is taken kHistBlock steps at a time.  For steps
+// n = n0 + b (b < nb <= kHistBlock) the terms with lag l > b only involve
+// densities alpha_{n-l} with n - l < n0, known before the block starts:
+// k_history_far streams the store once for the whole block and leaves
+// part_b = sum_{l=b+1}^{min(n0+b, L-1)} E_l alpha_{n0+b-l};
+// k_history then adds the remaining lags 1..min(b, L-1) step by step.
+// Store traffic per block: one pass plus lags 1..b per step, instead of one
+// full pass per step.
+constexpr int kHistBlock = 8;
+
+__global__ void __launch_bounds__(256) k_history_far(Store S, int N, int L, int n0, int nb,
+                                                     const double* __restrict__ alpha, double* __restrict__ part) {
+    __shared__ double red4[4];
     const int i = blockIdx.x;
     const int64_t n3 = 3 * (int64_t)N;
-    const int lmax = min(n, L - 1);
+    double acc[kHistBlock][3];
+#pragma unroll
+    for (int q = 0; q < kHistBlock; q++) acc[q][0] = acc[q][1] = acc[q][2] = 0.0;
+    const int lmax = min(L - 1, n0 + nb - 1);
+    for (int l = 1; l <= lmax; l++) {
+        const int bmin = max(0, l - n0), bmax = min(nb - 1, l - 1);
+        if (bmin > bmax) continue;
+        const int64_t b0 = S.seg[(int64_t)i * L + l], b1 = S.seg[(int64_t)i * L + l + 1];
+        for (int64_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+            const int j = S.col[b];
+            const double w00 = S.v[0][b], w01 = S.v[1][b], w02 = S.v[2][b];
+            const double w11 = S.v[3][b], w12 = S.v[4][b], w22 = S.v[5][b];
+#pragma unroll
+            for (int q = 0; q < kHistBlock; q++) {
+                if (q < bmin || q > bmax) continue;
+                const double* al = alpha + (int64_t)(n0 + q - l) * n3 + 3 * (int64_t)j;
+                const double x0 = al[0], x1 = al[1], x2 = al[2];
+                acc[q][0] += w00 * x0 + w01 * x1 + w02 * x2;
+                acc[q][1] += w01 * x0 + w11 * x1 + w12 * x2;
+                acc[q][2] += w02 * x0 + w12 * x1 + w22 * x2;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kHistBlock; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double v = wg_sum(acc[q][c], red4);
+            if (threadIdx.x == 0 && q < nb) part[(int64_t)q * n3 + 3 * i + c] = v;
+        }
+}
+
+// r = beta_n - part - sum_{l=1}^{lmax} E_l alpha_{n-l} (part may be null);
+// one workgroup of 256 per test row.
+__global__ void __launch_bounds__(256) k_history(Store S, int N, int L, int n, int lmax,
+                                                 const double* __restrict__ beta, const double* __restrict__ alpha,
+                                                 const double* __restrict__ part, double* __restrict__ r) {
+    const int i = blockIdx.x;
+    const int64_t n3 = 3 * (int64_t)N;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int l = 1; l <= lmax; l++) {
         const int64_t b0 = S.seg[(int64_t)i * L + l], b1 = S.seg[(int64_t)i * L + l + 1];
@@ -449,6 +510,7 @@ __global__ void __launch_bounds__(256) k_history(Store S, int N, int L, int n, c
     if (threadIdx.x < 3) {
         int c = threadIdx.x;
         double s = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+        if (part) s += part[3 * i + c];
         r[3 * i + c] = beta[(int64_t)n * n3 + 3 * i + c] - s;
     }
 }
@@ -537,22 +599,36 @@ struct PPParams {
 };
 
 // Window [ms, me] of m per (point, triangle); count = me - ms + 1 (0 if empty).
-__global__ void __launch_bounds__(256) k_pp_windows(const FieldTri* __restrict__ tri, const double* __restrict__ xs,
-                                                    int p0, int np, PPParams P, int32_t* __restrict__ ms,
-                                                    int32_t* __restrict__ me) {
+//  * M_m = 0 while c_P tau_m < Dlo = |x - c_F| - R_F (R_F the triangle's
+//    circumradius about its centroid, Dlo a lower bound of the distance from
+//    x to the triangle): ms is the first m that may reach it, so M_{ms-1} = 0;
+//  * M_m takes its static value once c_S tau_m > Dmax = max_v |x - v| (the
+//    whole triangle inside the S sphere: the P shell misses it and the S-ball
+//    integrals no longer depend on tau), so D_m = 0 for m > me with me the
+//    first such m.
+// Relative margins 1e-12 keep ties on the computed side.
+__global__ void __launch_bounds__(256) k_pp_windows(const FieldTri* __restrict__ tri, const double* __restrict__ verts,
+                                                    const double* __restrict__ xs, int p0, int np, PPParams P,
+                                                    int32_t* __restrict__ ms, int32_t* __restrict__ me) {
     int64_t tot = (int64_t)np * P.N;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
         int p = p0 + (int)(q / P.N), j = (int)(q % P.N);
         const FieldTri& F = tri[j];
-        double d0 = xs[3 * p] - F.center[0], d1 = xs[3 * p + 1] - F.center[1], d2 = xs[3 * p + 2] - F.center[2];
-        double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-        double dmin = d - F.maxL, dmax = d + F.maxL;
-        // M_{ms-1} = 0: c_P tau_{ms-1} <= dmin (one step of margin)
-        double a = floor((dmin / P.cP - P.delta) / P.dt);
+        const double* v = verts + (int64_t)j * 9;
+        const double x0 = xs[3 * p], x1 = xs[3 * p + 1], x2 = xs[3 * p + 2];
+        double rf2 = 0.0, dmax2 = 0.0;
+        for (int k = 0; k < 3; k++) {
+            double a0 = v[3 * k] - F.center[0], a1 = v[3 * k + 1] - F.center[1], a2 = v[3 * k + 2] - F.center[2];
+            rf2 = fmax(rf2, a0 * a0 + a1 * a1 + a2 * a2);
+            double b0 = x0 - v[3 * k], b1 = x1 - v[3 * k + 1], b2 = x2 - v[3 * k + 2];
+            dmax2 = fmax(dmax2, b0 * b0 + b1 * b1 + b2 * b2);
+        }
+        double d0 = x0 - F.center[0], d1 = x1 - F.center[1], d2 = x2 - F.center[2];
+        const double dlo = fmax(0.0, sqrt(d0 * d0 + d1 * d1 + d2 * d2) - sqrt(rf2));
+        const double a = ceil((dlo * (1.0 - 1e-12) / P.cP - P.delta) / P.dt);
         int lo = (int)fmax(0.0, fmin(a, (double)P.mmax + 1.0));
-        // D_m = 0 for m > me: c_S tau_{m-1} >= dmax (one step of margin)
-        double b = ceil((dmax / P.cS - P.delta) / P.dt) + 2.0;
-        int hi = (int)fmin(b, (double)P.mmax);
+        const double b = floor((sqrt(dmax2) * (1.0 + 1e-12) / P.cS - P.delta) / P.dt) + 1.0;
+        int hi = (int)fmax(-1.0, fmin(b, (double)P.mmax));
         ms[q] = lo;
         me[q] = hi;
     }

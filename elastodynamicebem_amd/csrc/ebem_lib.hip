@@ -300,7 +300,7 @@ struct ebem_handle {
     DevBuf<int32_t> d_col;
     DevBuf<double> d_val;  // 6 * nnzb
     // MOT
-    DevBuf<double> d_beta, d_alpha, d_r, d_A, d_Ainv;
+    DevBuf<double> d_beta, d_alpha, d_r, d_A, d_Ainv, d_part;
     DevBuf<rocblas_int> d_ipiv, d_info;
     bool have_beta = false, have_alpha = false;
     double tim[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -826,31 +826,49 @@ int mot(ebem_handle* h, const double* beta, double* alpha_out) {
     tf.stop(h->stream);
     h->tim[2] = tf.ms();
     Store S = h->store();
-    hipEvent_t ev[4];
+    const int L = h->store_L;
+    HIPC(h->d_part.alloc((size_t)kHistBlock * n3));
+    // events: [start of history work, end of history work = start of apply] per step
+    // (the far pass of a block is timed with its block's first step)
+    std::vector<hipEvent_t> ev(3 * (size_t)nT);
     for (auto& e : ev) HIPC(hipEventCreate(&e));
+    int launches = 0;
+    for (int n0 = 0; n0 < nT; n0 += kHistBlock) {
+        const int nb = std::min(kHistBlock, nT - n0);
+        HIPC(hipEventRecord(ev[3 * n0], h->stream));
+        if (n0 > 0) {
+            hipLaunchKernelGGL(k_history_far, dim3(h->N), dim3(256), 0, h->stream, S, h->N, L, n0, nb, h->d_alpha.p,
+                               h->d_part.p);
+            launches++;
+        }
+        for (int b = 0; b < nb; b++) {
+            const int n = n0 + b;
+            if (b > 0) HIPC(hipEventRecord(ev[3 * n], h->stream));
+            hipLaunchKernelGGL(k_history, dim3(h->N), dim3(256), 0, h->stream, S, h->N, L, n, std::min(b, L - 1),
+                               h->d_beta.p, h->d_alpha.p, n0 > 0 ? h->d_part.p + (size_t)b * n3 : nullptr, h->d_r.p);
+            launches++;
+            HIPC(hipEventRecord(ev[3 * n + 1], h->stream));
+            hipLaunchKernelGGL(k_gemv_rows, dim3((n3 + 3) / 4), dim3(256), 0, h->stream, h->d_Ainv.p, h->d_r.p,
+                               h->d_alpha.p + (size_t)n * n3, n3);
+            HIPC(hipEventRecord(ev[3 * n + 2], h->stream));
+        }
+    }
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(h->stream));
     double th = 0, ts = 0;
     for (int n = 0; n < nT; n++) {
-        HIPC(hipEventRecord(ev[0], h->stream));
-        hipLaunchKernelGGL(k_history, dim3(h->N), dim3(256), 0, h->stream, S, h->N, h->store_L, n, h->d_beta.p,
-                           h->d_alpha.p, h->d_r.p);
-        HIPC(hipEventRecord(ev[1], h->stream));
-        hipLaunchKernelGGL(k_gemv_rows, dim3((n3 + 3) / 4), dim3(256), 0, h->stream, h->d_Ainv.p, h->d_r.p,
-                           h->d_alpha.p + (size_t)n * n3, n3);
-        HIPC(hipEventRecord(ev[2], h->stream));
-        HIPC(hipEventSynchronize(ev[2]));
         float a = 0, b = 0;
-        HIPC(hipEventElapsedTime(&a, ev[0], ev[1]));
-        HIPC(hipEventElapsedTime(&b, ev[1], ev[2]));
+        HIPC(hipEventElapsedTime(&a, ev[3 * n], ev[3 * n + 1]));
+        HIPC(hipEventElapsedTime(&b, ev[3 * n + 1], ev[3 * n + 2]));
         th += a;
         ts += b;
     }
-    HIPC(hipGetLastError());
     for (auto& e : ev) (void)hipEventDestroy(e);
     tall.stop(h->stream);
     h->tim[5] = tall.ms();
     h->tim[3] = th;
     h->tim[4] = ts;
-    h->tim[7] = nT;
+    h->tim[7] = launches;
     if (alpha_out)
         HIPC(hipMemcpyAsync(alpha_out, h->d_alpha.p, sizeof(double) * (size_t)n3 * nT, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
@@ -955,8 +973,8 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             const int np = std::min(P0, npts - p0);
             int64_t tot = (int64_t)np * N;
             int64_t blocks = std::min<int64_t>((tot + 255) / 256, (int64_t)h->CUs * 64);
-            hipLaunchKernelGGL(k_pp_windows, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0,
-                               np, P, d_ms.p, d_me.p);
+            hipLaunchKernelGGL(k_pp_windows, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p,
+                               h->d_verts.p, d_x.p, p0, np, P, d_ms.p, d_me.p);
             HIPC(hipGetLastError());
             hipLaunchKernelGGL(k_pp_counts, dim3(np), dim3(256), 0, h->stream, d_ms.p, d_me.p, N, d_cnt.p);
             HIPC(hipGetLastError());
@@ -1067,6 +1085,7 @@ int ebem_destroy(ebem_handle* h) {
         h->d_r.reset();
         h->d_A.reset();
         h->d_Ainv.reset();
+        h->d_part.reset();
         h->d_ipiv.reset();
         h->d_info.reset();
         (void)hipStreamDestroy(h->stream);
