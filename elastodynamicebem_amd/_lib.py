@@ -27,6 +27,8 @@ SIGNATURES = {
     "ebem_set_mesh": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _ip, C.c_int]),
     "ebem_set_problem": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
     "ebem_set_rule": (C.c_int, [_vp, C.c_int]),
+    "ebem_set_inner_rule": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "ebem_inner_rule_tables": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp]),
     "ebem_num_blocks": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "ebem_assemble": (C.c_int, [_vp, C.c_int]),
     "ebem_store_info": (C.c_int, [_vp, _i64p, _i64p, _i64p, _i64p]),

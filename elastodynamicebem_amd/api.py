@@ -27,6 +27,7 @@ from .mesh import SpaceMesh, read_mesh
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_SURROGATE = 2  # flag: near-field pairs from the attached surrogate (surrogate.py)
+PREC_GHC = 4        # numeric inner quadrature of the reference's GPU path (kernelV.cu), FP64
 
 DATUM_SCREEN_SIN2 = 0
 DATUM_SPHERE_T2EXP = 1
@@ -66,7 +67,21 @@ class PbParam:
 class MethodInfo:
     typeIntg: str = "SA"
     numNodiExt: int = 7
+    numSubRegion: int = 0
+    numNodiSing: int = 0
+    numNodiInt: int = 0
+    numNodiDiag: int = 0
     extra: dict = field(default_factory=dict)
+
+
+def inner_rule_tables(nSub: int, nSing: int, nDiag: int):
+    """(GHCn, GHCw, DIAGn, DIAGw) of the numeric form: GaussHammerComposite(nSub, nSing)
+    and doppioGauss1D(nDiag), built by the native library."""
+    lib = _lib.load()
+    inner = np.zeros((nSub * nSing, 4))
+    diag = np.zeros((nDiag, 4))
+    _lib.check(lib.ebem_inner_rule_tables(nSub, nSing, nDiag, _lib.dptr(inner), _lib.dptr(diag)))
+    return inner[:, :3].copy(), inner[:, 3].copy(), diag[:, :3].copy(), diag[:, 3].copy()
 
 
 def readInputFile(path: str) -> PbParam:
@@ -98,22 +113,34 @@ def GaussHammerComposite_1(n: int):
 
 
 def BEMenerg_setupCore(methodSelected: str):
-    """Parse "SA nn" (BEMenerg_setupCore.m:3-21).  Other method families of the
-    reference (MX.*, GPU, FN) select numerical inner quadratures that this
-    library does not provide; they raise like an unknown method does."""
+    """Parse "SA nn" and "FN nn nSub nSing nDiag" (BEMenerg_setupCore.m:3-93).
+    "FN" is the numeric inner quadrature of the direct driver (kernelV.cu off the
+    diagonal, doppioGauss1D on the diagonal); the CPU families MX.* are not
+    provided and raise."""
     toks = methodSelected.split()
     if not toks:
         raise ValueError("Metodo non disponibile")
     mi = MethodInfo(typeIntg=toks[0])
-    if mi.typeIntg != "SA":
-        raise NotImplementedError(f"method family {mi.typeIntg!r} is not on the analytic SA path")
+    if mi.typeIntg not in ("SA", "FN"):
+        raise NotImplementedError(f"method family {mi.typeIntg!r} is not provided (SA and FN are)")
     if len(toks) < 2:
         raise ValueError("Valori nodi esterni non validi")
     mi.numNodiExt = int(toks[1])
     if mi.numNodiExt not in (3, 7, 12, 19):
         raise ValueError("Valori nodi esterni non validi")
     EXTn, EXTw = GaussHammerComposite_1(mi.numNodiExt)
-    return mi, EXTn, EXTw, None, None, None, None
+    if mi.typeIntg == "SA":
+        return mi, EXTn, EXTw, None, None, None, None
+    if len(toks) < 5:
+        raise ValueError("Valori nodi interni non validi")
+    mi.numSubRegion, mi.numNodiSing, mi.numNodiDiag = int(toks[2]), int(toks[3]), int(toks[4])
+    if mi.numSubRegion not in (1, 4, 16, 64) or mi.numNodiSing not in (1, 3, 7, 12, 19):
+        raise ValueError("Valori nodi interni non validi")
+    if mi.numNodiDiag not in (16, 64, 256, 1024):
+        raise ValueError("Valori nodi interni non validi")
+    mi.numNodiInt = mi.numSubRegion * mi.numNodiSing
+    INTn, INTw, DIAGn, DIAGw = inner_rule_tables(mi.numSubRegion, mi.numNodiSing, mi.numNodiDiag)
+    return mi, EXTn, EXTw, INTn, INTw, DIAGn, DIAGw
 
 
 class Solver:
@@ -153,6 +180,10 @@ class Solver:
         self.close()
 
     # -- pieces --------------------------------------------------------------
+    def set_inner_rule(self, nSub: int, nSing: int, nDiag: int) -> None:
+        """Inner rules of the numeric form (PREC_GHC)."""
+        _lib.check(self.lib.ebem_set_inner_rule(self.h, nSub, nSing, nDiag))
+
     def num_blocks(self) -> int:
         L = C.c_int()
         _lib.check(self.lib.ebem_num_blocks(self.h, C.byref(L)))
@@ -255,14 +286,22 @@ def BEMenerg_coreSA_timeMarching(pbParam: PbParam, domainMesh: SpaceMesh, method
 
 def BEMenerg_dir_timeMarchingD(pbParam, domainMesh, methodInfo, GHn=None, GHw=None, GHCn=None, GHCw=None,
                                DIAGn=None, DIAGw=None, *, datum=None, precision: int = PREC_FP64, device: int = 0):
-    """Single-layer (V) system of the direct Dirichlet driver with SA inner
-    integration.  The double-layer right-hand side of timeMarchingD.m:287 is
-    not part of this library (raises if a DD problem with lambda+mu == 0 is
-    given, like main.m:61-63)."""
+    """Single-layer (V) system of the direct Dirichlet driver: with methodInfo
+    "FN ..." the numeric inner quadrature of the reference's GPU kernel
+    (kernelV.cu + calcSubBlockDiagV.m, PREC_GHC), with "SA ..." the analytic
+    form.  The double-layer right-hand side of timeMarchingD.m:287 is not part
+    of this library (raises if a DD problem with lambda+mu == 0 is given, like
+    main.m:61-63)."""
     if pbParam.lambda_ + pbParam.mu == 0:
         raise ValueError("Problema non adatto a questa formulazione")
-    return BEMenerg_coreSA_timeMarching(pbParam, domainMesh, methodInfo, GHn, GHw, datum=datum,
-                                        precision=precision, device=device)
+    if methodInfo.typeIntg != "FN":
+        return BEMenerg_coreSA_timeMarching(pbParam, domainMesh, methodInfo, GHn, GHw, datum=datum,
+                                            precision=precision, device=device)
+    with Solver(domainMesh, pbParam, methodInfo.numNodiExt, device) as s:
+        s.set_inner_rule(methodInfo.numSubRegion, methodInfo.numNodiSing, methodInfo.numNodiDiag)
+        s.assemble(PREC_GHC)
+        s.rhs(_datum_for(pbParam, datum))
+        return s.mot_solve(None)
 
 
 def BEMenerg_postProcA_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, *, device: int = 0):

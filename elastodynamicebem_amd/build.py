@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libebem_hip.so")
 SOURCES = ["ebem_lib.hip"]
-HEADERS = ["ebem_sa.hpp", "ebem_kernels.hpp", "ebem_sur.hpp"]
+HEADERS = ["ebem_sa.hpp", "ebem_kernels.hpp", "ebem_sur.hpp", "ebem_ghc.hpp"]
 
 
 def hipcc() -> str:

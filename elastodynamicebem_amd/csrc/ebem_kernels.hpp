@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, const int16_t* __restrict__ hi,
                                              const int64_t* __restrict__ off, const double* __restrict__ W,
                                              const uint8_t* __restrict__ sflag, int N, int L, int r0,
-                                             double inv4pirho, Store S) {
+                                             double inv4pirho, double fourpirho, int ghc, Store S) {
     extern __shared__ int64_t run[];  // L entries
     const int r = blockIdx.x;
     const int i = r0 + r;
@@ -351,7 +351,7 @@ __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, con
             a = lo[pbase + j];
             b = hi[pbase + j];
             o = off[pbase + j];
-            sur = sflag && sflag[pbase + j];
+            sur = ghc || (sflag && sflag[pbase + j]);
         }
         int amin = a, bmax = b;
         for (int d = 32; d >= 1; d >>= 1) {
@@ -371,7 +371,7 @@ __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, con
 #pragma unroll
                 for (int q = 0; q < 6; q++) e[q] = 1.0 * w1[q];
                 if (sur) {
-                    // surrogate pair: the slot already holds 4 pi rho E_l
+                    // surrogate pair or numeric form: the slot already holds 4 pi rho E_l
                 } else if (l >= 1) {
                     const double* w0 = W + (o + (l - mlo)) * 6;
 #pragma unroll
@@ -382,10 +382,16 @@ __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, con
 #pragma unroll
                     for (int q = 0; q < 6; q++) e[q] = e[q] + wm[q];
                 }
+                if (!ghc) {  // calcMatrixBlock.m:77
 #pragma unroll
-                for (int q = 0; q < 6; q++) {
-                    double v = e[q] * inv4pirho;
-                    S.v[q][pos] = (fabs(v) < 1.0e-14) ? 0.0 : v;
+                    for (int q = 0; q < 6; q++) {
+                        double v = e[q] * inv4pirho;
+                        S.v[q][pos] = (fabs(v) < 1.0e-14) ? 0.0 : v;
+                    }
+                } else {  // kernelV.cu:150-157 off the diagonal, none on it (calcSubBlockDiagV.m)
+#pragma unroll
+                    for (int q = 0; q < 6; q++)
+                        S.v[q][pos] = (j == i || fabs(e[q]) > 1.0e-14) ? e[q] / fourpirho : 0.0;
                 }
                 S.col[pos] = j;
             }

@@ -22,6 +22,7 @@
 #include "../../include/ebem.h"
 #include "ebem_kernels.hpp"
 #include "ebem_sur.hpp"
+#include "ebem_ghc.hpp"
 
 using namespace ebem;
 
@@ -71,10 +72,14 @@ struct DevBuf {
 // Gauss-Hammer tables of util/quadRules/GaussHammer_base.m:19-255 mapped by
 // GaussHammerComposite(1, n) (util/quadRules/GaussHammerComposite.m:31-52):
 // barycentric columns (b1,b3,b2), weights doubled.
-int gauss_hammer(int n, double* nodes, double* w) {
-    double g[28][3] = {};
-    double gw[28] = {};
-    if (n == 3) {
+// util/quadRules/GaussHammer_base.m rows for n in {1, 3, 7, 12, 19}: raw
+// barycentric nodes and weights (summing to 1/2).
+int gh_base(int n, double g[28][3], double gw[28]) {
+    for (int i = 0; i < 28; i++) g[i][0] = g[i][1] = g[i][2] = gw[i] = 0.0;
+    if (n == 1) {
+        g[0][0] = g[0][1] = g[0][2] = 1.0 / 3;
+        gw[0] = 1.0 / 2;
+    } else if (n == 3) {
         const double v1 = 2.0 / 3, v2 = 1.0 / 6, v3 = 1.0 / 6;
         const double t[3][3] = {{v1, v2, v3}, {v3, v1, v2}, {v2, v3, v1}};
         for (int i = 0; i < 3; i++) {
@@ -139,12 +144,112 @@ int gauss_hammer(int n, double* nodes, double* w) {
     } else {
         return -1;
     }
+    return 0;
+}
+
+int gauss_hammer(int n, double* nodes, double* w) {
+    double g[28][3], gw[28];
+    if (gh_base(n, g, gw)) return -1;
     for (int i = 0; i < n; i++) {
         nodes[3 * i + 0] = g[i][0];
         nodes[3 * i + 1] = g[i][2];
         nodes[3 * i + 2] = g[i][1];
         w[i] = 2.0 * gw[i];
     }
+    return 0;
+}
+
+// GaussHammerComposite.m:5-52: the reference triangle cut into nSub = nDiv^2
+// congruent sub-triangles (rows of 2(nDiv - i) + 1 alternating triangles), the
+// nNod-point rule on each; weights (2/nSub) w.  out: nSub*nNod x 4 (b0, b1, b2, w).
+int gh_composite(int nSub, int nNod, std::vector<double>& out) {
+#pragma clang fp contract(off)
+    double g[28][3], gw[28];
+    if (gh_base(nNod, g, gw)) return -1;
+    int nDiv = (int)std::lround(std::sqrt((double)nSub));
+    if (nDiv < 1 || nDiv * nDiv != nSub) return -1;
+    out.assign((size_t)4 * nSub * nNod, 0.0);
+    const double f = 1.0 / nDiv;
+    int n = 0;
+    for (int i = 1; i <= nDiv; i++) {
+        const double zMin = (i - 1) * f, zMax = i * f;
+        const int nTr = 2 * (nDiv - i) + 1;
+        double x[3] = {1 - zMin, 1 - zMin, (1 - zMin) - f}, y[3] = {0 - f, 0, 0}, z[3] = {0, 0, 0};
+        for (int j = 1; j <= nTr; j++) {
+            const double x2 = x[1], x3 = x[2], y2 = y[1], y3 = y[2];
+            if (j % 2 == 1) {
+                x[0] = x2; x[1] = x3; x[2] = x3;
+                y[0] = y2; y[1] = y3; y[2] = y3 + f;
+                z[0] = zMin; z[1] = zMax; z[2] = zMin;
+            } else {
+                x[0] = x2; x[1] = x3; x[2] = x3 - f;
+                y[0] = y2; y[1] = y3; y[2] = y3;
+                z[0] = zMax; z[1] = zMin; z[2] = zMax;
+            }
+            for (int q = 0; q < nNod; q++) {
+                double* o = out.data() + 4 * ((size_t)n * nNod + q);
+                o[0] = g[q][0] * x[0] + g[q][1] * x[1] + g[q][2] * x[2];
+                o[1] = g[q][0] * y[0] + g[q][1] * y[1] + g[q][2] * y[2];
+                o[2] = g[q][0] * z[0] + g[q][1] * z[1] + g[q][2] * z[2];
+                o[3] = (2.0 / nSub) * gw[q];
+            }
+            n++;
+        }
+    }
+    return 0;
+}
+
+// Gauss-Legendre rule on (-1, 1), ascending (Newton on P_n).
+void gauss_legendre(int n, double* x, double* w) {
+    for (int i = 0; i < n; i++) {
+        double r = std::cos(kPi * (i + 0.75) / (n + 0.5));
+        double dp = 1.0;
+        for (int it = 0; it < 100; it++) {
+            double p0 = 1.0, p1 = r;
+            for (int k = 2; k <= n; k++) {
+                double p2 = ((2.0 * k - 1.0) * r * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (r * p1 - p0) / (r * r - 1.0);
+            double dr = p1 / dp;
+            r -= dr;
+            if (std::fabs(dr) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = r;
+        for (int k = 2; k <= n; k++) {
+            double p2 = ((2.0 * k - 1.0) * r * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (r * p1 - p0) / (r * r - 1.0);
+        x[n - 1 - i] = r;
+        w[n - 1 - i] = 2.0 / ((1.0 - r * r) * dp * dp);
+    }
+}
+
+// doppioGauss1D (util/setup/BEMenerg_setupCore.m:105-150): collapsed tensor
+// Gauss rule, nodes ((1-x_j)(1-x_i), x_j(1-x_i), x_i), weights 2 w_i w_j (1-x_i)
+// from the Gauss-Legendre rule on (0, 1).  out: nNodes x 4.
+int doppio_gauss(int nNodes, std::vector<double>& out) {
+#pragma clang fp contract(off)
+    const int n1 = (int)std::lround(std::sqrt((double)nNodes));
+    if (n1 < 1 || n1 * n1 != nNodes) return -1;
+    std::vector<double> x(n1), w(n1);
+    gauss_legendre(n1, x.data(), w.data());
+    for (int k = 0; k < n1; k++) {
+        x[k] = 0.5 + (x[k] / 2);
+        w[k] = w[k] / 2;
+    }
+    out.assign((size_t)4 * nNodes, 0.0);
+    for (int j = 0; j < n1; j++)
+        for (int i = 0; i < n1; i++) {
+            double* o = out.data() + 4 * ((size_t)i + (size_t)n1 * j);
+            o[0] = (1 - x[j]) * (1 - x[i]);
+            o[1] = x[j] * (1 - x[i]);
+            o[2] = x[i];
+            o[3] = 2 * w[j] * (w[i] * (1 - x[i]));
+        }
     return 0;
 }
 
@@ -287,6 +392,9 @@ struct ebem_handle {
     DevBuf<double> d_gh;
     DevBuf<double> d_verts;  // N x 9: the three vertices of every triangle, mesh order
     double hmax = 0;         // longest edge of the mesh
+    // inner rules of the numeric form (EBEM_PREC_GHC), ebem_set_inner_rule
+    int ghc_nSub = 0, ghc_nSing = 0, ghc_nDiag = 0;
+    DevBuf<double> d_ghc_in, d_ghc_dg;
     // surrogate attached by ebem_attach_surrogate, and what the last assembly gave it
     ebem_sur* sur = nullptr;
     int64_t sur_items = 0, sur_pairs_blocks = 0;
@@ -568,8 +676,12 @@ void launch_levels_split(ebem_handle* h, AsmParams P, int r0, int64_t nitems, in
 
 int assemble(ebem_handle* h, int precision) {
     if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set before ebem_assemble");
-    if (precision < 0 || precision > (EBEM_PREC_MIXED | EBEM_PREC_SURROGATE))
+    if (precision < 0 || precision > (EBEM_PREC_MIXED | EBEM_PREC_SURROGATE | EBEM_PREC_GHC))
         return fail(EBEM_EINVAL, "unknown precision %d", precision);
+    const bool ghc = (precision & EBEM_PREC_GHC) != 0;
+    if (ghc && precision != EBEM_PREC_GHC)
+        return fail(EBEM_EINVAL, "EBEM_PREC_GHC is an FP64 form of its own (no mixed precision or surrogate)");
+    if (ghc && !h->ghc_nSub) return fail(EBEM_EINVAL, "EBEM_PREC_GHC needs ebem_set_inner_rule first");
     const bool mixed = (precision & EBEM_PREC_MIXED) != 0;
     ebem_sur* sur = (precision & EBEM_PREC_SURROGATE) ? h->sur : nullptr;
     if ((precision & EBEM_PREC_SURROGATE) && !sur)
@@ -733,7 +845,27 @@ int assemble(ebem_handle* h, int precision) {
             h->sur_items += nsur;
             const int64_t nlev = acc - nsur;  // the surrogate items are the suffix of perm
             tlev.start(h->stream);
-            switch (h->nGH) {
+            if (ghc) {
+                GhcRules R{h->d_ghc_in.p, h->d_ghc_dg.p, h->ghc_nSub * h->ghc_nSing, h->ghc_nDiag};
+                const size_t lds = sizeof(double) * 4 * (size_t)(R.nInt + R.nDiag);  // <= 71.7 KB
+                if (lds > 65536) {
+                    HIPC(hipFuncSetAttribute((const void*)k_ghc_blocks<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    HIPC(hipFuncSetAttribute((const void*)k_ghc_blocks<7>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    HIPC(hipFuncSetAttribute((const void*)k_ghc_blocks<12>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    HIPC(hipFuncSetAttribute((const void*)k_ghc_blocks<19>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                }
+                const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((acc + 3) / 4, (int64_t)h->CUs * 32));
+#define GHC_LAUNCH(NG)                                                                                              \
+    hipLaunchKernelGGL(k_ghc_blocks<NG>, dim3((unsigned)blocks), dim3(256), lds, h->stream, h->d_tri.p, h->d_gh.p, \
+                       h->d_verts.p, P, R, r0, acc, d_ip.p, d_off.p, lo, d_W.p)
+                switch (h->nGH) {
+                    case 3: GHC_LAUNCH(3); break;
+                    case 7: GHC_LAUNCH(7); break;
+                    case 12: GHC_LAUNCH(12); break;
+                    default: GHC_LAUNCH(19); break;
+                }
+#undef GHC_LAUNCH
+            } else switch (h->nGH) {
                 case 3: launch_levels_split<3>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
                 case 7: launch_levels_split<7>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
                 case 12: launch_levels_split<12>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
@@ -751,7 +883,7 @@ int assemble(ebem_handle* h, int precision) {
             }
         }
         hipLaunchKernelGGL(k_emit, dim3(rows), dim3(64), L * sizeof(int64_t), h->stream, lo, hi, d_off.p, d_W.p, sflag,
-                           N, L, r0, inv4pirho, S);
+                           N, L, r0, inv4pirho, 4 * kPi * h->rho, ghc ? 1 : 0, S);
         HIPC(hipGetLastError());
         r0 = r1;
     }
@@ -1146,6 +1278,38 @@ int ebem_set_rule(ebem_handle* h, int nGH) {
     h->nGH = nGH;
     h->assembled = false;
     if (h->have_mesh) return upload_geometry(h);
+    return 0;
+}
+
+int ebem_set_inner_rule(ebem_handle* h, int nSub, int nSing, int nDiag) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    // BEMenerg_setupCore.m:58-93 (method families GPU / FN)
+    if (!(nSub == 1 || nSub == 4 || nSub == 16 || nSub == 64) ||
+        !(nSing == 1 || nSing == 3 || nSing == 7 || nSing == 12 || nSing == 19))
+        return fail(EBEM_EINVAL, "Valori nodi interni non validi (%d sub-regions x %d nodes)", nSub, nSing);
+    if (!(nDiag == 16 || nDiag == 64 || nDiag == 256 || nDiag == 1024))
+        return fail(EBEM_EINVAL, "Valori nodi interni non validi (%d diagonal nodes)", nDiag);
+    HIPC(hipSetDevice(h->device));
+    std::vector<double> in, dg;
+    if (gh_composite(nSub, nSing, in) || doppio_gauss(nDiag, dg)) return fail(EBEM_EINVAL, "inner rule construction failed");
+    HIPC(h->d_ghc_in.alloc(in.size()));
+    HIPC(h->d_ghc_dg.alloc(dg.size()));
+    HIPC(hipMemcpyAsync(h->d_ghc_in.p, in.data(), sizeof(double) * in.size(), hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->d_ghc_dg.p, dg.data(), sizeof(double) * dg.size(), hipMemcpyHostToDevice, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    h->ghc_nSub = nSub;
+    h->ghc_nSing = nSing;
+    h->ghc_nDiag = nDiag;
+    h->assembled = false;
+    return 0;
+}
+
+int ebem_inner_rule_tables(int nSub, int nSing, int nDiag, double* inner, double* diag) {
+    std::vector<double> in, dg;
+    if (gh_composite(nSub, nSing, in) || doppio_gauss(nDiag, dg)) return fail(EBEM_EINVAL, "invalid inner rule");
+    if (inner) std::memcpy(inner, in.data(), sizeof(double) * in.size());
+    if (diag) std::memcpy(diag, dg.data(), sizeof(double) * dg.size());
     return 0;
 }
 

@@ -41,6 +41,7 @@ extern "C" {
 #define EBEM_PREC_FP64 0   /* plain form: everything FP64 */
 #define EBEM_PREC_MIXED 1  /* far-field pairs (centroid distance > 3h) integrated in FP32 */
 #define EBEM_PREC_SURROGATE 2 /* flag: near-field pairs predicted by the attached surrogate MLP */
+#define EBEM_PREC_GHC 4    /* numeric inner quadrature of the reference's GPU path (kernelV.cu), FP64 */
 
 /* Dirichlet data of ebem_rhs (Tn/BEMenerg_core_calcNucleoTn.m:133-183). */
 #define EBEM_DATUM_SCREEN_SIN2 0    /* (0,0,sin^2(pi t/2) H(t)) */
@@ -70,6 +71,20 @@ int ebem_set_problem(ebem_handle *h, double rho, double mu, double lambda, doubl
 /* Outer Gauss-Hammer rule of methodInfo "SA nn" (util/setup/BEMenerg_setupCore.m:9-21,
  * GaussHammerComposite(1, nn)); nn in {3, 7, 12, 19}. */
 int ebem_set_rule(ebem_handle *h, int nGH);
+
+/* Inner rules of the numeric form EBEM_PREC_GHC, method "FN nn nSub nSing nDiag"
+ * of util/setup/BEMenerg_setupCore.m:58-93: composite Gauss-Hammer
+ * GaussHammerComposite(nSub, nSing) for pairs i != j (kernelV.cu, launched by
+ * dir/BEMenerg_dir_timeMarchingD.m:97-161) and the collapsed rule
+ * doppioGauss1D(nDiag) on the three children of the diagonal blocks
+ * (dir/BEMenerg_dir_calcSubBlockDiagV.m).  nSub in {1,4,16,64}, nSing in
+ * {1,3,7,12,19}, nDiag in {16,64,256,1024}. */
+int ebem_set_inner_rule(ebem_handle *h, int nSub, int nSing, int nDiag);
+
+/* The two inner rules as tables (GHCn/GHCw, DIAGn/DIAGw outputs of
+ * BEMenerg_setupCore.m): inner (nSub*nSing) x 4 and diag nDiag x 4 row-major,
+ * columns = three barycentric coordinates and the weight.  Either may be NULL. */
+int ebem_inner_rule_tables(int nSub, int nSing, int nDiag, double *inner, double *diag);
 
 /* Number of lag blocks L = min(nT, ceil(maxVertexDist/(c_S dt)) + 1)
  * (dir/BEMenerg_dir_calcNumBlocks.m:1-17). */

@@ -100,11 +100,9 @@ void orc_mesh_derive(int N, const double *X, const int *tri, double *normal, dou
 /* util/quadRules/GaussHammer_base.m:19-255 (rules 1,3,7,12,19) and
  * util/quadRules/GaussHammerComposite.m:5-52 with nSubPart = 1, whose map sends
  * barycentric (b1,b2,b3) to (b1,b3,b2) and doubles the weights. */
-int orc_gauss_hammer(int n, double *nodes, double *w) {
-    double g[28][3];
-    double gw[28];
-    memset(g, 0, sizeof g);
-    memset(gw, 0, sizeof gw);
+static int gh_base(int n, double g[28][3], double gw[28]) {
+    memset(g, 0, 28 * sizeof g[0]);
+    memset(gw, 0, 28 * sizeof gw[0]);
     if (n == 1) {
         g[0][0] = g[0][1] = g[0][2] = 1.0 / 3;
         gw[0] = 1.0 / 2;
@@ -173,6 +171,12 @@ int orc_gauss_hammer(int n, double *nodes, double *w) {
     } else {
         return -1;
     }
+    return 0;
+}
+
+int orc_gauss_hammer(int n, double *nodes, double *w) {
+    double g[28][3], gw[28];
+    if (gh_base(n, g, gw)) return -1;
     /* GaussHammerComposite.m:31-52, nDiv = 1: nodes = nGHstd * [[1 0 0];[0 0 1];[0 1 0]],
      * weights = 2 * wGHstd. */
     for (int i = 0; i < n; i++) {
@@ -1195,3 +1199,232 @@ void orc_field_record(const double *T, const double *n, double *rec) {
     for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) rec[16 + 3 * i + j] = C[i][j];
     for (int k = 0; k < 3; k++) rec[25 + k] = (T[k] + T[3 + k] + T[6 + k]) / 3;
 }
+
+/* ------------------------------------------- numeric inner quadrature (GHC) --
+ * The reference's GPU single-layer path: functions/BEMenerg_dir/CUDAkernels/
+ * kernelV.cu (off-diagonal pairs, Gauss-Hammer outer x composite
+ * Gauss-Hammer inner rule on the kernel "nucleo") and
+ * BEMenerg_dir_calcSubBlockDiagV.m (diagonal blocks: the field triangle split
+ * at each outer node into three children, each with the doppioGauss1D rule). */
+
+/* util/quadRules/GaussHammerComposite.m:5-52 for any nSubPart (a square). */
+int orc_gh_composite(int nSub, int nNod, double *nodes, double *w) {
+    double g[28][3], gw[28];
+    if (gh_base(nNod, g, gw)) return -1;
+    int nDiv = (int)lround(sqrt((double)nSub));
+    if (nDiv < 1 || nDiv * nDiv != nSub) return -1;
+    double f = 1.0 / nDiv;
+    int n = 0;
+    for (int i = 1; i <= nDiv; i++) {
+        double zMin = (i - 1) * f, zMax = i * f;
+        int nTr = 2 * (nDiv - i) + 1;
+        double x[3] = {1 - zMin, 1 - zMin, (1 - zMin) - f};
+        double y[3] = {0 - f, 0, 0};
+        double z[3] = {0, 0, 0};
+        for (int j = 1; j <= nTr; j++) {
+            double x2 = x[1], x3 = x[2], y2 = y[1], y3 = y[2];
+            if (j % 2 == 1) {
+                x[0] = x2; x[1] = x3; x[2] = x3;
+                y[0] = y2; y[1] = y3; y[2] = y3 + f;
+                z[0] = zMin; z[1] = zMax; z[2] = zMin;
+            } else {
+                x[0] = x2; x[1] = x3; x[2] = x3 - f;
+                y[0] = y2; y[1] = y3; y[2] = y3;
+                z[0] = zMax; z[1] = zMin; z[2] = zMax;
+            }
+            for (int q = 0; q < nNod; q++) {
+                double *o = nodes + 3 * (n * nNod + q);
+                o[0] = g[q][0] * x[0] + g[q][1] * x[1] + g[q][2] * x[2];
+                o[1] = g[q][0] * y[0] + g[q][1] * y[1] + g[q][2] * y[2];
+                o[2] = g[q][0] * z[0] + g[q][1] * z[1] + g[q][2] * z[2];
+                w[n * nNod + q] = (2.0 / nSub) * gw[q];
+            }
+            n++;
+        }
+    }
+    return 0;
+}
+
+/* Gauss-Legendre n-point rule on (-1, 1) (util/quadRules/Gauss1D.m type 1:
+ * eigenvalues of the Jacobi matrix; here Newton on P_n, same nodes and weights
+ * to rounding), ascending. */
+static void gauss_legendre(int n, double *x, double *w) {
+    for (int i = 0; i < n; i++) {
+        double r = cos(PI * (i + 0.75) / (n + 0.5));
+        for (int it = 0; it < 100; it++) {
+            double p0 = 1.0, p1 = r;
+            for (int k = 2; k <= n; k++) {
+                double p2 = ((2.0 * k - 1.0) * r * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            double dp = n * (r * p1 - p0) / (r * r - 1.0);
+            double dr = p1 / dp;
+            r -= dr;
+            if (fabs(dr) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = r;
+        for (int k = 2; k <= n; k++) {
+            double p2 = ((2.0 * k - 1.0) * r * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        double dp = n * (r * p1 - p0) / (r * r - 1.0);
+        x[n - 1 - i] = r;
+        w[n - 1 - i] = 2.0 / ((1.0 - r * r) * dp * dp);
+    }
+}
+
+/* doppioGauss1D (util/setup/BEMenerg_setupCore.m:105-150): collapsed
+ * tensor rule on the reference triangle, nodes ((1-x_j)(1-x_i), x_j(1-x_i), x_i),
+ * weights 2 w_i w_j (1-x_i), x, w the Gauss-Legendre rule on (0, 1);
+ * node k = i + n1 j (MATLAB column-major reshape).  nNodes a square. */
+int orc_doppio_gauss(int nNodes, double *nodes, double *w) {
+    int n1 = (int)lround(sqrt((double)nNodes));
+    if (n1 < 1 || n1 * n1 != nNodes || n1 > 64) return -1;
+    double x[64], wx[64];
+    gauss_legendre(n1, x, wx);
+    for (int k = 0; k < n1; k++) {
+        x[k] = 0.5 + (x[k] / 2);
+        wx[k] = wx[k] / 2;
+    }
+    for (int j = 0; j < n1; j++)
+        for (int i = 0; i < n1; i++) {
+            int k = i + n1 * j;
+            nodes[3 * k + 0] = (1 - x[j]) * (1 - x[i]);
+            nodes[3 * k + 1] = x[j] * (1 - x[i]);
+            nodes[3 * k + 2] = x[i];
+            w[k] = 2 * wx[j] * (wx[i] * (1 - x[i]));
+        }
+    return 0;
+}
+
+/* kernelV.cu:188-258 nucleo (and the anonymous function nu of
+ * calcSubBlockDiagV.m:9-53): the time-integrated kernel at offset x, r = |x|. */
+void orc_nucleo(const double *x, double r, double t, double cP, double cS, double *nu) {
+    double HP = (t - (r / cP) > 0), HS = (t - (r / cS) > 0);
+    double r3 = pow(r, 3), r5 = pow(r, 5);
+    double br = (cP * cP * t * t - r * r) / (cP * cP) * HP - (cS * cS * t * t - r * r) / (cS * cS) * HS;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double xx = x[i] * x[j];
+            double v;
+            if (i == j)
+                v = (xx / r3) * (1 / (cP * cP)) * HP + ((1 / r) - (xx / r3)) * (1 / (cS * cS)) * HS -
+                    0.5 * ((1 / r3) - 3 * (xx / r5)) * br;
+            else
+                v = (xx / r3) * (1 / (cP * cP)) * HP + (-(xx / r3)) * (1 / (cS * cS)) * HS -
+                    0.5 * (-3 * (xx / r5)) * br;
+            nu[i + 3 * j] = v;
+        }
+}
+
+/* kernelV.cu:158-186 isBlockNull: every vertex pair farther than c_S t_{lag-1}. */
+static int ghc_block_null(const double *TS, const double *TF, double cS, double dt, int lag) {
+    double dmax = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double a = TS[3 * i] - TF[3 * j], b = TS[3 * i + 1] - TF[3 * j + 1], c = TS[3 * i + 2] - TF[3 * j + 2];
+            double d = sqrt(a * a + b * b + c * c);
+            if (d > dmax) dmax = d;
+        }
+    return (lag - 1) * cS * dt > dmax;
+}
+
+/* One 3x3 block of E_lag in the GHC mode (column-major out), with the
+ * off-diagonal cleanup |sum| > 1e-14 before the division by 4 pi rho
+ * (kernelV.cu:150-157); diagonal blocks uncleaned (calcSubBlockDiagV.m:104). */
+int orc_ghc_pair_block(const orc_param *p, const orc_mesh *m, int nGH, const double *ghn, const double *ghw,
+                       int nInt, const double *ghcn, const double *ghcw, int nDiag, const double *dgn,
+                       const double *dgw, int lag, int S, int F, double *out) {
+    double dt = p->Tfin / p->nT, cP = p->velP, cS = p->velS;
+    memset(out, 0, 9 * sizeof(double));
+    double TS[9], TF[9], xq[28 * 3], wq[28];
+    tri_verts(m, S, TS);
+    tri_verts(m, F, TF);
+    tri_gh(m, S, nGH, ghn, ghw, xq, wq);
+    double acc[9];
+    memset(acc, 0, sizeof acc);
+    if (S != F) {
+        if (ghc_block_null(TS, TF, cS, dt, lag)) return 0;
+        const double coef[3] = {1, -2, 1}, ct[3] = {-1, 0, 1};
+        for (int l = 0; l < nGH; l++)
+            for (int pi = 0; pi < nInt; pi++) {
+                const double *b = ghcn + 3 * pi;
+                double y[3], d[3];
+                for (int k = 0; k < 3; k++) y[k] = b[0] * TF[k] + b[1] * TF[3 + k] + b[2] * TF[6 + k];
+                for (int k = 0; k < 3; k++) d[k] = y[k] - xq[3 * l + k];
+                double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                double pw = wq[l] * (ghcw[pi] * m->area[F]);
+                for (int k = 0; k < 3; k++) {
+                    double t = dt * (lag + ct[k]);
+                    if (t < 0) continue;
+                    double nu[9];
+                    orc_nucleo(d, r, t, cP, cS, nu);
+                    for (int e = 0; e < 9; e++) acc[e] += pw * coef[k] * nu[e];
+                }
+            }
+        for (int e = 0; e < 9; e++)
+            if (fabs(acc[e]) > 1.0e-14) out[e] = acc[e] / (4 * PI * p->rho);
+        return 1;
+    }
+    const double coef[3] = {1, -2, 1};
+    for (int v = 0; v < 3; v++) {
+        int it = lag + 1 - v;
+        if (it <= 0) continue;
+        double t = it * dt;
+        double ghsum[9];
+        memset(ghsum, 0, sizeof ghsum);
+        for (int l = 0; l < nGH; l++) {
+            const double *P = xq + 3 * l;
+            double g2[9];
+            memset(g2, 0, sizeof g2);
+            /* children (V1,V3,P), (V2,V1,P), (V3,V2,P): calcCostantData.m:49-55 */
+            const int ca[3] = {0, 1, 2}, cb[3] = {2, 0, 1};
+            for (int c = 0; c < 3; c++) {
+                const double *A = TS + 3 * ca[c], *B = TS + 3 * cb[c];
+                double u[3] = {A[0] - P[0], A[1] - P[1], A[2] - P[2]}, w2[3] = {B[0] - P[0], B[1] - P[1], B[2] - P[2]};
+                double cr[3] = {u[1] * w2[2] - u[2] * w2[1], u[2] * w2[0] - u[0] * w2[2], u[0] * w2[1] - u[1] * w2[0]};
+                double ac = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]) / 2;
+                for (int k = 0; k < nDiag; k++) {
+                    const double *b = dgn + 3 * k;
+                    double y[3], d[3];
+                    for (int e = 0; e < 3; e++) y[e] = b[0] * A[e] + b[1] * B[e] + b[2] * P[e];
+                    for (int e = 0; e < 3; e++) d[e] = y[e] - P[e];
+                    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                    double nu[9];
+                    orc_nucleo(d, r, t, cP, cS, nu);
+                    for (int e = 0; e < 9; e++) g2[e] += (dgw[k] * ac) * nu[e];
+                }
+            }
+            for (int e = 0; e < 9; e++) ghsum[e] += wq[l] * g2[e];
+        }
+        for (int e = 0; e < 9; e++) acc[e] += coef[v] * ghsum[e];
+    }
+    for (int e = 0; e < 9; e++) out[e] = acc[e] / (4 * PI * p->rho);
+    return 1;
+}
+
+/* Rows [r0, r1) of E_lag in the GHC mode (timeMarchingD.m:97-161), 3N x 3N column-major. */
+int orc_ghc_matrix_block_rows(const orc_param *p, const orc_mesh *m, int nGH, const double *ghn, const double *ghw,
+                              int nInt, const double *ghcn, const double *ghcw, int nDiag, const double *dgn,
+                              const double *dgw, int lag, int r0, int r1, double *out, int nthreads) {
+    int N = m->N;
+    long n3 = 3L * N;
+    long npairs = (long)(r1 - r0) * N;
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+#endif
+    for (long ind = 0; ind < npairs; ind++) {
+        int S = r0 + (int)(ind % (r1 - r0));
+        int F = (int)(ind / (r1 - r0));
+        double blk[9];
+        orc_ghc_pair_block(p, m, nGH, ghn, ghw, nInt, ghcn, ghcw, nDiag, dgn, dgw, lag, S, F, blk);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) out[(3L * S + i) + n3 * (3L * F + j)] = blk[i + 3 * j];
+    }
+    (void)nthreads;
+    return 0;
+}
+

@@ -113,6 +113,23 @@ void orc_field_record(const double *T, const double *n, double *rec);
 void orc_counters(long long *out12);
 void orc_counters_reset(void);
 
+/* ---- numeric inner quadrature mode of the reference's GPU path ----------
+ * (functions/BEMenerg_dir/CUDAkernels/kernelV.cu, BEMenerg_dir_calcSubBlockDiagV.m) */
+
+/* GaussHammerComposite(nSub, nNod): nSub*nNod nodes (barycentric, 3 each) and weights. */
+int orc_gh_composite(int nSub, int nNod, double *nodes, double *w);
+/* doppioGauss1D(nNodes): collapsed Gauss rule on the reference triangle. */
+int orc_doppio_gauss(int nNodes, double *nodes, double *w);
+/* kernelV.cu nucleo: 3x3 column-major kernel at offset x (r = |x|), time t. */
+void orc_nucleo(const double *x, double r, double t, double cP, double cS, double *nu);
+/* One block (column-major) / rows [r0, r1) of E_lag, GHC mode. */
+int orc_ghc_pair_block(const orc_param *p, const orc_mesh *m, int nGH, const double *ghn, const double *ghw,
+                       int nInt, const double *ghcn, const double *ghcw, int nDiag, const double *dgn,
+                       const double *dgw, int lag, int S, int F, double *out);
+int orc_ghc_matrix_block_rows(const orc_param *p, const orc_mesh *m, int nGH, const double *ghn, const double *ghw,
+                              int nInt, const double *ghcn, const double *ghcw, int nDiag, const double *dgn,
+                              const double *dgw, int lag, int r0, int r1, double *out, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

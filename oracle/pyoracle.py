@@ -90,6 +90,17 @@ class Oracle:
         L.orc_pp_point.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), _dp, _dp, C.c_double, _dp]
         L.orc_counters.argtypes = [C.POINTER(C.c_longlong)]
         L.orc_field_record.argtypes = [_dp, _dp, _dp]
+        L.orc_gh_composite.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.orc_gh_composite.restype = C.c_int
+        L.orc_doppio_gauss.argtypes = [C.c_int, _dp, _dp]
+        L.orc_doppio_gauss.restype = C.c_int
+        L.orc_nucleo.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_ghc_pair_block.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, _dp, C.c_int, _dp, _dp,
+                                         C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_ghc_pair_block.restype = C.c_int
+        L.orc_ghc_matrix_block_rows.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, _dp, C.c_int,
+                                                _dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp,
+                                                C.c_int]
 
     # -- pieces -------------------------------------------------------------
     def num_blocks(self) -> int:
@@ -170,3 +181,47 @@ class Oracle:
 
     def counters_reset(self):
         self.lib.orc_counters_reset()
+
+    # -- numeric inner quadrature mode (kernelV.cu / calcSubBlockDiagV.m) ----
+    def gh_composite(self, nSub, nNod):
+        n = nSub * nNod
+        nodes, w = np.zeros((n, 3)), np.zeros(n)
+        if self.lib.orc_gh_composite(nSub, nNod, _p(nodes), _p(w)) != 0:
+            raise ValueError("invalid composite rule")
+        return nodes, w
+
+    def doppio_gauss(self, n):
+        nodes, w = np.zeros((n, 3)), np.zeros(n)
+        if self.lib.orc_doppio_gauss(n, _p(nodes), _p(w)) != 0:
+            raise ValueError("invalid collapsed rule")
+        return nodes, w
+
+    def nucleo(self, x, t):
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros(9)
+        self.lib.orc_nucleo(_p(x), float(np.linalg.norm(x)), float(t), self.velP, self.velS, _p(out))
+        return out.reshape(3, 3).T.copy()
+
+    def set_ghc(self, nSub=64, nSing=3, nDiag=256):
+        """Inner rules of the GHC mode ("FN nExt nSub nSing nDiag")."""
+        self.ghcn, self.ghcw = self.gh_composite(nSub, nSing)
+        self.dgn, self.dgw = self.doppio_gauss(nDiag)
+
+    def ghc_pair_block(self, lag, S, F):
+        out = np.zeros(9)
+        self.lib.orc_ghc_pair_block(C.byref(self.param), C.byref(self.mesh), self.nGH, _p(self.ghn), _p(self.ghw),
+                                    self.ghcw.size, _p(self.ghcn), _p(self.ghcw), self.dgw.size, _p(self.dgn),
+                                    _p(self.dgw), int(lag), int(S), int(F), _p(out))
+        return out.reshape(3, 3).T.copy()
+
+    def ghc_matrix_block(self, lag, rows=None, nthreads=None):
+        N = self.N
+        out = np.zeros((3 * N, 3 * N), order="F")
+        r0, r1 = (0, N) if rows is None else rows
+        nt = nthreads or os.cpu_count() or 1
+        self.lib.orc_ghc_matrix_block_rows(C.byref(self.param), C.byref(self.mesh), self.nGH, _p(self.ghn),
+                                           _p(self.ghw), self.ghcw.size, _p(self.ghcn), _p(self.ghcw),
+                                           self.dgw.size, _p(self.dgn), _p(self.dgw), int(lag), r0, r1,
+                                           out.ctypes.data_as(_dp), nt)
+        return np.ascontiguousarray(out)
+

@@ -41,9 +41,32 @@ def test_api_rejects_bad_method():
     with pytest.raises(ValueError):
         api.BEMenerg_setupCore("SA 05")
     with pytest.raises(NotImplementedError):
-        api.BEMenerg_setupCore("FN 19 64 3 256")
+        api.BEMenerg_setupCore("MX.G2D 7 64")
+    with pytest.raises(ValueError):
+        api.BEMenerg_setupCore("FN 7 32 3 256")
     mi, n, w, *_ = api.BEMenerg_setupCore("SA 07")
     assert mi.numNodiExt == 7 and n.shape == (7, 3) and abs(w.sum() - 1) < 1e-12
+    mi, n, w, GHCn, GHCw, DIAGn, DIAGw = api.BEMenerg_setupCore("FN 7 64 3 256")
+    assert mi.numNodiInt == 192 and GHCn.shape == (192, 3) and DIAGn.shape == (256, 3)
+    assert abs(GHCw.sum() - 1) < 1e-12 and abs(DIAGw.sum() - 1) < 1e-14
+
+
+def test_inner_rules_match_the_oracle():
+    """The library's composite and collapsed rules (host code of the shipped
+    library) equal the oracle's restatement of GaussHammerComposite.m and
+    doppioGauss1D."""
+    import numpy as np
+    from elastodynamicebem_amd import api
+    from elastodynamicebem_amd.mesh import screen_uniform
+    from oracle import Oracle
+    m = screen_uniform(1)
+    o = Oracle(m.coordinates, m.triangles)
+    for nsub, nsing, ndiag in ((64, 3, 256), (16, 7, 64), (4, 19, 1024), (1, 1, 16)):
+        a, aw, b, bw = api.inner_rule_tables(nsub, nsing, ndiag)
+        oa, oaw = o.gh_composite(nsub, nsing)
+        ob, obw = o.doppio_gauss(ndiag)
+        assert np.abs(a - oa).max() < 1e-15 and np.abs(aw - oaw).max() < 1e-17
+        assert np.abs(b - ob).max() < 1e-15 and np.abs(bw - obw).max() < 1e-17
 
 
 def test_read_input_file(tmp_path):
