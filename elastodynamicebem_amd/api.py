@@ -248,6 +248,20 @@ class Solver:
                                          _lib.dptr(x), t.size, _lib.dptr(t), _lib.dptr(out)))
         return out.transpose(2, 1, 0).copy()
 
+    def set_pp_rule(self, nSub: int, nSing: int) -> None:
+        """Rule of the numeric post-processing (pp_eval_numeric)."""
+        _lib.check(self.lib.ebem_set_pp_rule(self.h, nSub, nSing))
+
+    def pp_eval_numeric(self, x: np.ndarray, t: np.ndarray, density: np.ndarray | None = None) -> np.ndarray:
+        """u (3, npts, nt) by the reference's numeric GPU post-processing (kernelLayer.cu)."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(t, np.float64).ravel()
+        out = np.zeros((t.size, x.shape[0], 3))
+        d = None if density is None else np.asfortranarray(density, dtype=np.float64)
+        _lib.check(self.lib.ebem_pp_eval_numeric(self.h, None if d is None else d.ctypes.data_as(_lib._dp),
+                                                 x.shape[0], _lib.dptr(x), t.size, _lib.dptr(t), _lib.dptr(out)))
+        return out.transpose(2, 1, 0).copy()
+
     def node_integral(self, x, tri: int, t: float) -> np.ndarray:
         x = np.ascontiguousarray(x, np.float64)
         out = np.zeros(9)

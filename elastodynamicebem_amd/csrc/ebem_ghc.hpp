@@ -156,4 +156,56 @@ __global__ void __launch_bounds__(256) k_ghc_blocks(const FieldTri* __restrict__
     }
 }
 
+// Numeric post-processing (BEMenerg_postProcGPU/CUDAkernels/kernelLayer.cu:
+// 23-83): one thread per (point, triangle, m) item of the post-processing
+// windows, M_m = sum_p (w_p area_F) G(y_p - x, tau_m) over the rule in LDS,
+// divided by 4 pi rho, entries below 1e-14 zeroed (kernelLayer.cu:84-93);
+// tau_m as the analytic M kernel.  Writes M[item*6 ..].
+__global__ void __launch_bounds__(256) k_pp_M_num(const FieldTri* __restrict__ tri, const double* __restrict__ verts,
+                                                  const double* __restrict__ xs, int p0, PPParams P,
+                                                  const double* __restrict__ rule_g, int nInt, int64_t nitems,
+                                                  const int32_t* __restrict__ item_q, const int64_t* __restrict__ off,
+                                                  const int32_t* __restrict__ ms, double fourpirho,
+                                                  double* __restrict__ M) {
+    extern __shared__ double rule[];  // nInt x 4
+    for (int k = threadIdx.x; k < 4 * nInt; k += blockDim.x) rule[k] = rule_g[k];
+    __syncthreads();
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < nitems;
+         it += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t q = item_q[it];
+        const int p = p0 + q / P.N, j = q % P.N;
+        const int m = ms[q] + (int)(it - off[q]);
+        double tau;
+        {
+#pragma clang fp contract(off)
+            tau = (m == P.nrep) ? P.trep : P.trep - (double)(P.nrep - m) * P.dt;
+        }
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        if (tau > 0.0) {
+            const double* vf = verts + (int64_t)j * 9;
+            const double aF = tri[j].area;
+            const double x0 = xs[3 * p], x1 = xs[3 * p + 1], x2 = xs[3 * p + 2];
+            for (int k = 0; k < nInt; k++) {
+                const double* b = rule + 4 * k;
+                double d[3];
+                d[0] = (b[0] * vf[0] + b[1] * vf[3] + b[2] * vf[6]) - x0;
+                d[1] = (b[0] * vf[1] + b[1] * vf[4] + b[2] * vf[7]) - x1;
+                d[2] = (b[0] * vf[2] + b[1] * vf[5] + b[2] * vf[8]) - x2;
+                const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                double nu[6];
+                nucleo6(d, r, tau, P.cP, P.cS, nu);
+                const double w = b[3] * aF;
+#pragma unroll
+                for (int c = 0; c < 6; c++) acc[c] += w * nu[c];
+            }
+        }
+        double* o = M + it * 6;
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            const double v = acc[c] / fourpirho;
+            o[c] = fabs(v) < 1.0e-14 ? 0.0 : v;
+        }
+    }
+}
+
 }  // namespace ebem

@@ -395,6 +395,9 @@ struct ebem_handle {
     // inner rules of the numeric form (EBEM_PREC_GHC), ebem_set_inner_rule
     int ghc_nSub = 0, ghc_nSing = 0, ghc_nDiag = 0;
     DevBuf<double> d_ghc_in, d_ghc_dg;
+    // rule of the numeric post-processing, ebem_set_pp_rule
+    int pp_nInt = 0;
+    DevBuf<double> d_pp_rule;
     // surrogate attached by ebem_attach_surrogate, and what the last assembly gave it
     ebem_sur* sur = nullptr;
     int64_t sur_items = 0, sur_pairs_blocks = 0;
@@ -1011,11 +1014,12 @@ int mot(ebem_handle* h, const double* beta, double* alpha_out) {
 // Post-processing: times grouped by delta = t - nhat dt (shared M_m), points
 // chunked so that the M scratch stays within budget.
 int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, int nt, const double* t,
-            double* u_out) {
+            double* u_out, bool numeric = false) {
 #pragma clang fp contract(off)
     if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
     if (npts <= 0 || nt <= 0 || !x || !t || !u_out) return fail(EBEM_EINVAL, "bad post-processing arguments");
     if (!density && !h->have_alpha) return fail(EBEM_EINVAL, "no density: pass one or run ebem_mot_solve first");
+    if (numeric && !h->pp_nInt) return fail(EBEM_EINVAL, "numeric post-processing needs ebem_set_pp_rule first");
     const int N = h->N, nT = h->nT;
     const double dt = h->dt;
     std::vector<int> nhat(nt);
@@ -1147,8 +1151,13 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
                                d_perm.p);
             HIPC(hipGetLastError());
             int64_t mb = std::min<int64_t>((nitems + 255) / 256, (int64_t)h->CUs * 16);
-            hipLaunchKernelGGL(k_pp_M, dim3((unsigned)mb), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P, nitems,
-                               d_iq.p, d_off.p, d_ms.p, h->sort_items ? d_perm.p : nullptr, d_M.p);
+            if (numeric)
+                hipLaunchKernelGGL(k_pp_M_num, dim3((unsigned)mb), dim3(256), sizeof(double) * 4 * h->pp_nInt,
+                                   h->stream, h->d_tri.p, h->d_verts.p, d_x.p, p0, P, h->d_pp_rule.p, h->pp_nInt,
+                                   nitems, d_iq.p, d_off.p, d_ms.p, 4 * kPi * h->rho, d_M.p);
+            else
+                hipLaunchKernelGGL(k_pp_M, dim3((unsigned)mb), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P,
+                                   nitems, d_iq.p, d_off.p, d_ms.p, h->sort_items ? d_perm.p : nullptr, d_M.p);
             HIPC(hipGetLastError());
             hipLaunchKernelGGL(k_pp_conv, dim3(np), dim3(128 * kConvSlices), 0, h->stream, d_M.p, d_off.p, d_ms.p,
                                d_me.p, d_aT.p, N, nT, p0, d_nh.p, d_kidx.p, (int)nh.size(), npts, d_u.p);
@@ -1447,6 +1456,30 @@ int ebem_pp_eval(ebem_handle* h, const double* density, int npts, const double* 
     std::lock_guard<std::mutex> lk(h->lock);
     HIPC(hipSetDevice(h->device));
     return pp_eval(h, density, npts, x, nt, t, u_out);
+}
+
+int ebem_set_pp_rule(ebem_handle* h, int nSub, int nSing) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!(nSub == 1 || nSub == 4 || nSub == 16 || nSub == 64) ||
+        !(nSing == 1 || nSing == 3 || nSing == 7 || nSing == 12 || nSing == 19))
+        return fail(EBEM_EINVAL, "Valori nodi interni non validi (%d sub-regions x %d nodes)", nSub, nSing);
+    HIPC(hipSetDevice(h->device));
+    std::vector<double> r;
+    if (gh_composite(nSub, nSing, r)) return fail(EBEM_EINVAL, "rule construction failed");
+    HIPC(h->d_pp_rule.alloc(r.size()));
+    HIPC(hipMemcpyAsync(h->d_pp_rule.p, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    h->pp_nInt = nSub * nSing;
+    return 0;
+}
+
+int ebem_pp_eval_numeric(ebem_handle* h, const double* density, int npts, const double* x, int nt, const double* t,
+                         double* u_out) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    HIPC(hipSetDevice(h->device));
+    return pp_eval(h, density, npts, x, nt, t, u_out, true);
 }
 
 int ebem_timings(ebem_handle* h, double* out8) {

@@ -139,6 +139,16 @@ int ebem_mot_solve(ebem_handle *h, const double *beta, double *alpha_out);
 int ebem_pp_eval(ebem_handle *h, const double *density, int npts, const double *x, int nt, const double *t,
                  double *u_out);
 
+/* Numeric post-processing of the reference's GPU path
+ * (BEMenerg_postProcGPU_setupLayer.m / _calcLayer.m, CUDAkernels/kernelLayer.cu;
+ * kernelSP.cu is the same sum for one point): the M(x, tau) of each triangle
+ * by the composite rule GaussHammerComposite(nSub, nSing) of methodInfo
+ * numSubRegionPP / numNodiSingPP, then the same time sum and layout as
+ * ebem_pp_eval. */
+int ebem_set_pp_rule(ebem_handle *h, int nSub, int nSing);
+int ebem_pp_eval_numeric(ebem_handle *h, const double *density, int npts, const double *x, int nt, const double *t,
+                         double *u_out);
+
 /* Unit hook for the analytic inner integration (functions/BEMenerg_analiticIntg via calcMatrixSubBlock.m):
  * the kernel-combined, /(4 pi rho)-scaled 3x3 integral (column-major) over
  * triangle `tri` of the mesh for outer point x at time t. */

@@ -1428,3 +1428,56 @@ int orc_ghc_matrix_block_rows(const orc_param *p, const orc_mesh *m, int nGH, co
     return 0;
 }
 
+/* Numeric post-processing of the reference's GPU path
+ * (BEMenerg_postProcGPU/CUDAkernels/kernelLayer.cu:23-103): M(x, tau) of one
+ * triangle by the rule (nInt nodes, barycentric b0..b2 and weight), divided by
+ * 4 pi rho, entries below 1e-14 zeroed; zero when c_P tau < |x - c_F| - maxL_F
+ * (isBlockNull).  out column-major. */
+void orc_pp_block_num(const orc_param *p, const orc_mesh *m, int F, const double *x, double tau, int nInt,
+                      const double *rn, const double *rw, double *out) {
+    memset(out, 0, 9 * sizeof(double));
+    const double *c = m->center + 3 * F;
+    double v[3] = {x[0] - c[0], x[1] - c[1], x[2] - c[2]};
+    double dmin = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) - m->maxL[F];
+    if (tau * p->velP < dmin) return;
+    double T[9];
+    tri_verts(m, F, T);
+    double acc[9];
+    memset(acc, 0, sizeof acc);
+    for (int k = 0; k < nInt; k++) {
+        const double *b = rn + 3 * k;
+        double y[3], d[3];
+        for (int e = 0; e < 3; e++) y[e] = b[0] * T[e] + b[1] * T[3 + e] + b[2] * T[6 + e];
+        for (int e = 0; e < 3; e++) d[e] = y[e] - x[e];
+        double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double nu[9];
+        orc_nucleo(d, r, tau, p->velP, p->velS, nu);
+        double w = rw[k] * m->area[F];
+        for (int e = 0; e < 9; e++) acc[e] += w * nu[e];
+    }
+    for (int e = 0; e < 9; e++) {
+        double val = acc[e] / (4 * PI * p->rho);
+        out[e] = fabs(val) < 1.0e-14 ? 0 : val;
+    }
+}
+
+/* u(x, t) of the numeric post-processing: sum_{c=0}^{nHat} M(x, t - c dt)
+ * (alpha_c - alpha_{c-1}) (kernelLayer.cu:85-103, postProcGPU_calcLayer.m). */
+void orc_pp_point_num(const orc_param *p, const orc_mesh *m, const double *density, const double *x, double t,
+                      int nInt, const double *rn, const double *rw, double *u) {
+    u[0] = u[1] = u[2] = 0;
+    if (t <= 0) return;
+    double dt = p->Tfin / p->nT;
+    int nHat = (int)ceil(t / dt) - 1;
+    long n3 = 3L * m->N;
+    for (int j = 0; j < m->N; j++)
+        for (int c = 0; c <= nHat; c++) {
+            double M[9];
+            orc_pp_block_num(p, m, j, x, t - c * dt, nInt, rn, rw, M);
+            const double *ap = density + n3 * c + 3 * j;
+            double d[3];
+            for (int e = 0; e < 3; e++) d[e] = ap[e] - (c > 0 ? ap[e - n3] : 0.0);
+            for (int r = 0; r < 3; r++) u[r] += M[r] * d[0] + M[r + 3] * d[1] + M[r + 6] * d[2];
+        }
+}
+

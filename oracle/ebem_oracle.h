@@ -130,6 +130,13 @@ int orc_ghc_matrix_block_rows(const orc_param *p, const orc_mesh *m, int nGH, co
                               int nInt, const double *ghcn, const double *ghcw, int nDiag, const double *dgn,
                               const double *dgw, int lag, int r0, int r1, double *out, int nthreads);
 
+/* Numeric post-processing (BEMenerg_postProcGPU/CUDAkernels/kernelLayer.cu):
+ * one triangle's M(x, tau) by the rule rn (nInt x 3) / rw, and u(x, t). */
+void orc_pp_block_num(const orc_param *p, const orc_mesh *m, int F, const double *x, double tau, int nInt,
+                      const double *rn, const double *rw, double *out);
+void orc_pp_point_num(const orc_param *p, const orc_mesh *m, const double *density, const double *x, double t,
+                      int nInt, const double *rn, const double *rw, double *u);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,10 @@ class Oracle:
         L.orc_pp_point.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), _dp, _dp, C.c_double, _dp]
         L.orc_counters.argtypes = [C.POINTER(C.c_longlong)]
         L.orc_field_record.argtypes = [_dp, _dp, _dp]
+        L.orc_pp_block_num.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, C.c_double, C.c_int,
+                                       _dp, _dp, _dp]
+        L.orc_pp_point_num.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), _dp, _dp, C.c_double, C.c_int, _dp,
+                                       _dp, _dp]
         L.orc_gh_composite.argtypes = [C.c_int, C.c_int, _dp, _dp]
         L.orc_gh_composite.restype = C.c_int
         L.orc_doppio_gauss.argtypes = [C.c_int, _dp, _dp]
@@ -224,4 +228,23 @@ class Oracle:
                                            self.dgw.size, _p(self.dgn), _p(self.dgw), int(lag), r0, r1,
                                            out.ctypes.data_as(_dp), nt)
         return np.ascontiguousarray(out)
+
+    def set_pp_rule(self, nSub=64, nSing=3):
+        """Rule of the numeric post-processing (PPn, PPw)."""
+        self.ppn, self.ppw = self.gh_composite(nSub, nSing)
+
+    def pp_block_num(self, x, j, tau):
+        out = np.zeros(9)
+        self.lib.orc_pp_block_num(C.byref(self.param), C.byref(self.mesh), int(j),
+                                  _p(np.ascontiguousarray(x, np.float64)), float(tau), self.ppw.size, _p(self.ppn),
+                                  _p(self.ppw), _p(out))
+        return out.reshape(3, 3).T.copy()
+
+    def pp_point_num(self, density, x, t):
+        d = np.asfortranarray(density)
+        u = np.zeros(3)
+        self.lib.orc_pp_point_num(C.byref(self.param), C.byref(self.mesh), d.ctypes.data_as(_dp),
+                                  _p(np.ascontiguousarray(x, np.float64)), float(t), self.ppw.size, _p(self.ppn),
+                                  _p(self.ppw), _p(u))
+        return u
 

@@ -79,3 +79,24 @@ def test_ghc_is_not_mixed_with_other_forms(screen2):
     with pytest.raises(_lib.EbemError):
         s.assemble(api.PREC_GHC | api.PREC_MIXED)
     s.close()
+
+
+def test_numeric_pp_matches_oracle(screen2):
+    from elastodynamicebem_amd import api
+    from oracle import Oracle
+    pb = api.PbParam(rho=1.0, mu=1.0, lambda_=1.0, Tfin=1.0, nT=16)
+    s = api.Solver(screen2, pb, 7)
+    s.set_pp_rule(64, 3)
+    o = Oracle(screen2.coordinates, screen2.triangles, Tfin=1.0, nT=16)
+    o.set_pp_rule(64, 3)
+    rng = np.random.default_rng(4)
+    dens = rng.normal(size=(3 * o.N, o.nT))
+    x = np.array([[0.1, -0.2, 0.3], [0.6, 0.2, 0.05], [-0.3, 0.4, 0.8], [0.0, 0.0, -0.2]])
+    t = np.array([0.1, 0.37, 0.5, 0.8125, 1.0])
+    u = s.pp_eval_numeric(x, t, dens)
+    for a in range(x.shape[0]):
+        for k in range(t.size):
+            ref = o.pp_point_num(dens, x[a], t[k])
+            sc = max(np.abs(ref).max(), 1e-300)
+            assert np.abs(u[:, a, k] - ref).max() <= 1e-10 * sc + 1e-13, (a, k, u[:, a, k], ref)
+    s.close()

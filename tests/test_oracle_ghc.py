@@ -122,3 +122,43 @@ def test_ghc_diagonal_is_close_to_analytic(orc):
             R, _ = orc.pair_block(lag, S, S)
             errs.append(np.abs(E - R).max() / np.abs(R).max())
     assert max(errs) < 5e-2, errs
+
+
+def test_numeric_pp_block_matches_analytic_for_smooth_cases(orc, screen2):
+    """Points and times for which the whole triangle lies strictly between the
+    S and the P sphere: the numeric M of the GPU post-processing reproduces the
+    analytic M of ppA."""
+    o = orc
+    o.set_pp_rule(64, 3)
+    V = o.coords[o.tri]
+    rng = np.random.default_rng(2)
+    found, worst = 0, 0.0
+    for _ in range(400):
+        x = rng.uniform(-1.0, 1.0, 3)
+        x[2] = rng.uniform(0.2, 1.0)
+        j = int(rng.integers(o.N))
+        d = np.linalg.norm(V[j] - x, axis=1)
+        lo, hi = d.min() * 0.97, d.max()
+        # S sphere inside the smallest distance, P sphere beyond the largest
+        t_lo, t_hi = hi / o.velP, lo / o.velS
+        if t_lo >= t_hi:
+            continue
+        tau = 0.5 * (t_lo + t_hi)
+        A = o.pp_block(x, j, tau)
+        B = o.pp_block_num(x, j, tau)
+        sc = np.abs(A).max()
+        if sc == 0:
+            continue
+        worst = max(worst, np.abs(A - B).max() / sc)
+        found += 1
+    assert found > 50 and worst < 2e-6, (found, worst)  # 64x3 quadrature error at ~h distance: 3.5e-7
+
+
+def test_numeric_pp_point_is_causal(orc, screen2):
+    o = orc
+    o.set_pp_rule(64, 3)
+    dens = np.random.default_rng(1).normal(size=(3 * o.N, o.nT))
+    x = np.array([0.0, 0.0, 1.5])
+    # the P front reaches the screen's nearest point (distance 1.5) at t = 1.5 / c_P
+    assert not np.any(o.pp_point_num(dens, x, 0.8 * 1.5 / o.velP))
+    assert np.any(o.pp_point_num(dens, x, 0.999))
