@@ -11,6 +11,8 @@ cited below; the computation runs in the native HIP library (libebem_hip.so).
   BEMenerg_dir_timeMarchingD    functions/BEMenerg_dir/BEMenerg_dir_timeMarchingD.m (single-layer system)
   BEMenerg_postProcA_setup      functions/BEMenerg_indir/BEMenerg_postProc/BEMenerg_postProcA/BEMenerg_postProcA_setup.m
   BEMenerg_ppD_setup            functions/BEMenerg_dirPP/BEMenerg_ppD_setup.m (single-layer part)
+  BEMenerg_postProcGPU_setupLayer / _setupSP
+                                functions/BEMenerg_indir/BEMenerg_postProc/BEMenerg_postProcGPU/*.m
 """
 from __future__ import annotations
 
@@ -71,6 +73,8 @@ class MethodInfo:
     numNodiSing: int = 0
     numNodiInt: int = 0
     numNodiDiag: int = 0
+    numSubRegionPP: int = 0
+    numNodiSingPP: int = 0
     extra: dict = field(default_factory=dict)
 
 
@@ -329,7 +333,50 @@ def BEMenerg_postProcA_setup(pbParam, domainMesh, density, numPoints, xVal, tVal
     return u.transpose(1, 2, 0)  # (xSize, tSize, 3)
 
 
+def _pp_rule_of(methodInfo, PPn):
+    nsub = getattr(methodInfo, "numSubRegionPP", 0) or methodInfo.extra.get("numSubRegionPP", 0)
+    nsing = getattr(methodInfo, "numNodiSingPP", 0) or methodInfo.extra.get("numNodiSingPP", 0)
+    if not nsub or not nsing:
+        if PPn is None:
+            raise ValueError("numeric post-processing needs numSubRegionPP / numNodiSingPP")
+        nsing = 3
+        nsub = int(np.asarray(PPn).shape[0]) // nsing
+    return int(nsub), int(nsing)
+
+
+def BEMenerg_postProcGPU_setupLayer(pbParam, domainMesh, density, numPoints, xVal, tVal, methodInfo, PPn=None,
+                                    PPw=None, *, device: int = 0):
+    """campoVett (xSize x tSize of 3-vectors) by the numeric GPU post-processing
+    (BEMenerg_postProcGPU_setupLayer.m / kernelLayer.cu; setupSP is the same
+    sum point by point): composite rule GaussHammerComposite(numSubRegionPP, numNodiSingPP)."""
+    xVal = np.asarray(xVal, float).reshape(-1, 3)
+    tVal = np.asarray(tVal, float).ravel()
+    if numPoints != xVal.shape[0] * tVal.size:
+        raise ValueError("numPoints does not match xVal x tVal")
+    nsub, nsing = _pp_rule_of(methodInfo, PPn)
+    with Solver(domainMesh, pbParam, 7, device) as s:
+        s.set_pp_rule(nsub, nsing)
+        u = s.pp_eval_numeric(xVal, tVal, density)
+    out = np.empty((xVal.shape[0], tVal.size), dtype=object)
+    for a in range(xVal.shape[0]):
+        for k in range(tVal.size):
+            out[a, k] = u[:, a, k].reshape(3, 1)
+    return out
+
+
+BEMenerg_postProcGPU_setupSP = BEMenerg_postProcGPU_setupLayer
+
+
 def BEMenerg_ppD_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, methodInfo=None, PPn=None, PPw=None,
                        *, device: int = 0):
-    """Single-layer part of the direct post-processing (BEMenerg_ppD_setup.m:1)."""
+    """Single-layer part G alpha of the direct post-processing (BEMenerg_ppD_setup.m:1,
+    BEMenerg_ppD_calc.m:96-124 with kernelG.cu): the numeric sum of the GPU
+    post-processing when methodInfo carries a post-processing rule, the
+    analytic one otherwise.  kernelG.cu applies its 1e-14 cleanup to
+    M(t - y dt) - M(t - (y+1) dt) rather than to each M, which changes u only
+    through entries below 1e-14.  The double-layer part P g (kernelP.cu) is not
+    part of this library."""
+    if methodInfo is not None and (getattr(methodInfo, "numSubRegionPP", 0) or PPn is not None):
+        return BEMenerg_postProcGPU_setupLayer(pbParam, domainMesh, density, numPoints, xVal, tVal, methodInfo,
+                                               PPn, PPw, device=device)
     return BEMenerg_postProcA_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, device=device)
