@@ -11,6 +11,7 @@ cited below; the computation runs in the native HIP library (libebem_hip.so).
   BEMenerg_dir_timeMarchingD    functions/BEMenerg_dir/BEMenerg_dir_timeMarchingD.m (single-layer system)
   BEMenerg_postProcA_setup      functions/BEMenerg_indir/BEMenerg_postProc/BEMenerg_postProcA/BEMenerg_postProcA_setup.m
   BEMenerg_ppD_setup            functions/BEMenerg_dirPP/BEMenerg_ppD_setup.m (single-layer part)
+  BEMenerg_postProc_loadPoints   functions/BEMenerg_indir/BEMenerg_postProc/BEMenerg_postProc_loadPoints.m
   BEMenerg_postProcGPU_setupLayer / _setupSP
                                 functions/BEMenerg_indir/BEMenerg_postProc/BEMenerg_postProcGPU/*.m
 """
@@ -342,6 +343,46 @@ def _pp_rule_of(methodInfo, PPn):
         nsing = 3
         nsub = int(np.asarray(PPn).shape[0]) // nsing
     return int(nsub), int(nsing)
+
+
+def BEMenerg_postProc_loadPoints(pbParam: PbParam):
+    """(xVal, tVal, iVal, numPoints, typePlot): the post-processing point and time
+    sets of each domain type (BEMenerg_postProc_loadPoints.m:1-71).  Grids are
+    flattened in MATLAB's column-major order."""
+    d, T = pbParam.domainType, pbParam.Tfin
+    empty = np.zeros((0, 3))
+    typePlot = ""
+    if d in ("screenTest", "screenGraded", "sphereUniform", "sphereNotUniform"):
+        xVal, tVal, iVal = empty, np.zeros(0), []
+    elif d == "screenUniform":
+        xVal, tVal, iVal = empty, np.zeros(0), [3]
+    elif d == "barH1":
+        xVal = np.array([[0, 0, 0.25], [0, 0, 0.50], [0, 0, 0.75]])
+        tVal = np.linspace(0, T, int(np.floor(100 * T)))
+        iVal, typePlot = [3], "u(x, :)"
+    elif d == "barH3":
+        xVal = np.array([[0.25, -0.25, 2]])
+        tVal = np.linspace(0, T, int(np.floor(100 * T)))
+        iVal, typePlot = [3], "u(x, :)"
+    elif d == "sphereWave":
+        c = np.linspace(-5, 5, 250)
+        X2D, Y2D = np.meshgrid(c, c)
+        X, Y = X2D.ravel(order="F"), Y2D.ravel(order="F")
+        keep = X ** 2 + Y ** 2 > 1
+        xVal = np.stack([X[keep], Y[keep], np.zeros(int(keep.sum()))], axis=1)
+        tVal = np.array([1.2676, 1.5211, 1.7746, 2.0282, 2.2817, 2.5352, 2.7887, 3.0423])
+        iVal, typePlot = [1, 2], "u(:, t)"
+    elif d == "elementoIndustriale":
+        Y2D, Z2D = np.meshgrid(np.linspace(-0.2, 0.2, 250), np.linspace(-0.4, 0.0, 250))
+        Y, Z = Y2D.ravel(order="F"), Z2D.ravel(order="F")
+        keep = ~((np.abs(Y) > 0.17) & (Z < -0.01))
+        xVal = np.stack([np.zeros(int(keep.sum())), Y[keep], Z[keep]], axis=1)
+        tVal = np.array([0.0005, 0.0010, 0.0015, 0.0020, 0.0025, 0.0030, 0.0035, 0.0040, 0.0045, 0.0050, 0.0055,
+                         0.0065, 0.0070, 0.0075])
+        iVal, typePlot = [1, 2, 3], "u(:, t)"
+    else:
+        raise ValueError("Caso ancora non riportato/Errore nei dati")
+    return xVal, tVal, iVal, xVal.shape[0] * tVal.size, typePlot
 
 
 def BEMenerg_postProcGPU_setupLayer(pbParam, domainMesh, density, numPoints, xVal, tVal, methodInfo, PPn=None,

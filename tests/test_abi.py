@@ -76,3 +76,18 @@ def test_read_input_file(tmp_path):
                  "%BIE\nEFIE\n%BOU\nDIR\n")
     pb = api.readInputFile(str(p))
     assert pb.nT == 128 and pb.domainType == "sphereUniform" and abs(pb.velP - 3 ** 0.5) < 1e-15
+
+
+def test_load_points_tables():
+    import numpy as np
+    from elastodynamicebem_amd import api
+    pb = api.PbParam(domainType="sphereWave", Tfin=4.0)
+    x, t, i, n, tp = api.BEMenerg_postProc_loadPoints(pb)
+    c = np.linspace(-5, 5, 250)
+    assert x.shape[0] == sum(1 for a in c for b in c if a * a + b * b > 1) and n == x.shape[0] * 8
+    # column-major order of meshgrid: the first column (x = -5) runs through every y first
+    assert np.all(x[:250, 0] == -5) and np.allclose(x[:250, 1], c) and tp == "u(:, t)"
+    x, t, i, n, tp = api.BEMenerg_postProc_loadPoints(api.PbParam(domainType="barH1", Tfin=2.0))
+    assert x.shape == (3, 3) and t.size == 200 and t[-1] == 2.0 and i == [3]
+    x, t, i, n, tp = api.BEMenerg_postProc_loadPoints(api.PbParam(domainType="screenUniform"))
+    assert n == 0 and i == [3]
