@@ -35,6 +35,10 @@ sys.path.insert(0, ROOT)
 # build of the device math (python tests/opcount.py 200000 -> 9325.5 +- 7.4);
 # each + - * / (a fused multiply-add as 2), sqrt and libm call counts once.
 ALG_FLOP_PER_ITEM = 9325.5
+# HBM bytes per launch of k_levels on this workload from the rocprofv3 --pmc
+# passes FETCH_SIZE (x2.0: the counter reads 512 MiB for a 1 GiB device copy)
+# and WRITE_SIZE (x1.0): 158.3 GB + 74.4 GB, almost all register-spill scratch.
+K_LEVELS_TRAFFIC_BYTES = 232.7e9
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -57,11 +61,26 @@ def shell_points(n=4096, seed=0):
 
 
 def history_bytes(per_lag, nT):
-    """Algorithmic bytes the nT history sums stream: 48 B of block values + 4 B
-    column index per stored block with 1 <= lag <= min(n, L-1), summed over steps
+    """Bytes of the nT history sums if every step streamed its blocks: 48 B of
+    values + 4 B column index per stored block with 1 <= lag <= min(n, L-1)
     (per_lag: stored blocks per lag)."""
     L = len(per_lag)
     blocks = sum(int(per_lag[1:min(n, L - 1) + 1].sum()) for n in range(nT))
+    return blocks * 52
+
+
+def history_bytes_blocked(per_lag, nT, B=8):
+    """Store bytes the blocked history sum actually streams: per block of B
+    steps starting at n0 > 0 one pass over lags 1..min(L-1, n0+B-1)
+    (k_history_far), plus lags 1..min(b, L-1) for step n0+b (k_history)."""
+    L = len(per_lag)
+    blocks = 0
+    for n0 in range(0, nT, B):
+        nb = min(B, nT - n0)
+        if n0 > 0:
+            blocks += int(per_lag[1:min(L - 1, n0 + nb - 1) + 1].sum())
+        for b in range(nb):
+            blocks += int(per_lag[1:min(b, L - 1) + 1].sum())
     return blocks * 52
 
 
@@ -207,14 +226,17 @@ def main():
     hist_ms = float(np.mean([t["history_ms"] for t in tims]))
     fac_ms = float(np.mean([t["factor_ms"] for t in tims]))
     pp_ms = float(np.mean([t["pp_ms"] for t in tims]))
-    hb = history_bytes(s.lag_blocks(), pb.nT)
+    per_lag = s.lag_blocks()
+    hb = history_bytes(per_lag, pb.nT)
+    hb_streamed = history_bytes_blocked(per_lag, pb.nT)
 
     roof = None
     if ALG_FLOP_PER_ITEM:
         ach = ALG_FLOP_PER_ITEM * info["items"] / (lev_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_levels", "achieved": ach,
-                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
-    hist_gbs = hb / (hist_ms * 1e-3) / 1e9
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
+                "traffic": K_LEVELS_TRAFFIC_BYTES}
+    hist_gbs = hb_streamed / (hist_ms * 1e-3) / 1e9
     out = {
         "metric": "element-pair·lag integrals/s + solve time at ≤1e-6 rel err vs FP64, one MI355X",
         "value": units / (ms_step * 1e-3),
@@ -229,8 +251,9 @@ def main():
                    "pair_lags": units, "items": info["items"], "store_bytes": info["bytes"]},
         "breakdown_ms": {"assembly": asm_ms, "assembly_integral_kernel": lev_ms, "mot_total": mot_ms,
                          "e0_factor": fac_ms, "history_sum": hist_ms, "post_processing": pp_ms},
-        "history_sum": {"bytes": hb, "achieved_GBs": hist_gbs, "peak_GBs": PEAK_HBM_GBS,
-                        "frac": hist_gbs / PEAK_HBM_GBS},
+        "history_sum": {"bytes_streamed": hb_streamed, "bytes_if_per_step": hb, "achieved_GBs": hist_gbs,
+                        "peak_GBs": PEAK_HBM_GBS, "frac": hist_gbs / PEAK_HBM_GBS,
+                        "note": "blocked over 8 steps: one store pass per block plus the short lags per step"},
         "roofline": roof,
         "cpu_baseline": None,
     }
