@@ -16,6 +16,6 @@ switch pbParam.domainType
     otherwise
         error('Caso ancora non riportato/Errore nei dati')
 end
-ebem_mex('rhs', datum, 3*N, pbParam.nT);
-density = ebem_mex('mot', [], 3*N, pbParam.nT);
+ebem_mex('rhs', datum);
+density = ebem_mex('mot', []);
 end

@@ -10,12 +10,15 @@
  *   ebem_mex('mesh', coordinates(Nv x 3), triangles(N x 3 or N x 4, 1-based))
  *   ebem_mex('problem', rho, mu, lambda, Tfin, nT)
  *   ebem_mex('rule', numNodiExt)
+ *   ebem_mex('inner_rule', numSubRegion, numNodiSing, numNodiDiag)   % method "FN ..."
+ *   ebem_mex('pp_rule', numSubRegionPP, numNodiSingPP)
  *   L       = ebem_mex('numblocks')
- *   ebem_mex('assemble', precision)              % 0 FP64, 1 mixed
+ *   ebem_mex('assemble', form)                   % 0 FP64, 1 mixed, +2 surrogate, 4 numeric (FN)
  *   E       = ebem_mex('block', lag)              % dense 3N x 3N
  *   beta    = ebem_mex('rhs', datum)              % 3N x nT
  *   density = ebem_mex('mot', beta_or_empty)      % 3N x nT
- *   u       = ebem_mex('pp', density_or_empty, xVal(k x 3), tVal)   % 3 x k x numel(tVal)
+ *   u       = ebem_mex('pp', density_or_empty, xVal(k x 3), tVal)          % 3 x k x numel(tVal)
+ *   u       = ebem_mex('pp_numeric', density_or_empty, xVal(k x 3), tVal)  % numeric GPU post-processing
  *   ebem_mex('free')
  * Every non-zero return code of the library becomes mexErrMsgIdAndTxt.
  */
@@ -25,6 +28,7 @@
 #include "mex.h"
 
 static ebem_handle *g_h = NULL;
+static mwSize g_n3 = 0, g_nT = 0; /* 3N of the mesh, nT of the problem */
 
 static void chk(int rc) {
     if (rc != 0) mexErrMsgIdAndTxt("ebem:lib", "%s (code %d)", ebem_last_error(), rc);
@@ -65,36 +69,33 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
         mxFree(xr);
         mxFree(tr);
         chk(rc);
+        g_n3 = 3 * N;
     } else if (!strcmp(cmd, "problem")) {
         chk(ebem_set_problem(g_h, scal(prhs[1]), scal(prhs[2]), scal(prhs[3]), scal(prhs[4]), (int)scal(prhs[5])));
+        g_nT = (mwSize)scal(prhs[5]);
     } else if (!strcmp(cmd, "rule")) {
         chk(ebem_set_rule(g_h, (int)scal(prhs[1])));
+    } else if (!strcmp(cmd, "inner_rule")) {
+        chk(ebem_set_inner_rule(g_h, (int)scal(prhs[1]), (int)scal(prhs[2]), (int)scal(prhs[3])));
+    } else if (!strcmp(cmd, "pp_rule")) {
+        chk(ebem_set_pp_rule(g_h, (int)scal(prhs[1]), (int)scal(prhs[2])));
     } else if (!strcmp(cmd, "numblocks")) {
         int L = 0;
         chk(ebem_num_blocks(g_h, &L));
         plhs[0] = mxCreateDoubleScalar(L);
     } else if (!strcmp(cmd, "assemble")) {
         chk(ebem_assemble(g_h, nrhs > 1 ? (int)scal(prhs[1]) : EBEM_PREC_FP64));
-    } else if (!strcmp(cmd, "block") || !strcmp(cmd, "rhs") || !strcmp(cmd, "mot")) {
-        int64_t nnzb;
-        chk(ebem_store_info(g_h, &nnzb, NULL, NULL, NULL));
-    }
-    (void)nlhs;
-    if (!strcmp(cmd, "block")) {
-        /* size from the mesh: ask the library for lag 0 dims via the MOT arrays */
-        mwSize n3 = (mwSize)scal(prhs[2]);
-        plhs[0] = mxCreateDoubleMatrix(n3, n3, mxREAL);
+    } else if (!strcmp(cmd, "block")) {
+        plhs[0] = mxCreateDoubleMatrix(g_n3, g_n3, mxREAL);
         chk(ebem_get_block(g_h, (int)scal(prhs[1]), mxGetDoubles(plhs[0])));
     } else if (!strcmp(cmd, "rhs")) {
-        mwSize n3 = (mwSize)scal(prhs[2]), nT = (mwSize)scal(prhs[3]);
-        plhs[0] = mxCreateDoubleMatrix(n3, nT, mxREAL);
+        plhs[0] = mxCreateDoubleMatrix(g_n3, g_nT, mxREAL);
         chk(ebem_rhs(g_h, (int)scal(prhs[1]), mxGetDoubles(plhs[0])));
     } else if (!strcmp(cmd, "mot")) {
-        mwSize n3 = (mwSize)scal(prhs[2]), nT = (mwSize)scal(prhs[3]);
-        plhs[0] = mxCreateDoubleMatrix(n3, nT, mxREAL);
+        plhs[0] = mxCreateDoubleMatrix(g_n3, g_nT, mxREAL);
         const double *beta = mxIsEmpty(prhs[1]) ? NULL : mxGetDoubles(prhs[1]);
         chk(ebem_mot_solve(g_h, beta, mxGetDoubles(plhs[0])));
-    } else if (!strcmp(cmd, "pp")) {
+    } else if (!strcmp(cmd, "pp") || !strcmp(cmd, "pp_numeric")) {
         const double *dens = mxIsEmpty(prhs[1]) ? NULL : mxGetDoubles(prhs[1]);
         mwSize k = mxGetM(prhs[2]), nt = mxGetNumberOfElements(prhs[3]);
         double *xv = mxGetDoubles(prhs[2]);
@@ -103,8 +104,16 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
             for (int c = 0; c < 3; c++) xr[3 * p + c] = xv[p + k * c];
         mwSize dims[3] = {3, k, nt};
         plhs[0] = mxCreateNumericArray(3, dims, mxDOUBLE_CLASS, mxREAL);
-        int rc = ebem_pp_eval(g_h, dens, (int)k, xr, (int)nt, mxGetDoubles(prhs[3]), mxGetDoubles(plhs[0]));
+        int rc = !strcmp(cmd, "pp")
+                     ? ebem_pp_eval(g_h, dens, (int)k, xr, (int)nt, mxGetDoubles(prhs[3]), mxGetDoubles(plhs[0]))
+                     : ebem_pp_eval_numeric(g_h, dens, (int)k, xr, (int)nt, mxGetDoubles(prhs[3]),
+                                            mxGetDoubles(plhs[0]));
         mxFree(xr);
         chk(rc);
+    } else if (strcmp(cmd, "mesh") && strcmp(cmd, "problem") && strcmp(cmd, "rule") && strcmp(cmd, "inner_rule") &&
+               strcmp(cmd, "pp_rule") && strcmp(cmd, "numblocks") && strcmp(cmd, "assemble") && strcmp(cmd, "block") &&
+               strcmp(cmd, "rhs") && strcmp(cmd, "mot")) {
+        mexErrMsgIdAndTxt("ebem:args", "unknown command '%s'", cmd);
     }
+    (void)nlhs;
 }
