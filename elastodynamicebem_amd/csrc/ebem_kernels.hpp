@@ -225,17 +225,24 @@ __global__ void __launch_bounds__(256) k_pair_items(const int16_t* __restrict__ 
 
 // Front-state key of an item: where the P and the S front of level m sit
 // relative to the span [d - e, d + e] of the pair (d centroid distance, e sum of
-// longest edges), each in 5 classes (not arrived, three thirds of the span,
-// passed).  Items with equal keys take the same region branches in most
-// outer-node evaluations, so sorting by key keeps wavefronts convergent.
-constexpr int kFrontKeys = 25;
-constexpr int kNumKeys = 2 * kFrontKeys + 1;  // far-field pairs (mixed precision) take keys 0..24
+// longest edges), each in kFrontBins + 2 classes (not arrived, kFrontBins equal
+// parts of the span, passed).  Items with equal keys take the same region
+// branches in most outer-node evaluations, so sorting by key keeps wavefronts
+// convergent.
+#ifndef EBEM_FRONT_BINS
+#define EBEM_FRONT_BINS 3
+#endif
+constexpr int kFrontBins = EBEM_FRONT_BINS;
+constexpr int kFrontClasses = kFrontBins + 2;
+constexpr int kFrontKeys = kFrontClasses * kFrontClasses;
+constexpr int kNumKeys = 2 * kFrontKeys + 1;  // far-field pairs (mixed precision) take keys 0..kFrontKeys-1
 constexpr int kSurKey = kNumKeys - 1;         // pairs predicted by the surrogate: the last key
+static_assert(kNumKeys <= 256, "keys are stored as uint8");
 
 __device__ inline int front_class(double r, double lo, double span) {
     double u = (r - lo) / span;
-    int q = (int)floor(u * 3.0);
-    return q < -1 ? -1 : (q > 3 ? 3 : q);
+    int q = (int)floor(u * (double)kFrontBins);
+    return q < -1 ? -1 : (q > kFrontBins ? kFrontBins : q);
 }
 
 // Pass 1 of the counting sort: block b keys its contiguous item range
@@ -264,7 +271,7 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
         double t = (double)m * P.dt;
         int qp = front_class(P.cP * t, d - ex, 2.0 * ex) + 1;
         int qs = front_class(P.cS * t, d - ex, 2.0 * ex) + 1;
-        int k = qp * 5 + qs + ((P.far2 >= 0.0 && d2 > P.far2) ? 0 : kFrontKeys);
+        int k = qp * kFrontClasses + qs + ((P.far2 >= 0.0 && d2 > P.far2) ? 0 : kFrontKeys);
         if (sflag && sflag[p]) k = kSurKey;
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
@@ -814,7 +821,7 @@ __global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tr
         double tau = P.delta + (double)m * P.dt;
         int qp = front_class(P.cP * tau, d - F.maxL, 2.0 * F.maxL) + 1;
         int qs = front_class(P.cS * tau, d - F.maxL, 2.0 * F.maxL) + 1;
-        int k = qp * 5 + qs + kFrontKeys;
+        int k = qp * kFrontClasses + qs + kFrontKeys;
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
     }
