@@ -15,13 +15,16 @@ namespace ebem {
 
 // kernelV.cu:188-258 (nucleo): the time-integrated elastodynamic kernel at
 // offset x, r = |x|, as the 6 entries (00, 01, 02, 11, 12, 22) of the
-// symmetric 3x3, same expressions and Heaviside products as the reference.
+// symmetric 3x3, same expressions and Heaviside products as the reference
+// (r^3 and r^5 as products instead of pow calls).
 __device__ __attribute__((always_inline)) inline void nucleo6(const double x[3], double r, double t, double cP,
                                                               double cS, double o[6]) {
 #pragma clang fp contract(off)
     const double HP = (t - (r / cP) > 0) ? 1.0 : 0.0;
     const double HS = (t - (r / cS) > 0) ? 1.0 : 0.0;
-    const double r3 = pow(r, 3.0), r5 = pow(r, 5.0);
+    // pow(r, 3) and pow(r, 5) of the reference as products (within an ulp of pow)
+    const double r2 = r * r;
+    const double r3 = r2 * r, r5 = r3 * r2;
     const double br = (cP * cP * t * t - r * r) / (cP * cP) * HP - (cS * cS * t * t - r * r) / (cS * cS) * HS;
     const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll

@@ -547,6 +547,17 @@ __global__ void __launch_bounds__(256) k_gemv_rows(const double* __restrict__ A,
     if (lane == 0) y[row] = s;
 }
 
+// y = A x for A column-major n x n: one thread per row, a column sweep
+// (consecutive threads read consecutive rows of each column).
+__global__ void __launch_bounds__(256) k_gemv_cols(const double* __restrict__ A, const double* __restrict__ x,
+                                                   double* __restrict__ y, int n) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < n; k++) s += A[(int64_t)k * n + row] * x[k];
+    y[row] = s;
+}
+
 // E_0 dense (col-major) from the lag-0 segments; one thread per block.
 __global__ void k_e0_dense(Store S, int N, int L, double* __restrict__ A) {
     int i = blockIdx.y;

@@ -177,3 +177,17 @@ def test_pp_after_mot_uses_device_density(screen2):
     u1 = s.pp_eval(x, t)          # device density of the MOT
     u2 = s.pp_eval(x, t, a)       # same density from the host
     assert np.array_equal(u1, u2)
+
+
+def test_e0_unpivoted_and_pivoted_factorisations_agree(sphere2, monkeypatch):
+    """E_0 is inverted by LU without pivoting when that passes the residual
+    check, else with partial pivoting; both give the same density."""
+    from elastodynamicebem_amd import api
+    s = _solver(sphere2, 4.0, 32)
+    s.assemble()
+    s.rhs(1)
+    a0 = s.mot_solve(None)
+    monkeypatch.setenv("EBEM_E0_PIVOT", "1")
+    a1 = s.mot_solve(None)
+    assert frob_rel(a0, a1) < 1e-12
+    s.close()

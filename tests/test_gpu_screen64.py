@@ -60,8 +60,8 @@ def test_screen64_rows_store_and_density(screen64):
         s.rhs(api.DATUM_SCREEN_SIN2)
         a = s.mot_solve(None)
         tm = s.timings()
-        per_lag = s.lag_blocks()
-        hist_bytes = 52 * sum(int(per_lag[1:min(n, 46) + 1].sum()) for n in range(64))
+        import bench
+        hist_bytes = bench.history_bytes_blocked(s.lag_blocks(), 64)
         assert np.all(np.isfinite(a)) and np.linalg.norm(a) > 0
         P = _rot_pairs(screen64)
         A = a.reshape(-1, 3, a.shape[1])
