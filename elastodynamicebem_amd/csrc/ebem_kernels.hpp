@@ -744,7 +744,10 @@ __global__ void __launch_bounds__(1024) k_pp_conv(const double* __restrict__ M, 
     const int r = blockIdx.x;
     const int p = p0 + r;
     const int lane_k = threadIdx.x % 128;
-    const int slice = threadIdx.x / 128;  // blockDim = 128 * kConvSlices
+    // the slice is uniform over a wave (128 threads per slice): the window and
+    // M loads below then have wave-uniform addresses (scalar loads), only the
+    // density loads and the accumulation are per lane
+    const int slice = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 128));  // blockDim = 128 * kConvSlices
     for (int kb = 0; kb < nk; kb += 128) {
         const int kk = kb + lane_k;
         const bool act = kk < nk;
@@ -753,12 +756,11 @@ __global__ void __launch_bounds__(1024) k_pp_conv(const double* __restrict__ M, 
         for (int j = slice; j < N; j += kConvSlices) {
             const int64_t q = (int64_t)r * N + j;
             const int a = ms[q], b = me[q];
-            if (a > b || a > nh) continue;
+            if (a > b) continue;
             const int64_t o = off[q];
             const double* aj = aT + (int64_t)j * nT * 3;
             double pv[6] = {0, 0, 0, 0, 0, 0};
-            const int mend = b < nh ? b : nh;
-            for (int m = a; m <= mend; m++) {
+            for (int m = a; m <= b; m++) {
                 const double* Mm = M + (o + (m - a)) * 6;
                 double d[6];
 #pragma unroll
@@ -766,11 +768,13 @@ __global__ void __launch_bounds__(1024) k_pp_conv(const double* __restrict__ M, 
                     d[k] = Mm[k] - pv[k];
                     pv[k] = Mm[k];
                 }
-                const double* al = aj + (int64_t)(nh - m) * 3;
-                const double x0 = al[0], x1 = al[1], x2 = al[2];
-                u0 += d[0] * x0 + d[1] * x1 + d[2] * x2;
-                u1 += d[1] * x0 + d[3] * x1 + d[4] * x2;
-                u2 += d[2] * x0 + d[4] * x1 + d[5] * x2;
+                if (m <= nh) {
+                    const double* al = aj + (int64_t)(nh - m) * 3;
+                    const double x0 = al[0], x1 = al[1], x2 = al[2];
+                    u0 += d[0] * x0 + d[1] * x1 + d[2] * x2;
+                    u1 += d[1] * x0 + d[3] * x1 + d[4] * x2;
+                    u2 += d[2] * x0 + d[4] * x1 + d[5] * x2;
+                }
             }
         }
         red[slice][lane_k][0] = u0;
