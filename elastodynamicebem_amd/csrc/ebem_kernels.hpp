@@ -547,6 +547,12 @@ __global__ void __launch_bounds__(256) k_gemv_rows(const double* __restrict__ A,
     if (lane == 0) y[row] = s;
 }
 
+// A[i + n i] = v for a zeroed n x n array (identity for v = 1).
+__global__ void k_set_diag(double* __restrict__ A, int n, double v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) A[(int64_t)i * n + i] = v;
+}
+
 // y = A x for A column-major n x n: one thread per row, a column sweep
 // (consecutive threads read consecutive rows of each column).
 __global__ void __launch_bounds__(256) k_gemv_cols(const double* __restrict__ A, const double* __restrict__ x,

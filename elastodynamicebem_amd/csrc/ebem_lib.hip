@@ -912,7 +912,7 @@ int rocb(rocblas_status s, const char* what) {
     return 0;
 }
 
-// Inverse of E_0 (rocSOLVER getrf + getri) into the row-major d_Ainv.
+// Inverse of E_0 (rocSOLVER getrf + getrs on the identity) into the row-major d_Ainv.
 // E_0 is dominated by its diagonal (self-interaction) blocks, so LU without
 // pivoting is tried first (its panels need a fraction of the kernel launches
 // of the pivoted factorisation: 21 vs 48 ms at 3840 unknowns); the inverse is
@@ -957,16 +957,19 @@ int factor_e0(ebem_handle* h) {
             if (!pivot) continue;
             return fail(EBEM_ESINGULAR, "E_0 is singular (zero pivot at %d)", (int)info);
         }
-        if (int e = rocb(rocsolver_dgetri(h->blas, n3, h->d_A.p, n3, h->d_ipiv.p, h->d_info.p), "rocsolver_dgetri"))
-            return e;
-        HIPC(hipMemcpyAsync(&info, h->d_info.p, sizeof(info), hipMemcpyDeviceToHost, h->stream));
-        HIPC(hipStreamSynchronize(h->stream));
-        if (info != 0) {
-            if (!pivot) continue;
-            return fail(EBEM_ESINGULAR, "E_0 inversion failed (info %d)", (int)info);
-        }
-        hipLaunchKernelGGL(k_transpose, grid, dim3(32, 8), 0, h->stream, h->d_A.p, h->d_Ainv.p, n3);
+        // inverse = LU solve of the identity (two triangular solves with n
+        // right-hand sides, rocBLAS trsm) into d_Ainv (column-major), then
+        // transposed to row-major through d_A
+        HIPC(hipMemsetAsync(h->d_Ainv.p, 0, sizeof(double) * (size_t)n3 * n3, h->stream));
+        hipLaunchKernelGGL(k_set_diag, dim3((n3 + 255) / 256), dim3(256), 0, h->stream, h->d_Ainv.p, n3, 1.0);
         HIPC(hipGetLastError());
+        if (int e = rocb(rocsolver_dgetrs(h->blas, rocblas_operation_none, n3, n3, h->d_A.p, n3, h->d_ipiv.p,
+                                          h->d_Ainv.p, n3),
+                         "rocsolver_dgetrs"))
+            return e;
+        hipLaunchKernelGGL(k_transpose, grid, dim3(32, 8), 0, h->stream, h->d_Ainv.p, h->d_A.p, n3);
+        HIPC(hipGetLastError());
+        std::swap(h->d_A.p, h->d_Ainv.p);
         if (pivot) {
             h->e0_pivoted = true;
             return 0;
