@@ -213,6 +213,12 @@ class Solver:
         _lib.check(self.lib.ebem_assembly_stats(self.h, C.byref(a), C.byref(b)))
         return dict(far_items=a.value, fp64_items=b.value)
 
+    def assembly_kp_stats(self) -> dict:
+        """kP ranges of the last assembly: units, levels covered, level-by-level fallbacks."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        _lib.check(self.lib.ebem_assembly_kp_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(ranges=a.value, levels=b.value, fallbacks=c.value)
+
     def surrogate_stats(self) -> dict:
         a, b, ms = C.c_int64(), C.c_int64(), C.c_double()
         _lib.check(self.lib.ebem_surrogate_stats(self.h, C.byref(a), C.byref(b), C.byref(ms)))

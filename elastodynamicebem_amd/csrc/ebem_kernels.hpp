@@ -44,6 +44,7 @@ struct AsmParams {
     double dt, cP, cS, rho;
     int nGH;
     double far2;  // squared centroid distance above which a pair is far field (mixed precision); <0: none
+    int kp;       // 1: levels of a pair in the kP state go through k_levels_kp (kp_range)
 };
 
 // (lag-1)*cS*dt < dmax && (lag+1)*cP*dt >= dmin  (calcMatrixBlock.m:61), same
@@ -128,6 +129,7 @@ __device__ inline void exact_window(const AsmParams& P, const double* __restrict
 __global__ void __launch_bounds__(256) k_windows(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
                                                  const double* __restrict__ verts, AsmParams P, int r0, int r1,
                                                  int16_t* __restrict__ lo, int16_t* __restrict__ hi,
+                                                 int16_t* __restrict__ ka, int16_t* __restrict__ kb,
                                                  unsigned long long* __restrict__ culled) {
     __shared__ unsigned long long part;
     if (threadIdx.x == 0) part = 0;
@@ -143,6 +145,13 @@ __global__ void __launch_bounds__(256) k_windows(const FieldTri* __restrict__ tr
         if (l <= h) {
             my += (unsigned long long)(h - l + 1);  // pair.lags of the reference's test
             exact_window(P, gh + (int64_t)i * 128, verts + (int64_t)j * 9, tri[j].center, l, h);
+        }
+        if (P.kp) {
+            int a = 1, b = 0;
+            if (l <= h)
+                kp_range(P.nGH, P.cP, P.cS, P.dt, gh + (int64_t)i * 128, tri[j], verts + (int64_t)j * 9, l, h, a, b);
+            ka[p] = (int16_t)a;
+            kb[p] = (int16_t)b;
         }
         lo[p] = (int16_t)l;
         hi[p] = (int16_t)h;
@@ -235,8 +244,14 @@ __global__ void __launch_bounds__(256) k_pair_items(const int16_t* __restrict__ 
 constexpr int kFrontBins = EBEM_FRONT_BINS;
 constexpr int kFrontClasses = kFrontBins + 2;
 constexpr int kFrontKeys = kFrontClasses * kFrontClasses;
-constexpr int kNumKeys = 2 * kFrontKeys + 1;  // far-field pairs (mixed precision) take keys 0..kFrontKeys-1
-constexpr int kSurKey = kNumKeys - 1;         // pairs predicted by the surrogate: the last key
+// Key order = order of the sorted permutation: far-field items (mixed
+// precision) 0..kFrontKeys-1, the other per-level items, the first level of
+// each kP range (one k_levels_kp item per pair), its other levels (no work),
+// surrogate pairs last.
+constexpr int kKpKey = 2 * kFrontKeys;
+constexpr int kKpRestKey = kKpKey + 1;
+constexpr int kSurKey = kKpKey + 2;
+constexpr int kNumKeys = kSurKey + 1;
 static_assert(kNumKeys <= 256, "keys are stored as uint8");
 
 __device__ inline int front_class(double r, double lo, double span) {
@@ -250,6 +265,7 @@ __device__ inline int front_class(double r, double lo, double span) {
 __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ tri, AsmParams P, int r0,
                                                    int64_t nitems, int64_t per, const int32_t* __restrict__ item_pair,
                                                    const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
+                                                   const int16_t* __restrict__ ka, const int16_t* __restrict__ kb,
                                                    const uint8_t* __restrict__ sflag, uint8_t* __restrict__ key,
                                                    unsigned int* __restrict__ bh) {
     __shared__ unsigned int sh[kNumKeys];
@@ -272,6 +288,7 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
         int qp = front_class(P.cP * t, d - ex, 2.0 * ex) + 1;
         int qs = front_class(P.cS * t, d - ex, 2.0 * ex) + 1;
         int k = qp * kFrontClasses + qs + ((P.far2 >= 0.0 && d2 > P.far2) ? 0 : kFrontKeys);
+        if (ka && m >= ka[p] && m <= kb[p]) k = (m == ka[p]) ? kKpKey : kKpRestKey;
         if (sflag && sflag[p]) k = kSurKey;
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
@@ -333,6 +350,70 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         double* o = W + it * 6;
 #pragma unroll
         for (int q = 0; q < 6; q++) o[q] = Wv[q];
+    }
+}
+
+// One thread per kP range (the items of key kKpKey, the first level ka of the
+// range): WP and WQ of node_eval_kp at the ends ka, kb, then
+// W_m = WP + t_m^2 WQ for m = ka..kb into the range's W slots (consecutive
+// items of the pair).  If some node is not in the assumed state, the range is
+// evaluated level by level as in k_levels.
+template <int NGH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_levels_kp(
+    const FieldTri* __restrict__ tri, const double* __restrict__ gh, AsmParams P, int r0, int64_t nitems,
+    const int32_t* __restrict__ item_pair, const int16_t* __restrict__ ka, const int16_t* __restrict__ kb,
+    const int32_t* __restrict__ perm, double* __restrict__ W, unsigned int* __restrict__ nfallback) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nitems;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t it = perm[k];
+        const int32_t p = item_pair[it];
+        const int i = r0 + p / P.N, j = p % P.N;
+        const int ma = ka[p], mb = kb[p];
+        const FieldTri& F = tri[j];
+        const double* g = gh + (int64_t)i * 4 * 32;
+        const Level KA = make_level((double)ma * P.dt, P.cP, P.cS);
+        const Level KB = make_level((double)mb * P.dt, P.cP, P.cS);
+        NodeAcc aP, aQ;
+        aP.s = aQ.s = 0.0;
+        for (int q = 0; q < 6; q++) aP.m[q] = aQ.m[q] = 0.0;
+        bool ok = true;
+#pragma unroll 1
+        for (int q = 0; q < NGH && ok; q++) {
+            double x[3] = {g[q], g[32 + q], g[64 + q]};
+            NodeGeo G = node_geo(x, F);
+            ok = node_eval_kp(G, KA, KB, g[96 + q], aP, aQ);
+        }
+        if (ok) {
+            double WP[6], WQ[6];
+            finish_rotation(aP, F.C, WP);
+            finish_rotation(aQ, F.C, WQ);
+            for (int m = ma; m <= mb; m++) {
+                const double t = (double)m * P.dt;
+                const double t2 = t * t;
+                double* o = W + (it + (m - ma)) * 6;
+#pragma unroll
+                for (int q = 0; q < 6; q++) o[q] = WP[q] + t2 * WQ[q];
+            }
+        } else {
+            atomicAdd(nfallback, 1u);
+            for (int m = ma; m <= mb; m++) {
+                Level K = make_level((double)m * P.dt, P.cP, P.cS);
+                NodeAcc acc;
+                acc.s = 0.0;
+                for (int q = 0; q < 6; q++) acc.m[q] = 0.0;
+#pragma unroll 1
+                for (int q = 0; q < NGH; q++) {
+                    double x[3] = {g[q], g[32 + q], g[64 + q]};
+                    NodeGeo G = node_geo(x, F);
+                    node_eval<double>(G, K, g[96 + q], acc);
+                }
+                double Wv[6];
+                finish_rotation(acc, F.C, Wv);
+                double* o = W + (it + (m - ma)) * 6;
+#pragma unroll
+                for (int q = 0; q < 6; q++) o[q] = Wv[q];
+            }
+        }
     }
 }
 

@@ -417,6 +417,8 @@ struct ebem_handle {
     bool e0_pivoted = false;  // the last factorisation needed partial pivoting
     double tim[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool sort_items = true;
+    bool kp_split = true;  // kP ranges through k_levels_kp (EBEM_KP_SPLIT=0 turns it off)
+    int64_t kp_ranges = 0, kp_levels = 0, kp_fallbacks = 0;
     int64_t far_items = 0;
     int precision = 0;
 
@@ -712,6 +714,10 @@ int assemble(ebem_handle* h, int precision) {
         for (int t = 0; t < N; t++) hmax = std::max(hmax, h->tri[t].maxL);
         P.far2 = mixed ? (3.0 * hmax) * (3.0 * hmax) : -1.0;
     }
+    // kP ranges need the key-sorted item order
+    const bool kp = !ghc && h->sort_items && h->kp_split;
+    P.kp = kp ? 1 : 0;
+    h->kp_ranges = h->kp_levels = h->kp_fallbacks = 0;
     const double near2 = (3.0 * h->hmax) * (3.0 * h->hmax);  // surrogate pairs: centroid distance <= 3h
     Timer tall, tlev;
     double lev_ms = 0;
@@ -729,16 +735,23 @@ int assemble(ebem_handle* h, int precision) {
         HIPC(hipMemsetAsync(d_nsb.p, 0, sizeof(unsigned long long), h->stream));
     }
     Timer tsur;
-    DevBuf<int16_t> d_lo, d_hi;
+    DevBuf<int16_t> d_lo, d_hi, d_ka, d_kb;
     HIPC(d_lo.alloc(np));
     HIPC(d_hi.alloc(np));
+    if (kp) {
+        HIPC(d_ka.alloc(np));
+        HIPC(d_kb.alloc(np));
+    }
     DevBuf<unsigned long long> d_culled;
     HIPC(d_culled.alloc(1));
     HIPC(hipMemsetAsync(d_culled.p, 0, sizeof(unsigned long long), h->stream));
+    DevBuf<unsigned int> d_nfb;
+    HIPC(d_nfb.alloc(1));
+    HIPC(hipMemsetAsync(d_nfb.p, 0, sizeof(unsigned int), h->stream));
     {
         int64_t blocks = std::min<int64_t>((np + 255) / 256, (int64_t)h->CUs * 64);
         hipLaunchKernelGGL(k_windows, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p,
-                           h->d_verts.p, P, 0, N, d_lo.p, d_hi.p, d_culled.p);
+                           h->d_verts.p, P, 0, N, d_lo.p, d_hi.p, d_ka.p, d_kb.p, d_culled.p);
         HIPC(hipGetLastError());
     }
     unsigned long long culled = 0;
@@ -825,8 +838,10 @@ int assemble(ebem_handle* h, int precision) {
         if (acc > 0) {
             // bucket items by front-state key (counting sort) for convergent wavefronts
             const int64_t per = (acc + G - 1) / G;
+            const int16_t* ka = kp ? d_ka.p + (int64_t)r0 * N : nullptr;
+            const int16_t* kb = kp ? d_kb.p + (int64_t)r0 * N : nullptr;
             hipLaunchKernelGGL(k_item_keys, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, P, r0, acc, per, d_ip.p,
-                               d_off.p, lo, sflag, d_key.p, d_hist.p);
+                               d_off.p, lo, ka, kb, sflag, d_key.p, d_hist.p);
             HIPC(hipGetLastError());
             HIPC(hipMemcpyAsync(hist.data(), d_hist.p, sizeof(unsigned int) * hist.size(), hipMemcpyDeviceToHost,
                                 h->stream));
@@ -842,12 +857,19 @@ int assemble(ebem_handle* h, int precision) {
                                d_perm.p);
             HIPC(hipGetLastError());
             const int32_t* perm = (h->sort_items || mixed || sur) ? d_perm.p : nullptr;
-            int64_t nfar = 0, nsur = 0;
+            int64_t nfar = 0, nsur = 0, nkp = 0, nkrest = 0;
             for (size_t k = 0; k < (size_t)kFrontKeys * G; k++) nfar += hist[k];
-            for (int b = 0; b < G; b++) nsur += hist[(size_t)kSurKey * G + b];
+            for (int b = 0; b < G; b++) {
+                nsur += hist[(size_t)kSurKey * G + b];
+                nkp += hist[(size_t)kKpKey * G + b];
+                nkrest += hist[(size_t)kKpRestKey * G + b];
+            }
             h->far_items += nfar;
             h->sur_items += nsur;
-            const int64_t nlev = acc - nsur;  // the surrogate items are the suffix of perm
+            h->kp_ranges += nkp;
+            h->kp_levels += nkp + nkrest;
+            // perm: [per-level items | kP range heads | other kP levels | surrogate items]
+            const int64_t nlev = acc - nsur - nkp - nkrest;
             tlev.start(h->stream);
             if (ghc) {
                 GhcRules R{h->d_ghc_in.p, h->d_ghc_dg.p, h->ghc_nSub * h->ghc_nSing, h->ghc_nDiag};
@@ -876,11 +898,25 @@ int assemble(ebem_handle* h, int precision) {
                 default: launch_levels_split<19>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
             }
             HIPC(hipGetLastError());
+            if (nkp > 0) {
+                const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((nkp + 255) / 256, (int64_t)h->CUs * 16));
+#define KP_LAUNCH(NG)                                                                                              \
+    hipLaunchKernelGGL(k_levels_kp<NG>, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, \
+                       r0, nkp, d_ip.p, ka, kb, d_perm.p + nlev, d_W.p, d_nfb.p)
+                switch (h->nGH) {
+                    case 3: KP_LAUNCH(3); break;
+                    case 7: KP_LAUNCH(7); break;
+                    case 12: KP_LAUNCH(12); break;
+                    default: KP_LAUNCH(19); break;
+                }
+#undef KP_LAUNCH
+                HIPC(hipGetLastError());
+            }
             tlev.stop(h->stream);
             lev_ms += tlev.ms();
             if (nsur > 0) {
                 tsur.start(h->stream);
-                if (int r = sur_assemble_items(h, sur, r0, nsur, d_perm.p + nlev, d_ip.p, d_off.p, lo, hi, d_W.p))
+                if (int r = sur_assemble_items(h, sur, r0, nsur, d_perm.p + (acc - nsur), d_ip.p, d_off.p, lo, hi, d_W.p))
                     return r;
                 tsur.stop(h->stream);
                 h->sur_ms += tsur.ms();
@@ -894,6 +930,12 @@ int assemble(ebem_handle* h, int precision) {
     tall.stop(h->stream);
     h->tim[1] = tall.ms();
     h->tim[0] = lev_ms;
+    {
+        unsigned int nfb = 0;
+        HIPC(hipMemcpyAsync(&nfb, d_nfb.p, sizeof nfb, hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        h->kp_fallbacks = nfb;
+    }
     if (sur) {
         unsigned long long nb = 0;
         HIPC(hipMemcpyAsync(&nb, d_nsb.p, sizeof nb, hipMemcpyDeviceToHost, h->stream));
@@ -1260,6 +1302,8 @@ int ebem_create(ebem_handle** out, int device) {
     // of bucketing them by front-state key (same results, for A/B timing).
     const char* order = std::getenv("EBEM_ITEM_ORDER");
     h->sort_items = !(order && std::strcmp(order, "pair") == 0);
+    const char* kps = std::getenv("EBEM_KP_SPLIT");
+    h->kp_split = !(kps && std::strcmp(kps, "0") == 0);
     *out = h;
     return 0;
 }
@@ -1418,6 +1462,16 @@ int ebem_store_lag_blocks(ebem_handle* h, int64_t* counts) {
     for (int l = 0; l < L; l++) counts[l] = 0;
     for (int i = 0; i < h->N; i++)
         for (int l = 0; l < L; l++) counts[l] += h->seg[(size_t)i * L + l + 1] - h->seg[(size_t)i * L + l];
+    return 0;
+}
+
+int ebem_assembly_kp_stats(ebem_handle* h, int64_t* ranges, int64_t* levels, int64_t* fallbacks) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
+    if (ranges) *ranges = h->kp_ranges;
+    if (levels) *levels = h->kp_levels;
+    if (fallbacks) *fallbacks = h->kp_fallbacks;
     return 0;
 }
 

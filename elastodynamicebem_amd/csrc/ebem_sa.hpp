@@ -576,6 +576,57 @@ __host__ __device__ inline int classify(double zeta, double rP, double rS, doubl
     return kNone;
 }
 
+// Child k of the fan around the projection (O, V_k, V_{k+1 mod 3}) in
+// counter-clockwise order (fieldTriangle.m:163-167); false for a degenerate
+// child (|doubled area| < 1e-6), which the reference skips.  Selects keep the
+// fan in registers.
+__host__ __device__ __attribute__((always_inline)) inline bool fan_child(const NodeGeo& G, int k, double& C,
+                                                                         double& v2x, double& v2y, double& v3x,
+                                                                         double& v3y) {
+    C = (k == 0) ? G.c[0] : ((k == 1) ? G.c[1] : G.c[2]);
+    if (fabs(C) < 1.0e-06) return false;
+    v2x = (k == 0) ? G.t[0][0] : ((k == 1) ? G.t[1][0] : G.t[2][0]);
+    v2y = (k == 0) ? G.t[0][1] : ((k == 1) ? G.t[1][1] : G.t[2][1]);
+    v3x = (k == 0) ? G.t[1][0] : ((k == 1) ? G.t[2][0] : G.t[0][0]);
+    v3y = (k == 0) ? G.t[1][1] : ((k == 1) ? G.t[2][1] : G.t[0][1]);
+    if (C < 0.0) {
+        double tx = v2x, ty = v2y;
+        v2x = v3x; v2y = v3y;
+        v3x = tx; v3y = ty;
+    }
+    return true;
+}
+
+// Folds one child's coefficients into the node accumulator: child matrix
+// sum_i b_i B_i (coeffB.m:136-160, symmetric, child frame) rotated by
+// A(theta0) * M * A(theta0)' (fieldTriangle.m:171-204), theta0 the clockwise
+// angle of V2 (cos = V2x/|V2|, sin = -V2y/|V2|), weighted by w sign(C).
+__host__ __device__ __attribute__((always_inline)) inline void add_child(const ChildAcc& A, const ChildGeo& g,
+                                                                         double v2x, double v2y, double zs,
+                                                                         double ws, NodeAcc& acc) {
+    double isg = 1.0 / g.sg;
+    double isg2 = isg * isg;
+    double m00 = (A.b[3] + 2.0 * g.cg * A.b[4] + g.cg * g.cg * A.b[5]) * isg2;
+    double m01 = (A.b[4] + g.cg * A.b[5]) * isg;
+    double m11 = A.b[5];
+    double m02 = -zs * (A.b[1] + g.cg * A.b[2]) * isg;
+    double m12 = -zs * A.b[2];
+    double m22 = zs * zs * A.b[0];
+    double ct = v2x / g.a, st = -v2y / g.a;
+    double r00 = ct * ct * m00 + 2.0 * ct * st * m01 + st * st * m11;
+    double r01 = -ct * st * m00 + (ct * ct - st * st) * m01 + ct * st * m11;
+    double r11 = st * st * m00 - 2.0 * ct * st * m01 + ct * ct * m11;
+    double r02 = ct * m02 + st * m12;
+    double r12 = -st * m02 + ct * m12;
+    acc.s += ws * A.s;
+    acc.m[0] += ws * r00;
+    acc.m[1] += ws * r01;
+    acc.m[2] += ws * r02;
+    acc.m[3] += ws * r11;
+    acc.m[4] += ws * r12;
+    acc.m[5] += ws * m22;
+}
+
 // One outer point at one level: adds w * (s, M_local) into acc.
 // Returns the integration class (0 when the point is skipped).
 template <typename R = double>
@@ -587,51 +638,117 @@ __host__ __device__ __attribute__((always_inline)) inline int node_eval(const No
     const double sgn = G.sgn;
 #pragma unroll 1
     for (int k = 0; k < 3; k++) {
-        // child k = (O, V_k, V_{k+1 mod 3}); selects keep the fan in registers
-        const double C = (k == 0) ? G.c[0] : ((k == 1) ? G.c[1] : G.c[2]);
-        if (fabs(C) < 1.0e-06) continue;
-        double v2x = (k == 0) ? G.t[0][0] : ((k == 1) ? G.t[1][0] : G.t[2][0]);
-        double v2y = (k == 0) ? G.t[0][1] : ((k == 1) ? G.t[1][1] : G.t[2][1]);
-        double v3x = (k == 0) ? G.t[1][0] : ((k == 1) ? G.t[2][0] : G.t[0][0]);
-        double v3y = (k == 0) ? G.t[1][1] : ((k == 1) ? G.t[2][1] : G.t[0][1]);
-        if (C < 0.0) {  // fieldTriangle.m:163-167: counter-clockwise order
-            double tx = v2x, ty = v2y;
-            v2x = v3x; v2y = v3y;
-            v3x = tx; v3y = ty;
-        }
+        double C, v2x, v2y, v3x, v3y;
+        if (!fan_child(G, k, C, v2x, v2y, v3x, v3y)) continue;
         ChildGeo g = child_geo(v2x, v2y, v3x, v3y);
         ChildAcc A;
         A.s = 0.0;
         for (int i = 0; i < 6; i++) A.b[i] = 0.0;
         child_regions<R>(g, zeta, K.rP, rhoP, K.rS, rhoS, type, K, A);
-        // Child matrix sum_i b_i B_i (coeffB.m:136-160), symmetric, child frame.
-        double isg = 1.0 / g.sg;
-        double isg2 = isg * isg;
-        double m00 = (A.b[3] + 2.0 * g.cg * A.b[4] + g.cg * g.cg * A.b[5]) * isg2;
-        double m01 = (A.b[4] + g.cg * A.b[5]) * isg;
-        double m11 = A.b[5];
-        double zs = zeta * sgn;
-        double m02 = -zs * (A.b[1] + g.cg * A.b[2]) * isg;
-        double m12 = -zs * A.b[2];
-        double m22 = zs * zs * A.b[0];
-        // A(theta0) * M * A(theta0)' (fieldTriangle.m:171-204) with theta0 the
-        // clockwise angle of V2: cos(theta0) = V2x/|V2|, sin(theta0) = -V2y/|V2|.
-        double ct = v2x / g.a, st = -v2y / g.a;
-        double r00 = ct * ct * m00 + 2.0 * ct * st * m01 + st * st * m11;
-        double r01 = -ct * st * m00 + (ct * ct - st * st) * m01 + ct * st * m11;
-        double r11 = st * st * m00 - 2.0 * ct * st * m01 + ct * ct * m11;
-        double r02 = ct * m02 + st * m12;
-        double r12 = -st * m02 + ct * m12;
-        double ws = w * msign(C);
-        acc.s += ws * A.s;
-        acc.m[0] += ws * r00;
-        acc.m[1] += ws * r01;
-        acc.m[2] += ws * r02;
-        acc.m[3] += ws * r11;
-        acc.m[4] += ws * r12;
-        acc.m[5] += ws * m22;
+        add_child(A, g, v2x, v2y, zeta * sgn, w * msign(C), acc);
     }
     return type;
+}
+
+// The same sum over the fan for an outer point that stays, over the levels
+// [KA.t, KB.t], in the class kP of classify() (S sphere short of the field
+// plane) with the P circle covering each child's far edge (TH2 = 0 and
+// TH3 = TH5 = gamma in child_regions).  Each child then has the one region
+// E7 over [0, gamma] (coeffG_TR_P.m), whose G coefficients do not depend on t,
+// and every accumulation is linear in the level weights, so the node adds
+// w (accP + t^2 accQ) at each of these levels: accP with the weights (k1, k2)
+// and k3 = k4 = 0, accQ with k1 = k2 = 0 and (k3, k4) / t^2 = (-1/2, 3/2).
+// The class is checked at both ends and the covering at KA (it only grows
+// with t); returns false, leaving accP and accQ untouched, if either fails.
+__host__ __device__ __attribute__((always_inline)) inline bool node_eval_kp(const NodeGeo& G, const Level& KA,
+                                                                            const Level& KB, double w,
+                                                                            NodeAcc& accP, NodeAcc& accQ) {
+    double rhoP, rhoS, rhoP2, rhoS2;
+    if (classify(G.zeta, KA.rP, KA.rS, G.dMIN, rhoP, rhoS) != kP) return false;
+    if (classify(G.zeta, KB.rP, KB.rS, G.dMIN, rhoP2, rhoS2) != kP) return false;
+    const double zeta = G.zeta;
+    const double zs = zeta * G.sgn;
+    Level KPw = KA, KQw = KA;
+    KPw.k3 = 0.0;
+    KPw.k4 = 0.0;
+    KQw.k1 = 0.0;
+    KQw.k2 = 0.0;
+    KQw.k3 = -0.5;
+    KQw.k4 = 1.5;
+    NodeAcc nP, nQ;
+    nP.s = nQ.s = 0.0;
+    for (int q = 0; q < 6; q++) nP.m[q] = nQ.m[q] = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < 3; k++) {
+        double C, v2x, v2y, v3x, v3y;
+        if (!fan_child(G, k, C, v2x, v2y, v3x, v3y)) continue;
+        ChildGeo g = child_geo(v2x, v2y, v3x, v3y);
+        // front cutting of child_regions for class kP at KA
+        const double a = g.a;
+        const double vt0 = (g.b * g.cg - a) / g.c;
+        const double Bm = a * vt0;
+        const double DeltaP = Bm * Bm - (a * a - rhoP * rhoP);
+        if (!(DeltaP > 1.0e-6)) return false;
+        const double as = asin(clamp1(a * g.sb / rhoP));
+        const double O1 = as - g.be, O2 = kPi - as - g.be;
+        if (fmin(fmax(0.0, fmin(O1, O2)), g.ga) != 0.0 || fmax(0.0, fmin(fmax(O1, O2), g.ga)) != g.ga) return false;
+        const Phi<double> P = make_phi<double>(0.0, g.ga, g);
+        ChildAcc AP, AQ;
+        AP.s = AQ.s = 0.0;
+        for (int i = 0; i < 6; i++) AP.b[i] = AQ.b[i] = 0.0;
+        region_edgeP<double>(false, KA.rS, -1.0, zeta, 0.0, g.ga, g, P, KPw, AP);
+        region_edgeP<double>(false, KA.rS, -1.0, zeta, 0.0, g.ga, g, P, KQw, AQ);
+        const double ws = w * msign(C);
+        add_child(AP, g, v2x, v2y, zs, ws, nP);
+        add_child(AQ, g, v2x, v2y, zs, ws, nQ);
+    }
+    accP.s += nP.s;
+    accQ.s += nQ.s;
+    for (int q = 0; q < 6; q++) {
+        accP.m[q] += nP.m[q];
+        accQ.m[q] += nQ.m[q];
+    }
+    return true;
+}
+
+// Levels [ka, kb] of a (test, field) pair whose W_m = WP + t_m^2 WQ
+// (node_eval_kp): every outer node q in class kP, c_S t_m < zeta_q (its
+// distance to the field plane, zeroed below 1e-6 as in node_geo), and the P
+// sphere past every field vertex, c_P t_m > Dmax.  Relative margins 1e-7 keep
+// the levels clear of both ends (node_eval_kp checks the state again).  Each
+// field edge must have (c/2)^2 > 1e-5: P covering an edge of length c gives
+// DeltaP >= (c/2)^2 in child_regions, clear of its 1e-6 threshold.  The range
+// is cut to the pair's item levels [max(1, l-1), h+1] and dropped below 3
+// levels.  gS: the test triangle's outer nodes as x, y, z rows of 32.
+__host__ __device__ inline void kp_range(int nGH, double cP, double cS, double dt, const double* gS,
+                                         const FieldTri& F, const double* vF, int l, int h, int& ka, int& kb) {
+    ka = 1;
+    kb = 0;
+    double cmin2 = 1e300, dmax2 = 0.0, zmin = 1e300;
+    for (int v = 0; v < 3; v++) {
+        const int w = (v + 1) % 3;
+        double a = vF[3 * w] - vF[3 * v], b = vF[3 * w + 1] - vF[3 * v + 1], c = vF[3 * w + 2] - vF[3 * v + 2];
+        cmin2 = fmin(cmin2, a * a + b * b + c * c);
+    }
+    if (!(0.25 * cmin2 > 1.0e-5)) return;
+    for (int q = 0; q < nGH; q++) {
+        const double x = gS[q], y = gS[32 + q], z = gS[64 + q];
+        for (int v = 0; v < 3; v++) {
+            double a = x - vF[3 * v], b = y - vF[3 * v + 1], c = z - vF[3 * v + 2];
+            dmax2 = fmax(dmax2, a * a + b * b + c * c);
+        }
+        double d0 = x - F.v0[0], d1 = y - F.v0[1], d2 = z - F.v0[2];
+        double zq = fabs(F.n[0] * d0 + F.n[1] * d1 + F.n[2] * d2);
+        zmin = fmin(zmin, zq < 1.0e-6 ? 0.0 : zq);
+    }
+    const double fa = ceil(sqrt(dmax2) * (1.0 + 1e-7) / (cP * dt));
+    const double fb = ceil(zmin * (1.0 - 1e-7) / (cS * dt)) - 1.0;
+    const int mlo = l - 1 < 1 ? 1 : l - 1;
+    const double a = fmax(fa, (double)mlo), b = fmin(fb, (double)(h + 1));
+    if (b - a + 1.0 >= 3.0) {
+        ka = (int)a;
+        kb = (int)b;
+    }
 }
 
 // W = s I + C' M C  (rotazioneInversaEulero.m:133), symmetric output

@@ -112,6 +112,16 @@ int ebem_store_lag_blocks(ebem_handle *h, int64_t *counts);
  * (the reference has no mixed precision); reports the split of the work. */
 int ebem_assembly_stats(ebem_handle *h, int64_t *far_items, int64_t *fp64_items);
 
+/* kP ranges of the last ebem_assemble: for a pair whose outer nodes all stay
+ * in the kP class of calcMatrixSubBlock.m:290-314 (S sphere short of the
+ * field plane, P sphere past the whole field triangle) over consecutive
+ * levels, W_m = WP + t_m^2 WQ with WP, WQ from one evaluation of the
+ * t-independent region coefficients, instead of one evaluation per level.
+ * ranges: such (pair, range) units; levels: the levels they cover;
+ * fallbacks: ranges evaluated level by level because a node failed the state
+ * check (0 expected).  No reference counterpart (reports the split). */
+int ebem_assembly_kp_stats(ebem_handle *h, int64_t *ranges, int64_t *levels, int64_t *fallbacks);
+
 /* Export E_lag as a dense 3N x 3N column-major host matrix (the `matrix` of
  * calcMatrixBlock.m / matrixSavedV{lag+1} of timeMarchingD.m:141-158). */
 int ebem_get_block(ebem_handle *h, int lag, double *out);

@@ -50,3 +50,35 @@ void sa_item(const double* rec, const double* gh, int nGH, double t, double cP, 
     finish_rotation(acc, F.C, W6);
 }
 }
+
+extern "C" {
+// kP range of a pair as k_windows computes it (kp_range): rec = field record,
+// gh = test nodes (x, y, z, w rows of 32), vF = field vertices (9), [l, h] the
+// pair's lag window.  Writes ka, kb (ka > kb: none).
+void sa_kp_range(const double* rec, const double* gh, const double* vF, int nGH, double cP, double cS, double dt,
+                 int l, int h, int* ka, int* kb) {
+    FieldTri F;
+    std::memcpy(&F, rec, sizeof(FieldTri));
+    kp_range(nGH, cP, cS, dt, gh, F, vF, l, h, *ka, *kb);
+}
+
+// WP, WQ of a kP range [ta, tb] as k_levels_kp computes them; returns 0 if a
+// node fails the state check (k_levels_kp then evaluates level by level).
+int sa_item_kp(const double* rec, const double* gh, int nGH, double ta, double tb, double cP, double cS,
+               double* WP6, double* WQ6) {
+    FieldTri F;
+    std::memcpy(&F, rec, sizeof(FieldTri));
+    Level KA = make_level(ta, cP, cS), KB = make_level(tb, cP, cS);
+    NodeAcc aP, aQ;
+    aP.s = aQ.s = 0.0;
+    for (int q = 0; q < 6; q++) aP.m[q] = aQ.m[q] = 0.0;
+    for (int q = 0; q < nGH; q++) {
+        double x[3] = {gh[q], gh[32 + q], gh[64 + q]};
+        NodeGeo G = node_geo(x, F);
+        if (!node_eval_kp(G, KA, KB, gh[96 + q], aP, aQ)) return 0;
+    }
+    finish_rotation(aP, F.C, WP6);
+    finish_rotation(aQ, F.C, WQ6);
+    return 1;
+}
+}

@@ -1,0 +1,65 @@
+"""kP ranges on the device: the assembly with k_levels_kp (default) against the
+same assembly with every level through k_levels (EBEM_KP_SPLIT=0), on the
+1280-triangle sphere, T = 4, 128 steps.  Every lag matrix agrees to 1e-12
+(Frobenius-relative, absolute against ||E_0|| for lags below 1e-6 ||E_0||);
+the ranges cover a large share of the items and no range falls back to the
+level-by-level evaluation.  Parity of the default path with the oracle is in
+test_gpu_parity.py."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+
+def _assemble(split):
+    from elastodynamicebem_amd import api
+    from elastodynamicebem_amd.mesh import read_mesh
+    m = read_mesh(os.path.join(ROOT, "tests", "golden", "meshes", "sphereUniform_3.mesh"))
+    pb = api.PbParam(rho=1.0, mu=1.0, lambda_=1.0, Tfin=4.0, nT=128)
+    old = os.environ.get("EBEM_KP_SPLIT")
+    os.environ["EBEM_KP_SPLIT"] = "1" if split else "0"
+    try:
+        s = api.Solver(m, pb, 7)
+    finally:
+        if old is None:
+            del os.environ["EBEM_KP_SPLIT"]
+        else:
+            os.environ["EBEM_KP_SPLIT"] = old
+    s.assemble(api.PREC_FP64)
+    return s
+
+
+def test_kp_split_matches_per_level_assembly():
+    a = _assemble(True)
+    b = _assemble(False)
+    try:
+        st = a.assembly_kp_stats()
+        stb = b.assembly_kp_stats()
+        info = a.store_info()
+        L = a.num_blocks()
+        assert L == b.num_blocks() and info == b.store_info()
+        ta, tb = a.timings()["levels_ms"], b.timings()["levels_ms"]
+        print(f"kP ranges {st['ranges']}, levels {st['levels']} of {info['items']} items "
+              f"({st['levels'] / info['items']:.1%}), fallbacks {st['fallbacks']}; "
+              f"integral kernels {ta:.1f} ms vs {tb:.1f} ms per level")
+        assert stb == dict(ranges=0, levels=0, fallbacks=0)
+        assert st["fallbacks"] == 0
+        assert st["levels"] > 0.15 * info["items"]
+        e0 = None
+        worst = 0.0
+        for lag in range(L):
+            G, R = a.get_block(lag), b.get_block(lag)
+            nr = np.linalg.norm(R)
+            if e0 is None:
+                e0 = nr
+            e = np.linalg.norm(G - R) / (nr if nr >= 1e-6 * e0 else e0)
+            worst = max(worst, e)
+            assert e < 1e-12, (lag, e)
+        print(f"worst lag difference {worst:.2e}")
+    finally:
+        a.close()
+        b.close()
