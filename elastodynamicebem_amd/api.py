@@ -219,6 +219,12 @@ class Solver:
         _lib.check(self.lib.ebem_assembly_kp_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return dict(ranges=a.value, levels=b.value, fallbacks=c.value)
 
+    def pp_kp_stats(self) -> dict:
+        """kP ranges of the last post-processing: units, items covered, per-m fallbacks."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        _lib.check(self.lib.ebem_pp_kp_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(ranges=a.value, levels=b.value, fallbacks=c.value)
+
     def surrogate_stats(self) -> dict:
         a, b, ms = C.c_int64(), C.c_int64(), C.c_double()
         _lib.check(self.lib.ebem_surrogate_stats(self.h, C.byref(a), C.byref(b), C.byref(ms)))

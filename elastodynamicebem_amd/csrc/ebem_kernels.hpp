@@ -707,7 +707,42 @@ struct PPParams {
     int mmax;
     double trep;  // representative time of the group (largest nhat)
     int nrep;     // its nhat: tau_m = trep - (nrep - m) dt, as singlePoint.m:37
+    int kp;       // 1: kP ranges of m through k_pp_M_kp (pp_kp_range)
 };
+
+__device__ inline double pp_tau(const PPParams& P, int m) {
+#pragma clang fp contract(off)
+    return (m == P.nrep) ? P.trep : P.trep - (double)(P.nrep - m) * P.dt;
+}
+
+// kP range [ka, kb] of m for point x and field triangle F within the window
+// [lo, hi] (kp_range with the single outer point x and tau_m ~ delta + m dt;
+// the 1e-7 margins cover the difference to pp_tau).  ka > kb: none.
+__device__ inline void pp_kp_range(const PPParams& P, double x0, double x1, double x2, const FieldTri& F,
+                                   const double* __restrict__ vF, int lo, int hi, int& ka, int& kb) {
+#pragma clang fp contract(off)
+    ka = 1;
+    kb = 0;
+    double cmin2 = 1e300, dmax2 = 0.0;
+    for (int v = 0; v < 3; v++) {
+        const int w = (v + 1) % 3;
+        double a = vF[3 * w] - vF[3 * v], b = vF[3 * w + 1] - vF[3 * v + 1], c = vF[3 * w + 2] - vF[3 * v + 2];
+        cmin2 = fmin(cmin2, a * a + b * b + c * c);
+        double d0 = x0 - vF[3 * v], d1 = x1 - vF[3 * v + 1], d2 = x2 - vF[3 * v + 2];
+        dmax2 = fmax(dmax2, d0 * d0 + d1 * d1 + d2 * d2);
+    }
+    if (!(0.25 * cmin2 > 1.0e-5)) return;
+    double d0 = x0 - F.v0[0], d1 = x1 - F.v0[1], d2 = x2 - F.v0[2];
+    double z = fabs(F.n[0] * d0 + F.n[1] * d1 + F.n[2] * d2);
+    if (z < 1.0e-6) return;
+    const double fa = ceil((sqrt(dmax2) * (1.0 + 1e-7) / P.cP - P.delta) / P.dt);
+    const double fb = ceil((z * (1.0 - 1e-7) / P.cS - P.delta) / P.dt) - 1.0;
+    const double a = fmax(fa, (double)lo), b = fmin(fb, (double)hi);
+    if (b - a + 1.0 >= 3.0) {
+        ka = (int)a;
+        kb = (int)b;
+    }
+}
 
 // Window [ms, me] of m per (point, triangle); count = me - ms + 1 (0 if empty).
 //  * M_m = 0 while c_P tau_m < Dlo = |x - c_F| - R_F (R_F the triangle's
@@ -720,7 +755,8 @@ struct PPParams {
 // Relative margins 1e-12 keep ties on the computed side.
 __global__ void __launch_bounds__(256) k_pp_windows(const FieldTri* __restrict__ tri, const double* __restrict__ verts,
                                                     const double* __restrict__ xs, int p0, int np, PPParams P,
-                                                    int32_t* __restrict__ ms, int32_t* __restrict__ me) {
+                                                    int32_t* __restrict__ ms, int32_t* __restrict__ me,
+                                                    int32_t* __restrict__ ka, int32_t* __restrict__ kb) {
     int64_t tot = (int64_t)np * P.N;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
         int p = p0 + (int)(q / P.N), j = (int)(q % P.N);
@@ -742,6 +778,12 @@ __global__ void __launch_bounds__(256) k_pp_windows(const FieldTri* __restrict__
         int hi = (int)fmax(-1.0, fmin(b, (double)P.mmax));
         ms[q] = lo;
         me[q] = hi;
+        if (P.kp) {
+            int a0 = 1, b0 = 0;
+            if (lo <= hi) pp_kp_range(P, x0, x1, x2, F, v, lo, hi, a0, b0);
+            ka[q] = a0;
+            kb[q] = b0;
+        }
     }
 }
 
@@ -792,11 +834,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         const int32_t q = item_q[it];
         const int p = p0 + q / P.N, j = q % P.N;
         const int m = ms[q] + (int)(it - off[q]);
-        double tau;
-        {
-#pragma clang fp contract(off)
-            tau = (m == P.nrep) ? P.trep : P.trep - (double)(P.nrep - m) * P.dt;
-        }
+        const double tau = pp_tau(P, m);
         double W[6] = {0, 0, 0, 0, 0, 0};
         if (tau > 0.0) {
             const FieldTri& F = tri[j];
@@ -813,6 +851,62 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         for (int k = 0; k < 6; k++) {
             double v = W[k] * P.inv4pirho;
             o[k] = fabs(v) < 1.0e-14 ? 0.0 : v;
+        }
+    }
+}
+
+// One thread per kP range of m (the items of key kKpKey, m = ka): WP, WQ of
+// node_eval_kp for the single point at tau_ka, tau_kb, then
+// M_m = (WP + tau_m^2 WQ) / (4 pi rho) with the 1e-14 cleanup for m = ka..kb
+// (consecutive items of the point-triangle pair).  A range whose state check
+// fails is evaluated per m as in k_pp_M.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_pp_M_kp(
+    const FieldTri* __restrict__ tri, const double* __restrict__ xs, int p0, PPParams P, int64_t nheads,
+    const int32_t* __restrict__ item_q, const int32_t* __restrict__ ka, const int32_t* __restrict__ kb,
+    const int32_t* __restrict__ perm, double* __restrict__ M, unsigned int* __restrict__ nfallback) {
+    for (int64_t kx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; kx < nheads;
+         kx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t it = perm[kx];
+        const int32_t q = item_q[it];
+        const int p = p0 + q / P.N, j = q % P.N;
+        const int ma = ka[q], mb = kb[q];
+        const FieldTri& F = tri[j];
+        double x[3] = {xs[3 * p], xs[3 * p + 1], xs[3 * p + 2]};
+        const NodeGeo G = node_geo(x, F);
+        const Level KA = make_level(pp_tau(P, ma), P.cP, P.cS);
+        const Level KB = make_level(pp_tau(P, mb), P.cP, P.cS);
+        NodeAcc aP, aQ;
+        aP.s = aQ.s = 0.0;
+        for (int k = 0; k < 6; k++) aP.m[k] = aQ.m[k] = 0.0;
+        const bool ok = node_eval_kp(G, KA, KB, 1.0, aP, aQ);
+        double WP[6], WQ[6];
+        if (ok) {
+            finish_rotation(aP, F.C, WP);
+            finish_rotation(aQ, F.C, WQ);
+        } else {
+            atomicAdd(nfallback, 1u);
+        }
+        for (int m = ma; m <= mb; m++) {
+            const double tau = pp_tau(P, m);
+            double W[6];
+            if (ok) {
+                const double t2 = tau * tau;
+#pragma unroll
+                for (int k = 0; k < 6; k++) W[k] = WP[k] + t2 * WQ[k];
+            } else {
+                for (int k = 0; k < 6; k++) W[k] = 0.0;
+                Level K = make_level(tau, P.cP, P.cS);
+                NodeAcc acc;
+                acc.s = 0.0;
+                for (int k = 0; k < 6; k++) acc.m[k] = 0.0;
+                if (node_eval(G, K, 1.0, acc) != kNone) finish_rotation(acc, F.C, W);
+            }
+            double* o = M + (it + (m - ma)) * 6;
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                double v = W[k] * P.inv4pirho;
+                o[k] = fabs(v) < 1.0e-14 ? 0.0 : v;
+            }
         }
     }
 }
@@ -906,7 +1000,8 @@ __global__ void __launch_bounds__(256) k_pp_counts(const int32_t* __restrict__ m
 __global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tri, const double* __restrict__ xs,
                                                  int p0, PPParams P, int64_t nitems, int64_t per,
                                                  const int32_t* __restrict__ item_q, const int64_t* __restrict__ off,
-                                                 const int32_t* __restrict__ ms, uint8_t* __restrict__ key,
+                                                 const int32_t* __restrict__ ms, const int32_t* __restrict__ ka,
+                                                 const int32_t* __restrict__ kb, uint8_t* __restrict__ key,
                                                  unsigned int* __restrict__ bh) {
     __shared__ unsigned int sh[kNumKeys];
     for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) sh[k] = 0;
@@ -924,6 +1019,7 @@ __global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tr
         int qp = front_class(P.cP * tau, d - F.maxL, 2.0 * F.maxL) + 1;
         int qs = front_class(P.cS * tau, d - F.maxL, 2.0 * F.maxL) + 1;
         int k = qp * kFrontClasses + qs + kFrontKeys;
+        if (ka && m >= ka[q] && m <= kb[q]) k = (m == ka[q]) ? kKpKey : kKpRestKey;
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
     }

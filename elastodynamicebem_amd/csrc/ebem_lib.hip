@@ -419,6 +419,7 @@ struct ebem_handle {
     bool sort_items = true;
     bool kp_split = true;  // kP ranges through k_levels_kp (EBEM_KP_SPLIT=0 turns it off)
     int64_t kp_ranges = 0, kp_levels = 0, kp_fallbacks = 0;
+    int64_t pp_kp_ranges = 0, pp_kp_levels = 0, pp_kp_fallbacks = 0;  // of the last post-processing
     int64_t far_items = 0;
     int precision = 0;
 
@@ -1163,13 +1164,23 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     std::sort(order.begin(), order.end(), [&](int a, int b) { return delta[a] < delta[b]; });
     const int P0 = std::min(npts, 1024);  // points per chunk
     const int G = h->CUs * 8;             // counting-sort blocks
-    DevBuf<int32_t> d_ms, d_me, d_iq, d_nh, d_kidx, d_perm;
+    // kP ranges of m through k_pp_M_kp (analytic form with key-sorted items)
+    const bool kp = !numeric && h->sort_items && h->kp_split;
+    DevBuf<int32_t> d_ms, d_me, d_iq, d_nh, d_kidx, d_perm, d_ka, d_kb;
+    DevBuf<unsigned int> d_nfb;
+    HIPC(d_nfb.alloc(1));
+    HIPC(hipMemsetAsync(d_nfb.p, 0, sizeof(unsigned int), h->stream));
     DevBuf<int64_t> d_off, d_rb, d_cnt;
     DevBuf<uint8_t> d_key;
     DevBuf<unsigned int> d_bh;
     DevBuf<unsigned long long> d_bo;
     HIPC(d_ms.alloc((size_t)P0 * N));
     HIPC(d_me.alloc((size_t)P0 * N));
+    if (kp) {
+        HIPC(d_ka.alloc((size_t)P0 * N));
+        HIPC(d_kb.alloc((size_t)P0 * N));
+    }
+    h->pp_kp_ranges = h->pp_kp_levels = 0;
     HIPC(d_off.alloc((size_t)P0 * N));
     HIPC(d_rb.alloc(P0));
     HIPC(d_cnt.alloc(P0));
@@ -1203,6 +1214,7 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
         P.mmax = mmax;
         P.trep = t[krep];
         P.nrep = mmax;
+        P.kp = kp ? 1 : 0;
         HIPC(d_nh.alloc(nh.size()));
         HIPC(d_kidx.alloc(kid.size()));
         HIPC(hipMemcpyAsync(d_nh.p, nh.data(), sizeof(int32_t) * nh.size(), hipMemcpyHostToDevice, h->stream));
@@ -1212,7 +1224,7 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             int64_t tot = (int64_t)np * N;
             int64_t blocks = std::min<int64_t>((tot + 255) / 256, (int64_t)h->CUs * 64);
             hipLaunchKernelGGL(k_pp_windows, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p,
-                               h->d_verts.p, d_x.p, p0, np, P, d_ms.p, d_me.p);
+                               h->d_verts.p, d_x.p, p0, np, P, d_ms.p, d_me.p, d_ka.p, d_kb.p);
             HIPC(hipGetLastError());
             hipLaunchKernelGGL(k_pp_counts, dim3(np), dim3(256), 0, h->stream, d_ms.p, d_me.p, N, d_cnt.p);
             HIPC(hipGetLastError());
@@ -1237,7 +1249,8 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             // bucket items by front-state key (two-pass counting sort)
             const int64_t per = (nitems + G - 1) / G;
             hipLaunchKernelGGL(k_pp_keys, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P, nitems, per,
-                               d_iq.p, d_off.p, d_ms.p, d_key.p, d_bh.p);
+                               d_iq.p, d_off.p, d_ms.p, kp ? d_ka.p : nullptr, kp ? d_kb.p : nullptr, d_key.p,
+                               d_bh.p);
             HIPC(hipGetLastError());
             HIPC(hipMemcpyAsync(bh.data(), d_bh.p, sizeof(unsigned int) * bh.size(), hipMemcpyDeviceToHost,
                                 h->stream));
@@ -1252,15 +1265,31 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             hipLaunchKernelGGL(k_item_scatter, dim3(G), dim3(256), 0, h->stream, nitems, per, d_key.p, d_bo.p,
                                d_perm.p);
             HIPC(hipGetLastError());
-            int64_t mb = std::min<int64_t>((nitems + 255) / 256, (int64_t)h->CUs * 16);
+            // perm: [per-m items | kP range heads | other kP items]
+            int64_t nkp = 0, nkrest = 0;
+            if (kp)
+                for (int b = 0; b < G; b++) {
+                    nkp += bh[(size_t)kKpKey * G + b];
+                    nkrest += bh[(size_t)kKpRestKey * G + b];
+                }
+            h->pp_kp_ranges += nkp;
+            h->pp_kp_levels += nkp + nkrest;
+            const int64_t nper = nitems - nkp - nkrest;
+            int64_t mb = std::min<int64_t>(std::max<int64_t>(1, (nper + 255) / 256), (int64_t)h->CUs * 16);
             if (numeric)
                 hipLaunchKernelGGL(k_pp_M_num, dim3((unsigned)mb), dim3(256), sizeof(double) * 4 * h->pp_nInt,
                                    h->stream, h->d_tri.p, h->d_verts.p, d_x.p, p0, P, h->d_pp_rule.p, h->pp_nInt,
                                    nitems, d_iq.p, d_off.p, d_ms.p, 4 * kPi * h->rho, d_M.p);
             else
                 hipLaunchKernelGGL(k_pp_M, dim3((unsigned)mb), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P,
-                                   nitems, d_iq.p, d_off.p, d_ms.p, h->sort_items ? d_perm.p : nullptr, d_M.p);
+                                   nper, d_iq.p, d_off.p, d_ms.p, h->sort_items ? d_perm.p : nullptr, d_M.p);
             HIPC(hipGetLastError());
+            if (nkp > 0) {
+                const int64_t kblocks = std::min<int64_t>((nkp + 255) / 256, (int64_t)h->CUs * 16);
+                hipLaunchKernelGGL(k_pp_M_kp, dim3((unsigned)kblocks), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P,
+                                   nkp, d_iq.p, d_ka.p, d_kb.p, d_perm.p + nper, d_M.p, d_nfb.p);
+                HIPC(hipGetLastError());
+            }
             hipLaunchKernelGGL(k_pp_conv, dim3(np), dim3(128 * kConvSlices), 0, h->stream, d_M.p, d_off.p, d_ms.p,
                                d_me.p, d_aT.p, N, nT, p0, d_nh.p, d_kidx.p, (int)nh.size(), npts, d_u.p);
             HIPC(hipGetLastError());
@@ -1269,7 +1298,10 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     }
     tall.stop(h->stream);
     HIPC(hipMemcpyAsync(u_out, d_u.p, sizeof(double) * 3 * npts * nt, hipMemcpyDeviceToHost, h->stream));
+    unsigned int nfb = 0;
+    HIPC(hipMemcpyAsync(&nfb, d_nfb.p, sizeof nfb, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
+    h->pp_kp_fallbacks = nfb;
     h->tim[6] = tall.ms();
     return 0;
 }
@@ -1472,6 +1504,15 @@ int ebem_assembly_kp_stats(ebem_handle* h, int64_t* ranges, int64_t* levels, int
     if (ranges) *ranges = h->kp_ranges;
     if (levels) *levels = h->kp_levels;
     if (fallbacks) *fallbacks = h->kp_fallbacks;
+    return 0;
+}
+
+int ebem_pp_kp_stats(ebem_handle* h, int64_t* ranges, int64_t* levels, int64_t* fallbacks) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (ranges) *ranges = h->pp_kp_ranges;
+    if (levels) *levels = h->pp_kp_levels;
+    if (fallbacks) *fallbacks = h->pp_kp_fallbacks;
     return 0;
 }
 

@@ -122,6 +122,11 @@ int ebem_assembly_stats(ebem_handle *h, int64_t *far_items, int64_t *fp64_items)
  * check (0 expected).  No reference counterpart (reports the split). */
 int ebem_assembly_kp_stats(ebem_handle *h, int64_t *ranges, int64_t *levels, int64_t *fallbacks);
 
+/* The same for the last ebem_pp_eval (analytic form): M(x, tau_m) =
+ * WP + tau_m^2 WQ over the kP ranges of m of each (point, field triangle)
+ * pair.  No reference counterpart (reports the split). */
+int ebem_pp_kp_stats(ebem_handle *h, int64_t *ranges, int64_t *levels, int64_t *fallbacks);
+
 /* Export E_lag as a dense 3N x 3N column-major host matrix (the `matrix` of
  * calcMatrixBlock.m / matrixSavedV{lag+1} of timeMarchingD.m:141-158). */
 int ebem_get_block(ebem_handle *h, int lag, double *out);

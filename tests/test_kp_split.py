@@ -141,3 +141,40 @@ def test_kp_range_none_when_coplanar(setup):
             lib.sa_kp_range(rec.ctypes.data_as(_dp), gh.ctypes.data_as(_dp), vF.ctypes.data_as(_dp), s.nGH, s.velP,
                             s.velS, s.dt, int(l2[i, j]), int(h2[i, j]), C.byref(ka), C.byref(kb))
             assert ka.value > kb.value
+
+
+def test_pp_point_kp_form_matches_oracle(setup):
+    """Post-processing kernel M(x, tau) of ppA/BEMenerg_postProcA_calcMatrixBlock.m
+    for one point between the fronts in class kP: WP + tau^2 WQ from
+    node_eval_kp (one node, weight 1) against the oracle's M at each tau."""
+    lib, o, lo, hi = setup
+    rng = np.random.default_rng(5)
+    d = rng.normal(size=(40, 3))
+    X = d / np.linalg.norm(d, axis=1)[:, None] * rng.uniform(1.2, 2.0, size=(40, 1))
+    inv = 1.0 / (4 * np.pi * o.rho)
+    n_ranges = 0
+    worst = 0.0
+    for x in X:
+        for j in rng.integers(0, o.N, 8):
+            j = int(j)
+            V = o.coords[o.tri[j]]
+            dmax = np.sqrt(((V - x) ** 2).sum(1)).max()
+            zeta = abs(np.dot(x - V[0], o.normal[j]))
+            ta, tb = dmax * (1 + 1e-6) / o.velP, zeta * (1 - 1e-6) / o.velS
+            if tb <= ta:
+                continue
+            gh = np.zeros(128)
+            gh[0], gh[32], gh[64], gh[96] = x[0], x[1], x[2], 1.0
+            rec = o.field_record(V, o.normal[j])
+            WP, WQ = np.zeros(6), np.zeros(6)
+            ok = lib.sa_item_kp(rec.ctypes.data_as(_dp), gh.ctypes.data_as(_dp), 1, ta, tb, o.velP, o.velS,
+                                WP.ctypes.data_as(_dp), WQ.ctypes.data_as(_dp))
+            assert ok == 1
+            n_ranges += 1
+            for tau in np.linspace(ta, tb, 4):
+                R = o.pp_block(x, j, tau)
+                G = _blk(WP + tau * tau * WQ, inv)
+                worst = max(worst, np.abs(G - R).max() / np.abs(R).max())
+    print(f"{n_ranges} point-triangle kP ranges, worst rel {worst:.1e}")
+    assert n_ranges >= 50
+    assert worst < 1e-10
