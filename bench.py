@@ -30,15 +30,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Algorithmic FP64 operations per (pair, level) item of k_levels on this
-# workload (7 outer nodes): mean over a uniform sample of 200000 of the
-# 43938820 items of the pruned lag windows, counted by the op-counting host
-# build of the device math (python tests/opcount.py 200000 -> 9325.5 +- 7.4);
-# each + - * / (a fused multiply-add as 2), sqrt and libm call counts once.
-ALG_FLOP_PER_ITEM = 9325.5
+# workload (7 outer nodes): mean over the items outside the kP ranges (which
+# k_levels_kp evaluates) in a uniform sample of 200000 of the 43938820 items of
+# the pruned lag windows, counted by the op-counting host build of the device
+# math (python tests/opcount.py 200000 -> 9049.8 +- 9.8; kP share 0.2585,
+# 17393 flops per kP range); each + - * / (a fused multiply-add as 2), sqrt and
+# libm call counts once.
+ALG_FLOP_PER_ITEM = 9049.8
 # HBM bytes per launch of k_levels on this workload from the rocprofv3 --pmc
 # passes FETCH_SIZE (x2.0: the counter reads 512 MiB for a 1 GiB device copy)
-# and WRITE_SIZE (x1.0): 158.3 GB + 74.4 GB, almost all register-spill scratch.
-K_LEVELS_TRAFFIC_BYTES = 232.7e9
+# and WRITE_SIZE (x1.0): 128.5 GB + 54.2 GB, almost all register-spill scratch.
+K_LEVELS_TRAFFIC_BYTES = 182.7e9
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -221,6 +223,10 @@ def main():
     ms_step = 1e3 * float(np.mean(times))
     units = info["pair_lags"]
     lev_ms = float(np.mean([t["levels_ms"] for t in tims]))
+    kp_ms = float(np.mean([t["levels_kp_ms"] for t in tims]))
+    ppkp_ms = float(np.mean([t["pp_kp_ms"] for t in tims]))
+    kp = s.assembly_kp_stats()
+    level_items = info["items"] - kp["levels"]
     asm_ms = float(np.mean([t["assembly_ms"] for t in tims]))
     mot_ms = float(np.mean([t["mot_ms"] for t in tims]))
     hist_ms = float(np.mean([t["history_ms"] for t in tims]))
@@ -232,7 +238,7 @@ def main():
 
     roof = None
     if ALG_FLOP_PER_ITEM:
-        ach = ALG_FLOP_PER_ITEM * info["items"] / (lev_ms * 1e-3) / 1e12
+        ach = ALG_FLOP_PER_ITEM * level_items / (lev_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_levels", "achieved": ach,
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
                 "traffic": K_LEVELS_TRAFFIC_BYTES}
@@ -248,9 +254,12 @@ def main():
         "config": {"workload": "sphereUniform_3 icosphere (1280 triangles, 3840 DOF), T=4, nT=128, L=66, "
                                "SA 7-point outer rule, plain FP64: assembly + RHS + E0 factorisation + MOT "
                                "+ post-processing at 4096 shell points x 128 steps",
-                   "pair_lags": units, "items": info["items"], "store_bytes": info["bytes"]},
-        "breakdown_ms": {"assembly": asm_ms, "assembly_integral_kernel": lev_ms, "mot_total": mot_ms,
-                         "e0_factor": fac_ms, "history_sum": hist_ms, "post_processing": pp_ms},
+                   "pair_lags": units, "items": info["items"], "items_per_level": level_items,
+                   "kp_ranges": kp["ranges"], "kp_levels": kp["levels"], "store_bytes": info["bytes"]},
+        "breakdown_ms": {"assembly": asm_ms, "assembly_integral_kernel": lev_ms,
+                         "assembly_kp_kernel": kp_ms, "mot_total": mot_ms,
+                         "e0_factor": fac_ms, "history_sum": hist_ms, "post_processing": pp_ms,
+                         "post_processing_kp_kernel": ppkp_ms},
         "history_sum": {"bytes_streamed": hb_streamed, "bytes_if_per_step": hb, "achieved_GBs": hist_gbs,
                         "peak_GBs": PEAK_HBM_GBS, "frac": hist_gbs / PEAK_HBM_GBS,
                         "note": "blocked over 8 steps: one store pass per block plus the short lags per step"},

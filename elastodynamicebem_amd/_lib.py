@@ -44,7 +44,7 @@ SIGNATURES = {
     "ebem_node_integral": (C.c_int, [_vp, _dp, C.c_int, C.c_double, _dp]),
     "ebem_set_pp_rule": (C.c_int, [_vp, C.c_int, C.c_int]),
     "ebem_pp_eval_numeric": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
-    "ebem_timings": (C.c_int, [_vp, _dp]),
+    "ebem_timings": (C.c_int, [_vp, _dp, C.c_int]),
     "ebem_sur_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
     "ebem_sur_destroy": (C.c_int, [_vp]),
     "ebem_sur_add_samples": (C.c_int, [_vp, _vp, C.c_int64, _ip, _ip, _ip]),

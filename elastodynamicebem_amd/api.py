@@ -286,10 +286,10 @@ class Solver:
         return out.reshape(3, 3).T.copy()
 
     def timings(self) -> dict:
-        out = np.zeros(8)
-        _lib.check(self.lib.ebem_timings(self.h, _lib.dptr(out)))
         keys = ["levels_ms", "assembly_ms", "factor_ms", "history_ms", "apply_ms", "mot_ms", "pp_ms",
-                "history_launches"]
+                "history_launches", "levels_kp_ms", "pp_kp_ms"]
+        out = np.zeros(len(keys))
+        _lib.check(self.lib.ebem_timings(self.h, _lib.dptr(out), len(keys)))
         return dict(zip(keys, out.tolist()))
 
 

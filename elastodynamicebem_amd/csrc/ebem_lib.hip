@@ -415,7 +415,7 @@ struct ebem_handle {
     DevBuf<rocblas_int> d_ipiv, d_info;
     bool have_beta = false, have_alpha = false;
     bool e0_pivoted = false;  // the last factorisation needed partial pivoting
-    double tim[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double tim[EBEM_NUM_TIMINGS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool sort_items = true;
     bool kp_split = true;  // kP ranges through k_levels_kp (EBEM_KP_SPLIT=0 turns it off)
     int64_t kp_ranges = 0, kp_levels = 0, kp_fallbacks = 0;
@@ -720,8 +720,8 @@ int assemble(ebem_handle* h, int precision) {
     P.kp = kp ? 1 : 0;
     h->kp_ranges = h->kp_levels = h->kp_fallbacks = 0;
     const double near2 = (3.0 * h->hmax) * (3.0 * h->hmax);  // surrogate pairs: centroid distance <= 3h
-    Timer tall, tlev;
-    double lev_ms = 0;
+    Timer tall, tlev, tkp;
+    double lev_ms = 0, kp_ms = 0;
     tall.start(h->stream);
     const int64_t np = (int64_t)N * N;
     h->far_items = 0;
@@ -899,7 +899,10 @@ int assemble(ebem_handle* h, int precision) {
                 default: launch_levels_split<19>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
             }
             HIPC(hipGetLastError());
+            tlev.stop(h->stream);
+            lev_ms += tlev.ms();
             if (nkp > 0) {
+                tkp.start(h->stream);
                 const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((nkp + 255) / 256, (int64_t)h->CUs * 16));
 #define KP_LAUNCH(NG)                                                                                              \
     hipLaunchKernelGGL(k_levels_kp<NG>, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, \
@@ -912,9 +915,9 @@ int assemble(ebem_handle* h, int precision) {
                 }
 #undef KP_LAUNCH
                 HIPC(hipGetLastError());
+                tkp.stop(h->stream);
+                kp_ms += tkp.ms();
             }
-            tlev.stop(h->stream);
-            lev_ms += tlev.ms();
             if (nsur > 0) {
                 tsur.start(h->stream);
                 if (int r = sur_assemble_items(h, sur, r0, nsur, d_perm.p + (acc - nsur), d_ip.p, d_off.p, lo, hi, d_W.p))
@@ -931,6 +934,7 @@ int assemble(ebem_handle* h, int precision) {
     tall.stop(h->stream);
     h->tim[1] = tall.ms();
     h->tim[0] = lev_ms;
+    h->tim[8] = kp_ms;
     {
         unsigned int nfb = 0;
         HIPC(hipMemcpyAsync(&nfb, d_nfb.p, sizeof nfb, hipMemcpyDeviceToHost, h->stream));
@@ -1137,7 +1141,8 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
         if (nhat[k] > nT - 1) return fail(EBEM_EINVAL, "time %g beyond the density's last step", t[k]);
         delta[k] = t[k] - nhat[k] * dt;
     }
-    Timer tall;
+    Timer tall, tppk;
+    double ppk_ms = 0;
     tall.start(h->stream);
     DevBuf<double> d_x, d_aT, d_u, d_dens, d_M;
     HIPC(d_x.alloc(3 * (size_t)npts));
@@ -1285,10 +1290,13 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
                                    nper, d_iq.p, d_off.p, d_ms.p, h->sort_items ? d_perm.p : nullptr, d_M.p);
             HIPC(hipGetLastError());
             if (nkp > 0) {
+                tppk.start(h->stream);
                 const int64_t kblocks = std::min<int64_t>((nkp + 255) / 256, (int64_t)h->CUs * 16);
                 hipLaunchKernelGGL(k_pp_M_kp, dim3((unsigned)kblocks), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P,
                                    nkp, d_iq.p, d_ka.p, d_kb.p, d_perm.p + nper, d_M.p, d_nfb.p);
                 HIPC(hipGetLastError());
+                tppk.stop(h->stream);
+                ppk_ms += tppk.ms();
             }
             hipLaunchKernelGGL(k_pp_conv, dim3(np), dim3(128 * kConvSlices), 0, h->stream, d_M.p, d_off.p, d_ms.p,
                                d_me.p, d_aT.p, N, nT, p0, d_nh.p, d_kidx.p, (int)nh.size(), npts, d_u.p);
@@ -1303,6 +1311,7 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     HIPC(hipStreamSynchronize(h->stream));
     h->pp_kp_fallbacks = nfb;
     h->tim[6] = tall.ms();
+    h->tim[9] = ppk_ms;
     return 0;
 }
 
@@ -1637,11 +1646,11 @@ int ebem_pp_eval_numeric(ebem_handle* h, const double* density, int npts, const 
     return pp_eval(h, density, npts, x, nt, t, u_out, true);
 }
 
-int ebem_timings(ebem_handle* h, double* out8) {
+int ebem_timings(ebem_handle* h, double* out, int n) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
-    if (!out8) return fail(EBEM_EINVAL, "null output");
-    for (int k = 0; k < 8; k++) out8[k] = h->tim[k];
+    if (!out || n < 0) return fail(EBEM_EINVAL, "null output or negative count");
+    for (int k = 0; k < n && k < EBEM_NUM_TIMINGS; k++) out[k] = h->tim[k];
     return 0;
 }
 

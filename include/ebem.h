@@ -170,11 +170,15 @@ int ebem_pp_eval_numeric(ebem_handle *h, const double *density, int npts, const 
 int ebem_node_integral(ebem_handle *h, const double *x, int tri, double t, double *out9);
 
 /* Timing of the last ebem_assemble / ebem_mot_solve / ebem_pp_eval call, in
- * milliseconds, from HIP events recorded on the library's stream:
- * out[0] assembly integral kernel, out[1] assembly total, out[2] E_0 factorisation,
- * out[3] history-sum kernel total, out[4] E_0 apply total, out[5] MOT total,
- * out[6] post-processing total, out[7] history-sum launches. */
-int ebem_timings(ebem_handle *h, double *out8);
+ * milliseconds, from HIP events recorded on the library's stream; writes the
+ * first min(n, EBEM_NUM_TIMINGS) of:
+ * out[0] assembly integral kernel (per-level items), out[1] assembly total,
+ * out[2] E_0 factorisation, out[3] history-sum kernel total, out[4] E_0 apply
+ * total, out[5] MOT total, out[6] post-processing total, out[7] history-sum
+ * launches, out[8] assembly kP-range kernel, out[9] post-processing kP-range
+ * kernel. */
+#define EBEM_NUM_TIMINGS 10
+int ebem_timings(ebem_handle *h, double *out, int n);
 
 /* ------------------------------------------------------------------------
  * Learned near-field surrogate.  No reference counterpart: the reference

@@ -74,4 +74,27 @@ long long opcount_item(const double* rec, const double* gh, int nGH, double t, d
     finish_rotation(acc, F.C, W);
     return g_ops - setup + setup;
 }
+
+// Flops of one kP range [ta, tb] of k_levels_kp: node_eval_kp over the nGH
+// nodes and the two rotations (the writes W_m = WP + t_m^2 WQ add 13 flops per
+// level, counted by the caller).  -1 if a node fails the state check.
+long long opcount_kp(const double* rec, const double* gh, int nGH, double ta, double tb, double cP, double cS) {
+    FieldTri F;
+    std::memcpy((void*)&F, rec, sizeof(FieldTri));
+    Level KA = make_level(cdouble(ta), cdouble(cP), cdouble(cS));
+    Level KB = make_level(cdouble(tb), cdouble(cP), cdouble(cS));
+    g_ops = 0;
+    NodeAcc aP, aQ;
+    aP.s = aQ.s = 0.0;
+    for (int q = 0; q < 6; q++) aP.m[q] = aQ.m[q] = 0.0;
+    for (int q = 0; q < nGH; q++) {
+        cdouble x[3] = {gh[q], gh[32 + q], gh[64 + q]};
+        NodeGeo G = node_geo(x, F);
+        if (!node_eval_kp(G, KA, KB, cdouble(gh[96 + q]), aP, aQ)) return -1;
+    }
+    cdouble WP[6], WQ[6];
+    finish_rotation(aP, F.C, WP);
+    finish_rotation(aQ, F.C, WQ);
+    return g_ops;
+}
 }

@@ -1,5 +1,6 @@
 """Algorithmic FP64 operation count per (pair, level) item of the assembly
-kernel, from the op-counting host build of the device math
+kernel k_levels (items outside the kP ranges) and per kP range of
+k_levels_kp, from the op-counting host build of the device math
 (tests/native/opcount.cpp), on a uniform random sample of the items of a
 workload.  Run:  python tests/opcount.py [nsample]   (sphereUniform_3, T=4, nT=128).
 """
@@ -28,12 +29,23 @@ def items_of(o, L):
 
 
 def count(mesh, Tfin, nT, L, nsample, seed=0, nGH=7):
+    """Uniform sample of the items: mean flops of the per-level items (k_levels)
+    and of the kP ranges (k_levels_kp, per range, writes included), with the
+    share of items inside kP ranges."""
+    from hostasm import load
     from oracle import Oracle
     subprocess.run(["make", "-s", "-C", os.path.join(HERE, "native")], check=True)
     lib = C.CDLL(os.path.join(HERE, "native", "libopcount.so"))
     lib.opcount_item.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double]
     lib.opcount_item.restype = C.c_longlong
+    lib.opcount_kp.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+    lib.opcount_kp.restype = C.c_longlong
+    host = load()
+    host.sa_kp_range.argtypes = [_dp, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]
     o = Oracle(mesh.coordinates, mesh.triangles, Tfin=Tfin, nT=nT, nGH=nGH)
+    from hostasm import exact_windows
+    lo, hi = exact_windows(o, L)
     mlo, n = items_of(o, L)
     flat = n.ravel()
     off = np.concatenate([[0], np.cumsum(flat)])
@@ -43,8 +55,8 @@ def count(mesh, Tfin, nT, L, nsample, seed=0, nGH=7):
     pair = np.searchsorted(off, picks, side="right") - 1
     N = mesh.numberTriangles
     recs = {}
-    ops = np.zeros(nsample)
-    for k, (it, p) in enumerate(zip(picks, pair)):
+    lev_ops, kp_ops, kp_w = [], [], []
+    for it, p in zip(picks, pair):
         i, j = divmod(int(p), N)
         m = int(mlo.ravel()[p] + (it - off[p]))
         if j not in recs:
@@ -52,17 +64,33 @@ def count(mesh, Tfin, nT, L, nsample, seed=0, nGH=7):
             recs[j] = o.field_record(T, o.normal[j])
         V = o.coords[o.tri[i]]
         gh = np.zeros(128)
-        nodes = o.ghn @ V
+        nodes = (o.ghn[:, 0:1] * V[0][None, :] + o.ghn[:, 1:2] * V[1][None, :]) + o.ghn[:, 2:3] * V[2][None, :]
         gh[0:nGH], gh[32:32 + nGH], gh[64:64 + nGH] = nodes[:, 0], nodes[:, 1], nodes[:, 2]
         gh[96:96 + nGH] = o.area[i] * o.ghw
-        ops[k] = lib.opcount_item(recs[j].ctypes.data_as(_dp), gh.ctypes.data_as(_dp), nGH, m * o.dt, o.velP,
-                                  o.velS)
-    return total, ops.mean(), ops.std() / np.sqrt(nsample)
+        vF = np.ascontiguousarray(o.coords[o.tri[j]].ravel())
+        ka, kb = C.c_int(), C.c_int()
+        host.sa_kp_range(recs[j].ctypes.data_as(_dp), gh.ctypes.data_as(_dp), vF.ctypes.data_as(_dp), nGH, o.velP,
+                         o.velS, o.dt, int(lo[i, j]), int(hi[i, j]), C.byref(ka), C.byref(kb))
+        if ka.value <= m <= kb.value:
+            nl = kb.value - ka.value + 1
+            c = lib.opcount_kp(recs[j].ctypes.data_as(_dp), gh.ctypes.data_as(_dp), nGH, ka.value * o.dt,
+                               kb.value * o.dt, o.velP, o.velS)
+            kp_ops.append(c + 13 * nl)
+            kp_w.append(1.0 / nl)
+        else:
+            lev_ops.append(lib.opcount_item(recs[j].ctypes.data_as(_dp), gh.ctypes.data_as(_dp), nGH, m * o.dt,
+                                            o.velP, o.velS))
+    lev = np.array(lev_ops, float)
+    kpo, kpw = np.array(kp_ops, float), np.array(kp_w)
+    return dict(items=total, kp_share=len(kpo) / nsample, level_flops=lev.mean(),
+                level_flops_se=lev.std() / np.sqrt(len(lev)),
+                kp_range_flops=float((kpo * kpw).sum() / kpw.sum()) if len(kpo) else 0.0)
 
 
 if __name__ == "__main__":
     from elastodynamicebem_amd.mesh import read_mesh
     ns = int(sys.argv[1]) if len(sys.argv) > 1 else 20000
     m = read_mesh(os.path.join(HERE, "golden", "meshes", "sphereUniform_3.mesh"))
-    total, mean, se = count(m, 4.0, 128, 66, ns)
-    print(f"items {total}  flops/item {mean:.1f} +- {se:.1f} (n={ns})")
+    r = count(m, 4.0, 128, 66, ns)
+    print(f"items {r['items']}  kP share {r['kp_share']:.4f}  per-level flops/item {r['level_flops']:.1f} "
+          f"+- {r['level_flops_se']:.1f}  flops per kP range {r['kp_range_flops']:.0f} (n={ns})")

@@ -42,7 +42,8 @@ def test_kp_split_matches_per_level_assembly():
         info = a.store_info()
         L = a.num_blocks()
         assert L == b.num_blocks() and info == b.store_info()
-        ta, tb = a.timings()["levels_ms"], b.timings()["levels_ms"]
+        ta = a.timings()["levels_ms"] + a.timings()["levels_kp_ms"]
+        tb = b.timings()["levels_ms"]
         print(f"kP ranges {st['ranges']}, levels {st['levels']} of {info['items']} items "
               f"({st['levels'] / info['items']:.1%}), fallbacks {st['fallbacks']}; "
               f"integral kernels {ta:.1f} ms vs {tb:.1f} ms per level")
