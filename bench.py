@@ -225,8 +225,10 @@ def main():
     lev_ms = float(np.mean([t["levels_ms"] for t in tims]))
     kp_ms = float(np.mean([t["levels_kp_ms"] for t in tims]))
     ppkp_ms = float(np.mean([t["pp_kp_ms"] for t in tims]))
+    bf_ms = float(np.mean([t["levels_bf_ms"] for t in tims]))
+    ppbf_ms = float(np.mean([t["pp_bf_ms"] for t in tims]))
     kp = s.assembly_kp_stats()
-    level_items = info["items"] - kp["levels"]
+    level_items = info["items"] - kp["levels"] - kp["bf_levels"]
     asm_ms = float(np.mean([t["assembly_ms"] for t in tims]))
     mot_ms = float(np.mean([t["mot_ms"] for t in tims]))
     hist_ms = float(np.mean([t["history_ms"] for t in tims]))
@@ -255,11 +257,12 @@ def main():
                                "SA 7-point outer rule, plain FP64: assembly + RHS + E0 factorisation + MOT "
                                "+ post-processing at 4096 shell points x 128 steps",
                    "pair_lags": units, "items": info["items"], "items_per_level": level_items,
-                   "kp_ranges": kp["ranges"], "kp_levels": kp["levels"], "store_bytes": info["bytes"]},
+                   "kp_ranges": kp["ranges"], "kp_levels": kp["levels"], "bf_ranges": kp["bf_ranges"],
+                   "bf_levels": kp["bf_levels"], "store_bytes": info["bytes"]},
         "breakdown_ms": {"assembly": asm_ms, "assembly_integral_kernel": lev_ms,
-                         "assembly_kp_kernel": kp_ms, "mot_total": mot_ms,
+                         "assembly_kp_kernel": kp_ms, "assembly_bf_kernel": bf_ms, "mot_total": mot_ms,
                          "e0_factor": fac_ms, "history_sum": hist_ms, "post_processing": pp_ms,
-                         "post_processing_kp_kernel": ppkp_ms},
+                         "post_processing_kp_kernel": ppkp_ms, "post_processing_bf_kernel": ppbf_ms},
         "history_sum": {"bytes_streamed": hb_streamed, "bytes_if_per_step": hb, "achieved_GBs": hist_gbs,
                         "peak_GBs": PEAK_HBM_GBS, "frac": hist_gbs / PEAK_HBM_GBS,
                         "note": "blocked over 8 steps: one store pass per block plus the short lags per step"},

@@ -33,8 +33,8 @@ SIGNATURES = {
     "ebem_assemble": (C.c_int, [_vp, C.c_int]),
     "ebem_store_info": (C.c_int, [_vp, _i64p, _i64p, _i64p, _i64p]),
     "ebem_assembly_stats": (C.c_int, [_vp, _i64p, _i64p]),
-    "ebem_assembly_kp_stats": (C.c_int, [_vp, _i64p, _i64p, _i64p]),
-    "ebem_pp_kp_stats": (C.c_int, [_vp, _i64p, _i64p, _i64p]),
+    "ebem_assembly_kp_stats": (C.c_int, [_vp, _i64p]),
+    "ebem_pp_kp_stats": (C.c_int, [_vp, _i64p]),
     "ebem_store_lag_blocks": (C.c_int, [_vp, _i64p]),
     "ebem_get_block": (C.c_int, [_vp, C.c_int, _dp]),
     "ebem_get_block_rows": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),

@@ -213,17 +213,20 @@ class Solver:
         _lib.check(self.lib.ebem_assembly_stats(self.h, C.byref(a), C.byref(b)))
         return dict(far_items=a.value, fp64_items=b.value)
 
+    _KP_KEYS = ("ranges", "levels", "bf_ranges", "bf_levels", "fallbacks")
+
     def assembly_kp_stats(self) -> dict:
-        """kP ranges of the last assembly: units, levels covered, level-by-level fallbacks."""
-        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
-        _lib.check(self.lib.ebem_assembly_kp_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
-        return dict(ranges=a.value, levels=b.value, fallbacks=c.value)
+        """Level ranges of the last assembly evaluated per pair: kP ranges and their
+        levels, between-front ranges and their levels, level-by-level fallbacks."""
+        out = np.zeros(5, np.int64)
+        _lib.check(self.lib.ebem_assembly_kp_stats(self.h, out.ctypes.data_as(_lib._i64p)))
+        return dict(zip(self._KP_KEYS, out.tolist()))
 
     def pp_kp_stats(self) -> dict:
-        """kP ranges of the last post-processing: units, items covered, per-m fallbacks."""
-        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
-        _lib.check(self.lib.ebem_pp_kp_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
-        return dict(ranges=a.value, levels=b.value, fallbacks=c.value)
+        """The same for the last post-processing (ranges of m per point-triangle pair)."""
+        out = np.zeros(5, np.int64)
+        _lib.check(self.lib.ebem_pp_kp_stats(self.h, out.ctypes.data_as(_lib._i64p)))
+        return dict(zip(self._KP_KEYS, out.tolist()))
 
     def surrogate_stats(self) -> dict:
         a, b, ms = C.c_int64(), C.c_int64(), C.c_double()
@@ -287,7 +290,7 @@ class Solver:
 
     def timings(self) -> dict:
         keys = ["levels_ms", "assembly_ms", "factor_ms", "history_ms", "apply_ms", "mot_ms", "pp_ms",
-                "history_launches", "levels_kp_ms", "pp_kp_ms"]
+                "history_launches", "levels_kp_ms", "pp_kp_ms", "levels_bf_ms", "pp_bf_ms"]
         out = np.zeros(len(keys))
         _lib.check(self.lib.ebem_timings(self.h, _lib.dptr(out), len(keys)))
         return dict(zip(keys, out.tolist()))

@@ -130,6 +130,7 @@ __global__ void __launch_bounds__(256) k_windows(const FieldTri* __restrict__ tr
                                                  const double* __restrict__ verts, AsmParams P, int r0, int r1,
                                                  int16_t* __restrict__ lo, int16_t* __restrict__ hi,
                                                  int16_t* __restrict__ ka, int16_t* __restrict__ kb,
+                                                 int16_t* __restrict__ sa, int16_t* __restrict__ sb,
                                                  unsigned long long* __restrict__ culled) {
     __shared__ unsigned long long part;
     if (threadIdx.x == 0) part = 0;
@@ -148,8 +149,12 @@ __global__ void __launch_bounds__(256) k_windows(const FieldTri* __restrict__ tr
         }
         if (P.kp) {
             int a = 1, b = 0;
+            int c = 1, d = 0;
             if (l <= h)
-                kp_range(P.nGH, P.cP, P.cS, P.dt, gh + (int64_t)i * 128, tri[j], verts + (int64_t)j * 9, l, h, a, b);
+                kp_range(P.nGH, P.cP, P.cS, P.dt, gh + (int64_t)i * 128, tri[j], verts + (int64_t)j * 9, l, h, a, b, c,
+                         d);
+            sa[p] = (int16_t)c;
+            sb[p] = (int16_t)d;
             ka[p] = (int16_t)a;
             kb[p] = (int16_t)b;
         }
@@ -246,11 +251,14 @@ constexpr int kFrontClasses = kFrontBins + 2;
 constexpr int kFrontKeys = kFrontClasses * kFrontClasses;
 // Key order = order of the sorted permutation: far-field items (mixed
 // precision) 0..kFrontKeys-1, the other per-level items, the first level of
-// each kP range (one k_levels_kp item per pair), its other levels (no work),
+// each kP range (one k_levels_kp item per pair) and its other levels (no work
+// of their own), the same for the between-front ranges (k_levels_bf),
 // surrogate pairs last.
 constexpr int kKpKey = 2 * kFrontKeys;
 constexpr int kKpRestKey = kKpKey + 1;
-constexpr int kSurKey = kKpKey + 2;
+constexpr int kBfKey = kKpKey + 2;
+constexpr int kBfRestKey = kKpKey + 3;
+constexpr int kSurKey = kKpKey + 4;
 constexpr int kNumKeys = kSurKey + 1;
 static_assert(kNumKeys <= 256, "keys are stored as uint8");
 
@@ -266,6 +274,7 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
                                                    int64_t nitems, int64_t per, const int32_t* __restrict__ item_pair,
                                                    const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
                                                    const int16_t* __restrict__ ka, const int16_t* __restrict__ kb,
+                                                   const int16_t* __restrict__ sa, const int16_t* __restrict__ sb,
                                                    const uint8_t* __restrict__ sflag, uint8_t* __restrict__ key,
                                                    unsigned int* __restrict__ bh) {
     __shared__ unsigned int sh[kNumKeys];
@@ -289,6 +298,7 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
         int qs = front_class(P.cS * t, d - ex, 2.0 * ex) + 1;
         int k = qp * kFrontClasses + qs + ((P.far2 >= 0.0 && d2 > P.far2) ? 0 : kFrontKeys);
         if (ka && m >= ka[p] && m <= kb[p]) k = (m == ka[p]) ? kKpKey : kKpRestKey;
+        if (sa && m >= sa[p] && m <= sb[p]) k = (m == sa[p]) ? kBfKey : kBfRestKey;
         if (sflag && sflag[p]) k = kSurKey;
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
@@ -411,6 +421,76 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                 finish_rotation(acc, F.C, Wv);
                 double* o = W + (it + (m - ma)) * 6;
 #pragma unroll
+                for (int q = 0; q < 6; q++) o[q] = Wv[q];
+            }
+        }
+    }
+}
+
+// One thread per between-front range (the items of key kBfKey, the first
+// level sa of the range): levels sa..sb in chunks of kBfChunk through
+// node_eval_bf, which yields k_levels' values with the level-independent
+// region terms evaluated once per chunk.  From a chunk whose state check
+// fails, the rest of the range is evaluated level by level as in k_levels.
+template <int NGH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_levels_bf(
+    const FieldTri* __restrict__ tri, const double* __restrict__ gh, AsmParams P, int r0, int64_t nitems,
+    const int32_t* __restrict__ item_pair, const int16_t* __restrict__ sa, const int16_t* __restrict__ sb,
+    const int32_t* __restrict__ perm, double* __restrict__ W, unsigned int* __restrict__ nfallback) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nitems;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t it = perm[k];
+        const int32_t p = item_pair[it];
+        const int i = r0 + p / P.N, j = p % P.N;
+        const int ma = sa[p], mb = sb[p];
+        const FieldTri& F = tri[j];
+        const double* g = gh + (int64_t)i * 4 * 32;
+        int m = ma;
+        for (; m <= mb; m += kBfChunk) {
+            const int n = mb - m + 1 < kBfChunk ? mb - m + 1 : kBfChunk;
+            double t[kBfChunk];
+#pragma unroll
+            for (int l = 0; l < kBfChunk; l++) t[l] = (double)(m + (l < n ? l : 0)) * P.dt;
+            NodeAcc acc[kBfChunk];
+#pragma unroll
+            for (int l = 0; l < kBfChunk; l++) {
+                acc[l].s = 0.0;
+                for (int q = 0; q < 6; q++) acc[l].m[q] = 0.0;
+            }
+            bool ok = true;
+#pragma unroll 1
+            for (int q = 0; q < NGH && ok; q++) {
+                double x[3] = {g[q], g[32 + q], g[64 + q]};
+                NodeGeo G = node_geo(x, F);
+                ok = node_eval_bf<kBfChunk>(G, t, n, P.cP, P.cS, g[96 + q], acc);
+            }
+            if (!ok) break;
+#pragma unroll
+            for (int l = 0; l < kBfChunk; l++) {
+                if (l < n) {
+                    double Wv[6];
+                    finish_rotation(acc[l], F.C, Wv);
+                    double* o = W + (it + (m + l - ma)) * 6;
+                    for (int q = 0; q < 6; q++) o[q] = Wv[q];
+                }
+            }
+        }
+        if (m <= mb) {
+            atomicAdd(nfallback, 1u);
+            for (; m <= mb; m++) {
+                Level K = make_level((double)m * P.dt, P.cP, P.cS);
+                NodeAcc acc;
+                acc.s = 0.0;
+                for (int q = 0; q < 6; q++) acc.m[q] = 0.0;
+#pragma unroll 1
+                for (int q = 0; q < NGH; q++) {
+                    double x[3] = {g[q], g[32 + q], g[64 + q]};
+                    NodeGeo G = node_geo(x, F);
+                    node_eval<double>(G, K, g[96 + q], acc);
+                }
+                double Wv[6];
+                finish_rotation(acc, F.C, Wv);
+                double* o = W + (it + (m - ma)) * 6;
                 for (int q = 0; q < 6; q++) o[q] = Wv[q];
             }
         }
@@ -716,13 +796,17 @@ __device__ inline double pp_tau(const PPParams& P, int m) {
 }
 
 // kP range [ka, kb] of m for point x and field triangle F within the window
-// [lo, hi] (kp_range with the single outer point x and tau_m ~ delta + m dt;
-// the 1e-7 margins cover the difference to pp_tau).  ka > kb: none.
+// [lo, hi], and the between-front range [sa, sb] after it (kp_range with the
+// single outer point x and tau_m ~ delta + m dt; the 1e-7 margins cover the
+// difference to pp_tau).  first > last: none.
 __device__ inline void pp_kp_range(const PPParams& P, double x0, double x1, double x2, const FieldTri& F,
-                                   const double* __restrict__ vF, int lo, int hi, int& ka, int& kb) {
+                                   const double* __restrict__ vF, int lo, int hi, int& ka, int& kb, int& sa,
+                                   int& sb) {
 #pragma clang fp contract(off)
     ka = 1;
     kb = 0;
+    sa = 1;
+    sb = 0;
     double cmin2 = 1e300, dmax2 = 0.0;
     for (int v = 0; v < 3; v++) {
         const int w = (v + 1) % 3;
@@ -734,13 +818,20 @@ __device__ inline void pp_kp_range(const PPParams& P, double x0, double x1, doub
     if (!(0.25 * cmin2 > 1.0e-5)) return;
     double d0 = x0 - F.v0[0], d1 = x1 - F.v0[1], d2 = x2 - F.v0[2];
     double z = fabs(F.n[0] * d0 + F.n[1] * d1 + F.n[2] * d2);
-    if (z < 1.0e-6) return;
+    if (z < 1.0e-6) z = 0.0;
+    const double d3 = tri_dist3(x0, x1, x2, vF);
     const double fa = ceil((sqrt(dmax2) * (1.0 + 1e-7) / P.cP - P.delta) / P.dt);
     const double fb = ceil((z * (1.0 - 1e-7) / P.cS - P.delta) / P.dt) - 1.0;
+    const double fs = ceil(((d3 * (1.0 - 1e-7) - 1e-9) / P.cS - P.delta) / P.dt) - 1.0;
     const double a = fmax(fa, (double)lo), b = fmin(fb, (double)hi);
     if (b - a + 1.0 >= 3.0) {
         ka = (int)a;
         kb = (int)b;
+    }
+    const double s0 = ka <= kb ? (double)(kb + 1) : a, s1 = fmin(fs, (double)hi);
+    if (s1 - s0 + 1.0 >= 2.0) {
+        sa = (int)s0;
+        sb = (int)s1;
     }
 }
 
@@ -756,7 +847,8 @@ __device__ inline void pp_kp_range(const PPParams& P, double x0, double x1, doub
 __global__ void __launch_bounds__(256) k_pp_windows(const FieldTri* __restrict__ tri, const double* __restrict__ verts,
                                                     const double* __restrict__ xs, int p0, int np, PPParams P,
                                                     int32_t* __restrict__ ms, int32_t* __restrict__ me,
-                                                    int32_t* __restrict__ ka, int32_t* __restrict__ kb) {
+                                                    int32_t* __restrict__ ka, int32_t* __restrict__ kb,
+                                                    int32_t* __restrict__ sa, int32_t* __restrict__ sb) {
     int64_t tot = (int64_t)np * P.N;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
         int p = p0 + (int)(q / P.N), j = (int)(q % P.N);
@@ -779,10 +871,12 @@ __global__ void __launch_bounds__(256) k_pp_windows(const FieldTri* __restrict__
         ms[q] = lo;
         me[q] = hi;
         if (P.kp) {
-            int a0 = 1, b0 = 0;
-            if (lo <= hi) pp_kp_range(P, x0, x1, x2, F, v, lo, hi, a0, b0);
+            int a0 = 1, b0 = 0, c0 = 1, e0 = 0;
+            if (lo <= hi) pp_kp_range(P, x0, x1, x2, F, v, lo, hi, a0, b0, c0, e0);
             ka[q] = a0;
             kb[q] = b0;
+            sa[q] = c0;
+            sb[q] = e0;
         }
     }
 }
@@ -911,6 +1005,71 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     }
 }
 
+// One thread per between-front range of m (the items of key kBfKey, m = sa):
+// M_m for m = sa..sb through node_eval_bf (one outer point, weight 1) in
+// chunks of kBfChunk, the same values as k_pp_M; from a chunk whose state
+// check fails, per m as in k_pp_M.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_pp_M_bf(
+    const FieldTri* __restrict__ tri, const double* __restrict__ xs, int p0, PPParams P, int64_t nheads,
+    const int32_t* __restrict__ item_q, const int32_t* __restrict__ sa, const int32_t* __restrict__ sb,
+    const int32_t* __restrict__ perm, double* __restrict__ M, unsigned int* __restrict__ nfallback) {
+    for (int64_t kx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; kx < nheads;
+         kx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t it = perm[kx];
+        const int32_t q = item_q[it];
+        const int p = p0 + q / P.N, j = q % P.N;
+        const int ma = sa[q], mb = sb[q];
+        const FieldTri& F = tri[j];
+        double x[3] = {xs[3 * p], xs[3 * p + 1], xs[3 * p + 2]};
+        const NodeGeo G = node_geo(x, F);
+        int m = ma;
+        for (; m <= mb; m += kBfChunk) {
+            const int n = mb - m + 1 < kBfChunk ? mb - m + 1 : kBfChunk;
+            double t[kBfChunk];
+#pragma unroll
+            for (int l = 0; l < kBfChunk; l++) t[l] = pp_tau(P, m + (l < n ? l : 0));
+            NodeAcc acc[kBfChunk];
+#pragma unroll
+            for (int l = 0; l < kBfChunk; l++) {
+                acc[l].s = 0.0;
+                for (int k = 0; k < 6; k++) acc[l].m[k] = 0.0;
+            }
+            if (!node_eval_bf<kBfChunk>(G, t, n, P.cP, P.cS, 1.0, acc)) break;
+#pragma unroll
+            for (int l = 0; l < kBfChunk; l++) {
+                if (l < n) {
+                    double W[6];
+                    finish_rotation(acc[l], F.C, W);
+                    double* o = M + (it + (m + l - ma)) * 6;
+                    for (int k = 0; k < 6; k++) {
+                        double v = W[k] * P.inv4pirho;
+                        o[k] = fabs(v) < 1.0e-14 ? 0.0 : v;
+                    }
+                }
+            }
+        }
+        if (m <= mb) {
+            atomicAdd(nfallback, 1u);
+            for (; m <= mb; m++) {
+                const double tau = pp_tau(P, m);
+                double W[6] = {0, 0, 0, 0, 0, 0};
+                if (tau > 0.0) {
+                    Level K = make_level(tau, P.cP, P.cS);
+                    NodeAcc acc;
+                    acc.s = 0.0;
+                    for (int k = 0; k < 6; k++) acc.m[k] = 0.0;
+                    if (node_eval(G, K, 1.0, acc) != kNone) finish_rotation(acc, F.C, W);
+                }
+                double* o = M + (it + (m - ma)) * 6;
+                for (int k = 0; k < 6; k++) {
+                    double v = W[k] * P.inv4pirho;
+                    o[k] = fabs(v) < 1.0e-14 ? 0.0 : v;
+                }
+            }
+        }
+    }
+}
+
 // One workgroup per point: thread (slice s, time k) sums the triangles
 // j = s, s + S, s + 2S, ... for requested time k (nhat[k]); the S slice
 // partials are reduced through LDS.  aT: density as [triangle][step][3].
@@ -1001,7 +1160,8 @@ __global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tr
                                                  int p0, PPParams P, int64_t nitems, int64_t per,
                                                  const int32_t* __restrict__ item_q, const int64_t* __restrict__ off,
                                                  const int32_t* __restrict__ ms, const int32_t* __restrict__ ka,
-                                                 const int32_t* __restrict__ kb, uint8_t* __restrict__ key,
+                                                 const int32_t* __restrict__ kb, const int32_t* __restrict__ sa,
+                                                 const int32_t* __restrict__ sb, uint8_t* __restrict__ key,
                                                  unsigned int* __restrict__ bh) {
     __shared__ unsigned int sh[kNumKeys];
     for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) sh[k] = 0;
@@ -1020,6 +1180,7 @@ __global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tr
         int qs = front_class(P.cS * tau, d - F.maxL, 2.0 * F.maxL) + 1;
         int k = qp * kFrontClasses + qs + kFrontKeys;
         if (ka && m >= ka[q] && m <= kb[q]) k = (m == ka[q]) ? kKpKey : kKpRestKey;
+        if (sa && m >= sa[q] && m <= sb[q]) k = (m == sa[q]) ? kBfKey : kBfRestKey;
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
     }

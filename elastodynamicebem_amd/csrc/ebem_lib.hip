@@ -415,11 +415,13 @@ struct ebem_handle {
     DevBuf<rocblas_int> d_ipiv, d_info;
     bool have_beta = false, have_alpha = false;
     bool e0_pivoted = false;  // the last factorisation needed partial pivoting
-    double tim[EBEM_NUM_TIMINGS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double tim[EBEM_NUM_TIMINGS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool sort_items = true;
     bool kp_split = true;  // kP ranges through k_levels_kp (EBEM_KP_SPLIT=0 turns it off)
-    int64_t kp_ranges = 0, kp_levels = 0, kp_fallbacks = 0;
-    int64_t pp_kp_ranges = 0, pp_kp_levels = 0, pp_kp_fallbacks = 0;  // of the last post-processing
+    // kP and between-front ranges of the last assembly / post-processing:
+    // kP ranges, kP levels, between-front ranges, their levels, fallbacks
+    int64_t kp_stats[5] = {0, 0, 0, 0, 0};
+    int64_t pp_kp_stats[5] = {0, 0, 0, 0, 0};
     int64_t far_items = 0;
     int precision = 0;
 
@@ -718,10 +720,10 @@ int assemble(ebem_handle* h, int precision) {
     // kP ranges need the key-sorted item order
     const bool kp = !ghc && h->sort_items && h->kp_split;
     P.kp = kp ? 1 : 0;
-    h->kp_ranges = h->kp_levels = h->kp_fallbacks = 0;
+    for (int k = 0; k < 5; k++) h->kp_stats[k] = 0;
     const double near2 = (3.0 * h->hmax) * (3.0 * h->hmax);  // surrogate pairs: centroid distance <= 3h
-    Timer tall, tlev, tkp;
-    double lev_ms = 0, kp_ms = 0;
+    Timer tall, tlev, tkp, tbf;
+    double lev_ms = 0, kp_ms = 0, bf_ms = 0;
     tall.start(h->stream);
     const int64_t np = (int64_t)N * N;
     h->far_items = 0;
@@ -736,12 +738,14 @@ int assemble(ebem_handle* h, int precision) {
         HIPC(hipMemsetAsync(d_nsb.p, 0, sizeof(unsigned long long), h->stream));
     }
     Timer tsur;
-    DevBuf<int16_t> d_lo, d_hi, d_ka, d_kb;
+    DevBuf<int16_t> d_lo, d_hi, d_ka, d_kb, d_sa, d_sb;
     HIPC(d_lo.alloc(np));
     HIPC(d_hi.alloc(np));
     if (kp) {
         HIPC(d_ka.alloc(np));
         HIPC(d_kb.alloc(np));
+        HIPC(d_sa.alloc(np));
+        HIPC(d_sb.alloc(np));
     }
     DevBuf<unsigned long long> d_culled;
     HIPC(d_culled.alloc(1));
@@ -752,7 +756,7 @@ int assemble(ebem_handle* h, int precision) {
     {
         int64_t blocks = std::min<int64_t>((np + 255) / 256, (int64_t)h->CUs * 64);
         hipLaunchKernelGGL(k_windows, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p,
-                           h->d_verts.p, P, 0, N, d_lo.p, d_hi.p, d_ka.p, d_kb.p, d_culled.p);
+                           h->d_verts.p, P, 0, N, d_lo.p, d_hi.p, d_ka.p, d_kb.p, d_sa.p, d_sb.p, d_culled.p);
         HIPC(hipGetLastError());
     }
     unsigned long long culled = 0;
@@ -841,8 +845,10 @@ int assemble(ebem_handle* h, int precision) {
             const int64_t per = (acc + G - 1) / G;
             const int16_t* ka = kp ? d_ka.p + (int64_t)r0 * N : nullptr;
             const int16_t* kb = kp ? d_kb.p + (int64_t)r0 * N : nullptr;
+            const int16_t* sa = kp ? d_sa.p + (int64_t)r0 * N : nullptr;
+            const int16_t* sb = kp ? d_sb.p + (int64_t)r0 * N : nullptr;
             hipLaunchKernelGGL(k_item_keys, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, P, r0, acc, per, d_ip.p,
-                               d_off.p, lo, ka, kb, sflag, d_key.p, d_hist.p);
+                               d_off.p, lo, ka, kb, sa, sb, sflag, d_key.p, d_hist.p);
             HIPC(hipGetLastError());
             HIPC(hipMemcpyAsync(hist.data(), d_hist.p, sizeof(unsigned int) * hist.size(), hipMemcpyDeviceToHost,
                                 h->stream));
@@ -858,19 +864,24 @@ int assemble(ebem_handle* h, int precision) {
                                d_perm.p);
             HIPC(hipGetLastError());
             const int32_t* perm = (h->sort_items || mixed || sur) ? d_perm.p : nullptr;
-            int64_t nfar = 0, nsur = 0, nkp = 0, nkrest = 0;
+            int64_t nfar = 0, nsur = 0, nkp = 0, nkrest = 0, nbf = 0, nbrest = 0;
             for (size_t k = 0; k < (size_t)kFrontKeys * G; k++) nfar += hist[k];
             for (int b = 0; b < G; b++) {
                 nsur += hist[(size_t)kSurKey * G + b];
                 nkp += hist[(size_t)kKpKey * G + b];
                 nkrest += hist[(size_t)kKpRestKey * G + b];
+                nbf += hist[(size_t)kBfKey * G + b];
+                nbrest += hist[(size_t)kBfRestKey * G + b];
             }
             h->far_items += nfar;
             h->sur_items += nsur;
-            h->kp_ranges += nkp;
-            h->kp_levels += nkp + nkrest;
-            // perm: [per-level items | kP range heads | other kP levels | surrogate items]
-            const int64_t nlev = acc - nsur - nkp - nkrest;
+            h->kp_stats[0] += nkp;
+            h->kp_stats[1] += nkp + nkrest;
+            h->kp_stats[2] += nbf;
+            h->kp_stats[3] += nbf + nbrest;
+            // perm: [per-level items | kP range heads | other kP levels |
+            //        between-front range heads | their other levels | surrogate items]
+            const int64_t nlev = acc - nsur - nkp - nkrest - nbf - nbrest;
             tlev.start(h->stream);
             if (ghc) {
                 GhcRules R{h->d_ghc_in.p, h->d_ghc_dg.p, h->ghc_nSub * h->ghc_nSing, h->ghc_nDiag};
@@ -918,6 +929,24 @@ int assemble(ebem_handle* h, int precision) {
                 tkp.stop(h->stream);
                 kp_ms += tkp.ms();
             }
+            if (nbf > 0) {
+                tbf.start(h->stream);
+                const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((nbf + 255) / 256, (int64_t)h->CUs * 16));
+                const int32_t* pb = d_perm.p + nlev + nkp + nkrest;
+#define BF_LAUNCH(NG)                                                                                              \
+    hipLaunchKernelGGL(k_levels_bf<NG>, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, \
+                       r0, nbf, d_ip.p, sa, sb, pb, d_W.p, d_nfb.p)
+                switch (h->nGH) {
+                    case 3: BF_LAUNCH(3); break;
+                    case 7: BF_LAUNCH(7); break;
+                    case 12: BF_LAUNCH(12); break;
+                    default: BF_LAUNCH(19); break;
+                }
+#undef BF_LAUNCH
+                HIPC(hipGetLastError());
+                tbf.stop(h->stream);
+                bf_ms += tbf.ms();
+            }
             if (nsur > 0) {
                 tsur.start(h->stream);
                 if (int r = sur_assemble_items(h, sur, r0, nsur, d_perm.p + (acc - nsur), d_ip.p, d_off.p, lo, hi, d_W.p))
@@ -935,11 +964,12 @@ int assemble(ebem_handle* h, int precision) {
     h->tim[1] = tall.ms();
     h->tim[0] = lev_ms;
     h->tim[8] = kp_ms;
+    h->tim[10] = bf_ms;
     {
         unsigned int nfb = 0;
         HIPC(hipMemcpyAsync(&nfb, d_nfb.p, sizeof nfb, hipMemcpyDeviceToHost, h->stream));
         HIPC(hipStreamSynchronize(h->stream));
-        h->kp_fallbacks = nfb;
+        h->kp_stats[4] = nfb;
     }
     if (sur) {
         unsigned long long nb = 0;
@@ -1141,8 +1171,8 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
         if (nhat[k] > nT - 1) return fail(EBEM_EINVAL, "time %g beyond the density's last step", t[k]);
         delta[k] = t[k] - nhat[k] * dt;
     }
-    Timer tall, tppk;
-    double ppk_ms = 0;
+    Timer tall, tppk, tppb;
+    double ppk_ms = 0, ppb_ms = 0;
     tall.start(h->stream);
     DevBuf<double> d_x, d_aT, d_u, d_dens, d_M;
     HIPC(d_x.alloc(3 * (size_t)npts));
@@ -1171,7 +1201,7 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     const int G = h->CUs * 8;             // counting-sort blocks
     // kP ranges of m through k_pp_M_kp (analytic form with key-sorted items)
     const bool kp = !numeric && h->sort_items && h->kp_split;
-    DevBuf<int32_t> d_ms, d_me, d_iq, d_nh, d_kidx, d_perm, d_ka, d_kb;
+    DevBuf<int32_t> d_ms, d_me, d_iq, d_nh, d_kidx, d_perm, d_ka, d_kb, d_sa, d_sb;
     DevBuf<unsigned int> d_nfb;
     HIPC(d_nfb.alloc(1));
     HIPC(hipMemsetAsync(d_nfb.p, 0, sizeof(unsigned int), h->stream));
@@ -1184,8 +1214,10 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     if (kp) {
         HIPC(d_ka.alloc((size_t)P0 * N));
         HIPC(d_kb.alloc((size_t)P0 * N));
+        HIPC(d_sa.alloc((size_t)P0 * N));
+        HIPC(d_sb.alloc((size_t)P0 * N));
     }
-    h->pp_kp_ranges = h->pp_kp_levels = 0;
+    for (int k = 0; k < 5; k++) h->pp_kp_stats[k] = 0;
     HIPC(d_off.alloc((size_t)P0 * N));
     HIPC(d_rb.alloc(P0));
     HIPC(d_cnt.alloc(P0));
@@ -1229,7 +1261,7 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             int64_t tot = (int64_t)np * N;
             int64_t blocks = std::min<int64_t>((tot + 255) / 256, (int64_t)h->CUs * 64);
             hipLaunchKernelGGL(k_pp_windows, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p,
-                               h->d_verts.p, d_x.p, p0, np, P, d_ms.p, d_me.p, d_ka.p, d_kb.p);
+                               h->d_verts.p, d_x.p, p0, np, P, d_ms.p, d_me.p, d_ka.p, d_kb.p, d_sa.p, d_sb.p);
             HIPC(hipGetLastError());
             hipLaunchKernelGGL(k_pp_counts, dim3(np), dim3(256), 0, h->stream, d_ms.p, d_me.p, N, d_cnt.p);
             HIPC(hipGetLastError());
@@ -1254,8 +1286,8 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             // bucket items by front-state key (two-pass counting sort)
             const int64_t per = (nitems + G - 1) / G;
             hipLaunchKernelGGL(k_pp_keys, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P, nitems, per,
-                               d_iq.p, d_off.p, d_ms.p, kp ? d_ka.p : nullptr, kp ? d_kb.p : nullptr, d_key.p,
-                               d_bh.p);
+                               d_iq.p, d_off.p, d_ms.p, kp ? d_ka.p : nullptr, kp ? d_kb.p : nullptr,
+                               kp ? d_sa.p : nullptr, kp ? d_sb.p : nullptr, d_key.p, d_bh.p);
             HIPC(hipGetLastError());
             HIPC(hipMemcpyAsync(bh.data(), d_bh.p, sizeof(unsigned int) * bh.size(), hipMemcpyDeviceToHost,
                                 h->stream));
@@ -1270,16 +1302,21 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             hipLaunchKernelGGL(k_item_scatter, dim3(G), dim3(256), 0, h->stream, nitems, per, d_key.p, d_bo.p,
                                d_perm.p);
             HIPC(hipGetLastError());
-            // perm: [per-m items | kP range heads | other kP items]
-            int64_t nkp = 0, nkrest = 0;
+            // perm: [per-m items | kP range heads | other kP items |
+            //        between-front range heads | their other items]
+            int64_t nkp = 0, nkrest = 0, nbf = 0, nbrest = 0;
             if (kp)
                 for (int b = 0; b < G; b++) {
                     nkp += bh[(size_t)kKpKey * G + b];
                     nkrest += bh[(size_t)kKpRestKey * G + b];
+                    nbf += bh[(size_t)kBfKey * G + b];
+                    nbrest += bh[(size_t)kBfRestKey * G + b];
                 }
-            h->pp_kp_ranges += nkp;
-            h->pp_kp_levels += nkp + nkrest;
-            const int64_t nper = nitems - nkp - nkrest;
+            h->pp_kp_stats[0] += nkp;
+            h->pp_kp_stats[1] += nkp + nkrest;
+            h->pp_kp_stats[2] += nbf;
+            h->pp_kp_stats[3] += nbf + nbrest;
+            const int64_t nper = nitems - nkp - nkrest - nbf - nbrest;
             int64_t mb = std::min<int64_t>(std::max<int64_t>(1, (nper + 255) / 256), (int64_t)h->CUs * 16);
             if (numeric)
                 hipLaunchKernelGGL(k_pp_M_num, dim3((unsigned)mb), dim3(256), sizeof(double) * 4 * h->pp_nInt,
@@ -1298,6 +1335,16 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
                 tppk.stop(h->stream);
                 ppk_ms += tppk.ms();
             }
+            if (nbf > 0) {
+                tppb.start(h->stream);
+                const int64_t bblocks = std::min<int64_t>((nbf + 255) / 256, (int64_t)h->CUs * 16);
+                hipLaunchKernelGGL(k_pp_M_bf, dim3((unsigned)bblocks), dim3(256), 0, h->stream, h->d_tri.p, d_x.p,
+                                   p0, P, nbf, d_iq.p, d_sa.p, d_sb.p, d_perm.p + nper + nkp + nkrest, d_M.p,
+                                   d_nfb.p);
+                HIPC(hipGetLastError());
+                tppb.stop(h->stream);
+                ppb_ms += tppb.ms();
+            }
             hipLaunchKernelGGL(k_pp_conv, dim3(np), dim3(128 * kConvSlices), 0, h->stream, d_M.p, d_off.p, d_ms.p,
                                d_me.p, d_aT.p, N, nT, p0, d_nh.p, d_kidx.p, (int)nh.size(), npts, d_u.p);
             HIPC(hipGetLastError());
@@ -1309,9 +1356,10 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     unsigned int nfb = 0;
     HIPC(hipMemcpyAsync(&nfb, d_nfb.p, sizeof nfb, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
-    h->pp_kp_fallbacks = nfb;
+    h->pp_kp_stats[4] = nfb;
     h->tim[6] = tall.ms();
     h->tim[9] = ppk_ms;
+    h->tim[11] = ppb_ms;
     return 0;
 }
 
@@ -1506,22 +1554,20 @@ int ebem_store_lag_blocks(ebem_handle* h, int64_t* counts) {
     return 0;
 }
 
-int ebem_assembly_kp_stats(ebem_handle* h, int64_t* ranges, int64_t* levels, int64_t* fallbacks) {
+int ebem_assembly_kp_stats(ebem_handle* h, int64_t* out5) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
     if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
-    if (ranges) *ranges = h->kp_ranges;
-    if (levels) *levels = h->kp_levels;
-    if (fallbacks) *fallbacks = h->kp_fallbacks;
+    if (!out5) return fail(EBEM_EINVAL, "null output");
+    for (int k = 0; k < 5; k++) out5[k] = h->kp_stats[k];
     return 0;
 }
 
-int ebem_pp_kp_stats(ebem_handle* h, int64_t* ranges, int64_t* levels, int64_t* fallbacks) {
+int ebem_pp_kp_stats(ebem_handle* h, int64_t* out5) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
-    if (ranges) *ranges = h->pp_kp_ranges;
-    if (levels) *levels = h->pp_kp_levels;
-    if (fallbacks) *fallbacks = h->pp_kp_fallbacks;
+    if (!out5) return fail(EBEM_EINVAL, "null output");
+    for (int k = 0; k < 5; k++) out5[k] = h->pp_kp_stats[k];
     return 0;
 }
 

@@ -233,69 +233,90 @@ __host__ __device__ inline EdgeG<R> make_edge(R zeta, R th1, R th2, const ChildG
     return E;
 }
 
-// TC (P region between the S circle and the edge, coeffG_TC.m) when tc is
-// true; TR_P (P region between the plane foot and the edge, coeffG_TR_P.m)
-// otherwise.  zeta > 1e-6 branch of both.
+// The parts of region_edgeP that do not depend on the level (the angular
+// interval, the child and the plane distance fix them): evaluated once, they
+// serve every level whose region structure is the same.
 template <typename R>
-__host__ __device__ inline void region_edgeP(bool tc, R RS, R rhoS, R zeta, R th1, R th2,
-                                   const ChildGeoT<R>& g, const Phi<R>& P, const LevelT<R>& K, ChildAcc& A) {
-    R T1t = th2 - th1;
+struct EdgePre {
+    R T1t, F, z2, iz, g3, g7, h5, h6, dcgt, dct, f7, f8, f9, u15, u16, v17, v18, v19, w14;
+};
+
+template <typename R>
+__host__ __device__ inline EdgePre<R> edge_pre(R zeta, R th1, R th2, const ChildGeoT<R>& g) {
+    EdgePre<R> X;
+    X.T1t = th2 - th1;
     EdgeG<R> E = make_edge<R>(zeta, th1, th2, g);
     const R F = E.F;
+    X.F = F;
+    X.g3 = E.g3;
+    X.g7 = E.g7;
     R ca_g_1 = cos(g.al - g.ga + th1), ca_g_2 = cos(g.al - g.ga + th2);
     R cta1 = cos(th1 + g.al), cta2 = cos(th2 + g.al);
     R ctb1 = cos(th1 - g.be), ctb2 = cos(th2 - g.be);
     R cgt1 = cos(g.ga - th1), cgt2 = cos(g.ga - th2);
     R ct1 = cos(th1), ct2 = cos(th2);
+    X.z2 = zeta * zeta;
+    X.iz = R(1.0) / zeta;
+    X.h5 = cgt2 * E.g5 - cgt1 * E.g4 - g.ca * E.g3;
+    X.h6 = -ct2 * E.g5 + ct1 * E.g4 - g.cb * E.g3;
+    X.dcgt = cgt2 - cgt1;
+    X.dct = ct2 - ct1;
+    X.f7 = F * (-E.g1 * ca_g_1 + E.g2 * ca_g_2 + E.g3 * g.sa * g.sa);
+    X.f8 = F * (-g.sa * g.sb * E.g3 + E.g1 * cta1 - E.g2 * cta2);
+    X.f9 = F * (g.sb * g.sb * E.g3 + E.g1 * ctb1 - E.g2 * ctb2);
+    X.u15 = (g.sa * E.s2b - E.g8 * g.ca * E.c2b) / E.g2 + (-g.sa * E.s1b + E.g8 * g.ca * E.c1b) / E.g1;
+    X.u16 = (g.sb * E.s1b + E.g8 * g.cb * E.c1b) / E.g1 + (-g.sb * E.s2b - E.g8 * g.cb * E.c2b) / E.g2;
+    X.v17 = (-(E.s2b * E.s2b) * ca_g_2 + E.g8 * (g.ca * g.ca) * E.c2b) / E.g2 -
+            (-(E.s1b * E.s1b) * ca_g_1 + E.g8 * (g.ca * g.ca) * E.c1b) / E.g1;
+    X.v18 = ((E.s2b * E.s2b) * cta2 + E.g8 * (g.ca * g.cb * E.c2b)) / E.g2 -
+            ((E.s1b * E.s1b) * cta1 + E.g8 * (g.ca * g.cb * E.c1b)) / E.g1;
+    X.v19 = ((E.s2b * E.s2b) * ctb2 + E.g8 * (g.cb * g.cb * E.c2b)) / E.g2 -
+            ((E.s1b * E.s1b) * ctb1 + E.g8 * (g.cb * g.cb * E.c1b)) / E.g1;
+    X.w14 = F / E.ADD * (E.c1b / E.g1 - E.c2b / E.g2);
+    return X;
+}
+
+// The level-dependent rest of region_edgeP: TC (P region between the S circle
+// and the edge, coeffG_TC.m) when tc is true, TR_P (P region between the plane
+// foot and the edge, coeffG_TR_P.m) otherwise; zeta > 1e-6 branch of both.
+template <typename R>
+__host__ __device__ inline void edge_fin(bool tc, R RS, R rhoS, R zeta, const EdgePre<R>& X, const ChildGeoT<R>& g,
+                                         const Phi<R>& P, const LevelT<R>& K, ChildAcc& A) {
+    const R T1t = X.T1t, F = X.F, z2 = X.z2, iz = X.iz;
     R G1, G4, G5, G6, G7, G8, G9, G14, G15, G16, G17, G18, G19;
-    R z2 = zeta * zeta, iz = R(1.0) / zeta;
-    R h5 = cgt2 * E.g5 - cgt1 * E.g4 - g.ca * E.g3;
-    R h6 = -ct2 * E.g5 + ct1 * E.g4 - g.cb * E.g3;
-    R f7 = F * (-E.g1 * ca_g_1 + E.g2 * ca_g_2 + E.g3 * g.sa * g.sa);
-    R f8 = F * (-g.sa * g.sb * E.g3 + E.g1 * cta1 - E.g2 * cta2);
-    R f9 = F * (g.sb * g.sb * E.g3 + E.g1 * ctb1 - E.g2 * ctb2);
-    R u15 = (g.sa * E.s2b - E.g8 * g.ca * E.c2b) / E.g2 + (-g.sa * E.s1b + E.g8 * g.ca * E.c1b) / E.g1;
-    R u16 = (g.sb * E.s1b + E.g8 * g.cb * E.c1b) / E.g1 + (-g.sb * E.s2b - E.g8 * g.cb * E.c2b) / E.g2;
-    R v17 = (-(E.s2b * E.s2b) * ca_g_2 + E.g8 * (g.ca * g.ca) * E.c2b) / E.g2 -
-                 (-(E.s1b * E.s1b) * ca_g_1 + E.g8 * (g.ca * g.ca) * E.c1b) / E.g1;
-    R v18 = ((E.s2b * E.s2b) * cta2 + E.g8 * (g.ca * g.cb * E.c2b)) / E.g2 -
-                 ((E.s1b * E.s1b) * cta1 + E.g8 * (g.ca * g.cb * E.c1b)) / E.g1;
-    R v19 = ((E.s2b * E.s2b) * ctb2 + E.g8 * (g.cb * g.cb * E.c2b)) / E.g2 -
-                 ((E.s1b * E.s1b) * ctb1 + E.g8 * (g.cb * g.cb * E.c1b)) / E.g1;
-    R w14 = F / E.ADD * (E.c1b / E.g1 - E.c2b / E.g2);
     if (tc) {  // coeffG_TC.m:44-136
         R lnS = (rhoS / RS) - log((rhoS + RS) * iz);
         R q = (RS * RS + z2) / RS;
         R iRS3 = R(1.0) / (R(3.0) * RS * RS * RS);
         R rS3 = rhoS * rhoS * rhoS / (R(3.0) * z2 * RS * RS * RS);
         R tail = (R(1.0) / RS) - z2 * iRS3;
-        G1 = -(RS * T1t) + zeta * E.g7 + F * E.g3;
-        G4 = (R(1.0) / RS) * T1t - iz * E.g7;
-        G5 = h5 + lnS * (cgt2 - cgt1);
-        G6 = h6 - lnS * (ct2 - ct1);
-        G7 = zeta * E.g7 - q * P.p5 + f7;
-        G8 = -zeta * g.cg * E.g7 - q * P.p4 + f8;
-        G9 = zeta * E.g7 - q * P.p3 + f9;
-        G14 = iRS3 * T1t + R(1.0) / (R(3.0) * z2) * (w14 - iz * E.g7);
-        G15 = -rS3 * P.p2 + R(1.0) / (R(3.0) * z2) * u15;
-        G16 = -rS3 * P.p1 + R(1.0) / (R(3.0) * z2) * u16;
-        G17 = R(1.0) / R(3.0) * v17 / F - E.g7 / (R(3.0) * zeta) + tail * P.p5;
-        G18 = R(1.0) / R(3.0) * v18 / F + E.g7 / (R(3.0) * zeta) * g.cg + tail * P.p4;
-        G19 = R(1.0) / R(3.0) * v19 / F - E.g7 / (R(3.0) * zeta) + tail * P.p3;
+        G1 = -(RS * T1t) + zeta * X.g7 + F * X.g3;
+        G4 = (R(1.0) / RS) * T1t - iz * X.g7;
+        G5 = X.h5 + lnS * X.dcgt;
+        G6 = X.h6 - lnS * X.dct;
+        G7 = zeta * X.g7 - q * P.p5 + X.f7;
+        G8 = -zeta * g.cg * X.g7 - q * P.p4 + X.f8;
+        G9 = zeta * X.g7 - q * P.p3 + X.f9;
+        G14 = iRS3 * T1t + R(1.0) / (R(3.0) * z2) * (X.w14 - iz * X.g7);
+        G15 = -rS3 * P.p2 + R(1.0) / (R(3.0) * z2) * X.u15;
+        G16 = -rS3 * P.p1 + R(1.0) / (R(3.0) * z2) * X.u16;
+        G17 = R(1.0) / R(3.0) * X.v17 / F - X.g7 / (R(3.0) * zeta) + tail * P.p5;
+        G18 = R(1.0) / R(3.0) * X.v18 / F + X.g7 / (R(3.0) * zeta) * g.cg + tail * P.p4;
+        G19 = R(1.0) / R(3.0) * X.v19 / F - X.g7 / (R(3.0) * zeta) + tail * P.p3;
     } else {  // coeffG_TR_P.m:63-119
-        G1 = zeta * (-T1t + E.g7) + F * E.g3;
-        G4 = iz * (T1t - E.g7);
-        G5 = h5;
-        G6 = h6;
-        G7 = zeta * (E.g7 - R(2.0) * P.p5) + f7;
-        G8 = -zeta * (g.cg * E.g7 + R(2.0) * P.p4) + f8;
-        G9 = zeta * (E.g7 - R(2.0) * P.p3) + f9;
-        G14 = R(1.0) / (R(3.0) * z2) * (iz * (T1t - E.g7) + w14);
-        G15 = R(1.0) / (R(3.0) * z2) * u15;
-        G16 = R(1.0) / (R(3.0) * z2) * u16;
-        G17 = R(1.0) / (R(3.0) * F) * v17 - R(1.0) / (R(3.0) * zeta) * (E.g7 - R(2.0) * P.p5);
-        G18 = R(1.0) / (R(3.0) * F) * v18 + R(1.0) / (R(3.0) * zeta) * (E.g7 * g.cg + R(2.0) * P.p4);
-        G19 = R(1.0) / (R(3.0) * F) * v19 - R(1.0) / (R(3.0) * zeta) * (E.g7 - R(2.0) * P.p3);
+        G1 = zeta * (-T1t + X.g7) + F * X.g3;
+        G4 = iz * (T1t - X.g7);
+        G5 = X.h5;
+        G6 = X.h6;
+        G7 = zeta * (X.g7 - R(2.0) * P.p5) + X.f7;
+        G8 = -zeta * (g.cg * X.g7 + R(2.0) * P.p4) + X.f8;
+        G9 = zeta * (X.g7 - R(2.0) * P.p3) + X.f9;
+        G14 = R(1.0) / (R(3.0) * z2) * (iz * (T1t - X.g7) + X.w14);
+        G15 = R(1.0) / (R(3.0) * z2) * X.u15;
+        G16 = R(1.0) / (R(3.0) * z2) * X.u16;
+        G17 = R(1.0) / (R(3.0) * F) * X.v17 - R(1.0) / (R(3.0) * zeta) * (X.g7 - R(2.0) * P.p5);
+        G18 = R(1.0) / (R(3.0) * F) * X.v18 + R(1.0) / (R(3.0) * zeta) * (X.g7 * g.cg + R(2.0) * P.p4);
+        G19 = R(1.0) / (R(3.0) * F) * X.v19 - R(1.0) / (R(3.0) * zeta) * (X.g7 - R(2.0) * P.p3);
     }
     // TC.m:18-29 / TR_P.m:65-76
     A.s += K.k1 * G1 + K.k3 * G4;
@@ -307,11 +328,26 @@ __host__ __device__ inline void region_edgeP(bool tc, R RS, R rhoS, R zeta, R th
     A.b[5] += K.k2 * G9 + K.k4 * G19;
 }
 
-// Coplanar (zeta <= 1e-6) branch of coeffG_TC.m:138-181 (TC.m:31-49).
 template <typename R>
-__host__ __device__ inline void region_tc_coplanar(R RS, R th1, R th2, const ChildGeoT<R>& g,
-                                          const Phi<R>& P, const LevelT<R>& K, ChildAcc& A) {
-    R T1t = th2 - th1;
+__host__ __device__ inline void region_edgeP(bool tc, R RS, R rhoS, R zeta, R th1, R th2,
+                                   const ChildGeoT<R>& g, const Phi<R>& P, const LevelT<R>& K, ChildAcc& A) {
+    const EdgePre<R> X = edge_pre<R>(zeta, th1, th2, g);
+    edge_fin<R>(tc, RS, rhoS, zeta, X, g, P, K, A);
+}
+
+// Coplanar (zeta <= 1e-6) branch of coeffG_TC.m:138-181 (TC.m:31-49), split
+// like region_edgeP into the level-independent terms and the rest: each G is
+// (level-independent term) +/- (term in RS), the same expression tree as the
+// one-piece form.
+template <typename R>
+struct CopPre {
+    R T1t, Fg3, X4, Y7, Y8, Y9, Z17, Z18, Z19;
+};
+
+template <typename R>
+__host__ __device__ inline CopPre<R> cop_pre(R th1, R th2, const ChildGeoT<R>& g) {
+    CopPre<R> X;
+    X.T1t = th2 - th1;
     const R F = g.a * g.sb;
     R c1b = cos(th1 + g.be), c2b = cos(th2 + g.be);
     R s1b = sin(th1 + g.be), s2b = sin(th2 + g.be);
@@ -319,25 +355,44 @@ __host__ __device__ inline void region_tc_coplanar(R RS, R th1, R th2, const Chi
     R ca_g_1 = cos(g.al - g.ga + th1), ca_g_2 = cos(g.al - g.ga + th2);
     R cta1 = cos(th1 + g.al), cta2 = cos(th2 + g.al);
     R ctb1 = cos(g.be - th1), ctb2 = cos(g.be - th2);
-    R iRS = R(1.0) / RS;
-    R G1 = F * g_3 - RS * T1t;
-    R G4 = R(1.0) / F * (c2b - c1b) + iRS * T1t;
-    R G7 = -RS * P.p5 + F * (ca_g_2 - ca_g_1 + g.sa * g.sa * g_3);
-    R G8 = -RS * P.p4 + F * (cta1 - cta2 - g.sa * g.sb * g_3);
-    R G9 = -RS * P.p3 + F * (ctb1 - ctb2 + g.sb * g.sb * g_3);
+    X.Fg3 = F * g_3;
+    X.X4 = R(1.0) / F * (c2b - c1b);
+    X.Y7 = F * (ca_g_2 - ca_g_1 + g.sa * g.sa * g_3);
+    X.Y8 = F * (cta1 - cta2 - g.sa * g.sb * g_3);
+    X.Y9 = F * (ctb1 - ctb2 + g.sb * g.sb * g_3);
     R i3F = R(1.0) / (R(3.0) * F);
-    R G17 = i3F * (((R(1.0) + g.ca * g.ca) * c2b - (s2b * s2b) * ca_g_2) -
-                        ((R(1.0) + g.ca * g.ca) * c1b - (s1b * s1b) * ca_g_1)) +
-                 iRS * P.p5;
+    X.Z17 = i3F * (((R(1.0) + g.ca * g.ca) * c2b - (s2b * s2b) * ca_g_2) -
+                   ((R(1.0) + g.ca * g.ca) * c1b - (s1b * s1b) * ca_g_1));
     R cab = g.ca * g.cb + cos(g.al + g.be);
-    R G18 = i3F * ((c2b * cab + (s2b * s2b) * cta2) - (c1b * cab + (s1b * s1b) * cta1)) + iRS * P.p4;
-    R G19 = i3F * (((R(1.0) + g.cb * g.cb) * c2b + (s2b * s2b) * ctb2) -
-                        ((R(1.0) + g.cb * g.cb) * c1b + (s1b * s1b) * ctb1)) +
-                 iRS * P.p3;
+    X.Z18 = i3F * ((c2b * cab + (s2b * s2b) * cta2) - (c1b * cab + (s1b * s1b) * cta1));
+    X.Z19 = i3F * (((R(1.0) + g.cb * g.cb) * c2b + (s2b * s2b) * ctb2) -
+                   ((R(1.0) + g.cb * g.cb) * c1b + (s1b * s1b) * ctb1));
+    return X;
+}
+
+template <typename R>
+__host__ __device__ inline void cop_fin(R RS, const CopPre<R>& X, const Phi<R>& P, const LevelT<R>& K,
+                                        ChildAcc& A) {
+    R iRS = R(1.0) / RS;
+    R G1 = X.Fg3 - RS * X.T1t;
+    R G4 = X.X4 + iRS * X.T1t;
+    R G7 = -RS * P.p5 + X.Y7;
+    R G8 = -RS * P.p4 + X.Y8;
+    R G9 = -RS * P.p3 + X.Y9;
+    R G17 = X.Z17 + iRS * P.p5;
+    R G18 = X.Z18 + iRS * P.p4;
+    R G19 = X.Z19 + iRS * P.p3;
     A.s += K.k1 * G1 + K.k3 * G4;
     A.b[3] += K.k2 * G7 + K.k4 * G17;
     A.b[4] += K.k2 * G8 + K.k4 * G18;
     A.b[5] += K.k2 * G9 + K.k4 * G19;
+}
+
+template <typename R>
+__host__ __device__ inline void region_tc_coplanar(R RS, R th1, R th2, const ChildGeoT<R>& g,
+                                          const Phi<R>& P, const LevelT<R>& K, ChildAcc& A) {
+    const CopPre<R> X = cop_pre<R>(th1, th2, g);
+    cop_fin<R>(RS, X, P, K, A);
 }
 
 // TR_SP (S disk part bounded by the edge, coeffG_TR_SP.m, TR_SP.m).
@@ -711,8 +766,129 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_kp(cons
     return true;
 }
 
+// Levels sharing one evaluation of the level-independent region terms in
+// node_eval_bf.
+constexpr int kBfChunk = 4;
+
+// Outer point between the fronts at each of the n <= C levels t[0..n) (in
+// increasing order): the P circle covers each child's far edge (TH2 = 0 and
+// TH5 = gamma in child_regions; checked at t[0], where the circle is smallest)
+// and the S sphere is short of the field triangle (class kP or kSPP of
+// classify() at every level), so each child has exactly one region, [0, gamma]:
+// E7 by coeffG_TR_P.m in class kP, by coeffG_TC.m in class kSPP, by its
+// coplanar branch when zeta = 0.  The level-independent terms (child
+// geometry, angular integrals, edge functions) are evaluated once per child
+// and finished per level with node_eval's own expressions, so acc[l] receives
+// exactly what node_eval adds at t[l].  Returns false (acc then partly
+// updated) if a level's class or the covering check does not hold.
+template <int C>
+__host__ __device__ __attribute__((always_inline)) inline bool node_eval_bf(const NodeGeo& G, const double* t, int n,
+                                                                            double cP, double cS, double w,
+                                                                            NodeAcc* acc) {
+    int type[C];
+    double rhoS[C], rhoP0 = 0.0;
+#pragma unroll
+    for (int l = 0; l < C; l++) {
+        type[l] = kP;
+        rhoS[l] = 0.0;
+        if (l < n) {
+            const Level K = make_level(t[l], cP, cS);
+            double rp, rs;
+            type[l] = classify(G.zeta, K.rP, K.rS, G.dMIN, rp, rs);
+            if (type[l] != kP && type[l] != kSPP) return false;
+            if (l == 0) rhoP0 = rp;
+            rhoS[l] = rs;
+        }
+    }
+    const double zeta = G.zeta;
+    const double zs = zeta * G.sgn;
+    const bool zpos = zeta > 1.0e-6;
+#pragma unroll 1
+    for (int k = 0; k < 3; k++) {
+        double Cs, v2x, v2y, v3x, v3y;
+        if (!fan_child(G, k, Cs, v2x, v2y, v3x, v3y)) continue;
+        const ChildGeo g = child_geo(v2x, v2y, v3x, v3y);
+        // front cutting of child_regions by the P circle at t[0]
+        const double a = g.a;
+        const double vt0 = (g.b * g.cg - a) / g.c;
+        const double Bm = a * vt0;
+        const double DeltaP = Bm * Bm - (a * a - rhoP0 * rhoP0);
+        if (!(DeltaP > 1.0e-6)) return false;
+        const double as = asin(clamp1(a * g.sb / rhoP0));
+        const double O1 = as - g.be, O2 = kPi - as - g.be;
+        if (fmin(fmax(0.0, fmin(O1, O2)), g.ga) != 0.0 || fmax(0.0, fmin(fmax(O1, O2), g.ga)) != g.ga) return false;
+        const Phi<double> P = make_phi<double>(0.0, g.ga, g);
+        const double ws = w * msign(Cs);
+        if (zpos) {
+            const EdgePre<double> X = edge_pre<double>(zeta, 0.0, g.ga, g);
+#pragma unroll
+            for (int l = 0; l < C; l++) {
+                if (l < n) {
+                    const Level K = make_level(t[l], cP, cS);
+                    ChildAcc A;
+                    A.s = 0.0;
+                    for (int i = 0; i < 6; i++) A.b[i] = 0.0;
+                    edge_fin<double>(type[l] != kP, K.rS, rhoS[l], zeta, X, g, P, K, A);
+                    add_child(A, g, v2x, v2y, zs, ws, acc[l]);
+                }
+            }
+        } else {
+            const CopPre<double> X = cop_pre<double>(0.0, g.ga, g);
+#pragma unroll
+            for (int l = 0; l < C; l++) {
+                if (l < n) {
+                    const Level K = make_level(t[l], cP, cS);
+                    ChildAcc A;
+                    A.s = 0.0;
+                    for (int i = 0; i < 6; i++) A.b[i] = 0.0;
+                    cop_fin<double>(K.rS, X, P, K, A);
+                    add_child(A, g, v2x, v2y, zs, ws, acc[l]);
+                }
+            }
+        }
+    }
+    return true;
+}
+
+// Distance from x to the triangle (v0, v1, v2) in 3D (vF: 9 doubles): the
+// plane distance if the projection falls inside, else the nearest edge.
+__host__ __device__ inline double tri_dist3(double x0, double x1, double x2, const double* vF) {
+    double e[3][3], d[3];
+    for (int c = 0; c < 3; c++) {
+        e[0][c] = vF[3 + c] - vF[c];
+        e[1][c] = vF[6 + c] - vF[3 + c];
+        e[2][c] = vF[c] - vF[6 + c];
+    }
+    const double n0 = e[0][1] * e[1][2] - e[0][2] * e[1][1];
+    const double n1 = e[0][2] * e[1][0] - e[0][0] * e[1][2];
+    const double n2 = e[0][0] * e[1][1] - e[0][1] * e[1][0];
+    bool inside = true;
+    double best = 1e300;
+    for (int k = 0; k < 3; k++) {
+        const double* A = vF + 3 * k;
+        const double p0 = x0 - A[0], p1 = x1 - A[1], p2 = x2 - A[2];
+        // side of the edge within the plane: (e_k x p) . n
+        const double c0 = e[k][1] * p2 - e[k][2] * p1, c1 = e[k][2] * p0 - e[k][0] * p2,
+                     c2 = e[k][0] * p1 - e[k][1] * p0;
+        if (c0 * n0 + c1 * n1 + c2 * n2 < 0.0) inside = false;
+        const double ee = e[k][0] * e[k][0] + e[k][1] * e[k][1] + e[k][2] * e[k][2];
+        double u = (p0 * e[k][0] + p1 * e[k][1] + p2 * e[k][2]) / ee;
+        u = fmin(1.0, fmax(0.0, u));
+        const double q0 = p0 - u * e[k][0], q1 = p1 - u * e[k][1], q2 = p2 - u * e[k][2];
+        d[k] = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+        best = fmin(best, d[k]);
+    }
+    if (inside) {
+        const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        return fabs(((x0 - vF[0]) * n0 + (x1 - vF[1]) * n1 + (x2 - vF[2]) * n2) / nn);
+    }
+    return best;
+}
+
 // Levels [ka, kb] of a (test, field) pair whose W_m = WP + t_m^2 WQ
-// (node_eval_kp): every outer node q in class kP, c_S t_m < zeta_q (its
+// (node_eval_kp), and the levels [sa, sb] after them that node_eval_bf
+// evaluates (S sphere of every node short of the field triangle,
+// c_S t_m < dist(x_q, T)): every outer node q in class kP, c_S t_m < zeta_q (its
 // distance to the field plane, zeroed below 1e-6 as in node_geo), and the P
 // sphere past every field vertex, c_P t_m > Dmax.  Relative margins 1e-7 keep
 // the levels clear of both ends (node_eval_kp checks the state again).  Each
@@ -721,10 +897,13 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_kp(cons
 // is cut to the pair's item levels [max(1, l-1), h+1] and dropped below 3
 // levels.  gS: the test triangle's outer nodes as x, y, z rows of 32.
 __host__ __device__ inline void kp_range(int nGH, double cP, double cS, double dt, const double* gS,
-                                         const FieldTri& F, const double* vF, int l, int h, int& ka, int& kb) {
+                                         const FieldTri& F, const double* vF, int l, int h, int& ka, int& kb,
+                                         int& sa, int& sb) {
     ka = 1;
     kb = 0;
-    double cmin2 = 1e300, dmax2 = 0.0, zmin = 1e300;
+    sa = 1;
+    sb = 0;
+    double cmin2 = 1e300, dmax2 = 0.0, zmin = 1e300, d3min = 1e300;
     for (int v = 0; v < 3; v++) {
         const int w = (v + 1) % 3;
         double a = vF[3 * w] - vF[3 * v], b = vF[3 * w + 1] - vF[3 * v + 1], c = vF[3 * w + 2] - vF[3 * v + 2];
@@ -740,14 +919,23 @@ __host__ __device__ inline void kp_range(int nGH, double cP, double cS, double d
         double d0 = x - F.v0[0], d1 = y - F.v0[1], d2 = z - F.v0[2];
         double zq = fabs(F.n[0] * d0 + F.n[1] * d1 + F.n[2] * d2);
         zmin = fmin(zmin, zq < 1.0e-6 ? 0.0 : zq);
+        d3min = fmin(d3min, tri_dist3(x, y, z, vF));
     }
     const double fa = ceil(sqrt(dmax2) * (1.0 + 1e-7) / (cP * dt));
     const double fb = ceil(zmin * (1.0 - 1e-7) / (cS * dt)) - 1.0;
+    const double fs = ceil((d3min * (1.0 - 1e-7) - 1e-9) / (cS * dt)) - 1.0;
     const int mlo = l - 1 < 1 ? 1 : l - 1;
     const double a = fmax(fa, (double)mlo), b = fmin(fb, (double)(h + 1));
     if (b - a + 1.0 >= 3.0) {
         ka = (int)a;
         kb = (int)b;
+    }
+    // levels after the kP range (or from the P covering on) with the S sphere
+    // still short of the field triangle: node_eval_bf, two levels at least
+    const double s0 = ka <= kb ? (double)(kb + 1) : a, s1 = fmin(fs, (double)(h + 1));
+    if (s1 - s0 + 1.0 >= 2.0) {
+        sa = (int)s0;
+        sb = (int)s1;
     }
 }
 

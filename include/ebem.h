@@ -112,20 +112,22 @@ int ebem_store_lag_blocks(ebem_handle *h, int64_t *counts);
  * (the reference has no mixed precision); reports the split of the work. */
 int ebem_assembly_stats(ebem_handle *h, int64_t *far_items, int64_t *fp64_items);
 
-/* kP ranges of the last ebem_assemble: for a pair whose outer nodes all stay
- * in the kP class of calcMatrixSubBlock.m:290-314 (S sphere short of the
- * field plane, P sphere past the whole field triangle) over consecutive
- * levels, W_m = WP + t_m^2 WQ with WP, WQ from one evaluation of the
- * t-independent region coefficients, instead of one evaluation per level.
- * ranges: such (pair, range) units; levels: the levels they cover;
- * fallbacks: ranges evaluated level by level because a node failed the state
- * check (0 expected).  No reference counterpart (reports the split). */
-int ebem_assembly_kp_stats(ebem_handle *h, int64_t *ranges, int64_t *levels, int64_t *fallbacks);
+/* Level ranges of the last ebem_assemble evaluated per pair rather than per
+ * level.  kP ranges: every outer node in class kP of
+ * calcMatrixSubBlock.m:290-314 (S sphere short of the field plane, P sphere
+ * past the whole field triangle), where W_m = WP + t_m^2 WQ from one
+ * evaluation of the t-independent region coefficients.  Between-front ranges
+ * (after them, S sphere still short of the field triangle): the region terms
+ * that do not depend on the level are evaluated once per few levels and
+ * finished per level with the reference's expressions (the same values as
+ * the level-by-level evaluation).  out5 = {kP ranges, levels in kP ranges,
+ * between-front ranges, levels in them, ranges that fell back to the
+ * level-by-level evaluation (0 expected)}.  No reference counterpart. */
+int ebem_assembly_kp_stats(ebem_handle *h, int64_t *out5);
 
-/* The same for the last ebem_pp_eval (analytic form): M(x, tau_m) =
- * WP + tau_m^2 WQ over the kP ranges of m of each (point, field triangle)
- * pair.  No reference counterpart (reports the split). */
-int ebem_pp_kp_stats(ebem_handle *h, int64_t *ranges, int64_t *levels, int64_t *fallbacks);
+/* The same for the last ebem_pp_eval (analytic form), over the ranges of m
+ * of each (point, field triangle) pair.  No reference counterpart. */
+int ebem_pp_kp_stats(ebem_handle *h, int64_t *out5);
 
 /* Export E_lag as a dense 3N x 3N column-major host matrix (the `matrix` of
  * calcMatrixBlock.m / matrixSavedV{lag+1} of timeMarchingD.m:141-158). */
@@ -176,8 +178,9 @@ int ebem_node_integral(ebem_handle *h, const double *x, int tri, double t, doubl
  * out[2] E_0 factorisation, out[3] history-sum kernel total, out[4] E_0 apply
  * total, out[5] MOT total, out[6] post-processing total, out[7] history-sum
  * launches, out[8] assembly kP-range kernel, out[9] post-processing kP-range
- * kernel. */
-#define EBEM_NUM_TIMINGS 10
+ * kernel, out[10] assembly between-front kernel, out[11] post-processing
+ * between-front kernel. */
+#define EBEM_NUM_TIMINGS 12
 int ebem_timings(ebem_handle *h, double *out, int n);
 
 /* ------------------------------------------------------------------------

@@ -97,4 +97,33 @@ long long opcount_kp(const double* rec, const double* gh, int nGH, double ta, do
     finish_rotation(aQ, F.C, WQ);
     return g_ops;
 }
+
+// Flops of one between-front range of n levels from m0 as k_levels_bf
+// evaluates it (node_eval_bf in chunks of kBfChunk, the rotations per level).
+// -1 if a chunk fails the state check.
+long long opcount_bf(const double* rec, const double* gh, int nGH, int m0, int n, double dt, double cP, double cS) {
+    FieldTri F;
+    std::memcpy((void*)&F, rec, sizeof(FieldTri));
+    g_ops = 0;
+    for (int c0 = 0; c0 < n; c0 += kBfChunk) {
+        const int nc = n - c0 < kBfChunk ? n - c0 : kBfChunk;
+        cdouble t[kBfChunk];
+        for (int l = 0; l < kBfChunk; l++) t[l] = cdouble((double)(m0 + c0 + (l < nc ? l : 0)) * dt);
+        NodeAcc acc[kBfChunk];
+        for (int l = 0; l < kBfChunk; l++) {
+            acc[l].s = 0.0;
+            for (int q = 0; q < 6; q++) acc[l].m[q] = 0.0;
+        }
+        for (int q = 0; q < nGH; q++) {
+            cdouble x[3] = {gh[q], gh[32 + q], gh[64 + q]};
+            NodeGeo G = node_geo(x, F);
+            if (!node_eval_bf<kBfChunk>(G, t, nc, cdouble(cP), cdouble(cS), cdouble(gh[96 + q]), acc)) return -1;
+        }
+        for (int l = 0; l < nc; l++) {
+            cdouble W[6];
+            finish_rotation(acc[l], F.C, W);
+        }
+    }
+    return g_ops;
+}
 }

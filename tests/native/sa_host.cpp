@@ -52,14 +52,41 @@ void sa_item(const double* rec, const double* gh, int nGH, double t, double cP, 
 }
 
 extern "C" {
-// kP range of a pair as k_windows computes it (kp_range): rec = field record,
-// gh = test nodes (x, y, z, w rows of 32), vF = field vertices (9), [l, h] the
-// pair's lag window.  Writes ka, kb (ka > kb: none).
+// kP and between-front ranges of a pair as k_windows computes them
+// (kp_range): rec = field record, gh = test nodes (x, y, z, w rows of 32),
+// vF = field vertices (9), [l, h] the pair's lag window.  Writes ka, kb and
+// sa, sb (first > last: none).
 void sa_kp_range(const double* rec, const double* gh, const double* vF, int nGH, double cP, double cS, double dt,
-                 int l, int h, int* ka, int* kb) {
+                 int l, int h, int* ka, int* kb, int* sa, int* sb) {
     FieldTri F;
     std::memcpy(&F, rec, sizeof(FieldTri));
-    kp_range(nGH, cP, cS, dt, gh, F, vF, l, h, *ka, *kb);
+    kp_range(nGH, cP, cS, dt, gh, F, vF, l, h, *ka, *kb, *sa, *sb);
+}
+
+// W (6 values per level, as sa_item) of the levels m0 .. m0+n-1 by
+// node_eval_bf in chunks of kBfChunk, as k_levels_bf computes them; returns
+// 0 if a chunk fails the state check.
+int sa_item_bf(const double* rec, const double* gh, int nGH, int m0, int n, double dt, double cP, double cS,
+               double* W) {
+    FieldTri F;
+    std::memcpy(&F, rec, sizeof(FieldTri));
+    for (int c0 = 0; c0 < n; c0 += kBfChunk) {
+        const int nc = n - c0 < kBfChunk ? n - c0 : kBfChunk;
+        double t[kBfChunk];
+        for (int l = 0; l < kBfChunk; l++) t[l] = (double)(m0 + c0 + (l < nc ? l : 0)) * dt;
+        NodeAcc acc[kBfChunk];
+        for (int l = 0; l < kBfChunk; l++) {
+            acc[l].s = 0.0;
+            for (int q = 0; q < 6; q++) acc[l].m[q] = 0.0;
+        }
+        for (int q = 0; q < nGH; q++) {
+            double x[3] = {gh[q], gh[32 + q], gh[64 + q]};
+            NodeGeo G = node_geo(x, F);
+            if (!node_eval_bf<kBfChunk>(G, t, nc, cP, cS, gh[96 + q], acc)) return 0;
+        }
+        for (int l = 0; l < nc; l++) finish_rotation(acc[l], F.C, W + 6 * (c0 + l));
+    }
+    return 1;
 }
 
 // WP, WQ of a kP range [ta, tb] as k_levels_kp computes them; returns 0 if a

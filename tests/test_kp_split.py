@@ -33,7 +33,9 @@ def setup():
     lib.sa_item_kp.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]
     lib.sa_item_kp.restype = C.c_int
     lib.sa_kp_range.argtypes = [_dp, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
-                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
+                                C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.sa_item_bf.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp]
+    lib.sa_item_bf.restype = C.c_int
     m = read_mesh(os.path.join(ROOT, "tests", "golden", "meshes", "sphereUniform_2.mesh"))
     o = Oracle(m.coordinates, m.triangles, Tfin=4.0, nT=128)
     lo, hi = exact_windows(o, o.num_blocks())
@@ -74,9 +76,9 @@ def test_kp_split_matches_levels_and_oracle(setup):
         gh = _gh(o, i)
         rec = o.field_record(o.coords[o.tri[j]], o.normal[j])
         vF = np.ascontiguousarray(o.coords[o.tri[j]].ravel())
-        ka, kb = C.c_int(), C.c_int()
+        ka, kb, sa, sb = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         lib.sa_kp_range(rec.ctypes.data_as(_dp), gh.ctypes.data_as(_dp), vF.ctypes.data_as(_dp), o.nGH, o.velP,
-                        o.velS, dt, int(lo[i, j]), int(hi[i, j]), C.byref(ka), C.byref(kb))
+                        o.velS, dt, int(lo[i, j]), int(hi[i, j]), C.byref(ka), C.byref(kb), C.byref(sa), C.byref(sb))
         a, b = ka.value, kb.value
         if a > b:
             continue
@@ -137,9 +139,10 @@ def test_kp_range_none_when_coplanar(setup):
                 continue
             rec = s.field_record(s.coords[s.tri[j]], s.normal[j])
             vF = np.ascontiguousarray(s.coords[s.tri[j]].ravel())
-            ka, kb = C.c_int(), C.c_int()
+            ka, kb, sa, sb = C.c_int(), C.c_int(), C.c_int(), C.c_int()
             lib.sa_kp_range(rec.ctypes.data_as(_dp), gh.ctypes.data_as(_dp), vF.ctypes.data_as(_dp), s.nGH, s.velP,
-                            s.velS, s.dt, int(l2[i, j]), int(h2[i, j]), C.byref(ka), C.byref(kb))
+                            s.velS, s.dt, int(l2[i, j]), int(h2[i, j]), C.byref(ka), C.byref(kb), C.byref(sa),
+                            C.byref(sb))
             assert ka.value > kb.value
 
 
