@@ -181,3 +181,50 @@ def test_pp_point_kp_form_matches_oracle(setup):
     print(f"{n_ranges} point-triangle kP ranges, worst rel {worst:.1e}")
     assert n_ranges >= 50
     assert worst < 1e-10
+
+
+@pytest.mark.parametrize("which", ["sphere", "screen"])
+def test_between_front_ranges_bit_identical(setup, which):
+    """Between-front ranges (k_levels_bf): node_eval_bf in chunks of levels gives
+    bit for bit the W_m of the level-by-level evaluation (node_eval), on the
+    sphere (classes kP and kSPP, zeta > 0) and on a flat screen (coplanar
+    branch of coeffG_TC.m)."""
+    lib, o, lo, hi = setup
+    if which == "screen":
+        from elastodynamicebem_amd.mesh import screen_uniform
+        from hostasm import exact_windows
+        from oracle import Oracle
+        m = screen_uniform(8)
+        o = Oracle(m.coordinates, m.triangles, Tfin=2.0, nT=64)
+        lo, hi = exact_windows(o, o.num_blocks())
+    rng = np.random.default_rng(17)
+    ranges = levels = 0
+    for _ in range(4000):
+        i, j = (int(x) for x in rng.integers(0, o.N, 2))
+        if lo[i, j] > hi[i, j]:
+            continue
+        gh = _gh(o, i)
+        rec = o.field_record(o.coords[o.tri[j]], o.normal[j])
+        vF = np.ascontiguousarray(o.coords[o.tri[j]].ravel())
+        ka, kb, sa, sb = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        lib.sa_kp_range(rec.ctypes.data_as(_dp), gh.ctypes.data_as(_dp), vF.ctypes.data_as(_dp), o.nGH, o.velP,
+                        o.velS, o.dt, int(lo[i, j]), int(hi[i, j]), C.byref(ka), C.byref(kb), C.byref(sa),
+                        C.byref(sb))
+        a, b = sa.value, sb.value
+        if a > b:
+            continue
+        assert b - a + 1 >= 2 and max(1, lo[i, j] - 1) <= a and b <= hi[i, j] + 1
+        assert ka.value > kb.value or a == kb.value + 1
+        n = b - a + 1
+        W = np.zeros(6 * n)
+        assert lib.sa_item_bf(rec.ctypes.data_as(_dp), gh.ctypes.data_as(_dp), o.nGH, a, n, o.dt, o.velP, o.velS,
+                              W.ctypes.data_as(_dp)) == 1
+        for k in range(n):
+            w = np.zeros(6)
+            lib.sa_item(rec.ctypes.data_as(_dp), gh.ctypes.data_as(_dp), o.nGH, (a + k) * o.dt, o.velP, o.velS,
+                        w.ctypes.data_as(_dp), 0)
+            assert np.array_equal(w.view(np.int64), W[6 * k:6 * k + 6].view(np.int64)), (i, j, a + k)
+        ranges += 1
+        levels += n
+    print(f"{which}: {ranges} between-front ranges, {levels} levels bit-identical")
+    assert ranges >= 100
