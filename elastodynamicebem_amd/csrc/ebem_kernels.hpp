@@ -252,14 +252,21 @@ constexpr int kFrontKeys = kFrontClasses * kFrontClasses;
 // Key order = order of the sorted permutation: far-field items (mixed
 // precision) 0..kFrontKeys-1, the other per-level items, the first level of
 // each kP range (one k_levels_kp item per pair) and its other levels (no work
-// of their own), the same for the between-front ranges (k_levels_bf),
-// surrogate pairs last.
+// of their own), the same for the between-front ranges (k_levels_bf; their
+// first levels keyed by the number of chunks, so that a wavefront's ranges
+// take similar time), surrogate pairs last.
 constexpr int kKpKey = 2 * kFrontKeys;
 constexpr int kKpRestKey = kKpKey + 1;
-constexpr int kBfKey = kKpKey + 2;
-constexpr int kBfRestKey = kKpKey + 3;
-constexpr int kSurKey = kKpKey + 4;
+constexpr int kBfHeadKeys = 4;
+constexpr int kBfKey = kKpKey + 2;  // .. kBfKey + kBfHeadKeys - 1
+constexpr int kBfRestKey = kBfKey + kBfHeadKeys;
+constexpr int kSurKey = kBfRestKey + 1;
 constexpr int kNumKeys = kSurKey + 1;
+
+__device__ inline int bf_head_key(int a, int b) {
+    const int chunks = (b - a + kBfChunk) / kBfChunk;
+    return kBfKey + (chunks < kBfHeadKeys ? chunks : kBfHeadKeys) - 1;
+}
 static_assert(kNumKeys <= 256, "keys are stored as uint8");
 
 __device__ inline int front_class(double r, double lo, double span) {
@@ -298,7 +305,7 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
         int qs = front_class(P.cS * t, d - ex, 2.0 * ex) + 1;
         int k = qp * kFrontClasses + qs + ((P.far2 >= 0.0 && d2 > P.far2) ? 0 : kFrontKeys);
         if (ka && m >= ka[p] && m <= kb[p]) k = (m == ka[p]) ? kKpKey : kKpRestKey;
-        if (sa && m >= sa[p] && m <= sb[p]) k = (m == sa[p]) ? kBfKey : kBfRestKey;
+        if (sa && m >= sa[p] && m <= sb[p]) k = (m == sa[p]) ? bf_head_key(sa[p], sb[p]) : kBfRestKey;
         if (sflag && sflag[p]) k = kSurKey;
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
@@ -1180,7 +1187,7 @@ __global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tr
         int qs = front_class(P.cS * tau, d - F.maxL, 2.0 * F.maxL) + 1;
         int k = qp * kFrontClasses + qs + kFrontKeys;
         if (ka && m >= ka[q] && m <= kb[q]) k = (m == ka[q]) ? kKpKey : kKpRestKey;
-        if (sa && m >= sa[q] && m <= sb[q]) k = (m == sa[q]) ? kBfKey : kBfRestKey;
+        if (sa && m >= sa[q] && m <= sb[q]) k = (m == sa[q]) ? bf_head_key(sa[q], sb[q]) : kBfRestKey;
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
     }

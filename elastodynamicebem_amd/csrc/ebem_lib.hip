@@ -870,7 +870,7 @@ int assemble(ebem_handle* h, int precision) {
                 nsur += hist[(size_t)kSurKey * G + b];
                 nkp += hist[(size_t)kKpKey * G + b];
                 nkrest += hist[(size_t)kKpRestKey * G + b];
-                nbf += hist[(size_t)kBfKey * G + b];
+                for (int c = 0; c < kBfHeadKeys; c++) nbf += hist[(size_t)(kBfKey + c) * G + b];
                 nbrest += hist[(size_t)kBfRestKey * G + b];
             }
             h->far_items += nfar;
@@ -1309,7 +1309,7 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
                 for (int b = 0; b < G; b++) {
                     nkp += bh[(size_t)kKpKey * G + b];
                     nkrest += bh[(size_t)kKpRestKey * G + b];
-                    nbf += bh[(size_t)kBfKey * G + b];
+                    for (int c = 0; c < kBfHeadKeys; c++) nbf += bh[(size_t)(kBfKey + c) * G + b];
                     nbrest += bh[(size_t)kBfRestKey * G + b];
                 }
             h->pp_kp_stats[0] += nkp;

@@ -768,7 +768,10 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_kp(cons
 
 // Levels sharing one evaluation of the level-independent region terms in
 // node_eval_bf.
-constexpr int kBfChunk = 4;
+#ifndef EBEM_BF_CHUNK
+#define EBEM_BF_CHUNK 6
+#endif
+constexpr int kBfChunk = EBEM_BF_CHUNK;
 
 // Outer point between the fronts at each of the n <= C levels t[0..n) (in
 // increasing order): the P circle covers each child's far edge (TH2 = 0 and
