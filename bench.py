@@ -30,17 +30,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Algorithmic FP64 operations per (pair, level) item of k_levels on this
-# workload (7 outer nodes): mean over the items outside the kP ranges (which
-# k_levels_kp evaluates) in a uniform sample of 200000 of the 43938820 items of
-# the pruned lag windows, counted by the op-counting host build of the device
-# math (python tests/opcount.py 200000 -> 9049.8 +- 9.8; kP share 0.2585,
-# 17393 flops per kP range); each + - * / (a fused multiply-add as 2), sqrt and
-# libm call counts once.
-ALG_FLOP_PER_ITEM = 9049.8
+# workload (7 outer nodes): mean over the items outside the kP and
+# between-front ranges (which k_levels_kp and k_levels_bf evaluate) in a
+# uniform sample of 200000 of the 43938820 items of the pruned lag windows,
+# counted by the op-counting host build of the device math
+# (python tests/opcount.py 200000 -> 8103.8 +- 14.4; kP share 0.2585,
+# between-front share 0.2634); each + - * / (a fused multiply-add as 2), sqrt
+# and libm call counts once.
+ALG_FLOP_PER_ITEM = 8103.8
 # HBM bytes per launch of k_levels on this workload from the rocprofv3 --pmc
 # passes FETCH_SIZE (x2.0: the counter reads 512 MiB for a 1 GiB device copy)
-# and WRITE_SIZE (x1.0): 128.5 GB + 54.2 GB, almost all register-spill scratch.
-K_LEVELS_TRAFFIC_BYTES = 182.7e9
+# and WRITE_SIZE (x1.0): 84.2 GB + 33.3 GB, almost all register-spill scratch.
+K_LEVELS_TRAFFIC_BYTES = 117.5e9
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 

@@ -434,13 +434,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     }
 }
 
+// Waves per SIMD the between-front kernels are compiled for.  One wave gives
+// them 256 VGPRs + 256 AGPRs (the AGPRs hold what would spill to scratch at
+// two waves: 1056 B/lane there): 10.3 vs 16.4 ms (assembly) and 8.8 vs 19.4 ms
+// (post-processing) on the 1280-triangle sphere.
+#ifndef EBEM_BF_WAVES
+#define EBEM_BF_WAVES 1
+#endif
+
 // One thread per between-front range (the items of key kBfKey, the first
 // level sa of the range): levels sa..sb in chunks of kBfChunk through
 // node_eval_bf, which yields k_levels' values with the level-independent
 // region terms evaluated once per chunk.  From a chunk whose state check
 // fails, the rest of the range is evaluated level by level as in k_levels.
 template <int NGH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_levels_bf(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_BF_WAVES))) k_levels_bf(
     const FieldTri* __restrict__ tri, const double* __restrict__ gh, AsmParams P, int r0, int64_t nitems,
     const int32_t* __restrict__ item_pair, const int16_t* __restrict__ sa, const int16_t* __restrict__ sb,
     const int32_t* __restrict__ perm, double* __restrict__ W, unsigned int* __restrict__ nfallback) {
@@ -1016,7 +1024,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 // M_m for m = sa..sb through node_eval_bf (one outer point, weight 1) in
 // chunks of kBfChunk, the same values as k_pp_M; from a chunk whose state
 // check fails, per m as in k_pp_M.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_pp_M_bf(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_BF_WAVES))) k_pp_M_bf(
     const FieldTri* __restrict__ tri, const double* __restrict__ xs, int p0, PPParams P, int64_t nheads,
     const int32_t* __restrict__ item_q, const int32_t* __restrict__ sa, const int32_t* __restrict__ sb,
     const int32_t* __restrict__ perm, double* __restrict__ M, unsigned int* __restrict__ nfallback) {

@@ -769,7 +769,7 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_kp(cons
 // Levels sharing one evaluation of the level-independent region terms in
 // node_eval_bf.
 #ifndef EBEM_BF_CHUNK
-#define EBEM_BF_CHUNK 6
+#define EBEM_BF_CHUNK 8
 #endif
 constexpr int kBfChunk = EBEM_BF_CHUNK;
 
