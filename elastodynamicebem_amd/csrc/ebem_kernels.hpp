@@ -330,6 +330,10 @@ __global__ void __launch_bounds__(256) k_item_scatter(int64_t nitems, int64_t pe
     }
 }
 
+#ifndef EBEM_LEVELS_WAVES
+#define EBEM_LEVELS_WAVES 2
+#endif
+
 // Persistent grid-stride kernel over (pair, level) items of a row chunk.
 // The whole node evaluation is inlined (an out-of-line call spills ~1 KB of
 // stack per lane); a 256-VGPR budget (2 waves per SIMD) is the measured optimum
@@ -337,7 +341,7 @@ __global__ void __launch_bounds__(256) k_item_scatter(int64_t nitems, int64_t pe
 // sphere, 5.9e7 items).
 // W[item*6 + q]: symmetric 3x3 in the global frame, NOT yet divided by 4 pi rho.
 template <int NGH, typename R>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_levels(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_LEVELS_WAVES))) k_levels(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
                                                 AsmParams P, int r0, int64_t nitems,
                                                 const int32_t* __restrict__ item_pair,
                                                 const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
@@ -933,7 +937,7 @@ __global__ void __launch_bounds__(256) k_pp_items(const int32_t* __restrict__ ms
 
 // M_m(x_p, tri j) for every item; 6 symmetric values, /(4 pi rho), entries
 // below 1e-14 zeroed (ppA/BEMenerg_postProcA_calcMatrix.m:20).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_pp_M(const FieldTri* __restrict__ tri, const double* __restrict__ xs, int p0,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_LEVELS_WAVES))) k_pp_M(const FieldTri* __restrict__ tri, const double* __restrict__ xs, int p0,
                                               PPParams P, int64_t nitems, const int32_t* __restrict__ item_q,
                                               const int64_t* __restrict__ off, const int32_t* __restrict__ ms,
                                               const int32_t* __restrict__ perm, double* __restrict__ M) {
