@@ -628,6 +628,9 @@ __device__ inline double wg_sum(double v, double* red4) {
 // full pass per step.
 constexpr int kHistBlock = 8;
 
+// alpha step-major (the blocks of a row segment are in column order, so a
+// wave's reads of one step share cache lines; a triangle-major layout measured
+// 3x slower here).
 __global__ void __launch_bounds__(256) k_history_far(Store S, int N, int L, int n0, int nb,
                                                      const double* __restrict__ alpha, double* __restrict__ part) {
     __shared__ double red4[4];
@@ -709,8 +712,11 @@ __global__ void __launch_bounds__(256) k_history(Store S, int N, int L, int n, i
 }
 
 // y[row] = sum_k A[row][k] x[k]; A row-major n x n; one wave per row, 4 rows per block.
+// If yT is set, the value also goes to yT[((row / 3) nT + step) 3 + row % 3]
+// (the density's triangle-major copy).
 __global__ void __launch_bounds__(256) k_gemv_rows(const double* __restrict__ A, const double* __restrict__ x,
-                                                   double* __restrict__ y, int n) {
+                                                   double* __restrict__ y, int n, double* __restrict__ yT, int nT,
+                                                   int step) {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
@@ -724,7 +730,10 @@ __global__ void __launch_bounds__(256) k_gemv_rows(const double* __restrict__ A,
     if (k < n) s0 += a[k] * x[k];
     double s = s0 + s1;
     for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
-    if (lane == 0) y[row] = s;
+    if (lane == 0) {
+        y[row] = s;
+        if (yT) yT[((int64_t)(row / 3) * nT + step) * 3 + row % 3] = s;
+    }
 }
 
 // A[i + n i] = v for a zeroed n x n array (identity for v = 1).
@@ -1089,29 +1098,39 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_B
     }
 }
 
-// One workgroup per point: thread (slice s, time k) sums the triangles
-// j = s, s + S, s + 2S, ... for requested time k (nhat[k]); the S slice
-// partials are reduced through LDS.  aT: density as [triangle][step][3].
+// One workgroup per point, one wave per slice of triangles (j = s, s + S,
+// s + 2S, ...); lane l of a wave takes the requested times kb + l + 64 i,
+// i < kConvPerLane, so one pass covers 64 kConvPerLane times.  The window and
+// the M values of a (point, triangle) pair are wave-uniform (scalar loads, one
+// difference D_m per wave); only the density reads and the accumulation are
+// per lane.  The slice partials are reduced through LDS.  aT: density as
+// [triangle][step][3].
 constexpr int kConvSlices = 8;
+constexpr int kConvPerLane = 2;
 
-__global__ void __launch_bounds__(1024) k_pp_conv(const double* __restrict__ M, const int64_t* __restrict__ off,
-                                                  const int32_t* __restrict__ ms, const int32_t* __restrict__ me,
-                                                  const double* __restrict__ aT, int N, int nT, int p0,
-                                                  const int32_t* __restrict__ nhat, const int32_t* __restrict__ kidx,
-                                                  int nk, int npts, double* __restrict__ u) {
-    __shared__ double red[kConvSlices][128][3];
+__global__ void __launch_bounds__(64 * kConvSlices) k_pp_conv(const double* __restrict__ M,
+                                                            const int64_t* __restrict__ off,
+                                                            const int32_t* __restrict__ ms,
+                                                            const int32_t* __restrict__ me,
+                                                            const double* __restrict__ aT, int N, int nT, int p0,
+                                                            const int32_t* __restrict__ nhat,
+                                                            const int32_t* __restrict__ kidx, int nk, int npts,
+                                                            double* __restrict__ u) {
+    constexpr int KP = 64 * kConvPerLane;  // times per pass
+    __shared__ double red[kConvSlices][KP][3];
     const int r = blockIdx.x;
     const int p = p0 + r;
-    const int lane_k = threadIdx.x % 128;
-    // the slice is uniform over a wave (128 threads per slice): the window and
-    // M loads below then have wave-uniform addresses (scalar loads), only the
-    // density loads and the accumulation are per lane
-    const int slice = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 128));  // blockDim = 128 * kConvSlices
-    for (int kb = 0; kb < nk; kb += 128) {
-        const int kk = kb + lane_k;
-        const bool act = kk < nk;
-        const int nh = act ? nhat[kk] : -1;
-        double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+    const int lane = threadIdx.x % 64;
+    const int slice = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));  // blockDim = 64 * kConvSlices
+    for (int kb = 0; kb < nk; kb += KP) {
+        int nh[kConvPerLane];
+        double acc[kConvPerLane][3];
+#pragma unroll
+        for (int i = 0; i < kConvPerLane; i++) {
+            const int kk = kb + i * 64 + lane;
+            nh[i] = kk < nk ? nhat[kk] : -1;
+            acc[i][0] = acc[i][1] = acc[i][2] = 0.0;
+        }
         for (int j = slice; j < N; j += kConvSlices) {
             const int64_t q = (int64_t)r * N + j;
             const int a = ms[q], b = me[q];
@@ -1127,27 +1146,35 @@ __global__ void __launch_bounds__(1024) k_pp_conv(const double* __restrict__ M, 
                     d[k] = Mm[k] - pv[k];
                     pv[k] = Mm[k];
                 }
-                if (m <= nh) {
-                    const double* al = aj + (int64_t)(nh - m) * 3;
-                    const double x0 = al[0], x1 = al[1], x2 = al[2];
-                    u0 += d[0] * x0 + d[1] * x1 + d[2] * x2;
-                    u1 += d[1] * x0 + d[3] * x1 + d[4] * x2;
-                    u2 += d[2] * x0 + d[4] * x1 + d[5] * x2;
+#pragma unroll
+                for (int i = 0; i < kConvPerLane; i++) {
+                    if (m <= nh[i]) {
+                        const double* al = aj + (int64_t)(nh[i] - m) * 3;
+                        const double x0 = al[0], x1 = al[1], x2 = al[2];
+                        acc[i][0] += d[0] * x0 + d[1] * x1 + d[2] * x2;
+                        acc[i][1] += d[1] * x0 + d[3] * x1 + d[4] * x2;
+                        acc[i][2] += d[2] * x0 + d[4] * x1 + d[5] * x2;
+                    }
                 }
             }
         }
-        red[slice][lane_k][0] = u0;
-        red[slice][lane_k][1] = u1;
-        red[slice][lane_k][2] = u2;
+#pragma unroll
+        for (int i = 0; i < kConvPerLane; i++) {
+            red[slice][i * 64 + lane][0] = acc[i][0];
+            red[slice][i * 64 + lane][1] = acc[i][1];
+            red[slice][i * 64 + lane][2] = acc[i][2];
+        }
         __syncthreads();
-        if (slice == 0 && act) {
+        for (int t = threadIdx.x; t < KP; t += blockDim.x) {
+            const int kk = kb + t;
+            if (kk >= nk) continue;
             double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-            for (int q = 0; q < kConvSlices; q++) {
-                s0 += red[q][lane_k][0];
-                s1 += red[q][lane_k][1];
-                s2 += red[q][lane_k][2];
+            for (int sl = 0; sl < kConvSlices; sl++) {
+                s0 += red[sl][t][0];
+                s1 += red[sl][t][1];
+                s2 += red[sl][t][2];
             }
-            int64_t k = kidx[kk];
+            const int64_t k = kidx[kk];
             u[3 * ((int64_t)p + (int64_t)npts * k) + 0] = s0;
             u[3 * ((int64_t)p + (int64_t)npts * k) + 1] = s1;
             u[3 * ((int64_t)p + (int64_t)npts * k) + 2] = s2;

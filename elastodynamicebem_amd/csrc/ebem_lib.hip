@@ -411,7 +411,7 @@ struct ebem_handle {
     DevBuf<int32_t> d_col;
     DevBuf<double> d_val;  // 6 * nnzb
     // MOT
-    DevBuf<double> d_beta, d_alpha, d_r, d_A, d_Ainv, d_part;
+    DevBuf<double> d_beta, d_alpha, d_alphaT, d_r, d_A, d_Ainv, d_part;  // alphaT: [triangle][step][3]
     DevBuf<rocblas_int> d_ipiv, d_info;
     bool have_beta = false, have_alpha = false;
     bool e0_pivoted = false;  // the last factorisation needed partial pivoting
@@ -1061,7 +1061,8 @@ int factor_e0(ebem_handle* h) {
         DevBuf<double> dv;
         HIPC(dv.alloc(2 * (size_t)n3));
         HIPC(hipMemcpyAsync(dv.p, v.data(), sizeof(double) * n3, hipMemcpyHostToDevice, h->stream));
-        hipLaunchKernelGGL(k_gemv_rows, dim3((n3 + 3) / 4), dim3(256), 0, h->stream, h->d_Ainv.p, dv.p, dv.p + n3, n3);
+        hipLaunchKernelGGL(k_gemv_rows, dim3((n3 + 3) / 4), dim3(256), 0, h->stream, h->d_Ainv.p, dv.p, dv.p + n3, n3,
+                           nullptr, 0, 0);
         HIPC(hipMemsetAsync(h->d_A.p, 0, sizeof(double) * (size_t)n3 * n3, h->stream));
         hipLaunchKernelGGL(k_e0_dense, dim3(4, h->N), dim3(64), 0, h->stream, h->store(), h->N, L, h->d_A.p);
         // d_A holds E_0 column-major: E_0 y by a column sweep
@@ -1090,6 +1091,7 @@ int mot(ebem_handle* h, const double* beta, double* alpha_out) {
         return fail(EBEM_EINVAL, "no right-hand side: pass beta or call ebem_rhs first");
     }
     HIPC(h->d_alpha.alloc((size_t)n3 * nT));
+    HIPC(h->d_alphaT.alloc((size_t)n3 * nT));
     HIPC(h->d_r.alloc(n3));
     Timer tall, tf;
     tall.start(h->stream);
@@ -1109,8 +1111,8 @@ int mot(ebem_handle* h, const double* beta, double* alpha_out) {
         const int nb = std::min(kHistBlock, nT - n0);
         HIPC(hipEventRecord(ev[3 * n0], h->stream));
         if (n0 > 0) {
-            hipLaunchKernelGGL(k_history_far, dim3(h->N), dim3(256), 0, h->stream, S, h->N, L, n0, nb, h->d_alpha.p,
-                               h->d_part.p);
+            hipLaunchKernelGGL(k_history_far, dim3(h->N), dim3(256), 0, h->stream, S, h->N, L, n0, nb,
+                               h->d_alpha.p, h->d_part.p);
             launches++;
         }
         for (int b = 0; b < nb; b++) {
@@ -1121,7 +1123,7 @@ int mot(ebem_handle* h, const double* beta, double* alpha_out) {
             launches++;
             HIPC(hipEventRecord(ev[3 * n + 1], h->stream));
             hipLaunchKernelGGL(k_gemv_rows, dim3((n3 + 3) / 4), dim3(256), 0, h->stream, h->d_Ainv.p, h->d_r.p,
-                               h->d_alpha.p + (size_t)n * n3, n3);
+                               h->d_alpha.p + (size_t)n * n3, n3, h->d_alphaT.p, nT, n);
             HIPC(hipEventRecord(ev[3 * n + 2], h->stream));
         }
     }
@@ -1183,12 +1185,15 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
         HIPC(hipMemcpyAsync(d_dens.p, density, sizeof(double) * 3 * N * nT, hipMemcpyHostToDevice, h->stream));
         dens = d_dens.p;
     }
-    HIPC(d_aT.alloc((size_t)3 * N * nT));
-    {
+    // the density triangle-major: the MOT's copy, or a transpose of the one passed in
+    const double* aT = h->d_alphaT.p;
+    if (density) {
+        HIPC(d_aT.alloc((size_t)3 * N * nT));
         int64_t tot = (int64_t)3 * N * nT;
         hipLaunchKernelGGL(k_alpha_T, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, dens, N, nT,
                            d_aT.p);
         HIPC(hipGetLastError());
+        aT = d_aT.p;
     }
     HIPC(d_u.alloc((size_t)3 * npts * nt));
     HIPC(hipMemsetAsync(d_u.p, 0, sizeof(double) * 3 * npts * nt, h->stream));
@@ -1345,8 +1350,8 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
                 tppb.stop(h->stream);
                 ppb_ms += tppb.ms();
             }
-            hipLaunchKernelGGL(k_pp_conv, dim3(np), dim3(128 * kConvSlices), 0, h->stream, d_M.p, d_off.p, d_ms.p,
-                               d_me.p, d_aT.p, N, nT, p0, d_nh.p, d_kidx.p, (int)nh.size(), npts, d_u.p);
+            hipLaunchKernelGGL(k_pp_conv, dim3(np), dim3(64 * kConvSlices), 0, h->stream, d_M.p, d_off.p, d_ms.p,
+                               d_me.p, aT, N, nT, p0, d_nh.p, d_kidx.p, (int)nh.size(), npts, d_u.p);
             HIPC(hipGetLastError());
         }
         g0 = g1;
@@ -1416,6 +1421,7 @@ int ebem_destroy(ebem_handle* h) {
         h->d_val.reset();
         h->d_beta.reset();
         h->d_alpha.reset();
+        h->d_alphaT.reset();
         h->d_r.reset();
         h->d_A.reset();
         h->d_Ainv.reset();
