@@ -97,3 +97,42 @@ def test_pp_kp_split_matches_per_m_evaluation():
     finally:
         a.close()
         b.close()
+
+
+def test_screen_between_front_ranges_bit_identical():
+    """Flat 16 x 16 screen (512 triangles), T = 2, 64 steps: every outer node lies
+    in the field plane, so there are no kP ranges and the between-front ranges
+    take the coplanar branch; their results are bit-identical to the
+    level-by-level evaluation, so every lag matrix must equal the
+    EBEM_KP_SPLIT=0 assembly exactly, and so must the solved density."""
+    from elastodynamicebem_amd import api
+    from elastodynamicebem_amd.mesh import screen_uniform
+    m = screen_uniform(16)
+    pb = api.PbParam(rho=1.0, mu=1.0, lambda_=1.0, Tfin=2.0, nT=64)
+    sol = []
+    for split in (True, False):
+        old = os.environ.get("EBEM_KP_SPLIT")
+        os.environ["EBEM_KP_SPLIT"] = "1" if split else "0"
+        try:
+            s = api.Solver(m, pb, 7)
+        finally:
+            if old is None:
+                del os.environ["EBEM_KP_SPLIT"]
+            else:
+                os.environ["EBEM_KP_SPLIT"] = old
+        s.assemble(api.PREC_FP64)
+        sol.append(s)
+    a, b = sol
+    try:
+        st = a.assembly_kp_stats()
+        print(f"screen16: between-front ranges {st['bf_ranges']}, levels {st['bf_levels']} of "
+              f"{a.store_info()['items']} items; kP ranges {st['ranges']}")
+        assert st["ranges"] == 0 and st["bf_ranges"] > 0 and st["fallbacks"] == 0
+        for lag in range(a.num_blocks()):
+            assert np.array_equal(a.get_block(lag), b.get_block(lag)), lag
+        a.rhs(api.DATUM_SCREEN_SIN2)
+        b.rhs(api.DATUM_SCREEN_SIN2)
+        assert np.array_equal(a.mot_solve(None), b.mot_solve(None))
+    finally:
+        a.close()
+        b.close()
