@@ -67,6 +67,12 @@ struct DevBuf {
         else p = nullptr;
         return e;
     }
+    // At least count elements, reallocating only to grow (per-call scratch kept
+    // in the handle, so repeated calls do not pay hipMalloc/hipFree).
+    hipError_t grow(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        return alloc(count);
+    }
 };
 
 // Gauss-Hammer tables of util/quadRules/GaussHammer_base.m:19-255 mapped by
@@ -416,6 +422,25 @@ struct ebem_handle {
     bool have_beta = false, have_alpha = false;
     bool e0_pivoted = false;  // the last factorisation needed partial pivoting
     double tim[EBEM_NUM_TIMINGS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // scratch of ebem_assemble and ebem_pp_eval, kept across calls (grow-only)
+    struct {
+        DevBuf<uint8_t> flag, key;
+        DevBuf<unsigned long long> nsb, culled, cur;
+        DevBuf<int16_t> lo, hi, ka, kb, sa, sb;
+        DevBuf<unsigned int> nfb, hist;
+        DevBuf<int64_t> cnt, items, off, rowbase;
+        DevBuf<double> W;
+        DevBuf<int32_t> ip, perm;
+    } aw;
+    struct {
+        DevBuf<double> x, aT, u, dens, M;
+        DevBuf<int32_t> ms, me, iq, nh, kidx, perm, ka, kb, sa, sb;
+        DevBuf<unsigned int> nfb, bh;
+        DevBuf<int64_t> off, rb, cnt;
+        DevBuf<uint8_t> key;
+        DevBuf<unsigned long long> bo;
+    } pw;
+    std::vector<hipEvent_t> mot_ev;  // per-step events of ebem_mot_solve
     bool sort_items = true;
     bool kp_split = true;  // kP ranges through k_levels_kp (EBEM_KP_SPLIT=0 turns it off)
     // kP and between-front ranges of the last assembly / post-processing:
@@ -731,27 +756,28 @@ int assemble(ebem_handle* h, int precision) {
     h->sur_pairs_blocks = 0;
     h->sur_ms = 0;
     h->precision = precision;
-    DevBuf<uint8_t> d_flag;
-    DevBuf<unsigned long long> d_nsb;
+    auto& d_flag = h->aw.flag;
+    auto& d_nsb = h->aw.nsb;
     if (sur) {
-        HIPC(d_nsb.alloc(1));
+        HIPC(d_nsb.grow(1));
         HIPC(hipMemsetAsync(d_nsb.p, 0, sizeof(unsigned long long), h->stream));
     }
     Timer tsur;
-    DevBuf<int16_t> d_lo, d_hi, d_ka, d_kb, d_sa, d_sb;
-    HIPC(d_lo.alloc(np));
-    HIPC(d_hi.alloc(np));
+    auto &d_lo = h->aw.lo, &d_hi = h->aw.hi, &d_ka = h->aw.ka, &d_kb = h->aw.kb, &d_sa = h->aw.sa,
+         &d_sb = h->aw.sb;
+    HIPC(d_lo.grow(np));
+    HIPC(d_hi.grow(np));
     if (kp) {
-        HIPC(d_ka.alloc(np));
-        HIPC(d_kb.alloc(np));
-        HIPC(d_sa.alloc(np));
-        HIPC(d_sb.alloc(np));
+        HIPC(d_ka.grow(np));
+        HIPC(d_kb.grow(np));
+        HIPC(d_sa.grow(np));
+        HIPC(d_sb.grow(np));
     }
-    DevBuf<unsigned long long> d_culled;
-    HIPC(d_culled.alloc(1));
+    auto& d_culled = h->aw.culled;
+    HIPC(d_culled.grow(1));
     HIPC(hipMemsetAsync(d_culled.p, 0, sizeof(unsigned long long), h->stream));
-    DevBuf<unsigned int> d_nfb;
-    HIPC(d_nfb.alloc(1));
+    auto& d_nfb = h->aw.nfb;
+    HIPC(d_nfb.grow(1));
     HIPC(hipMemsetAsync(d_nfb.p, 0, sizeof(unsigned int), h->stream));
     {
         int64_t blocks = std::min<int64_t>((np + 255) / 256, (int64_t)h->CUs * 64);
@@ -761,9 +787,9 @@ int assemble(ebem_handle* h, int precision) {
     }
     unsigned long long culled = 0;
     HIPC(hipMemcpyAsync(&culled, d_culled.p, sizeof culled, hipMemcpyDeviceToHost, h->stream));
-    DevBuf<int64_t> d_cnt, d_items;
-    HIPC(d_cnt.alloc((size_t)N * L));
-    HIPC(d_items.alloc(N));
+    auto &d_cnt = h->aw.cnt, &d_items = h->aw.items;
+    HIPC(d_cnt.grow((size_t)N * L));
+    HIPC(d_items.grow(N));
     hipLaunchKernelGGL(k_rowcount, dim3(N), dim3(256), (L + 1) * sizeof(int), h->stream, d_lo.p, d_hi.p, N, L, 0, N,
                        d_cnt.p, d_items.p);
     HIPC(hipGetLastError());
@@ -777,12 +803,10 @@ int assemble(ebem_handle* h, int precision) {
     h->pair_lags = (int64_t)culled;  // the reference's culled pair.lags; nnzb leaves out exact zeros
     h->items = 0;
     for (int i = 0; i < N; i++) h->items += rowitems[i];
-    HIPC(h->d_seg.alloc(h->seg.size()));
+    HIPC(h->d_seg.grow(h->seg.size()));
     HIPC(hipMemcpyAsync(h->d_seg.p, h->seg.data(), sizeof(int64_t) * h->seg.size(), hipMemcpyHostToDevice, h->stream));
-    h->d_val.reset();
-    h->d_col.reset();
-    HIPC(h->d_val.alloc((size_t)6 * std::max<int64_t>(h->nnzb, 1)));
-    HIPC(h->d_col.alloc((size_t)std::max<int64_t>(h->nnzb, 1)));
+    HIPC(h->d_val.grow((size_t)6 * std::max<int64_t>(h->nnzb, 1)));
+    HIPC(h->d_col.grow((size_t)std::max<int64_t>(h->nnzb, 1)));
     Store S = h->store();
 
     // row chunks bounded by the W scratch budget
@@ -792,21 +816,21 @@ int assemble(ebem_handle* h, int precision) {
     if (budget_items < 1) return fail(EBEM_ENOMEM, "not enough device memory for assembly scratch");
     int64_t maxrow_items = *std::max_element(rowitems.begin(), rowitems.end());
     if (maxrow_items > budget_items) budget_items = maxrow_items;
-    DevBuf<double> d_W;
-    DevBuf<int32_t> d_ip;
-    DevBuf<int64_t> d_off, d_rowbase;
+    auto& d_W = h->aw.W;
+    auto& d_ip = h->aw.ip;
+    auto &d_off = h->aw.off, &d_rowbase = h->aw.rowbase;
     int64_t chunk_cap = std::min<int64_t>(budget_items, std::max<int64_t>(h->items, 1));
-    HIPC(d_W.alloc((size_t)chunk_cap * 6));
-    HIPC(d_ip.alloc((size_t)chunk_cap));
-    DevBuf<int32_t> d_perm;
-    DevBuf<uint8_t> d_key;
-    DevBuf<unsigned int> d_hist;
-    DevBuf<unsigned long long> d_cur;
+    HIPC(d_W.grow((size_t)chunk_cap * 6));
+    HIPC(d_ip.grow((size_t)chunk_cap));
+    auto& d_perm = h->aw.perm;
+    auto& d_key = h->aw.key;
+    auto& d_hist = h->aw.hist;
+    auto& d_cur = h->aw.cur;
     const int G = h->CUs * 8;  // sort blocks
-    HIPC(d_perm.alloc((size_t)chunk_cap));
-    HIPC(d_key.alloc((size_t)chunk_cap));
-    HIPC(d_hist.alloc((size_t)kNumKeys * G));
-    HIPC(d_cur.alloc((size_t)kNumKeys * G));
+    HIPC(d_perm.grow((size_t)chunk_cap));
+    HIPC(d_key.grow((size_t)chunk_cap));
+    HIPC(d_hist.grow((size_t)kNumKeys * G));
+    HIPC(d_cur.grow((size_t)kNumKeys * G));
     std::vector<unsigned int> hist((size_t)kNumKeys * G);
     std::vector<unsigned long long> cur((size_t)kNumKeys * G);
     const double inv4pirho = 1.0 / (4 * kPi * h->rho);
@@ -823,16 +847,16 @@ int assemble(ebem_handle* h, int precision) {
             rb[r] = run;
             run += rowitems[r0 + r];
         }
-        HIPC(d_rowbase.alloc(rows));
+        HIPC(d_rowbase.grow(rows));
         HIPC(hipMemcpyAsync(d_rowbase.p, rb.data(), sizeof(int64_t) * rows, hipMemcpyHostToDevice, h->stream));
-        HIPC(d_off.alloc((size_t)rows * N));
+        HIPC(d_off.grow((size_t)rows * N));
         const int16_t* lo = d_lo.p + (int64_t)r0 * N;
         const int16_t* hi = d_hi.p + (int64_t)r0 * N;
         hipLaunchKernelGGL(k_pair_items, dim3(rows), dim3(256), 0, h->stream, lo, hi, N, d_rowbase.p, d_off.p, d_ip.p);
         HIPC(hipGetLastError());
         const uint8_t* sflag = nullptr;
         if (sur) {
-            HIPC(d_flag.alloc((size_t)rows * N));
+            HIPC(d_flag.grow((size_t)rows * N));
             const int64_t npr = (int64_t)rows * N;
             hipLaunchKernelGGL(k_sur_flag, dim3((unsigned)((npr + 255) / 256)), dim3(256), 0, h->stream, h->d_tri.p,
                                h->d_verts.p, N, r0, rows, lo, hi, h->dt, h->cP, h->cS, h->hmax, near2,
@@ -1104,8 +1128,12 @@ int mot(ebem_handle* h, const double* beta, double* alpha_out) {
     HIPC(h->d_part.alloc((size_t)kHistBlock * n3));
     // events: [start of history work, end of history work = start of apply] per step
     // (the far pass of a block is timed with its block's first step)
-    std::vector<hipEvent_t> ev(3 * (size_t)nT);
-    for (auto& e : ev) HIPC(hipEventCreate(&e));
+    std::vector<hipEvent_t>& ev = h->mot_ev;
+    while (ev.size() < 3 * (size_t)nT) {
+        hipEvent_t e;
+        HIPC(hipEventCreate(&e));
+        ev.push_back(e);
+    }
     int launches = 0;
     for (int n0 = 0; n0 < nT; n0 += kHistBlock) {
         const int nb = std::min(kHistBlock, nT - n0);
@@ -1137,7 +1165,6 @@ int mot(ebem_handle* h, const double* beta, double* alpha_out) {
         th += a;
         ts += b;
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
     tall.stop(h->stream);
     h->tim[5] = tall.ms();
     h->tim[3] = th;
@@ -1176,26 +1203,26 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     Timer tall, tppk, tppb;
     double ppk_ms = 0, ppb_ms = 0;
     tall.start(h->stream);
-    DevBuf<double> d_x, d_aT, d_u, d_dens, d_M;
-    HIPC(d_x.alloc(3 * (size_t)npts));
+    auto &d_x = h->pw.x, &d_aT = h->pw.aT, &d_u = h->pw.u, &d_dens = h->pw.dens, &d_M = h->pw.M;
+    HIPC(d_x.grow(3 * (size_t)npts));
     HIPC(hipMemcpyAsync(d_x.p, x, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, h->stream));
     const double* dens = h->d_alpha.p;
     if (density) {
-        HIPC(d_dens.alloc((size_t)3 * N * nT));
+        HIPC(d_dens.grow((size_t)3 * N * nT));
         HIPC(hipMemcpyAsync(d_dens.p, density, sizeof(double) * 3 * N * nT, hipMemcpyHostToDevice, h->stream));
         dens = d_dens.p;
     }
     // the density triangle-major: the MOT's copy, or a transpose of the one passed in
     const double* aT = h->d_alphaT.p;
     if (density) {
-        HIPC(d_aT.alloc((size_t)3 * N * nT));
+        HIPC(d_aT.grow((size_t)3 * N * nT));
         int64_t tot = (int64_t)3 * N * nT;
         hipLaunchKernelGGL(k_alpha_T, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, dens, N, nT,
                            d_aT.p);
         HIPC(hipGetLastError());
         aT = d_aT.p;
     }
-    HIPC(d_u.alloc((size_t)3 * npts * nt));
+    HIPC(d_u.grow((size_t)3 * npts * nt));
     HIPC(hipMemsetAsync(d_u.p, 0, sizeof(double) * 3 * npts * nt, h->stream));
     // group times by delta
     std::vector<int> order;
@@ -1206,28 +1233,29 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     const int G = h->CUs * 8;             // counting-sort blocks
     // kP ranges of m through k_pp_M_kp (analytic form with key-sorted items)
     const bool kp = !numeric && h->sort_items && h->kp_split;
-    DevBuf<int32_t> d_ms, d_me, d_iq, d_nh, d_kidx, d_perm, d_ka, d_kb, d_sa, d_sb;
-    DevBuf<unsigned int> d_nfb;
-    HIPC(d_nfb.alloc(1));
+    auto &d_ms = h->pw.ms, &d_me = h->pw.me, &d_iq = h->pw.iq, &d_nh = h->pw.nh, &d_kidx = h->pw.kidx;
+    auto &d_perm = h->pw.perm, &d_ka = h->pw.ka, &d_kb = h->pw.kb, &d_sa = h->pw.sa, &d_sb = h->pw.sb;
+    auto& d_nfb = h->pw.nfb;
+    HIPC(d_nfb.grow(1));
     HIPC(hipMemsetAsync(d_nfb.p, 0, sizeof(unsigned int), h->stream));
-    DevBuf<int64_t> d_off, d_rb, d_cnt;
-    DevBuf<uint8_t> d_key;
-    DevBuf<unsigned int> d_bh;
-    DevBuf<unsigned long long> d_bo;
-    HIPC(d_ms.alloc((size_t)P0 * N));
-    HIPC(d_me.alloc((size_t)P0 * N));
+    auto &d_off = h->pw.off, &d_rb = h->pw.rb, &d_cnt = h->pw.cnt;
+    auto& d_key = h->pw.key;
+    auto& d_bh = h->pw.bh;
+    auto& d_bo = h->pw.bo;
+    HIPC(d_ms.grow((size_t)P0 * N));
+    HIPC(d_me.grow((size_t)P0 * N));
     if (kp) {
-        HIPC(d_ka.alloc((size_t)P0 * N));
-        HIPC(d_kb.alloc((size_t)P0 * N));
-        HIPC(d_sa.alloc((size_t)P0 * N));
-        HIPC(d_sb.alloc((size_t)P0 * N));
+        HIPC(d_ka.grow((size_t)P0 * N));
+        HIPC(d_kb.grow((size_t)P0 * N));
+        HIPC(d_sa.grow((size_t)P0 * N));
+        HIPC(d_sb.grow((size_t)P0 * N));
     }
     for (int k = 0; k < 5; k++) h->pp_kp_stats[k] = 0;
-    HIPC(d_off.alloc((size_t)P0 * N));
-    HIPC(d_rb.alloc(P0));
-    HIPC(d_cnt.alloc(P0));
-    HIPC(d_bh.alloc((size_t)kNumKeys * G));
-    HIPC(d_bo.alloc((size_t)kNumKeys * G));
+    HIPC(d_off.grow((size_t)P0 * N));
+    HIPC(d_rb.grow(P0));
+    HIPC(d_cnt.grow(P0));
+    HIPC(d_bh.grow((size_t)kNumKeys * G));
+    HIPC(d_bo.grow((size_t)kNumKeys * G));
     std::vector<unsigned int> bh((size_t)kNumKeys * G);
     std::vector<unsigned long long> bo((size_t)kNumKeys * G);
     std::vector<int64_t> cnt(P0), rb(P0);
@@ -1257,8 +1285,8 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
         P.trep = t[krep];
         P.nrep = mmax;
         P.kp = kp ? 1 : 0;
-        HIPC(d_nh.alloc(nh.size()));
-        HIPC(d_kidx.alloc(kid.size()));
+        HIPC(d_nh.grow(nh.size()));
+        HIPC(d_kidx.grow(kid.size()));
         HIPC(hipMemcpyAsync(d_nh.p, nh.data(), sizeof(int32_t) * nh.size(), hipMemcpyHostToDevice, h->stream));
         HIPC(hipMemcpyAsync(d_kidx.p, kid.data(), sizeof(int32_t) * kid.size(), hipMemcpyHostToDevice, h->stream));
         for (int p0 = 0; p0 < npts; p0 += P0) {
@@ -1281,10 +1309,10 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             if (nitems == 0) continue;
             if (nitems > INT32_MAX) return fail(EBEM_EINVAL, "too many post-processing items in one chunk");
             HIPC(hipMemcpyAsync(d_rb.p, rb.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice, h->stream));
-            HIPC(d_iq.alloc((size_t)nitems));
-            HIPC(d_M.alloc((size_t)nitems * 6));
-            HIPC(d_key.alloc((size_t)nitems));
-            HIPC(d_perm.alloc((size_t)nitems));
+            HIPC(d_iq.grow((size_t)nitems));
+            HIPC(d_M.grow((size_t)nitems * 6));
+            HIPC(d_key.grow((size_t)nitems));
+            HIPC(d_perm.grow((size_t)nitems));
             hipLaunchKernelGGL(k_pp_items, dim3(np), dim3(256), 0, h->stream, d_ms.p, d_me.p, N, d_rb.p, d_off.p,
                                d_iq.p);
             HIPC(hipGetLastError());
@@ -1428,9 +1456,11 @@ int ebem_destroy(ebem_handle* h) {
         h->d_part.reset();
         h->d_ipiv.reset();
         h->d_info.reset();
+        for (hipEvent_t e : h->mot_ev) (void)hipEventDestroy(e);
+        h->mot_ev.clear();
         (void)hipStreamDestroy(h->stream);
     }
-    delete h;
+    delete h;  // the scratch buffers (aw, pw) are freed by their destructors
     return 0;
 }
 
