@@ -626,7 +626,10 @@ __device__ inline double wg_sum(double v, double* red4) {
 // k_history then adds the remaining lags 1..min(b, L-1) step by step.
 // Store traffic per block: one pass plus lags 1..b per step, instead of one
 // full pass per step.
-constexpr int kHistBlock = 8;
+#ifndef EBEM_HIST_BLOCK
+#define EBEM_HIST_BLOCK 8
+#endif
+constexpr int kHistBlock = EBEM_HIST_BLOCK;
 
 // alpha step-major (the blocks of a row segment are in column order, so a
 // wave's reads of one step share cache lines; a triangle-major layout measured
@@ -740,17 +743,6 @@ __global__ void __launch_bounds__(256) k_gemv_rows(const double* __restrict__ A,
 __global__ void k_set_diag(double* __restrict__ A, int n, double v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) A[(int64_t)i * n + i] = v;
-}
-
-// y = A x for A column-major n x n: one thread per row, a column sweep
-// (consecutive threads read consecutive rows of each column).
-__global__ void __launch_bounds__(256) k_gemv_cols(const double* __restrict__ A, const double* __restrict__ x,
-                                                   double* __restrict__ y, int n) {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= n) return;
-    double s = 0.0;
-    for (int k = 0; k < n; k++) s += A[(int64_t)k * n + row] * x[k];
-    y[row] = s;
 }
 
 // E_0 dense (col-major) from the lag-0 segments; one thread per block.

@@ -1089,9 +1089,15 @@ int factor_e0(ebem_handle* h) {
                            nullptr, 0, 0);
         HIPC(hipMemsetAsync(h->d_A.p, 0, sizeof(double) * (size_t)n3 * n3, h->stream));
         hipLaunchKernelGGL(k_e0_dense, dim3(4, h->N), dim3(64), 0, h->stream, h->store(), h->N, L, h->d_A.p);
-        // d_A holds E_0 column-major: E_0 y by a column sweep
-        hipLaunchKernelGGL(k_gemv_cols, dim3((n3 + 255) / 256), dim3(256), 0, h->stream, h->d_A.p, dv.p + n3, dv.p, n3);
+        // d_A holds E_0 column-major: E_0 y by rocBLAS dgemv
         HIPC(hipGetLastError());
+        {
+            const double one = 1.0, zero = 0.0;
+            if (int e = rocb(rocblas_dgemv(h->blas, rocblas_operation_none, n3, n3, &one, h->d_A.p, n3, dv.p + n3, 1,
+                                           &zero, dv.p, 1),
+                             "rocblas_dgemv"))
+                return e;
+        }
         HIPC(hipMemcpyAsync(z.data(), dv.p, sizeof(double) * n3, hipMemcpyDeviceToHost, h->stream));
         HIPC(hipStreamSynchronize(h->stream));
         double num = 0, den = 0;
