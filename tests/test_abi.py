@@ -91,3 +91,21 @@ def test_load_points_tables():
     assert x.shape == (3, 3) and t.size == 200 and t[-1] == 2.0 and i == [3]
     x, t, i, n, tp = api.BEMenerg_postProc_loadPoints(api.PbParam(domainType="screenUniform"))
     assert n == 0 and i == [3]
+
+
+def test_stats_and_timings_need_a_handle():
+    """The statistics and timing entry points reject a null handle or output
+    with an error code (no device needed)."""
+    import ctypes as C
+    import numpy as np
+    from elastodynamicebem_amd import _lib
+    lib = _lib.load()
+    out = np.zeros(12)
+    st = np.zeros(5, np.int64)
+    assert lib.ebem_timings(None, out.ctypes.data_as(_lib._dp), 12) < 0
+    assert lib.ebem_assembly_kp_stats(None, st.ctypes.data_as(_lib._i64p)) < 0
+    assert lib.ebem_pp_kp_stats(None, st.ctypes.data_as(_lib._i64p)) < 0
+    src = open(os.path.join(ROOT, "include", "ebem.h")).read()
+    assert "#define EBEM_NUM_TIMINGS 12" in src
+    from elastodynamicebem_amd import api
+    assert len(api.Solver._KP_KEYS) == 5
