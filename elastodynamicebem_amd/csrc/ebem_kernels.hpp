@@ -1097,7 +1097,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_B
 // difference D_m per wave); only the density reads and the accumulation are
 // per lane.  The slice partials are reduced through LDS.  aT: density as
 // [triangle][step][3].
-constexpr int kConvSlices = 8;
+#ifndef EBEM_CONV_SLICES
+#define EBEM_CONV_SLICES 8
+#endif
+constexpr int kConvSlices = EBEM_CONV_SLICES;
 constexpr int kConvPerLane = 2;
 
 __global__ void __launch_bounds__(64 * kConvSlices) k_pp_conv(const double* __restrict__ M,
