@@ -310,43 +310,65 @@ def BEMenerg_dir_calcNumBlocks(pbParam: PbParam, domainMesh: SpaceMesh) -> int:
 
 
 def BEMenerg_coreSA_timeMarching(pbParam: PbParam, domainMesh: SpaceMesh, methodInfo: MethodInfo, GHn=None,
-                                 GHw=None, *, datum=None, precision: int = PREC_FP64, device: int = 0):
-    """density (3N x nT): E_0..E_{L-1} assembly, RHS and MOT on the GPU."""
+                                 GHw=None, *, datum=None, precision: int = PREC_FP64, device: int = 0,
+                                 save_dir: str | None = None):
+    """density (3N x nT): E_0..E_{L-1} assembly, RHS and MOT on the GPU.  With
+    save_dir, also written as the reference does (SA_timeMarching.m:116-117):
+    <save_dir>/<domainType><lev>_SA_<numNodiExt>_density.mat."""
     with Solver(domainMesh, pbParam, methodInfo.numNodiExt, device) as s:
         s.assemble(precision)
         s.rhs(_datum_for(pbParam, datum))
-        return s.mot_solve(None)
+        density = s.mot_solve(None)
+    if save_dir is not None:
+        from . import matio
+        matio.save_density(matio.output_base(pbParam, "SA", methodInfo, save_dir), density)
+    return density
 
 
 def BEMenerg_dir_timeMarchingD(pbParam, domainMesh, methodInfo, GHn=None, GHw=None, GHCn=None, GHCw=None,
-                               DIAGn=None, DIAGw=None, *, datum=None, precision: int = PREC_FP64, device: int = 0):
+                               DIAGn=None, DIAGw=None, *, datum=None, precision: int = PREC_FP64, device: int = 0,
+                               save_dir: str | None = None):
     """Single-layer (V) system of the direct Dirichlet driver: with methodInfo
     "FN ..." the numeric inner quadrature of the reference's GPU kernel
     (kernelV.cu + calcSubBlockDiagV.m, PREC_GHC), with "SA ..." the analytic
     form.  The double-layer right-hand side of timeMarchingD.m:287 is not part
     of this library (raises if a DD problem with lambda+mu == 0 is given, like
-    main.m:61-63)."""
+    main.m:61-63).  With save_dir, the density is also written as the
+    reference does (timeMarchingD.m:300-301):
+    <save_dir>/<domainType><lev>_NEW_<ext>_<sub>_<sing>_<diag>_density.mat."""
     if pbParam.lambda_ + pbParam.mu == 0:
         raise ValueError("Problema non adatto a questa formulazione")
     if methodInfo.typeIntg != "FN":
-        return BEMenerg_coreSA_timeMarching(pbParam, domainMesh, methodInfo, GHn, GHw, datum=datum,
-                                            precision=precision, device=device)
-    with Solver(domainMesh, pbParam, methodInfo.numNodiExt, device) as s:
-        s.set_inner_rule(methodInfo.numSubRegion, methodInfo.numNodiSing, methodInfo.numNodiDiag)
-        s.assemble(PREC_GHC)
-        s.rhs(_datum_for(pbParam, datum))
-        return s.mot_solve(None)
+        density = BEMenerg_coreSA_timeMarching(pbParam, domainMesh, methodInfo, GHn, GHw, datum=datum,
+                                               precision=precision, device=device)
+    else:
+        with Solver(domainMesh, pbParam, methodInfo.numNodiExt, device) as s:
+            s.set_inner_rule(methodInfo.numSubRegion, methodInfo.numNodiSing, methodInfo.numNodiDiag)
+            s.assemble(PREC_GHC)
+            s.rhs(_datum_for(pbParam, datum))
+            density = s.mot_solve(None)
+    if save_dir is not None:
+        from . import matio
+        matio.save_density(matio.output_base(pbParam, "NEW", methodInfo, save_dir), density)
+    return density
 
 
-def BEMenerg_postProcA_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, *, device: int = 0):
-    """campoVett: xSize x tSize array of 3-vectors (postProcA_setup.m:1-26)."""
+def BEMenerg_postProcA_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, *, device: int = 0,
+                             save_dir: str | None = None):
+    """campoVett: xSize x tSize array of 3-vectors (postProcA_setup.m:1-26).  With
+    save_dir, also written as the reference does (:21-22):
+    <save_dir>/<domainType><lev>_A_campoVett.mat."""
     xVal = np.asarray(xVal, float).reshape(-1, 3)
     tVal = np.asarray(tVal, float).ravel()
     if numPoints != xVal.shape[0] * tVal.size:
         raise ValueError("numPoints does not match xVal x tVal")
     with Solver(domainMesh, pbParam, 7, device) as s:
         u = s.pp_eval(xVal, tVal, density)
-    return u.transpose(1, 2, 0)  # (xSize, tSize, 3)
+    campo = u.transpose(1, 2, 0)  # (xSize, tSize, 3)
+    if save_dir is not None:
+        from . import matio
+        matio.save_field(matio.output_base(pbParam, "A", None, save_dir), campo)
+    return campo
 
 
 def _pp_rule_of(methodInfo, PPn):
