@@ -1481,3 +1481,267 @@ void orc_pp_point_num(const orc_param *p, const orc_mesh *m, const double *densi
         }
 }
 
+
+/* ======================================================================
+ * Double-layer operator K of the direct Dirichlet driver
+ * (dir/BEMenerg_dir_timeMarchingD.m:173-274): regular blocks as
+ * dirK/kernelK.cu computes them, the blocks of a triangle with its own
+ * vertices as dir/BEMenerg_dir_calcSubBlockSingK.m + calcKIntgSing.m, and
+ * beta_K of dir/BEMenerg_dir_calcBetaKgpu.m.
+ * ====================================================================== */
+
+/* P1 basis of triangle T (vertices row-wise, 9 doubles):
+ * matCoeff = [-1 -1 0; 1 0 0; 0 1 0] / [l21, l31, n] (MATLAB mrdivide: X B = A),
+ * vetCoeff = [1;0;0] - matCoeff V1, phi_k(x) = vetCoeff[k] + matCoeff[k][.] x
+ * (dir/BEMenerg_dir_calcCostantData.m:16-20).  mat row-major [k][c]. */
+void orc_k_basis(const double *T, double *mat, double *vec) {
+    double l21[3], l31[3], n[3];
+    for (int c = 0; c < 3; c++) {
+        l21[c] = T[3 + c] - T[c];
+        l31[c] = T[6 + c] - T[c];
+    }
+    n[0] = l21[1] * l31[2] - l21[2] * l31[1];
+    n[1] = l21[2] * l31[0] - l21[0] * l31[2];
+    n[2] = l21[0] * l31[1] - l21[1] * l31[0];
+    /* B = [l21 l31 n] (columns); row k of X solves B^T x = A(k,:)^T */
+    const double A[3][3] = {{-1, -1, 0}, {1, 0, 0}, {0, 1, 0}};
+    for (int k = 0; k < 3; k++) {
+        double M[3][4];
+        for (int r = 0; r < 3; r++) {
+            const double *col = r == 0 ? l21 : (r == 1 ? l31 : n);
+            for (int c = 0; c < 3; c++) M[r][c] = col[c];
+            M[r][3] = A[k][r];
+        }
+        for (int c = 0; c < 3; c++) {
+            int piv = c;
+            for (int r = c + 1; r < 3; r++)
+                if (fabs(M[r][c]) > fabs(M[piv][c])) piv = r;
+            if (piv != c)
+                for (int e = 0; e < 4; e++) {
+                    double tmp = M[c][e];
+                    M[c][e] = M[piv][e];
+                    M[piv][e] = tmp;
+                }
+            for (int r = c + 1; r < 3; r++) {
+                double f = M[r][c] / M[c][c];
+                for (int e = c; e < 4; e++) M[r][e] -= f * M[c][e];
+            }
+        }
+        double x[3];
+        for (int c = 2; c >= 0; c--) {
+            double s = M[c][3];
+            for (int e = c + 1; e < 3; e++) s -= M[c][e] * x[e];
+            x[c] = s / M[c][c];
+        }
+        for (int c = 0; c < 3; c++) mat[3 * k + c] = x[c];
+    }
+    for (int k = 0; k < 3; k++)
+        vec[k] = (k == 0 ? 1.0 : 0.0) - (mat[3 * k] * T[0] + mat[3 * k + 1] * T[1] + mat[3 * k + 2] * T[2]);
+}
+
+static const int K_EPS[3][3][3] = {{{0, 0, 0}, {0, 0, 1}, {0, -1, 0}},
+                                   {{0, 0, -1}, {0, 0, 0}, {1, 0, 0}},
+                                   {{0, 1, 0}, {-1, 0, 0}, {0, 0, 0}}};
+
+/* R_j part of the kernel, nu[i][k] (kernelK.cu nucleoKRj; `typo` = 1 keeps its
+ * Heaviside (r - r/cP) > 0 in the P part of the lambda-mu term, as the GPU
+ * kernel has it; calcKIntgSing.m nuKsmRj has (t - r/cP) > 0). */
+static void k_nuRj(const double *x, double r, double t, const orc_param *p, int j, int typo, double nu[3][3]) {
+    const double cP = p->velP, cS = p->velS, lam = p->lambda, mu = p->mu, rho = p->rho;
+    const double HP = (t - r / cP) > 0, HS = (t - r / cS) > 0;
+    const double Hcub = (t - r / cS) * (t - r / cS) * (t + 2 * r / cS) / 6 * HS -
+                        (t - r / cP) * (t - r / cP) * (t + 2 * r / cP) / 6 * HP;
+    const double Dlin = ((t - r / cS) * HS / (cS * cS)) - ((t - r / cP) * HP / (cP * cP));
+    const double HPt = typo ? (double)((r - r / cP) > 0) : HP;
+    const double Dlam = ((t - r / cS) * HS / (cS * cS)) - ((t - r / cP) * HPt / (cP * cP));
+    const double r3 = pow(r, 3), r5 = pow(r, 5);
+    const double a = 2 * mu / rho, b = lam * mu / (rho * (lam + mu));
+    for (int i = 0; i < 3; i++) {
+        /* the two tensor components of row i: (i, i+1) and (i, i+2) in the order kernelK.cu uses */
+        for (int k = 0; k < 3; k++) {
+            double h = 0, d = 0;
+            for (int q = 0; q < 3; q++) {
+                if (q == i) continue;
+                const double e = K_EPS[j][i][q];
+                h += e * (3 * x[q] * x[k] / r5 - (q == k ? 1.0 / r3 : 0.0));
+                d += e * (x[q] * x[k] / r3);
+            }
+            double v = a * h * Hcub + a * d * Dlin;
+            if (i != k) v = b * K_EPS[j][k][i] / r * Dlam + v;
+            nu[i][k] = v;
+        }
+    }
+}
+
+/* KL + KT part (kernelK.cu nucleoKL + nucleoKT with `dotterm` = 1; calcKIntgSing.m
+ * nuKL + nuKT without the x.n diagonal terms with `dotterm` = 0). */
+static void k_nuLT(const double *x, const double *n, double r, double t, const orc_param *p, int dotterm,
+                   double nu[3][3]) {
+    const double cP = p->velP, cS = p->velS, lam = p->lambda, mu = p->mu;
+    const double HP = (t - r / cP) > 0, HS = (t - r / cS) > 0;
+    const double fL = (t - r / cP) * HP - (t - r / cS) * HS, fT = HP / cP - HS / cS;
+    const double cl = lam / (lam + mu), cm = mu / (lam + mu);
+    const double r2 = pow(r, 2), r3 = pow(r, 3);
+    const double xn = x[0] * n[0] + x[1] * n[1] + x[2] * n[2];
+    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) {
+            double L = cl * x[i] * n[k] / r3 * fL + cm * x[k] * n[i] / r3 * fL;
+            double T = cl * x[i] * n[k] / r2 * fT + cm * x[k] * n[i] / r2 * fT;
+            if (dotterm && i == k) {
+                L += xn / r3 * (t - r / cS) * HS;
+                T += xn / r2 * (HS / cS);
+            }
+            nu[i][k] = L + T;
+        }
+}
+
+/* kernel value for outer point xo, inner point y on field triangle F with
+ * normal nF and basis (mat, vec), local vertex v (0..2):
+ * -sum_j (nF x grad phi_v)_j R_j + phi_v(y) (KL + KT) */
+static void k_kernel(const double *xo, const double *y, const double *nF, const double *mat, const double *vec, int v,
+                     double t, const orc_param *p, int regular, double out[3][3]) {
+    double x[3] = {xo[0] - y[0], xo[1] - y[1], xo[2] - y[2]};
+    double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    const double *g = mat + 3 * v;
+    double w[3] = {nF[1] * g[2] - nF[2] * g[1], -nF[0] * g[2] + nF[2] * g[0], nF[0] * g[1] - nF[1] * g[0]};
+    double acc[3][3] = {{0}};
+    for (int j = 0; j < 3; j++) {
+        double R[3][3];
+        k_nuRj(x, r, t, p, j, regular, R);
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) acc[i][k] -= w[j] * R[i][k];
+    }
+    double LT[3][3];
+    k_nuLT(x, nF, r, t, p, regular, LT);
+    double phi = vec[v] + g[0] * y[0] + g[1] * y[1] + g[2] * y[2];
+    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) out[i][k] = acc[i][k] + LT[i][k] * phi;
+}
+
+/* isBlockNull of kernelK.cu: test triangle S against vertex s */
+static int k_block_null(const orc_mesh *m, int S, int s, int lag, double dt, double cP, double cS, double maxLen) {
+    double dmax = 0, dmin = 100;
+    const double *P = m->coords + 3 * s;
+    for (int v = 0; v < 3; v++) {
+        const double *V = m->coords + 3 * m->tri[3 * S + v];
+        double d = sqrt((V[0] - P[0]) * (V[0] - P[0]) + (V[1] - P[1]) * (V[1] - P[1]) + (V[2] - P[2]) * (V[2] - P[2]));
+        if (d > dmax) dmax = d;
+        if (d < dmin) dmin = d;
+    }
+    return ((lag - 2) * cS * dt > dmax + maxLen) || ((lag + 1) * cP * dt < dmin - 2 * maxLen);
+}
+
+/* Block (test triangle S, vertex s) of K_lag, 3x3 column-major.  `prune` = 1
+ * applies the reference's skip tests (isBlockNull, calcSubBlockSingK.m:12);
+ * 0 integrates every block (used to show those tests drop zeros only). */
+int orc_k_block(const orc_param *p, const orc_mesh *m, int nGH, const double *ghn, const double *ghw, int nInt,
+                const double *ghcn, const double *ghcw, int nDiag, const double *dgn, const double *dgw, int lag,
+                int S, int s, int prune, double *out) {
+    const double dt = p->Tfin / p->nT, cP = p->velP, cS = p->velS;
+    memset(out, 0, 9 * sizeof(double));
+    double maxLen = 0;
+    for (int t = 0; t < m->N; t++)
+        if (m->maxL[t] > maxLen) maxLen = m->maxL[t];
+    double xq[28 * 3], wq[28];
+    tri_gh(m, S, nGH, ghn, ghw, xq, wq);
+    const int ct[4] = {-2, -1, 0, 1};
+    const double cn[4] = {-1, 3, -3, 1};
+    int any = 0;
+    /* regular part: field triangles F != S with vertex s (kernelK.cu:63-160) */
+    if (!prune || !k_block_null(m, S, s, lag, dt, cP, cS, maxLen)) {
+        double acc[3][3] = {{0}};
+        for (int F = 0; F < m->N; F++) {
+            if (F == S) continue;
+            int v = -1;
+            for (int k = 0; k < 3; k++)
+                if (m->tri[3 * F + k] == s) v = k;
+            if (v < 0) continue;
+            double TF[9], mat[9], vec[3];
+            tri_verts(m, F, TF);
+            orc_k_basis(TF, mat, vec);
+            const double *nF = m->normal + 3 * F;
+            for (int pi = 0; pi < nInt; pi++) {
+                const double *b = ghcn + 3 * pi;
+                double y[3];
+                for (int e = 0; e < 3; e++) y[e] = b[0] * TF[e] + b[1] * TF[3 + e] + b[2] * TF[6 + e];
+                const double wi = ghcw[pi] * m->area[F];
+                for (int l = 0; l < nGH; l++)
+                    for (int k = 0; k < 4; k++) {
+                        const double t = dt * (lag + ct[k]);
+                        if (t <= 0) continue;
+                        double K[3][3];
+                        k_kernel(xq + 3 * l, y, nF, mat, vec, v, t, p, 1, K);
+                        for (int i = 0; i < 3; i++)
+                            for (int j = 0; j < 3; j++) acc[i][j] += wq[l] * wi * cn[k] * K[i][j];
+                    }
+            }
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                if (fabs(acc[i][j]) > 1.0e-14) out[i + 3 * j] = acc[i][j] / (4 * PI * dt);
+        any = 1;
+    }
+    /* the test triangle's own vertex: calcSubBlockSingK.m + calcKIntgSing.m */
+    int vS = -1;
+    for (int k = 0; k < 3; k++)
+        if (m->tri[3 * S + k] == s) vS = k;
+    if (vS >= 0 && (!prune || !(lag >= (int)ceil(m->maxL[S] / (cS * dt)) + 2))) {
+        double TS[9], mat[9], vec[3];
+        tri_verts(m, S, TS);
+        orc_k_basis(TS, mat, vec);
+        const double *nS = m->normal + 3 * S;
+        double blk[3][3] = {{0}};
+        for (int k = 0; k < 4; k++) {
+            const int it = lag + ct[k];
+            if (it <= 0) continue;
+            const double t = it * dt;
+            double tot[3][3] = {{0}};
+            for (int l = 0; l < nGH; l++) {
+                const double *P = xq + 3 * l;
+                double in[3][3] = {{0}};
+                /* children [V_c, V_{mod(c+1,3)+1}, P] (calcCostantData.m:49-55) */
+                const int ca[3] = {0, 1, 2}, cb[3] = {2, 0, 1};
+                for (int c = 0; c < 3; c++) {
+                    const double *A = TS + 3 * ca[c], *B = TS + 3 * cb[c];
+                    double u[3] = {A[0] - P[0], A[1] - P[1], A[2] - P[2]}, w2[3] = {B[0] - P[0], B[1] - P[1], B[2] - P[2]};
+                    double cr[3] = {u[1] * w2[2] - u[2] * w2[1], u[2] * w2[0] - u[0] * w2[2], u[0] * w2[1] - u[1] * w2[0]};
+                    double ac = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]) / 2;
+                    for (int d = 0; d < nDiag; d++) {
+                        const double *b = dgn + 3 * d;
+                        double y[3];
+                        for (int e = 0; e < 3; e++) y[e] = b[0] * A[e] + b[1] * B[e] + b[2] * P[e];
+                        double K[3][3];
+                        k_kernel(P, y, nS, mat, vec, vS, t, p, 0, K);
+                        for (int i = 0; i < 3; i++)
+                            for (int j = 0; j < 3; j++) in[i][j] += (dgw[d] * ac) * K[i][j];
+                    }
+                }
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) tot[i][j] += wq[l] * in[i][j];
+            }
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) blk[i][j] += cn[k] * tot[i][j];
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) out[i + 3 * j] += blk[i][j] / (4 * PI * dt);
+        any = 1;
+    }
+    return any;
+}
+
+/* K_lag for test rows [r0, r1) as a dense 3N x 3Nv column-major matrix. */
+int orc_k_matrix_rows(const orc_param *p, const orc_mesh *m, int nGH, const double *ghn, const double *ghw, int nInt,
+                      const double *ghcn, const double *ghcw, int nDiag, const double *dgn, const double *dgw,
+                      int lag, int r0, int r1, double *out, int nthreads) {
+    const long n3 = 3L * m->N;
+    memset(out, 0, sizeof(double) * n3 * 3L * m->Nv);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+    for (long q = 0; q < (long)(r1 - r0) * m->Nv; q++) {
+        int S = r0 + (int)(q / m->Nv), s = (int)(q % m->Nv);
+        double blk[9];
+        orc_k_block(p, m, nGH, ghn, ghw, nInt, ghcn, ghcw, nDiag, dgn, dgw, lag, S, s, 1, blk);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) out[(3L * S + i) + n3 * (3L * s + j)] = blk[i + 3 * j];
+    }
+    return 0;
+}

@@ -137,6 +137,22 @@ void orc_pp_block_num(const orc_param *p, const orc_mesh *m, int F, const double
 void orc_pp_point_num(const orc_param *p, const orc_mesh *m, const double *density, const double *x, double t,
                       int nInt, const double *rn, const double *rw, double *u);
 
+/* ---- double-layer K of the direct Dirichlet driver -------------------- */
+/* P1 basis of a triangle (dir/BEMenerg_dir_calcCostantData.m:16-20):
+ * phi_k(x) = vec[k] + mat[3k..3k+2] . x, phi_k(V_l) = delta_kl. */
+void orc_k_basis(const double *T, double *mat, double *vec);
+/* Block (test triangle S, vertex s) of K_lag, 3x3 column-major: kernelK.cu
+ * over the field triangles F != S that have vertex s, plus
+ * calcSubBlockSingK.m when s is a vertex of S.  `prune` = 1 applies the
+ * reference's skip tests, 0 integrates every block.  Returns 1 if computed. */
+int orc_k_block(const orc_param *p, const orc_mesh *m, int nGH, const double *ghn, const double *ghw, int nInt,
+                const double *ghcn, const double *ghcw, int nDiag, const double *dgn, const double *dgw, int lag,
+                int S, int s, int prune, double *out);
+/* K_lag restricted to test rows [r0, r1): dense 3N x 3Nv column-major. */
+int orc_k_matrix_rows(const orc_param *p, const orc_mesh *m, int nGH, const double *ghn, const double *ghw, int nInt,
+                      const double *ghcn, const double *ghcw, int nDiag, const double *dgn, const double *dgw,
+                      int lag, int r0, int r1, double *out, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,12 @@ class Oracle:
         L.orc_ghc_matrix_block_rows.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, _dp, C.c_int,
                                                 _dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp,
                                                 C.c_int]
+        L.orc_k_basis.argtypes = [_dp, _dp, _dp]
+        L.orc_k_block.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, _dp, C.c_int, _dp, _dp,
+                                  C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_k_block.restype = C.c_int
+        L.orc_k_matrix_rows.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, _dp, C.c_int, _dp, _dp,
+                                        C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_int]
 
     # -- pieces -------------------------------------------------------------
     def num_blocks(self) -> int:
@@ -228,6 +234,49 @@ class Oracle:
                                            self.dgw.size, _p(self.dgn), _p(self.dgw), int(lag), r0, r1,
                                            out.ctypes.data_as(_dp), nt)
         return np.ascontiguousarray(out)
+
+    # -- double-layer K of the direct Dirichlet driver ------------------------
+    def k_basis(self, T):
+        """P1 basis of triangle T (3x3, vertices row-wise): (mat 3x3, vec 3), phi_k(x) = vec[k] + mat[k] . x."""
+        mat, vec = np.zeros(9), np.zeros(3)
+        self.lib.orc_k_basis(_p(np.ascontiguousarray(T, np.float64)), _p(mat), _p(vec))
+        return mat.reshape(3, 3), vec
+
+    def k_num_blocks(self) -> int:
+        """numBlocksK = min(numBlocksV + 1, nT) (timeMarchingD.m:176)."""
+        return min(self.num_blocks() + 1, self.nT)
+
+    def k_block(self, lag, S, s, prune=True):
+        """Block (test triangle S, vertex s) of K_lag (needs set_ghc); prune=False
+        integrates it even where the reference's skip tests drop it."""
+        out = np.zeros(9)
+        self.lib.orc_k_block(C.byref(self.param), C.byref(self.mesh), self.nGH, _p(self.ghn), _p(self.ghw),
+                             self.ghcw.size, _p(self.ghcn), _p(self.ghcw), self.dgw.size, _p(self.dgn), _p(self.dgw),
+                             int(lag), int(S), int(s), int(bool(prune)), _p(out))
+        return out.reshape(3, 3).T.copy()
+
+    def k_matrix(self, lag, rows=None, nthreads=None):
+        """Dense K_lag (3N x 3Nv), test rows restricted to `rows`."""
+        N, Nv = self.N, self.coords.shape[0]
+        out = np.zeros((3 * N, 3 * Nv), order="F")
+        r0, r1 = (0, N) if rows is None else rows
+        nt = nthreads or os.cpu_count() or 1
+        self.lib.orc_k_matrix_rows(C.byref(self.param), C.byref(self.mesh), self.nGH, _p(self.ghn), _p(self.ghw),
+                                   self.ghcw.size, _p(self.ghcn), _p(self.ghcw), self.dgw.size, _p(self.dgn),
+                                   _p(self.dgw), int(lag), r0, r1, out.ctypes.data_as(_dp), nt)
+        return np.ascontiguousarray(out)
+
+    def beta_k(self, g, rows=None):
+        """beta_K (3N x nT) of dir/BEMenerg_dir_calcBetaKgpu.m:7-13 for the vertex datum
+        g (3Nv x nT, column j at time (j+1) dt): beta_n = sum_{j<=n} K_{n-j} g_j."""
+        LK = self.k_num_blocks()
+        K = [self.k_matrix(l, rows) for l in range(LK)]
+        out = np.zeros((3 * self.N, self.nT))
+        for n in range(self.nT):
+            for j in range(n + 1):
+                if n - j < LK:
+                    out[:, n] += K[n - j] @ g[:, j]
+        return out
 
     def set_pp_rule(self, nSub=64, nSing=3):
         """Rule of the numeric post-processing (PPn, PPw)."""
