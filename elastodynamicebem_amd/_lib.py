@@ -39,6 +39,13 @@ SIGNATURES = {
     "ebem_get_block": (C.c_int, [_vp, C.c_int, _dp]),
     "ebem_get_block_rows": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
     "ebem_rhs": (C.c_int, [_vp, C.c_int, _dp]),
+    "ebem_k_num_blocks": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "ebem_assemble_k": (C.c_int, [_vp]),
+    "ebem_k_store_info": (C.c_int, [_vp, _i64p, _i64p, _i64p]),
+    "ebem_get_k_block_rows": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
+    "ebem_dirichlet_datum": (C.c_int, [_vp, C.c_int, _dp]),
+    "ebem_beta_k": (C.c_int, [_vp, _dp, _dp]),
+    "ebem_rhs_dirichlet": (C.c_int, [_vp, C.c_int, _dp]),
     "ebem_mot_solve": (C.c_int, [_vp, _dp, _dp]),
     "ebem_pp_eval": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
     "ebem_node_integral": (C.c_int, [_vp, _dp, C.c_int, C.c_double, _dp]),

@@ -36,10 +36,16 @@ DATUM_SCREEN_SIN2 = 0
 DATUM_SPHERE_T2EXP = 1
 DATUM_REF_SPHERE = 2
 DATUM_REF_SCREEN = 3
+DATUM_REF_BAR_H1 = 4
+DATUM_REF_BAR_H3 = 5
+DATUM_REF_SPHERE_WAVE = 6
+DATUM_REF_ELEMENTO = 7
 
 # datum hard-coded per domainType in Tn/BEMenerg_core_calcNucleoTn.m:133-150
 _DOMAIN_DATUM = {"sphereUniform": DATUM_REF_SPHERE, "sphereNotUniform": DATUM_REF_SPHERE,
-                 "screenUniform": DATUM_REF_SCREEN, "screenGraded": DATUM_REF_SCREEN}
+                 "screenUniform": DATUM_REF_SCREEN, "screenGraded": DATUM_REF_SCREEN,
+                 "barH1": DATUM_REF_BAR_H1, "barH3": DATUM_REF_BAR_H3, "sphereWave": DATUM_REF_SPHERE_WAVE,
+                 "elementoIndustriale": DATUM_REF_ELEMENTO}
 
 
 @dataclass
@@ -251,6 +257,52 @@ class Solver:
         _lib.check(self.lib.ebem_rhs(self.h, datum, out.ctypes.data_as(_lib._dp)))
         return out
 
+    # -- double-layer K of the direct Dirichlet driver -------------------------
+    def k_num_blocks(self) -> int:
+        """numBlocksK = min(L + 1, nT)."""
+        L = C.c_int()
+        _lib.check(self.lib.ebem_k_num_blocks(self.h, C.byref(L)))
+        return L.value
+
+    def assemble_k(self) -> None:
+        """K_0 .. K_{LK-1} on the device (needs set_inner_rule)."""
+        _lib.check(self.lib.ebem_assemble_k(self.h))
+
+    def k_store_info(self) -> dict:
+        v = [C.c_int64() for _ in range(3)]
+        _lib.check(self.lib.ebem_k_store_info(self.h, *[C.byref(x) for x in v]))
+        return dict(pairs=v[0].value, slots=v[1].value, bytes=v[2].value)
+
+    def get_k_block(self, lag: int, rows=None) -> np.ndarray:
+        """K_lag (3N x 3Nv), row index = test DOF; with rows=(r0, r1) only those
+        test triangles' rows (3 (r1 - r0) x 3Nv)."""
+        r0, r1 = (0, self.N) if rows is None else rows
+        out = np.zeros((3 * (r1 - r0), 3 * self.mesh.number_nodes), order="F")
+        _lib.check(self.lib.ebem_get_k_block_rows(self.h, lag, r0, r1, out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def dirichlet_datum(self, datum: int) -> np.ndarray:
+        """Vertex datum g (3Nv x nT), column n at time (n+1) dt."""
+        out = np.zeros((3 * self.mesh.number_nodes, self.nT), order="F")
+        _lib.check(self.lib.ebem_dirichlet_datum(self.h, datum, out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def beta_k(self, g: np.ndarray) -> np.ndarray:
+        """beta_K (3N x nT) = sum_j K_{n-j} g_j for a vertex datum g (3Nv x nT)."""
+        g = np.asfortranarray(g, dtype=np.float64)
+        if g.shape != (3 * self.mesh.number_nodes, self.nT):
+            raise ValueError("g must be 3Nv x nT")
+        out = np.zeros((3 * self.N, self.nT), order="F")
+        _lib.check(self.lib.ebem_beta_k(self.h, g.ctypes.data_as(_lib._dp), out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def rhs_dirichlet(self, datum: int) -> np.ndarray:
+        """Right-hand side rhs / 2 + beta_K of the direct Dirichlet driver, kept on
+        the device for mot_solve(None)."""
+        out = np.zeros((3 * self.N, self.nT), order="F")
+        _lib.check(self.lib.ebem_rhs_dirichlet(self.h, datum, out.ctypes.data_as(_lib._dp)))
+        return out
+
     def mot_solve(self, beta: np.ndarray | None = None) -> np.ndarray:
         out = np.zeros((3 * self.N, self.nT), order="F")
         b = None if beta is None else np.asfortranarray(beta, dtype=np.float64)
@@ -290,7 +342,8 @@ class Solver:
 
     def timings(self) -> dict:
         keys = ["levels_ms", "assembly_ms", "factor_ms", "history_ms", "apply_ms", "mot_ms", "pp_ms",
-                "history_launches", "levels_kp_ms", "pp_kp_ms", "levels_bf_ms", "pp_bf_ms"]
+                "history_launches", "levels_kp_ms", "pp_kp_ms", "levels_bf_ms", "pp_bf_ms", "k_assembly_ms",
+                "beta_k_ms"]
         out = np.zeros(len(keys))
         _lib.check(self.lib.ebem_timings(self.h, _lib.dptr(out), len(keys)))
         return dict(zip(keys, out.tolist()))
@@ -328,29 +381,53 @@ def BEMenerg_coreSA_timeMarching(pbParam: PbParam, domainMesh: SpaceMesh, method
 def BEMenerg_dir_timeMarchingD(pbParam, domainMesh, methodInfo, GHn=None, GHw=None, GHCn=None, GHCw=None,
                                DIAGn=None, DIAGw=None, *, datum=None, precision: int = PREC_FP64, device: int = 0,
                                save_dir: str | None = None):
-    """Single-layer (V) system of the direct Dirichlet driver: with methodInfo
-    "FN ..." the numeric inner quadrature of the reference's GPU kernel
-    (kernelV.cu + calcSubBlockDiagV.m, PREC_GHC), with "SA ..." the analytic
-    form.  The double-layer right-hand side of timeMarchingD.m:287 is not part
-    of this library (raises if a DD problem with lambda+mu == 0 is given, like
-    main.m:61-63).  With save_dir, the density is also written as the
-    reference does (timeMarchingD.m:300-301):
+    """Direct Dirichlet driver (dir/BEMenerg_dir_timeMarchingD.m): with methodInfo
+    "FN nExt nSub nSing nDiag" the single-layer blocks by the numeric inner
+    quadrature of the reference's GPU kernel (kernelV.cu + calcSubBlockDiagV.m,
+    PREC_GHC), the double-layer blocks K_0 .. K_{LK-1} (kernelK.cu +
+    calcSubBlockSingK.m), the right-hand side rhs / 2 + beta_K (:163-166, 272,
+    287) and the MOT with E_0 inverted (:277-296), all on the device.  Raises
+    if lambda + mu == 0, like main.m:61-63.  With save_dir, the density is also
+    written as the reference does (:300-301):
     <save_dir>/<domainType><lev>_NEW_<ext>_<sub>_<sing>_<diag>_density.mat."""
     if pbParam.lambda_ + pbParam.mu == 0:
         raise ValueError("Problema non adatto a questa formulazione")
     if methodInfo.typeIntg != "FN":
-        density = BEMenerg_coreSA_timeMarching(pbParam, domainMesh, methodInfo, GHn, GHw, datum=datum,
-                                               precision=precision, device=device)
-    else:
-        with Solver(domainMesh, pbParam, methodInfo.numNodiExt, device) as s:
-            s.set_inner_rule(methodInfo.numSubRegion, methodInfo.numNodiSing, methodInfo.numNodiDiag)
-            s.assemble(PREC_GHC)
-            s.rhs(_datum_for(pbParam, datum))
-            density = s.mot_solve(None)
+        raise ValueError("the direct Dirichlet driver needs an FN method (numeric inner rules of K)")
+    kind = _datum_for(pbParam, datum)
+    with Solver(domainMesh, pbParam, methodInfo.numNodiExt, device) as s:
+        s.set_inner_rule(methodInfo.numSubRegion, methodInfo.numNodiSing, methodInfo.numNodiDiag)
+        s.assemble(PREC_GHC)
+        s.assemble_k()
+        s.rhs_dirichlet(kind)
+        density = s.mot_solve(None)
     if save_dir is not None:
         from . import matio
         matio.save_density(matio.output_base(pbParam, "NEW", methodInfo, save_dir), density)
     return density
+
+
+def BEMenerg_dir_calcBoundDataDirichlet(deltaT, pbParam, domainMesh, *, datum=None, device: int = 0):
+    """dirBoundData: list of nT vertex data (3Nv each), entry n at time (n+1) dt
+    (dir/BEMenerg_dir_calcBoundDataDirichlet.m)."""
+    if abs(deltaT - pbParam.Tfin / pbParam.nT) > 1e-12 * max(1.0, deltaT):
+        raise ValueError("deltaT must be Tfin / nT")
+    with Solver(domainMesh, pbParam, 7, device) as s:
+        g = s.dirichlet_datum(_datum_for(pbParam, datum))
+    return [g[:, n].copy() for n in range(pbParam.nT)]
+
+
+def BEMenerg_dir_calcBetaKgpu(deltaT, pbParam, domainMesh, methodInfo, *, datum=None, device: int = 0):
+    """betaK: list of nT vectors (3N each), beta_n = sum_{j<=n} K_{n-j} g_j with the
+    Dirichlet datum of the domain (dir/BEMenerg_dir_calcBetaKgpu.m); the K blocks
+    are assembled here (the reference passes matrixSavedK)."""
+    if abs(deltaT - pbParam.Tfin / pbParam.nT) > 1e-12 * max(1.0, deltaT):
+        raise ValueError("deltaT must be Tfin / nT")
+    with Solver(domainMesh, pbParam, methodInfo.numNodiExt, device) as s:
+        s.set_inner_rule(methodInfo.numSubRegion, methodInfo.numNodiSing, methodInfo.numNodiDiag)
+        s.assemble_k()
+        b = s.beta_k(s.dirichlet_datum(_datum_for(pbParam, datum)))
+    return [b[:, n].copy() for n in range(pbParam.nT)]
 
 
 def BEMenerg_postProcA_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, *, device: int = 0,

@@ -1,0 +1,239 @@
+// ebem_k.hpp -- device kernels of the double-layer operator K (math in
+// ebem_k_math.hpp): pair windows, slot map, blocks, Dirichlet datum, beta_K.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ebem_k_math.hpp"
+#include "ebem_kernels.hpp"
+
+namespace ebem {
+
+// Windows of the (S, s) pairs of row S = blockIdx.x; rowcnt[S] = slots of the
+// row, rowcnt[N + S] = its pairs with a nonempty window.
+__global__ void __launch_bounds__(256) k_kwin(const double* __restrict__ verts, const double* __restrict__ X,
+                                              const int32_t* __restrict__ T, const FieldTri* __restrict__ tri,
+                                              KParams K, int16_t* __restrict__ lo, int16_t* __restrict__ hi,
+                                              int64_t* __restrict__ rowcnt) {
+    __shared__ long long part[256], partp[256];
+    const int S = blockIdx.x;
+    const double* vS = verts + (int64_t)S * 9;
+    const int cut = k_sing_cut(tri[S].maxL, K);
+    const int t0 = T[3 * S], t1 = T[3 * S + 1], t2 = T[3 * S + 2];
+    long long cnt = 0, npr = 0;
+    for (int s = threadIdx.x; s < K.Nv; s += blockDim.x) {
+        const double* P = X + 3 * (int64_t)s;
+        const bool own = (s == t0) || (s == t1) || (s == t2);
+        int l0 = -1, l1 = -1;
+        for (int lag = 0; lag < K.LK; lag++) {
+            const bool nz = !k_block_null(vS, P, lag, K) || (own && lag < cut);
+            if (nz) {
+                if (l0 < 0) l0 = lag;
+                l1 = lag + 1;
+            }
+        }
+        const int64_t p = (int64_t)S * K.Nv + s;
+        lo[p] = (int16_t)(l0 < 0 ? 0 : l0);
+        hi[p] = (int16_t)(l0 < 0 ? 0 : l1);
+        cnt += l0 < 0 ? 0 : l1 - l0;
+        npr += l0 < 0 ? 0 : 1;
+    }
+    part[threadIdx.x] = cnt;
+    partp[threadIdx.x] = npr;
+    __syncthreads();
+    for (int d = 128; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d) {
+            part[threadIdx.x] += part[threadIdx.x + d];
+            partp[threadIdx.x] += partp[threadIdx.x + d];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        rowcnt[S] = part[0];
+        rowcnt[K.N + S] = partp[0];
+    }
+}
+
+// Slot offsets of the pairs of row S = blockIdx.x and the slot -> pair map.
+__global__ void __launch_bounds__(256) k_kslots(const int16_t* __restrict__ lo, const int16_t* __restrict__ hi,
+                                                int Nv, const int64_t* __restrict__ rowbase,
+                                                int64_t* __restrict__ off, int32_t* __restrict__ slot_pair) {
+    __shared__ long long part[256];
+    const int S = blockIdx.x;
+    const int nper = (Nv + blockDim.x - 1) / blockDim.x;
+    const int j0 = threadIdx.x * nper;
+    const int64_t pbase = (int64_t)S * Nv;
+    long long s = 0;
+    for (int k = 0; k < nper; k++) {
+        const int j = j0 + k;
+        if (j < Nv) s += hi[pbase + j] - lo[pbase + j];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        long long v = ((int)threadIdx.x >= d) ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    long long run = rowbase[S] + (threadIdx.x ? part[threadIdx.x - 1] : 0);
+    for (int k = 0; k < nper; k++) {
+        const int j = j0 + k;
+        if (j >= Nv) break;
+        const int64_t p = pbase + j;
+        const int n = hi[p] - lo[p];
+        off[p] = run;
+        for (int q = 0; q < n; q++) slot_pair[run + q] = (int32_t)p;
+        run += n;
+    }
+}
+
+// One wave per slot (grid-stride): the block (S, s) of K_lag, the lanes'
+// parts of k_reg_lane / k_sing_lane summed by an xor butterfly.  Regular part
+// cleaned below 1e-14 and divided by 4 pi dt (kernelK.cu:212-221); singular
+// part combined over levels with -1, 3, -3, 1 and divided by 4 pi dt
+// (calcSubBlockSingK.m:17-39), added to it (timeMarchingD.m:244-255).
+__global__ void __launch_bounds__(256) k_kblocks(const KTri* __restrict__ kt, const FieldTri* __restrict__ tri,
+                                                 const double* __restrict__ verts, const double* __restrict__ gh,
+                                                 const double* __restrict__ X, const int32_t* __restrict__ T,
+                                                 const int32_t* __restrict__ vadj_off,
+                                                 const int32_t* __restrict__ vadj, const double* __restrict__ rin_g,
+                                                 const double* __restrict__ rdg_g, int nGH, KParams K, int64_t nslots,
+                                                 const int32_t* __restrict__ slot_pair,
+                                                 const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
+                                                 double* __restrict__ val) {
+#pragma clang fp contract(off)
+    extern __shared__ double rule[];  // (nInt + nDiag) x 4
+    for (int k = threadIdx.x; k < 4 * (K.nInt + K.nDiag); k += blockDim.x)
+        rule[k] = k < 4 * K.nInt ? rin_g[k] : rdg_g[k - 4 * K.nInt];
+    __syncthreads();
+    KBlockArgs A{kt, verts, vadj_off, vadj, rule, rule + 4 * K.nInt, nullptr, nGH};
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int ct[4] = {-2, -1, 0, 1};
+    const double cn[4] = {-1, 3, -3, 1};
+    for (int64_t it = wave; it < nslots; it += nwaves) {
+        const int32_t p = slot_pair[it];
+        const int S = p / K.Nv, s = p % K.Nv;
+        const int lag = lo[p] + (int)(it - off[p]);
+        A.g = gh + (int64_t)S * 128;
+        double out[9];
+        for (int e = 0; e < 9; e++) out[e] = 0.0;
+        if (!k_block_null(verts + (int64_t)S * 9, X + 3 * (int64_t)s, lag, K)) {  // wave-uniform
+            double acc[3][3];
+            k_reg_lane(A, K, S, s, lag, lane, 64, acc);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    double a = acc[i][c];
+                    for (int m = 32; m >= 1; m >>= 1) a += __shfl_xor(a, m);
+                    out[i + 3 * c] = fabs(a) > 1.0e-14 ? a / K.fourpidt : 0.0;
+                }
+        }
+        const int vl = k_local_vertex(T, S, s);
+        if (vl >= 0 && lag < k_sing_cut(tri[S].maxL, K)) {  // wave-uniform
+            double blk[3][3];
+            for (int i = 0; i < 3; i++)
+                for (int c = 0; c < 3; c++) blk[i][c] = 0.0;
+            for (int l = 0; l < 4; l++) {
+                const int il = lag + ct[l];
+                if (il <= 0) continue;
+                double lev[3][3];
+                k_sing_lane(A, K, S, vl, (double)il * K.dt, lane, 64, lev);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        double a = lev[i][c];
+                        for (int m = 32; m >= 1; m >>= 1) a += __shfl_xor(a, m);
+                        blk[i][c] += cn[l] * a;
+                    }
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) out[i + 3 * c] += blk[i][c] / K.fourpidt;
+        }
+        if (lane == 0) {
+            double* o = val + it * 9;
+            for (int e = 0; e < 9; e++) o[e] = out[e];
+        }
+    }
+}
+
+// Dirichlet datum at the vertices (BEMenerg_dir_calcBoundDataDirichlet.m):
+// gT[(s * nT + n) * 3 + c] = g_c(X_s, (n + 1) dt).
+__global__ void k_dir_datum(const double* __restrict__ X, int Nv, int nT, double dt, double cP, double Tfin,
+                            int kind, double* __restrict__ gT) {
+    const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (id >= (int64_t)Nv * nT) return;
+    const int s = (int)(id / nT), n = (int)(id % nT);
+    double g[3];
+    dir_datum(kind, X + 3 * (int64_t)s, (double)(n + 1) * dt, cP, Tfin, g);
+    for (int c = 0; c < 3; c++) gT[id * 3 + c] = g[c];
+}
+
+// Vertex datum (3Nv x nT column-major) -> gT layout [s][n][c].
+__global__ void k_dir_transpose(const double* __restrict__ g, int Nv, int nT, double* __restrict__ gT) {
+    const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (id >= (int64_t)Nv * nT * 3) return;
+    const int c = (int)(id % 3);
+    const int64_t sn = id / 3;
+    const int s = (int)(sn / nT), n = (int)(sn % nT);
+    gT[id] = g[(int64_t)n * 3 * Nv + 3 * s + c];
+}
+
+// beta_K rows of test triangle S = blockIdx.x: out[n*3N + 3S + i] =
+// sum over the row's slots (pair s, lag l <= n) of K[S,s,l] g_s((n - l + 1) dt);
+// threads own time steps n = tid + 256 k (kKBetaPer per thread).
+constexpr int kKBetaPer = 16;
+__global__ void __launch_bounds__(256) k_kbeta(const int16_t* __restrict__ lo, const int16_t* __restrict__ hi,
+                                               const int64_t* __restrict__ off, const double* __restrict__ val,
+                                               const double* __restrict__ gT, int N, int Nv, int nT,
+                                               double* __restrict__ out) {
+    const int S = blockIdx.x;
+    double acc[kKBetaPer][3];
+#pragma unroll
+    for (int k = 0; k < kKBetaPer; k++) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
+    const int64_t pbase = (int64_t)S * Nv;
+    for (int s = 0; s < Nv; s++) {
+        const int l0 = lo[pbase + s], l1 = hi[pbase + s];
+        if (l1 <= l0) continue;
+        const double* vb = val + off[pbase + s] * 9;
+        const double* gs = gT + (int64_t)s * nT * 3;
+        for (int l = l0; l < l1; l++) {
+            const double* Kb = vb + (int64_t)(l - l0) * 9;  // column-major 3x3
+            const double k00 = Kb[0], k10 = Kb[1], k20 = Kb[2], k01 = Kb[3], k11 = Kb[4], k21 = Kb[5], k02 = Kb[6],
+                         k12 = Kb[7], k22 = Kb[8];
+#pragma unroll
+            for (int k = 0; k < kKBetaPer; k++) {
+                const int n = threadIdx.x + 256 * k;
+                if (n >= nT || n < l) continue;
+                const double* gg = gs + (int64_t)(n - l) * 3;
+                const double g0 = gg[0], g1 = gg[1], g2 = gg[2];
+                acc[k][0] += k00 * g0 + k01 * g1 + k02 * g2;
+                acc[k][1] += k10 * g0 + k11 * g1 + k12 * g2;
+                acc[k][2] += k20 * g0 + k21 * g1 + k22 * g2;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kKBetaPer; k++) {
+        const int n = threadIdx.x + 256 * k;
+        if (n >= nT) continue;
+        double* o = out + (int64_t)n * 3 * N + 3 * S;
+        o[0] = acc[k][0];
+        o[1] = acc[k][1];
+        o[2] = acc[k][2];
+    }
+}
+
+// beta = beta / 2 + betaK (timeMarchingD.m:287).
+__global__ void k_dd_combine(double* __restrict__ beta, const double* __restrict__ betak, int64_t n) {
+    const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (id < n) beta[id] = beta[id] / 2 + betak[id];
+}
+
+}  // namespace ebem

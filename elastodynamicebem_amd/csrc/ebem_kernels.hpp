@@ -1242,7 +1242,26 @@ __global__ void k_alpha_T(const double* __restrict__ alpha, int N, int nT, doubl
 
 // ------------------------------------------------------------------ RHS --
 
-__device__ inline void datum(int kind, const double* x, const double* nrm, double t, double* g) {
+// 1-D bar solution of height h (calcNucleoTn.m:148-164, calcBoundDataDirichlet.m:6-13):
+// sum_{k=0}^{khat} (-1)^k [(cP t - 2hk - (h - z))_+ - (cP t - 2h(k+1) + (h - z))_+],
+// khat = ceil(cP Tfin / (2h)) - 1.
+__host__ __device__ inline double bar_solution(double h, double cP, double Tfin, double z, double t) {
+#pragma clang fp contract(off)
+    const int khat = (int)ceil((cP * Tfin) / (2 * h)) - 1;
+    double s = 0.0;
+    for (int k = 0; k <= khat; k++) {
+        const double a = cP * t - 2 * h * k - (h - z);
+        const double b = cP * t - 2 * h * (k + 1) + (h - z);
+        const double sg = (k & 1) ? -1.0 : 1.0;
+        s = s + sg * (a * (a > 0 ? 1.0 : 0.0) - b * (b > 0 ? 1.0 : 0.0));
+    }
+    return s;
+}
+
+// Boundary datum of the right-hand side, g(x, t) of calcNucleoTn.m ('DIR'
+// branch for kinds 2..7).
+__device__ inline void datum(int kind, const double* x, const double* nrm, double t, double cP, double Tfin,
+                             double* g) {
     g[0] = g[1] = g[2] = 0.0;
     if (kind == 0) {
         double s = sin(kPi * t / 2.0);
@@ -1258,12 +1277,33 @@ __device__ inline void datum(int kind, const double* x, const double* nrm, doubl
         double s = sin(t);
         double a = (t < kPi / 2.0) ? s * s * s * s * s : 1.0;
         g[2] = a * (x[0] * x[0]);
+    } else if (kind == 4 || kind == 5) {  // barH1, barH3
+        g[2] = bar_solution(kind == 4 ? 1.0 : 3.0, cP, Tfin, x[2], t);
+    } else if (kind == 6) {  // sphereWave: calcNucleoTn.m:166-169 (minus sign)
+        const double d = x[0] - 2 + cP * t - 0.475;
+        g[0] = -exp(-20 * (d * d));
+    } else if (kind == 7) {  // elementoIndustriale: calcNucleoTn.m:171-174
+        const double d = x[2] + 1 - cP * t;
+        g[2] = -exp(-20 * (d * d));
+    }
+}
+
+// Dirichlet datum at a vertex (BEMenerg_dir_calcBoundDataDirichlet.m:5-25):
+// the bars as above, sphereWave with a plus sign, elementoIndustriale zero.
+__host__ __device__ inline void dir_datum(int kind, const double* x, double t, double cP, double Tfin, double* g) {
+#pragma clang fp contract(off)
+    g[0] = g[1] = g[2] = 0.0;
+    if (kind == 4 || kind == 5) {
+        g[2] = bar_solution(kind == 4 ? 1.0 : 3.0, cP, Tfin, x[2], t);
+    } else if (kind == 6) {
+        const double d = x[0] - 2 + cP * t - 0.475;
+        g[0] = exp(-20 * (d * d));
     }
 }
 
 // beta (3N x nT col-major): Tn/BEMenerg_core_calcTnBlock.m:27-58.
 __global__ void k_rhs(const FieldTri* __restrict__ tri, const double* __restrict__ gh, int N, int nT, int nGH,
-                      double dt, int kind, double* __restrict__ beta) {
+                      double dt, double cP, double Tfin, int kind, double* __restrict__ beta) {
     int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (id >= (int64_t)N * nT) return;
     int T = (int)(id % N), n = (int)(id / N);
@@ -1273,8 +1313,8 @@ __global__ void k_rhs(const FieldTri* __restrict__ tri, const double* __restrict
     for (int q = 0; q < nGH; q++) {
         double x[3] = {g[q], g[32 + q], g[64 + q]};
         double a[3], b[3];
-        datum(kind, x, tri[T].n, t1, a);
-        datum(kind, x, tri[T].n, t0, b);
+        datum(kind, x, tri[T].n, t1, cP, Tfin, a);
+        datum(kind, x, tri[T].n, t0, cP, Tfin, b);
         for (int k = 0; k < 3; k++) ris[k] = ris[k] + (a[k] - b[k]) * g[96 + q];
     }
     for (int k = 0; k < 3; k++) beta[(int64_t)n * 3 * N + 3 * T + k] = fabs(ris[k]) < 1.0e-14 ? 0.0 : ris[k];

@@ -23,6 +23,7 @@
 #include "ebem_kernels.hpp"
 #include "ebem_sur.hpp"
 #include "ebem_ghc.hpp"
+#include "ebem_k.hpp"
 
 using namespace ebem;
 
@@ -421,7 +422,7 @@ struct ebem_handle {
     DevBuf<rocblas_int> d_ipiv, d_info;
     bool have_beta = false, have_alpha = false;
     bool e0_pivoted = false;  // the last factorisation needed partial pivoting
-    double tim[EBEM_NUM_TIMINGS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double tim[EBEM_NUM_TIMINGS] = {};
     // scratch of ebem_assemble and ebem_pp_eval, kept across calls (grow-only)
     struct {
         DevBuf<uint8_t> flag, key;
@@ -441,6 +442,18 @@ struct ebem_handle {
         DevBuf<unsigned long long> bo;
     } pw;
     std::vector<hipEvent_t> mot_ev;  // per-step events of ebem_mot_solve
+    // double-layer K of the direct Dirichlet driver (ebem_assemble_k):
+    // pair windows [klo, khi) over (S, s) = S * Nv + s, slot offsets koff,
+    // values 9 per slot; rows S occupy slots [k_rowbase[S], k_rowbase[S+1])
+    bool k_ready = false;
+    int k_LK = 0;
+    int64_t k_pairs = 0, k_slots = 0;
+    std::vector<int64_t> k_rowbase;
+    DevBuf<KTri> d_ktri;
+    DevBuf<int32_t> d_T, d_vadj_off, d_vadj, d_kslot;
+    DevBuf<double> d_X, d_kval, d_gT, d_betak;
+    DevBuf<int16_t> d_klo, d_khi;
+    DevBuf<int64_t> d_koff, d_krow;
     bool sort_items = true;
     bool kp_split = true;  // kP ranges through k_levels_kp (EBEM_KP_SPLIT=0 turns it off)
     // kP and between-front ranges of the last assembly / post-processing:
@@ -1406,6 +1419,144 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
 
 // ================================================================= C ABI ==
 
+
+// ---------------------------------------------------------------------------
+// Double-layer K of the direct Dirichlet driver (timeMarchingD.m:173-274)
+// ---------------------------------------------------------------------------
+
+// numBlocksK = min(numBlocksV + 1, nT) (timeMarchingD.m:176)
+int k_num_blocks(ebem_handle* h) {
+    compute_L(h);
+    return std::min(h->L + 1, h->nT);
+}
+
+int assemble_k(ebem_handle* h) {
+#pragma clang fp contract(off)
+    if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
+    if (!h->ghc_nSub) return fail(EBEM_EINVAL, "ebem_assemble_k needs ebem_set_inner_rule first");
+    if (h->lambda + h->mu == 0) return fail(EBEM_EINVAL, "Problema non adatto a questa formulazione");
+    const int N = h->N, Nv = h->Nv;
+    if ((int64_t)N * Nv >= (int64_t)INT32_MAX) return fail(EBEM_EINVAL, "mesh too large for the K store");
+    const int LK = k_num_blocks(h);
+    if (LK > 32767) return fail(EBEM_EINVAL, "too many lags for the K store");
+    Timer tall;
+    tall.start(h->stream);
+    // P1 bases and the vertex -> (triangle, local vertex) lists, triangles ascending
+    std::vector<KTri> kt(N);
+    std::vector<int32_t> voff(Nv + 1, 0), vadj(3 * (size_t)N);
+    for (int F = 0; F < N; F++) {
+        const double* V[3] = {&h->X[3 * (size_t)h->T[3 * F]], &h->X[3 * (size_t)h->T[3 * F + 1]],
+                              &h->X[3 * (size_t)h->T[3 * F + 2]]};
+        k_tri_basis(V[0], V[1], V[2], h->tri[F].n, h->tri[F].area, kt[F]);
+        for (int v = 0; v < 3; v++) voff[h->T[3 * F + v] + 1]++;
+    }
+    for (int s = 0; s < Nv; s++) voff[s + 1] += voff[s];
+    {
+        std::vector<int32_t> pos(voff.begin(), voff.end() - 1);
+        for (int F = 0; F < N; F++)
+            for (int v = 0; v < 3; v++) vadj[pos[h->T[3 * F + v]]++] = F * 4 + v;
+    }
+    HIPC(h->d_ktri.grow(N));
+    HIPC(h->d_T.grow(3 * (size_t)N));
+    HIPC(h->d_X.grow(3 * (size_t)Nv));
+    HIPC(h->d_vadj_off.grow(Nv + 1));
+    HIPC(h->d_vadj.grow(3 * (size_t)N));
+    HIPC(hipMemcpyAsync(h->d_ktri.p, kt.data(), sizeof(KTri) * N, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->d_T.p, h->T.data(), sizeof(int32_t) * 3 * N, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->d_X.p, h->X.data(), sizeof(double) * 3 * Nv, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->d_vadj_off.p, voff.data(), sizeof(int32_t) * (Nv + 1), hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->d_vadj.p, vadj.data(), sizeof(int32_t) * 3 * N, hipMemcpyHostToDevice, h->stream));
+    KParams K{};
+    K.N = N;
+    K.Nv = Nv;
+    K.LK = LK;
+    K.nInt = h->ghc_nSub * h->ghc_nSing;
+    K.nDiag = h->ghc_nDiag;
+    K.dt = h->dt;
+    K.cP = h->cP;
+    K.cS = h->cS;
+    K.lam = h->lambda;
+    K.mu = h->mu;
+    K.rho = h->rho;
+    K.maxLen = h->hmax;
+    K.fourpidt = 4 * kPi * h->dt;
+    // pair windows and per-row slot counts
+    const int64_t np = (int64_t)N * Nv;
+    HIPC(h->d_klo.grow(np));
+    HIPC(h->d_khi.grow(np));
+    HIPC(h->d_krow.grow(2 * (size_t)N));
+    hipLaunchKernelGGL(k_kwin, dim3(N), dim3(256), 0, h->stream, h->d_verts.p, h->d_X.p, h->d_T.p, h->d_tri.p, K,
+                       h->d_klo.p, h->d_khi.p, h->d_krow.p);
+    HIPC(hipGetLastError());
+    std::vector<int64_t> rc(2 * (size_t)N);  // slots per row, then pairs per row
+    HIPC(hipMemcpyAsync(rc.data(), h->d_krow.p, sizeof(int64_t) * 2 * N, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    h->k_rowbase.assign(N + 1, 0);
+    for (int S = 0; S < N; S++) h->k_rowbase[S + 1] = h->k_rowbase[S] + rc[S];
+    const int64_t slots = h->k_rowbase[N];
+    {
+        size_t freeb = 0, totb = 0;
+        HIPC(hipMemGetInfo(&freeb, &totb));
+        const double need = (double)slots * (9 * 8 + 4) + (double)np * 8;
+        if (need > 0.8 * (double)freeb)
+            return fail(EBEM_ENOMEM, "K store needs %.1f GB of device memory", need / 1e9);
+    }
+    HIPC(h->d_koff.grow(np));
+    HIPC(h->d_kslot.grow(std::max<int64_t>(slots, 1)));
+    HIPC(h->d_kval.grow((size_t)9 * std::max<int64_t>(slots, 1)));
+    DevBuf<int64_t> d_rb;
+    HIPC(d_rb.alloc(N));
+    HIPC(hipMemcpyAsync(d_rb.p, h->k_rowbase.data(), sizeof(int64_t) * N, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_kslots, dim3(N), dim3(256), 0, h->stream, h->d_klo.p, h->d_khi.p, Nv, d_rb.p, h->d_koff.p,
+                       h->d_kslot.p);
+    HIPC(hipGetLastError());
+    if (slots > 0) {
+        const size_t lds = sizeof(double) * 4 * (size_t)(K.nInt + K.nDiag);
+        if (lds > 160 * 1024) return fail(EBEM_EINVAL, "inner rules too large for LDS");
+        int64_t blocks = std::min<int64_t>((slots + 3) / 4, (int64_t)h->CUs * 8);
+        HIPC(hipFuncSetAttribute((const void*)k_kblocks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_kblocks, dim3((unsigned)blocks), dim3(256), lds, h->stream, h->d_ktri.p, h->d_tri.p,
+                           h->d_verts.p, h->d_gh.p, h->d_X.p, h->d_T.p, h->d_vadj_off.p, h->d_vadj.p, h->d_ghc_in.p,
+                           h->d_ghc_dg.p, h->nGH, K, slots, h->d_kslot.p, h->d_koff.p, h->d_klo.p, h->d_kval.p);
+        HIPC(hipGetLastError());
+    }
+    tall.stop(h->stream);
+    h->tim[12] = tall.ms();
+    HIPC(hipStreamSynchronize(h->stream));
+    h->k_pairs = 0;
+    for (int S = 0; S < N; S++) h->k_pairs += rc[N + S];
+    h->k_slots = slots;
+    h->k_LK = LK;
+    h->k_ready = true;
+    return 0;
+}
+
+// Vertex datum (3Nv x nT) of kind `kind` into h->d_gT ([s][n][c]).
+int k_datum_device(ebem_handle* h, int kind) {
+    const int64_t n = (int64_t)h->Nv * h->nT;
+    HIPC(h->d_X.grow(3 * (size_t)h->Nv));
+    HIPC(hipMemcpyAsync(h->d_X.p, h->X.data(), sizeof(double) * 3 * h->Nv, hipMemcpyHostToDevice, h->stream));
+    HIPC(h->d_gT.grow(3 * (size_t)n));
+    hipLaunchKernelGGL(k_dir_datum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->d_X.p, h->Nv,
+                       h->nT, h->dt, h->cP, h->Tfin, kind, h->d_gT.p);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+// beta_K (3N x nT) into h->d_betak from the datum in h->d_gT.
+int k_beta_device(ebem_handle* h) {
+    if (h->nT > 256 * kKBetaPer) return fail(EBEM_EINVAL, "beta_K supports at most %d time steps", 256 * kKBetaPer);
+    HIPC(h->d_betak.grow(3 * (size_t)h->N * h->nT));
+    Timer tb;
+    tb.start(h->stream);
+    hipLaunchKernelGGL(k_kbeta, dim3(h->N), dim3(256), 0, h->stream, h->d_klo.p, h->d_khi.p, h->d_koff.p,
+                       h->d_kval.p, h->d_gT.p, h->N, h->Nv, h->nT, h->d_betak.p);
+    HIPC(hipGetLastError());
+    tb.stop(h->stream);
+    h->tim[13] = tb.ms();
+    return 0;
+}
+
 extern "C" {
 
 const char* ebem_last_error(void) { return g_err.c_str(); }
@@ -1490,6 +1641,7 @@ int ebem_set_mesh(ebem_handle* h, int N, int Nv, const double* coords, const int
         if (!(h->tri[t].area > 0)) return fail(EBEM_EINVAL, "triangle %d is degenerate", t);
     h->have_mesh = true;
     h->assembled = false;
+    h->k_ready = false;
     h->have_beta = h->have_alpha = false;
     return upload_geometry(h);
 }
@@ -1509,6 +1661,7 @@ int ebem_set_problem(ebem_handle* h, double rho, double mu, double lambda, doubl
     h->dt = Tfin / nT;
     h->have_problem = true;
     h->assembled = false;
+    h->k_ready = false;
     h->have_beta = h->have_alpha = false;
     return 0;
 }
@@ -1521,6 +1674,7 @@ int ebem_set_rule(ebem_handle* h, int nGH) {
     HIPC(hipSetDevice(h->device));
     h->nGH = nGH;
     h->assembled = false;
+    h->k_ready = false;
     if (h->have_mesh) return upload_geometry(h);
     return 0;
 }
@@ -1546,6 +1700,7 @@ int ebem_set_inner_rule(ebem_handle* h, int nSub, int nSing, int nDiag) {
     h->ghc_nSing = nSing;
     h->ghc_nDiag = nDiag;
     h->assembled = false;
+    h->k_ready = false;
     return 0;
 }
 
@@ -1660,16 +1815,142 @@ int ebem_rhs(ebem_handle* h, int datum_kind, double* beta_out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
     if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
-    if (datum_kind < 0 || datum_kind > 3) return fail(EBEM_EINVAL, "unknown datum %d", datum_kind);
+    if (datum_kind < 0 || datum_kind > EBEM_DATUM_MAX) return fail(EBEM_EINVAL, "unknown datum %d", datum_kind);
     HIPC(hipSetDevice(h->device));
     const size_t n3 = 3 * (size_t)h->N;
     HIPC(h->d_beta.alloc(n3 * h->nT));
     int64_t tot = (int64_t)h->N * h->nT;
     hipLaunchKernelGGL(k_rhs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, h->N,
-                       h->nT, h->nGH, h->dt, datum_kind, h->d_beta.p);
+                       h->nT, h->nGH, h->dt, h->cP, h->Tfin, datum_kind, h->d_beta.p);
     HIPC(hipGetLastError());
     if (beta_out)
         HIPC(hipMemcpyAsync(beta_out, h->d_beta.p, sizeof(double) * n3 * h->nT, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    h->have_beta = true;
+    return 0;
+}
+
+int ebem_k_num_blocks(ebem_handle* h, int* LK) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
+    if (!LK) return fail(EBEM_EINVAL, "bad argument");
+    *LK = k_num_blocks(h);
+    return 0;
+}
+
+int ebem_assemble_k(ebem_handle* h) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    HIPC(hipSetDevice(h->device));
+    return assemble_k(h);
+}
+
+int ebem_k_store_info(ebem_handle* h, int64_t* pairs, int64_t* slots, int64_t* bytes) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->k_ready) return fail(EBEM_EINVAL, "ebem_assemble_k has not run");
+    if (pairs) *pairs = h->k_pairs;
+    if (slots) *slots = h->k_slots;
+    if (bytes) *bytes = h->k_slots * (int64_t)(9 * sizeof(double)) + (int64_t)h->N * h->Nv * (2 * 2 + 8);
+    return 0;
+}
+
+int ebem_get_k_block_rows(ebem_handle* h, int lag, int r0, int r1, double* out) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->k_ready) return fail(EBEM_EINVAL, "ebem_assemble_k has not run");
+    if (!out || r0 < 0 || r1 > h->N || r0 >= r1) return fail(EBEM_EINVAL, "bad argument");
+    if (lag < 0 || lag >= h->nT) return fail(EBEM_EINVAL, "lag %d out of range [0, %d)", lag, h->nT);
+    HIPC(hipSetDevice(h->device));
+    const int Nv = h->Nv;
+    const int64_t rows = r1 - r0, ld = 3 * rows;
+    std::memset(out, 0, sizeof(double) * ld * 3 * Nv);
+    if (lag >= h->k_LK) return 0;  // the reference's null blocks (timeMarchingD.m:267-269)
+    const int64_t p0 = (int64_t)r0 * Nv, np = rows * Nv;
+    const int64_t s0 = h->k_rowbase[r0], ns = h->k_rowbase[r1] - s0;
+    std::vector<int16_t> lo(np), hi(np);
+    std::vector<int64_t> off(np);
+    std::vector<double> v((size_t)9 * ns);
+    HIPC(hipMemcpyAsync(lo.data(), h->d_klo.p + p0, sizeof(int16_t) * np, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipMemcpyAsync(hi.data(), h->d_khi.p + p0, sizeof(int16_t) * np, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipMemcpyAsync(off.data(), h->d_koff.p + p0, sizeof(int64_t) * np, hipMemcpyDeviceToHost, h->stream));
+    if (ns > 0)
+        HIPC(hipMemcpyAsync(v.data(), h->d_kval.p + 9 * s0, sizeof(double) * 9 * ns, hipMemcpyDeviceToHost,
+                            h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    for (int64_t q = 0; q < np; q++) {
+        if (lag < lo[q] || lag >= hi[q]) continue;
+        const int64_t r = q / Nv, s = q % Nv;
+        const double* b = v.data() + 9 * (off[q] - s0 + (lag - lo[q]));
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) out[(3 * r + i) + ld * (3 * s + j)] = b[i + 3 * j];
+    }
+    return 0;
+}
+
+int ebem_dirichlet_datum(ebem_handle* h, int kind, double* g_out) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
+    if (kind < EBEM_DATUM_REF_BAR_H1 || kind > EBEM_DATUM_MAX)
+        return fail(EBEM_EINVAL, "Problema non codificato (datum %d has no Dirichlet trace)", kind);
+    if (!g_out) return fail(EBEM_EINVAL, "bad argument");
+    HIPC(hipSetDevice(h->device));
+    if (int e = k_datum_device(h, kind)) return e;
+    const int64_t n = (int64_t)h->Nv * h->nT;
+    std::vector<double> gT(3 * (size_t)n);
+    HIPC(hipMemcpyAsync(gT.data(), h->d_gT.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    for (int s = 0; s < h->Nv; s++)
+        for (int t = 0; t < h->nT; t++)
+            for (int c = 0; c < 3; c++)
+                g_out[(int64_t)t * 3 * h->Nv + 3 * s + c] = gT[((int64_t)s * h->nT + t) * 3 + c];
+    return 0;
+}
+
+int ebem_beta_k(ebem_handle* h, const double* g, double* beta_k_out) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->k_ready) return fail(EBEM_EINVAL, "ebem_assemble_k has not run");
+    if (!g || !beta_k_out) return fail(EBEM_EINVAL, "bad argument");
+    HIPC(hipSetDevice(h->device));
+    const int64_t n = (int64_t)h->Nv * h->nT;
+    DevBuf<double> d_g;
+    HIPC(d_g.alloc(3 * (size_t)n));
+    HIPC(hipMemcpyAsync(d_g.p, g, sizeof(double) * 3 * n, hipMemcpyHostToDevice, h->stream));
+    HIPC(h->d_gT.grow(3 * (size_t)n));
+    hipLaunchKernelGGL(k_dir_transpose, dim3((unsigned)((3 * n + 255) / 256)), dim3(256), 0, h->stream, d_g.p, h->Nv,
+                       h->nT, h->d_gT.p);
+    HIPC(hipGetLastError());
+    if (int e = k_beta_device(h)) return e;
+    HIPC(hipMemcpyAsync(beta_k_out, h->d_betak.p, sizeof(double) * 3 * (size_t)h->N * h->nT, hipMemcpyDeviceToHost,
+                        h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int ebem_rhs_dirichlet(ebem_handle* h, int kind, double* beta_out) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->k_ready) return fail(EBEM_EINVAL, "ebem_assemble_k has not run");
+    if (kind < EBEM_DATUM_REF_BAR_H1 || kind > EBEM_DATUM_MAX)
+        return fail(EBEM_EINVAL, "Problema non codificato (datum %d has no Dirichlet trace)", kind);
+    HIPC(hipSetDevice(h->device));
+    const size_t n3 = 3 * (size_t)h->N;
+    HIPC(h->d_beta.alloc(n3 * h->nT));
+    const int64_t tot = (int64_t)h->N * h->nT;
+    hipLaunchKernelGGL(k_rhs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, h->N,
+                       h->nT, h->nGH, h->dt, h->cP, h->Tfin, kind, h->d_beta.p);
+    HIPC(hipGetLastError());
+    if (int e = k_datum_device(h, kind)) return e;
+    if (int e = k_beta_device(h)) return e;
+    const int64_t nb = (int64_t)n3 * h->nT;
+    hipLaunchKernelGGL(k_dd_combine, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, h->stream, h->d_beta.p,
+                       h->d_betak.p, nb);
+    HIPC(hipGetLastError());
+    if (beta_out)
+        HIPC(hipMemcpyAsync(beta_out, h->d_beta.p, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
     h->have_beta = true;
     return 0;

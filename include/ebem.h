@@ -48,6 +48,13 @@ extern "C" {
 #define EBEM_DATUM_SPHERE_T2EXP 1   /* n(x) t^2 exp(-4t) H(t), n = triangle normal */
 #define EBEM_DATUM_REF_SPHERE 2     /* reference 'sphereUniform': (0,0,0.5 t) */
 #define EBEM_DATUM_REF_SCREEN 3     /* reference 'screenUniform': (0,0,[sin^5 t | 1] x1^2) */
+/* Data of the direct Dirichlet problems; also have a vertex trace for
+ * ebem_dirichlet_datum / ebem_rhs_dirichlet (dir/BEMenerg_dir_calcBoundDataDirichlet.m). */
+#define EBEM_DATUM_REF_BAR_H1 4     /* 'barH1': (0,0,u(z,t)), the 1-D bar solution of height 1 */
+#define EBEM_DATUM_REF_BAR_H3 5     /* 'barH3': the same for height 3 */
+#define EBEM_DATUM_REF_SPHERE_WAVE 6 /* 'sphereWave': (-+exp(-20 (x1 - 2 + cP t - 0.475)^2), 0, 0) */
+#define EBEM_DATUM_REF_ELEMENTO 7   /* 'elementoIndustriale': (0,0,-exp(-20 (x3 + 1 - cP t)^2)); vertex trace 0 */
+#define EBEM_DATUM_MAX 7
 
 typedef struct ebem_handle ebem_handle;
 typedef struct ebem_sur ebem_sur;
@@ -141,6 +148,41 @@ int ebem_get_block_rows(ebem_handle *h, int lag, int r0, int r1, double *out);
  * non-NULL, copied to the host (Tn/BEMenerg_core_calcTnBlock.m:27-58). */
 int ebem_rhs(ebem_handle *h, int datum, double *beta_out);
 
+/* ---- double-layer operator K of the direct Dirichlet driver ----------------
+ * K_lag (3N x 3Nv) couples test triangle S with the P1 hat function of
+ * vertex s.  Needs ebem_set_inner_rule (the FN method's nSub, nSing, nDiag),
+ * as the reference's kernelK.cu does. */
+
+/* numBlocksK = min(L + 1, nT) (dir/BEMenerg_dir_timeMarchingD.m:176). */
+int ebem_k_num_blocks(ebem_handle *h, int *LK);
+
+/* K_0 .. K_{LK-1} on the device: regular blocks as dir/CUDAkernels/kernelK.cu
+ * (isBlockNull skip, 1e-14 clean-up, /(4 pi dt)), blocks of a triangle
+ * against its own vertices plus dir/BEMenerg_dir_calcSubBlockSingK.m +
+ * calcKIntgSing.m (timeMarchingD.m:173-274, BEMenerg_dir_launchKernelK.m). */
+int ebem_assemble_k(ebem_handle *h);
+
+/* Store of the last ebem_assemble_k: (test triangle, vertex) pairs with a
+ * nonempty lag window, (pair, lag) slots, device bytes. */
+int ebem_k_store_info(ebem_handle *h, int64_t *pairs, int64_t *slots, int64_t *bytes);
+
+/* Test rows [r0, r1) of K_lag: dense 3(r1-r0) x 3Nv column-major (the rows of
+ * matrixSavedK{lag+1}, timeMarchingD.m:244-258; lags >= LK are zero). */
+int ebem_get_k_block_rows(ebem_handle *h, int lag, int r0, int r1, double *out);
+
+/* Dirichlet datum at the vertices, 3Nv x nT column-major, column n at time
+ * (n+1) dt (dir/BEMenerg_dir_calcBoundDataDirichlet.m); datum 4..7. */
+int ebem_dirichlet_datum(ebem_handle *h, int datum, double *g_out);
+
+/* beta_K (3N x nT column-major): beta_n = sum_{j<=n} K_{n-j} g_j for a vertex
+ * datum g (3Nv x nT column-major) (dir/BEMenerg_dir_calcBetaKgpu.m:7-13). */
+int ebem_beta_k(ebem_handle *h, const double *g, double *beta_k_out);
+
+/* Right-hand side of the direct Dirichlet driver on the device,
+ * beta = rhs(datum) / 2 + beta_K(g(datum)) (timeMarchingD.m:163-166, 272, 287),
+ * for ebem_mot_solve(h, NULL, ...); copied to beta_out if non-NULL. */
+int ebem_rhs_dirichlet(ebem_handle *h, int datum, double *beta_out);
+
 /* MOT: factorise E_0 in FP64 on the GPU and march
  * alpha_n = E_0^{-1}(beta_n - sum_{l=1}^{min(n,L)-1} E_l alpha_{n-l})
  * (SA/BEMenerg_coreSA_timeMarching.m:45-113, FN/BEMenerg_coreFN_timeMarching.m:152-171).
@@ -179,8 +221,9 @@ int ebem_node_integral(ebem_handle *h, const double *x, int tri, double t, doubl
  * total, out[5] MOT total, out[6] post-processing total, out[7] history-sum
  * launches, out[8] assembly kP-range kernel, out[9] post-processing kP-range
  * kernel, out[10] assembly between-front kernel, out[11] post-processing
- * between-front kernel. */
-#define EBEM_NUM_TIMINGS 12
+ * between-front kernel, out[12] ebem_assemble_k total, out[13] beta_K kernel
+ * of the last ebem_beta_k / ebem_rhs_dirichlet. */
+#define EBEM_NUM_TIMINGS 14
 int ebem_timings(ebem_handle *h, double *out, int n);
 
 /* ------------------------------------------------------------------------

@@ -1,0 +1,108 @@
+// Test-only host build of the device K math (ebem_k.hpp: k_reg_lane and
+// k_sing_lane are __host__ __device__): one block (S, s) of K_lag computed by
+// the same code as the device kernel, the 64 lanes' parts summed by the same
+// xor butterfly, so the CPU test-suite can check it against the oracle.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../elastodynamicebem_amd/csrc/ebem_k_math.hpp"
+
+using namespace ebem;
+
+namespace {
+void butterfly(double a[64][9]) {
+    for (int m = 32; m >= 1; m >>= 1) {
+        double b[64][9];
+        for (int l = 0; l < 64; l++)
+            for (int e = 0; e < 9; e++) b[l][e] = a[l][e] + a[l ^ m][e];
+        std::memcpy(a, b, sizeof b);
+    }
+}
+}  // namespace
+
+extern "C" {
+
+// Block (S, s) of K_lag, column-major, for a mesh (X: Nv x 3, T: N x 3,
+// normals, areas, longest edges), material, outer rule (barycentric ghn, ghw)
+// and inner rules (rin: nInt x 4, rdg: nDiag x 4).
+int k_block_host(int N, int Nv, const double* X, const int32_t* T, const double* nrm, const double* area,
+                 const double* maxL, double rho, double mu, double lam, double Tfin, int nT, int nGH,
+                 const double* ghn, const double* ghw, int nInt, const double* rin, int nDiag, const double* rdg,
+                 int lag, int S, int s, double* out9) {
+#pragma clang fp contract(off)
+    std::vector<KTri> kt(N);
+    std::vector<double> verts((size_t)N * 9);
+    std::vector<int32_t> voff(Nv + 1, 0), vadj(3 * (size_t)N);
+    double maxLen = 0;
+    for (int F = 0; F < N; F++) {
+        const double* V[3] = {X + 3 * T[3 * F], X + 3 * T[3 * F + 1], X + 3 * T[3 * F + 2]};
+        k_tri_basis(V[0], V[1], V[2], nrm + 3 * F, area[F], kt[F]);
+        for (int v = 0; v < 3; v++) {
+            for (int c = 0; c < 3; c++) verts[9 * (size_t)F + 3 * v + c] = V[v][c];
+            voff[T[3 * F + v] + 1]++;
+        }
+        if (maxL[F] > maxLen) maxLen = maxL[F];
+    }
+    for (int v = 0; v < Nv; v++) voff[v + 1] += voff[v];
+    std::vector<int32_t> pos(voff.begin(), voff.end() - 1);
+    for (int F = 0; F < N; F++)
+        for (int v = 0; v < 3; v++) vadj[pos[T[3 * F + v]]++] = F * 4 + v;
+    double g[128] = {0};
+    const double* VS[3] = {X + 3 * T[3 * S], X + 3 * T[3 * S + 1], X + 3 * T[3 * S + 2]};
+    for (int q = 0; q < nGH; q++) {
+        for (int k = 0; k < 3; k++)
+            g[32 * k + q] = ghn[3 * q] * VS[0][k] + ghn[3 * q + 1] * VS[1][k] + ghn[3 * q + 2] * VS[2][k];
+        g[96 + q] = area[S] * ghw[q];
+    }
+    KParams K{};
+    K.N = N;
+    K.Nv = Nv;
+    K.LK = nT;
+    K.nInt = nInt;
+    K.nDiag = nDiag;
+    K.dt = Tfin / nT;
+    K.cP = std::sqrt((lam + 2 * mu) / rho);
+    K.cS = std::sqrt(mu / rho);
+    K.lam = lam;
+    K.mu = mu;
+    K.rho = rho;
+    K.maxLen = maxLen;
+    K.fourpidt = 4 * kPi * K.dt;
+    KBlockArgs A{kt.data(), verts.data(), voff.data(), vadj.data(), rin, rdg, g, nGH};
+    double out[9] = {0};
+    static double lanes[64][9];
+    if (!k_block_null(verts.data() + 9 * (size_t)S, X + 3 * s, lag, K)) {
+        for (int l = 0; l < 64; l++) {
+            double acc[3][3];
+            k_reg_lane(A, K, S, s, lag, l, 64, acc);
+            for (int i = 0; i < 3; i++)
+                for (int c = 0; c < 3; c++) lanes[l][i + 3 * c] = acc[i][c];
+        }
+        butterfly(lanes);
+        for (int e = 0; e < 9; e++) out[e] = std::fabs(lanes[0][e]) > 1.0e-14 ? lanes[0][e] / K.fourpidt : 0.0;
+    }
+    const int vl = k_local_vertex(T, S, s);
+    if (vl >= 0 && lag < k_sing_cut(maxL[S], K)) {
+        const int ct[4] = {-2, -1, 0, 1};
+        const double cn[4] = {-1, 3, -3, 1};
+        double blk[9] = {0};
+        for (int l = 0; l < 4; l++) {
+            const int il = lag + ct[l];
+            if (il <= 0) continue;
+            for (int ln = 0; ln < 64; ln++) {
+                double lev[3][3];
+                k_sing_lane(A, K, S, vl, (double)il * K.dt, ln, 64, lev);
+                for (int i = 0; i < 3; i++)
+                    for (int c = 0; c < 3; c++) lanes[ln][i + 3 * c] = lev[i][c];
+            }
+            butterfly(lanes);
+            for (int e = 0; e < 9; e++) blk[e] += cn[l] * lanes[0][e];
+        }
+        for (int e = 0; e < 9; e++) out[e] += blk[e] / K.fourpidt;
+    }
+    std::memcpy(out9, out, sizeof out);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,75 @@
+"""The device formulation of the double-layer operator K
+(elastodynamicebem_amd/csrc/ebem_k_math.hpp, compiled for the host by
+tests/native/Makefile: the same k_reg_lane / k_sing_lane as the kernel, the 64
+lanes summed by the same butterfly) against the oracle, on the CPU, before
+any GPU run.  FP64 bar 1e-10 on the scale of the pair's largest block (the
+singular part cancels large terms; late blocks are round-off around zero)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "native", "libk_host.so")
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+@pytest.fixture(scope="module")
+def host_k():
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "native")], check=True)
+    lib = C.CDLL(LIB)
+    lib.k_block_host.argtypes = ([C.c_int, C.c_int, _dp, _ip, _dp, _dp, _dp] + [C.c_double] * 4 +
+                                 [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int,
+                                  _dp])
+    lib.k_block_host.restype = C.c_int
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _host_block(lib, o, rules, lam, lag, S, s):
+    rin, rdg = rules
+    out = np.zeros(9)
+    lib.k_block_host(o.N, o.coords.shape[0], _p(o.coords), o.tri.ctypes.data_as(_ip), _p(o.normal), _p(o.area),
+                     _p(o.maxL), 1.0, 1.0, lam, o.Tfin, o.nT, o.nGH, _p(o.ghn), _p(o.ghw), rin.shape[0], _p(rin),
+                     rdg.shape[0], _p(rdg), lag, S, s, _p(out))
+    return out.reshape(3, 3).T.copy()
+
+
+@pytest.mark.parametrize("meshname,lam", [("sphere2", 2.0), ("screen2", 0.5)])
+def test_host_k_blocks_match_oracle(host_k, request, meshname, lam):
+    from elastodynamicebem_amd import api
+    from oracle import Oracle
+    mesh = request.getfixturevalue(meshname)
+    o = Oracle(mesh.coordinates, mesh.triangles, rho=1.0, mu=1.0, lam=lam, Tfin=4.0, nT=32)
+    o.set_ghc(16, 3, 64)
+    gn, gw, dn, dw = api.inner_rule_tables(16, 3, 64)
+    rules = (np.ascontiguousarray(np.column_stack([gn, gw])), np.ascontiguousarray(np.column_stack([dn, dw])))
+    assert np.allclose(gn, o.ghcn) and np.allclose(gw, o.ghcw) and np.allclose(dn, o.dgn) and np.allclose(dw, o.dgw)
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    checked = 0
+    for S in rng.choice(o.N, 5, replace=False):
+        S = int(S)
+        verts = list(o.tri[S])
+        nb = sorted({int(v) for F in range(o.N) if len(set(o.tri[F]) & set(verts)) > 0 for v in o.tri[F]})
+        cand = verts + [v for v in nb if v not in verts][:2] + [int(rng.integers(o.coords.shape[0]))]
+        for s in cand:
+            blocks = [(l, o.k_block(l, S, s)) for l in range(0, 14)]
+            peak = max(np.linalg.norm(b) for _, b in blocks)
+            for l, R in blocks:
+                G = _host_block(host_k, o, rules, lam, l, S, s)
+                if peak == 0:
+                    assert np.abs(G).max() == 0
+                    continue
+                e = np.linalg.norm(G - R) / peak
+                worst = max(worst, e)
+                assert e < 1e-10, (S, s, l, e)
+                checked += 1
+    assert checked > 100
+    print(f"worst host-vs-oracle K block error {worst:.2e} over {checked} blocks")
