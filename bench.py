@@ -44,6 +44,12 @@ ALG_FLOP_PER_ITEM = 8103.8
 K_LEVELS_TRAFFIC_BYTES = 117.5e9
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# Algorithmic FP64 operations of the double-layer block kernel k_kblocks, from
+# the op-counting host build of its math (tests/native/k_opcount.cpp): per
+# regular evaluation point (one outer node, one inner node of one field
+# triangle) 218 plus 385 per time level with t > 0; per level of the singular
+# sum 585 per evaluation point (three children x nDiag nodes per outer node).
+K_REG_POINT_BASE, K_REG_POINT_LEVEL, K_SING_POINT_LEVEL = 218, 385, 585
 
 
 def workload():
@@ -168,6 +174,116 @@ def chosen_forms(s, m, pb, xpts, tpts, train_samples=2_000_000, epochs=20):
     return out
 
 
+def dd_workload(level=2):
+    """The reference's barH1 problems (inputFiles/input_barH1_*.txt): rho 0.25,
+    mu 0.5, lambda 0, t in [0, 6]; level 2 is 'large' (768 triangles, 96 steps)."""
+    from elastodynamicebem_amd import api
+    from elastodynamicebem_amd.mesh import read_mesh
+    m = read_mesh(os.path.join(ROOT, "tests", "golden", "meshes", f"barH1_{level}.mesh"))
+    nT = {1: 48, 2: 96}[level]
+    pb = api.PbParam(rho=0.25, mu=0.5, lambda_=0.0, Tfin=6.0, nT=nT, domainType="barH1", lev=level)
+    return m, pb
+
+
+def k_windows_and_flops(m, pb, LK, nGH, nInt, nDiag, rows=None):
+    """Slots of the K store (pair windows of isBlockNull and the singular
+    cut-off, kernelK.cu:226-260, calcSubBlockSingK.m:12) and the algorithmic
+    FP64 operations of k_kblocks over them; rows = (r0, r1) restricts the
+    test triangles."""
+    X, T0 = m.coordinates, m.triangles
+    r0, r1 = (0, T0.shape[0]) if rows is None else rows
+    T = T0[r0:r1]
+    N, Nv = T.shape[0], X.shape[0]
+    dt, cP, cS = pb.Tfin / pb.nT, pb.velP, pb.velS
+    maxLen = float(m.maxL.max())
+    d = np.sqrt(((X[T][:, :, None, :] - X[None, None, :, :]) ** 2).sum(-1))  # N x 3 x Nv
+    dmax, dmin = d.max(1), d.min(1)
+    own = np.zeros((N, Nv), bool)
+    own[np.arange(N)[:, None], T] = True
+    deg = np.bincount(T0.ravel(), minlength=Nv)
+    nF = deg[None, :] - own  # field triangles F != S with vertex s
+    cut = np.ceil(m.maxL[r0:r1] / (cS * dt)).astype(int) + 2
+    slots = 0
+    flops = 0.0
+    for lag in range(LK):
+        reg = ~(((lag - 2) * cS * dt > dmax + maxLen) | ((lag + 1) * cP * dt < dmin - 2 * maxLen))
+        sing = own & (lag < cut[:, None])
+        slots += int(np.count_nonzero(reg | sing))
+        nlev = sum(1 for c in (-2, -1, 0, 1) if lag + c >= 1)
+        flops += float((nF * reg).sum()) * nGH * nInt * (K_REG_POINT_BASE + K_REG_POINT_LEVEL * nlev)
+        flops += float(sing.sum()) * nGH * 3 * nDiag * K_SING_POINT_LEVEL * nlev
+    return slots, flops
+
+
+def dd_main(args):
+    """Direct Dirichlet step on the level-2 bar with the reference's default
+    method FN 19 64 3 256: V (numeric GHC form), K_0..K_{LK-1}, rhs / 2 + beta_K
+    and the MOT, inputs resident on the device; value = K blocks / K assembly time."""
+    from elastodynamicebem_amd import api
+    m, pb = dd_workload(2)
+    nGH, nSub, nSing, nDiag = 19, 64, 3, 256
+    s = api.Solver(m, pb, nGH, device=0)
+    s.set_inner_rule(nSub, nSing, nDiag)
+    LK = s.k_num_blocks()
+
+    def step():
+        s.assemble(api.PREC_GHC)
+        s.assemble_k()
+        s.rhs_dirichlet(api.DATUM_REF_BAR_H1)
+        return s.mot_solve(None)
+
+    for w in range(args.warmup):
+        step()
+        print(f"bench dd: warmup step {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    times, tims = [], []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - t0)
+        tims.append(s.timings())
+        print(f"bench dd: timed step {k + 1}/{args.steps}: {1e3 * times[-1]:.1f} ms", file=sys.stderr, flush=True)
+    info = s.k_store_info()
+    s.close()
+    slots, flops = k_windows_and_flops(m, pb, LK, nGH, nSub * nSing, nDiag)
+    assert slots == info["slots"], (slots, info["slots"])
+    k_ms = float(np.mean([t["k_assembly_ms"] for t in tims]))
+    ach = flops / (k_ms * 1e-3) / 1e12
+    out = {
+        "metric": "double-layer K blocks (test triangle, vertex, lag) per second, direct Dirichlet driver",
+        "value": slots / (k_ms * 1e-3), "unit": "K blocks/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * float(np.mean(times)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "barH1 level 2 (768 triangles, 386 vertices), rho 0.25, mu 0.5, lambda 0, "
+                               "t in [0, 6], 96 steps, FN 19 64 3 256: V (GHC) + K + rhs/2 + beta_K + MOT",
+                   "k_lags": LK, "k_slots": slots, "k_pairs": info["pairs"], "k_store_bytes": info["bytes"]},
+        "breakdown_ms": {"v_assembly": float(np.mean([t["assembly_ms"] for t in tims])), "k_assembly": k_ms,
+                         "beta_k": float(np.mean([t["beta_k_ms"] for t in tims])),
+                         "mot_total": float(np.mean([t["mot_ms"] for t in tims]))},
+        "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_kblocks", "achieved": ach,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
+                     "traffic": None},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        from oracle import Oracle
+        threads = min(os.cpu_count() or 1, 16)
+        o = Oracle(m.coordinates, m.triangles, rho=pb.rho, mu=pb.mu, lam=pb.lambda_, Tfin=pb.Tfin, nT=pb.nT,
+                   nGH=nGH)
+        o.set_ghc(nSub, nSing, nDiag)
+        rows = (0, 1)
+        t0 = time.perf_counter()
+        for lag in range(LK):
+            o.k_matrix(lag, rows, nthreads=threads)
+        el = time.perf_counter() - t0
+        sub_slots, _ = k_windows_and_flops(m, pb, LK, nGH, nSub * nSing, nDiag, rows)
+        out["cpu_baseline"] = {"value": sub_slots / el, "unit": "K blocks/s", "cores": threads, "kind": "port",
+                               "sample": f"oracle (C/OpenMP FP64) K blocks of test row {rows[0]} for all "
+                                         f"{LK} lags ({sub_slots} blocks, {el:.1f} s)"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,10 +294,16 @@ def main():
                     help="skip the mixed-precision and surrogate forms (their timing and error vs FP64)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy kernel of MIB MiB (traffic-counter calibration)")
+    ap.add_argument("--workload", choices=("step", "dd"), default="step",
+                    help="step: the single-layer hot path (default); dd: the direct Dirichlet driver with K")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "dd":
+        import torch  # noqa: F401  (HIP runtime present)
+        dd_main(args)
+        return
 
     import torch  # noqa: F401  (HIP runtime present)
     from elastodynamicebem_amd import api

@@ -28,7 +28,7 @@
 #include "mex.h"
 
 static ebem_handle *g_h = NULL;
-static mwSize g_n3 = 0, g_nT = 0; /* 3N of the mesh, nT of the problem */
+static mwSize g_n3 = 0, g_nT = 0, g_nv = 0; /* 3N and Nv of the mesh, nT of the problem */
 
 static void chk(int rc) {
     if (rc != 0) mexErrMsgIdAndTxt("ebem:lib", "%s (code %d)", ebem_last_error(), rc);
@@ -70,6 +70,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
         mxFree(tr);
         chk(rc);
         g_n3 = 3 * N;
+        g_nv = Nv;
     } else if (!strcmp(cmd, "problem")) {
         chk(ebem_set_problem(g_h, scal(prhs[1]), scal(prhs[2]), scal(prhs[3]), scal(prhs[4]), (int)scal(prhs[5])));
         g_nT = (mwSize)scal(prhs[5]);
@@ -91,6 +92,14 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     } else if (!strcmp(cmd, "rhs")) {
         plhs[0] = mxCreateDoubleMatrix(g_n3, g_nT, mxREAL);
         chk(ebem_rhs(g_h, (int)scal(prhs[1]), mxGetDoubles(plhs[0])));
+    } else if (!strcmp(cmd, "assemble_k")) {
+        chk(ebem_assemble_k(g_h));
+    } else if (!strcmp(cmd, "kblock")) {
+        plhs[0] = mxCreateDoubleMatrix(g_n3, 3 * g_nv, mxREAL);
+        chk(ebem_get_k_block_rows(g_h, (int)scal(prhs[1]), 0, (int)(g_n3 / 3), mxGetDoubles(plhs[0])));
+    } else if (!strcmp(cmd, "rhs_dirichlet")) {
+        plhs[0] = mxCreateDoubleMatrix(g_n3, g_nT, mxREAL);
+        chk(ebem_rhs_dirichlet(g_h, (int)scal(prhs[1]), mxGetDoubles(plhs[0])));
     } else if (!strcmp(cmd, "mot")) {
         plhs[0] = mxCreateDoubleMatrix(g_n3, g_nT, mxREAL);
         const double *beta = mxIsEmpty(prhs[1]) ? NULL : mxGetDoubles(prhs[1]);
@@ -112,7 +121,8 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
         chk(rc);
     } else if (strcmp(cmd, "mesh") && strcmp(cmd, "problem") && strcmp(cmd, "rule") && strcmp(cmd, "inner_rule") &&
                strcmp(cmd, "pp_rule") && strcmp(cmd, "numblocks") && strcmp(cmd, "assemble") && strcmp(cmd, "block") &&
-               strcmp(cmd, "rhs") && strcmp(cmd, "mot")) {
+               strcmp(cmd, "rhs") && strcmp(cmd, "mot") && strcmp(cmd, "assemble_k") && strcmp(cmd, "kblock") &&
+               strcmp(cmd, "rhs_dirichlet")) {
         mexErrMsgIdAndTxt("ebem:args", "unknown command '%s'", cmd);
     }
     (void)nlhs;
