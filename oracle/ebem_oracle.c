@@ -1745,3 +1745,137 @@ int orc_k_matrix_rows(const orc_param *p, const orc_mesh *m, int nGH, const doub
     }
     return 0;
 }
+
+/* ======================================================================
+ * Direct post-processing u(x, t) = sum_y G_y alpha_y - P_y g_y
+ * (dirPP/BEMenerg_ppD_calc.m:1-124 with CUDAkernels/kernelG.cu and kernelP.cu).
+ * ====================================================================== */
+
+/* kernelP.cu nucleo (:169-246): the time-integrated traction kernel at
+ * offset x = source - node, r = |x|, normal n, time t (before the basis
+ * function), 3x3 in nu[i][j]. */
+static void ppP_nucleo(const double *x, const double *n, double r, double t, const orc_param *p, double nu[3][3]) {
+    const double lam = p->lambda, mu = p->mu, cP = p->velP, cS = p->velS;
+    const double HP = (t - (r / cP)) > 0, HS = (t - (r / cS)) > 0;
+    const double dot = x[0] * n[0] + x[1] * n[1] + x[2] * n[2];
+    const double r2 = pow(r, 2), r3 = pow(r, 3), r4 = pow(r, 4), r5 = pow(r, 5), r7 = pow(r, 7);
+    const double Hc = (t - (r / cP)) * (t - (r / cP)) * (t + (2 * r / cP)) / 6 * HP -
+                      (t - (r / cS)) * (t - (r / cS)) * (t + (2 * r / cS)) / 6 * HS;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            const int d = i == j;
+            const double A = ((lam * x[i] * n[j] / r2) + (2 * mu * x[j] * x[i] / r4 * dot)) * (HP / pow(cP, 3));
+            const double b2 = d ? ((1 / r2) - (2 * x[j] * x[i] / r4)) : (-(2 * x[j] * x[i] / r4));
+            const double B = mu * ((x[j] * n[i] / r2) + (b2 * dot)) * (HS / pow(cS, 3));
+            const double c2 = d ? ((1 / r3) - (6 * x[j] * x[i] / r5)) : (-(6 * x[j] * x[i] / r5));
+            const double C = (((lam - 2 * mu) * x[i] * n[j] / r3) - (2 * mu * x[j] * n[i] / r3) - (2 * mu * c2 * dot)) *
+                             ((t - (r / cP)) * HP / pow(cP, 2));
+            const double d2 = d ? ((1 / r3) - (4 * x[j] * x[i] / r5)) : (-(4 * x[j] * x[i] / r5));
+            const double D = mu * ((2 * x[i] * n[j] / r3) + (3 * x[j] * n[i] / r3) + (3 * d2 * dot)) *
+                             ((t - (r / cS)) * HS / pow(cS, 2));
+            const double e2 = d ? ((1 / r5) - (5 * x[j] * x[i] / r7)) : (-(5 * x[j] * x[i] / r7));
+            const double E = 6 * mu * ((x[i] * n[j] / r5) + (x[j] * n[i] / r5) + (e2 * dot)) * Hc;
+            nu[i][j] = A + B + C + D - E;
+        }
+}
+
+/* P_y block (3x3 column-major) of vertex s for the point x at time t:
+ * levels t - (y + k) dt, k = 0, 1, 2, weights 1, -2, 1 (non-positive skipped),
+ * every triangle with vertex s, the composite rule (rn barycentric, rw
+ * weights) scaled by the triangle's area, kernel times the hat function of s;
+ * entries cleaned below 1e-14 and divided by 4 pi rho dt (kernelP.cu:7-147). */
+void orc_ppP_block(const orc_param *p, const orc_mesh *m, int nInt, const double *rn, const double *rw,
+                   const double *x, double t, int y, int s, double *out) {
+    const double dt = p->Tfin / p->nT;
+    double acc[3][3] = {{0}};
+    const double cn[3] = {1, -2, 1};
+    for (int k = 0; k < 3; k++) {
+        const double tau = t - (double)(y + k) * dt;
+        if (tau <= 0) continue;
+        for (int F = 0; F < m->N; F++) {
+            int v = -1;
+            for (int q = 0; q < 3; q++)
+                if (m->tri[3 * F + q] == s) v = q;
+            if (v < 0) continue;
+            double T[9], mat[9], vec[3];
+            tri_verts(m, F, T);
+            orc_k_basis(T, mat, vec);
+            const double *n = m->normal + 3 * F;
+            for (int q = 0; q < nInt; q++) {
+                const double *b = rn + 3 * q;
+                double yv[3], xv[3];
+                for (int e = 0; e < 3; e++) yv[e] = b[0] * T[e] + b[1] * T[3 + e] + b[2] * T[6 + e];
+                const double w = rw[q] * m->area[F];
+                for (int e = 0; e < 3; e++) xv[e] = x[e] - yv[e];
+                const double r = sqrt(xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2]);
+                double nu[3][3];
+                ppP_nucleo(xv, n, r, tau, p, nu);
+                const double phi = vec[v] + mat[3 * v] * yv[0] + mat[3 * v + 1] * yv[1] + mat[3 * v + 2] * yv[2];
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) acc[i][j] += cn[k] * w * (nu[i][j] * phi);
+            }
+        }
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            out[i + 3 * j] = fabs(acc[i][j]) > 1.0e-14 ? acc[i][j] / (4 * PI * p->rho * dt) : 0.0;
+}
+
+/* G_y block (3x3 column-major) of triangle F: levels t - (y + k) dt, k = 0, 1,
+ * weights 1, -1, the single-layer kernel, cleaned below 1e-14 and divided by
+ * 4 pi rho (kernelG.cu:4-116). */
+void orc_ppG_block(const orc_param *p, const orc_mesh *m, int nInt, const double *rn, const double *rw,
+                   const double *x, double t, int y, int F, double *out) {
+    const double dt = p->Tfin / p->nT;
+    double acc[9] = {0};
+    const double cn[2] = {1, -1};
+    double T[9];
+    tri_verts(m, F, T);
+    for (int k = 0; k < 2; k++) {
+        const double tau = t - (double)(y + k) * dt;
+        if (tau <= 0) continue;
+        for (int q = 0; q < nInt; q++) {
+            const double *b = rn + 3 * q;
+            double d[3];
+            for (int e = 0; e < 3; e++) d[e] = (b[0] * T[e] + b[1] * T[3 + e] + b[2] * T[6 + e]) - x[e];
+            const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            double nu[9];
+            orc_nucleo(d, r, tau, p->velP, p->velS, nu);
+            const double w = rw[q] * m->area[F];
+            for (int e = 0; e < 9; e++) acc[e] += cn[k] * w * nu[e];
+        }
+    }
+    for (int e = 0; e < 9; e++) out[e] = fabs(acc[e]) > 1.0e-14 ? acc[e] / (4 * PI * p->rho) : 0.0;
+}
+
+/* u(x, t) = sum_{y < nHat} G_y alpha_y - P_y g_y, nHat = ceil(t / dt); density
+ * 3N x nT and g 3Nv x nT column-major (column y at time (y + 1) dt); u = 0 for
+ * t <= 0 (BEMenerg_ppD_calc.m:9-13, 113-123).  OpenMP over the blocks. */
+void orc_ppD_point(const orc_param *p, const orc_mesh *m, const double *density, const double *g, const double *x,
+                   double t, int nInt, const double *rn, const double *rw, double *u, int nthreads) {
+    u[0] = u[1] = u[2] = 0;
+    if (t <= 0) return;
+    const double dt = p->Tfin / p->nT;
+    const int nHat = (int)ceil(t / dt);
+    const long n3 = 3L * m->N, nv3 = 3L * m->Nv;
+    const long nG = (long)m->N * nHat, nP = (long)m->Nv * nHat;
+    double *part = (double *)calloc(3 * (size_t)(nG + nP), sizeof(double));
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads > 0 ? nthreads : 1)
+    for (long q = 0; q < nG + nP; q++) {
+        double B[9];
+        const double *v;
+        if (q < nG) {
+            const int y = (int)(q / m->N), F = (int)(q % m->N);
+            orc_ppG_block(p, m, nInt, rn, rw, x, t, y, F, B);
+            v = density + n3 * y + 3 * F;
+        } else {
+            const int y = (int)((q - nG) / m->Nv), s = (int)((q - nG) % m->Nv);
+            orc_ppP_block(p, m, nInt, rn, rw, x, t, y, s, B);
+            v = g + nv3 * y + 3 * s;
+        }
+        for (int r = 0; r < 3; r++) part[3 * q + r] = B[r] * v[0] + B[r + 3] * v[1] + B[r + 6] * v[2];
+    }
+    for (long q = 0; q < nG + nP; q++)
+        for (int r = 0; r < 3; r++) u[r] += (q < nG ? 1.0 : -1.0) * part[3 * q + r];
+    free(part);
+}

@@ -153,6 +153,14 @@ int orc_k_matrix_rows(const orc_param *p, const orc_mesh *m, int nGH, const doub
                       const double *ghcn, const double *ghcw, int nDiag, const double *dgn, const double *dgw,
                       int lag, int r0, int r1, double *out, int nthreads);
 
+/* ---- direct post-processing u = G alpha - P g (dirPP/BEMenerg_ppD_calc.m) ---- */
+void orc_ppP_block(const orc_param *p, const orc_mesh *m, int nInt, const double *rn, const double *rw,
+                   const double *x, double t, int y, int s, double *out);
+void orc_ppG_block(const orc_param *p, const orc_mesh *m, int nInt, const double *rn, const double *rw,
+                   const double *x, double t, int y, int F, double *out);
+void orc_ppD_point(const orc_param *p, const orc_mesh *m, const double *density, const double *g, const double *x,
+                   double t, int nInt, const double *rn, const double *rw, double *u, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

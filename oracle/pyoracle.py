@@ -106,6 +106,12 @@ class Oracle:
                                                 _dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp,
                                                 C.c_int]
         L.orc_k_basis.argtypes = [_dp, _dp, _dp]
+        L.orc_ppP_block.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, _dp, _dp, C.c_double,
+                                    C.c_int, C.c_int, _dp]
+        L.orc_ppG_block.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, _dp, _dp, C.c_double,
+                                    C.c_int, C.c_int, _dp]
+        L.orc_ppD_point.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), _dp, _dp, _dp, C.c_double, C.c_int, _dp,
+                                    _dp, _dp, C.c_int]
         L.orc_k_block.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, _dp, C.c_int, _dp, _dp,
                                   C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.orc_k_block.restype = C.c_int
@@ -277,6 +283,34 @@ class Oracle:
                 if n - j < LK:
                     out[:, n] += K[n - j] @ g[:, j]
         return out
+
+    # -- direct post-processing u = G alpha - P g (dirPP/BEMenerg_ppD_calc.m) --
+    def pp_p_block(self, x, t, y, s):
+        """P_y block of vertex s (kernelP.cu), 3x3 (needs set_pp_rule)."""
+        out = np.zeros(9)
+        x = np.ascontiguousarray(x, np.float64)
+        self.lib.orc_ppP_block(C.byref(self.param), C.byref(self.mesh), self.ppw.size, _p(self.ppn), _p(self.ppw),
+                               _p(x), float(t), int(y), int(s), _p(out))
+        return out.reshape(3, 3).T.copy()
+
+    def pp_g_block(self, x, t, y, F):
+        """G_y block of triangle F (kernelG.cu), 3x3 (needs set_pp_rule)."""
+        out = np.zeros(9)
+        x = np.ascontiguousarray(x, np.float64)
+        self.lib.orc_ppG_block(C.byref(self.param), C.byref(self.mesh), self.ppw.size, _p(self.ppn), _p(self.ppw),
+                               _p(x), float(t), int(y), int(F), _p(out))
+        return out.reshape(3, 3).T.copy()
+
+    def ppD_point(self, density, g, x, t, nthreads=None):
+        """u(x, t) = sum_y G_y alpha_y - P_y g_y (BEMenerg_ppD_calc.m)."""
+        d = np.asfortranarray(density, np.float64)
+        gg = np.asfortranarray(g, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        u = np.zeros(3)
+        self.lib.orc_ppD_point(C.byref(self.param), C.byref(self.mesh), d.ctypes.data_as(_dp), gg.ctypes.data_as(_dp),
+                               _p(x), float(t), self.ppw.size, _p(self.ppn), _p(self.ppw), _p(u),
+                               nthreads or os.cpu_count() or 1)
+        return u
 
     def set_pp_rule(self, nSub=64, nSing=3):
         """Rule of the numeric post-processing (PPn, PPw)."""
