@@ -46,6 +46,7 @@ SIGNATURES = {
     "ebem_dirichlet_datum": (C.c_int, [_vp, C.c_int, _dp]),
     "ebem_beta_k": (C.c_int, [_vp, _dp, _dp]),
     "ebem_rhs_dirichlet": (C.c_int, [_vp, C.c_int, _dp]),
+    "ebem_pp_eval_direct": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
     "ebem_mot_solve": (C.c_int, [_vp, _dp, _dp]),
     "ebem_pp_eval": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
     "ebem_node_integral": (C.c_int, [_vp, _dp, C.c_int, C.c_double, _dp]),

@@ -10,7 +10,8 @@ cited below; the computation runs in the native HIP library (libebem_hip.so).
   BEMenerg_coreSA_timeMarching  functions/BEMenerg_indir/BEMenerg_core/BEMenerg_coreSA/BEMenerg_coreSA_timeMarching.m
   BEMenerg_dir_timeMarchingD    functions/BEMenerg_dir/BEMenerg_dir_timeMarchingD.m (single-layer system)
   BEMenerg_postProcA_setup      functions/BEMenerg_indir/BEMenerg_postProc/BEMenerg_postProcA/BEMenerg_postProcA_setup.m
-  BEMenerg_ppD_setup            functions/BEMenerg_dirPP/BEMenerg_ppD_setup.m (single-layer part)
+  BEMenerg_ppD_setup            functions/BEMenerg_dirPP/BEMenerg_ppD_setup.m
+  BEMenerg_ppD_calc             functions/BEMenerg_dirPP/BEMenerg_ppD_calc.m
   BEMenerg_postProc_loadPoints   functions/BEMenerg_indir/BEMenerg_postProc/BEMenerg_postProc_loadPoints.m
   BEMenerg_postProcGPU_setupLayer / _setupSP
                                 functions/BEMenerg_indir/BEMenerg_postProc/BEMenerg_postProcGPU/*.m
@@ -334,6 +335,24 @@ class Solver:
                                                  x.shape[0], _lib.dptr(x), t.size, _lib.dptr(t), _lib.dptr(out)))
         return out.transpose(2, 1, 0).copy()
 
+    def pp_eval_direct(self, x: np.ndarray, t: np.ndarray, density: np.ndarray | None = None,
+                       g: np.ndarray | None = None) -> np.ndarray:
+        """u (3, npts, nt) = sum_y G_y alpha_y - P_y g_y of the direct
+        post-processing (BEMenerg_ppD_calc.m; needs set_pp_rule).  density None:
+        the device density of mot_solve; g None: the vertex datum of the last
+        rhs_dirichlet."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(t, np.float64).ravel()
+        out = np.zeros((t.size, x.shape[0], 3))
+        d = None if density is None else np.asfortranarray(density, dtype=np.float64)
+        gg = None if g is None else np.asfortranarray(g, dtype=np.float64)
+        if gg is not None and gg.shape != (3 * self.mesh.number_nodes, self.nT):
+            raise ValueError("g must be 3Nv x nT")
+        _lib.check(self.lib.ebem_pp_eval_direct(self.h, None if d is None else d.ctypes.data_as(_lib._dp),
+                                                None if gg is None else gg.ctypes.data_as(_lib._dp), x.shape[0],
+                                                _lib.dptr(x), t.size, _lib.dptr(t), _lib.dptr(out)))
+        return out.transpose(2, 1, 0).copy()
+
     def node_integral(self, x, tri: int, t: float) -> np.ndarray:
         x = np.ascontiguousarray(x, np.float64)
         out = np.zeros(9)
@@ -343,7 +362,7 @@ class Solver:
     def timings(self) -> dict:
         keys = ["levels_ms", "assembly_ms", "factor_ms", "history_ms", "apply_ms", "mot_ms", "pp_ms",
                 "history_launches", "levels_kp_ms", "pp_kp_ms", "levels_bf_ms", "pp_bf_ms", "k_assembly_ms",
-                "beta_k_ms"]
+                "beta_k_ms", "pp_direct_ms"]
         out = np.zeros(len(keys))
         _lib.check(self.lib.ebem_timings(self.h, _lib.dptr(out), len(keys)))
         return dict(zip(keys, out.tolist()))
@@ -523,15 +542,39 @@ BEMenerg_postProcGPU_setupSP = BEMenerg_postProcGPU_setupLayer
 
 
 def BEMenerg_ppD_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, methodInfo=None, PPn=None, PPw=None,
-                       *, device: int = 0):
-    """Single-layer part G alpha of the direct post-processing (BEMenerg_ppD_setup.m:1,
-    BEMenerg_ppD_calc.m:96-124 with kernelG.cu): the numeric sum of the GPU
-    post-processing when methodInfo carries a post-processing rule, the
-    analytic one otherwise.  kernelG.cu applies its 1e-14 cleanup to
-    M(t - y dt) - M(t - (y+1) dt) rather than to each M, which changes u only
-    through entries below 1e-14.  The double-layer part P g (kernelP.cu) is not
-    part of this library."""
-    if methodInfo is not None and (getattr(methodInfo, "numSubRegionPP", 0) or PPn is not None):
-        return BEMenerg_postProcGPU_setupLayer(pbParam, domainMesh, density, numPoints, xVal, tVal, methodInfo,
-                                               PPn, PPw, device=device)
-    return BEMenerg_postProcA_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, device=device)
+                       *, datum=None, device: int = 0, save_dir: str | None = None):
+    """campoVett (xSize x tSize x 3) of the direct post-processing
+    (dirPP/BEMenerg_ppD_setup.m:1-24): u = sum_y G_y alpha_y - P_y g_y at every
+    (x, t) of xVal x tVal, with the Dirichlet datum g of the domain
+    (calcBoundDataDirichlet.m; domains without one raise, as there) and the
+    rule numSubRegionPP x numNodiSingPP (or PPn / PPw) of methodInfo.  With
+    save_dir, also written as the reference does (:23-24):
+    <save_dir>/<domainType><lev>_NEW_campoVett.mat."""
+    xVal = np.asarray(xVal, float).reshape(-1, 3)
+    tVal = np.asarray(tVal, float).ravel()
+    if numPoints != xVal.shape[0] * tVal.size:
+        raise ValueError("numPoints does not match xVal x tVal")
+    kind = _datum_for(pbParam, datum)
+    if kind < DATUM_REF_BAR_H1:
+        raise ValueError("Problema non codificato")
+    nsub, nsing = _pp_rule_of(methodInfo, PPn)
+    with Solver(domainMesh, pbParam, 7, device) as s:
+        s.set_pp_rule(nsub, nsing)
+        g = s.dirichlet_datum(kind)
+        u = s.pp_eval_direct(xVal, tVal, density, g)
+    campo = u.transpose(1, 2, 0)
+    if save_dir is not None:
+        from . import matio
+        matio.save_field(matio.output_base(pbParam, "NEW_PP", None, save_dir), campo)
+    return campo
+
+
+def BEMenerg_ppD_calc(pbParam, domainMesh, density, gK, methodInfo, x, t, PPn=None, PPw=None, *, device: int = 0):
+    """uXT (3,) at one point and time (dirPP/BEMenerg_ppD_calc.m): gK the list
+    of nT vertex data of calcBoundDataDirichlet (or a 3Nv x nT array)."""
+    g = np.column_stack([np.asarray(v, float).ravel() for v in gK]) if isinstance(gK, (list, tuple)) else gK
+    nsub, nsing = _pp_rule_of(methodInfo, PPn)
+    with Solver(domainMesh, pbParam, 7, device) as s:
+        s.set_pp_rule(nsub, nsing)
+        u = s.pp_eval_direct(np.asarray(x, float).reshape(1, 3), np.array([float(t)]), density, g)
+    return u[:, 0, 0].copy()

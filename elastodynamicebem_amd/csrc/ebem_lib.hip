@@ -24,6 +24,7 @@
 #include "ebem_sur.hpp"
 #include "ebem_ghc.hpp"
 #include "ebem_k.hpp"
+#include "ebem_ppd.hpp"
 
 using namespace ebem;
 
@@ -446,6 +447,8 @@ struct ebem_handle {
     // pair windows [klo, khi) over (S, s) = S * Nv + s, slot offsets koff,
     // values 9 per slot; rows S occupy slots [k_rowbase[S], k_rowbase[S+1])
     bool k_ready = false;
+    bool kgeo_ready = false;  // d_ktri, d_T, d_X, vertex lists of the current mesh
+    bool have_gT = false;     // d_gT holds the vertex datum of the last ebem_rhs_dirichlet
     int k_LK = 0;
     int64_t k_pairs = 0, k_slots = 0;
     std::vector<int64_t> k_rowbase;
@@ -1430,18 +1433,11 @@ int k_num_blocks(ebem_handle* h) {
     return std::min(h->L + 1, h->nT);
 }
 
-int assemble_k(ebem_handle* h) {
-#pragma clang fp contract(off)
-    if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
-    if (!h->ghc_nSub) return fail(EBEM_EINVAL, "ebem_assemble_k needs ebem_set_inner_rule first");
-    if (h->lambda + h->mu == 0) return fail(EBEM_EINVAL, "Problema non adatto a questa formulazione");
+// P1 bases and the vertex -> (triangle, local vertex) lists, triangles
+// ascending, on the device (kept until the mesh changes).
+int k_geometry(ebem_handle* h) {
+    if (h->kgeo_ready) return 0;
     const int N = h->N, Nv = h->Nv;
-    if ((int64_t)N * Nv >= (int64_t)INT32_MAX) return fail(EBEM_EINVAL, "mesh too large for the K store");
-    const int LK = k_num_blocks(h);
-    if (LK > 32767) return fail(EBEM_EINVAL, "too many lags for the K store");
-    Timer tall;
-    tall.start(h->stream);
-    // P1 bases and the vertex -> (triangle, local vertex) lists, triangles ascending
     std::vector<KTri> kt(N);
     std::vector<int32_t> voff(Nv + 1, 0), vadj(3 * (size_t)N);
     for (int F = 0; F < N; F++) {
@@ -1466,6 +1462,23 @@ int assemble_k(ebem_handle* h) {
     HIPC(hipMemcpyAsync(h->d_X.p, h->X.data(), sizeof(double) * 3 * Nv, hipMemcpyHostToDevice, h->stream));
     HIPC(hipMemcpyAsync(h->d_vadj_off.p, voff.data(), sizeof(int32_t) * (Nv + 1), hipMemcpyHostToDevice, h->stream));
     HIPC(hipMemcpyAsync(h->d_vadj.p, vadj.data(), sizeof(int32_t) * 3 * N, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    h->kgeo_ready = true;
+    return 0;
+}
+
+int assemble_k(ebem_handle* h) {
+#pragma clang fp contract(off)
+    if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
+    if (!h->ghc_nSub) return fail(EBEM_EINVAL, "ebem_assemble_k needs ebem_set_inner_rule first");
+    if (h->lambda + h->mu == 0) return fail(EBEM_EINVAL, "Problema non adatto a questa formulazione");
+    const int N = h->N, Nv = h->Nv;
+    if ((int64_t)N * Nv >= (int64_t)INT32_MAX) return fail(EBEM_EINVAL, "mesh too large for the K store");
+    const int LK = k_num_blocks(h);
+    if (LK > 32767) return fail(EBEM_EINVAL, "too many lags for the K store");
+    Timer tall;
+    tall.start(h->stream);
+    if (int e = k_geometry(h)) return e;
     KParams K{};
     K.N = N;
     K.Nv = Nv;
@@ -1642,6 +1655,8 @@ int ebem_set_mesh(ebem_handle* h, int N, int Nv, const double* coords, const int
     h->have_mesh = true;
     h->assembled = false;
     h->k_ready = false;
+    h->kgeo_ready = false;
+    h->have_gT = false;
     h->have_beta = h->have_alpha = false;
     return upload_geometry(h);
 }
@@ -1662,6 +1677,7 @@ int ebem_set_problem(ebem_handle* h, double rho, double mu, double lambda, doubl
     h->have_problem = true;
     h->assembled = false;
     h->k_ready = false;
+    h->have_gT = false;
     h->have_beta = h->have_alpha = false;
     return 0;
 }
@@ -1923,6 +1939,7 @@ int ebem_beta_k(ebem_handle* h, const double* g, double* beta_k_out) {
     hipLaunchKernelGGL(k_dir_transpose, dim3((unsigned)((3 * n + 255) / 256)), dim3(256), 0, h->stream, d_g.p, h->Nv,
                        h->nT, h->d_gT.p);
     HIPC(hipGetLastError());
+    h->have_gT = false;
     if (int e = k_beta_device(h)) return e;
     HIPC(hipMemcpyAsync(beta_k_out, h->d_betak.p, sizeof(double) * 3 * (size_t)h->N * h->nT, hipMemcpyDeviceToHost,
                         h->stream));
@@ -1944,6 +1961,7 @@ int ebem_rhs_dirichlet(ebem_handle* h, int kind, double* beta_out) {
                        h->nT, h->nGH, h->dt, h->cP, h->Tfin, kind, h->d_beta.p);
     HIPC(hipGetLastError());
     if (int e = k_datum_device(h, kind)) return e;
+    h->have_gT = true;
     if (int e = k_beta_device(h)) return e;
     const int64_t nb = (int64_t)n3 * h->nT;
     hipLaunchKernelGGL(k_dd_combine, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, h->stream, h->d_beta.p,
@@ -1953,6 +1971,96 @@ int ebem_rhs_dirichlet(ebem_handle* h, int kind, double* beta_out) {
         HIPC(hipMemcpyAsync(beta_out, h->d_beta.p, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
     h->have_beta = true;
+    return 0;
+}
+
+int ebem_pp_eval_direct(ebem_handle* h, const double* density, const double* g, int npts, const double* x, int nt,
+                        const double* t, double* u_out) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
+    if (!h->pp_nInt) return fail(EBEM_EINVAL, "the direct post-processing needs ebem_set_pp_rule first");
+    if (npts < 0 || nt < 0 || (npts > 0 && !x) || (nt > 0 && !t) || !u_out) return fail(EBEM_EINVAL, "bad argument");
+    if (!density && !h->have_alpha) return fail(EBEM_EINVAL, "no density: pass one or run ebem_mot_solve first");
+    if (!g && !h->have_gT) return fail(EBEM_EINVAL, "no vertex datum: pass g or run ebem_rhs_dirichlet first");
+    const int N = h->N, Nv = h->Nv, nT = h->nT;
+    std::memset(u_out, 0, sizeof(double) * 3 * (size_t)npts * nt);
+    // (point, time) pairs with t > 0; nHat = ceil(t / dt) must not pass nT (density(:, nHat))
+    std::vector<int32_t> pq;
+    for (int k = 0; k < nt; k++) {
+        if (!(t[k] > 0)) continue;  // BEMenerg_ppD_calc.m:9-13
+        const int nHat = (int)std::ceil(t[k] / h->dt);
+        if (nHat > nT) return fail(EBEM_EINVAL, "time %g is past Tfin: the density has no step %d", t[k], nHat);
+        for (int p = 0; p < npts; p++) pq.push_back(p), pq.push_back(k);
+    }
+    const int nq = (int)(pq.size() / 2);
+    if (nq == 0) return 0;
+    HIPC(hipSetDevice(h->device));
+    if (int e = k_geometry(h)) return e;
+    Timer tall;
+    tall.start(h->stream);
+    DevBuf<double> d_x, d_t, d_a, d_g, d_gT, d_oG, d_oP, d_u;
+    DevBuf<int32_t> d_pq;
+    HIPC(d_x.alloc(3 * (size_t)std::max(npts, 1)));
+    HIPC(d_t.alloc(std::max(nt, 1)));
+    HIPC(d_pq.alloc(pq.size()));
+    HIPC(d_u.alloc(3 * (size_t)npts * nt));
+    HIPC(hipMemcpyAsync(d_x.p, x, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(d_t.p, t, sizeof(double) * nt, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(d_pq.p, pq.data(), sizeof(int32_t) * pq.size(), hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemsetAsync(d_u.p, 0, sizeof(double) * 3 * (size_t)npts * nt, h->stream));
+    const double* alpha = h->d_alpha.p;
+    if (density) {
+        HIPC(d_a.alloc(3 * (size_t)N * nT));
+        HIPC(hipMemcpyAsync(d_a.p, density, sizeof(double) * 3 * (size_t)N * nT, hipMemcpyHostToDevice, h->stream));
+        alpha = d_a.p;
+    }
+    const double* gT = h->d_gT.p;
+    if (g) {
+        const int64_t n = (int64_t)Nv * nT;
+        HIPC(d_g.alloc(3 * (size_t)n));
+        HIPC(d_gT.alloc(3 * (size_t)n));
+        HIPC(hipMemcpyAsync(d_g.p, g, sizeof(double) * 3 * n, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_dir_transpose, dim3((unsigned)((3 * n + 255) / 256)), dim3(256), 0, h->stream, d_g.p, Nv,
+                           nT, d_gT.p);
+        HIPC(hipGetLastError());
+        gT = d_gT.p;
+    }
+    PPDParams P{};
+    P.N = N;
+    P.Nv = Nv;
+    P.nT = nT;
+    P.nInt = h->pp_nInt;
+    P.dt = h->dt;
+    P.cP = h->cP;
+    P.cS = h->cS;
+    P.lam = h->lambda;
+    P.mu = h->mu;
+    P.rho = h->rho;
+    P.maxLen = h->hmax;
+    const size_t lds = sizeof(double) * 4 * (size_t)h->pp_nInt;
+    // pairs per launch bounded by the per-wave output buffers (64 MB)
+    const int qc = (int)std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t)(64e6 / (24.0 * (N + Nv)))));
+    HIPC(d_oG.alloc(3 * (size_t)qc * N));
+    HIPC(d_oP.alloc(3 * (size_t)qc * Nv));
+    for (int q0 = 0; q0 < nq; q0 += qc) {
+        const int n = std::min(qc, nq - q0);
+        const int64_t bG = std::min<int64_t>(((int64_t)n * N + 3) / 4, (int64_t)h->CUs * 8);
+        hipLaunchKernelGGL(k_ppd_G, dim3((unsigned)bG), dim3(256), lds, h->stream, h->d_tri.p, h->d_verts.p, d_x.p,
+                           d_t.p, d_pq.p, q0, n, P, h->d_pp_rule.p, alpha, d_oG.p);
+        HIPC(hipGetLastError());
+        const int64_t bP = std::min<int64_t>(((int64_t)n * Nv + 3) / 4, (int64_t)h->CUs * 8);
+        hipLaunchKernelGGL(k_ppd_P, dim3((unsigned)bP), dim3(256), lds, h->stream, h->d_ktri.p, h->d_verts.p, h->d_X.p,
+                           h->d_vadj_off.p, h->d_vadj.p, d_x.p, d_t.p, d_pq.p, q0, n, P, h->d_pp_rule.p, gT, d_oP.p);
+        HIPC(hipGetLastError());
+        hipLaunchKernelGGL(k_ppd_sum, dim3(n), dim3(256), 0, h->stream, d_oG.p, d_oP.p, d_pq.p, q0, N, Nv, npts,
+                           d_u.p);
+        HIPC(hipGetLastError());
+    }
+    tall.stop(h->stream);
+    HIPC(hipMemcpyAsync(u_out, d_u.p, sizeof(double) * 3 * (size_t)npts * nt, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    h->tim[14] = tall.ms();
     return 0;
 }
 

@@ -1,0 +1,186 @@
+// ebem_ppd.hpp -- device kernels of the direct post-processing
+// u(x, t) = sum_{y < nHat} G_y alpha_y - P_y g_y, nHat = ceil(t / dt)
+// (functions/BEMenerg_dirPP/BEMenerg_ppD_calc.m:9-123; G as
+// CUDAkernels/kernelG.cu, P as CUDAkernels/kernelP.cu, math of P in
+// ebem_ppd_math.hpp).  One wave per (point-time pair q, triangle F) for G and
+// per (q, vertex s) for P: the wave walks y = 0 .. nHat-1, the lanes split the
+// rule's nodes, each block is butterflied, cleaned below 1e-14 and divided as
+// the reference does, and the wave keeps sum_y block * vector in order; a
+// last kernel sums each pair's waves in a fixed order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ebem_ghc.hpp"
+#include "ebem_ppd_math.hpp"
+
+namespace ebem {
+
+struct PPDParams {
+    int N, Nv, nT, nInt;
+    double dt, cP, cS, lam, mu, rho, maxLen;
+};
+
+__device__ __attribute__((always_inline)) inline double wave_sum(double a) {
+    for (int m = 32; m >= 1; m >>= 1) a += __shfl_xor(a, m);
+    return a;
+}
+
+// nHat = ceil(t / dt) of BEMenerg_ppD_calc.m:18 (0 for t <= 0).
+__device__ __attribute__((always_inline)) inline int ppd_nhat(double t, double dt) {
+#pragma clang fp contract(off)
+    return t <= 0 ? 0 : (int)ceil(t / dt);
+}
+
+// Single-layer part: wave per (q, F), q = pair index of this launch (pairs
+// q0 + q of pq: point, time index).  out[(q * N + F) * 3 + c] = sum_y (G_y alpha_y)_c.
+// G_y (kernelG.cu:26-116): levels t - y dt and t - (y + 1) dt with weights 1, -1
+// (non-positive skipped), clean-up below 1e-14, / (4 pi rho).  A block whose
+// triangle lies beyond the P front of its later level, |x - c_F| - maxL_F >
+// cP (t - y dt), is zero in every entry, and so are those of larger y.
+__global__ void __launch_bounds__(256) k_ppd_G(const FieldTri* __restrict__ tri, const double* __restrict__ verts,
+                                               const double* __restrict__ xs, const double* __restrict__ ts,
+                                               const int32_t* __restrict__ pq, int q0, int nq, PPDParams P,
+                                               const double* __restrict__ rule_g, const double* __restrict__ alpha,
+                                               double* __restrict__ out) {
+#pragma clang fp contract(off)
+    extern __shared__ double rule[];
+    for (int k = threadIdx.x; k < 4 * P.nInt; k += blockDim.x) rule[k] = rule_g[k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const double fourpirho = 4 * kPi * P.rho;
+    for (int64_t it = wave; it < (int64_t)nq * P.N; it += nwaves) {
+        const int q = (int)(it / P.N), F = (int)(it % P.N);
+        const int p = pq[2 * (q0 + q)], kt = pq[2 * (q0 + q) + 1];
+        const double x[3] = {xs[3 * p], xs[3 * p + 1], xs[3 * p + 2]};
+        const double t = ts[kt];
+        const int nHat = ppd_nhat(t, P.dt);
+        const double* vf = verts + (int64_t)F * 9;
+        const double aF = tri[F].area;
+        const double c0 = x[0] - tri[F].center[0], c1 = x[1] - tri[F].center[1], c2 = x[2] - tri[F].center[2];
+        const double dmin = sqrt(c0 * c0 + c1 * c1 + c2 * c2) - tri[F].maxL;
+        double u[3] = {0, 0, 0};
+        for (int y = 0; y < nHat; y++) {
+            const double t0 = t - (double)y * P.dt;
+            if (dmin > P.cP * t0 * (1 + 1e-12)) break;  // wave-uniform; exact zeros from here on
+            double acc[6] = {0, 0, 0, 0, 0, 0};
+            const double cn[2] = {1, -1};
+            for (int k = lane; k < P.nInt; k += 64) {
+                const double* b = rule + 4 * k;
+                double d[3];
+                for (int e = 0; e < 3; e++) d[e] = (b[0] * vf[e] + b[1] * vf[3 + e] + b[2] * vf[6 + e]) - x[e];
+                const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                const double w = b[3] * aF;
+#pragma unroll
+                for (int l = 0; l < 2; l++) {
+                    const double tau = t - (double)(y + l) * P.dt;
+                    if (tau <= 0) continue;
+                    double nu[6];
+                    nucleo6(d, r, tau, P.cP, P.cS, nu);
+#pragma unroll
+                    for (int c = 0; c < 6; c++) acc[c] += cn[l] * w * nu[c];
+                }
+            }
+            double G[6];
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                const double a = wave_sum(acc[c]);
+                G[c] = fabs(a) > 1.0e-14 ? a / fourpirho : 0.0;
+            }
+            const double* al = alpha + (int64_t)y * 3 * P.N + 3 * F;
+            const double a0 = al[0], a1 = al[1], a2 = al[2];
+            u[0] += G[0] * a0 + G[1] * a1 + G[2] * a2;
+            u[1] += G[1] * a0 + G[3] * a1 + G[4] * a2;
+            u[2] += G[2] * a0 + G[4] * a1 + G[5] * a2;
+        }
+        if (lane == 0) {
+            double* o = out + it * 3;
+            o[0] = u[0];
+            o[1] = u[1];
+            o[2] = u[2];
+        }
+    }
+}
+
+// Double-layer part: wave per (q, s).  out[(q * Nv + s) * 3 + c] =
+// sum_y (P_y g_y)_c with g_y = gT[s][y] (the datum at (y + 1) dt).  A vertex
+// whose triangles all lie beyond the P front of the latest level,
+// |x - X_s| - maxLen > cP (t - y dt), gives zero blocks from that y on.
+__global__ void __launch_bounds__(256) k_ppd_P(const KTri* __restrict__ kt, const double* __restrict__ verts,
+                                               const double* __restrict__ X, const int32_t* __restrict__ vadj_off,
+                                               const int32_t* __restrict__ vadj, const double* __restrict__ xs,
+                                               const double* __restrict__ ts, const int32_t* __restrict__ pq, int q0,
+                                               int nq, PPDParams P, const double* __restrict__ rule_g,
+                                               const double* __restrict__ gT, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    extern __shared__ double rule[];
+    for (int k = threadIdx.x; k < 4 * P.nInt; k += blockDim.x) rule[k] = rule_g[k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const double scale = 4 * kPi * P.rho * P.dt;
+    PPDArgs A{kt, verts, vadj_off, vadj, rule, P.nInt, P.lam, P.mu, P.cP, P.cS, P.dt};
+    for (int64_t it = wave; it < (int64_t)nq * P.Nv; it += nwaves) {
+        const int q = (int)(it / P.Nv), s = (int)(it % P.Nv);
+        const int p = pq[2 * (q0 + q)], kk = pq[2 * (q0 + q) + 1];
+        const double x[3] = {xs[3 * p], xs[3 * p + 1], xs[3 * p + 2]};
+        const double t = ts[kk];
+        const int nHat = ppd_nhat(t, P.dt);
+        const double d0 = x[0] - X[3 * s], d1 = x[1] - X[3 * s + 1], d2 = x[2] - X[3 * s + 2];
+        const double dmin = sqrt(d0 * d0 + d1 * d1 + d2 * d2) - P.maxLen;
+        double u[3] = {0, 0, 0};
+        for (int y = 0; y < nHat; y++) {
+            if (dmin > P.cP * (t - (double)y * P.dt) * (1 + 1e-12)) break;  // wave-uniform
+            double acc[3][3];
+            ppd_p_lane(A, x, t, y, s, lane, 64, acc);
+            double B[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const double a = wave_sum(acc[i][j]);
+                    B[i][j] = fabs(a) > 1.0e-14 ? a / scale : 0.0;
+                }
+            const double* g = gT + ((int64_t)s * P.nT + y) * 3;
+            const double g0 = g[0], g1 = g[1], g2 = g[2];
+#pragma unroll
+            for (int i = 0; i < 3; i++) u[i] += B[i][0] * g0 + B[i][1] * g1 + B[i][2] * g2;
+        }
+        if (lane == 0) {
+            double* o = out + it * 3;
+            o[0] = u[0];
+            o[1] = u[1];
+            o[2] = u[2];
+        }
+    }
+}
+
+// u of pair q0 + q = sum_F G-part - sum_s P-part, one workgroup per pair,
+// fixed strided order and tree: u_out[3 * (p + npts * k) + c].
+__global__ void __launch_bounds__(256) k_ppd_sum(const double* __restrict__ outG, const double* __restrict__ outP,
+                                                 const int32_t* __restrict__ pq, int q0, int N, int Nv, int npts,
+                                                 double* __restrict__ u_out) {
+    __shared__ double red[3][256];
+    const int q = blockIdx.x;
+    double s[3] = {0, 0, 0};
+    for (int j = threadIdx.x; j < N; j += blockDim.x)
+        for (int c = 0; c < 3; c++) s[c] += outG[((int64_t)q * N + j) * 3 + c];
+    for (int j = threadIdx.x; j < Nv; j += blockDim.x)
+        for (int c = 0; c < 3; c++) s[c] -= outP[((int64_t)q * Nv + j) * 3 + c];
+    for (int c = 0; c < 3; c++) red[c][threadIdx.x] = s[c];
+    __syncthreads();
+    for (int d = 128; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d)
+            for (int c = 0; c < 3; c++) red[c][threadIdx.x] += red[c][threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int p = pq[2 * (q0 + q)], k = pq[2 * (q0 + q) + 1];
+        for (int c = 0; c < 3; c++) u_out[3 * ((int64_t)p + (int64_t)npts * k) + c] = red[c][0];
+    }
+}
+
+}  // namespace ebem

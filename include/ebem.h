@@ -183,6 +183,17 @@ int ebem_beta_k(ebem_handle *h, const double *g, double *beta_k_out);
  * for ebem_mot_solve(h, NULL, ...); copied to beta_out if non-NULL. */
 int ebem_rhs_dirichlet(ebem_handle *h, int datum, double *beta_out);
 
+/* Direct post-processing (dirPP/BEMenerg_ppD_calc.m:9-123, BEMenerg_ppD_setup.m):
+ * u(x_p, t_k) = sum_{y < nHat} G_y alpha_y - P_y g_y, nHat = ceil(t_k / dt), with
+ * G_y as dirPP/CUDAkernels/kernelG.cu and P_y as kernelP.cu over the rule of
+ * ebem_set_pp_rule (numSubRegionPP x numNodiSingPP); u = 0 for t_k <= 0, an
+ * error for t_k past Tfin.  density: host 3N x nT or NULL for the device
+ * density of ebem_mot_solve; g: host vertex datum 3Nv x nT (column y at
+ * (y+1) dt) or NULL for the one of the last ebem_rhs_dirichlet.  x: npts x 3
+ * row-major; u_out: 3 x npts x nt (u_out[3*(p + npts*k) + c]). */
+int ebem_pp_eval_direct(ebem_handle *h, const double *density, const double *g, int npts, const double *x, int nt,
+                        const double *t, double *u_out);
+
 /* MOT: factorise E_0 in FP64 on the GPU and march
  * alpha_n = E_0^{-1}(beta_n - sum_{l=1}^{min(n,L)-1} E_l alpha_{n-l})
  * (SA/BEMenerg_coreSA_timeMarching.m:45-113, FN/BEMenerg_coreFN_timeMarching.m:152-171).
@@ -222,8 +233,9 @@ int ebem_node_integral(ebem_handle *h, const double *x, int tri, double t, doubl
  * launches, out[8] assembly kP-range kernel, out[9] post-processing kP-range
  * kernel, out[10] assembly between-front kernel, out[11] post-processing
  * between-front kernel, out[12] ebem_assemble_k total, out[13] beta_K kernel
- * of the last ebem_beta_k / ebem_rhs_dirichlet. */
-#define EBEM_NUM_TIMINGS 14
+ * of the last ebem_beta_k / ebem_rhs_dirichlet, out[14] ebem_pp_eval_direct
+ * total. */
+#define EBEM_NUM_TIMINGS 15
 int ebem_timings(ebem_handle *h, double *out, int n);
 
 /* ------------------------------------------------------------------------

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../elastodynamicebem_amd/csrc/ebem_k_math.hpp"
+#include "../../elastodynamicebem_amd/csrc/ebem_ppd_math.hpp"
 
 using namespace ebem;
 
@@ -102,6 +103,44 @@ int k_block_host(int N, int Nv, const double* X, const int32_t* T, const double*
         for (int e = 0; e < 9; e++) out[e] += blk[e] / K.fourpidt;
     }
     std::memcpy(out9, out, sizeof out);
+    return 0;
+}
+
+// P_y block of vertex s for the point x at time t (the direct
+// post-processing's kernelP part), column-major, by the device math with the
+// 64-lane butterfly; rule rin: nInt x 4.
+int ppp_block_host(int N, int Nv, const double* X, const int32_t* T, const double* nrm, const double* area,
+                   double rho, double mu, double lam, double Tfin, int nT, int nInt, const double* rin,
+                   const double* x, double t, int y, int s, double* out9) {
+#pragma clang fp contract(off)
+    std::vector<KTri> kt(N);
+    std::vector<double> verts((size_t)N * 9);
+    std::vector<int32_t> voff(Nv + 1, 0), vadj(3 * (size_t)N);
+    for (int F = 0; F < N; F++) {
+        const double* V[3] = {X + 3 * T[3 * F], X + 3 * T[3 * F + 1], X + 3 * T[3 * F + 2]};
+        k_tri_basis(V[0], V[1], V[2], nrm + 3 * F, area[F], kt[F]);
+        for (int v = 0; v < 3; v++) {
+            for (int c = 0; c < 3; c++) verts[9 * (size_t)F + 3 * v + c] = V[v][c];
+            voff[T[3 * F + v] + 1]++;
+        }
+    }
+    for (int v = 0; v < Nv; v++) voff[v + 1] += voff[v];
+    std::vector<int32_t> pos(voff.begin(), voff.end() - 1);
+    for (int F = 0; F < N; F++)
+        for (int v = 0; v < 3; v++) vadj[pos[T[3 * F + v]]++] = F * 4 + v;
+    const double dt = Tfin / nT;
+    PPDArgs A{kt.data(), verts.data(), voff.data(), vadj.data(), rin, nInt, lam, mu,
+              std::sqrt((lam + 2 * mu) / rho), std::sqrt(mu / rho), dt};
+    static double lanes[64][9];
+    for (int l = 0; l < 64; l++) {
+        double acc[3][3];
+        ppd_p_lane(A, x, t, y, s, l, 64, acc);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) lanes[l][i + 3 * j] = acc[i][j];
+    }
+    butterfly(lanes);
+    const double scale = 4 * kPi * rho * dt;
+    for (int e = 0; e < 9; e++) out9[e] = std::fabs(lanes[0][e]) > 1.0e-14 ? lanes[0][e] / scale : 0.0;
     return 0;
 }
 

@@ -25,6 +25,9 @@ def host_k():
                                  [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int,
                                   _dp])
     lib.k_block_host.restype = C.c_int
+    lib.ppp_block_host.argtypes = ([C.c_int, C.c_int, _dp, _ip, _dp, _dp] + [C.c_double] * 4 +
+                                   [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int, _dp])
+    lib.ppp_block_host.restype = C.c_int
     return lib
 
 
@@ -73,3 +76,35 @@ def test_host_k_blocks_match_oracle(host_k, request, meshname, lam):
                 checked += 1
     assert checked > 100
     print(f"worst host-vs-oracle K block error {worst:.2e} over {checked} blocks")
+
+
+def test_host_pp_p_blocks_match_oracle(host_k, sphere2):
+    """The device math of the direct post-processing's double-layer part
+    (ebem_ppd_math.hpp) against the oracle's kernelP restatement."""
+    from elastodynamicebem_amd import api
+    from oracle import Oracle
+    o = Oracle(sphere2.coordinates, sphere2.triangles, rho=1.0, mu=1.0, lam=2.0, Tfin=8.0, nT=32)
+    o.set_pp_rule(16, 3)
+    gn, gw, _, _ = api.inner_rule_tables(16, 3, 16)
+    rule = np.ascontiguousarray(np.column_stack([gn, gw]))
+    assert np.allclose(gn, o.ppn) and np.allclose(gw, o.ppw)
+    rng = np.random.default_rng(8)
+    checked = 0
+    for _ in range(150):
+        x = rng.normal(size=3) * rng.choice([0.3, 0.9, 1.6])
+        s = int(rng.integers(sphere2.number_nodes))
+        t = rng.uniform(0.2, 8.0)
+        y = int(rng.integers(int(np.ceil(t / o.dt))))
+        R = o.pp_p_block(x, t, y, s)
+        out = np.zeros(9)
+        host_k.ppp_block_host(o.N, sphere2.number_nodes, _p(o.coords), o.tri.ctypes.data_as(_ip), _p(o.normal),
+                              _p(o.area), 1.0, 1.0, 2.0, 8.0, 32, rule.shape[0], _p(rule),
+                              _p(np.ascontiguousarray(x)), t, y, s, _p(out))
+        G = out.reshape(3, 3).T
+        sc = np.abs(R).max()
+        if sc == 0:
+            assert np.abs(G).max() == 0
+            continue
+        assert np.abs(G - R).max() < 1e-11 * sc, (x, t, y, s)
+        checked += 1
+    assert checked > 40
