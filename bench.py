@@ -217,20 +217,26 @@ def k_windows_and_flops(m, pb, LK, nGH, nInt, nDiag, rows=None):
 
 def dd_main(args):
     """Direct Dirichlet step on the level-2 bar with the reference's default
-    method FN 19 64 3 256: V (numeric GHC form), K_0..K_{LK-1}, rhs / 2 + beta_K
-    and the MOT, inputs resident on the device; value = K blocks / K assembly time."""
+    method FN 19 64 3 256 and post-processing "GPU 64 3" (main.m:16-20):
+    V (numeric GHC form), K_0..K_{LK-1}, rhs / 2 + beta_K, the MOT and the
+    direct post-processing u = G alpha - P g at the reference's barH1 point
+    set (BEMenerg_postProc_loadPoints: 3 points x 600 times), inputs resident
+    on the device; value = K blocks / K assembly time."""
     from elastodynamicebem_amd import api
     m, pb = dd_workload(2)
     nGH, nSub, nSing, nDiag = 19, 64, 3, 256
     s = api.Solver(m, pb, nGH, device=0)
     s.set_inner_rule(nSub, nSing, nDiag)
+    s.set_pp_rule(64, 3)
+    xpp, tpp, _, npp, _ = api.BEMenerg_postProc_loadPoints(pb)
     LK = s.k_num_blocks()
 
     def step():
         s.assemble(api.PREC_GHC)
         s.assemble_k()
         s.rhs_dirichlet(api.DATUM_REF_BAR_H1)
-        return s.mot_solve(None)
+        s.mot_solve(None)
+        return s.pp_eval_direct(xpp, tpp)
 
     for w in range(args.warmup):
         step()
@@ -256,11 +262,14 @@ def dd_main(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "barH1 level 2 (768 triangles, 386 vertices), rho 0.25, mu 0.5, lambda 0, "
-                               "t in [0, 6], 96 steps, FN 19 64 3 256: V (GHC) + K + rhs/2 + beta_K + MOT",
+                               "t in [0, 6], 96 steps, FN 19 64 3 256 + PP GPU 64 3: V (GHC) + K + rhs/2 + "
+                               "beta_K + MOT + direct post-processing at 3 points x 600 times",
+                   "pp_points": int(npp),
                    "k_lags": LK, "k_slots": slots, "k_pairs": info["pairs"], "k_store_bytes": info["bytes"]},
         "breakdown_ms": {"v_assembly": float(np.mean([t["assembly_ms"] for t in tims])), "k_assembly": k_ms,
                          "beta_k": float(np.mean([t["beta_k_ms"] for t in tims])),
-                         "mot_total": float(np.mean([t["mot_ms"] for t in tims]))},
+                         "mot_total": float(np.mean([t["mot_ms"] for t in tims])),
+                         "post_processing_direct": float(np.mean([t["pp_direct_ms"] for t in tims]))},
         "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_kblocks", "achieved": ach,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
                      "traffic": None},

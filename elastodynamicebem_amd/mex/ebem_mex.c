@@ -100,6 +100,20 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     } else if (!strcmp(cmd, "rhs_dirichlet")) {
         plhs[0] = mxCreateDoubleMatrix(g_n3, g_nT, mxREAL);
         chk(ebem_rhs_dirichlet(g_h, (int)scal(prhs[1]), mxGetDoubles(plhs[0])));
+    } else if (!strcmp(cmd, "pp_direct")) {
+        /* ebem_mex('pp_direct', density_or_[], g_or_[], X (k x 3), t) */
+        const double *dens = mxIsEmpty(prhs[1]) ? NULL : mxGetDoubles(prhs[1]);
+        const double *gk = mxIsEmpty(prhs[2]) ? NULL : mxGetDoubles(prhs[2]);
+        mwSize k = mxGetM(prhs[3]), nt = mxGetNumberOfElements(prhs[4]);
+        double *xv = mxGetDoubles(prhs[3]);
+        double *xr = (double *)mxMalloc(sizeof(double) * 3 * k);
+        for (mwSize p = 0; p < k; p++)
+            for (int c = 0; c < 3; c++) xr[3 * p + c] = xv[p + k * c];
+        mwSize dims[3] = {3, k, nt};
+        plhs[0] = mxCreateNumericArray(3, dims, mxDOUBLE_CLASS, mxREAL);
+        int rc = ebem_pp_eval_direct(g_h, dens, gk, (int)k, xr, (int)nt, mxGetDoubles(prhs[4]), mxGetDoubles(plhs[0]));
+        mxFree(xr);
+        chk(rc);
     } else if (!strcmp(cmd, "mot")) {
         plhs[0] = mxCreateDoubleMatrix(g_n3, g_nT, mxREAL);
         const double *beta = mxIsEmpty(prhs[1]) ? NULL : mxGetDoubles(prhs[1]);
@@ -122,7 +136,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     } else if (strcmp(cmd, "mesh") && strcmp(cmd, "problem") && strcmp(cmd, "rule") && strcmp(cmd, "inner_rule") &&
                strcmp(cmd, "pp_rule") && strcmp(cmd, "numblocks") && strcmp(cmd, "assemble") && strcmp(cmd, "block") &&
                strcmp(cmd, "rhs") && strcmp(cmd, "mot") && strcmp(cmd, "assemble_k") && strcmp(cmd, "kblock") &&
-               strcmp(cmd, "rhs_dirichlet")) {
+               strcmp(cmd, "rhs_dirichlet") && strcmp(cmd, "pp_direct")) {
         mexErrMsgIdAndTxt("ebem:args", "unknown command '%s'", cmd);
     }
     (void)nlhs;
