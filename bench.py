@@ -47,8 +47,10 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic FP64 operations of the double-layer block kernel k_kblocks, from
 # the op-counting host build of its math (tests/native/k_opcount.cpp): per
 # regular evaluation point (one outer node, one inner node of one field
-# triangle) 218 plus 385 per time level with t > 0; per level of the singular
-# sum 585 per evaluation point (three children x nDiag nodes per outer node).
+# triangle) 218 for its geometry plus 385 per time level with t > 0 (the
+# kernel evaluates a point once for up to 4 consecutive lags of a pair, whose
+# levels overlap); per level of the singular sum 585 per evaluation point
+# (three children x nDiag nodes per outer node).
 K_REG_POINT_BASE, K_REG_POINT_LEVEL, K_SING_POINT_LEVEL = 218, 385, 585
 
 
@@ -185,11 +187,13 @@ def dd_workload(level=2):
     return m, pb
 
 
-def k_windows_and_flops(m, pb, LK, nGH, nInt, nDiag, rows=None):
+def k_windows_and_flops(m, pb, LK, nGH, nInt, nDiag, rows=None, chunk=4):
     """Slots of the K store (pair windows of isBlockNull and the singular
     cut-off, kernelK.cu:226-260, calcSubBlockSingK.m:12) and the algorithmic
-    FP64 operations of k_kblocks over them; rows = (r0, r1) restricts the
-    test triangles."""
+    FP64 operations of k_kblocks over them: per chunk of `chunk` consecutive
+    slots of a pair, every regular evaluation point once for the geometry and
+    once per distinct positive level of the chunk's kept lags; singular parts
+    per lag and level.  rows = (r0, r1) restricts the test triangles."""
     X, T0 = m.coordinates, m.triangles
     r0, r1 = (0, T0.shape[0]) if rows is None else rows
     T = T0[r0:r1]
@@ -203,16 +207,35 @@ def k_windows_and_flops(m, pb, LK, nGH, nInt, nDiag, rows=None):
     deg = np.bincount(T0.ravel(), minlength=Nv)
     nF = deg[None, :] - own  # field triangles F != S with vertex s
     cut = np.ceil(m.maxL[r0:r1] / (cS * dt)).astype(int) + 2
+    big = 1 << 30
+    lo, hi = np.full((N, Nv), big), np.full((N, Nv), -1)
+    rlo, rhi = np.full((N, Nv), big), np.full((N, Nv), -1)
     slots = 0
-    flops = 0.0
+    sing_flops = 0.0
     for lag in range(LK):
         reg = ~(((lag - 2) * cS * dt > dmax + maxLen) | ((lag + 1) * cP * dt < dmin - 2 * maxLen))
         sing = own & (lag < cut[:, None])
-        slots += int(np.count_nonzero(reg | sing))
+        keep = reg | sing
+        slots += int(np.count_nonzero(keep))
+        lo = np.where(keep, np.minimum(lo, lag), lo)
+        hi = np.where(keep, np.maximum(hi, lag + 1), hi)
+        rlo = np.where(reg, np.minimum(rlo, lag), rlo)
+        rhi = np.where(reg, np.maximum(rhi, lag + 1), rhi)
         nlev = sum(1 for c in (-2, -1, 0, 1) if lag + c >= 1)
-        flops += float((nF * reg).sum()) * nGH * nInt * (K_REG_POINT_BASE + K_REG_POINT_LEVEL * nlev)
-        flops += float(sing.sum()) * nGH * 3 * nDiag * K_SING_POINT_LEVEL * nlev
-    return slots, flops
+        sing_flops += float(sing.sum()) * nGH * 3 * nDiag * K_SING_POINT_LEVEL * nlev
+    reg_flops = 0.0
+    has = hi > lo
+    width = np.where(has, hi - lo, 0)
+    for c0 in range(0, int(width.max()) if width.size else 0, chunk):
+        a = lo + c0
+        b = np.minimum(a + chunk, hi)
+        ra = np.maximum(a, rlo)
+        rb = np.minimum(b, rhi)
+        ok = has & (c0 < width) & (rb > ra)
+        lev = np.maximum(0, rb - np.maximum(ra - 2, 1) + 1)  # levels ra-2 .. rb, positive
+        pts = nF * nGH * nInt
+        reg_flops += float((ok * pts * (K_REG_POINT_BASE + K_REG_POINT_LEVEL * lev)).sum())
+    return slots, reg_flops + sing_flops
 
 
 def dd_main(args):

@@ -88,11 +88,14 @@ __global__ void __launch_bounds__(256) k_kslots(const int16_t* __restrict__ lo, 
     }
 }
 
-// One wave per slot (grid-stride): the block (S, s) of K_lag, the lanes'
-// parts of k_reg_lane / k_sing_lane summed by an xor butterfly.  Regular part
-// cleaned below 1e-14 and divided by 4 pi dt (kernelK.cu:212-221); singular
-// part combined over levels with -1, 3, -3, 1 and divided by 4 pi dt
-// (calcSubBlockSingK.m:17-39), added to it (timeMarchingD.m:244-255).
+// One wave per chunk of up to kKChunk consecutive slots of a pair (the wave
+// of a slot whose offset in its pair's window is a multiple of kKChunk; the
+// others pass): the blocks (S, s) of K_lag for those lags.  Regular part
+// (k_reg_lane_chunk over the chunk's lags that isBlockNull keeps, an
+// interval), each lag's lanes summed by an xor butterfly, cleaned below
+// 1e-14 and divided by 4 pi dt (kernelK.cu:212-221); singular part per lag,
+// its levels combined with -1, 3, -3, 1 and divided by 4 pi dt
+// (calcSubBlockSingK.m:17-39), added (timeMarchingD.m:244-255).
 __global__ void __launch_bounds__(256) k_kblocks(const KTri* __restrict__ kt, const FieldTri* __restrict__ tri,
                                                  const double* __restrict__ verts, const double* __restrict__ gh,
                                                  const double* __restrict__ X, const int32_t* __restrict__ T,
@@ -101,7 +104,7 @@ __global__ void __launch_bounds__(256) k_kblocks(const KTri* __restrict__ kt, co
                                                  const double* __restrict__ rdg_g, int nGH, KParams K, int64_t nslots,
                                                  const int32_t* __restrict__ slot_pair,
                                                  const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
-                                                 double* __restrict__ val) {
+                                                 const int16_t* __restrict__ hi, double* __restrict__ val) {
 #pragma clang fp contract(off)
     extern __shared__ double rule[];  // (nInt + nDiag) x 4
     for (int k = threadIdx.x; k < 4 * (K.nInt + K.nDiag); k += blockDim.x)
@@ -115,50 +118,68 @@ __global__ void __launch_bounds__(256) k_kblocks(const KTri* __restrict__ kt, co
     const double cn[4] = {-1, 3, -3, 1};
     for (int64_t it = wave; it < nslots; it += nwaves) {
         const int32_t p = slot_pair[it];
+        const int rel = (int)(it - off[p]);
+        if (rel % kKChunk != 0) continue;  // wave-uniform
         const int S = p / K.Nv, s = p % K.Nv;
-        const int lag = lo[p] + (int)(it - off[p]);
+        const int lag0 = lo[p] + rel;
+        const int nc = min(kKChunk, (int)hi[p] - lag0);
         A.g = gh + (int64_t)S * 128;
-        double out[9];
-        for (int e = 0; e < 9; e++) out[e] = 0.0;
-        if (!k_block_null(verts + (int64_t)S * 9, X + 3 * (int64_t)s, lag, K)) {  // wave-uniform
-            double acc[3][3];
-            k_reg_lane(A, K, S, s, lag, lane, 64, acc);
+        const double* vS = verts + (int64_t)S * 9;
+        const double* Ps = X + 3 * (int64_t)s;
+        // regular parts first, written by lane 0 (zero for the lags the skip test drops)
+        int ra = -1, rb = -1;  // the chunk's lags that isBlockNull keeps (an interval)
+        for (int j = 0; j < nc; j++)
+            if (!k_block_null(vS, Ps, lag0 + j, K)) {
+                if (ra < 0) ra = j;
+                rb = j + 1;
+            }
+        if (lane == 0)
+            for (int j = 0; j < nc; j++)
+                if (j < ra || j >= rb)
+                    for (int e = 0; e < 9; e++) val[(it + j) * 9 + e] = 0.0;
+        if (ra >= 0) {  // wave-uniform
+            double acc[kKChunk][3][3];
+            k_reg_lane_chunk(A, K, S, s, lag0 + ra, rb - ra, lane, 64, acc);
 #pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    double a = acc[i][c];
-                    for (int m = 32; m >= 1; m >>= 1) a += __shfl_xor(a, m);
-                    out[i + 3 * c] = fabs(a) > 1.0e-14 ? a / K.fourpidt : 0.0;
-                }
-        }
-        const int vl = k_local_vertex(T, S, s);
-        if (vl >= 0 && lag < k_sing_cut(tri[S].maxL, K)) {  // wave-uniform
-            double blk[3][3];
-            for (int i = 0; i < 3; i++)
-                for (int c = 0; c < 3; c++) blk[i][c] = 0.0;
-            for (int l = 0; l < 4; l++) {
-                const int il = lag + ct[l];
-                if (il <= 0) continue;
-                double lev[3][3];
-                k_sing_lane(A, K, S, vl, (double)il * K.dt, lane, 64, lev);
+            for (int j = 0; j < kKChunk; j++) {
+                if (j >= rb - ra) break;
 #pragma unroll
                 for (int i = 0; i < 3; i++)
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
-                        double a = lev[i][c];
+                        double a = acc[j][i][c];
                         for (int m = 32; m >= 1; m >>= 1) a += __shfl_xor(a, m);
-                        blk[i][c] += cn[l] * a;
+                        if (lane == 0) val[(it + ra + j) * 9 + i + 3 * c] = fabs(a) > 1.0e-14 ? a / K.fourpidt : 0.0;
                     }
             }
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) out[i + 3 * c] += blk[i][c] / K.fourpidt;
         }
-        if (lane == 0) {
-            double* o = val + it * 9;
-            for (int e = 0; e < 9; e++) o[e] = out[e];
+        const int vl = k_local_vertex(T, S, s);
+        if (vl >= 0) {
+            const int cut = k_sing_cut(tri[S].maxL, K);
+            for (int j = 0; j < nc; j++) {
+                const int lag = lag0 + j;
+                if (lag >= cut) break;  // wave-uniform
+                double blk[3][3];
+                for (int i = 0; i < 3; i++)
+                    for (int c = 0; c < 3; c++) blk[i][c] = 0.0;
+                for (int l = 0; l < 4; l++) {
+                    const int il = lag + ct[l];
+                    if (il <= 0) continue;
+                    double lev[3][3];
+                    k_sing_lane(A, K, S, vl, (double)il * K.dt, lane, 64, lev);
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+#pragma unroll
+                        for (int c = 0; c < 3; c++) {
+                            double a = lev[i][c];
+                            for (int m = 32; m >= 1; m >>= 1) a += __shfl_xor(a, m);
+                            blk[i][c] += cn[l] * a;
+                        }
+                }
+                if (lane == 0)
+                    for (int i = 0; i < 3; i++)
+                        for (int c = 0; c < 3; c++) val[(it + j) * 9 + i + 3 * c] += blk[i][c] / K.fourpidt;
+            }
         }
     }
 }

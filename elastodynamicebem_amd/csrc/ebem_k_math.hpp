@@ -258,6 +258,71 @@ __host__ __device__ inline void k_reg_lane(const KBlockArgs& A, const KParams& K
     }
 }
 
+// Lags per wave of the regular K blocks: consecutive lags of a pair share
+// three of their four time levels and every point's geometry.
+#ifndef EBEM_K_CHUNK
+#define EBEM_K_CHUNK 4
+#endif
+constexpr int kKChunk = EBEM_K_CHUNK;
+
+// k_reg_lane for the nA <= kKChunk consecutive lags lagA .. lagA+nA-1 at
+// once: each point's geometry once, each level lagA-2 .. lagA+nA once, its
+// value added to every lag whose four levels contain it with that lag's
+// weight.  Per lag the additions come in the order of k_reg_lane (field
+// triangle, outer node, inner node, level ascending), so every acc[j] equals
+// k_reg_lane's for lag lagA + j bit for bit.
+__host__ __device__ inline void k_reg_lane_chunk(const KBlockArgs& A, const KParams& K, int S, int s, int lagA,
+                                                 int nA, int lane, int nl, double acc[kKChunk][3][3]) {
+#pragma clang fp contract(off)
+    const double cn[4] = {-1, 3, -3, 1};
+    for (int j = 0; j < kKChunk; j++)
+        for (int i = 0; i < 3; i++)
+            for (int c = 0; c < 3; c++) acc[j][i][c] = 0.0;
+    for (int e = A.vadj_off[s]; e < A.vadj_off[s + 1]; e++) {
+        const int F = A.vadj[e] >> 2, v = A.vadj[e] & 3;
+        if (F == S) continue;
+        const KTri& B = A.kt[F];
+        const double* gv = B.mat + 3 * v;
+        const double w[3] = {+B.n[1] * gv[2] - B.n[2] * gv[1], -B.n[0] * gv[2] + B.n[2] * gv[0],
+                             +B.n[0] * gv[1] - B.n[1] * gv[0]};
+        const double* vf = A.verts + (int64_t)F * 9;
+        const double aF = B.area;
+        for (int q = 0; q < A.nGH; q++) {
+            const double xq[3] = {A.g[q], A.g[32 + q], A.g[64 + q]};
+            const double wq = A.g[96 + q];
+            for (int k = lane; k < K.nInt; k += nl) {
+                const double* b = A.rin + 4 * k;
+                double y[3];
+                for (int c = 0; c < 3; c++) y[c] = b[0] * vf[c] + b[1] * vf[3 + c] + b[2] * vf[6 + c];
+                const double pw = wq * (b[3] * aF);
+                const double x[3] = {xq[0] - y[0], xq[1] - y[1], xq[2] - y[2]};
+                const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+                const double phi = B.vec[v] + gv[0] * y[0] + gv[1] * y[1] + gv[2] * y[2];
+                KGeom G;
+                k_geom(x, r, B.n, K, G);
+#pragma unroll
+                for (int li = 0; li < kKChunk + 3; li++) {
+                    if (li >= nA + 3) break;
+                    const int il = lagA - 2 + li;
+                    const double t = K.dt * (double)il;
+                    if (t <= 0) continue;
+                    double kv[3][3];
+                    k_kernel<true>(G, t, w, phi, K, kv);
+#pragma unroll
+                    for (int j = 0; j < kKChunk; j++) {
+                        const int d = li - j;  // level index of il within lag lagA + j
+                        if (j >= nA || d < 0 || d > 3) continue;
+#pragma unroll
+                        for (int i = 0; i < 3; i++)
+#pragma unroll
+                            for (int c = 0; c < 3; c++) acc[j][i][c] += pw * cn[d] * kv[i][c];
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Lane `lane` of `nl` lanes: its part of one level's singular sum
 // sum_q w_q sum_children sum_d (w_d area_child) K (calcKIntgSing.m:142-171)
 // for local vertex vl of S, time t.

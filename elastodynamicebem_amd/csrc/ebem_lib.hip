@@ -1530,7 +1530,8 @@ int assemble_k(ebem_handle* h) {
         HIPC(hipFuncSetAttribute((const void*)k_kblocks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_kblocks, dim3((unsigned)blocks), dim3(256), lds, h->stream, h->d_ktri.p, h->d_tri.p,
                            h->d_verts.p, h->d_gh.p, h->d_X.p, h->d_T.p, h->d_vadj_off.p, h->d_vadj.p, h->d_ghc_in.p,
-                           h->d_ghc_dg.p, h->nGH, K, slots, h->d_kslot.p, h->d_koff.p, h->d_klo.p, h->d_kval.p);
+                           h->d_ghc_dg.p, h->nGH, K, slots, h->d_kslot.p, h->d_koff.p, h->d_klo.p, h->d_khi.p,
+                           h->d_kval.p);
         HIPC(hipGetLastError());
     }
     tall.stop(h->stream);

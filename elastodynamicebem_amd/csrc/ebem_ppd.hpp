@@ -132,22 +132,31 @@ __global__ void __launch_bounds__(256) k_ppd_P(const KTri* __restrict__ kt, cons
         const double d0 = x[0] - X[3 * s], d1 = x[1] - X[3 * s + 1], d2 = x[2] - X[3 * s + 2];
         const double dmin = sqrt(d0 * d0 + d1 * d1 + d2 * d2) - P.maxLen;
         double u[3] = {0, 0, 0};
-        for (int y = 0; y < nHat; y++) {
-            if (dmin > P.cP * (t - (double)y * P.dt) * (1 + 1e-12)) break;  // wave-uniform
-            double acc[3][3];
-            ppd_p_lane(A, x, t, y, s, lane, 64, acc);
-            double B[3][3];
+        for (int y0 = 0; y0 < nHat; y0 += kPpdChunk) {
+            // blocks of this chunk before the first one beyond the P front
+            int nA = 0;
+            while (nA < kPpdChunk && y0 + nA < nHat && !(dmin > P.cP * (t - (double)(y0 + nA) * P.dt) * (1 + 1e-12)))
+                nA++;
+            if (nA == 0) break;  // wave-uniform
+            double acc[kPpdChunk][3][3];
+            ppd_p_lane_chunk(A, x, t, y0, nA, s, lane, 64, acc);
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+            for (int j = 0; j < kPpdChunk; j++) {
+                if (j >= nA) break;
+                double B[3][3];
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const double a = wave_sum(acc[i][j]);
-                    B[i][j] = fabs(a) > 1.0e-14 ? a / scale : 0.0;
-                }
-            const double* g = gT + ((int64_t)s * P.nT + y) * 3;
-            const double g0 = g[0], g1 = g[1], g2 = g[2];
+                for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int i = 0; i < 3; i++) u[i] += B[i][0] * g0 + B[i][1] * g1 + B[i][2] * g2;
+                    for (int c = 0; c < 3; c++) {
+                        const double a = wave_sum(acc[j][i][c]);
+                        B[i][c] = fabs(a) > 1.0e-14 ? a / scale : 0.0;
+                    }
+                const double* g = gT + ((int64_t)s * P.nT + y0 + j) * 3;
+                const double g0 = g[0], g1 = g[1], g2 = g[2];
+#pragma unroll
+                for (int i = 0; i < 3; i++) u[i] += B[i][0] * g0 + B[i][1] * g1 + B[i][2] * g2;
+            }
+            if (nA < kPpdChunk) break;
         }
         if (lane == 0) {
             double* o = out + it * 3;

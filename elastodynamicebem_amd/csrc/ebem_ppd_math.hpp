@@ -111,4 +111,59 @@ __host__ __device__ inline void ppd_p_lane(const PPDArgs& A, const double* x, do
     }
 }
 
+// Time blocks per chunk of the double-layer post-processing: consecutive
+// blocks share two of their three levels and every point's geometry.
+#ifndef EBEM_PPD_CHUNK
+#define EBEM_PPD_CHUNK 4
+#endif
+constexpr int kPpdChunk = EBEM_PPD_CHUNK;
+
+// ppd_p_lane for the nA <= kPpdChunk blocks yA .. yA+nA-1 at once: each point's
+// geometry once, each level t - l dt (l = yA .. yA+nA+1) once, added to every
+// block whose three levels contain it with that block's weight; per block the
+// additions come in ppd_p_lane's order, so acc[j] equals ppd_p_lane's for
+// y = yA + j bit for bit.
+__host__ __device__ inline void ppd_p_lane_chunk(const PPDArgs& A, const double* x, double t, int yA, int nA, int s,
+                                                 int lane, int nl, double acc[kPpdChunk][3][3]) {
+#pragma clang fp contract(off)
+    const double cn[3] = {1, -2, 1};
+    for (int j = 0; j < kPpdChunk; j++)
+        for (int i = 0; i < 3; i++)
+            for (int c = 0; c < 3; c++) acc[j][i][c] = 0.0;
+    for (int e = A.vadj_off[s]; e < A.vadj_off[s + 1]; e++) {
+        const int F = A.vadj[e] >> 2, v = A.vadj[e] & 3;
+        const KTri& B = A.kt[F];
+        const double* gv = B.mat + 3 * v;
+        const double* vf = A.verts + (int64_t)F * 9;
+        for (int q = lane; q < A.nInt; q += nl) {
+            const double* b = A.rule + 4 * q;
+            double yv[3];
+            for (int c = 0; c < 3; c++) yv[c] = b[0] * vf[c] + b[1] * vf[3 + c] + b[2] * vf[6 + c];
+            const double w = b[3] * B.area;
+            const double xv[3] = {x[0] - yv[0], x[1] - yv[1], x[2] - yv[2]};
+            const double r = sqrt(xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2]);
+            const double phi = B.vec[v] + gv[0] * yv[0] + gv[1] * yv[1] + gv[2] * yv[2];
+            PGeom G;
+            p_geom(xv, r, B.n, A.lam, A.mu, G);
+#pragma unroll
+            for (int li = 0; li < kPpdChunk + 2; li++) {
+                if (li >= nA + 2) break;
+                const double tau = t - (double)(yA + li) * A.dt;
+                if (tau <= 0) continue;
+                double nu[3][3];
+                p_kernel(G, tau, A.cP, A.cS, phi, nu);
+#pragma unroll
+                for (int j = 0; j < kPpdChunk; j++) {
+                    const int d = li - j;
+                    if (j >= nA || d < 0 || d > 2) continue;
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+#pragma unroll
+                        for (int c = 0; c < 3; c++) acc[j][i][c] += cn[d] * w * nu[i][c];
+                }
+            }
+        }
+    }
+}
+
 }  // namespace ebem

@@ -106,6 +106,80 @@ int k_block_host(int N, int Nv, const double* X, const int32_t* T, const double*
     return 0;
 }
 
+// Regular parts (before clean-up) of the blocks (S, s) of lags lagA ..
+// lagA+nA-1 by k_reg_lane_chunk (the kernel's chunked form) and, lag by lag,
+// by k_reg_lane: out[0 .. 9 nA) and out[9 nA .. 18 nA), column-major, each
+// butterflied over 64 lanes.  The two must agree bit for bit.
+int k_chunk_vs_single_host(int N, int Nv, const double* X, const int32_t* T, const double* nrm, const double* area,
+                           const double* maxL, double rho, double mu, double lam, double Tfin, int nT, int nGH,
+                           const double* ghn, const double* ghw, int nInt, const double* rin, int lagA, int nA, int S,
+                           int s, double* out) {
+#pragma clang fp contract(off)
+    if (nA < 1 || nA > kKChunk) return -1;
+    std::vector<KTri> kt(N);
+    std::vector<double> verts((size_t)N * 9);
+    std::vector<int32_t> voff(Nv + 1, 0), vadj(3 * (size_t)N);
+    double maxLen = 0;
+    for (int F = 0; F < N; F++) {
+        const double* V[3] = {X + 3 * T[3 * F], X + 3 * T[3 * F + 1], X + 3 * T[3 * F + 2]};
+        k_tri_basis(V[0], V[1], V[2], nrm + 3 * F, area[F], kt[F]);
+        for (int v = 0; v < 3; v++) {
+            for (int c = 0; c < 3; c++) verts[9 * (size_t)F + 3 * v + c] = V[v][c];
+            voff[T[3 * F + v] + 1]++;
+        }
+        if (maxL[F] > maxLen) maxLen = maxL[F];
+    }
+    for (int v = 0; v < Nv; v++) voff[v + 1] += voff[v];
+    std::vector<int32_t> pos(voff.begin(), voff.end() - 1);
+    for (int F = 0; F < N; F++)
+        for (int v = 0; v < 3; v++) vadj[pos[T[3 * F + v]]++] = F * 4 + v;
+    double g[128] = {0};
+    const double* VS[3] = {X + 3 * T[3 * S], X + 3 * T[3 * S + 1], X + 3 * T[3 * S + 2]};
+    for (int q = 0; q < nGH; q++) {
+        for (int k = 0; k < 3; k++)
+            g[32 * k + q] = ghn[3 * q] * VS[0][k] + ghn[3 * q + 1] * VS[1][k] + ghn[3 * q + 2] * VS[2][k];
+        g[96 + q] = area[S] * ghw[q];
+    }
+    KParams K{};
+    K.N = N;
+    K.Nv = Nv;
+    K.LK = nT;
+    K.nInt = nInt;
+    K.nDiag = 1;
+    K.dt = Tfin / nT;
+    K.cP = std::sqrt((lam + 2 * mu) / rho);
+    K.cS = std::sqrt(mu / rho);
+    K.lam = lam;
+    K.mu = mu;
+    K.rho = rho;
+    K.maxLen = maxLen;
+    K.fourpidt = 4 * kPi * K.dt;
+    KBlockArgs A{kt.data(), verts.data(), voff.data(), vadj.data(), rin, rin, g, nGH};
+    static double lanes[kKChunk][64][9];
+    for (int l = 0; l < 64; l++) {
+        double acc[kKChunk][3][3];
+        k_reg_lane_chunk(A, K, S, s, lagA, nA, l, 64, acc);
+        for (int j = 0; j < nA; j++)
+            for (int i = 0; i < 3; i++)
+                for (int c = 0; c < 3; c++) lanes[j][l][i + 3 * c] = acc[j][i][c];
+    }
+    for (int j = 0; j < nA; j++) {
+        butterfly(lanes[j]);
+        for (int e = 0; e < 9; e++) out[9 * j + e] = lanes[j][0][e];
+    }
+    for (int j = 0; j < nA; j++) {
+        for (int l = 0; l < 64; l++) {
+            double acc[3][3];
+            k_reg_lane(A, K, S, s, lagA + j, l, 64, acc);
+            for (int i = 0; i < 3; i++)
+                for (int c = 0; c < 3; c++) lanes[0][l][i + 3 * c] = acc[i][c];
+        }
+        butterfly(lanes[0]);
+        for (int e = 0; e < 9; e++) out[9 * nA + 9 * j + e] = lanes[0][0][e];
+    }
+    return 0;
+}
+
 // P_y block of vertex s for the point x at time t (the direct
 // post-processing's kernelP part), column-major, by the device math with the
 // 64-lane butterfly; rule rin: nInt x 4.
@@ -135,8 +209,15 @@ int ppp_block_host(int N, int Nv, const double* X, const int32_t* T, const doubl
     for (int l = 0; l < 64; l++) {
         double acc[3][3];
         ppd_p_lane(A, x, t, y, s, l, 64, acc);
+        // the kernel's chunked form must give the same sum bit for bit
+        double accc[kPpdChunk][3][3];
+        const int j = y % kPpdChunk;
+        ppd_p_lane_chunk(A, x, t, y - j, j + 1, s, l, 64, accc);
         for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) lanes[l][i + 3 * j] = acc[i][j];
+            for (int c = 0; c < 3; c++)
+                if (accc[j][i][c] != acc[i][c]) return 1;
+        for (int i = 0; i < 3; i++)
+            for (int jj = 0; jj < 3; jj++) lanes[l][i + 3 * jj] = acc[i][jj];
     }
     butterfly(lanes);
     const double scale = 4 * kPi * rho * dt;
