@@ -312,6 +312,60 @@ class Oracle:
                                nthreads or os.cpu_count() or 1)
         return u
 
+    # -- Neumann driver (dir/BEMenerg_dir_timeMarchingN.m), NumPy over the blocks --
+    def neumann_datum(self, h):
+        """gV (3N x nT) of dir/BEMenerg_dir_calcBoundDataNeumann.m:5-25 for the bar
+        of height h: (0, 0, 2 mu tildeP(c_F, t) n_z) at the triangle centres,
+        column n at time (n+1) dt, tildeP = sum_k (-1)^k [H(a_k) + H(b_k)]."""
+        cP, T = self.velP, self.Tfin
+        khat = int(np.ceil((cP * T) / (2 * h))) - 1
+        out = np.zeros((3 * self.N, self.nT))
+        for n in range(self.nT):
+            t = (n + 1) * self.dt
+            for F in range(self.N):
+                z = self.center[F, 2]
+                tp = 0.0
+                for k in range(khat + 1):
+                    a = cP * t - 2 * h * k - (h - z)
+                    b = cP * t - 2 * h * (k + 1) + (h - z)
+                    tp += (-1.0) ** k * (float(a > 0) + float(b > 0))
+                out[3 * F + 2, n] = 2 * self.param.mu * tp * self.normal[F, 2]
+        return out
+
+    def mass_gamma(self):
+        """I_Gamma = kron((indSM > 0) .* area / 6, eye(3)) (timeMarchingN.m:275): 3N x 3Nv."""
+        Nv = self.coords.shape[0]
+        M = np.zeros((3 * self.N, 3 * Nv))
+        for S in range(self.N):
+            for v in self.tri[S]:
+                for c in range(3):
+                    M[3 * S + c, 3 * v + c] = self.area[S] / 6
+        return M
+
+    def dn_mot(self, V, K, gV):
+        """density (3Nv x nT) of timeMarchingN.m:170-296 from dense V_l (3N x 3N,
+        l < len(V)), K_l (3N x 3Nv, l < len(K)) and the Neumann datum gV:
+        beta_V as calcBetaVgpu.m, then (K_0 + I_Gamma) d_n = beta_V,n + I_Gamma d_{n-1}
+        - sum_{l=1}^{min(n, LK-1)} K_l d_{n-l} by least squares."""
+        nT = self.nT
+        Nv = self.coords.shape[0]
+        betaV = np.zeros((3 * self.N, nT))
+        for n in range(nT):
+            for j in range(n + 1):
+                if n - j < len(V):
+                    betaV[:, n] += V[n - j] @ gV[:, j]
+        I = self.mass_gamma()
+        A = K[0] + I
+        d = np.zeros((3 * Nv, nT))
+        for n in range(nT):
+            r = betaV[:, n].copy()
+            if n > 0:
+                r += I @ d[:, n - 1]
+            for l in range(1, min(n + 1, len(K))):
+                r -= K[l] @ d[:, n - l]
+            d[:, n] = np.linalg.lstsq(A, r, rcond=None)[0]
+        return d, betaV
+
     def set_pp_rule(self, nSub=64, nSing=3):
         """Rule of the numeric post-processing (PPn, PPw)."""
         self.ppn, self.ppw = self.gh_composite(nSub, nSing)
