@@ -272,6 +272,11 @@ def dd_main(args):
         tims.append(s.timings())
         print(f"bench dd: timed step {k + 1}/{args.steps}: {1e3 * times[-1]:.1f} ms", file=sys.stderr, flush=True)
     info = s.k_store_info()
+    # the Neumann driver on the same V and K (timeMarchingN.m:170-296), once
+    t0 = time.perf_counter()
+    s.mot_solve_neumann(api.DATUM_REF_BAR_H1)
+    neu_wall = 1e3 * (time.perf_counter() - t0)
+    neu_ms = s.timings()["mot_neumann_ms"]
     s.close()
     slots, flops = k_windows_and_flops(m, pb, LK, nGH, nSub * nSing, nDiag)
     assert slots == info["slots"], (slots, info["slots"])
@@ -293,6 +298,9 @@ def dd_main(args):
                          "beta_k": float(np.mean([t["beta_k_ms"] for t in tims])),
                          "mot_total": float(np.mean([t["mot_ms"] for t in tims])),
                          "post_processing_direct": float(np.mean([t["pp_direct_ms"] for t in tims]))},
+        "neumann_driver_ms": {"mot_neumann": neu_ms, "wall": neu_wall,
+                              "note": "datum, beta_V, QR of K_0 + I_Gamma (rocSOLVER), 96 least-squares steps; "
+                                      "outside the timed step"},
         "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_kblocks", "achieved": ach,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
                      "traffic": None},

@@ -47,6 +47,9 @@ SIGNATURES = {
     "ebem_beta_k": (C.c_int, [_vp, _dp, _dp]),
     "ebem_rhs_dirichlet": (C.c_int, [_vp, C.c_int, _dp]),
     "ebem_pp_eval_direct": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
+    "ebem_neumann_datum": (C.c_int, [_vp, C.c_int, _dp]),
+    "ebem_beta_v": (C.c_int, [_vp, _dp, _dp]),
+    "ebem_mot_solve_neumann": (C.c_int, [_vp, C.c_int, _dp]),
     "ebem_mot_solve": (C.c_int, [_vp, _dp, _dp]),
     "ebem_pp_eval": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
     "ebem_node_integral": (C.c_int, [_vp, _dp, C.c_int, C.c_double, _dp]),

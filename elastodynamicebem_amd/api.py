@@ -304,6 +304,28 @@ class Solver:
         _lib.check(self.lib.ebem_rhs_dirichlet(self.h, datum, out.ctypes.data_as(_lib._dp)))
         return out
 
+    # -- Neumann driver -----------------------------------------------------------
+    def neumann_datum(self, datum: int) -> np.ndarray:
+        """Traction datum gV (3N x nT) at the triangle centres, column n at (n+1) dt."""
+        out = np.zeros((3 * self.N, self.nT), order="F")
+        _lib.check(self.lib.ebem_neumann_datum(self.h, datum, out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def beta_v(self, gV: np.ndarray) -> np.ndarray:
+        """beta_V (3N x nT) = sum_j V_{n-j} gV_j over the assembled single-layer blocks."""
+        g = np.asfortranarray(gV, dtype=np.float64)
+        if g.shape != (3 * self.N, self.nT):
+            raise ValueError("gV must be 3N x nT")
+        out = np.zeros((3 * self.N, self.nT), order="F")
+        _lib.check(self.lib.ebem_beta_v(self.h, g.ctypes.data_as(_lib._dp), out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def mot_solve_neumann(self, datum: int) -> np.ndarray:
+        """Density (3Nv x nT) of the Neumann driver (needs assemble and assemble_k)."""
+        out = np.zeros((3 * self.mesh.number_nodes, self.nT), order="F")
+        _lib.check(self.lib.ebem_mot_solve_neumann(self.h, datum, out.ctypes.data_as(_lib._dp)))
+        return out
+
     def mot_solve(self, beta: np.ndarray | None = None) -> np.ndarray:
         out = np.zeros((3 * self.N, self.nT), order="F")
         b = None if beta is None else np.asfortranarray(beta, dtype=np.float64)
@@ -362,7 +384,7 @@ class Solver:
     def timings(self) -> dict:
         keys = ["levels_ms", "assembly_ms", "factor_ms", "history_ms", "apply_ms", "mot_ms", "pp_ms",
                 "history_launches", "levels_kp_ms", "pp_kp_ms", "levels_bf_ms", "pp_bf_ms", "k_assembly_ms",
-                "beta_k_ms", "pp_direct_ms"]
+                "beta_k_ms", "pp_direct_ms", "mot_neumann_ms"]
         out = np.zeros(len(keys))
         _lib.check(self.lib.ebem_timings(self.h, _lib.dptr(out), len(keys)))
         return dict(zip(keys, out.tolist()))
@@ -424,6 +446,60 @@ def BEMenerg_dir_timeMarchingD(pbParam, domainMesh, methodInfo, GHn=None, GHw=No
         from . import matio
         matio.save_density(matio.output_base(pbParam, "NEW", methodInfo, save_dir), density)
     return density
+
+
+def BEMenerg_dir_timeMarchingN(pbParam, domainMesh, methodInfo, GHn=None, GHw=None, GHCn=None, GHCw=None,
+                               DIAGn=None, DIAGw=None, *, datum=None, device: int = 0,
+                               save_dir: str | None = None):
+    """Neumann driver (dir/BEMenerg_dir_timeMarchingN.m): V in the numeric form,
+    K, beta_V of the traction datum, and the marching with the least-squares
+    solve of (K_0 + I_Gamma); density 3Nv x nT at the vertices.  With save_dir,
+    written as the reference does (:299-301)."""
+    if pbParam.lambda_ + pbParam.mu == 0:
+        raise ValueError("Problema non adatto a questa formulazione")
+    if methodInfo.typeIntg != "FN":
+        raise ValueError("the Neumann driver needs an FN method (numeric inner rules of V and K)")
+    kind = _datum_for(pbParam, datum)
+    with Solver(domainMesh, pbParam, methodInfo.numNodiExt, device) as s:
+        s.set_inner_rule(methodInfo.numSubRegion, methodInfo.numNodiSing, methodInfo.numNodiDiag)
+        s.assemble(PREC_GHC)
+        s.assemble_k()
+        density = s.mot_solve_neumann(kind)
+    if save_dir is not None:
+        from . import matio
+        matio.save_density(matio.output_base(pbParam, "NEW", methodInfo, save_dir), density)
+    return density
+
+
+def BEMenerg_dir_calcBoundDataNeumann(deltaT, pbParam, domainMesh, *, datum=None, device: int = 0):
+    """neuBoundData: list of nT traction data (3N each, at the triangle centres),
+    entry n at time (n+1) dt (dir/BEMenerg_dir_calcBoundDataNeumann.m)."""
+    if abs(deltaT - pbParam.Tfin / pbParam.nT) > 1e-12 * max(1.0, deltaT):
+        raise ValueError("deltaT must be Tfin / nT")
+    with Solver(domainMesh, pbParam, 7, device) as s:
+        g = s.neumann_datum(_datum_for(pbParam, datum))
+    return [g[:, n].copy() for n in range(pbParam.nT)]
+
+
+def BEMenerg_ppN_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, methodInfo=None, PPn=None, PPw=None,
+                       *, datum=None, device: int = 0, save_dir: str | None = None):
+    """campoVett of the Neumann post-processing (dirPP/BEMenerg_ppN_setup.m):
+    u = sum_y G_y gV_y - P_y d_y, gV the traction datum, d the vertex density
+    (BEMenerg_ppN_calc.m:121)."""
+    xVal = np.asarray(xVal, float).reshape(-1, 3)
+    tVal = np.asarray(tVal, float).ravel()
+    if numPoints != xVal.shape[0] * tVal.size:
+        raise ValueError("numPoints does not match xVal x tVal")
+    nsub, nsing = _pp_rule_of(methodInfo, PPn)
+    with Solver(domainMesh, pbParam, 7, device) as s:
+        s.set_pp_rule(nsub, nsing)
+        gV = s.neumann_datum(_datum_for(pbParam, datum))
+        u = s.pp_eval_direct(xVal, tVal, gV, density)
+    campo = u.transpose(1, 2, 0)
+    if save_dir is not None:
+        from . import matio
+        matio.save_field(matio.output_base(pbParam, "NEW_PP", None, save_dir), campo)
+    return campo
 
 
 def BEMenerg_dir_calcBoundDataDirichlet(deltaT, pbParam, domainMesh, *, datum=None, device: int = 0):

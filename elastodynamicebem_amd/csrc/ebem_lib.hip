@@ -25,6 +25,7 @@
 #include "ebem_ghc.hpp"
 #include "ebem_k.hpp"
 #include "ebem_ppd.hpp"
+#include "ebem_dn.hpp"
 
 using namespace ebem;
 
@@ -457,6 +458,9 @@ struct ebem_handle {
     DevBuf<double> d_X, d_kval, d_gT, d_betak;
     DevBuf<int16_t> d_klo, d_khi;
     DevBuf<int64_t> d_koff, d_krow;
+    // Neumann driver: traction datum [triangle][step][3], beta_V, density 3Nv x nT
+    DevBuf<double> d_gVT, d_betaV, d_dn;
+    bool have_dn = false;
     bool sort_items = true;
     bool kp_split = true;  // kP ranges through k_levels_kp (EBEM_KP_SPLIT=0 turns it off)
     // kP and between-front ranges of the last assembly / post-processing:
@@ -1571,6 +1575,85 @@ int k_beta_device(ebem_handle* h) {
     return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// Neumann driver (timeMarchingN.m:170-296)
+// ---------------------------------------------------------------------------
+
+int neu_datum_device(ebem_handle* h, int kind) {
+    const int64_t n = (int64_t)h->N * h->nT;
+    HIPC(h->d_gVT.grow(3 * (size_t)n));
+    hipLaunchKernelGGL(k_neu_datum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->d_tri.p, h->N,
+                       h->nT, h->dt, h->cP, h->Tfin, h->mu, kind == EBEM_DATUM_REF_BAR_H1 ? 1.0 : 3.0, h->d_gVT.p);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+int betav_device(ebem_handle* h) {
+    if (h->nT > 256 * kBetaVPer) return fail(EBEM_EINVAL, "beta_V supports at most %d time steps", 256 * kBetaVPer);
+    HIPC(h->d_betaV.grow(3 * (size_t)h->N * h->nT));
+    hipLaunchKernelGGL(k_betav, dim3(h->N), dim3(256), 0, h->stream, h->store(), h->store_L, h->N, h->nT, h->d_gVT.p,
+                       h->d_betaV.p);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+// (K_0 + I_Gamma) d_n = beta_V,n + I_Gamma d_{n-1} - sum_{l>=1} K_l d_{n-l} in the
+// least-squares sense (MATLAB's backslash on the 3N x 3Nv system): A = QR once
+// (rocSOLVER geqrf), P = R^{-1} Q^T (ormqr on the identity, trsm), then one
+// gemv per step.
+int mot_neumann(ebem_handle* h, int kind) {
+    const int N = h->N, Nv = h->Nv, nT = h->nT;
+    const int m = 3 * N, n = 3 * Nv;
+    if (m < n) return fail(EBEM_EINVAL, "the Neumann system needs 3N >= 3Nv (N = %d, Nv = %d)", N, Nv);
+    if (!h->blas) {
+        if (int e = rocb(rocblas_create_handle(&h->blas), "rocblas_create_handle")) return e;
+    }
+    if (int e = rocb(rocblas_set_stream(h->blas, h->stream), "rocblas_set_stream")) return e;
+    Timer tall;
+    tall.start(h->stream);
+    if (int e = neu_datum_device(h, kind)) return e;
+    if (int e = betav_device(h)) return e;
+    DevBuf<double> A, tau, Pm, tnf;
+    HIPC(A.alloc((size_t)m * n));
+    HIPC(tau.alloc(n));
+    HIPC(Pm.alloc((size_t)m * m));
+    HIPC(tnf.alloc(m));
+    HIPC(hipMemsetAsync(A.p, 0, sizeof(double) * (size_t)m * n, h->stream));
+    const int64_t np = (int64_t)N * Nv;
+    hipLaunchKernelGGL(k_dn_sysmat, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, h->stream, h->d_klo.p,
+                       h->d_khi.p, h->d_koff.p, h->d_kval.p, h->d_T.p, h->d_tri.p, N, Nv, A.p);
+    HIPC(hipGetLastError());
+    if (int e = rocb(rocsolver_dgeqrf(h->blas, m, n, A.p, m, tau.p), "rocsolver_dgeqrf")) return e;
+    HIPC(hipMemsetAsync(Pm.p, 0, sizeof(double) * (size_t)m * m, h->stream));
+    hipLaunchKernelGGL(k_set_diag, dim3((m + 255) / 256), dim3(256), 0, h->stream, Pm.p, m, 1.0);
+    HIPC(hipGetLastError());
+    if (int e = rocb(rocsolver_dormqr(h->blas, rocblas_side_left, rocblas_operation_transpose, m, m, n, A.p, m, tau.p,
+                                      Pm.p, m),
+                     "rocsolver_dormqr"))
+        return e;
+    const double one = 1.0, zero = 0.0;
+    if (int e = rocb(rocblas_dtrsm(h->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                   rocblas_diagonal_non_unit, n, m, &one, A.p, m, Pm.p, m),
+                     "rocblas_dtrsm"))
+        return e;
+    HIPC(h->d_dn.grow((size_t)n * nT));
+    for (int s = 0; s < nT; s++) {
+        hipLaunchKernelGGL(k_dn_rhs, dim3(N), dim3(256), 0, h->stream, h->d_klo.p, h->d_khi.p, h->d_koff.p,
+                           h->d_kval.p, h->d_T.p, h->d_tri.p, N, Nv, h->k_LK, s, h->d_betaV.p, h->d_dn.p, tnf.p);
+        HIPC(hipGetLastError());
+        if (int e = rocb(rocblas_dgemv(h->blas, rocblas_operation_none, n, m, &one, Pm.p, m, tnf.p, 1, &zero,
+                                       h->d_dn.p + (size_t)s * n, 1),
+                         "rocblas_dgemv"))
+            return e;
+    }
+    tall.stop(h->stream);
+    HIPC(hipStreamSynchronize(h->stream));
+    h->tim[15] = tall.ms();
+    h->have_dn = true;
+    return 0;
+}
+
 extern "C" {
 
 const char* ebem_last_error(void) { return g_err.c_str(); }
@@ -1658,6 +1741,7 @@ int ebem_set_mesh(ebem_handle* h, int N, int Nv, const double* coords, const int
     h->k_ready = false;
     h->kgeo_ready = false;
     h->have_gT = false;
+    h->have_dn = false;
     h->have_beta = h->have_alpha = false;
     return upload_geometry(h);
 }
@@ -2062,6 +2146,63 @@ int ebem_pp_eval_direct(ebem_handle* h, const double* density, const double* g, 
     HIPC(hipMemcpyAsync(u_out, d_u.p, sizeof(double) * 3 * (size_t)npts * nt, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
     h->tim[14] = tall.ms();
+    return 0;
+}
+
+int ebem_neumann_datum(ebem_handle* h, int datum, double* gV_out) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
+    if (datum != EBEM_DATUM_REF_BAR_H1 && datum != EBEM_DATUM_REF_BAR_H3)
+        return fail(EBEM_EINVAL, "Problema non codificato (datum %d has no Neumann trace)", datum);
+    if (!gV_out) return fail(EBEM_EINVAL, "bad argument");
+    HIPC(hipSetDevice(h->device));
+    if (int e = neu_datum_device(h, datum)) return e;
+    const int64_t n = (int64_t)h->N * h->nT;
+    std::vector<double> gT(3 * (size_t)n);
+    HIPC(hipMemcpyAsync(gT.data(), h->d_gVT.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    for (int F = 0; F < h->N; F++)
+        for (int t = 0; t < h->nT; t++)
+            for (int c = 0; c < 3; c++)
+                gV_out[(int64_t)t * 3 * h->N + 3 * F + c] = gT[((int64_t)F * h->nT + t) * 3 + c];
+    return 0;
+}
+
+int ebem_beta_v(ebem_handle* h, const double* gV, double* beta_v_out) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->assembled) return fail(EBEM_EINVAL, "ebem_assemble must run before ebem_beta_v");
+    if (!gV || !beta_v_out) return fail(EBEM_EINVAL, "bad argument");
+    HIPC(hipSetDevice(h->device));
+    const int N = h->N, nT = h->nT;
+    const int64_t n = (int64_t)N * nT;
+    std::vector<double> gT(3 * (size_t)n);
+    for (int F = 0; F < N; F++)
+        for (int t = 0; t < nT; t++)
+            for (int c = 0; c < 3; c++) gT[((int64_t)F * nT + t) * 3 + c] = gV[(int64_t)t * 3 * N + 3 * F + c];
+    HIPC(h->d_gVT.grow(3 * (size_t)n));
+    HIPC(hipMemcpyAsync(h->d_gVT.p, gT.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, h->stream));
+    if (int e = betav_device(h)) return e;
+    HIPC(hipMemcpyAsync(beta_v_out, h->d_betaV.p, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int ebem_mot_solve_neumann(ebem_handle* h, int datum, double* density_out) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (!h->assembled) return fail(EBEM_EINVAL, "ebem_assemble must run before ebem_mot_solve_neumann");
+    if (!h->k_ready) return fail(EBEM_EINVAL, "ebem_assemble_k must run before ebem_mot_solve_neumann");
+    if (datum != EBEM_DATUM_REF_BAR_H1 && datum != EBEM_DATUM_REF_BAR_H3)
+        return fail(EBEM_EINVAL, "Problema non codificato (datum %d has no Neumann trace)", datum);
+    HIPC(hipSetDevice(h->device));
+    if (int e = mot_neumann(h, datum)) return e;
+    if (density_out) {
+        HIPC(hipMemcpyAsync(density_out, h->d_dn.p, sizeof(double) * 3 * (size_t)h->Nv * h->nT,
+                            hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+    }
     return 0;
 }
 

@@ -194,6 +194,27 @@ int ebem_rhs_dirichlet(ebem_handle *h, int datum, double *beta_out);
 int ebem_pp_eval_direct(ebem_handle *h, const double *density, const double *g, int npts, const double *x, int nt,
                         const double *t, double *u_out);
 
+/* ---- Neumann driver (dir/BEMenerg_dir_timeMarchingN.m) -------------------- */
+
+/* Traction datum at the triangle centres, 3N x nT column-major, column n at
+ * time (n+1) dt: (0, 0, 2 mu tildeP n_z) of dir/BEMenerg_dir_calcBoundDataNeumann.m
+ * (datum 4 and 5, the bars). */
+int ebem_neumann_datum(ebem_handle *h, int datum, double *gV_out);
+
+/* beta_V (3N x nT column-major): beta_n = sum_{j<=n} V_{n-j} gV_j over the
+ * assembled single-layer blocks (dir/BEMenerg_dir_calcBetaVgpu.m:7-13). */
+int ebem_beta_v(ebem_handle *h, const double *gV, double *beta_v_out);
+
+/* Density (3Nv x nT column-major, at the vertices) of the Neumann driver:
+ * (K_0 + I_Gamma) d_n = beta_V,n + I_Gamma d_{n-1} - sum_{l=1}^{min(n,LK-1)} K_l d_{n-l},
+ * I_Gamma = kron((indSM > 0) .* area / 6, eye(3)), solved in the least-squares
+ * sense as MATLAB's backslash does for the rectangular system
+ * (timeMarchingN.m:170-296).  Needs ebem_assemble (V; the reference uses the
+ * numeric form EBEM_PREC_GHC) and ebem_assemble_k; kept on the device for
+ * ebem_pp_eval_direct (the Neumann post-processing passes the datum as
+ * `density` and this density as `g`, BEMenerg_ppN_calc.m:121). */
+int ebem_mot_solve_neumann(ebem_handle *h, int datum, double *density_out);
+
 /* MOT: factorise E_0 in FP64 on the GPU and march
  * alpha_n = E_0^{-1}(beta_n - sum_{l=1}^{min(n,L)-1} E_l alpha_{n-l})
  * (SA/BEMenerg_coreSA_timeMarching.m:45-113, FN/BEMenerg_coreFN_timeMarching.m:152-171).
@@ -234,8 +255,8 @@ int ebem_node_integral(ebem_handle *h, const double *x, int tri, double t, doubl
  * kernel, out[10] assembly between-front kernel, out[11] post-processing
  * between-front kernel, out[12] ebem_assemble_k total, out[13] beta_K kernel
  * of the last ebem_beta_k / ebem_rhs_dirichlet, out[14] ebem_pp_eval_direct
- * total. */
-#define EBEM_NUM_TIMINGS 15
+ * total, out[15] ebem_mot_solve_neumann total. */
+#define EBEM_NUM_TIMINGS 16
 int ebem_timings(ebem_handle *h, double *out, int n);
 
 /* ------------------------------------------------------------------------
