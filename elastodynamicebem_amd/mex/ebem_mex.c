@@ -114,6 +114,9 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
         int rc = ebem_pp_eval_direct(g_h, dens, gk, (int)k, xr, (int)nt, mxGetDoubles(prhs[4]), mxGetDoubles(plhs[0]));
         mxFree(xr);
         chk(rc);
+    } else if (!strcmp(cmd, "mot_neumann")) {
+        plhs[0] = mxCreateDoubleMatrix(3 * g_nv, g_nT, mxREAL);
+        chk(ebem_mot_solve_neumann(g_h, (int)scal(prhs[1]), mxGetDoubles(plhs[0])));
     } else if (!strcmp(cmd, "mot")) {
         plhs[0] = mxCreateDoubleMatrix(g_n3, g_nT, mxREAL);
         const double *beta = mxIsEmpty(prhs[1]) ? NULL : mxGetDoubles(prhs[1]);
@@ -136,7 +139,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     } else if (strcmp(cmd, "mesh") && strcmp(cmd, "problem") && strcmp(cmd, "rule") && strcmp(cmd, "inner_rule") &&
                strcmp(cmd, "pp_rule") && strcmp(cmd, "numblocks") && strcmp(cmd, "assemble") && strcmp(cmd, "block") &&
                strcmp(cmd, "rhs") && strcmp(cmd, "mot") && strcmp(cmd, "assemble_k") && strcmp(cmd, "kblock") &&
-               strcmp(cmd, "rhs_dirichlet") && strcmp(cmd, "pp_direct")) {
+               strcmp(cmd, "rhs_dirichlet") && strcmp(cmd, "pp_direct") && strcmp(cmd, "mot_neumann")) {
         mexErrMsgIdAndTxt("ebem:args", "unknown command '%s'", cmd);
     }
     (void)nlhs;
