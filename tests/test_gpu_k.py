@@ -182,3 +182,32 @@ def test_dd_bar_known_answer():
     assert cos1 > 0.99 and rel1 < 0.15
     cos2, rel2 = _dd_bar(2, 96)
     assert cos2 > 0.99 and rel2 < 0.8 * rel1
+
+
+def test_k_rows_sphere_wave():
+    """The reference's sphereWave problem (inputFiles/input_waveOnSphere.txt:
+    rho 1, mu 1, lambda 2 -- the kernelK.cu typo has weight --, t in [0, 12],
+    192 steps, 1488 triangles, 746 vertices), rule FN 7 16 3 64: sampled test
+    rows of every K_lag and of the direct Dirichlet right-hand side against
+    the oracle."""
+    from elastodynamicebem_amd import api
+    from elastodynamicebem_amd.mesh import read_mesh
+    from oracle import Oracle
+    m = read_mesh(mesh_path("sphereWave_1"))
+    pb = _pb(12.0, 192, rho=1.0, mu=1.0, lam=2.0, domain="sphereWave")
+    o = Oracle(m.coordinates, m.triangles, rho=1.0, mu=1.0, lam=2.0, Tfin=12.0, nT=192, nGH=7)
+    o.set_ghc(16, 3, 64)
+    with api.Solver(m, pb, 7) as s:
+        s.set_inner_rule(16, 3, 64)
+        LK = s.k_num_blocks()
+        assert LK == o.k_num_blocks()
+        s.assemble_k()
+        info = s.k_store_info()
+        worst = _check_k_rows(s, o, (700, 702), LK)
+        g = s.dirichlet_datum(api.DATUM_REF_SPHERE_WAVE)
+        bk = s.beta_k(g)
+        ms = s.timings()["k_assembly_ms"]
+    rows = (700, 702)
+    bo = o.beta_k(g, rows)[3 * rows[0]:3 * rows[1]]
+    assert frob_rel(bk[3 * rows[0]:3 * rows[1]], bo) < 1e-12
+    print(f"K sphereWave_1: LK {LK}, {info['slots']} blocks in {ms:.0f} ms, worst lag error {worst:.2e}")
