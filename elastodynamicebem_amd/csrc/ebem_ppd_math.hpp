@@ -114,7 +114,7 @@ __host__ __device__ inline void ppd_p_lane(const PPDArgs& A, const double* x, do
 // Time blocks per chunk of the double-layer post-processing: consecutive
 // blocks share two of their three levels and every point's geometry.
 #ifndef EBEM_PPD_CHUNK
-#define EBEM_PPD_CHUNK 4
+#define EBEM_PPD_CHUNK 8
 #endif
 constexpr int kPpdChunk = EBEM_PPD_CHUNK;
 
