@@ -39,6 +39,13 @@ __device__ __attribute__((always_inline)) inline void nucleo6(const double x[3],
     }
 }
 
+// Lags per wave of the numeric off-diagonal blocks: consecutive lags share
+// two of their three time levels.
+#ifndef EBEM_GHC_CHUNK
+#define EBEM_GHC_CHUNK 4
+#endif
+constexpr int kGhcChunk = EBEM_GHC_CHUNK;
+
 struct GhcRules {
     const double* inner;  // nInt x 4: barycentric b0, b1, b2 and weight (composite Gauss-Hammer)
     const double* diag;   // nDiag x 4: collapsed rule (doppioGauss1D)
@@ -75,12 +82,21 @@ __global__ void __launch_bounds__(256) k_ghc_blocks(const FieldTri* __restrict__
         const int32_t p = item_pair[it];
         const int i = r0 + p / P.N, j = p % P.N;
         const int l0 = lo[p];
-        const int lag = (l0 - 1 < 1 ? 1 : l0 - 1) + (int)(it - off[p]) - 1;
-        if (lag < l0) continue;  // wave-uniform
+        const int lag0 = (l0 - 1 < 1 ? 1 : l0 - 1) + (int)(it - off[p]) - 1;
+        if (lag0 < l0) continue;                      // wave-uniform
+        if ((lag0 - l0) % kGhcChunk != 0) continue;   // chunk leaders only (wave-uniform)
+        int nc = 1;  // the pair's items are consecutive, one lag each
+        while (nc < kGhcChunk && it + nc < nitems && item_pair[it + nc] == p) nc++;
         const double* g = gh + (int64_t)i * 128;
         const double* vf = verts + (int64_t)j * 9;
-        double acc[6] = {0, 0, 0, 0, 0, 0};
         if (i != j) {
+            // each level lag0-1 .. lag0+nc once, added to every lag of the chunk
+            // whose three levels contain it (per lag in kernelV.cu's order)
+            double acc[kGhcChunk][6];
+#pragma unroll
+            for (int u = 0; u < kGhcChunk; u++)
+#pragma unroll
+                for (int c = 0; c < 6; c++) acc[u][c] = 0.0;
             const double aF = tri[j].area;
             const double coef[3] = {1, -2, 1};
             for (int q = 0; q < NGH; q++) {
@@ -94,17 +110,37 @@ __global__ void __launch_bounds__(256) k_ghc_blocks(const FieldTri* __restrict__
                     const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
                     const double w = wq * (b[3] * aF);
 #pragma unroll
-                    for (int v = 0; v < 3; v++) {
-                        const double t = P.dt * (double)(lag + v - 1);
+                    for (int li = 0; li < kGhcChunk + 2; li++) {
+                        if (li >= nc + 2) break;
+                        const double t = P.dt * (double)(lag0 - 1 + li);
                         if (t < 0) continue;
                         double nu[6];
                         nucleo6(d, r, t, P.cP, P.cS, nu);
 #pragma unroll
-                        for (int c = 0; c < 6; c++) acc[c] += w * coef[v] * nu[c];
+                        for (int u = 0; u < kGhcChunk; u++) {
+                            const int v = li - u;
+                            if (u >= nc || v < 0 || v > 2) continue;
+#pragma unroll
+                            for (int c = 0; c < 6; c++) acc[u][c] += w * coef[v] * nu[c];
+                        }
                     }
                 }
             }
-        } else {
+#pragma unroll
+            for (int u = 0; u < kGhcChunk; u++) {
+                if (u >= nc) break;
+#pragma unroll
+                for (int c = 0; c < 6; c++) {
+                    double a = acc[u][c];
+                    for (int s = 32; s >= 1; s >>= 1) a += __shfl_xor(a, s);
+                    if (lane == 0) W[(it + u) * 6 + c] = a;
+                }
+            }
+            continue;
+        }
+        for (int u = 0; u < nc; u++) {
+            const int lag = lag0 + u;
+            double acc[6] = {0, 0, 0, 0, 0, 0};
             const int ca[3] = {0, 1, 2}, cb[3] = {2, 0, 1};
             const double coef[3] = {1, -2, 1};
             for (int v = 0; v < 3; v++) {
@@ -142,19 +178,11 @@ __global__ void __launch_bounds__(256) k_ghc_blocks(const FieldTri* __restrict__
                     acc[e] += coef[v] * lev[e];
                 }
             }
-            if (lane != 0) continue;
-            double* o = W + it * 6;
+            if (lane == 0) {
+                double* o = W + (it + u) * 6;
 #pragma unroll
-            for (int c = 0; c < 6; c++) o[c] = acc[c];
-            continue;
-        }
-#pragma unroll
-        for (int c = 0; c < 6; c++)
-            for (int s = 32; s >= 1; s >>= 1) acc[c] += __shfl_xor(acc[c], s);
-        if (lane == 0) {
-            double* o = W + it * 6;
-#pragma unroll
-            for (int c = 0; c < 6; c++) o[c] = acc[c];
+                for (int c = 0; c < 6; c++) o[c] = acc[c];
+            }
         }
     }
 }
