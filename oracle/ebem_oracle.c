@@ -16,6 +16,25 @@
  * argument pushed just past its domain by rounding) the argument is clamped,
  * which equals the real part MATLAB would carry to within rounding.
  *
+ * Also restated, beyond the SA path: the numeric inner quadrature of the
+ * reference's GPU path (kernelV.cu, calcSubBlockDiagV.m, kernelLayer.cu), the
+ * double-layer operator K of the direct drivers (kernelK.cu,
+ * calcSubBlockSingK.m, calcKIntgSing.m) and the direct post-processing
+ * u = G alpha - P g (kernelG.cu, kernelP.cu).
+ *
+ * What pins each part independently of its own formulas (tests/test_oracle*.py):
+ * SA -- Gauss-Hammer exactness, an independent polar quadrature of the node
+ * integrals, closed-form disk/annulus integrals, light-cone zeros, block
+ * symmetry, the RHS closed form; numeric form -- rule exactness, the kernel's
+ * trace identity, agreement with SA where the integrand is smooth; K -- the P1
+ * basis values, tensor covariance under rotations and reflections, late-time
+ * cancellation of every term; direct post-processing -- the rigid-translation
+ * (Somigliana) identity inside and outside a closed surface, tensor
+ * covariance, causality; the drivers (DD, DN; NumPy in pyoracle.py) -- the 1-D
+ * bar's closed-form solution.  Parity unpinned: the MATLAB runtime's sparse
+ * LU / QR (UMFPACK / SuiteSparseQR), replaced by LAPACK-style solves checked
+ * by residuals.
+ *
  * Build with -ffp-contract=off: MATLAB evaluates scalar expressions without
  * fused multiply-add.
  */
