@@ -44,14 +44,34 @@ ALG_FLOP_PER_ITEM = 8103.8
 K_LEVELS_TRAFFIC_BYTES = 117.5e9
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-# Algorithmic FP64 operations of the double-layer block kernel k_kblocks, from
-# the op-counting host build of its math (tests/native/k_opcount.cpp): per
-# regular evaluation point (one outer node, one inner node of one field
-# triangle) 218 for its geometry plus 385 per time level with t > 0 (the
-# kernel evaluates a point once for up to 4 consecutive lags of a pair, whose
-# levels overlap); per level of the singular sum 585 per evaluation point
-# (three children x nDiag nodes per outer node).
-K_REG_POINT_BASE, K_REG_POINT_LEVEL, K_SING_POINT_LEVEL = 218, 385, 585
+# Algorithmic FP64 operations of the double-layer assembly, from the
+# op-counting host build of its math (tests/native/k_opcount.cpp).  Regular
+# part (k_ksf, by field triangle): per evaluation point (one outer node, one
+# inner node of the field triangle F, F's three hat functions) and chunk of
+# nA <= 2 lags from lagA, 244 for its geometry and basis values, 1 per level
+# slot lagA-2 .. lagA+nA, 466 per level with t > 0 (R_j and KL + KT once,
+# combined for the three vertices) and 81 per (lag, level) term added.
+# Singular part (k_ksing): 585 per level and evaluation point (three children
+# x nDiag nodes per outer node).
+K_SF_POINT_BASE, K_SF_SLOT, K_SF_LEVEL, K_SF_TERM = 244, 1, 466, 81
+K_SING_POINT_LEVEL = 585
+
+
+def k_sf_point_flops(lagA, nA):
+    """Operations of one field-triangle evaluation point for the chunk of nA
+    lags from lagA (levels lag-2 .. lag+1 of each lag; level <= 0 skipped)."""
+    lagA = np.asarray(lagA)
+    nA = np.asarray(nA)
+    lev = np.zeros(np.broadcast(lagA, nA).shape)
+    terms = np.zeros_like(lev)
+    for li in range(5):
+        pos = (li < nA + 3) & (lagA - 2 + li >= 1)
+        lev += pos
+        for j in range(2):
+            d = li - j
+            if 0 <= d <= 3:
+                terms += pos & (j < nA)
+    return K_SF_POINT_BASE + K_SF_SLOT * (nA + 3) + K_SF_LEVEL * lev + K_SF_TERM * terms
 
 
 def workload():
@@ -187,13 +207,14 @@ def dd_workload(level=2):
     return m, pb
 
 
-def k_windows_and_flops(m, pb, LK, nGH, nInt, nDiag, rows=None, chunk=4):
+def k_windows_and_flops(m, pb, LK, nGH, nInt, nDiag, rows=None):
     """Slots of the K store (pair windows of isBlockNull and the singular
     cut-off, kernelK.cu:226-260, calcSubBlockSingK.m:12) and the algorithmic
-    FP64 operations of k_kblocks over them: per chunk of `chunk` consecutive
-    slots of a pair, every regular evaluation point once for the geometry and
-    once per distinct positive level of the chunk's kept lags; singular parts
-    per lag and level.  rows = (r0, r1) restricts the test triangles."""
+    FP64 operations of the assembly: the regular part by (test triangle S,
+    field triangle F != S) over the union of the windows of F's vertices in
+    chunks of 2 lags, each evaluation point once for its geometry and once per
+    positive level of the chunk; singular parts per lag and level.
+    rows = (r0, r1) restricts the test triangles."""
     X, T0 = m.coordinates, m.triangles
     r0, r1 = (0, T0.shape[0]) if rows is None else rows
     T = T0[r0:r1]
@@ -204,38 +225,33 @@ def k_windows_and_flops(m, pb, LK, nGH, nInt, nDiag, rows=None, chunk=4):
     dmax, dmin = d.max(1), d.min(1)
     own = np.zeros((N, Nv), bool)
     own[np.arange(N)[:, None], T] = True
-    deg = np.bincount(T0.ravel(), minlength=Nv)
-    nF = deg[None, :] - own  # field triangles F != S with vertex s
     cut = np.ceil(m.maxL[r0:r1] / (cS * dt)).astype(int) + 2
     big = 1 << 30
-    lo, hi = np.full((N, Nv), big), np.full((N, Nv), -1)
     rlo, rhi = np.full((N, Nv), big), np.full((N, Nv), -1)
     slots = 0
     sing_flops = 0.0
     for lag in range(LK):
         reg = ~(((lag - 2) * cS * dt > dmax + maxLen) | ((lag + 1) * cP * dt < dmin - 2 * maxLen))
         sing = own & (lag < cut[:, None])
-        keep = reg | sing
-        slots += int(np.count_nonzero(keep))
-        lo = np.where(keep, np.minimum(lo, lag), lo)
-        hi = np.where(keep, np.maximum(hi, lag + 1), hi)
+        slots += int(np.count_nonzero(reg | sing))
         rlo = np.where(reg, np.minimum(rlo, lag), rlo)
         rhi = np.where(reg, np.maximum(rhi, lag + 1), rhi)
         nlev = sum(1 for c in (-2, -1, 0, 1) if lag + c >= 1)
         sing_flops += float(sing.sum()) * nGH * 3 * nDiag * K_SING_POINT_LEVEL * nlev
+    # window of (S, F): lags where some vertex of F is not null against S
+    a = rlo[:, T0].min(-1)  # N x N_all
+    b = rhi[:, T0].max(-1)
+    S_glob = np.arange(r0, r1)
+    a[np.arange(N), S_glob] = 0
+    b[np.arange(N), S_glob] = 0
+    w = np.where(b > a, b - a, 0)
     reg_flops = 0.0
-    has = hi > lo
-    width = np.where(has, hi - lo, 0)
-    for c0 in range(0, int(width.max()) if width.size else 0, chunk):
-        a = lo + c0
-        b = np.minimum(a + chunk, hi)
-        ra = np.maximum(a, rlo)
-        rb = np.minimum(b, rhi)
-        ok = has & (c0 < width) & (rb > ra)
-        lev = np.maximum(0, rb - np.maximum(ra - 2, 1) + 1)  # levels ra-2 .. rb, positive
-        pts = nF * nGH * nInt
-        reg_flops += float((ok * pts * (K_REG_POINT_BASE + K_REG_POINT_LEVEL * lev)).sum())
-    return slots, reg_flops + sing_flops
+    for u in range(0, (int(w.max()) + 1) // 2 if w.size else 0):
+        ok = w > 2 * u
+        lagA = a + 2 * u
+        nA = np.minimum(2, b - lagA)
+        reg_flops += float((ok * k_sf_point_flops(np.where(ok, lagA, 0), np.where(ok, nA, 1))).sum())
+    return slots, reg_flops * nGH * nInt + sing_flops
 
 
 def dd_main(args):
@@ -301,7 +317,8 @@ def dd_main(args):
         "neumann_driver_ms": {"mot_neumann": neu_ms, "wall": neu_wall,
                               "note": "datum, beta_V, QR of K_0 + I_Gamma (rocSOLVER), 96 least-squares steps; "
                                       "outside the timed step"},
-        "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_kblocks", "achieved": ach,
+        "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "K assembly (k_ksf + k_kgather + k_ksing)",
+                     "achieved": ach,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
                      "traffic": None},
         "cpu_baseline": None,

@@ -184,6 +184,230 @@ __global__ void __launch_bounds__(256) k_kblocks(const KTri* __restrict__ kt, co
     }
 }
 
+// ---- regular part by field triangle: one kernel evaluation for the three
+// vertices of F, then a gather per (S, s, lag) over the triangles of s ----
+
+// Rows r0 + blockIdx.x: the lag window of every (S, F != S) pair, the union of
+// the lags isBlockNull keeps for F's three vertices; rowcnt[r] = partial slots
+// of the row, rowcnt[rows + r] = its chunks of kKsfChunk lags.
+__global__ void __launch_bounds__(256) k_ksf_win(const double* __restrict__ verts, const double* __restrict__ X,
+                                                 const int32_t* __restrict__ T, KParams K, int r0, int rows,
+                                                 int16_t* __restrict__ sflo, int16_t* __restrict__ sfhi,
+                                                 int64_t* __restrict__ rowcnt) {
+    __shared__ long long part[256], partc[256];
+    const int r = blockIdx.x, S = r0 + r;
+    const double* vS = verts + (int64_t)S * 9;
+    long long cnt = 0, chk = 0;
+    for (int F = threadIdx.x; F < K.N; F += blockDim.x) {
+        int a = K.LK, b = 0;
+        if (F != S)
+            for (int v = 0; v < 3; v++) {
+                const double* P = X + 3 * (int64_t)T[3 * F + v];
+                for (int lag = 0; lag < K.LK; lag++)
+                    if (!k_block_null(vS, P, lag, K)) {
+                        a = min(a, lag);
+                        b = max(b, lag + 1);
+                    }
+            }
+        if (b <= a) a = b = 0;
+        const int64_t q = (int64_t)r * K.N + F;
+        sflo[q] = (int16_t)a;
+        sfhi[q] = (int16_t)b;
+        cnt += b - a;
+        chk += (b - a + kKsfChunk - 1) / kKsfChunk;
+    }
+    part[threadIdx.x] = cnt;
+    partc[threadIdx.x] = chk;
+    __syncthreads();
+    for (int d = 128; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d) {
+            part[threadIdx.x] += part[threadIdx.x + d];
+            partc[threadIdx.x] += partc[threadIdx.x + d];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        rowcnt[r] = part[0];
+        rowcnt[rows + r] = partc[0];
+    }
+}
+
+// Per row: partial-slot offsets pOff of the (S, F) pairs and the chunk items
+// (pair index, chunk number); rowbase[r], rowbase[rows + r] the row's bases.
+__global__ void __launch_bounds__(256) k_ksf_items(const int16_t* __restrict__ sflo, const int16_t* __restrict__ sfhi,
+                                                   int N, int rows, const int64_t* __restrict__ rowbase,
+                                                   int64_t* __restrict__ pOff, int32_t* __restrict__ itemP,
+                                                   int16_t* __restrict__ itemC) {
+    __shared__ long long part[256], partc[256];
+    const int r = blockIdx.x;
+    const int nper = (N + blockDim.x - 1) / blockDim.x;
+    const int j0 = threadIdx.x * nper;
+    const int64_t pbase = (int64_t)r * N;
+    long long s = 0, c = 0;
+    for (int k = 0; k < nper; k++) {
+        const int j = j0 + k;
+        if (j < N) {
+            const int w = sfhi[pbase + j] - sflo[pbase + j];
+            s += w;
+            c += (w + kKsfChunk - 1) / kKsfChunk;
+        }
+    }
+    part[threadIdx.x] = s;
+    partc[threadIdx.x] = c;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        long long v = ((int)threadIdx.x >= d) ? part[threadIdx.x - d] : 0;
+        long long vc = ((int)threadIdx.x >= d) ? partc[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        partc[threadIdx.x] += vc;
+        __syncthreads();
+    }
+    long long run = rowbase[r] + (threadIdx.x ? part[threadIdx.x - 1] : 0);
+    long long runc = rowbase[rows + r] + (threadIdx.x ? partc[threadIdx.x - 1] : 0);
+    for (int k = 0; k < nper; k++) {
+        const int j = j0 + k;
+        if (j >= N) break;
+        const int64_t q = pbase + j;
+        const int w = sfhi[q] - sflo[q];
+        pOff[q] = run;
+        const int nc = (w + kKsfChunk - 1) / kKsfChunk;
+        for (int u = 0; u < nc; u++) {
+            itemP[runc + u] = (int32_t)q;
+            itemC[runc + u] = (int16_t)u;
+        }
+        run += w;
+        runc += nc;
+    }
+}
+
+// One wave per chunk item: the partial sums of pair (S, F) for F's three
+// vertices and the item's lags, butterflied, raw (before clean-up):
+// part[(pOff + lag - sflo) * 27 + 9 v + (i + 3 c)].
+__global__ void __launch_bounds__(256) k_ksf(const KTri* __restrict__ kt, const double* __restrict__ verts,
+                                             const double* __restrict__ gh, const double* __restrict__ rin_g, int nGH,
+                                             KParams K, int r0, int64_t nitems, const int32_t* __restrict__ itemP,
+                                             const int16_t* __restrict__ itemC, const int16_t* __restrict__ sflo,
+                                             const int16_t* __restrict__ sfhi, const int64_t* __restrict__ pOff,
+                                             double* __restrict__ part) {
+#pragma clang fp contract(off)
+    extern __shared__ double rule[];  // nInt x 4
+    for (int k = threadIdx.x; k < 4 * K.nInt; k += blockDim.x) rule[k] = rin_g[k];
+    __syncthreads();
+    KBlockArgs A{kt, verts, nullptr, nullptr, rule, rule, nullptr, nGH};
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t it = wave; it < nitems; it += nwaves) {
+        const int32_t q = itemP[it];
+        const int S = r0 + q / K.N, F = q % K.N;
+        const int lagA = sflo[q] + kKsfChunk * itemC[it];
+        const int nA = min(kKsfChunk, (int)sfhi[q] - lagA);
+        A.g = gh + (int64_t)S * 128;
+        double acc[3][kKsfChunk][9];
+        k_sf_lane(A, K, F, lagA, nA, lane, 64, acc);
+        double* o = part + (pOff[q] + (lagA - sflo[q])) * 27;
+#pragma unroll
+        for (int j = 0; j < kKsfChunk; j++) {
+            if (j >= nA) break;
+#pragma unroll
+            for (int v = 0; v < 3; v++)
+#pragma unroll
+                for (int e = 0; e < 9; e++) {
+                    double a = acc[v][j][e];
+                    for (int m = 32; m >= 1; m >>= 1) a += __shfl_xor(a, m);
+                    if (lane == 0) o[j * 27 + v * 9 + e] = a;
+                }
+        }
+    }
+}
+
+// One thread per store slot of rows [r0, r0 + rows): the regular block
+// (S, s, lag) = sum over the triangles F != S of s (ascending) of their
+// partial sums, zero where isBlockNull drops it, cleaned below 1e-14 and
+// divided by 4 pi dt (kernelK.cu:212-221).
+__global__ void k_kgather(const double* __restrict__ verts, const double* __restrict__ X, KParams K, int r0,
+                          int64_t s0, int64_t ns, const int32_t* __restrict__ slot_pair,
+                          const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
+                          const int32_t* __restrict__ vadj_off, const int32_t* __restrict__ vadj,
+                          const int16_t* __restrict__ sflo, const int16_t* __restrict__ sfhi,
+                          const int64_t* __restrict__ pOff, const double* __restrict__ part, double* __restrict__ val) {
+#pragma clang fp contract(off)
+    const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (id >= ns) return;
+    const int64_t slot = s0 + id;
+    const int32_t p = slot_pair[slot];
+    const int S = p / K.Nv, s = p % K.Nv;
+    const int lag = lo[p] + (int)(slot - off[p]);
+    double o[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!k_block_null(verts + (int64_t)S * 9, X + 3 * (int64_t)s, lag, K)) {
+        double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int e = vadj_off[s]; e < vadj_off[s + 1]; e++) {
+            const int F = vadj[e] >> 2, v = vadj[e] & 3;
+            if (F == S) continue;
+            const int64_t q = (int64_t)(S - r0) * K.N + F;
+            if (lag < sflo[q] || lag >= sfhi[q]) continue;
+            const double* pp = part + (pOff[q] + (lag - sflo[q])) * 27 + v * 9;
+            for (int c = 0; c < 9; c++) a[c] += pp[c];
+        }
+        for (int c = 0; c < 9; c++) o[c] = fabs(a[c]) > 1.0e-14 ? a[c] / K.fourpidt : 0.0;
+    }
+    double* out = val + slot * 9;
+    for (int c = 0; c < 9; c++) out[c] = o[c];
+}
+
+// One wave per (test triangle S, local vertex v): the singular part of the
+// blocks (S, T[S][v], lag) for the lags of the pair's window below the
+// cut-off, per lag and level (calcSubBlockSingK.m:17-39), added to the store.
+__global__ void __launch_bounds__(256) k_ksing(const KTri* __restrict__ kt, const FieldTri* __restrict__ tri,
+                                               const double* __restrict__ verts, const double* __restrict__ gh,
+                                               const int32_t* __restrict__ T, const double* __restrict__ rdg_g, int nGH,
+                                               KParams K, const int64_t* __restrict__ off,
+                                               const int16_t* __restrict__ lo, const int16_t* __restrict__ hi,
+                                               double* __restrict__ val) {
+#pragma clang fp contract(off)
+    extern __shared__ double rule[];  // nDiag x 4
+    for (int k = threadIdx.x; k < 4 * K.nDiag; k += blockDim.x) rule[k] = rdg_g[k];
+    __syncthreads();
+    KBlockArgs A{kt, verts, nullptr, nullptr, rule, rule, nullptr, nGH};
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int ct[4] = {-2, -1, 0, 1};
+    const double cn[4] = {-1, 3, -3, 1};
+    for (int64_t it = wave; it < 3 * (int64_t)K.N; it += nwaves) {
+        const int S = (int)(it / 3), vl = (int)(it % 3);
+        const int s = T[3 * S + vl];
+        const int64_t p = (int64_t)S * K.Nv + s;
+        const int cut = min(k_sing_cut(tri[S].maxL, K), (int)hi[p]);
+        A.g = gh + (int64_t)S * 128;
+        for (int lag = lo[p]; lag < cut; lag++) {
+            double blk[3][3];
+            for (int i = 0; i < 3; i++)
+                for (int c = 0; c < 3; c++) blk[i][c] = 0.0;
+            for (int l = 0; l < 4; l++) {
+                const int il = lag + ct[l];
+                if (il <= 0) continue;
+                double lev[3][3];
+                k_sing_lane(A, K, S, vl, (double)il * K.dt, lane, 64, lev);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        double a = lev[i][c];
+                        for (int m = 32; m >= 1; m >>= 1) a += __shfl_xor(a, m);
+                        blk[i][c] += cn[l] * a;
+                    }
+            }
+            if (lane == 0) {
+                double* o = val + (off[p] + (lag - lo[p])) * 9;
+                for (int i = 0; i < 3; i++)
+                    for (int c = 0; c < 3; c++) o[i + 3 * c] += blk[i][c] / K.fourpidt;
+            }
+        }
+    }
+}
+
 // Dirichlet datum at the vertices (BEMenerg_dir_calcBoundDataDirichlet.m):
 // gT[(s * nT + n) * 3 + c] = g_c(X_s, (n + 1) dt).
 __global__ void k_dir_datum(const double* __restrict__ X, int Nv, int nT, double dt, double cP, double Tfin,

@@ -373,6 +373,144 @@ __host__ __device__ inline void k_sing_lane(const KBlockArgs& A, const KParams& 
     }
 }
 
+// The vertex-independent parts of the regular kernel (k_kernel<true>) at time
+// t: the R_j entries R[j][i][k] (rows i == j unused) and KL + KT.  For a hat
+// function with w = n x grad phi and value phi,
+//   ((0 - w_0 R_0) - w_1 R_1) - w_2 R_2 + (KL + KT) phi   (terms j == i left out)
+// equals k_kernel<true>'s entry bit for bit.
+__host__ __device__ __attribute__((always_inline)) inline void k_kernel_parts(const KGeom& G, double t,
+                                                                              const KParams& K, double R[3][3][3],
+                                                                              double LT[3][3]) {
+#pragma clang fp contract(off)
+    const double r = G.r, cP = K.cP, cS = K.cS;
+    const double HP = (t - (r / cP)) > 0 ? 1.0 : 0.0;
+    const double HS = (t - (r / cS)) > 0 ? 1.0 : 0.0;
+    const double HPl = (r - (r / cP)) > 0 ? 1.0 : 0.0;
+    const double Hcub = (t - (r / cS)) * (t - (r / cS)) * (t + (2 * r / cS)) / 6 * HS -
+                        (t - (r / cP)) * (t - (r / cP)) * (t + (2 * r / cP)) / 6 * HP;
+    const double Dlin = ((t - (r / cS)) * HS / (cS * cS)) - ((t - (r / cP)) * HP / (cP * cP));
+    const double Dlam = ((t - (r / cS)) * HS / (cS * cS)) - ((t - (r / cP)) * HPl / (cP * cP));
+    const double a = 2 * K.mu / K.rho;
+    const double b = K.lam * K.mu / (K.rho * (K.lam + K.mu));
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (i == j) continue;
+            const int q = 3 - i - j;
+            const double e = ((i - j + 3) % 3 == 1) ? 1.0 : -1.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double hp = a * (e * G.g5[q][k]) * Hcub;
+                const double dp = a * (e * G.g3[q][k]) * Dlin;
+                R[j][i][k] = (k != i && k != j) ? b * (-e) / r * Dlam + hp + dp : hp + dp;
+            }
+        }
+    const double fL = (t - (r / cP)) * HP - (t - (r / cS)) * HS;
+    const double fT = (HP / cP) - (HS / cS);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double L = G.a3[i][k] * fL + G.b3[i][k] * fL;
+            double T = G.a2[i][k] * fT + G.b2[i][k] * fT;
+            if (i == k) {
+                L = L + G.d3 * (t - (r / cS)) * HS;
+                T = T + G.d2 * (HS / cS);
+            }
+            LT[i][k] = L + T;
+        }
+}
+
+// Regular kernel for the three hat functions of one field triangle at once:
+// out[v] equals k_kernel<true>(G, t, w[v], phi[v], K, .) bit for bit.
+__host__ __device__ __attribute__((always_inline)) inline void k_kernel3(const KGeom& G, double t,
+                                                                         const double w[3][3], const double phi[3],
+                                                                         const KParams& K, double out[3][3][3]) {
+#pragma clang fp contract(off)
+    double R[3][3][3], LT[3][3];
+    k_kernel_parts(G, t, K, R, LT);
+#pragma unroll
+    for (int v = 0; v < 3; v++)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double tmp = 0.0;
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    if (j != i) tmp = tmp - w[v][j] * R[j][i][k];
+                out[v][i][k] = tmp + LT[i][k] * phi[v];
+            }
+}
+
+// Lags per wave of the field-triangle items of the regular K part.
+#ifndef EBEM_KSF_CHUNK
+#define EBEM_KSF_CHUNK 2
+#endif
+constexpr int kKsfChunk = EBEM_KSF_CHUNK;
+
+// Lane `lane` of `nl`: the regular sums of test triangle S against field
+// triangle F != S for the hat functions of F's three vertices and the nA <=
+// kKsfChunk lags lagA .. lagA+nA-1 (kernelK.cu:63-160 restricted to F): outer
+// nodes of S, inner nodes k = lane, lane + nl, ... of F, each point's
+// geometry once, each level lagA-2 .. lagA+nA once, its value added to every
+// lag whose four levels contain it with that lag's weight -1, 3, -3, 1.
+__host__ __device__ inline void k_sf_lane(const KBlockArgs& A, const KParams& K, int F, int lagA, int nA, int lane,
+                                          int nl, double acc[3][kKsfChunk][9]) {
+#pragma clang fp contract(off)
+    const double cn[4] = {-1, 3, -3, 1};
+    for (int v = 0; v < 3; v++)
+        for (int j = 0; j < kKsfChunk; j++)
+            for (int e = 0; e < 9; e++) acc[v][j][e] = 0.0;
+    const KTri& B = A.kt[F];
+    double w[3][3];
+    for (int v = 0; v < 3; v++) {
+        const double* gv = B.mat + 3 * v;
+        w[v][0] = +B.n[1] * gv[2] - B.n[2] * gv[1];
+        w[v][1] = -B.n[0] * gv[2] + B.n[2] * gv[0];
+        w[v][2] = +B.n[0] * gv[1] - B.n[1] * gv[0];
+    }
+    const double* vf = A.verts + (int64_t)F * 9;
+    const double aF = B.area;
+    for (int q = 0; q < A.nGH; q++) {
+        const double xq[3] = {A.g[q], A.g[32 + q], A.g[64 + q]};
+        const double wq = A.g[96 + q];
+        for (int k = lane; k < K.nInt; k += nl) {
+            const double* bb = A.rin + 4 * k;
+            double y[3];
+            for (int c = 0; c < 3; c++) y[c] = bb[0] * vf[c] + bb[1] * vf[3 + c] + bb[2] * vf[6 + c];
+            const double pw = wq * (bb[3] * aF);
+            const double x[3] = {xq[0] - y[0], xq[1] - y[1], xq[2] - y[2]};
+            const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+            double phi[3];
+            for (int v = 0; v < 3; v++)
+                phi[v] = B.vec[v] + B.mat[3 * v] * y[0] + B.mat[3 * v + 1] * y[1] + B.mat[3 * v + 2] * y[2];
+            KGeom G;
+            k_geom(x, r, B.n, K, G);
+#pragma unroll
+            for (int li = 0; li < kKsfChunk + 3; li++) {
+                if (li >= nA + 3) break;
+                const double t = K.dt * (double)(lagA - 2 + li);
+                if (t <= 0) continue;
+                double kv[3][3][3];
+                k_kernel3(G, t, w, phi, K, kv);
+#pragma unroll
+                for (int j = 0; j < kKsfChunk; j++) {
+                    const int d = li - j;
+                    if (j >= nA || d < 0 || d > 3) continue;
+#pragma unroll
+                    for (int v = 0; v < 3; v++)
+#pragma unroll
+                        for (int i = 0; i < 3; i++)
+#pragma unroll
+                            for (int c = 0; c < 3; c++) acc[v][j][i + 3 * c] += pw * cn[d] * kv[v][i][c];
+                }
+            }
+        }
+    }
+}
+
 // Local vertex of s in S (or -1).
 __host__ __device__ inline int k_local_vertex(const int32_t* T, int S, int s) {
     return (T[3 * S] == s) ? 0 : (T[3 * S + 1] == s) ? 1 : (T[3 * S + 2] == s) ? 2 : -1;

@@ -463,6 +463,7 @@ struct ebem_handle {
     bool have_dn = false;
     bool sort_items = true;
     bool kp_split = true;  // kP ranges through k_levels_kp (EBEM_KP_SPLIT=0 turns it off)
+    bool k_by_field = true;  // regular K part by (S, F) pairs + gather (EBEM_K_BY_FIELD=0: by (S, s) slots)
     // kP and between-front ranges of the last assembly / post-processing:
     // kP ranges, kP levels, between-front ranges, their levels, fallbacks
     int64_t kp_stats[5] = {0, 0, 0, 0, 0};
@@ -1527,7 +1528,7 @@ int assemble_k(ebem_handle* h) {
     hipLaunchKernelGGL(k_kslots, dim3(N), dim3(256), 0, h->stream, h->d_klo.p, h->d_khi.p, Nv, d_rb.p, h->d_koff.p,
                        h->d_kslot.p);
     HIPC(hipGetLastError());
-    if (slots > 0) {
+    if (slots > 0 && !h->k_by_field) {
         const size_t lds = sizeof(double) * 4 * (size_t)(K.nInt + K.nDiag);
         if (lds > 160 * 1024) return fail(EBEM_EINVAL, "inner rules too large for LDS");
         int64_t blocks = std::min<int64_t>((slots + 3) / 4, (int64_t)h->CUs * 8);
@@ -1536,6 +1537,85 @@ int assemble_k(ebem_handle* h) {
                            h->d_verts.p, h->d_gh.p, h->d_X.p, h->d_T.p, h->d_vadj_off.p, h->d_vadj.p, h->d_ghc_in.p,
                            h->d_ghc_dg.p, h->nGH, K, slots, h->d_kslot.p, h->d_koff.p, h->d_klo.p, h->d_khi.p,
                            h->d_kval.p);
+        HIPC(hipGetLastError());
+    } else if (slots > 0) {
+        // regular part by (S, F) pairs: windows of all rows, then row chunks
+        // bounded by the partial-sum buffer, each gathered into the store
+        DevBuf<int16_t> sflo, sfhi, itemC;
+        DevBuf<int64_t> sfrow, pOff, rb;
+        DevBuf<int32_t> itemP;
+        DevBuf<double> part;
+        const int64_t nsf = (int64_t)N * N;
+        HIPC(sflo.alloc(nsf));
+        HIPC(sfhi.alloc(nsf));
+        HIPC(sfrow.alloc(2 * (size_t)N));
+        hipLaunchKernelGGL(k_ksf_win, dim3(N), dim3(256), 0, h->stream, h->d_verts.p, h->d_X.p, h->d_T.p, K, 0, N,
+                           sflo.p, sfhi.p, sfrow.p);
+        HIPC(hipGetLastError());
+        std::vector<int64_t> src(2 * (size_t)N);
+        HIPC(hipMemcpyAsync(src.data(), sfrow.p, sizeof(int64_t) * 2 * N, hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        size_t freeb = 0, totb = 0;
+        HIPC(hipMemGetInfo(&freeb, &totb));
+        int64_t cap = (int64_t)(0.3 * (double)freeb / (27 * 8));
+        int64_t maxrow = 0;
+        for (int S = 0; S < N; S++) maxrow = std::max(maxrow, src[S]);
+        if (cap < maxrow) cap = maxrow;
+        const size_t lds_in = sizeof(double) * 4 * (size_t)K.nInt;
+        if (lds_in > 160 * 1024) return fail(EBEM_EINVAL, "inner rule too large for LDS");
+        HIPC(hipFuncSetAttribute((const void*)k_ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_in));
+        int r0 = 0;
+        while (r0 < N) {
+            int r1 = r0;
+            int64_t acc = 0, accI = 0;
+            while (r1 < N && (r1 == r0 || acc + src[r1] <= cap)) {
+                acc += src[r1];
+                accI += src[N + r1];
+                r1++;
+            }
+            const int rows = r1 - r0;
+            std::vector<int64_t> base(2 * (size_t)rows);
+            int64_t run = 0, runI = 0;
+            for (int r = 0; r < rows; r++) {
+                base[r] = run;
+                base[rows + r] = runI;
+                run += src[r0 + r];
+                runI += src[N + r0 + r];
+            }
+            HIPC(rb.grow(2 * (size_t)rows));
+            HIPC(hipMemcpyAsync(rb.p, base.data(), sizeof(int64_t) * 2 * rows, hipMemcpyHostToDevice, h->stream));
+            HIPC(pOff.grow((size_t)rows * N));
+            HIPC(itemP.grow(std::max<int64_t>(accI, 1)));
+            HIPC(itemC.grow(std::max<int64_t>(accI, 1)));
+            HIPC(part.grow(27 * (size_t)std::max<int64_t>(acc, 1)));
+            const int16_t* lo_c = sflo.p + (int64_t)r0 * N;
+            const int16_t* hi_c = sfhi.p + (int64_t)r0 * N;
+            hipLaunchKernelGGL(k_ksf_items, dim3(rows), dim3(256), 0, h->stream, lo_c, hi_c, N, rows, rb.p, pOff.p,
+                               itemP.p, itemC.p);
+            HIPC(hipGetLastError());
+            if (accI > 0) {
+                const int64_t blocks = std::min<int64_t>((accI + 3) / 4, (int64_t)h->CUs * 8);
+                hipLaunchKernelGGL(k_ksf, dim3((unsigned)blocks), dim3(256), lds_in, h->stream, h->d_ktri.p,
+                                   h->d_verts.p, h->d_gh.p, h->d_ghc_in.p, h->nGH, K, r0, accI, itemP.p, itemC.p,
+                                   lo_c, hi_c, pOff.p, part.p);
+                HIPC(hipGetLastError());
+            }
+            const int64_t s0 = h->k_rowbase[r0], ns = h->k_rowbase[r1] - s0;
+            if (ns > 0) {
+                hipLaunchKernelGGL(k_kgather, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, h->stream,
+                                   h->d_verts.p, h->d_X.p, K, r0, s0, ns, h->d_kslot.p, h->d_koff.p, h->d_klo.p,
+                                   h->d_vadj_off.p, h->d_vadj.p, lo_c, hi_c, pOff.p, part.p, h->d_kval.p);
+                HIPC(hipGetLastError());
+            }
+            HIPC(hipStreamSynchronize(h->stream));  // part, pOff and the item map are reused by the next chunk
+            r0 = r1;
+        }
+        const size_t lds_dg = sizeof(double) * 4 * (size_t)K.nDiag;
+        HIPC(hipFuncSetAttribute((const void*)k_ksing, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_dg));
+        const int64_t blocks = std::min<int64_t>((3 * (int64_t)N + 3) / 4, (int64_t)h->CUs * 8);
+        hipLaunchKernelGGL(k_ksing, dim3((unsigned)blocks), dim3(256), lds_dg, h->stream, h->d_ktri.p, h->d_tri.p,
+                           h->d_verts.p, h->d_gh.p, h->d_T.p, h->d_ghc_dg.p, h->nGH, K, h->d_koff.p, h->d_klo.p,
+                           h->d_khi.p, h->d_kval.p);
         HIPC(hipGetLastError());
     }
     tall.stop(h->stream);
@@ -1680,6 +1760,8 @@ int ebem_create(ebem_handle** out, int device) {
     h->sort_items = !(order && std::strcmp(order, "pair") == 0);
     const char* kps = std::getenv("EBEM_KP_SPLIT");
     h->kp_split = !(kps && std::strcmp(kps, "0") == 0);
+    const char* kbf = std::getenv("EBEM_K_BY_FIELD");
+    h->k_by_field = !(kbf && std::strcmp(kbf, "0") == 0);
     *out = h;
     return 0;
 }

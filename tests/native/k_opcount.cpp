@@ -86,6 +86,32 @@ long long k_opcount_reg_point(int lag) {
     return g_ops;
 }
 
+// Flops of one evaluation point of the field-triangle form (one outer node,
+// one inner node of F, F's three hat functions) over the levels of nA lags
+// from lagA.
+long long k_opcount_sf_point(int lagA, int nA) {
+    KTri kt[2];
+    const cdouble n[3] = {0, 0, 1};
+    k_tri_basis(kX + 0, kX + 3, kX + 6, n, 0.5, kt[0]);
+    k_tri_basis(kX + 0, kX + 9, kX + 12, n, 0.5, kt[1]);
+    cdouble verts[18];
+    for (int t = 0; t < 2; t++)
+        for (int v = 0; v < 3; v++)
+            for (int c = 0; c < 3; c++) verts[9 * t + 3 * v + c] = kX[3 * kT[3 * t + v] + c];
+    const cdouble rin[4] = {1.0 / 3, 1.0 / 3, 1.0 / 3, 1.0};
+    cdouble g[128];
+    for (int k = 0; k < 128; k++) g[k] = 0.0;
+    g[0] = 0.3, g[32] = 0.3, g[64] = 0.0, g[96] = 0.5;
+    KParams K{};
+    K.N = 2, K.Nv = 5, K.LK = 64, K.nInt = 1, K.nDiag = 1;
+    K.dt = 0.1, K.cP = 2, K.cS = 1, K.lam = 2, K.mu = 1, K.rho = 1, K.maxLen = 1.5, K.fourpidt = 1.2566;
+    KBlockArgs A{kt, verts, nullptr, nullptr, rin, rin, g, 1};
+    cdouble acc[3][kKsfChunk][9];
+    g_ops = 0;
+    k_sf_lane(A, K, 1, lagA, nA, 0, 1, acc);
+    return g_ops;
+}
+
 // Flops of one level of the singular sum for one outer node and one node of
 // the collapsed rule in each of the three children (3 evaluation points).
 long long k_opcount_sing_level() {

@@ -134,3 +134,22 @@ def test_chunked_regular_sum_is_bitwise_the_per_lag_sum(host_k, sphere2):
                                              _p(o.ghn), _p(o.ghw), rin.shape[0], _p(rin), lagA, nA, S, s,
                                              _p(out)) == 0
         assert np.array_equal(out[:9 * nA], out[9 * nA:]), (S, s, lagA, nA)
+
+
+def test_bench_k_op_counts_match_the_counting_build():
+    """The per-point operation counts behind the K roofline in bench.py equal
+    the op-counting host build of the device math (every double arithmetic
+    operation and sqrt counted once)."""
+    import sys
+    sys.path.insert(0, os.path.dirname(HERE))
+    import bench
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "native")], check=True)
+    lib = C.CDLL(os.path.join(HERE, "native", "libk_opcount.so"))
+    lib.k_opcount_sf_point.argtypes = [C.c_int, C.c_int]
+    lib.k_opcount_sf_point.restype = C.c_longlong
+    lib.k_opcount_sing_level.restype = C.c_longlong
+    for lagA in range(0, 9):
+        for nA in (1, 2):
+            assert int(bench.k_sf_point_flops(lagA, nA)) == lib.k_opcount_sf_point(lagA, nA), (lagA, nA)
+    # one outer node, one collapsed-rule node in each of the three children
+    assert 3 * bench.K_SING_POINT_LEVEL == lib.k_opcount_sing_level()
