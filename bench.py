@@ -46,32 +46,23 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # Algorithmic FP64 operations of the double-layer assembly, from the
 # op-counting host build of its math (tests/native/k_opcount.cpp).  Regular
-# part (k_ksf, by field triangle): per evaluation point (one outer node, one
-# inner node of the field triangle F, F's three hat functions) and chunk of
-# nA <= 2 lags from lagA, 244 for its geometry and basis values, 1 per level
-# slot lagA-2 .. lagA+nA, 466 per level with t > 0 (R_j and KL + KT once,
-# combined for the three vertices) and 81 per (lag, level) term added.
-# Singular part (k_ksing): 585 per level and evaluation point (three children
-# x nDiag nodes per outer node).
-K_SF_POINT_BASE, K_SF_SLOT, K_SF_LEVEL, K_SF_TERM = 244, 1, 466, 81
+# part (k_ksf, by field triangle, level sums): per evaluation point (one outer
+# node, one inner node of the field triangle F, F's three hat functions) and
+# chunk of nL <= 2 levels, 244 for its geometry and basis values and 521 per
+# level (R_j and KL + KT once, combined for the three vertices, 27 sums).  The
+# gather's combination -1, 3, -3, 1 of the level sums (at most 81 per slot
+# and triangle, ~1e-4 of the total) is not counted.  Singular part (k_ksing):
+# 585 per level and evaluation point (three children x nDiag nodes per outer
+# node).
+K_SF_POINT_BASE, K_SF_LEVEL = 244, 521
 K_SING_POINT_LEVEL = 585
+K_SF_CHUNK = 2
 
 
-def k_sf_point_flops(lagA, nA):
-    """Operations of one field-triangle evaluation point for the chunk of nA
-    lags from lagA (levels lag-2 .. lag+1 of each lag; level <= 0 skipped)."""
-    lagA = np.asarray(lagA)
-    nA = np.asarray(nA)
-    lev = np.zeros(np.broadcast(lagA, nA).shape)
-    terms = np.zeros_like(lev)
-    for li in range(5):
-        pos = (li < nA + 3) & (lagA - 2 + li >= 1)
-        lev += pos
-        for j in range(2):
-            d = li - j
-            if 0 <= d <= 3:
-                terms += pos & (j < nA)
-    return K_SF_POINT_BASE + K_SF_SLOT * (nA + 3) + K_SF_LEVEL * lev + K_SF_TERM * terms
+def k_sf_point_flops(mA, nL):
+    """Operations of one field-triangle evaluation point for the chunk of nL
+    levels from mA."""
+    return K_SF_POINT_BASE + K_SF_LEVEL * np.asarray(nL)
 
 
 def workload():
@@ -211,9 +202,10 @@ def k_windows_and_flops(m, pb, LK, nGH, nInt, nDiag, rows=None):
     """Slots of the K store (pair windows of isBlockNull and the singular
     cut-off, kernelK.cu:226-260, calcSubBlockSingK.m:12) and the algorithmic
     FP64 operations of the assembly: the regular part by (test triangle S,
-    field triangle F != S) over the union of the windows of F's vertices in
-    chunks of 2 lags, each evaluation point once for its geometry and once per
-    positive level of the chunk; singular parts per lag and level.
+    field triangle F != S) over the positive levels of the union of the
+    windows of F's vertices, in chunks of 2 levels, each evaluation point once
+    per chunk for its geometry and once per level; singular parts per lag and
+    level.
     rows = (r0, r1) restricts the test triangles."""
     X, T0 = m.coordinates, m.triangles
     r0, r1 = (0, T0.shape[0]) if rows is None else rows
@@ -244,13 +236,9 @@ def k_windows_and_flops(m, pb, LK, nGH, nInt, nDiag, rows=None):
     S_glob = np.arange(r0, r1)
     a[np.arange(N), S_glob] = 0
     b[np.arange(N), S_glob] = 0
-    w = np.where(b > a, b - a, 0)
-    reg_flops = 0.0
-    for u in range(0, (int(w.max()) + 1) // 2 if w.size else 0):
-        ok = w > 2 * u
-        lagA = a + 2 * u
-        nA = np.minimum(2, b - lagA)
-        reg_flops += float((ok * k_sf_point_flops(np.where(ok, lagA, 0), np.where(ok, nA, 1))).sum())
+    nlev = np.where(b > a, b - np.maximum(a - 2, 1) + 1, 0)
+    nchunk = (nlev + K_SF_CHUNK - 1) // K_SF_CHUNK
+    reg_flops = float((nchunk * K_SF_POINT_BASE + nlev * K_SF_LEVEL).sum())
     return slots, reg_flops * nGH * nInt + sing_flops
 
 

@@ -213,8 +213,9 @@ __global__ void __launch_bounds__(256) k_ksf_win(const double* __restrict__ vert
         const int64_t q = (int64_t)r * K.N + F;
         sflo[q] = (int16_t)a;
         sfhi[q] = (int16_t)b;
-        cnt += b - a;
-        chk += (b - a + kKsfChunk - 1) / kKsfChunk;
+        const int nlev = k_sf_nlev(a, b);
+        cnt += nlev;
+        chk += (nlev + kKsfChunk - 1) / kKsfChunk;
     }
     part[threadIdx.x] = cnt;
     partc[threadIdx.x] = chk;
@@ -247,7 +248,7 @@ __global__ void __launch_bounds__(256) k_ksf_items(const int16_t* __restrict__ s
     for (int k = 0; k < nper; k++) {
         const int j = j0 + k;
         if (j < N) {
-            const int w = sfhi[pbase + j] - sflo[pbase + j];
+            const int w = k_sf_nlev(sflo[pbase + j], sfhi[pbase + j]);
             s += w;
             c += (w + kKsfChunk - 1) / kKsfChunk;
         }
@@ -269,7 +270,7 @@ __global__ void __launch_bounds__(256) k_ksf_items(const int16_t* __restrict__ s
         const int j = j0 + k;
         if (j >= N) break;
         const int64_t q = pbase + j;
-        const int w = sfhi[q] - sflo[q];
+        const int w = k_sf_nlev(sflo[q], sfhi[q]);
         pOff[q] = run;
         const int nc = (w + kKsfChunk - 1) / kKsfChunk;
         for (int u = 0; u < nc; u++) {
@@ -281,9 +282,9 @@ __global__ void __launch_bounds__(256) k_ksf_items(const int16_t* __restrict__ s
     }
 }
 
-// One wave per chunk item: the partial sums of pair (S, F) for F's three
-// vertices and the item's lags, butterflied, raw (before clean-up):
-// part[(pOff + lag - sflo) * 27 + 9 v + (i + 3 c)].
+// One wave per chunk item: the level sums of pair (S, F) for F's three
+// vertices and the item's levels, butterflied:
+// part[(pOff + m - mlo) * 27 + 9 v + (i + 3 c)], mlo = k_sf_mlo(sflo).
 __global__ void __launch_bounds__(256) k_ksf(const KTri* __restrict__ kt, const double* __restrict__ verts,
                                              const double* __restrict__ gh, const double* __restrict__ rin_g, int nGH,
                                              KParams K, int r0, int64_t nitems, const int32_t* __restrict__ itemP,
@@ -301,15 +302,16 @@ __global__ void __launch_bounds__(256) k_ksf(const KTri* __restrict__ kt, const 
     for (int64_t it = wave; it < nitems; it += nwaves) {
         const int32_t q = itemP[it];
         const int S = r0 + q / K.N, F = q % K.N;
-        const int lagA = sflo[q] + kKsfChunk * itemC[it];
-        const int nA = min(kKsfChunk, (int)sfhi[q] - lagA);
+        const int mlo = k_sf_mlo(sflo[q]);
+        const int mA = mlo + kKsfChunk * itemC[it];
+        const int nL = min(kKsfChunk, (int)sfhi[q] + 1 - mA);
         A.g = gh + (int64_t)S * 128;
         double acc[3][kKsfChunk][9];
-        k_sf_lane(A, K, F, lagA, nA, lane, 64, acc);
-        double* o = part + (pOff[q] + (lagA - sflo[q])) * 27;
+        k_sf_lane(A, K, F, mA, nL, lane, 64, acc);
+        double* o = part + (pOff[q] + (mA - mlo)) * 27;
 #pragma unroll
         for (int j = 0; j < kKsfChunk; j++) {
-            if (j >= nA) break;
+            if (j >= nL) break;
 #pragma unroll
             for (int v = 0; v < 3; v++)
 #pragma unroll
@@ -323,8 +325,9 @@ __global__ void __launch_bounds__(256) k_ksf(const KTri* __restrict__ kt, const 
 }
 
 // One thread per store slot of rows [r0, r0 + rows): the regular block
-// (S, s, lag) = sum over the triangles F != S of s (ascending) of their
-// partial sums, zero where isBlockNull drops it, cleaned below 1e-14 and
+// (S, s, lag) = sum over the triangles F != S of s (ascending) of
+// -W_{lag-2} + 3 W_{lag-1} - 3 W_lag + W_{lag+1} (levels m <= 0 left out) of
+// their level sums, zero where isBlockNull drops it, cleaned below 1e-14 and
 // divided by 4 pi dt (kernelK.cu:212-221).
 __global__ void k_kgather(const double* __restrict__ verts, const double* __restrict__ X, KParams K, int r0,
                           int64_t s0, int64_t ns, const int32_t* __restrict__ slot_pair,
@@ -347,8 +350,16 @@ __global__ void k_kgather(const double* __restrict__ verts, const double* __rest
             if (F == S) continue;
             const int64_t q = (int64_t)(S - r0) * K.N + F;
             if (lag < sflo[q] || lag >= sfhi[q]) continue;
-            const double* pp = part + (pOff[q] + (lag - sflo[q])) * 27 + v * 9;
-            for (int c = 0; c < 9; c++) a[c] += pp[c];
+            const int mlo = k_sf_mlo(sflo[q]);
+            const double cn[4] = {-1, 3, -3, 1};
+            double b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int d = 0; d < 4; d++) {
+                const int m = lag - 2 + d;
+                if (m < 1) continue;
+                const double* pp = part + (pOff[q] + (m - mlo)) * 27 + v * 9;
+                for (int c = 0; c < 9; c++) b[c] += cn[d] * pp[c];
+            }
+            for (int c = 0; c < 9; c++) a[c] += b[c];
         }
         for (int c = 0; c < 9; c++) o[c] = fabs(a[c]) > 1.0e-14 ? a[c] / K.fourpidt : 0.0;
     }

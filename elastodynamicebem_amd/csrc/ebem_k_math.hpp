@@ -444,22 +444,29 @@ __host__ __device__ __attribute__((always_inline)) inline void k_kernel3(const K
             }
 }
 
-// Lags per wave of the field-triangle items of the regular K part.
+// Time levels per wave of the field-triangle items of the regular K part.
 #ifndef EBEM_KSF_CHUNK
 #define EBEM_KSF_CHUNK 2
 #endif
 constexpr int kKsfChunk = EBEM_KSF_CHUNK;
 
-// Lane `lane` of `nl`: the regular sums of test triangle S against field
-// triangle F != S for the hat functions of F's three vertices and the nA <=
-// kKsfChunk lags lagA .. lagA+nA-1 (kernelK.cu:63-160 restricted to F): outer
-// nodes of S, inner nodes k = lane, lane + nl, ... of F, each point's
-// geometry once, each level lagA-2 .. lagA+nA once, its value added to every
-// lag whose four levels contain it with that lag's weight -1, 3, -3, 1.
-__host__ __device__ inline void k_sf_lane(const KBlockArgs& A, const KParams& K, int F, int lagA, int nA, int lane,
+// Positive levels of a lag window [a, b): the lags a .. b-1 use the levels
+// lag-2 .. lag+1, of which those m >= 1 (t = m dt > 0) are nonzero.
+__host__ __device__ __attribute__((always_inline)) inline int k_sf_mlo(int a) { return a - 2 > 1 ? a - 2 : 1; }
+__host__ __device__ __attribute__((always_inline)) inline int k_sf_nlev(int a, int b) {
+    return b > a ? b - k_sf_mlo(a) + 1 : 0;
+}
+
+// Lane `lane` of `nl`: the level sums of test triangle S against field
+// triangle F != S for the hat functions of F's three vertices at the nL <=
+// kKsfChunk levels m = mA .. mA+nL-1 (all >= 1): acc[v][l][i + 3c] = sum over
+// the outer nodes of S and the inner nodes k = lane, lane + nl, ... of F of
+// w_q w_k |F| times the regular kernel of vertex v at t = m dt
+// (kernelK.cu:63-160 restricted to F and one level); each point's geometry
+// once.  The block of lag l is sum_d c_d W_{l-2+d}, c = (-1, 3, -3, 1).
+__host__ __device__ inline void k_sf_lane(const KBlockArgs& A, const KParams& K, int F, int mA, int nL, int lane,
                                           int nl, double acc[3][kKsfChunk][9]) {
 #pragma clang fp contract(off)
-    const double cn[4] = {-1, 3, -3, 1};
     for (int v = 0; v < 3; v++)
         for (int j = 0; j < kKsfChunk; j++)
             for (int e = 0; e < 9; e++) acc[v][j][e] = 0.0;
@@ -489,23 +496,17 @@ __host__ __device__ inline void k_sf_lane(const KBlockArgs& A, const KParams& K,
             KGeom G;
             k_geom(x, r, B.n, K, G);
 #pragma unroll
-            for (int li = 0; li < kKsfChunk + 3; li++) {
-                if (li >= nA + 3) break;
-                const double t = K.dt * (double)(lagA - 2 + li);
-                if (t <= 0) continue;
+            for (int l = 0; l < kKsfChunk; l++) {
+                if (l >= nL) break;
+                const double t = K.dt * (double)(mA + l);
                 double kv[3][3][3];
                 k_kernel3(G, t, w, phi, K, kv);
 #pragma unroll
-                for (int j = 0; j < kKsfChunk; j++) {
-                    const int d = li - j;
-                    if (j >= nA || d < 0 || d > 3) continue;
+                for (int v = 0; v < 3; v++)
 #pragma unroll
-                    for (int v = 0; v < 3; v++)
+                    for (int i = 0; i < 3; i++)
 #pragma unroll
-                        for (int i = 0; i < 3; i++)
-#pragma unroll
-                            for (int c = 0; c < 3; c++) acc[v][j][i + 3 * c] += pw * cn[d] * kv[v][i][c];
-                }
+                        for (int c = 0; c < 3; c++) acc[v][l][i + 3 * c] += pw * kv[v][i][c];
             }
         }
     }

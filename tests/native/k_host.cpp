@@ -20,9 +20,15 @@ void butterfly(double a[64][9]) {
         std::memcpy(a, b, sizeof b);
     }
 }
+// 0: the regular part per (S, s) pair as k_kblocks (k_reg_lane); 1: by field
+// triangle as k_ksf + k_kgather (k_sf_lane level sums of every F != S with
+// vertex s, ascending F, combined with -1, 3, -3, 1).
+int g_sf_mode = 0;
 }  // namespace
 
 extern "C" {
+
+void k_host_set_sf_mode(int mode) { g_sf_mode = mode; }
 
 // Block (S, s) of K_lag, column-major, for a mesh (X: Nv x 3, T: N x 3,
 // normals, areas, longest edges), material, outer rule (barycentric ghn, ghw)
@@ -73,7 +79,28 @@ int k_block_host(int N, int Nv, const double* X, const int32_t* T, const double*
     KBlockArgs A{kt.data(), verts.data(), voff.data(), vadj.data(), rin, rdg, g, nGH};
     double out[9] = {0};
     static double lanes[64][9];
-    if (!k_block_null(verts.data() + 9 * (size_t)S, X + 3 * s, lag, K)) {
+    if (g_sf_mode == 1 && !k_block_null(verts.data() + 9 * (size_t)S, X + 3 * s, lag, K)) {
+        const double cn[4] = {-1, 3, -3, 1};
+        double a[9] = {0};
+        for (int e = voff[s]; e < voff[s + 1]; e++) {
+            const int F = vadj[e] >> 2, v = vadj[e] & 3;
+            if (F == S) continue;
+            double b[9] = {0};
+            for (int d = 0; d < 4; d++) {
+                const int m = lag - 2 + d;
+                if (m < 1) continue;
+                for (int l = 0; l < 64; l++) {
+                    double acc[3][kKsfChunk][9];
+                    k_sf_lane(A, K, F, m, 1, l, 64, acc);
+                    for (int c = 0; c < 9; c++) lanes[l][c] = acc[v][0][c];
+                }
+                butterfly(lanes);
+                for (int c = 0; c < 9; c++) b[c] += cn[d] * lanes[0][c];
+            }
+            for (int c = 0; c < 9; c++) a[c] += b[c];
+        }
+        for (int e = 0; e < 9; e++) out[e] = std::fabs(a[e]) > 1.0e-14 ? a[e] / K.fourpidt : 0.0;
+    } else if (!k_block_null(verts.data() + 9 * (size_t)S, X + 3 * s, lag, K)) {
         for (int l = 0; l < 64; l++) {
             double acc[3][3];
             k_reg_lane(A, K, S, s, lag, l, 64, acc);

@@ -87,9 +87,8 @@ long long k_opcount_reg_point(int lag) {
 }
 
 // Flops of one evaluation point of the field-triangle form (one outer node,
-// one inner node of F, F's three hat functions) over the levels of nA lags
-// from lagA.
-long long k_opcount_sf_point(int lagA, int nA) {
+// one inner node of F, F's three hat functions) over the nL levels from mA.
+long long k_opcount_sf_point(int mA, int nL) {
     KTri kt[2];
     const cdouble n[3] = {0, 0, 1};
     k_tri_basis(kX + 0, kX + 3, kX + 6, n, 0.5, kt[0]);
@@ -108,7 +107,7 @@ long long k_opcount_sf_point(int lagA, int nA) {
     KBlockArgs A{kt, verts, nullptr, nullptr, rin, rin, g, 1};
     cdouble acc[3][kKsfChunk][9];
     g_ops = 0;
-    k_sf_lane(A, K, 1, lagA, nA, 0, 1, acc);
+    k_sf_lane(A, K, 1, mA, nL, 0, 1, acc);
     return g_ops;
 }
 

@@ -31,6 +31,7 @@ def host_k():
     lib.k_chunk_vs_single_host.argtypes = ([C.c_int, C.c_int, _dp, _ip, _dp, _dp, _dp] + [C.c_double] * 4 +
                                            [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp] + [C.c_int] * 4 + [_dp])
     lib.k_chunk_vs_single_host.restype = C.c_int
+    lib.k_host_set_sf_mode.argtypes = [C.c_int]
     return lib
 
 
@@ -47,8 +48,12 @@ def _host_block(lib, o, rules, lam, lag, S, s):
     return out.reshape(3, 3).T.copy()
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["per_vertex", "by_field_levels"])
 @pytest.mark.parametrize("meshname,lam", [("sphere2", 2.0), ("screen2", 0.5)])
-def test_host_k_blocks_match_oracle(host_k, request, meshname, lam):
+def test_host_k_blocks_match_oracle(host_k, request, meshname, lam, mode):
+    """mode 0: the regular part per (S, s) pair lag by lag (k_reg_lane);
+    mode 1: by field triangle as the device's default path, level sums of
+    every F != S with vertex s combined -1, 3, -3, 1 (k_sf_lane)."""
     from elastodynamicebem_amd import api
     from oracle import Oracle
     mesh = request.getfixturevalue(meshname)
@@ -57,6 +62,7 @@ def test_host_k_blocks_match_oracle(host_k, request, meshname, lam):
     gn, gw, dn, dw = api.inner_rule_tables(16, 3, 64)
     rules = (np.ascontiguousarray(np.column_stack([gn, gw])), np.ascontiguousarray(np.column_stack([dn, dw])))
     assert np.allclose(gn, o.ghcn) and np.allclose(gw, o.ghcw) and np.allclose(dn, o.dgn) and np.allclose(dw, o.dgw)
+    host_k.k_host_set_sf_mode(mode)
     rng = np.random.default_rng(7)
     worst = 0.0
     checked = 0
@@ -77,8 +83,9 @@ def test_host_k_blocks_match_oracle(host_k, request, meshname, lam):
                 worst = max(worst, e)
                 assert e < 1e-10, (S, s, l, e)
                 checked += 1
+    host_k.k_host_set_sf_mode(0)
     assert checked > 100
-    print(f"worst host-vs-oracle K block error {worst:.2e} over {checked} blocks")
+    print(f"worst host-vs-oracle K block error {worst:.2e} over {checked} blocks (mode {mode})")
 
 
 def test_host_pp_p_blocks_match_oracle(host_k, sphere2):
@@ -148,8 +155,8 @@ def test_bench_k_op_counts_match_the_counting_build():
     lib.k_opcount_sf_point.argtypes = [C.c_int, C.c_int]
     lib.k_opcount_sf_point.restype = C.c_longlong
     lib.k_opcount_sing_level.restype = C.c_longlong
-    for lagA in range(0, 9):
-        for nA in (1, 2):
-            assert int(bench.k_sf_point_flops(lagA, nA)) == lib.k_opcount_sf_point(lagA, nA), (lagA, nA)
+    for mA in range(1, 9):
+        for nL in range(1, bench.K_SF_CHUNK + 1):
+            assert int(bench.k_sf_point_flops(mA, nL)) == lib.k_opcount_sf_point(mA, nL), (mA, nL)
     # one outer node, one collapsed-rule node in each of the three children
     assert 3 * bench.K_SING_POINT_LEVEL == lib.k_opcount_sing_level()
