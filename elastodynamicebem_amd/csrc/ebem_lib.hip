@@ -464,6 +464,7 @@ struct ebem_handle {
     bool sort_items = true;
     bool kp_split = true;  // kP ranges through k_levels_kp (EBEM_KP_SPLIT=0 turns it off)
     bool k_by_field = true;  // regular K part by (S, F) pairs + gather (EBEM_K_BY_FIELD=0: by (S, s) slots)
+    bool ppd_by_field = true;  // direct PP double layer by (pair, F) + gather (EBEM_PPD_BY_FIELD=0: by vertex)
     // kP and between-front ranges of the last assembly / post-processing:
     // kP ranges, kP levels, between-front ranges, their levels, fallbacks
     int64_t kp_stats[5] = {0, 0, 0, 0, 0};
@@ -1762,6 +1763,8 @@ int ebem_create(ebem_handle** out, int device) {
     h->kp_split = !(kps && std::strcmp(kps, "0") == 0);
     const char* kbf = std::getenv("EBEM_K_BY_FIELD");
     h->k_by_field = !(kbf && std::strcmp(kbf, "0") == 0);
+    const char* pbf = std::getenv("EBEM_PPD_BY_FIELD");
+    h->ppd_by_field = !(pbf && std::strcmp(pbf, "0") == 0);
     *out = h;
     return 0;
 }
@@ -2210,9 +2213,80 @@ int ebem_pp_eval_direct(ebem_handle* h, const double* density, const double* g, 
     const int qc = (int)std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t)(64e6 / (24.0 * (N + Nv)))));
     HIPC(d_oG.alloc(3 * (size_t)qc * N));
     HIPC(d_oP.alloc(3 * (size_t)qc * Nv));
-    for (int q0 = 0; q0 < nq; q0 += qc) {
+    if (h->ppd_by_field) {
+        // double-layer part by (pair, triangle) and level: level windows of
+        // every pair, then batches of pairs whose level sums fit the budget
+        DevBuf<int16_t> lF, itemC;
+        DevBuf<int64_t> cnt, woff, base;
+        DevBuf<int32_t> itemP;
+        DevBuf<double> W;
+        HIPC(lF.alloc((size_t)nq * N));
+        HIPC(cnt.alloc(2 * (size_t)nq));
+        hipLaunchKernelGGL(k_ppf_win, dim3(nq), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, d_t.p, d_pq.p, nq, P, lF.p,
+                           cnt.p);
+        HIPC(hipGetLastError());
+        std::vector<int64_t> hc(2 * (size_t)nq);
+        HIPC(hipMemcpyAsync(hc.data(), cnt.p, sizeof(int64_t) * 2 * nq, hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        size_t freeb = 0, totb = 0;
+        HIPC(hipMemGetInfo(&freeb, &totb));
+        int64_t cap = (int64_t)(0.3 * (double)freeb / (27 * 8));
+        for (int q = 0; q < nq; q++) cap = std::max(cap, hc[q]);
+        HIPC(hipFuncSetAttribute((const void*)k_ppf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int q0 = 0;
+        while (q0 < nq) {
+            int q1 = q0;
+            int64_t lev = 0, items = 0;
+            while (q1 < nq && q1 - q0 < qc && (q1 == q0 || lev + hc[q1] <= cap)) {
+                lev += hc[q1];
+                items += hc[nq + q1];
+                q1++;
+            }
+            const int n = q1 - q0;
+            std::vector<int64_t> hb(2 * (size_t)n);
+            int64_t run = 0, runI = 0;
+            for (int r = 0; r < n; r++) {
+                hb[r] = run;
+                hb[n + r] = runI;
+                run += hc[q0 + r];
+                runI += hc[nq + q0 + r];
+            }
+            HIPC(base.grow(2 * (size_t)n));
+            HIPC(hipMemcpyAsync(base.p, hb.data(), sizeof(int64_t) * 2 * n, hipMemcpyHostToDevice, h->stream));
+            HIPC(woff.grow((size_t)n * N));
+            HIPC(itemP.grow(std::max<int64_t>(items, 1)));
+            HIPC(itemC.grow(std::max<int64_t>(items, 1)));
+            HIPC(W.grow(27 * (size_t)std::max<int64_t>(lev, 1)));
+            const int16_t* lFb = lF.p + (int64_t)q0 * N;
+            hipLaunchKernelGGL(k_ppf_items, dim3(n), dim3(256), 0, h->stream, lFb, N, n, base.p, woff.p, itemP.p,
+                               itemC.p);
+            HIPC(hipGetLastError());
+            if (items > 0) {
+                const int64_t bI = std::min<int64_t>((items + 256 / kPpfLanes - 1) / (256 / kPpfLanes),
+                                                     (int64_t)h->CUs * 8);
+                hipLaunchKernelGGL(k_ppf, dim3((unsigned)bI), dim3(256), lds, h->stream, h->d_ktri.p, h->d_verts.p,
+                                   d_x.p, d_t.p, d_pq.p, q0, P, h->d_pp_rule.p, items, itemP.p, itemC.p, lFb, woff.p,
+                                   W.p);
+                HIPC(hipGetLastError());
+            }
+            const int64_t bP = std::min<int64_t>(((int64_t)n * Nv + 3) / 4, (int64_t)h->CUs * 8);
+            hipLaunchKernelGGL(k_ppf_gather, dim3((unsigned)bP), dim3(256), 0, h->stream, h->d_vadj_off.p,
+                               h->d_vadj.p, d_t.p, d_pq.p, q0, n, P, lFb, woff.p, W.p, gT, d_oP.p);
+            HIPC(hipGetLastError());
+            const int64_t bG = std::min<int64_t>(((int64_t)n * N + 15) / 16, (int64_t)h->CUs * 8);
+            hipLaunchKernelGGL(k_ppd_G, dim3((unsigned)bG), dim3(256), lds, h->stream, h->d_tri.p, h->d_verts.p, d_x.p,
+                               d_t.p, d_pq.p, q0, n, P, h->d_pp_rule.p, alpha, d_oG.p);
+            HIPC(hipGetLastError());
+            hipLaunchKernelGGL(k_ppd_sum, dim3(n), dim3(256), 0, h->stream, d_oG.p, d_oP.p, d_pq.p, q0, N, Nv, npts,
+                               d_u.p);
+            HIPC(hipGetLastError());
+            HIPC(hipStreamSynchronize(h->stream));  // hb, the level sums and the item map are reused
+            q0 = q1;
+        }
+    }
+    for (int q0 = 0; q0 < nq && !h->ppd_by_field; q0 += qc) {
         const int n = std::min(qc, nq - q0);
-        const int64_t bG = std::min<int64_t>(((int64_t)n * N + 3) / 4, (int64_t)h->CUs * 8);
+        const int64_t bG = std::min<int64_t>(((int64_t)n * N + 15) / 16, (int64_t)h->CUs * 8);
         hipLaunchKernelGGL(k_ppd_G, dim3((unsigned)bG), dim3(256), lds, h->stream, h->d_tri.p, h->d_verts.p, d_x.p,
                            d_t.p, d_pq.p, q0, n, P, h->d_pp_rule.p, alpha, d_oG.p);
         HIPC(hipGetLastError());

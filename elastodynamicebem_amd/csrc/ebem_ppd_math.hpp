@@ -166,4 +166,57 @@ __host__ __device__ inline void ppd_p_lane_chunk(const PPDArgs& A, const double*
     }
 }
 
+// Levels per wave of the field-triangle form of the double-layer part.
+#ifndef EBEM_PPF_CHUNK
+#define EBEM_PPF_CHUNK 4
+#endif
+constexpr int kPpfChunk = EBEM_PPF_CHUNK;
+// Lanes per item (a wave takes 64 / kPpfLanes items): the rules have a few
+// hundred nodes, so narrower groups cut the butterflies per item.
+constexpr int kPpfLanes = 16;
+
+// Lane `lane` of `nl`: the level sums of field triangle F for the point x at
+// time t, the nL <= kPpfChunk levels l = lA .. lA+nL-1 (tau = t - l dt) and
+// F's three hat functions: acc[v][j][i + 3c] = sum over the rule's nodes k =
+// lane, lane + nl, ... of w_k |F| nucleo(tau) phi_v (kernelP.cu:27-107 for one
+// level and triangle); each node's geometry once, nucleo without phi once per
+// level, then times each phi_v as the reference's (...) * phi.  The block y
+// of vertex s is sum_{F with vertex s} W_y - 2 W_{y+1} + W_{y+2}.
+__host__ __device__ inline void ppf_lane(const PPDArgs& A, const double* x, double t, int F, int lA, int nL, int lane,
+                                         int nl, double acc[3][kPpfChunk][9]) {
+#pragma clang fp contract(off)
+    for (int v = 0; v < 3; v++)
+        for (int j = 0; j < kPpfChunk; j++)
+            for (int e = 0; e < 9; e++) acc[v][j][e] = 0.0;
+    const KTri& B = A.kt[F];
+    const double* vf = A.verts + (int64_t)F * 9;
+    for (int q = lane; q < A.nInt; q += nl) {
+        const double* b = A.rule + 4 * q;
+        double yv[3];
+        for (int c = 0; c < 3; c++) yv[c] = b[0] * vf[c] + b[1] * vf[3 + c] + b[2] * vf[6 + c];
+        const double w = b[3] * B.area;
+        const double xv[3] = {x[0] - yv[0], x[1] - yv[1], x[2] - yv[2]};
+        const double r = sqrt(xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2]);
+        double phi[3];
+        for (int v = 0; v < 3; v++)
+            phi[v] = B.vec[v] + B.mat[3 * v] * yv[0] + B.mat[3 * v + 1] * yv[1] + B.mat[3 * v + 2] * yv[2];
+        PGeom G;
+        p_geom(xv, r, B.n, A.lam, A.mu, G);
+#pragma unroll
+        for (int j = 0; j < kPpfChunk; j++) {
+            if (j >= nL) break;
+            const double tau = t - (double)(lA + j) * A.dt;
+            if (tau <= 0) continue;
+            double nu[3][3];
+            p_kernel(G, tau, A.cP, A.cS, 1.0, nu);
+#pragma unroll
+            for (int v = 0; v < 3; v++)
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) acc[v][j][i + 3 * c] += w * (nu[i][c] * phi[v]);
+        }
+    }
+}
+
 }  // namespace ebem

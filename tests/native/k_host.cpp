@@ -213,6 +213,7 @@ int k_chunk_vs_single_host(int N, int Nv, const double* X, const int32_t* T, con
 int ppp_block_host(int N, int Nv, const double* X, const int32_t* T, const double* nrm, const double* area,
                    double rho, double mu, double lam, double Tfin, int nT, int nInt, const double* rin,
                    const double* x, double t, int y, int s, double* out9) {
+    // out9[0..8]: per vertex as k_ppd_P; out9[9..17]: by field triangle and level
 #pragma clang fp contract(off)
     std::vector<KTri> kt(N);
     std::vector<double> verts((size_t)N * 9);
@@ -249,6 +250,29 @@ int ppp_block_host(int N, int Nv, const double* X, const int32_t* T, const doubl
     butterfly(lanes);
     const double scale = 4 * kPi * rho * dt;
     for (int e = 0; e < 9; e++) out9[e] = std::fabs(lanes[0][e]) > 1.0e-14 ? lanes[0][e] / scale : 0.0;
+    // the field-triangle form (k_ppf + k_ppf_gather): level sums W_l of every
+    // triangle F with vertex s, combined W_y - 2 W_{y+1} + W_{y+2}
+    const double cn[3] = {1, -2, 1};
+    double B[9] = {0};
+    for (int e = voff[s]; e < voff[s + 1]; e++) {
+        const int F = vadj[e] >> 2, v = vadj[e] & 3;
+        double b[9] = {0};
+        for (int k = 0; k < 3; k++) {
+            // kPpfLanes lanes per item; the zero lanes above them add exact zeros
+            // in the first butterfly steps, then the steps match the device's
+            for (int l = 0; l < 64; l++) {
+                for (int c = 0; c < 9; c++) lanes[l][c] = 0.0;
+                if (l >= kPpfLanes) continue;
+                double acc[3][kPpfChunk][9];
+                ppf_lane(A, x, t, F, y + k, 1, l, kPpfLanes, acc);
+                for (int c = 0; c < 9; c++) lanes[l][c] = acc[v][0][c];
+            }
+            butterfly(lanes);
+            for (int c = 0; c < 9; c++) b[c] += cn[k] * lanes[0][c];
+        }
+        for (int c = 0; c < 9; c++) B[c] += b[c];
+    }
+    for (int e = 0; e < 9; e++) out9[9 + e] = std::fabs(B[e]) > 1.0e-14 ? B[e] / scale : 0.0;
     return 0;
 }
 

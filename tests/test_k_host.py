@@ -100,25 +100,30 @@ def test_host_pp_p_blocks_match_oracle(host_k, sphere2):
     assert np.allclose(gn, o.ppn) and np.allclose(gw, o.ppw)
     rng = np.random.default_rng(8)
     checked = 0
+    worst = 0.0
     for _ in range(150):
         x = rng.normal(size=3) * rng.choice([0.3, 0.9, 1.6])
         s = int(rng.integers(sphere2.number_nodes))
         t = rng.uniform(0.2, 8.0)
         y = int(rng.integers(int(np.ceil(t / o.dt))))
         R = o.pp_p_block(x, t, y, s)
-        out = np.zeros(9)
+        out = np.zeros(18)
         rc = host_k.ppp_block_host(o.N, sphere2.number_nodes, _p(o.coords), o.tri.ctypes.data_as(_ip),
                                    _p(o.normal), _p(o.area), 1.0, 1.0, 2.0, 8.0, 32, rule.shape[0], _p(rule),
                                    _p(np.ascontiguousarray(x)), t, y, s, _p(out))
         assert rc == 0, "chunked and per-block double-layer sums differ"
-        G = out.reshape(3, 3).T
+        G = out[:9].reshape(3, 3).T  # per vertex (k_ppd_P)
+        Gf = out[9:].reshape(3, 3).T  # by field triangle and level (k_ppf, k_ppf_gather)
         sc = np.abs(R).max()
         if sc == 0:
-            assert np.abs(G).max() == 0
+            assert np.abs(G).max() == 0 and np.abs(Gf).max() == 0
             continue
         assert np.abs(G - R).max() < 1e-11 * sc, (x, t, y, s)
+        assert np.abs(Gf - R).max() < 1e-11 * sc, (x, t, y, s)
+        worst = max(worst, np.abs(Gf - R).max() / sc)
         checked += 1
     assert checked > 40
+    print(f"double-layer PP blocks: field-triangle form vs oracle worst {worst:.2e} over {checked} blocks")
 
 
 def test_chunked_regular_sum_is_bitwise_the_per_lag_sum(host_k, sphere2):
