@@ -35,7 +35,9 @@ __device__ __attribute__((always_inline)) inline int ppd_nhat(double t, double d
 // Single-layer part: one group of kPpfLanes lanes per (q, F), q = pair index
 // of this launch (pairs q0 + q of pq: point, time index).  out[(q * N + F) * 3 + c] = sum_y (G_y alpha_y)_c.
 // G_y (kernelG.cu:26-116): levels t - y dt and t - (y + 1) dt with weights 1, -1
-// (non-positive skipped), clean-up below 1e-14, / (4 pi rho).  A block whose
+// (non-positive skipped), clean-up below 1e-14, / (4 pi rho); each level is
+// evaluated once and its lane partial kept for the next block (the reference
+// combines the two levels node by node: the rounding differs, not its size).  A block whose
 // triangle lies beyond the P front of its later level, |x - c_F| - maxL_F >
 // cP (t - y dt), is zero in every entry, and so are those of larger y.
 __global__ void __launch_bounds__(256) k_ppd_G(const FieldTri* __restrict__ tri, const double* __restrict__ verts,
@@ -62,33 +64,37 @@ __global__ void __launch_bounds__(256) k_ppd_G(const FieldTri* __restrict__ tri,
         const double c0 = x[0] - tri[F].center[0], c1 = x[1] - tri[F].center[1], c2 = x[2] - tri[F].center[2];
         const double dmin = sqrt(c0 * c0 + c1 * c1 + c2 * c2) - tri[F].maxL;
         double u[3] = {0, 0, 0};
-        for (int y = 0; y < nHat; y++) {
-            const double t0 = t - (double)y * P.dt;
-            if (dmin > P.cP * t0 * (1 + 1e-12)) break;  // wave-uniform; exact zeros from here on
-            double acc[6] = {0, 0, 0, 0, 0, 0};
-            const double cn[2] = {1, -1};
+        // lane partial of level l: sum over its nodes of w nucleo(t - l dt)
+        auto level = [&](int l, double L[6]) {
+            for (int c = 0; c < 6; c++) L[c] = 0.0;
+            const double tau = t - (double)l * P.dt;
+            if (tau <= 0) return;
             for (int k = lane; k < P.nInt; k += kPpfLanes) {
                 const double* b = rule + 4 * k;
                 double d[3];
                 for (int e = 0; e < 3; e++) d[e] = (b[0] * vf[e] + b[1] * vf[3 + e] + b[2] * vf[6 + e]) - x[e];
                 const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
                 const double w = b[3] * aF;
+                double nu[6];
+                nucleo6(d, r, tau, P.cP, P.cS, nu);
 #pragma unroll
-                for (int l = 0; l < 2; l++) {
-                    const double tau = t - (double)(y + l) * P.dt;
-                    if (tau <= 0) continue;
-                    double nu[6];
-                    nucleo6(d, r, tau, P.cP, P.cS, nu);
-#pragma unroll
-                    for (int c = 0; c < 6; c++) acc[c] += cn[l] * w * nu[c];
-                }
+                for (int c = 0; c < 6; c++) L[c] += w * nu[c];
             }
+        };
+        double Lc[6] = {0, 0, 0, 0, 0, 0};
+        if (nHat > 0 && !(dmin > P.cP * t * (1 + 1e-12))) level(0, Lc);
+        for (int y = 0; y < nHat; y++) {
+            const double t0 = t - (double)y * P.dt;
+            if (dmin > P.cP * t0 * (1 + 1e-12)) break;  // group-uniform; exact zeros from here on
+            double Ln[6];
+            level(y + 1, Ln);
             double G[6];
 #pragma unroll
             for (int c = 0; c < 6; c++) {
-                double a = acc[c];
+                double a = Lc[c] - Ln[c];
                 for (int m = kPpfLanes / 2; m >= 1; m >>= 1) a += __shfl_xor(a, m);
                 G[c] = fabs(a) > 1.0e-14 ? a / fourpirho : 0.0;
+                Lc[c] = Ln[c];
             }
             const double* al = alpha + (int64_t)y * 3 * P.N + 3 * F;
             const double a0 = al[0], a1 = al[1], a2 = al[2];
