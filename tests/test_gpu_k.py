@@ -211,3 +211,24 @@ def test_k_rows_sphere_wave():
     bo = o.beta_k(g, rows)[3 * rows[0]:3 * rows[1]]
     assert frob_rel(bk[3 * rows[0]:3 * rows[1]], bo) < 1e-12
     print(f"K sphereWave_1: LK {LK}, {info['slots']} blocks in {ms:.0f} ms, worst lag error {worst:.2e}")
+
+
+def test_k_per_pair_path_agrees(sphere2, monkeypatch):
+    """The per-(test triangle, vertex) form of the regular part
+    (EBEM_K_BY_FIELD=0: k_kblocks, the reference's accumulation order) and
+    the default by field triangle and level (k_ksf + k_kgather) give the same
+    K_lag rows to round-off."""
+    from elastodynamicebem_amd import api
+    pb = _pb(4.0, 32)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("EBEM_K_BY_FIELD", mode)
+        with api.Solver(sphere2, pb, 7) as s:
+            s.set_inner_rule(16, 3, 64)
+            LK = s.k_num_blocks()
+            s.assemble_k()
+            out[mode] = [s.get_k_block(l, (100, 104)) for l in range(LK)]
+    peak = max(np.linalg.norm(K) for K in out["0"])
+    worst = max(np.linalg.norm(a - b) for a, b in zip(out["0"], out["1"])) / peak
+    print(f"K per pair vs by field triangle and level, sphere2 rows 100-103: {worst:.2e}")
+    assert worst < 1e-12

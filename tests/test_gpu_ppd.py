@@ -94,3 +94,26 @@ def test_ppd_bar_after_dd_solve():
     print(f"bar interior field after the DD solve: rel L2 error {rel:.3e} (max |u_x|, |u_y|: "
           f"{np.abs(u[:2]).max():.2e}, max |u_z| {np.abs(ex[2]).max():.2f})")
     assert rel < 0.03
+
+
+def test_ppd_per_vertex_path_agrees(sphere2, monkeypatch):
+    """The per-vertex double layer (EBEM_PPD_BY_FIELD=0: k_ppd_P, the
+    reference's accumulation order) and the default by field triangle and
+    level (k_ppf + k_ppf_gather) give the same field to round-off."""
+    from elastodynamicebem_amd import api
+    pb = api.PbParam(rho=1.0, mu=1.0, lambda_=2.0, Tfin=8.0, nT=32)
+    rng = np.random.default_rng(22)
+    N, Nv = sphere2.numberTriangles, sphere2.number_nodes
+    dens = rng.normal(size=(3 * N, 32))
+    g = rng.normal(size=(3 * Nv, 32))
+    pts = np.array([[0.1, -0.2, 0.15], [0.0, 0.0, 0.93], [1.3, 0.4, -0.2]])
+    times = np.array([8.0, 5.1, 2.0, 0.6])
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("EBEM_PPD_BY_FIELD", mode)
+        with api.Solver(sphere2, pb, 7) as s:
+            s.set_pp_rule(16, 3)
+            out[mode] = s.pp_eval_direct(pts, times, dens, g)
+    err = np.abs(out["0"] - out["1"]).max() / np.abs(out["0"]).max()
+    print(f"direct PP per vertex vs by field triangle and level: {err:.2e}")
+    assert err < 1e-12
