@@ -48,13 +48,14 @@ PEAK_HBM_GBS = 8000.0
 # op-counting host build of its math (tests/native/k_opcount.cpp).  Regular
 # part (k_ksf, by field triangle, level sums): per evaluation point (one outer
 # node, one inner node of the field triangle F, F's three hat functions) and
-# chunk of nL <= 2 levels, 244 for its geometry and basis values and 521 per
-# level (R_j and KL + KT once, combined for the three vertices, 27 sums).  The
+# chunk of nL <= 2 levels, 242 for its geometry and basis values (powers of
+# 1/r once) and 488 per level (R_j and KL + KT once, combined for the three
+# vertices, 27 sums).  The
 # gather's combination -1, 3, -3, 1 of the level sums (at most 81 per slot
 # and triangle, ~1e-4 of the total) is not counted.  Singular part (k_ksing):
 # 585 per level and evaluation point (three children x nDiag nodes per outer
 # node).
-K_SF_POINT_BASE, K_SF_LEVEL = 244, 521
+K_SF_POINT_BASE, K_SF_LEVEL = 242, 488
 K_SING_POINT_LEVEL = 585
 K_SF_CHUNK = 2
 

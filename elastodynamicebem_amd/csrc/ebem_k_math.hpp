@@ -130,6 +130,35 @@ __host__ __device__ __attribute__((always_inline)) inline void k_geom(const doub
     G.d2 = xn / r2;
 }
 
+// k_geom with the powers of 1/r taken once and multiplied (values within
+// an ulp or two of k_geom's quotients; used by the level-sum path, whose
+// sums already differ from the reference's order).  G.r keeps r; ir = 1/r is
+// returned for the lambda-mu term.
+__host__ __device__ __attribute__((always_inline)) inline void k_geom_fast(const double x[3], double r,
+                                                                           const double n[3], const KParams& K,
+                                                                           KGeom& G, double& ir) {
+#pragma clang fp contract(off)
+    ir = 1 / r;
+    const double ir2 = ir * ir, ir3 = ir2 * ir, ir5 = ir3 * ir2;
+    const double cl = K.lam / (K.lam + K.mu), cm = K.mu / (K.lam + K.mu);
+    G.r = r;
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double t5 = 3 * x[q] * x[k] * ir5;
+            G.g5[q][k] = (q == k) ? t5 - ir3 : t5;
+            G.g3[q][k] = x[q] * x[k] * ir3;
+            G.a3[q][k] = cl * x[q] * n[k] * ir3;
+            G.b3[q][k] = cm * x[k] * n[q] * ir3;
+            G.a2[q][k] = cl * x[q] * n[k] * ir2;
+            G.b2[q][k] = cm * x[k] * n[q] * ir2;
+        }
+    const double xn = x[0] * n[0] + x[1] * n[1] + x[2] * n[2];
+    G.d3 = xn * ir3;
+    G.d2 = xn * ir2;
+}
+
 // Kernel of vertex basis function v at time t, 3x3 row-major in out[i][k]:
 // -sum_j w_j R_j + phi (KL + KT), w = n x grad phi_v (kernelK.cu:296-459).
 // REG: the regular form of kernelK.cu (the x.n diagonal terms of KL / KT and
@@ -422,14 +451,71 @@ __host__ __device__ __attribute__((always_inline)) inline void k_kernel_parts(co
         }
 }
 
+// k_kernel_parts with the divisions by the wave speeds, by 6 and by r taken
+// as products with reciprocals (ir = 1/r from k_geom_fast): same terms,
+// values within a few ulps.
+__host__ __device__ __attribute__((always_inline)) inline void k_kernel_parts_fast(const KGeom& G, double ir,
+                                                                                   double t, const KParams& K,
+                                                                                   double R[3][3][3],
+                                                                                   double LT[3][3]) {
+#pragma clang fp contract(off)
+    const double r = G.r;
+    const double icP = 1 / K.cP, icS = 1 / K.cS, icP2 = 1 / (K.cP * K.cP), icS2 = 1 / (K.cS * K.cS);
+    const double rcP = r * icP, rcS = r * icS;
+    const double tP = t - rcP, tS = t - rcS;
+    const double HP = tP > 0 ? 1.0 : 0.0;
+    const double HS = tS > 0 ? 1.0 : 0.0;
+    const double HPl = (r - rcP) > 0 ? 1.0 : 0.0;
+    const double Hcub = (tS * tS * (t + 2 * rcS) * HS - tP * tP * (t + 2 * rcP) * HP) * (1.0 / 6);
+    const double Dlin = tS * HS * icS2 - tP * HP * icP2;
+    const double Dlam = tS * HS * icS2 - tP * HPl * icP2;
+    const double a = 2 * K.mu / K.rho;
+    const double bir = K.lam * K.mu / (K.rho * (K.lam + K.mu)) * ir;
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (i == j) continue;
+            const int q = 3 - i - j;
+            const double e = ((i - j + 3) % 3 == 1) ? 1.0 : -1.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double hp = a * (e * G.g5[q][k]) * Hcub;
+                const double dp = a * (e * G.g3[q][k]) * Dlin;
+                R[j][i][k] = (k != i && k != j) ? (-e) * bir * Dlam + hp + dp : hp + dp;
+            }
+        }
+    const double fL = tP * HP - tS * HS;
+    const double fT = HP * icP - HS * icS;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double L = G.a3[i][k] * fL + G.b3[i][k] * fL;
+            double T = G.a2[i][k] * fT + G.b2[i][k] * fT;
+            if (i == k) {
+                L = L + G.d3 * tS * HS;
+                T = T + G.d2 * (HS * icS);
+            }
+            LT[i][k] = L + T;
+        }
+}
+
 // Regular kernel for the three hat functions of one field triangle at once:
-// out[v] equals k_kernel<true>(G, t, w[v], phi[v], K, .) bit for bit.
+// out[v] equals k_kernel<true>(G, t, w[v], phi[v], K, .) bit for bit (FAST:
+// to a few ulps).
+// FAST: the parts from k_kernel_parts_fast (ir = 1/r), equal to a few ulps.
+template <bool FAST = false>
 __host__ __device__ __attribute__((always_inline)) inline void k_kernel3(const KGeom& G, double t,
                                                                          const double w[3][3], const double phi[3],
-                                                                         const KParams& K, double out[3][3][3]) {
+                                                                         const KParams& K, double out[3][3][3],
+                                                                         double ir = 0.0) {
 #pragma clang fp contract(off)
     double R[3][3][3], LT[3][3];
-    k_kernel_parts(G, t, K, R, LT);
+    if (FAST)
+        k_kernel_parts_fast(G, ir, t, K, R, LT);
+    else
+        k_kernel_parts(G, t, K, R, LT);
 #pragma unroll
     for (int v = 0; v < 3; v++)
 #pragma unroll
@@ -462,8 +548,9 @@ __host__ __device__ __attribute__((always_inline)) inline int k_sf_nlev(int a, i
 // kKsfChunk levels m = mA .. mA+nL-1 (all >= 1): acc[v][l][i + 3c] = sum over
 // the outer nodes of S and the inner nodes k = lane, lane + nl, ... of F of
 // w_q w_k |F| times the regular kernel of vertex v at t = m dt
-// (kernelK.cu:63-160 restricted to F and one level); each point's geometry
-// once.  The block of lag l is sum_d c_d W_{l-2+d}, c = (-1, 3, -3, 1).
+// (kernelK.cu:63-160 restricted to F and one level, in the reciprocal forms
+// k_geom_fast / k_kernel_parts_fast); each point's geometry once.  The block
+// of lag l is sum_d c_d W_{l-2+d}, c = (-1, 3, -3, 1).
 __host__ __device__ inline void k_sf_lane(const KBlockArgs& A, const KParams& K, int F, int mA, int nL, int lane,
                                           int nl, double acc[3][kKsfChunk][9]) {
 #pragma clang fp contract(off)
@@ -494,13 +581,14 @@ __host__ __device__ inline void k_sf_lane(const KBlockArgs& A, const KParams& K,
             for (int v = 0; v < 3; v++)
                 phi[v] = B.vec[v] + B.mat[3 * v] * y[0] + B.mat[3 * v + 1] * y[1] + B.mat[3 * v + 2] * y[2];
             KGeom G;
-            k_geom(x, r, B.n, K, G);
+            double ir;
+            k_geom_fast(x, r, B.n, K, G, ir);
 #pragma unroll
             for (int l = 0; l < kKsfChunk; l++) {
                 if (l >= nL) break;
                 const double t = K.dt * (double)(mA + l);
                 double kv[3][3][3];
-                k_kernel3(G, t, w, phi, K, kv);
+                k_kernel3<true>(G, t, w, phi, K, kv, ir);
 #pragma unroll
                 for (int v = 0; v < 3; v++)
 #pragma unroll
