@@ -453,7 +453,8 @@ __host__ __device__ __attribute__((always_inline)) inline void k_kernel_parts(co
 
 // k_kernel_parts with the divisions by the wave speeds, by 6 and by r taken
 // as products with reciprocals (ir = 1/r from k_geom_fast): same terms,
-// values within a few ulps.
+// values within a few ulps; r / cP and r / cS stay quotients, so every
+// Heaviside front test decides as k_kernel_parts'.
 __host__ __device__ __attribute__((always_inline)) inline void k_kernel_parts_fast(const KGeom& G, double ir,
                                                                                    double t, const KParams& K,
                                                                                    double R[3][3][3],
@@ -461,7 +462,8 @@ __host__ __device__ __attribute__((always_inline)) inline void k_kernel_parts_fa
 #pragma clang fp contract(off)
     const double r = G.r;
     const double icP = 1 / K.cP, icS = 1 / K.cS, icP2 = 1 / (K.cP * K.cP), icS2 = 1 / (K.cS * K.cS);
-    const double rcP = r * icP, rcS = r * icS;
+    // the quotients themselves: the front tests t - r/c > 0 decide as the reference's
+    const double rcP = r / K.cP, rcS = r / K.cS;
     const double tP = t - rcP, tS = t - rcS;
     const double HP = tP > 0 ? 1.0 : 0.0;
     const double HS = tS > 0 ? 1.0 : 0.0;
