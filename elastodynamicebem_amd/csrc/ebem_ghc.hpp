@@ -39,6 +39,30 @@ __device__ __attribute__((always_inline)) inline void nucleo6(const double x[3],
     }
 }
 
+// nucleo6 with 1/r (ir) taken once and the reciprocal squares of the wave
+// speeds (icP2, icS2) as products; r / cP and r / cS (rcP, rcS) stay
+// quotients, so the front tests decide as nucleo6's.  Values within a few
+// ulps of nucleo6's (used by the direct post-processing).
+__device__ __attribute__((always_inline)) inline void nucleo6_fast(const double x[3], double r, double ir, double t,
+                                                                   double rcP, double rcS, double icP2, double icS2,
+                                                                   double o[6]) {
+#pragma clang fp contract(off)
+    const double HP = (t - rcP > 0) ? 1.0 : 0.0;
+    const double HS = (t - rcS > 0) ? 1.0 : 0.0;
+    const double ir2 = ir * ir, ir3 = ir2 * ir, ir5 = ir3 * ir2;
+    const double br = (t * t - r * r * icP2) * HP - (t * t - r * r * icS2) * HS;
+    const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const double xx = x[ii[q]] * x[jj[q]];
+        const double a3 = xx * ir3, a5 = xx * ir5;
+        if (ii[q] == jj[q])
+            o[q] = a3 * icP2 * HP + (ir - a3) * icS2 * HS - 0.5 * (ir3 - 3 * a5) * br;
+        else
+            o[q] = a3 * icP2 * HP + (-a3) * icS2 * HS - 0.5 * (-3 * a5) * br;
+    }
+}
+
 // Lags per wave of the numeric off-diagonal blocks: consecutive lags share
 // two of their three time levels.
 #ifndef EBEM_GHC_CHUNK

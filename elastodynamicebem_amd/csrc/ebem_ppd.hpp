@@ -37,7 +37,8 @@ __device__ __attribute__((always_inline)) inline int ppd_nhat(double t, double d
 // G_y (kernelG.cu:26-116): levels t - y dt and t - (y + 1) dt with weights 1, -1
 // (non-positive skipped), clean-up below 1e-14, / (4 pi rho); each level is
 // evaluated once and its lane partial kept for the next block (the reference
-// combines the two levels node by node: the rounding differs, not its size).  A block whose
+// combines the two levels node by node: the rounding differs, not its size),
+// the kernel in nucleo6_fast's reciprocal form.  A block whose
 // triangle lies beyond the P front of its later level, |x - c_F| - maxL_F >
 // cP (t - y dt), is zero in every entry, and so are those of larger y.
 __global__ void __launch_bounds__(256) k_ppd_G(const FieldTri* __restrict__ tri, const double* __restrict__ verts,
@@ -53,6 +54,7 @@ __global__ void __launch_bounds__(256) k_ppd_G(const FieldTri* __restrict__ tri,
     const int64_t grp = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kPpfLanes;
     const int64_t ngrp = ((int64_t)gridDim.x * blockDim.x) / kPpfLanes;
     const double fourpirho = 4 * kPi * P.rho;
+    const double icP2 = 1 / (P.cP * P.cP), icS2 = 1 / (P.cS * P.cS);
     for (int64_t it = grp; it < (int64_t)nq * P.N; it += ngrp) {
         const int q = (int)(it / P.N), F = (int)(it % P.N);
         const int p = pq[2 * (q0 + q)], kt = pq[2 * (q0 + q) + 1];
@@ -76,7 +78,7 @@ __global__ void __launch_bounds__(256) k_ppd_G(const FieldTri* __restrict__ tri,
                 const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
                 const double w = b[3] * aF;
                 double nu[6];
-                nucleo6(d, r, tau, P.cP, P.cS, nu);
+                nucleo6_fast(d, r, 1 / r, tau, r / P.cP, r / P.cS, icP2, icS2, nu);
 #pragma unroll
                 for (int c = 0; c < 6; c++) L[c] += w * nu[c];
             }
