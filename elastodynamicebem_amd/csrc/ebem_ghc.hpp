@@ -42,7 +42,8 @@ __device__ __attribute__((always_inline)) inline void nucleo6(const double x[3],
 // nucleo6 with 1/r (ir) taken once and the reciprocal squares of the wave
 // speeds (icP2, icS2) as products; r / cP and r / cS (rcP, rcS) stay
 // quotients, so the front tests decide as nucleo6's.  Values within a few
-// ulps of nucleo6's (used by the direct post-processing).
+// ulps of nucleo6's (used by the direct post-processing and the off-diagonal
+// numeric blocks).
 __device__ __attribute__((always_inline)) inline void nucleo6_fast(const double x[3], double r, double ir, double t,
                                                                    double rcP, double rcS, double icP2, double icS2,
                                                                    double o[6]) {
@@ -115,7 +116,8 @@ __global__ void __launch_bounds__(256) k_ghc_blocks(const FieldTri* __restrict__
         const double* vf = verts + (int64_t)j * 9;
         if (i != j) {
             // each level lag0-1 .. lag0+nc once, added to every lag of the chunk
-            // whose three levels contain it (per lag in kernelV.cu's order)
+            // whose three levels contain it (per lag in kernelV.cu's order; the
+            // kernel in nucleo6_fast's reciprocal form)
             double acc[kGhcChunk][6];
 #pragma unroll
             for (int u = 0; u < kGhcChunk; u++)
@@ -123,6 +125,7 @@ __global__ void __launch_bounds__(256) k_ghc_blocks(const FieldTri* __restrict__
                 for (int c = 0; c < 6; c++) acc[u][c] = 0.0;
             const double aF = tri[j].area;
             const double coef[3] = {1, -2, 1};
+            const double icP2 = 1 / (P.cP * P.cP), icS2 = 1 / (P.cS * P.cS);
             for (int q = 0; q < NGH; q++) {
                 const double x0 = g[q], x1 = g[32 + q], x2 = g[64 + q], wq = g[96 + q];
                 for (int k = lane; k < R.nInt; k += 64) {
@@ -133,13 +136,14 @@ __global__ void __launch_bounds__(256) k_ghc_blocks(const FieldTri* __restrict__
                     d[2] = (b[0] * vf[2] + b[1] * vf[5] + b[2] * vf[8]) - x2;
                     const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
                     const double w = wq * (b[3] * aF);
+                    const double ir = 1 / r, rcP = r / P.cP, rcS = r / P.cS;
 #pragma unroll
                     for (int li = 0; li < kGhcChunk + 2; li++) {
                         if (li >= nc + 2) break;
                         const double t = P.dt * (double)(lag0 - 1 + li);
                         if (t < 0) continue;
                         double nu[6];
-                        nucleo6(d, r, t, P.cP, P.cS, nu);
+                        nucleo6_fast(d, r, ir, t, rcP, rcS, icP2, icS2, nu);
 #pragma unroll
                         for (int u = 0; u < kGhcChunk; u++) {
                             const int v = li - u;
