@@ -204,9 +204,10 @@ def k_windows_and_flops(m, pb, LK, nGH, nInt, nDiag, rows=None):
     cut-off, kernelK.cu:226-260, calcSubBlockSingK.m:12) and the algorithmic
     FP64 operations of the assembly: the regular part by (test triangle S,
     field triangle F != S) over the positive levels of the union of the
-    windows of F's vertices, in chunks of 2 levels, each evaluation point once
-    per chunk for its geometry and once per level; singular parts per lag and
-    level.
+    windows of F's vertices, each evaluation point's geometry counted once per
+    K_SF_CHUNK = 2 levels (k_ksf recomputes it per level, which runs faster at
+    two waves per SIMD; the count does not charge that) and its kernel once
+    per level; singular parts per lag and level.
     rows = (r0, r1) restricts the test triangles."""
     X, T0 = m.coordinates, m.triangles
     r0, r1 = (0, T0.shape[0]) if rows is None else rows

@@ -285,7 +285,11 @@ __global__ void __launch_bounds__(256) k_ksf_items(const int16_t* __restrict__ s
 // One wave per chunk item: the level sums of pair (S, F) for F's three
 // vertices and the item's levels, butterflied:
 // part[(pOff + m - mlo) * 27 + 9 v + (i + 3 c)], mlo = k_sf_mlo(sflo).
-__global__ void __launch_bounds__(256) k_ksf(const KTri* __restrict__ kt, const double* __restrict__ verts,
+// waves per SIMD that k_ksf is compiled for (registers per wave <= 512 / this)
+#ifndef EBEM_KSF_WAVES
+#define EBEM_KSF_WAVES 2
+#endif
+__global__ void __launch_bounds__(256, EBEM_KSF_WAVES) k_ksf(const KTri* __restrict__ kt, const double* __restrict__ verts,
                                              const double* __restrict__ gh, const double* __restrict__ rin_g, int nGH,
                                              KParams K, int r0, int64_t nitems, const int32_t* __restrict__ itemP,
                                              const int16_t* __restrict__ itemC, const int16_t* __restrict__ sflo,

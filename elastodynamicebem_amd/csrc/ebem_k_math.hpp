@@ -532,9 +532,11 @@ __host__ __device__ __attribute__((always_inline)) inline void k_kernel3(const K
             }
 }
 
-// Time levels per wave of the field-triangle items of the regular K part.
+// Time levels per wave of the field-triangle items of the regular K part
+// (one: k_ksf then fits two waves per SIMD, which beats sharing each point's
+// geometry between two levels at one wave).
 #ifndef EBEM_KSF_CHUNK
-#define EBEM_KSF_CHUNK 2
+#define EBEM_KSF_CHUNK 1
 #endif
 constexpr int kKsfChunk = EBEM_KSF_CHUNK;
 

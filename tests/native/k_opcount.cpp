@@ -87,7 +87,9 @@ long long k_opcount_reg_point(int lag) {
 }
 
 // Flops of one evaluation point of the field-triangle form (one outer node,
-// one inner node of F, F's three hat functions) over the nL levels from mA.
+// one inner node of F, F's three hat functions) over the nL levels from mA
+// (this file is built with EBEM_KSF_CHUNK=2: the counted form shares a
+// point's geometry between two levels).
 long long k_opcount_sf_point(int mA, int nL) {
     KTri kt[2];
     const cdouble n[3] = {0, 0, 1};
