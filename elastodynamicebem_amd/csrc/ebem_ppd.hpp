@@ -303,7 +303,11 @@ __global__ void __launch_bounds__(256) k_ppf_items(const int16_t* __restrict__ l
 // One group of kPpfLanes lanes per item: the level sums of (pair q0 + qq, F)
 // for F's three vertices at the item's levels, butterflied in the group:
 // W[(woff + l) * 27 + 9 v + (i + 3 c)].
-__global__ void __launch_bounds__(256) k_ppf(const KTri* __restrict__ kt, const double* __restrict__ verts,
+// waves per SIMD that k_ppf is compiled for (registers per wave <= 512 / this)
+#ifndef EBEM_PPF_WAVES
+#define EBEM_PPF_WAVES 1
+#endif
+__global__ void __launch_bounds__(256, EBEM_PPF_WAVES) k_ppf(const KTri* __restrict__ kt, const double* __restrict__ verts,
                                              const double* __restrict__ xs, const double* __restrict__ ts,
                                              const int32_t* __restrict__ pq, int q0, PPDParams P,
                                              const double* __restrict__ rule_g, int64_t nitems,
