@@ -2216,6 +2216,7 @@ int ebem_pp_eval_direct(ebem_handle* h, const double* density, const double* g, 
     if (h->ppd_by_field) {
         // double-layer part by (pair, triangle) and level: level windows of
         // every pair, then batches of pairs whose level sums fit the budget
+        if (nT > 32767) return fail(EBEM_EINVAL, "too many time steps for the level windows (%d)", nT);
         DevBuf<int16_t> lF, itemC;
         DevBuf<int64_t> cnt, woff, base;
         DevBuf<int32_t> itemP;
