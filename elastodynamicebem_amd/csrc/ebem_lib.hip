@@ -1569,7 +1569,8 @@ int assemble_k(ebem_handle* h) {
         while (r0 < N) {
             int r1 = r0;
             int64_t acc = 0, accI = 0;
-            while (r1 < N && (r1 == r0 || acc + src[r1] <= cap)) {
+            // (rows of a chunk) x N stays below 2^31: the item map holds int32 pair indices
+            while (r1 < N && (r1 == r0 || (acc + src[r1] <= cap && (int64_t)(r1 - r0 + 1) * N <= INT32_MAX))) {
                 acc += src[r1];
                 accI += src[N + r1];
                 r1++;
