@@ -83,13 +83,40 @@ __global__ void __launch_bounds__(256) k_ppd_G(const FieldTri* __restrict__ tri,
                 for (int c = 0; c < 6; c++) L[c] += w * nu[c];
             }
         };
-        double Lc[6] = {0, 0, 0, 0, 0, 0};
+        // levels l and l + 1 in one pass over the nodes (each node's geometry
+        // once; per level the same sums, in the same order, as level())
+        auto level2 = [&](int l, double L1[6], double L2[6]) {
+            for (int c = 0; c < 6; c++) L1[c] = L2[c] = 0.0;
+            const double tau1 = t - (double)l * P.dt, tau2 = t - (double)(l + 1) * P.dt;
+            if (tau1 <= 0) return;
+            for (int k = lane; k < P.nInt; k += kPpfLanes) {
+                const double* b = rule + 4 * k;
+                double d[3];
+                for (int e = 0; e < 3; e++) d[e] = (b[0] * vf[e] + b[1] * vf[3 + e] + b[2] * vf[6 + e]) - x[e];
+                const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                const double w = b[3] * aF;
+                const double ir = 1 / r, rcP = r / P.cP, rcS = r / P.cS;
+                double nu[6];
+                nucleo6_fast(d, r, ir, tau1, rcP, rcS, icP2, icS2, nu);
+#pragma unroll
+                for (int c = 0; c < 6; c++) L1[c] += w * nu[c];
+                if (tau2 > 0) {
+                    nucleo6_fast(d, r, ir, tau2, rcP, rcS, icP2, icS2, nu);
+#pragma unroll
+                    for (int c = 0; c < 6; c++) L2[c] += w * nu[c];
+                }
+            }
+        };
+        double Lc[6] = {0, 0, 0, 0, 0, 0}, Lnext[6] = {0, 0, 0, 0, 0, 0};
         if (nHat > 0 && !(dmin > P.cP * t * (1 + 1e-12))) level(0, Lc);
         for (int y = 0; y < nHat; y++) {
             const double t0 = t - (double)y * P.dt;
             if (dmin > P.cP * t0 * (1 + 1e-12)) break;  // group-uniform; exact zeros from here on
             double Ln[6];
-            level(y + 1, Ln);
+            if ((y & 1) == 0)
+                level2(y + 1, Ln, Lnext);  // levels y + 1 and y + 2
+            else
+                for (int c = 0; c < 6; c++) Ln[c] = Lnext[c];
             double G[6];
 #pragma unroll
             for (int c = 0; c < 6; c++) {
