@@ -42,6 +42,8 @@ ALG_FLOP_PER_ITEM = 8103.8
 # passes FETCH_SIZE (x2.0: the counter reads 512 MiB for a 1 GiB device copy)
 # and WRITE_SIZE (x1.0): 84.2 GB + 33.3 GB, almost all register-spill scratch.
 K_LEVELS_TRAFFIC_BYTES = 117.5e9
+K_LEVELS_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this bench's k_levels "
+                           "(profiles/pmc_counters_sphere3_fp64.md), not measured in this run")
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # Algorithmic FP64 operations of the double-layer assembly, from the
@@ -107,13 +109,54 @@ def history_bytes_blocked(per_lag, nT, B=8):
     return blocks * 52
 
 
-def cpu_baseline(m, pb, seconds_hint=True):
-    """Oracle assembly on the first rows of every lag; pair.lags / s on the host."""
+def _cpu_info():
+    """Host cores this process may use and the CPU model (lscpu's model name)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = min(avail, int(env)) if env and env.isdigit() and int(env) > 0 else avail
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, {"nproc": os.cpu_count(), "affinity": avail, "omp_num_threads": env, "model": model}
+
+
+def _median3(fn):
+    """Median wall time of three runs of fn (and its last result)."""
+    ts, r = [], None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        r = fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), r
+
+
+def cpu_baseline(m, pb):
+    """The oracle (FP64 C/OpenMP restatement of the reference's SA path) on the
+    host cores this process may use (the GPU box allots a share of the
+    machine; OMP_NUM_THREADS states it), median of 3 runs per part:
+    * this workload's assembly on a sample of test rows (all lags), in the
+      metric's unit (pair.lag integrals/s) -- the reported value;
+    * the analytic post-processing of the solved density on this workload at a
+      sample of the shell points and times (single-threaded per evaluation, as
+      the reference's per-point loop);
+    * the whole plain method on the CPU-runnable 8x8 screen problem (128
+      triangles, 16 steps): assembly of E_0..E_15, the marching (LU of E_0 +
+      16 steps) and the post-processing at 16 points x 16 times."""
     from oracle import Oracle
-    threads = min(os.cpu_count() or 1, 16)
+    from elastodynamicebem_amd.mesh import read_mesh
+    threads, info = _cpu_info()
     o = Oracle(m.coordinates, m.triangles, Tfin=pb.Tfin, nT=pb.nT)
     L = o.num_blocks()
-    rows = (0, 96)
+    rows = (0, 32)
     c, mx = m.center, m.maxL
     dt = pb.Tfin / pb.nT
     d = np.sqrt(((c[rows[0]:rows[1], None, :] - c[None, :, :]) ** 2).sum(-1))
@@ -121,13 +164,36 @@ def cpu_baseline(m, pb, seconds_hint=True):
     dmax = d + mx[None, :] + mx[rows[0]:rows[1], None]
     units = sum(np.count_nonzero(((l - 1) * pb.velS * dt < dmax) & ((l + 1) * pb.velP * dt >= dmin))
                 for l in range(L))
-    t0 = time.perf_counter()
-    for l in range(L):
-        o.matrix_block(l, rows=rows, nthreads=threads)
-    el = time.perf_counter() - t0
-    return {"value": units / el, "unit": "pair·lag integrals/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C/OpenMP FP64 restatement of the reference SA path) assembling test rows "
-                      f"{rows[0]}-{rows[1] - 1} of E_0..E_{L - 1} ({units} culled pair.lags, {el:.1f} s)"}
+    t_asm, _ = _median3(lambda: [o.matrix_block(l, rows=rows, nthreads=threads) for l in range(L)])
+    # post-processing sample on this workload: a smooth random density (the
+    # evaluation's cost does not depend on the values), 4 points x 8 times
+    rng = np.random.default_rng(0)
+    dens = rng.normal(size=(3 * m.numberTriangles, pb.nT))
+    xs = shell_points()[:4]
+    ts = pb.Tfin * np.arange(1, 9) / 8
+    t_pp1, _ = _median3(lambda: [o.pp_point(dens, x, t) for x in xs for t in ts])
+    # the whole plain method on the screen problem
+    sm = read_mesh(os.path.join(ROOT, "tests", "golden", "meshes", "screenUniform_2.mesh"))
+    so = Oracle(sm.coordinates, sm.triangles, Tfin=1.0, nT=16)
+    sL = so.num_blocks()
+    t0_asm, E0 = _median3(lambda: np.stack([so.matrix_block(l, nthreads=threads)[0] for l in range(sL)]))
+    beta0 = so.rhs(0)
+    t0_mot, a0 = _median3(lambda: so.mot(E0, beta0))
+    x0 = np.stack([sm.center[j] + np.array([0.0, 0.0, 0.25]) for j in range(0, 128, 8)])
+    t0s = np.arange(1, 17) / 16
+    t0_pp, _ = _median3(lambda: [so.pp_point(a0, x, t) for x in x0 for t in t0s])
+    return {"value": units / t_asm, "unit": "pair·lag integrals/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C/OpenMP FP64 restatement of the reference SA path), median of 3: this "
+                      f"workload's assembly of test rows {rows[0]}-{rows[1] - 1} for E_0..E_{L - 1} "
+                      f"({units} culled pair.lags, {t_asm:.2f} s)",
+            "host": info,
+            "parts": {
+                "workload_assembly_s": t_asm, "workload_assembly_pair_lags": int(units),
+                "workload_pp_s_per_point_time": t_pp1 / (len(xs) * len(ts)),
+                "workload_pp_sample": f"{len(xs)} shell points x {len(ts)} times, one thread per evaluation",
+                "screen8x8_assembly_s": t0_asm, "screen8x8_mot_s": t0_mot,
+                "screen8x8_pp_s": t0_pp, "screen8x8_pp_sample": "16 points x 16 times",
+            }}
 
 
 def _solve(s, api, xpts, tpts, precision):
@@ -171,6 +237,7 @@ def chosen_forms(s, m, pb, xpts, tpts, train_samples=2_000_000, epochs=20):
     x, y = sur.get_samples(ntr, nhe)
     lab_err = rel(sur.predict(x), y)
     sur.attach(s)
+    s.set_option(api.OPT_SURROGATE_SOLVE, 1)  # measured here against the FP64 solve
     res = {}
     for name, prec in (("surrogate", api.PREC_SURROGATE), ("mixed_surrogate", api.PREC_MIXED | api.PREC_SURROGATE)):
         ts, as_, us = _solve(s, api, xpts, tpts, prec)
@@ -307,7 +374,7 @@ def dd_main(args):
         "neumann_driver_ms": {"mot_neumann": neu_ms, "wall": neu_wall,
                               "note": "datum, beta_V, QR of K_0 + I_Gamma (rocSOLVER), 96 least-squares steps; "
                                       "outside the timed step"},
-        "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "K assembly (k_ksf + k_kgather + k_ksing)",
+        "roofline": {"bound": "valu", "pipe": "fp64-valu", "kernel": "K assembly (k_ksf + k_kgather + k_ksing)",
                      "achieved": ach,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
                      "traffic": None},
@@ -315,7 +382,7 @@ def dd_main(args):
     }
     if not args.no_cpu:
         from oracle import Oracle
-        threads = min(os.cpu_count() or 1, 16)
+        threads, _ = _cpu_info()
         o = Oracle(m.coordinates, m.triangles, rho=pb.rho, mu=pb.mu, lam=pb.lambda_, Tfin=pb.Tfin, nT=pb.nT,
                    nGH=nGH)
         o.set_ghc(nSub, nSing, nDiag)
@@ -411,9 +478,9 @@ def main():
     roof = None
     if ALG_FLOP_PER_ITEM:
         ach = ALG_FLOP_PER_ITEM * level_items / (lev_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "pipe": "fp64-valu", "kernel": "k_levels", "achieved": ach,
+        roof = {"bound": "valu", "pipe": "fp64-valu", "kernel": "k_levels", "achieved": ach,
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
-                "traffic": K_LEVELS_TRAFFIC_BYTES}
+                "traffic": K_LEVELS_TRAFFIC_BYTES, "traffic_source": K_LEVELS_TRAFFIC_SOURCE}
     hist_gbs = hb_streamed / (hist_ms * 1e-3) / 1e9
     out = {
         "metric": "element-pair·lag integrals/s + solve time at ≤1e-6 rel err vs FP64, one MI355X",

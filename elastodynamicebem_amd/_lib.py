@@ -56,6 +56,14 @@ SIGNATURES = {
     "ebem_set_pp_rule": (C.c_int, [_vp, C.c_int, C.c_int]),
     "ebem_pp_eval_numeric": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
     "ebem_timings": (C.c_int, [_vp, _dp, C.c_int]),
+    "ebem_analiticIntg_CC": (C.c_int, [_vp] + [C.c_double] * 7 + [_dp, _dp, _dp]),
+    "ebem_analiticIntg_SC_P": (C.c_int, [_vp] + [C.c_double] * 5 + [_dp, _dp, _dp]),
+    "ebem_analiticIntg_SC_SP": (C.c_int, [_vp] + [C.c_double] * 5 + [_dp, _dp, _dp]),
+    "ebem_analiticIntg_TC": (C.c_int, [_vp] + [C.c_double] * 5 + [_dp, _dp, _dp]),
+    "ebem_analiticIntg_TR_P": (C.c_int, [_vp] + [C.c_double] * 3 + [_dp, _dp, _dp]),
+    "ebem_analiticIntg_TR_SP": (C.c_int, [_vp] + [C.c_double] * 3 + [_dp, _dp, _dp]),
+    "ebem_analitic_intg": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _dp, _dp]),
+    "ebem_set_option": (C.c_int, [_vp, C.c_int, C.c_double]),
     "ebem_sur_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
     "ebem_sur_destroy": (C.c_int, [_vp]),
     "ebem_sur_add_samples": (C.c_int, [_vp, _vp, C.c_int64, _ip, _ip, _ip]),

@@ -15,6 +15,8 @@ cited below; the computation runs in the native HIP library (libebem_hip.so).
   BEMenerg_postProc_loadPoints   functions/BEMenerg_indir/BEMenerg_postProc/BEMenerg_postProc_loadPoints.m
   BEMenerg_postProcGPU_setupLayer / _setupSP
                                 functions/BEMenerg_indir/BEMenerg_postProc/BEMenerg_postProcGPU/*.m
+  BEMenerg_analiticIntg_{CC,SC_P,SC_SP,TC,TR_P,TR_SP}
+                                functions/BEMenerg_analiticIntg/*.m
 """
 from __future__ import annotations
 
@@ -32,6 +34,12 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_SURROGATE = 2  # flag: near-field pairs from the attached surrogate (surrogate.py)
 PREC_GHC = 4        # numeric inner quadrature of the reference's GPU path (kernelV.cu), FP64
+
+OPT_FAR_FACTOR = 1        # mixed form: FP32 beyond this many longest edges (default 3)
+OPT_SURROGATE_SOLVE = 2   # allow mot_solve on a surrogate-assembled store
+OPT_SURROGATE_PROJECT = 3  # moment projection of the surrogate pairs (default on)
+
+AI_CC, AI_SC_P, AI_SC_SP, AI_TC, AI_TR_P, AI_TR_SP = range(6)
 
 DATUM_SCREEN_SIN2 = 0
 DATUM_SPHERE_T2EXP = 1
@@ -51,7 +59,7 @@ _DOMAIN_DATUM = {"sphereUniform": DATUM_REF_SPHERE, "sphereNotUniform": DATUM_RE
 
 @dataclass
 class PbParam:
-    """pbParam of readInputFile.m:36-93."""
+    """pbParam of readInputFile.m:10-76."""
     rho: float = 1.0
     mu: float = 1.0
     nu: float = 0.25
@@ -97,7 +105,7 @@ def inner_rule_tables(nSub: int, nSing: int, nDiag: int):
 
 
 def readInputFile(path: str) -> PbParam:
-    """Ten '%header'/value line pairs (readInputFile.m:38-93)."""
+    """Ten '%header'/value line pairs (readInputFile.m:10-76)."""
     if not os.path.exists(path):
         raise FileNotFoundError("Impossibile aprire file coi dati del problema")
     with open(path, "r", errors="replace") as fh:
@@ -195,6 +203,25 @@ class Solver:
     def set_inner_rule(self, nSub: int, nSing: int, nDiag: int) -> None:
         """Inner rules of the numeric form (PREC_GHC)."""
         _lib.check(self.lib.ebem_set_inner_rule(self.h, nSub, nSing, nDiag))
+
+    def set_option(self, option: int, value: float) -> None:
+        """OPT_FAR_FACTOR / OPT_SURROGATE_SOLVE (include/ebem.h)."""
+        _lib.check(self.lib.ebem_set_option(self.h, int(option), float(value)))
+
+    def analitic_intg(self, kind, args, info, B) -> np.ndarray:
+        """Batch of region routines of functions/BEMenerg_analiticIntg/: kind (n,),
+        args (n, 8) = RS, rhoS, RP, rhoP, zeta, THETA1, THETA2, -; info (n, 12)
+        infoTri2D fields; B (n, 6, 3, 3).  Returns (n, 4, 3, 3): ris1..ris4, or
+        ris5, ris6 and two zero matrices."""
+        kind = np.ascontiguousarray(kind, np.int32).ravel()
+        n = kind.size
+        args = np.ascontiguousarray(args, np.float64).reshape(n, 8)
+        info = np.ascontiguousarray(info, np.float64).reshape(n, 12)
+        Bc = np.ascontiguousarray(np.asarray(B, np.float64).reshape(n, 6, 3, 3).transpose(0, 1, 3, 2))
+        out = np.zeros((n, 4, 3, 3))
+        _lib.check(self.lib.ebem_analitic_intg(self.h, n, kind.ctypes.data_as(_lib._ip), _lib.dptr(args),
+                                               _lib.dptr(info), _lib.dptr(Bc), _lib.dptr(out)))
+        return out.transpose(0, 1, 3, 2).copy()
 
     def num_blocks(self) -> int:
         L = C.c_int()
@@ -326,9 +353,17 @@ class Solver:
         _lib.check(self.lib.ebem_mot_solve_neumann(self.h, datum, out.ctypes.data_as(_lib._dp)))
         return out
 
+    def _density_arg(self, a, what):
+        if a is None:
+            return None
+        a = np.asfortranarray(a, dtype=np.float64)
+        if a.shape != (3 * self.N, self.nT):
+            raise ValueError(f"{what} must be 3N x nT = {(3 * self.N, self.nT)}, got {a.shape}")
+        return a
+
     def mot_solve(self, beta: np.ndarray | None = None) -> np.ndarray:
         out = np.zeros((3 * self.N, self.nT), order="F")
-        b = None if beta is None else np.asfortranarray(beta, dtype=np.float64)
+        b = self._density_arg(beta, "beta")
         _lib.check(self.lib.ebem_mot_solve(self.h, None if b is None else b.ctypes.data_as(_lib._dp),
                                            out.ctypes.data_as(_lib._dp)))
         return out
@@ -338,7 +373,7 @@ class Solver:
         x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
         t = np.ascontiguousarray(t, np.float64).ravel()
         out = np.zeros((t.size, x.shape[0], 3))
-        d = None if density is None else np.asfortranarray(density, dtype=np.float64)
+        d = self._density_arg(density, "density")
         _lib.check(self.lib.ebem_pp_eval(self.h, None if d is None else d.ctypes.data_as(_lib._dp), x.shape[0],
                                          _lib.dptr(x), t.size, _lib.dptr(t), _lib.dptr(out)))
         return out.transpose(2, 1, 0).copy()
@@ -352,7 +387,7 @@ class Solver:
         x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
         t = np.ascontiguousarray(t, np.float64).ravel()
         out = np.zeros((t.size, x.shape[0], 3))
-        d = None if density is None else np.asfortranarray(density, dtype=np.float64)
+        d = self._density_arg(density, "density")
         _lib.check(self.lib.ebem_pp_eval_numeric(self.h, None if d is None else d.ctypes.data_as(_lib._dp),
                                                  x.shape[0], _lib.dptr(x), t.size, _lib.dptr(t), _lib.dptr(out)))
         return out.transpose(2, 1, 0).copy()
@@ -366,7 +401,7 @@ class Solver:
         x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
         t = np.ascontiguousarray(t, np.float64).ravel()
         out = np.zeros((t.size, x.shape[0], 3))
-        d = None if density is None else np.asfortranarray(density, dtype=np.float64)
+        d = self._density_arg(density, "density")
         gg = None if g is None else np.asfortranarray(g, dtype=np.float64)
         if gg is not None and gg.shape != (3 * self.mesh.number_nodes, self.nT):
             raise ValueError("g must be 3Nv x nT")
@@ -527,7 +562,7 @@ def BEMenerg_dir_calcBetaKgpu(deltaT, pbParam, domainMesh, methodInfo, *, datum=
 
 def BEMenerg_postProcA_setup(pbParam, domainMesh, density, numPoints, xVal, tVal, *, device: int = 0,
                              save_dir: str | None = None):
-    """campoVett: xSize x tSize array of 3-vectors (postProcA_setup.m:1-26).  With
+    """campoVett: xSize x tSize array of 3-vectors (postProcA_setup.m:1-22).  With
     save_dir, also written as the reference does (:21-22):
     <save_dir>/<domainType><lev>_A_campoVett.mat."""
     xVal = np.asarray(xVal, float).reshape(-1, 3)
@@ -654,3 +689,105 @@ def BEMenerg_ppD_calc(pbParam, domainMesh, density, gK, methodInfo, x, t, PPn=No
         s.set_pp_rule(nsub, nsing)
         u = s.pp_eval_direct(np.asarray(x, float).reshape(1, 3), np.array([float(t)]), density, g)
     return u[:, 0, 0].copy()
+
+
+# ---- functions/BEMenerg_analiticIntg/ -----------------------------------------
+
+class _RegionDevice:
+    """A mesh-less native handle on one device for the region routines."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self.lib.ebem_create(C.byref(h), device))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.ebem_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+_REGION_DEV: dict = {}
+
+
+def _region_call(fn: str, scalars, infoTri2D, B, nris: int, device: int):
+    d = _REGION_DEV.get(device)
+    if d is None:
+        d = _REGION_DEV[device] = _RegionDevice(device)
+    info = _info_array(infoTri2D)
+    Bc = np.ascontiguousarray(np.asarray(B, np.float64).reshape(3, 3, 6).transpose(2, 1, 0))  # B(:,:,k) col-major
+    out = np.zeros(9 * nris)
+    _lib.check(getattr(d.lib, fn)(d.h, *[float(v) for v in scalars], _lib.dptr(info), _lib.dptr(Bc),
+                                  _lib.dptr(out)))
+    return tuple(out[9 * k:9 * k + 9].reshape(3, 3).T.copy() for k in range(nris))
+
+
+_INFO_FIELDS = ("a", "b", "c", "cos_alpha", "cos_beta", "cos_gamma", "sin_alpha", "sin_beta", "sin_gamma",
+                "alpha", "beta", "gamma")
+
+
+def _info_array(infoTri2D) -> np.ndarray:
+    """infoTri2D as the 12 doubles of the C ABI: a dict / object with the fields
+    of calcoloDatiTriangolo.m:10-32, or already a 12-vector."""
+    if isinstance(infoTri2D, dict):
+        return np.array([float(infoTri2D[k]) for k in _INFO_FIELDS])
+    if hasattr(infoTri2D, "cos_gamma"):
+        return np.array([float(getattr(infoTri2D, k)) for k in _INFO_FIELDS])
+    a = np.ascontiguousarray(infoTri2D, np.float64).ravel()
+    if a.size != 12:
+        raise ValueError("infoTri2D must carry the 12 fields of calcoloDatiTriangolo")
+    return a
+
+
+def calcoloDatiTriangolo(verticiTriangolo):
+    """(tri2D, infoTri2D) of a 2-D triangle (geo/calcoloDatiTriangolo.m:1-39):
+    side lengths, cosines by the law of cosines, sines, angles, and the
+    triangle rotated so its first edge lies on +x."""
+    v = np.asarray(verticiTriangolo, float)
+    a = math.hypot(v[1, 0] - v[0, 0], v[1, 1] - v[0, 1])
+    b = math.hypot(v[2, 0] - v[0, 0], v[2, 1] - v[0, 1])
+    c = math.hypot(v[2, 0] - v[1, 0], v[2, 1] - v[1, 1])
+    ca = (b * b + c * c - a * a) / (2 * b * c)
+    cb = (a * a + c * c - b * b) / (2 * a * c)
+    cg = (a * a + b * b - c * c) / (2 * a * b)
+    info = dict(a=a, b=b, c=c, cos_alpha=ca, cos_beta=cb, cos_gamma=cg,
+                sin_alpha=math.sqrt(max(0.0, 1 - ca * ca)), sin_beta=math.sqrt(max(0.0, 1 - cb * cb)),
+                sin_gamma=math.sqrt(max(0.0, 1 - cg * cg)),
+                alpha=math.acos(max(-1.0, min(1.0, ca))), beta=math.acos(max(-1.0, min(1.0, cb))),
+                gamma=math.acos(max(-1.0, min(1.0, cg))))
+    tri2D = np.array([[0.0, 0.0], [a, 0.0], [b * cg, b * info["sin_gamma"]]])
+    return tri2D, info
+
+
+def BEMenerg_analiticIntg_CC(RS, rhoS, RP, rhoP, zeta, THETA1, THETA2, infoTri2D, B, *, device: int = 0):
+    """[ris1, ris2, ris3, ris4] of AI/BEMenerg_analiticIntg_CC.m (B: 3x3x6)."""
+    return _region_call("ebem_analiticIntg_CC", (RS, rhoS, RP, rhoP, zeta, THETA1, THETA2), infoTri2D, B, 4, device)
+
+
+def BEMenerg_analiticIntg_SC_P(RP, rhoP, zeta, THETA1, THETA2, infoTri2D, B, *, device: int = 0):
+    """[ris1, ris2, ris3, ris4] of AI/BEMenerg_analiticIntg_SC_P.m."""
+    return _region_call("ebem_analiticIntg_SC_P", (RP, rhoP, zeta, THETA1, THETA2), infoTri2D, B, 4, device)
+
+
+def BEMenerg_analiticIntg_SC_SP(RS, rhoS, zeta, THETA1, THETA2, infoTri2D, B, *, device: int = 0):
+    """[ris5, ris6] of AI/BEMenerg_analiticIntg_SC_SP.m."""
+    return _region_call("ebem_analiticIntg_SC_SP", (RS, rhoS, zeta, THETA1, THETA2), infoTri2D, B, 2, device)
+
+
+def BEMenerg_analiticIntg_TC(zeta, RS, rhoS, THETA1, THETA2, infoTri2D, B, *, device: int = 0):
+    """[ris1, ris2, ris3, ris4] of AI/BEMenerg_analiticIntg_TC.m."""
+    return _region_call("ebem_analiticIntg_TC", (zeta, RS, rhoS, THETA1, THETA2), infoTri2D, B, 4, device)
+
+
+def BEMenerg_analiticIntg_TR_P(zeta, THETA1, THETA2, infoTri2D, B, *, device: int = 0):
+    """[ris1, ris2, ris3, ris4] of AI/BEMenerg_analiticIntg_TR_P.m."""
+    return _region_call("ebem_analiticIntg_TR_P", (zeta, THETA1, THETA2), infoTri2D, B, 4, device)
+
+
+def BEMenerg_analiticIntg_TR_SP(zeta, THETA1, THETA2, infoTri2D, B, *, device: int = 0):
+    """[ris5, ris6] of AI/BEMenerg_analiticIntg_TR_SP.m."""
+    return _region_call("ebem_analiticIntg_TR_SP", (zeta, THETA1, THETA2), infoTri2D, B, 2, device)

@@ -9,7 +9,7 @@
 //   k_pair_items   per row: item offsets of every pair and the item -> pair map;
 //   k_levels       per (pair, level m) item: W_m = sum_q w_q * integral at
 //                  t_m = m dt over the field triangle (the time-level term of
-//                  calcMatrixSubBlock.m:259-337), rotated to the global frame.
+//                  calcMatrixSubBlock.m:20-111), rotated to the global frame.
 //                  Each W_m serves the three lags m-1, m, m+1, so the
 //                  analytic work is a third of the reference's;
 //   k_emit         per row: E_l = (W_{l+1} - 2 W_l + W_{l-1}) / (4 pi rho) with
@@ -45,6 +45,7 @@ struct AsmParams {
     int nGH;
     double far2;  // squared centroid distance above which a pair is far field (mixed precision); <0: none
     int kp;       // 1: levels of a pair in the kP state go through k_levels_kp (kp_range)
+    int sur_project;  // 1: surrogate pairs keep E_0 and the static level analytic and k_emit projects
 };
 
 // (lag-1)*cS*dt < dmax && (lag+1)*cP*dt >= dmin  (calcMatrixBlock.m:61), same
@@ -280,6 +281,7 @@ __device__ inline int front_class(double r, double lo, double span) {
 __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ tri, AsmParams P, int r0,
                                                    int64_t nitems, int64_t per, const int32_t* __restrict__ item_pair,
                                                    const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
+                                                   const int16_t* __restrict__ hi,
                                                    const int16_t* __restrict__ ka, const int16_t* __restrict__ kb,
                                                    const int16_t* __restrict__ sa, const int16_t* __restrict__ sb,
                                                    const uint8_t* __restrict__ sflag, uint8_t* __restrict__ key,
@@ -306,7 +308,13 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
         int k = qp * kFrontClasses + qs + ((P.far2 >= 0.0 && d2 > P.far2) ? 0 : kFrontKeys);
         if (ka && m >= ka[p] && m <= kb[p]) k = (m == ka[p]) ? kKpKey : kKpRestKey;
         if (sa && m >= sa[p] && m <= sb[p]) k = (m == sa[p]) ? bf_head_key(sa[p], sb[p]) : kBfRestKey;
-        if (sflag && sflag[p]) k = kSurKey;
+        // Surrogate pairs: the network predicts the lags max(lo,1)..hi-1; the
+        // analytic routine (FP64, level by level) keeps E_0 = W_1 / (4 pi rho),
+        // which the marching inverts, and the static level hi+1, which the
+        // moment conditions of k_emit need.
+        if (sflag && sflag[p])
+            k = ((m == 1 && l0 == 0) || (P.sur_project && m == hi[p] + 1)) ? qp * kFrontClasses + qs + kFrontKeys
+                                                                           : kSurKey;
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
     }
@@ -518,10 +526,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_B
 
 // One wave (64 threads) per row of the chunk.  Writes the blocks of lags
 // lo..hi of every pair of the row into the segmented store.
+//
+// Surrogate pairs (sflag set): slot l+1 holds the network's 4 pi rho E_l for
+// the lags max(lo,1)..hi-1, while slot 1 (if lo = 0: W_1 = 4 pi rho E_0) and
+// slot hi+1 (W_{hi+1}, the static level: every outer node already sees the
+// whole field triangle inside its S disk) come from the analytic routine.
+// The exact blocks of every pair satisfy sum_l E_l = W_{hi+1} - W_hi = 0 and
+// sum_l l E_l = -W_hi = -W_static (the second difference telescopes); the
+// first makes z = 1 an exact root of the marching's symbol, and a predicted
+// sequence that breaks it moves that root off the unit circle (the solve
+// then grows geometrically).  So the predicted lags get one constant shift
+// and E_hi is set so that both sums hold exactly.
 __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, const int16_t* __restrict__ hi,
                                              const int64_t* __restrict__ off, const double* __restrict__ W,
                                              const uint8_t* __restrict__ sflag, int N, int L, int r0,
-                                             double inv4pirho, double fourpirho, int ghc, Store S) {
+                                             double inv4pirho, double fourpirho, int ghc, int sur_project,
+                                             Store S) {
     extern __shared__ int64_t run[];  // L entries
     const int r = blockIdx.x;
     const int i = r0 + r;
@@ -533,12 +553,13 @@ __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, con
         int j = j0 + lane;
         int a = 1 << 30, b = -1;
         int64_t o = 0;
-        bool sur = false;
+        bool direct = false, sur = false;
         if (j < N) {
             a = lo[pbase + j];
             b = hi[pbase + j];
             o = off[pbase + j];
-            sur = ghc || (sflag && sflag[pbase + j]);
+            direct = ghc != 0;
+            sur = !direct && sflag && sflag[pbase + j];
         }
         int amin = a, bmax = b;
         for (int d = 32; d >= 1; d >>= 1) {
@@ -546,6 +567,35 @@ __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, con
             bmax = max(bmax, __shfl_xor(bmax, d));
         }
         const int mlo = a - 1 < 1 ? 1 : a - 1;
+        double shift[6] = {0, 0, 0, 0, 0, 0}, ehi[6] = {0, 0, 0, 0, 0, 0};
+        if (sur && sur_project && b >= 1) {
+            double S0[6], S1[6];
+            const double* e0 = W + o * 6;  // slot of level 1 (mlo = 1 when a = 0)
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                S0[q] = (a == 0) ? e0[q] : 0.0;
+                S1[q] = 0.0;
+            }
+            int n = 0;
+            double m1 = 0.0;
+            for (int l = (a > 1 ? a : 1); l < b; l++) {
+                const double* e = W + (o + (l + 1 - mlo)) * 6;
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    S0[q] += e[q];
+                    S1[q] += (double)l * e[q];
+                }
+                n++;
+                m1 += (double)l;
+            }
+            const double* ws = W + (o + (b + 1 - mlo)) * 6;
+            const double den = m1 - (double)b * (double)n;  // < 0 whenever n > 0
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                shift[q] = n > 0 ? ((double)b * S0[q] - S1[q] - ws[q]) / den : 0.0;
+                ehi[q] = -S0[q] - (double)n * shift[q];
+            }
+        }
         for (int l = amin; l <= bmax; l++) {
             bool in = (a <= l && l <= b);
             unsigned long long mask = __ballot(in);
@@ -557,14 +607,24 @@ __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, con
                 double e[6];
 #pragma unroll
                 for (int q = 0; q < 6; q++) e[q] = 1.0 * w1[q];
-                if (sur) {
-                    // surrogate pair or numeric form: the slot already holds 4 pi rho E_l
+                if (direct) {
+                    // numeric form: the slot already holds 4 pi rho E_l
+                } else if (sur && !sur_project) {
+                    // raw network output (E_0 still analytic)
+                } else if (sur) {
+                    if (l == b && l >= 1) {
+#pragma unroll
+                        for (int q = 0; q < 6; q++) e[q] = ehi[q];
+                    } else if (l >= 1) {
+#pragma unroll
+                        for (int q = 0; q < 6; q++) e[q] = e[q] + shift[q];
+                    }  // l = 0: W_1 = 4 pi rho E_0 from the analytic routine
                 } else if (l >= 1) {
                     const double* w0 = W + (o + (l - mlo)) * 6;
 #pragma unroll
                     for (int q = 0; q < 6; q++) e[q] = e[q] + (-2.0) * w0[q];
                 }
-                if (!sur && l >= 2) {
+                if (!direct && !sur && l >= 2) {
                     const double* wm = W + (o + (l - 1 - mlo)) * 6;
 #pragma unroll
                     for (int q = 0; q < 6; q++) e[q] = e[q] + wm[q];
@@ -587,6 +647,16 @@ __global__ void __launch_bounds__(64) k_emit(const int16_t* __restrict__ lo, con
             __syncthreads();
         }
     }
+}
+
+// One thread per analytic region call (ai_region): kind[k], args 8, info 12,
+// B 54 and ris 36 doubles per call.
+__global__ void __launch_bounds__(64) k_ai_regions(int n, const int32_t* __restrict__ kind,
+                                                   const double* __restrict__ args, const double* __restrict__ info,
+                                                   const double* __restrict__ B, double* __restrict__ ris) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    ai_region(kind[k], args + 8 * (int64_t)k, info + 12 * (int64_t)k, B + 54 * (int64_t)k, ris + 36 * (int64_t)k);
 }
 
 // Dense export of lag l: out (3N x 3N col-major), one thread per block of the segment.

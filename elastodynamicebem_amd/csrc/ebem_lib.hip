@@ -237,7 +237,7 @@ void gauss_legendre(int n, double* x, double* w) {
     }
 }
 
-// doppioGauss1D (util/setup/BEMenerg_setupCore.m:105-150): collapsed tensor
+// doppioGauss1D (util/quadRules/doppioGauss1D.m): collapsed tensor
 // Gauss rule, nodes ((1-x_j)(1-x_i), x_j(1-x_i), x_i), weights 2 w_i w_j (1-x_i)
 // from the Gauss-Legendre rule on (0, 1).  out: nNodes x 4.
 int doppio_gauss(int nNodes, std::vector<double>& out) {
@@ -265,7 +265,7 @@ int doppio_gauss(int nNodes, std::vector<double>& out) {
 double clamp1h(double x) { return x > 1.0 ? 1.0 : (x < -1.0 ? -1.0 : x); }
 
 // Triangle records: readSpaceMesh.m:93-139, calcoloSistemaRiferimento.m:1-45
-// and the rotation matrix of rotazioneInversaEulero.m:47-130.
+// and the rotation matrix of rotazioneInversaEulero.m:19-91.
 void build_tri(const double* X, const int* tri, int N, std::vector<FieldTri>& out) {
 #pragma clang fp contract(off)
     out.resize(N);
@@ -471,6 +471,9 @@ struct ebem_handle {
     int64_t pp_kp_stats[5] = {0, 0, 0, 0, 0};
     int64_t far_items = 0;
     int precision = 0;
+    double far_factor = 3.0;        // mixed form: FP32 beyond far_factor x longest edge (EBEM_OPT_FAR_FACTOR)
+    bool allow_sur_solve = false;   // EBEM_OPT_SURROGATE_SOLVE: MOT / PP on a surrogate-assembled store
+    bool sur_project = true;        // EBEM_OPT_SURROGATE_PROJECT: moment projection of surrogate pairs
 
     Store store() {
         Store S;
@@ -762,11 +765,12 @@ int assemble(ebem_handle* h, int precision) {
         // far field: centroid distance > 3h, h = longest edge of the mesh
         double hmax = 0;
         for (int t = 0; t < N; t++) hmax = std::max(hmax, h->tri[t].maxL);
-        P.far2 = mixed ? (3.0 * hmax) * (3.0 * hmax) : -1.0;
+        P.far2 = (mixed && h->far_factor > 0.0) ? (h->far_factor * hmax) * (h->far_factor * hmax) : -1.0;
     }
     // kP ranges need the key-sorted item order
     const bool kp = !ghc && h->sort_items && h->kp_split;
     P.kp = kp ? 1 : 0;
+    P.sur_project = h->sur_project ? 1 : 0;
     for (int k = 0; k < 5; k++) h->kp_stats[k] = 0;
     const double near2 = (3.0 * h->hmax) * (3.0 * h->hmax);  // surrogate pairs: centroid distance <= 3h
     Timer tall, tlev, tkp, tbf;
@@ -881,7 +885,8 @@ int assemble(ebem_handle* h, int precision) {
             HIPC(d_flag.grow((size_t)rows * N));
             const int64_t npr = (int64_t)rows * N;
             hipLaunchKernelGGL(k_sur_flag, dim3((unsigned)((npr + 255) / 256)), dim3(256), 0, h->stream, h->d_tri.p,
-                               h->d_verts.p, N, r0, rows, lo, hi, h->dt, h->cP, h->cS, h->hmax, near2,
+                               h->d_verts.p, N, L, P.sur_project, r0, rows, lo, hi, h->dt, h->cP, h->cS, h->hmax,
+                               near2,
                                sur->d_norm.p + 2 * kSurIn + 2 * kSurOut, d_flag.p, d_nsb.p);
             HIPC(hipGetLastError());
             sflag = d_flag.p;
@@ -894,7 +899,7 @@ int assemble(ebem_handle* h, int precision) {
             const int16_t* sa = kp ? d_sa.p + (int64_t)r0 * N : nullptr;
             const int16_t* sb = kp ? d_sb.p + (int64_t)r0 * N : nullptr;
             hipLaunchKernelGGL(k_item_keys, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, P, r0, acc, per, d_ip.p,
-                               d_off.p, lo, ka, kb, sa, sb, sflag, d_key.p, d_hist.p);
+                               d_off.p, lo, hi, ka, kb, sa, sb, sflag, d_key.p, d_hist.p);
             HIPC(hipGetLastError());
             HIPC(hipMemcpyAsync(hist.data(), d_hist.p, sizeof(unsigned int) * hist.size(), hipMemcpyDeviceToHost,
                                 h->stream));
@@ -1002,7 +1007,7 @@ int assemble(ebem_handle* h, int precision) {
             }
         }
         hipLaunchKernelGGL(k_emit, dim3(rows), dim3(64), L * sizeof(int64_t), h->stream, lo, hi, d_off.p, d_W.p, sflag,
-                           N, L, r0, inv4pirho, 4 * kPi * h->rho, ghc ? 1 : 0, S);
+                           N, L, r0, inv4pirho, 4 * kPi * h->rho, ghc ? 1 : 0, P.sur_project, S);
         HIPC(hipGetLastError());
         r0 = r1;
     }
@@ -1134,6 +1139,10 @@ int factor_e0(ebem_handle* h) {
 
 int mot(ebem_handle* h, const double* beta, double* alpha_out) {
     if (!h->assembled) return fail(EBEM_EINVAL, "ebem_assemble must run before ebem_mot_solve");
+    if ((h->precision & EBEM_PREC_SURROGATE) && !h->allow_sur_solve)
+        return fail(EBEM_EINVAL,
+                    "the store holds surrogate-predicted blocks, whose errors the time marching amplifies; "
+                    "set EBEM_OPT_SURROGATE_SOLVE to solve on it anyway");
     const int n3 = 3 * h->N, nT = h->nT;
     if (beta) {
         HIPC(h->d_beta.alloc((size_t)n3 * nT));
@@ -1810,26 +1819,38 @@ int ebem_set_mesh(ebem_handle* h, int N, int Nv, const double* coords, const int
     if (N <= 0 || Nv <= 0 || !coords || !tri) return fail(EBEM_EINVAL, "invalid mesh (N=%d, Nv=%d)", N, Nv);
     if ((int64_t)N * N > (int64_t)1 << 31) return fail(EBEM_EINVAL, "mesh too large (N=%d)", N);
     HIPC(hipSetDevice(h->device));
-    h->N = N;
-    h->Nv = Nv;
-    h->X.assign(coords, coords + 3 * (size_t)Nv);
-    h->T.resize(3 * (size_t)N);
-    for (size_t k = 0; k < h->T.size(); k++) {
-        int v = tri[k] - (one_based ? 1 : 0);
-        if (v < 0 || v >= Nv) return fail(EBEM_EINVAL, "triangle %zu has vertex index %d out of range", k / 3, tri[k]);
-        h->T[k] = v;
-    }
-    build_tri(h->X.data(), h->T.data(), N, h->tri);
-    for (int t = 0; t < N; t++)
-        if (!(h->tri[t].area > 0)) return fail(EBEM_EINVAL, "triangle %d is degenerate", t);
-    h->have_mesh = true;
+    // Everything derived from the previous mesh (store, factors, densities,
+    // device geometry) is invalid from here on, whether or not this mesh is
+    // accepted: a rejected mesh leaves the handle without a mesh.
+    h->have_mesh = false;
     h->assembled = false;
     h->k_ready = false;
     h->kgeo_ready = false;
     h->have_gT = false;
     h->have_dn = false;
     h->have_beta = h->have_alpha = false;
-    return upload_geometry(h);
+    // validate into locals; the handle takes the mesh only once it is valid
+    std::vector<double> X(coords, coords + 3 * (size_t)Nv);
+    std::vector<int> T(3 * (size_t)N);
+    for (size_t k = 0; k < T.size(); k++) {
+        int v = tri[k] - (one_based ? 1 : 0);
+        if (v < 0 || v >= Nv) return fail(EBEM_EINVAL, "triangle %zu has vertex index %d out of range", k / 3, tri[k]);
+        T[k] = v;
+    }
+    for (size_t k = 0; k < X.size(); k++)
+        if (!std::isfinite(X[k])) return fail(EBEM_EINVAL, "vertex %zu has a non-finite coordinate", k / 3);
+    std::vector<FieldTri> F;
+    build_tri(X.data(), T.data(), N, F);
+    for (int t = 0; t < N; t++)
+        if (!(F[t].area > 0)) return fail(EBEM_EINVAL, "triangle %d is degenerate", t);
+    h->N = N;
+    h->Nv = Nv;
+    h->X.swap(X);
+    h->T.swap(T);
+    h->tri.swap(F);
+    if (int e = upload_geometry(h)) return e;
+    h->have_mesh = true;
+    return 0;
 }
 
 int ebem_set_problem(ebem_handle* h, double rho, double mu, double lambda, double Tfin, int nT) {
@@ -2389,6 +2410,96 @@ int ebem_node_integral(ebem_handle* h, const double* x, int tri, double t, doubl
     const int qq[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
     for (int k = 0; k < 9; k++) out9[k] = w[qq[k]];
     return 0;
+}
+
+int ebem_analitic_intg(ebem_handle* h, int n, const int32_t* kind, const double* args, const double* info,
+                       const double* B, double* ris) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (n < 0 || (n > 0 && (!kind || !args || !info || !B || !ris))) return fail(EBEM_EINVAL, "bad argument");
+    if (n > (1 << 24)) return fail(EBEM_EINVAL, "too many region calls in one batch (%d > %d)", n, 1 << 24);
+    for (int k = 0; k < n; k++)
+        if (kind[k] < EBEM_AI_CC || kind[k] > EBEM_AI_TR_SP) return fail(EBEM_EINVAL, "unknown region kind %d", kind[k]);
+    if (n == 0) return 0;
+    HIPC(hipSetDevice(h->device));
+    DevBuf<int32_t> dk;
+    DevBuf<double> d;
+    const size_t nn = (size_t)n;
+    HIPC(dk.alloc(nn));
+    HIPC(d.alloc(nn * (8 + 12 + 54 + 36)));
+    double *da = d.p, *di = da + 8 * nn, *dB = di + 12 * nn, *dr = dB + 54 * nn;
+    HIPC(hipMemcpyAsync(dk.p, kind, sizeof(int32_t) * nn, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(da, args, sizeof(double) * 8 * nn, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(di, info, sizeof(double) * 12 * nn, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(dB, B, sizeof(double) * 54 * nn, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_ai_regions, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, h->stream, n, dk.p, da, di, dB, dr);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(ris, dr, sizeof(double) * 36 * nn, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+static int ai_one(ebem_handle* h, int kind, double RS, double rhoS, double RP, double rhoP, double zeta, double TH1,
+                  double TH2, const double* info, const double* B, double* ris, int nris) {
+    if (!info || !B || !ris) return fail(EBEM_EINVAL, "bad argument");
+    const double a[8] = {RS, rhoS, RP, rhoP, zeta, TH1, TH2, 0.0};
+    const int32_t k = kind;
+    double r[36];
+    if (int e = ebem_analitic_intg(h, 1, &k, a, info, B, r)) return e;
+    std::memcpy(ris, r, sizeof(double) * 9 * nris);
+    return 0;
+}
+
+int ebem_analiticIntg_CC(ebem_handle* h, double RS, double rhoS, double RP, double rhoP, double zeta, double THETA1,
+                         double THETA2, const double* infoTri2D, const double* B, double* ris1to4) {
+    return ai_one(h, EBEM_AI_CC, RS, rhoS, RP, rhoP, zeta, THETA1, THETA2, infoTri2D, B, ris1to4, 4);
+}
+
+int ebem_analiticIntg_SC_P(ebem_handle* h, double RP, double rhoP, double zeta, double THETA1, double THETA2,
+                           const double* infoTri2D, const double* B, double* ris1to4) {
+    return ai_one(h, EBEM_AI_SC_P, 0.0, 0.0, RP, rhoP, zeta, THETA1, THETA2, infoTri2D, B, ris1to4, 4);
+}
+
+int ebem_analiticIntg_SC_SP(ebem_handle* h, double RS, double rhoS, double zeta, double THETA1, double THETA2,
+                            const double* infoTri2D, const double* B, double* ris56) {
+    return ai_one(h, EBEM_AI_SC_SP, RS, rhoS, 0.0, 0.0, zeta, THETA1, THETA2, infoTri2D, B, ris56, 2);
+}
+
+int ebem_analiticIntg_TC(ebem_handle* h, double zeta, double RS, double rhoS, double THETA1, double THETA2,
+                         const double* infoTri2D, const double* B, double* ris1to4) {
+    return ai_one(h, EBEM_AI_TC, RS, rhoS, 0.0, 0.0, zeta, THETA1, THETA2, infoTri2D, B, ris1to4, 4);
+}
+
+int ebem_analiticIntg_TR_P(ebem_handle* h, double zeta, double THETA1, double THETA2, const double* infoTri2D,
+                           const double* B, double* ris1to4) {
+    return ai_one(h, EBEM_AI_TR_P, 0.0, 0.0, 0.0, 0.0, zeta, THETA1, THETA2, infoTri2D, B, ris1to4, 4);
+}
+
+int ebem_analiticIntg_TR_SP(ebem_handle* h, double zeta, double THETA1, double THETA2, const double* infoTri2D,
+                            const double* B, double* ris56) {
+    return ai_one(h, EBEM_AI_TR_SP, 0.0, 0.0, 0.0, 0.0, zeta, THETA1, THETA2, infoTri2D, B, ris56, 2);
+}
+
+int ebem_set_option(ebem_handle* h, int option, double value) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    switch (option) {
+        case EBEM_OPT_FAR_FACTOR:
+            if (!std::isfinite(value)) return fail(EBEM_EINVAL, "far-field factor must be finite");
+            h->far_factor = value;
+            h->assembled = false;
+            h->have_alpha = false;
+            return 0;
+        case EBEM_OPT_SURROGATE_SOLVE:
+            h->allow_sur_solve = value != 0.0;
+            return 0;
+        case EBEM_OPT_SURROGATE_PROJECT:
+            h->sur_project = value != 0.0;
+            h->assembled = false;
+            h->have_alpha = false;
+            return 0;
+        default: return fail(EBEM_EINVAL, "unknown option %d", option);
+    }
 }
 
 int ebem_pp_eval(ebem_handle* h, const double* density, int npts, const double* x, int nt, const double* t,

@@ -6,15 +6,15 @@
 // (functions/BEMenerg_indir/BEMenerg_core/BEMenerg_coreSA/calcoloMatrice/ and
 // functions/BEMenerg_analiticIntg/); the evaluation is reorganised for the GPU:
 //
-//  * the six kernel integrals ris1..ris6 of calcMatrixSubBlock.m:267-272 are
+//  * the six kernel integrals ris1..ris6 of calcMatrixSubBlock.m:34-39 are
 //    never formed.  ris1/ris3/ris5 are multiples of the identity and ris2/4/6
-//    are linear in the six basis matrices B_1..B_6 of coeffB.m:150-160, so each
+//    are linear in the six basis matrices B_1..B_6 of coeffB.m:32-42, so each
 //    region routine folds its contribution straight into one scalar (identity
 //    part) and six basis coefficients, already weighted by the level
-//    coefficients of calcMatrixSubBlock.m:334-337;
+//    coefficients of calcMatrixSubBlock.m:101-104;
 //  * B_1..B_6 are sparse; the child matrix is built once per child from the six
 //    coefficients (6 symmetric entries, not 6 dense 3x3 products) and rotated
-//    once by the child angle theta0 (fieldTriangle.m:191-204);
+//    once by the child angle theta0 (fieldTriangle.m:57-70);
 //  * the inverse Euler rotation (rotazioneInversaEulero.m) depends only on the
 //    field triangle: it is applied once per (pair, level) by the caller, after
 //    the Gauss-Hammer sum over outer nodes, instead of 3x per node.
@@ -49,7 +49,7 @@ struct FieldTri {
 static_assert(sizeof(FieldTri) == 256, "FieldTri must stay 256 B");
 
 // Level (time) constants.  s = k1*r1 + k3*r3 + k5*r5 and
-// M = k2*ris2 + k4*ris4 + k6*ris6 reproduce calcMatrixSubBlock.m:334-337.
+// M = k2*ris2 + k4*ris4 + k6*ris6 reproduce calcMatrixSubBlock.m:101-104.
 template <typename R>
 struct LevelT {
     R t, rP, rS;
@@ -155,7 +155,7 @@ __host__ __device__ inline void region_annulus(bool cc, bool zpos, R RS, R rhoS,
         I5 = -iRP + iRS + (z2 / R(3.0)) * (iRP3 - iRS3);
         I6 = (R(1.0) / (R(3.0) * z2)) * (qP * qP * qP - qS * qS * qS);
         I7 = -R(1.0) / R(3.0) * (iRP3 - iRS3);
-    } else {  // SC_P.m:114-120
+    } else {  // SC_P.m:20-26
         I1 = RP - zeta;
         I2 = RP + z2 * iRP - R(2.0) * zeta;
         I3 = log((rhoP + RP) / zeta) - qP;
@@ -435,6 +435,52 @@ __host__ __device__ inline void region_edgeS(bool zpos, R zeta, R th1, R th2, co
     }
 }
 
+// One region routine of functions/BEMenerg_analiticIntg/ on its own, with the
+// argument lists of BEMenerg_analiticIntg_{CC,SC_P,SC_SP,TC,TR_P,TR_SP}.m:
+// kind 0..5 in that order; a = {RS, rhoS, RP, rhoP, zeta, TH1, TH2, -} (each
+// routine reads the ones its .m file takes); info = the twelve fields of
+// infoTri2D (calcoloDatiTriangolo.m:10-32) in ChildGeo order; B = B(:,:,1..6)
+// of coeffB.m, column-major.  ris = ris1..ris4 (P routines) or ris5, ris6 (S
+// routines, the last two matrices zero), column-major.  Each ris is the region
+// routine run with one level weight set to 1 and the others 0, so its
+// identity part and B-basis coefficients are exactly the G_i / I_i * Phi_i of
+// the .m file; ris = s I + sum_q b_q B_{q+1}.
+__host__ __device__ inline void ai_region(int kind, const double* a, const double* info, const double* B,
+                                          double* ris) {
+    const ChildGeo g{info[0], info[1], info[2], info[3], info[4], info[5],
+                     info[6], info[7], info[8], info[9], info[10], info[11]};
+    const double RS = a[0], rhoS = a[1], RP = a[2], rhoP = a[3], zeta = a[4], th1 = a[5], th2 = a[6];
+    const bool zpos = zeta > 1.0e-6;
+    const bool sreg = (kind == 2 || kind == 5);
+    const Phi<double> P = make_phi<double>(th1, th2, g);
+    for (int r = 0; r < 4; r++) {
+        double* o = ris + 9 * r;
+        for (int q = 0; q < 9; q++) o[q] = 0.0;
+        if (r >= (sreg ? 2 : 4)) continue;
+        Level K{};
+        if (sreg) (r == 0 ? K.k5 : K.k6) = 1.0;
+        else (r == 0 ? K.k1 : r == 1 ? K.k2 : r == 2 ? K.k3 : K.k4) = 1.0;
+        ChildAcc A{0.0, {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
+        switch (kind) {
+            case 0: region_annulus<double>(true, zpos, RS, rhoS, RP, rhoP, zeta, th1, th2, P, K, A); break;
+            case 1: region_annulus<double>(false, zpos, RS, rhoS, RP, rhoP, zeta, th1, th2, P, K, A); break;
+            case 2: region_sdisk<double>(zpos, RS, rhoS, zeta, th1, th2, P, K, A); break;
+            case 3:
+                if (zpos) region_edgeP<double>(true, RS, rhoS, zeta, th1, th2, g, P, K, A);
+                else region_tc_coplanar<double>(RS, th1, th2, g, P, K, A);
+                break;
+            case 4: region_edgeP<double>(false, RS, rhoS, zeta, th1, th2, g, P, K, A); break;
+            default: region_edgeS<double>(zpos, zeta, th1, th2, g, P, K, A); break;
+        }
+        if (r == 0 || r == 2) {  // ris1, ris3, ris5: multiples of the identity
+            o[0] = o[4] = o[8] = A.s;
+        } else {
+            for (int q = 0; q < 6; q++)
+                for (int e = 0; e < 9; e++) o[e] = o[e] + A.b[q] * B[9 * q + e];
+        }
+    }
+}
+
 enum : int { kNone = 0, kSPSP = 1, kSPP = 2, kP = 3 };
 
 // calcoloDatiTriangolo.m:1-39 for the child (O, V2, V3), O at the origin.
@@ -609,7 +655,7 @@ __host__ __device__ inline NodeGeo node_geo(const double* x, const FieldTri& F) 
     return G;
 }
 
-// calcMatrixSubBlock.m:290-314: in-plane front radii and integration class.
+// calcMatrixSubBlock.m:58-81: in-plane front radii and integration class.
 __host__ __device__ inline int classify(double zeta, double rP, double rS, double dMIN, double& rhoP, double& rhoS) {
 #pragma clang fp contract(off)
     if (zeta == 0.0) {
@@ -632,7 +678,7 @@ __host__ __device__ inline int classify(double zeta, double rP, double rS, doubl
 }
 
 // Child k of the fan around the projection (O, V_k, V_{k+1 mod 3}) in
-// counter-clockwise order (fieldTriangle.m:163-167); false for a degenerate
+// counter-clockwise order (fieldTriangle.m:22-33); false for a degenerate
 // child (|doubled area| < 1e-6), which the reference skips.  Selects keep the
 // fan in registers.
 __host__ __device__ __attribute__((always_inline)) inline bool fan_child(const NodeGeo& G, int k, double& C,
@@ -653,8 +699,8 @@ __host__ __device__ __attribute__((always_inline)) inline bool fan_child(const N
 }
 
 // Folds one child's coefficients into the node accumulator: child matrix
-// sum_i b_i B_i (coeffB.m:136-160, symmetric, child frame) rotated by
-// A(theta0) * M * A(theta0)' (fieldTriangle.m:171-204), theta0 the clockwise
+// sum_i b_i B_i (coeffB.m:20-42, symmetric, child frame) rotated by
+// A(theta0) * M * A(theta0)' (fieldTriangle.m:37-70), theta0 the clockwise
 // angle of V2 (cos = V2x/|V2|, sin = -V2y/|V2|), weighted by w sign(C).
 __host__ __device__ __attribute__((always_inline)) inline void add_child(const ChildAcc& A, const ChildGeo& g,
                                                                          double v2x, double v2y, double zs,
@@ -942,7 +988,7 @@ __host__ __device__ inline void kp_range(int nGH, double cP, double cS, double d
     }
 }
 
-// W = s I + C' M C  (rotazioneInversaEulero.m:133), symmetric output
+// W = s I + C' M C  (rotazioneInversaEulero.m:94), symmetric output
 // (w00, w01, w02, w11, w12, w22).
 __host__ __device__ inline void finish_rotation(const NodeAcc& acc, const double* Cm, double* W) {
     double M[3][3] = {{acc.m[0], acc.m[1], acc.m[2]}, {acc.m[1], acc.m[3], acc.m[4]}, {acc.m[2], acc.m[4], acc.m[5]}};

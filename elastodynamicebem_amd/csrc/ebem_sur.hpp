@@ -115,7 +115,7 @@ __host__ __device__ inline double sur_scale(const FieldTri& S, const FieldTri& F
 
 // E_lag[S, F] split into its P-front part (the ris1..ris4 terms, integrals over
 // the annulus c_S t < r < c_P t) and its S-front part (the ris5, ris6 terms,
-// over the disk r < c_S t) of calcMatrixSubBlock.m:334-337, each summed over the
+// over the disk r < c_S t) of calcMatrixSubBlock.m:101-104, each summed over the
 // levels lag+1, lag, lag-1 with weights 1, -2, 1 and divided by 4 pi rho
 // (calcMatrixSubBlock.m:101-111).  out: 9 P values then 9 S values, 3 x 3
 // row-major in the global frame.
@@ -395,10 +395,12 @@ __global__ void k_to_bf16(const float* __restrict__ p, uint16_t* __restrict__ pb
 // ------------------------------------------------------ assembly inference --
 
 // Surrogate pair flag of every pair of a row chunk: not coincident, centroid
-// distance <= sqrt(near2), and the features of its lag window inside the
-// training box [box[c], box[24 + c]].
+// distance <= sqrt(near2), a lag window that ends before the last lag L - 1
+// (so that its level hi + 1 is static, see k_emit), and the features of the
+// window inside the training box [box[c], box[24 + c]].
 __global__ void __launch_bounds__(256) k_sur_flag(const FieldTri* __restrict__ tri, const double* __restrict__ verts,
-                                                  int N, int r0, int rows, const int16_t* __restrict__ lo,
+                                                  int N, int L, int project, int r0, int rows,
+                                                  const int16_t* __restrict__ lo,
                                                   const int16_t* __restrict__ hi, double dt, double cP, double cS,
                                                   double h, double near2, const float* __restrict__ box,
                                                   uint8_t* __restrict__ flag, unsigned long long* __restrict__ nblk) {
@@ -414,7 +416,7 @@ __global__ void __launch_bounds__(256) k_sur_flag(const FieldTri* __restrict__ t
         l0 = lo[p];
         l1 = hi[p];
     }
-    if (p < (int64_t)rows * N && i != j && l0 <= l1) {
+    if (p < (int64_t)rows * N && i != j && l0 <= l1 && l1 < L - 1) {
         const FieldTri& S = tri[i];
         const FieldTri& F = tri[j];
         double v0 = F.center[0] - S.center[0], v1 = F.center[1] - S.center[1], v2 = F.center[2] - S.center[2];
@@ -430,7 +432,10 @@ __global__ void __launch_bounds__(256) k_sur_flag(const FieldTri* __restrict__ t
         }
     }
     if (p < (int64_t)rows * N) flag[p] = f;
-    if (f) atomicAdd(&cnt, (unsigned int)(l1 - l0 + 1));
+    // predicted blocks: lags max(lo,1)..hi-1 with the projection (E_0 and E_hi
+    // then come from the analytic routine), max(lo,1)..hi without it
+    const int np = f ? l1 - (l0 > 1 ? l0 : 1) + (project ? 0 : 1) : 0;
+    if (np > 0) atomicAdd(&cnt, (unsigned int)np);
     __syncthreads();
     if (threadIdx.x == 0 && cnt) atomicAdd(nblk, (unsigned long long)cnt);
 }

@@ -121,7 +121,7 @@ int ebem_assembly_stats(ebem_handle *h, int64_t *far_items, int64_t *fp64_items)
 
 /* Level ranges of the last ebem_assemble evaluated per pair rather than per
  * level.  kP ranges: every outer node in class kP of
- * calcMatrixSubBlock.m:290-314 (S sphere short of the field plane, P sphere
+ * calcMatrixSubBlock.m:58-81 (S sphere short of the field plane, P sphere
  * past the whole field triangle), where W_m = WP + t_m^2 WQ from one
  * evaluation of the t-independent region coefficients.  Between-front ranges
  * (after them, S sphere still short of the field triangle): the region terms
@@ -244,6 +244,74 @@ int ebem_pp_eval_numeric(ebem_handle *h, const double *density, int npts, const 
  * the kernel-combined, /(4 pi rho)-scaled 3x3 integral (column-major) over
  * triangle `tri` of the mesh for outer point x at time t. */
 int ebem_node_integral(ebem_handle *h, const double *x, int tri, double t, double *out9);
+
+/* ---- analytic region integrals (functions/BEMenerg_analiticIntg/) ----------
+ * The six closed-form region routines that the reference's child-triangle
+ * integration calls (SAc/BEMenerg_coreSA_calcIntgInt_childTriangle.m:126-234),
+ * evaluated on the handle's device with the same code the assembly kernels
+ * inline.  infoTri2D: the twelve fields of calcoloDatiTriangolo.m:10-32 in the
+ * order a, b, c, cos_alpha, cos_beta, cos_gamma, sin_alpha, sin_beta,
+ * sin_gamma, alpha, beta, gamma.  B: the six 3x3 matrices B(:,:,1..6) of
+ * AI/coefficientiB/BEMenerg_coreSA_coeffB.m, column-major (54 doubles).
+ * Outputs column-major 3x3: ris1..ris4 (36 doubles) for the P-front routines,
+ * ris5, ris6 (18 doubles) for the S-front ones.  The handle needs no mesh. */
+#define EBEM_AI_CC 0
+#define EBEM_AI_SC_P 1
+#define EBEM_AI_SC_SP 2
+#define EBEM_AI_TC 3
+#define EBEM_AI_TR_P 4
+#define EBEM_AI_TR_SP 5
+
+/* [ris1, ris2, ris3, ris4] = BEMenerg_analiticIntg_CC(RS, rhoS, RP, rhoP, zeta, THETA1, THETA2, infoTri2D, B)
+ * (AI/BEMenerg_analiticIntg_CC.m:1): P annulus between the S and P circles. */
+int ebem_analiticIntg_CC(ebem_handle *h, double RS, double rhoS, double RP, double rhoP, double zeta, double THETA1,
+                         double THETA2, const double *infoTri2D, const double *B, double *ris1to4);
+/* [ris1, ris2, ris3, ris4] = BEMenerg_analiticIntg_SC_P(RP, rhoP, zeta, THETA1, THETA2, infoTri2D, B)
+ * (AI/BEMenerg_analiticIntg_SC_P.m:1): P sector from the plane foot. */
+int ebem_analiticIntg_SC_P(ebem_handle *h, double RP, double rhoP, double zeta, double THETA1, double THETA2,
+                           const double *infoTri2D, const double *B, double *ris1to4);
+/* [ris5, ris6] = BEMenerg_analiticIntg_SC_SP(RS, rhoS, zeta, THETA1, THETA2, infoTri2D, B)
+ * (AI/BEMenerg_analiticIntg_SC_SP.m:1): S disk sector. */
+int ebem_analiticIntg_SC_SP(ebem_handle *h, double RS, double rhoS, double zeta, double THETA1, double THETA2,
+                            const double *infoTri2D, const double *B, double *ris56);
+/* [ris1, ris2, ris3, ris4] = BEMenerg_analiticIntg_TC(zeta, RS, rhoS, THETA1, THETA2, infoTri2D, B)
+ * (AI/BEMenerg_analiticIntg_TC.m:1): P region between the S circle and the edge. */
+int ebem_analiticIntg_TC(ebem_handle *h, double zeta, double RS, double rhoS, double THETA1, double THETA2,
+                         const double *infoTri2D, const double *B, double *ris1to4);
+/* [ris1, ris2, ris3, ris4] = BEMenerg_analiticIntg_TR_P(zeta, THETA1, THETA2, infoTri2D, B)
+ * (AI/BEMenerg_analiticIntg_TR_P.m:1): P region between the plane foot and the edge. */
+int ebem_analiticIntg_TR_P(ebem_handle *h, double zeta, double THETA1, double THETA2, const double *infoTri2D,
+                           const double *B, double *ris1to4);
+/* [ris5, ris6] = BEMenerg_analiticIntg_TR_SP(zeta, THETA1, THETA2, infoTri2D, B)
+ * (AI/BEMenerg_analiticIntg_TR_SP.m:1): S region bounded by the edge. */
+int ebem_analiticIntg_TR_SP(ebem_handle *h, double zeta, double THETA1, double THETA2, const double *infoTri2D,
+                            const double *B, double *ris56);
+/* Batch form of the six: n calls, kind[k] = EBEM_AI_*, args[8k..8k+7] =
+ * {RS, rhoS, RP, rhoP, zeta, THETA1, THETA2, unused} (each routine reads the
+ * ones in its argument list), info[12k..], B[54k..], ris[36k..] (ris1..ris4,
+ * or ris5, ris6 followed by 18 zeros).  Replaces the 13 call sites of
+ * calcIntgInt_childTriangle.m:129-222 for callers that batch them. */
+int ebem_analitic_intg(ebem_handle *h, int n, const int32_t *kind, const double *args, const double *info,
+                       const double *B, double *ris);
+
+/* Options of a handle.  No reference counterpart (the reference has neither
+ * form these select).
+ * EBEM_OPT_FAR_FACTOR: the mixed-precision form (EBEM_PREC_MIXED) evaluates in
+ *   FP32 the pairs whose centroid distance exceeds value x (longest edge of the
+ *   mesh); default 3; value <= 0 evaluates every pair in FP64.
+ * EBEM_OPT_SURROGATE_SOLVE: value != 0 lets ebem_mot_solve run on a store
+ *   assembled with EBEM_PREC_SURROGATE; without it that call fails, because
+ *   the network's block errors are amplified by the time marching.
+ * EBEM_OPT_SURROGATE_PROJECT: value != 0 (default) keeps, for every pair the
+ *   surrogate predicts, E_0 and the pair's static level on the analytic
+ *   routine and adjusts the predicted lags so that the pair's blocks sum to
+ *   zero over its lags and their first moment equals minus its static
+ *   integral, as the exact blocks do; 0 stores the network's blocks as they
+ *   come (E_0 still analytic).  Without the projection the marching diverges. */
+#define EBEM_OPT_FAR_FACTOR 1
+#define EBEM_OPT_SURROGATE_SOLVE 2
+#define EBEM_OPT_SURROGATE_PROJECT 3
+int ebem_set_option(ebem_handle *h, int option, double value);
 
 /* Timing of the last ebem_assemble / ebem_mot_solve / ebem_pp_eval call, in
  * milliseconds, from HIP events recorded on the library's stream; writes the

@@ -472,7 +472,7 @@ static void intg_SC_SP(double RS, double rhoS, double zeta, double TH1, double T
         I[1] = RS;
         I[2] = RS;
         phi_t P = phis(T1t, T2t, f->gamma, f->cos_gamma);
-        /* Phi(4) is written with its two terms swapped at SC_SP.m:205; same value */
+        /* Phi(4) is written with its two terms swapped at SC_SP.m:58; same value */
         m3_axpy(B1, P.p5, B[3]); m3_axpy(B1, P.p4, B[4]); m3_axpy(B1, P.p3, B[5]);
         m3_eye(r5, T1t * I[1]);
         m3_zero(r6); m3_axpy(r6, I[2], B1);
@@ -706,6 +706,84 @@ static void intg_TR_SP(double zeta, double TH1, double TH2, const tinfo_t *f, ma
     }
 }
 
+/* ------------------------------------------- per-region entry and trace -- */
+
+/* Region kinds of orc_intg and the trace: the six routines of
+ * functions/BEMenerg_analiticIntg/.  args = {RS, rhoS, RP, rhoP, zeta, TH1, TH2, 0}
+ * (each routine reads the ones in its argument list), info = the twelve
+ * fields of infoTri2D (calcoloDatiTriangolo.m:10-32: a, b, c, cos_alpha,
+ * cos_beta, cos_gamma, sin_alpha, sin_beta, sin_gamma, alpha, beta, gamma),
+ * B = B(:,:,1..6) column-major, ris = ris1..ris4 (P routines) or ris5, ris6
+ * (S routines) column-major. */
+enum { AI_CC = 0, AI_SC_P = 1, AI_SC_SP = 2, AI_TC = 3, AI_TR_P = 4, AI_TR_SP = 5 };
+
+static void info_unpack(const double *info, tinfo_t *f) {
+    f->a = info[0]; f->b = info[1]; f->c = info[2];
+    f->cos_alpha = info[3]; f->cos_beta = info[4]; f->cos_gamma = info[5];
+    f->sin_alpha = info[6]; f->sin_beta = info[7]; f->sin_gamma = info[8];
+    f->alpha = info[9]; f->beta = info[10]; f->gamma = info[11];
+}
+
+static void info_pack(const tinfo_t *f, double *info) {
+    const double v[12] = {f->a, f->b, f->c, f->cos_alpha, f->cos_beta, f->cos_gamma,
+                          f->sin_alpha, f->sin_beta, f->sin_gamma, f->alpha, f->beta, f->gamma};
+    memcpy(info, v, sizeof v);
+}
+
+static void intg_dispatch(int kind, const double *a, const tinfo_t *f, mat3 B[6], mat3 r[4]) {
+    for (int k = 0; k < 4; k++) m3_zero(r[k]);
+    switch (kind) {
+        case AI_CC: intg_CC(a[0], a[1], a[2], a[3], a[4], a[5], a[6], f, B, r[0], r[1], r[2], r[3]); break;
+        case AI_SC_P: intg_SC_P(a[2], a[3], a[4], a[5], a[6], f, B, r[0], r[1], r[2], r[3]); break;
+        case AI_SC_SP: intg_SC_SP(a[0], a[1], a[4], a[5], a[6], f, B, r[0], r[1]); break;
+        case AI_TC: intg_TC(a[4], a[0], a[1], a[5], a[6], f, B, r[0], r[1], r[2], r[3]); break;
+        case AI_TR_P: intg_TR_P(a[4], a[5], a[6], f, B, r[0], r[1], r[2], r[3]); break;
+        default: intg_TR_SP(a[4], a[5], a[6], f, B, r[0], r[1]); break;
+    }
+}
+
+int orc_intg(int kind, const double *args, const double *info, const double *Bin, double *ris) {
+    if (kind < AI_CC || kind > AI_TR_SP) return -1;
+    tinfo_t f;
+    info_unpack(info, &f);
+    mat3 B[6], r[4];
+    for (int k = 0; k < 6; k++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) B[k][i][j] = Bin[9 * k + i + 3 * j];
+    intg_dispatch(kind, args, &f, B, r);
+    for (int k = 0; k < 4; k++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) ris[9 * k + i + 3 * j] = r[k][i][j];
+    return 0;
+}
+
+typedef struct {
+    int cap, n;
+    int *kind;
+    double *args, *info, *B, *ris;
+} trace_t;
+static trace_t *g_trace; /* set only inside orc_region_trace (single-threaded use) */
+
+static void trace_call(int kind, double RS, double rhoS, double RP, double rhoP, double zeta, double TH1,
+                       double TH2, const tinfo_t *f, mat3 B[6]) {
+    trace_t *T = g_trace;
+    if (!T) return;
+    int k = T->n++;
+    if (k >= T->cap) return;
+    const double a[8] = {RS, rhoS, RP, rhoP, zeta, TH1, TH2, 0};
+    T->kind[k] = kind;
+    memcpy(T->args + 8 * k, a, sizeof a);
+    info_pack(f, T->info + 12 * k);
+    for (int q = 0; q < 6; q++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) T->B[54 * k + 9 * q + i + 3 * j] = B[q][i][j];
+    mat3 r[4];
+    intg_dispatch(kind, a, f, B, r);
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) T->ris[36 * k + 9 * q + i + 3 * j] = r[q][i][j];
+}
+
 /* ------------------------------------------------- child / field triangle -- */
 
 enum { T_NONE = 0, T_SPSP = 1, T_SPP = 2, T_P = 3 };
@@ -761,6 +839,8 @@ static void child_triangle(double t2[3][2], const tinfo_t *f, double zeta, doubl
         if (sp) intg_SC_SP(RS, rhoS, zeta, TH1, TH2, f, B, q5, q6);
         if (type == T_P) intg_SC_P(RP, rhoP, zeta, TH1, TH2, f, B, q1, q2, q3, q4);
         else intg_CC(RS, rhoS, RP, rhoP, zeta, TH1, TH2, f, B, q1, q2, q3, q4);
+        if (sp) trace_call(AI_SC_SP, RS, rhoS, RP, rhoP, zeta, TH1, TH2, f, B);
+        trace_call(type == T_P ? AI_SC_P : AI_CC, RS, rhoS, RP, rhoP, zeta, TH1, TH2, f, B);
         m3_axpy(R[0], 1, q1); m3_axpy(R[1], 1, q2); m3_axpy(R[2], 1, q3); m3_axpy(R[3], 1, q4);
         if (sp) { m3_axpy(R[4], 1, q5); m3_axpy(R[5], 1, q6); }
     }
@@ -768,18 +848,24 @@ static void child_triangle(double t2[3][2], const tinfo_t *f, double zeta, doubl
         if (sp) intg_SC_SP(RS, rhoS, zeta, TH2, TH3, f, B, q5, q6);
         if (type == T_P) intg_TR_P(zeta, TH2, TH5, f, B, q1, q2, q3, q4);
         else intg_TC(zeta, RS, rhoS, TH2, TH3, f, B, q1, q2, q3, q4);
+        if (sp) trace_call(AI_SC_SP, RS, rhoS, RP, rhoP, zeta, TH2, TH3, f, B);
+        if (type == T_P) trace_call(AI_TR_P, RS, rhoS, RP, rhoP, zeta, TH2, TH5, f, B);
+        else trace_call(AI_TC, RS, rhoS, RP, rhoP, zeta, TH2, TH3, f, B);
         m3_axpy(R[0], 1, q1); m3_axpy(R[1], 1, q2); m3_axpy(R[2], 1, q3); m3_axpy(R[3], 1, q4);
         if (sp) { m3_axpy(R[4], 1, q5); m3_axpy(R[5], 1, q6); }
     }
     if (TH3 < TH4) { /* E3 */
         if (sp) {
             intg_TR_SP(zeta, TH3, TH4, f, B, q5, q6);
+            trace_call(AI_TR_SP, RS, rhoS, RP, rhoP, zeta, TH3, TH4, f, B);
             m3_axpy(R[4], 1, q5); m3_axpy(R[5], 1, q6);
         }
     }
     if (TH4 < TH5) { /* E4 + E8 */
         if (sp) intg_SC_SP(RS, rhoS, zeta, TH4, TH5, f, B, q5, q6);
         intg_TC(zeta, RS, rhoS, TH4, TH5, f, B, q1, q2, q3, q4);
+        if (sp) trace_call(AI_SC_SP, RS, rhoS, RP, rhoP, zeta, TH4, TH5, f, B);
+        trace_call(AI_TC, RS, rhoS, RP, rhoP, zeta, TH4, TH5, f, B);
         m3_axpy(R[0], 1, q1); m3_axpy(R[1], 1, q2); m3_axpy(R[2], 1, q3); m3_axpy(R[3], 1, q4);
         if (sp) { m3_axpy(R[4], 1, q5); m3_axpy(R[5], 1, q6); }
     }
@@ -787,6 +873,8 @@ static void child_triangle(double t2[3][2], const tinfo_t *f, double zeta, doubl
         if (sp) intg_SC_SP(RS, rhoS, zeta, TH5, TH6, f, B, q5, q6);
         if (type == T_P) intg_SC_P(RP, rhoP, zeta, TH5, TH6, f, B, q1, q2, q3, q4);
         else intg_CC(RS, rhoS, RP, rhoP, zeta, TH5, TH6, f, B, q1, q2, q3, q4);
+        if (sp) trace_call(AI_SC_SP, RS, rhoS, RP, rhoP, zeta, TH5, TH6, f, B);
+        trace_call(type == T_P ? AI_SC_P : AI_CC, RS, rhoS, RP, rhoP, zeta, TH5, TH6, f, B);
         m3_axpy(R[0], 1, q1); m3_axpy(R[1], 1, q2); m3_axpy(R[2], 1, q3); m3_axpy(R[3], 1, q4);
         if (sp) { m3_axpy(R[4], 1, q5); m3_axpy(R[5], 1, q6); }
     }
@@ -866,8 +954,8 @@ static void rot_inv_apply(mat3 C, mat3 ris) {
     memcpy(ris, out, sizeof(mat3));
 }
 
-/* SAc/BEMenerg_coreSA_calcMatrixSubBlock.m:290-314 (identical in
- * BEMenerg_postProcA_calcMatrixBlock.m:69-92). */
+/* SAc/BEMenerg_coreSA_calcMatrixSubBlock.m:58-81 (identical in
+ * BEMenerg_postProcA_calcMatrixBlock.m:28-52). */
 static int classify(double zeta, double rP, double rS, double dMIN, double *rhoP, double *rhoS) {
     if (zeta == 0) {
         *rhoP = rP;
@@ -906,6 +994,18 @@ static int node_integrals(const orc_param *p, const double *x, const double *T, 
     return type;
 }
 
+int orc_region_trace(const orc_param *p, const double *x, const double *T, const double *n, double t, int cap,
+                     int *kind, double *args, double *info, double *B, double *ris) {
+    trace_t tr = {cap, 0, kind, args, info, B, ris};
+    sdr_t s;
+    sdr_compute(T, n, &s);
+    mat3 R[6];
+    g_trace = &tr;
+    node_integrals(p, x, T, n, &s, t, R);
+    g_trace = NULL;
+    return tr.n;
+}
+
 int orc_node_ris(const orc_param *p, const double *x, const double *T, const double *n, double t, double *out) {
     sdr_t s;
     sdr_compute(T, n, &s);
@@ -917,7 +1017,7 @@ int orc_node_ris(const orc_param *p, const double *x, const double *T, const dou
     return type;
 }
 
-/* Kernel combination of calcMatrixSubBlock.m:334-337 for one time t. */
+/* Kernel combination of calcMatrixSubBlock.m:101-104 for one time t. */
 static void combine(const orc_param *p, double t, mat3 R[6], mat3 out) {
     double vP2 = p->velP * p->velP, vS2 = p->velS * p->velS;
     double den = (p->velP * p->velS) * (p->velP * p->velS);
@@ -1294,7 +1394,7 @@ static void gauss_legendre(int n, double *x, double *w) {
     }
 }
 
-/* doppioGauss1D (util/setup/BEMenerg_setupCore.m:105-150): collapsed
+/* doppioGauss1D (util/quadRules/doppioGauss1D.m): collapsed
  * tensor rule on the reference triangle, nodes ((1-x_j)(1-x_i), x_j(1-x_i), x_i),
  * weights 2 w_i w_j (1-x_i), x, w the Gauss-Legendre rule on (0, 1);
  * node k = i + n1 j (MATLAB column-major reshape).  nNodes a square. */

@@ -50,9 +50,19 @@ int orc_gauss_hammer(int n, double *nodes, double *weights);
 /* L = min(nT, ceil(maxVertexDist/(velS*dt)) + 1)   (dir/BEMenerg_dir_calcNumBlocks.m:1-17) */
 int orc_num_blocks(const orc_param *p, int Nv, const double *coords);
 
+/* One analytic region routine of functions/BEMenerg_analiticIntg/ (kind 0..5 =
+ * CC, SC_P, SC_SP, TC, TR_P, TR_SP): args {RS, rhoS, RP, rhoP, zeta, TH1, TH2, 0},
+ * info the 12 fields of infoTri2D (calcoloDatiTriangolo.m:10-32), B 6 x (3x3)
+ * column-major; ris 4 x (3x3) column-major (ris1..ris4, or ris5, ris6). */
+int orc_intg(int kind, const double *args, const double *info, const double *B, double *ris);
+/* Every region call node_integrals makes for point x, triangle T at time t, in
+ * call order (arguments, infoTri2D, B, results of orc_intg); returns the number
+ * of calls (only the first cap are stored).  Not thread-safe. */
+int orc_region_trace(const orc_param *p, const double *x, const double *T, const double *n, double t, int cap,
+                     int *kind, double *args, double *info, double *B, double *ris);
 /* One outer point x, field triangle (vertices T row-major 3x3, unit normal n),
  * time t (> 0): the six kernel integrals ris1..ris6 of
- * SAc/BEMenerg_coreSA_calcMatrixSubBlock.m:288-322 *after* the inverse Euler
+ * SAc/BEMenerg_coreSA_calcMatrixSubBlock.m:84-97 *after* the inverse Euler
  * rotation of ris2/4/6.  out is 6 x 9 (each 3x3 column-major).  Returns the
  * integration class (1 SP_SP, 2 SP_P, 3 P, 0 skipped). */
 int orc_node_ris(const orc_param *p, const double *x, const double *T, const double *n,

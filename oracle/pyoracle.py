@@ -73,6 +73,11 @@ class Oracle:
         L.orc_num_blocks.restype = C.c_int
         L.orc_node_ris.argtypes = [C.POINTER(_Param), _dp, _dp, _dp, C.c_double, _dp]
         L.orc_node_ris.restype = C.c_int
+        L.orc_intg.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_intg.restype = C.c_int
+        L.orc_region_trace.argtypes = [C.POINTER(_Param), _dp, _dp, _dp, C.c_double, C.c_int, _ip, _dp, _dp, _dp,
+                                       _dp]
+        L.orc_region_trace.restype = C.c_int
         L.orc_pair_block.argtypes = [C.POINTER(_Param), C.POINTER(_Mesh), C.c_int, _dp, _dp, C.c_int,
                                      C.c_int, C.c_int, _dp]
         L.orc_pair_block.restype = C.c_int
@@ -130,6 +135,34 @@ class Oracle:
         n = np.ascontiguousarray(n, np.float64)
         cls = self.lib.orc_node_ris(C.byref(self.param), _p(x), _p(T), _p(n), float(t), _p(out))
         return out.reshape(6, 3, 3).transpose(0, 2, 1).copy(), cls
+
+    def intg(self, kind, args, info, B):
+        """One region routine of functions/BEMenerg_analiticIntg/ (kind 0..5 = CC,
+        SC_P, SC_SP, TC, TR_P, TR_SP): args (8,), info (12,), B (6, 3, 3) ->
+        ris (4, 3, 3) (ris1..ris4, or ris5, ris6 and two zero matrices)."""
+        a = np.ascontiguousarray(args, np.float64)
+        f = np.ascontiguousarray(info, np.float64)
+        b = np.ascontiguousarray(np.asarray(B, np.float64).transpose(0, 2, 1))
+        out = np.zeros(36)
+        if self.lib.orc_intg(int(kind), _p(a), _p(f), _p(b), _p(out)) != 0:
+            raise ValueError("unknown region kind")
+        return out.reshape(4, 3, 3).transpose(0, 2, 1).copy()
+
+    def region_trace(self, x, T, n, t, cap=256):
+        """Every region call of the node integral of point x over triangle T at
+        time t: kinds (k,), args (k, 8), info (k, 12), B (k, 6, 3, 3), ris (k, 4, 3, 3)."""
+        x = np.ascontiguousarray(x, np.float64)
+        T = np.ascontiguousarray(T, np.float64)
+        n = np.ascontiguousarray(n, np.float64)
+        kind = np.zeros(cap, np.int32)
+        args, info = np.zeros((cap, 8)), np.zeros((cap, 12))
+        B, ris = np.zeros((cap, 54)), np.zeros((cap, 36))
+        k = self.lib.orc_region_trace(C.byref(self.param), _p(x), _p(T), _p(n), float(t), cap,
+                                      kind.ctypes.data_as(_ip), _p(args), _p(info), _p(B), _p(ris))
+        k = min(k, cap)
+        return (kind[:k].copy(), args[:k].copy(), info[:k].copy(),
+                B[:k].reshape(k, 6, 3, 3).transpose(0, 1, 3, 2).copy(),
+                ris[:k].reshape(k, 4, 3, 3).transpose(0, 1, 3, 2).copy())
 
     def pair_block(self, lag, S, F):
         out = np.zeros(9)

@@ -2,7 +2,7 @@
 
 For an outer point x, a flat field triangle T and a time t it integrates,
 over T, the six kernels the analytic routines of the reference produce
-(SAc/BEMenerg_coreSA_calcMatrixSubBlock.m:267-272):
+(SAc/BEMenerg_coreSA_calcMatrixSubBlock.m:34-39):
 
   ris1 = I * int_{T, cS t < r < cP t} 1/r          ris2 = int_{same} d d^T / r^3
   ris3 = I * int_{same} 1/r^3                      ris4 = int_{same} d d^T / r^5
@@ -122,7 +122,7 @@ def kernel_integrals(x, T, t, cP, cS):
 
 
 def combine(ris, t, cP, cS, rho=1.0):
-    """Single-layer kernel combination (calcMatrixSubBlock.m:334-344) of one level."""
+    """Single-layer kernel combination (calcMatrixSubBlock.m:101-104) of one level."""
     vP2, vS2 = cP * cP, cS * cS
     out = ((ris[0] - ris[1]) / vP2 + t * t * (-ris[2] + 3 * ris[3])
            + ((vP2 + vS2) * ris[4] + (vP2 - vS2) * ris[5]) / (cP * cS) ** 2) / 2

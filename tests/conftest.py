@@ -4,6 +4,11 @@ import sys
 import numpy as np
 import pytest
 
+# torch first: its bundled HIP runtime must be the one the process loads, as
+# in bench.py; loading the library's system runtime first leaves torch
+# without devices.
+import torch  # noqa: F401
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))

@@ -8,7 +8,7 @@ from conftest import ROOT
 def _declared():
     src = open(os.path.join(ROOT, "include", "ebem.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(ebem_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(ebem_[A-Za-z0-9_]+)\s*\(", src)))
 
 
 def test_header_symbols_exported():

@@ -320,3 +320,99 @@ def test_sur_out_of_box_pairs_fall_back(screen2):
             assert np.array_equal(g_, r)
     finally:
         s.close()
+
+
+@pytest.mark.gpu
+def test_sur_scatter_constant_network(sphere2):
+    """Pins the prediction-to-block path on its own: a network whose output is
+    a fixed label y0 (all weights zero, last bias y0, identity output
+    normalisation, training box unbounded) must give, for every near-field
+    pair (S, F) and every lag >= 1 of its window, the block
+    R_F^T sym(y0_P + y0_S) R_F * A_S A_F / (4 pi rho h) -- R_F the field
+    triangle's frame (first edge, normal x first edge, normal) -- to FP32
+    precision; E_0 and every far-field block stay bit-identical to the
+    analytic assembly, and the predicted lags form the pair's window without
+    lag 0 (a lag off by one would move or drop one).  That is the raw
+    prediction (EBEM_OPT_SURROGATE_PROJECT off); with the projection on, every
+    predicted pair's blocks sum to zero over its lags and their first moment
+    equals the plain form's (minus the static integral), as the exact blocks'
+    do."""
+    from elastodynamicebem_amd import api
+    from elastodynamicebem_amd import surrogate as SG
+    rng = np.random.default_rng(3)
+    y0 = rng.normal(size=SG.N_OUT).astype(np.float32)
+    p = np.zeros(SG.PARAMS, np.float32)
+    _, boff, _, _ = SG.layer_offsets()[-1]
+    p[boff:boff + SG.N_OUT] = y0
+    nm = np.zeros(SG.NORM, np.float32)
+    nm[24:48] = 1.0            # x 1/std
+    nm[48 + 18:48 + 36] = 1.0  # y std (y mean 0)
+    nm[84:108] = -1e30         # box lo
+    nm[108:132] = 1e30         # box hi
+    m = sphere2
+    pb = api.PbParam(rho=1.3, mu=1.0, lambda_=1.0, Tfin=4.0, nT=32)
+    s = SG.Surrogate()
+    try:
+        s.set_params(p, nm)
+        with api.Solver(m, pb, 7) as sol:
+            sol.assemble(api.PREC_FP64)
+            L = sol.num_blocks()
+            ref = np.stack([sol.get_block(l) for l in range(L)])
+            s.attach(sol)
+            sol.set_option(api.OPT_SURROGATE_PROJECT, 0)
+            sol.assemble(api.PREC_FP64 | api.PREC_SURROGATE)
+            st = sol.surrogate_stats()
+            got = np.stack([sol.get_block(l) for l in range(L)])
+            sol.set_option(api.OPT_SURROGATE_PROJECT, 1)
+            sol.assemble(api.PREC_FP64 | api.PREC_SURROGATE)
+            proj = np.stack([sol.get_block(l) for l in range(L)])
+            s.detach(sol)
+    finally:
+        s.close()
+    lagv = np.arange(L)[:, None, None]
+    n3 = proj.shape[1]
+    blk0 = lambda A: A.reshape(A.shape[0], n3 // 3, 3, n3 // 3, 3)
+    S0p = blk0(proj).sum(0)
+    S1p, S1r = blk0(lagv * proj).sum(0), blk0(lagv * ref).sum(0)
+    Sr = np.abs(blk0(ref)).max(0)  # scale of each pair's blocks
+    N = m.numberTriangles
+    V = m.coordinates[m.triangles]
+    c = V.mean(1)
+    d = np.sqrt(((c[:, None] - c[None]) ** 2).sum(-1))
+    h = m.lMax
+    near = (d <= 3 * h * (1 - 1e-12)) & ~np.eye(N, dtype=bool)
+    tol = (1e-12 * np.abs(S1r).max() + 1e-9 * Sr).transpose(0, 2, 1, 3)[near]
+    assert np.all(np.abs(S0p.transpose(0, 2, 1, 3)[near]) <= tol)
+    assert np.all(np.abs((S1p - S1r).transpose(0, 2, 1, 3)[near]) <= tol)
+    assert st["blocks"] > 0
+    # E_0 analytic (level 1 evaluated level by level, the plain form may have
+    # taken it from a kP range: same value up to that split's rounding)
+    assert np.abs(got[0] - ref[0]).max() <= 1e-13 * np.abs(ref[0]).max()
+    blk = lambda A, i, j: A[:, 3 * i:3 * i + 3, 3 * j:3 * j + 3]
+    M = (y0[:9] + y0[9:]).reshape(3, 3).astype(np.float64)
+    Ms = 0.5 * (M + M.T)
+    area = m.area
+    checked = 0
+    for i in range(N):
+        for j in range(N):
+            G, R0 = blk(got, i, j), blk(ref, i, j)
+            if not near[i, j]:
+                if d[i, j] > 3 * h * (1 + 1e-12):
+                    assert np.array_equal(G, R0), (i, j)
+                continue
+            e1 = V[j, 1] - V[j, 0]
+            e1 /= np.linalg.norm(e1)
+            n = m.normal[j]
+            R = np.array([e1, np.cross(n, e1), n])
+            want = R.T @ Ms @ R * area[i] * area[j] / (4 * np.pi * pb.rho * h)
+            lags = [l for l in range(1, L) if np.any(G[l] != 0)]
+            ref_lags = [l for l in range(1, L) if np.any(R0[l] != 0)]
+            if not lags:
+                assert not ref_lags, (i, j)
+                continue
+            assert lags == list(range(lags[0], lags[-1] + 1)), (i, j, lags)
+            assert set(ref_lags) <= set(lags), (i, j, ref_lags, lags)
+            for l in lags:
+                assert np.abs(G[l] - want).max() <= 1e-6 * np.abs(want).max(), (i, j, l)
+            checked += 1
+    assert checked > 100
