@@ -63,6 +63,7 @@ SIGNATURES = {
     "ebem_analiticIntg_TR_P": (C.c_int, [_vp] + [C.c_double] * 3 + [_dp, _dp, _dp]),
     "ebem_analiticIntg_TR_SP": (C.c_int, [_vp] + [C.c_double] * 3 + [_dp, _dp, _dp]),
     "ebem_analitic_intg": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _dp, _dp]),
+    "ebem_analitic_intg_noalloc": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _dp, _dp]),
     "ebem_set_option": (C.c_int, [_vp, C.c_int, C.c_double]),
     "ebem_sur_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
     "ebem_sur_destroy": (C.c_int, [_vp]),

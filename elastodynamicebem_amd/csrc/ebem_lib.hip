@@ -29,7 +29,23 @@
 
 using namespace ebem;
 
+// Staging of ebem_analitic_intg_noalloc: a fixed device array per GPU (each
+// GPU the library runs on holds its own copy until the process ends), so that
+// entry point allocates nothing.  Layout for kAiScratchCalls calls: args (8
+// doubles per call), infoTri2D (12), B (54), ris (36), then the kinds (int32).
+// Host code never uses the array's name as an address: ai_scratch_ptr[device]
+// holds the address hipGetSymbolAddress returned on that device.
+constexpr int kAiScratchCalls = 8192;
+constexpr size_t kAiScratchBytes = (size_t)kAiScratchCalls * (sizeof(double) * (8 + 12 + 54 + 36) + sizeof(int32_t));
+static __device__ __align__(256) unsigned char scratch[kAiScratchBytes];
+static_assert(kAiScratchCalls == EBEM_AI_NOALLOC_MAX_CALLS, "staging capacity of ebem_analitic_intg_noalloc");
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex ai_scratch_lock[kMaxDevices];     // serialises the calls that share one device's array
+void* ai_scratch_ptr[kMaxDevices] = {};      // device address of `scratch` per device (set once)
+std::mutex ai_scratch_init_lock;
 
 thread_local std::string g_err;
 
@@ -1756,6 +1772,19 @@ int ebem_create(ebem_handle** out, int device) {
     HIPC(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(EBEM_EINVAL, "device %d not available (%d devices)", device, ndev);
     HIPC(hipSetDevice(device));
+    if (device >= kMaxDevices) return fail(EBEM_EINVAL, "device %d beyond the library's %d", device, kMaxDevices);
+    {
+        // address of this device's staging array, once per device, with the device current
+        std::lock_guard<std::mutex> lk(ai_scratch_init_lock);
+        if (!ai_scratch_ptr[device]) {
+            void* ptr = nullptr;
+            hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !ptr)
+                return fail(EBEM_EHIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                            hipGetErrorString(e));
+            ai_scratch_ptr[device] = ptr;
+        }
+    }
     ebem_handle* h = new ebem_handle();
     h->device = device;
     hipError_t e = hipStreamCreate(&h->stream);
@@ -2433,6 +2462,36 @@ int ebem_analitic_intg(ebem_handle* h, int n, const int32_t* kind, const double*
     HIPC(hipMemcpyAsync(di, info, sizeof(double) * 12 * nn, hipMemcpyHostToDevice, h->stream));
     HIPC(hipMemcpyAsync(dB, B, sizeof(double) * 54 * nn, hipMemcpyHostToDevice, h->stream));
     hipLaunchKernelGGL(k_ai_regions, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, h->stream, n, dk.p, da, di, dB, dr);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(ris, dr, sizeof(double) * 36 * nn, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int ebem_analitic_intg_noalloc(ebem_handle* h, int n, const int32_t* kind, const double* args, const double* info,
+                               const double* B, double* ris) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (n < 0 || (n > 0 && (!kind || !args || !info || !B || !ris))) return fail(EBEM_EINVAL, "bad argument");
+    if (n > kAiScratchCalls)
+        return fail(EBEM_EINVAL, "ebem_analitic_intg_noalloc takes at most %d calls per batch (got %d)",
+                    kAiScratchCalls, n);
+    for (int k = 0; k < n; k++)
+        if (kind[k] < EBEM_AI_CC || kind[k] > EBEM_AI_TR_SP) return fail(EBEM_EINVAL, "unknown region kind %d", kind[k]);
+    if (n == 0) return 0;
+    void* base = ai_scratch_ptr[h->device];
+    if (!base) return fail(EBEM_EHIP, "no staging array on device %d", h->device);
+    HIPC(hipSetDevice(h->device));
+    std::lock_guard<std::mutex> dlk(ai_scratch_lock[h->device]);
+    const size_t nn = (size_t)n, cap = (size_t)kAiScratchCalls;
+    double* da = static_cast<double*>(base);
+    double *di = da + 8 * cap, *dB = di + 12 * cap, *dr = dB + 54 * cap;
+    int32_t* dk = reinterpret_cast<int32_t*>(dr + 36 * cap);
+    HIPC(hipMemcpyAsync(dk, kind, sizeof(int32_t) * nn, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(da, args, sizeof(double) * 8 * nn, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(di, info, sizeof(double) * 12 * nn, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(dB, B, sizeof(double) * 54 * nn, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_ai_regions, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, h->stream, n, dk, da, di, dB, dr);
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(ris, dr, sizeof(double) * 36 * nn, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));

@@ -294,6 +294,19 @@ int ebem_analiticIntg_TR_SP(ebem_handle *h, double zeta, double THETA1, double T
 int ebem_analitic_intg(ebem_handle *h, int n, const int32_t *kind, const double *args, const double *info,
                        const double *B, double *ris);
 
+/* The same batch without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * inputs and results are staged in a fixed array that each GPU holds for
+ * the life of the process (EBEM_AI_NOALLOC_MAX_CALLS calls, 7.2 MB); larger
+ * batches are rejected with EBEM_EINVAL.  Calls on one GPU share that array:
+ * the library serialises them (a per-device lock), so calls that overlap on
+ * one GPU wait for each other instead of racing; the array keeps the last
+ * call's contents.  Same results as ebem_analitic_intg, bit for bit. */
+#define EBEM_AI_NOALLOC_MAX_CALLS 8192
+int ebem_analitic_intg_noalloc(ebem_handle *h, int n, const int32_t *kind, const double *args, const double *info,
+                               const double *B, double *ris);
+
 /* Options of a handle.  No reference counterpart (the reference has neither
  * form these select).
  * EBEM_OPT_FAR_FACTOR: the mixed-precision form (EBEM_PREC_MIXED) evaluates in
