@@ -553,34 +553,37 @@ __host__ __device__ inline void child_regions(const ChildGeo& g, double zeta, do
     const R rs = R(RS), rhs = R(rhoS), rp = R(RP), rhp = R(rhoP), z = R(zeta);
     // One Phi per angular interval, shared by the S-sector and P routines on it
     // (identical arguments, identical values).
-    if (TH1 < TH2) {  // E1 (S sector) + E6 (P annulus)
-        Phi<R> P = make_phi<R>(R(TH1), R(TH2), gr);
-        if (sp) region_sdisk<R>(zpos, rs, rhs, z, R(TH1), R(TH2), P, K, A);
-        region_annulus<R>(type != kP, zpos, rs, rhs, rp, rhp, z, R(TH1), R(TH2), P, K, A);
+    // The two sector + annulus intervals (E1/E6 on [TH1, TH2], E5/E9 on
+    // [TH5, TH6]) run the same code, and so do the two intervals bounded by
+    // the edge (E2/E7 on [TH2, TH3], E4/E8 on [TH4, TH5]; TH5 == TH3 in class
+    // P, where E4 is empty): each pair is one loop whose body a wavefront
+    // executes as often as its busiest lane needs it, instead of once per
+    // interval that any lane needs (region branches are the main divergence
+    // of the assembly and post-processing kernels).  The regions are added in
+    // the order E1, E5, E2, E4, E3.
+    const int nA = (TH1 < TH2 ? 1 : 0) + (TH5 < TH6 ? 1 : 0);
+#pragma unroll 1
+    for (int it = 0; it < nA; it++) {
+        const bool first = (it == 0) && (TH1 < TH2);
+        const R lo = R(first ? TH1 : TH5), hi = R(first ? TH2 : TH6);
+        Phi<R> P = make_phi<R>(lo, hi, gr);
+        if (sp) region_sdisk<R>(zpos, rs, rhs, z, lo, hi, P, K, A);
+        region_annulus<R>(type != kP, zpos, rs, rhs, rp, rhp, z, lo, hi, P, K, A);
     }
-    if (TH2 < TH3) {  // E2 (S sector) + E7 (P region bounded by the edge); TH5 == TH3 for class P
-        Phi<R> P = make_phi<R>(R(TH2), R(TH3), gr);
-        if (sp) region_sdisk<R>(zpos, rs, rhs, z, R(TH2), R(TH3), P, K, A);
-        if (type == kP) region_edgeP<R>(false, rs, rhs, z, R(TH2), R(TH5), gr, P, K, A);
-        else if (!zpos) region_tc_coplanar<R>(rs, R(TH2), R(TH3), gr, P, K, A);
-        else region_edgeP<R>(true, rs, rhs, z, R(TH2), R(TH3), gr, P, K, A);
+    const int nB = (TH2 < TH3 ? 1 : 0) + (TH4 < TH5 ? 1 : 0);
+#pragma unroll 1
+    for (int it = 0; it < nB; it++) {
+        const bool first = (it == 0) && (TH2 < TH3);
+        const R lo = R(first ? TH2 : TH4), hi = R(first ? TH3 : TH5);
+        Phi<R> P = make_phi<R>(lo, hi, gr);
+        if (sp) region_sdisk<R>(zpos, rs, rhs, z, lo, hi, P, K, A);
+        if (type == kP) region_edgeP<R>(false, rs, rhs, z, lo, hi, gr, P, K, A);
+        else if (!zpos) region_tc_coplanar<R>(rs, lo, hi, gr, P, K, A);
+        else region_edgeP<R>(true, rs, rhs, z, lo, hi, gr, P, K, A);
     }
-    if (TH3 < TH4) {  // E3 (S region bounded by the edge)
-        if (sp) {
-            Phi<R> P = make_phi<R>(R(TH3), R(TH4), gr);
-            region_edgeS<R>(zpos, z, R(TH3), R(TH4), gr, P, K, A);
-        }
-    }
-    if (TH4 < TH5) {  // E4 (S sector) + E8 (P region between S circle and edge)
-        Phi<R> P = make_phi<R>(R(TH4), R(TH5), gr);
-        if (sp) region_sdisk<R>(zpos, rs, rhs, z, R(TH4), R(TH5), P, K, A);
-        if (!zpos) region_tc_coplanar<R>(rs, R(TH4), R(TH5), gr, P, K, A);
-        else region_edgeP<R>(true, rs, rhs, z, R(TH4), R(TH5), gr, P, K, A);
-    }
-    if (TH5 < TH6) {  // E5 (S sector) + E9 (P annulus)
-        Phi<R> P = make_phi<R>(R(TH5), R(TH6), gr);
-        if (sp) region_sdisk<R>(zpos, rs, rhs, z, R(TH5), R(TH6), P, K, A);
-        region_annulus<R>(type != kP, zpos, rs, rhs, rp, rhp, z, R(TH5), R(TH6), P, K, A);
+    if (TH3 < TH4 && sp) {  // E3 (S region bounded by the edge)
+        Phi<R> P = make_phi<R>(R(TH3), R(TH4), gr);
+        region_edgeS<R>(zpos, z, R(TH3), R(TH4), gr, P, K, A);
     }
 }
 
