@@ -40,8 +40,8 @@ sys.path.insert(0, ROOT)
 ALG_FLOP_PER_ITEM = 8103.8
 # HBM bytes per launch of k_levels on this workload from the rocprofv3 --pmc
 # passes FETCH_SIZE (x2.0: the counter reads 512 MiB for a 1 GiB device copy)
-# and WRITE_SIZE (x1.0): 84.2 GB + 33.3 GB, almost all register-spill scratch.
-K_LEVELS_TRAFFIC_BYTES = 117.5e9
+# and WRITE_SIZE (x1.0): 136.7 GB + 86.0 GB, almost all register-spill scratch.
+K_LEVELS_TRAFFIC_BYTES = 222.8e9
 K_LEVELS_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this bench's k_levels "
                            "(profiles/pmc_counters_sphere3_fp64.md), not measured in this run")
 PEAK_FP64_TFLOPS = 78.6

@@ -205,7 +205,7 @@ class Solver:
         _lib.check(self.lib.ebem_set_inner_rule(self.h, nSub, nSing, nDiag))
 
     def set_option(self, option: int, value: float) -> None:
-        """OPT_FAR_FACTOR / OPT_SURROGATE_SOLVE (include/ebem.h)."""
+        """OPT_FAR_FACTOR / OPT_SURROGATE_SOLVE / OPT_SURROGATE_PROJECT (include/ebem.h)."""
         _lib.check(self.lib.ebem_set_option(self.h, int(option), float(value)))
 
     def analitic_intg(self, kind, args, info, B) -> np.ndarray:
