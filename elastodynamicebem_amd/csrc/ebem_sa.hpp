@@ -115,6 +115,15 @@ __host__ __device__ __attribute__((always_inline)) inline Phi<R> make_phi(R th1,
     return P;
 }
 
+// sin and cos of one argument from one argument reduction: the same bits as
+// separate sin and cos calls (checked on the device), at about half the cost.
+__host__ __device__ __attribute__((always_inline)) inline void sincos_r(double x, double& s, double& c) {
+    sincos(x, &s, &c);
+}
+__host__ __device__ __attribute__((always_inline)) inline void sincos_r(float x, float& s, float& c) {
+    sincosf(x, &s, &c);
+}
+
 // Accumulator of one child: identity part and coefficients of B_1..B_6.
 struct ChildAcc {
     double s;
@@ -214,10 +223,8 @@ template <typename R>
 __host__ __device__ inline EdgeG<R> make_edge(R zeta, R th1, R th2, const ChildGeoT<R>& g) {
     EdgeG<R> E;
     E.F = g.a * g.sb;
-    E.s1b = sin(th1 + g.be);
-    E.s2b = sin(th2 + g.be);
-    E.c1b = cos(th1 + g.be);
-    E.c2b = cos(th2 + g.be);
+    sincos_r(th1 + g.be, E.s1b, E.c1b);
+    sincos_r(th2 + g.be, E.s2b, E.c2b);
     R R1 = E.F / E.s1b, R2 = E.F / E.s2b;
     R fz1 = zeta / R1, fz2 = zeta / R2;
     E.ADD = zeta * zeta + E.F * E.F;
@@ -349,8 +356,9 @@ __host__ __device__ inline CopPre<R> cop_pre(R th1, R th2, const ChildGeoT<R>& g
     CopPre<R> X;
     X.T1t = th2 - th1;
     const R F = g.a * g.sb;
-    R c1b = cos(th1 + g.be), c2b = cos(th2 + g.be);
-    R s1b = sin(th1 + g.be), s2b = sin(th2 + g.be);
+    R c1b, c2b, s1b, s2b;
+    sincos_r(th1 + g.be, s1b, c1b);
+    sincos_r(th2 + g.be, s2b, c2b);
     R g_3 = R(0.5) * (log((R(1.0) + c1b) * (R(1.0) - c2b)) - log((R(1.0) - c1b) * (R(1.0) + c2b)));
     R ca_g_1 = cos(g.al - g.ga + th1), ca_g_2 = cos(g.al - g.ga + th2);
     R cta1 = cos(th1 + g.al), cta2 = cos(th2 + g.al);

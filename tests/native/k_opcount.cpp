@@ -35,6 +35,12 @@ inline bool operator==(cdouble a, cdouble b) { return a.v == b.v; }
 inline bool operator!=(cdouble a, cdouble b) { return a.v != b.v; }
 #define CD1(f) inline cdouble f(cdouble a) { g_ops++; return cdouble(std::f(a.v)); }
 CD1(sqrt) CD1(sin) CD1(cos) CD1(log) CD1(acos) CD1(asin) CD1(exp)
+// sin and cos of one argument: one libm call
+inline void sincos(cdouble a, cdouble* s, cdouble* c) {
+    g_ops++;
+    *s = cdouble(std::sin(a.v));
+    *c = cdouble(std::cos(a.v));
+}
 inline cdouble fabs(cdouble a) { return cdouble(std::fabs(a.v)); }
 inline cdouble floor(cdouble a) { return cdouble(std::floor(a.v)); }
 inline cdouble ceil(cdouble a) { return cdouble(std::ceil(a.v)); }
