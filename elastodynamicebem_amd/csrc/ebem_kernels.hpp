@@ -251,12 +251,16 @@ constexpr int kFrontBins = EBEM_FRONT_BINS;
 constexpr int kFrontClasses = kFrontBins + 2;
 constexpr int kFrontKeys = kFrontClasses * kFrontClasses;
 // Key order = order of the sorted permutation: far-field items (mixed
-// precision) 0..kFrontKeys-1, the other per-level items, the first level of
-// each kP range (one k_levels_kp item per pair) and its other levels (no work
-// of their own), the same for the between-front ranges (k_levels_bf; their
-// first levels keyed by the number of chunks, so that a wavefront's ranges
-// take similar time), surrogate pairs last.
-constexpr int kKpKey = 2 * kFrontKeys;
+// precision) 0..kFrontKeys-1, the per-level items of coplanar pairs (every
+// outer node in the field plane: cheap coplanar branches, evaluated item per
+// lane), the other per-level items (evaluated by outer node), the first level
+// of each kP range (one k_levels_kp item per pair) and its other levels (no
+// work of their own), the same for the between-front ranges (k_levels_bf;
+// their first levels keyed by the number of chunks, so that a wavefront's
+// ranges take similar time), surrogate pairs last.
+constexpr int kCopKeys = kFrontKeys;     // kFrontKeys .. 2 kFrontKeys - 1
+constexpr int kNearKeys = 2 * kFrontKeys;  // .. 3 kFrontKeys - 1
+constexpr int kKpKey = 3 * kFrontKeys;
 constexpr int kKpRestKey = kKpKey + 1;
 constexpr int kBfHeadKeys = 4;
 constexpr int kBfKey = kKpKey + 2;  // .. kBfKey + kBfHeadKeys - 1
@@ -269,6 +273,23 @@ __device__ inline int bf_head_key(int a, int b) {
     return kBfKey + (chunks < kBfHeadKeys ? chunks : kBfHeadKeys) - 1;
 }
 static_assert(kNumKeys <= 256, "keys are stored as uint8");
+
+// Test triangle S within 1e-6 of the plane of F at all three vertices: every
+// outer node of the pair then has zeta = 0 in node_geo (the nodes are convex
+// combinations of the vertices).  Only routes the pair's items to a kernel;
+// both evaluate any node.
+__device__ inline bool coplanar_pair(const FieldTri& S, const FieldTri& F) {
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+        const double a = (v == 1) ? S.L1 : (v == 2 ? S.L3 * S.cosd : 0.0);
+        const double b = (v == 2) ? S.L3 * S.sind : 0.0;
+        double dn = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) dn += F.n[c] * (S.v0[c] + a * S.e1[c] + b * S.e2[c] - F.v0[c]);
+        if (fabs(dn) >= 1.0e-6) return false;
+    }
+    return true;
+}
 
 __device__ inline int front_class(double r, double lo, double span) {
     double u = (r - lo) / span;
@@ -305,7 +326,8 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
         double t = (double)m * P.dt;
         int qp = front_class(P.cP * t, d - ex, 2.0 * ex) + 1;
         int qs = front_class(P.cS * t, d - ex, 2.0 * ex) + 1;
-        int k = qp * kFrontClasses + qs + ((P.far2 >= 0.0 && d2 > P.far2) ? 0 : kFrontKeys);
+        const int near_key = qp * kFrontClasses + qs + (coplanar_pair(S, F) ? kCopKeys : kNearKeys);
+        int k = (P.far2 >= 0.0 && d2 > P.far2) ? qp * kFrontClasses + qs : near_key;
         if (ka && m >= ka[p] && m <= kb[p]) k = (m == ka[p]) ? kKpKey : kKpRestKey;
         if (sa && m >= sa[p] && m <= sb[p]) k = (m == sa[p]) ? bf_head_key(sa[p], sb[p]) : kBfRestKey;
         // Surrogate pairs: the network predicts the lags max(lo,1)..hi-1; the
@@ -313,8 +335,7 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
         // which the marching inverts, and the static level hi+1, which the
         // moment conditions of k_emit need.
         if (sflag && sflag[p])
-            k = ((m == 1 && l0 == 0) || (P.sur_project && m == hi[p] + 1)) ? qp * kFrontClasses + qs + kFrontKeys
-                                                                           : kSurKey;
+            k = ((m == 1 && l0 == 0) || (P.sur_project && m == hi[p] + 1)) ? near_key : kSurKey;
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
     }
@@ -380,6 +401,147 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_L
 #pragma unroll
         for (int q = 0; q < 6; q++) o[q] = Wv[q];
     }
+}
+
+// ---- per-level items by outer node --------------------------------------
+//
+// k_levels gives a lane a whole item, so at each step of its node loop a
+// wavefront runs the region code of every front state its lanes' nodes are
+// in.  Here the outer nodes of the per-level items are the units of work:
+// k_node_keys keys each (item, node) task by the node's front state
+// (classify: S and P circles crossing, only P, or P alone, and where each
+// circle's radius falls between the node's nearest and farthest distance to
+// the field triangle), tasks whose node no front reaches are dropped, a
+// counting sort by key orders the rest, k_node_eval integrates one node per
+// lane into its own slot, and k_node_reduce adds an item's node results in
+// node order and applies the rotation.  Task k * NGH + q is node q of the
+// k-th item of perm; its slot is part[(k * NGH + q) * 8 ..] (7 values and
+// one pad, a 64-byte line per task).
+constexpr int kNodeKeys = 48;
+constexpr uint8_t kNodeSkip = 255;
+
+__device__ inline int node_task_key(const NodeGeo& G, const Level& K) {
+    double rhoP, rhoS;
+    const int c = classify(G.zeta, K.rP, K.rS, G.dMIN, rhoP, rhoS);
+    if (c == kNone) return -1;
+    double dmax2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) dmax2 = fmax(dmax2, G.t[k][0] * G.t[k][0] + G.t[k][1] * G.t[k][1]);
+    const double dmin = G.dMIN, dmax = sqrt(dmax2);
+    auto bin = [&](double rho) {
+        if (rho < dmin) return 0;
+        if (rho >= dmax) return 3;
+        const int b = 1 + (int)(2.0 * (rho - dmin) / (dmax - dmin));
+        return b > 2 ? 2 : b;
+    };
+    return (c - 1) * 16 + bin(rhoP) * 4 + bin(rhoS);
+}
+
+template <int NGH>
+__global__ void __launch_bounds__(256) k_node_keys(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
+                                                   AsmParams P, int r0, int64_t nitems, int64_t per,
+                                                   const int32_t* __restrict__ item_pair,
+                                                   const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
+                                                   const int32_t* __restrict__ perm, uint8_t* __restrict__ tkey,
+                                                   unsigned int* __restrict__ bh) {
+    __shared__ unsigned int sh[kNodeKeys];
+    for (int k = threadIdx.x; k < kNodeKeys; k += blockDim.x) sh[k] = 0;
+    __syncthreads();
+    const int64_t a = blockIdx.x * per;
+    const int64_t e = a + per < nitems ? a + per : nitems;
+    for (int64_t k = a + threadIdx.x; k < e; k += blockDim.x) {
+        const int64_t it = perm[k];
+        const int32_t p = item_pair[it];
+        const int i = r0 + p / P.N, j = p % P.N;
+        const int l0 = lo[p];
+        const int m = (l0 - 1 < 1 ? 1 : l0 - 1) + (int)(it - off[p]);
+        const FieldTri& F = tri[j];
+        const double* g = gh + (int64_t)i * 4 * 32;
+        const Level K = make_level((double)m * P.dt, P.cP, P.cS);
+        for (int q = 0; q < NGH; q++) {
+            const double x[3] = {g[q], g[32 + q], g[64 + q]};
+            const int key = node_task_key(node_geo(x, F), K);
+            tkey[k * NGH + q] = key < 0 ? kNodeSkip : (uint8_t)key;
+            if (key >= 0) atomicAdd(&sh[key], 1u);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kNodeKeys; k += blockDim.x) bh[(int64_t)k * gridDim.x + blockIdx.x] = sh[k];
+}
+
+__global__ void __launch_bounds__(256) k_node_scatter(int64_t ntask, int64_t per, const uint8_t* __restrict__ tkey,
+                                                      const unsigned long long* __restrict__ bo,
+                                                      int32_t* __restrict__ tperm) {
+    __shared__ unsigned long long cur[kNodeKeys];
+    for (int k = threadIdx.x; k < kNodeKeys; k += blockDim.x) cur[k] = bo[(int64_t)k * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const int64_t a = blockIdx.x * per;
+    const int64_t e = a + per < ntask ? a + per : ntask;
+    for (int64_t t = a + threadIdx.x; t < e; t += blockDim.x) {
+        const uint8_t key = tkey[t];
+        if (key == kNodeSkip) continue;
+        const unsigned long long pos = atomicAdd(&cur[key], 1ull);
+        tperm[pos] = (int32_t)t;
+    }
+}
+
+template <int NGH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_LEVELS_WAVES)))
+k_node_eval(const FieldTri* __restrict__ tri, const double* __restrict__ gh, AsmParams P, int r0, int64_t ntask,
+            const int32_t* __restrict__ item_pair, const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
+            const int32_t* __restrict__ perm, const int32_t* __restrict__ tperm, double* __restrict__ part) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < ntask;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t task = tperm[s];
+        const int64_t k = task / NGH;
+        const int q = task - (int)k * NGH;
+        const int64_t it = perm[k];
+        const int32_t p = item_pair[it];
+        const int i = r0 + p / P.N, j = p % P.N;
+        const int l0 = lo[p];
+        const int m = (l0 - 1 < 1 ? 1 : l0 - 1) + (int)(it - off[p]);
+        const FieldTri& F = tri[j];
+        const double* g = gh + (int64_t)i * 4 * 32;
+        const Level K = make_level((double)m * P.dt, P.cP, P.cS);
+        const double x[3] = {g[q], g[32 + q], g[64 + q]};
+        const NodeGeo G = node_geo(x, F);
+        NodeAcc acc;
+        acc.s = 0.0;
+        for (int e = 0; e < 6; e++) acc.m[e] = 0.0;
+        node_eval<double>(G, K, g[96 + q], acc);
+        double* o = part + (int64_t)task * 8;
+        o[0] = acc.s;
+#pragma unroll
+        for (int e = 0; e < 6; e++) o[1 + e] = acc.m[e];
+        o[7] = 0.0;
+    }
+}
+
+template <int NGH>
+__global__ void __launch_bounds__(256) k_node_reduce(const FieldTri* __restrict__ tri, AsmParams P, int64_t nitems,
+                                                     const int32_t* __restrict__ item_pair,
+                                                     const int32_t* __restrict__ perm,
+                                                     const uint8_t* __restrict__ tkey,
+                                                     const double* __restrict__ part, double* __restrict__ W) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= nitems) return;
+    const int64_t it = perm[k];
+    const int j = item_pair[it] % P.N;
+    NodeAcc acc;
+    acc.s = 0.0;
+    for (int e = 0; e < 6; e++) acc.m[e] = 0.0;
+    for (int q = 0; q < NGH; q++) {
+        if (tkey[k * NGH + q] == kNodeSkip) continue;
+        const double* o = part + (k * NGH + q) * 8;
+        acc.s += o[0];
+#pragma unroll
+        for (int e = 0; e < 6; e++) acc.m[e] += o[1 + e];
+    }
+    double Wv[6];
+    finish_rotation(acc, tri[j].C, Wv);
+    double* w = W + it * 6;
+#pragma unroll
+    for (int e = 0; e < 6; e++) w[e] = Wv[e];
 }
 
 // One thread per kP range (the items of key kKpKey, the first level ka of the
@@ -701,23 +863,42 @@ __device__ inline double wg_sum(double v, double* red4) {
 #endif
 constexpr int kHistBlock = EBEM_HIST_BLOCK;
 
-// alpha step-major (the blocks of a row segment are in column order, so a
-// wave's reads of one step share cache lines; a triangle-major layout measured
-// 3x slower here).
+// Far part: one workgroup per test row.  The store keeps a row's lag
+// segments contiguous, so the row's lags 1..lmax are one flat range
+// [seg(i, 1), seg(i, lmax + 1)) that the workgroup walks with stride
+// blockDim; each thread tracks the lag of its block from the row's segment
+// starts, staged in LDS (kFlatSeg lags at a time).  Against a loop over the
+// lags with a strided loop over each segment (segments of a few hundred
+// blocks, each ending in a partly filled pass): 10.6 ms instead of 11.7 ms
+// for the 1280-triangle sphere at 128 steps.  Taking two blocks per thread
+// and iteration (all loads first), splitting a row between two workgroups,
+// and blocks of 12, 16 or 24 steps were measured no faster.  alpha stays
+// step-major: the blocks of a row segment are in column order, so a wave's
+// reads of one step share cache lines (a triangle-major copy was 2.5x
+// slower).
+constexpr int kFlatSeg = 1024;
+
 __global__ void __launch_bounds__(256) k_history_far(Store S, int N, int L, int n0, int nb,
                                                      const double* __restrict__ alpha, double* __restrict__ part) {
     __shared__ double red4[4];
+    __shared__ int64_t sseg[kFlatSeg + 1];
     const int i = blockIdx.x;
     const int64_t n3 = 3 * (int64_t)N;
     double acc[kHistBlock][3];
 #pragma unroll
     for (int q = 0; q < kHistBlock; q++) acc[q][0] = acc[q][1] = acc[q][2] = 0.0;
     const int lmax = min(L - 1, n0 + nb - 1);
-    for (int l = 1; l <= lmax; l++) {
-        const int bmin = max(0, l - n0), bmax = min(nb - 1, l - 1);
-        if (bmin > bmax) continue;
-        const int64_t b0 = S.seg[(int64_t)i * L + l], b1 = S.seg[(int64_t)i * L + l + 1];
-        for (int64_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+    for (int lc = 1; lc <= lmax; lc += kFlatSeg) {
+        const int nl = min(lmax - lc + 1, kFlatSeg);  // lags lc .. lc + nl - 1
+        __syncthreads();
+        for (int k = threadIdx.x; k <= nl; k += blockDim.x) sseg[k] = S.seg[(int64_t)i * L + lc + k];
+        __syncthreads();
+        const int64_t r1 = sseg[nl];
+        int li = 0;  // the thread's current block is in lag lc + li
+        for (int64_t b = sseg[0] + threadIdx.x; b < r1; b += blockDim.x) {
+            while (b >= sseg[li + 1]) li++;
+            const int l = lc + li;
+            const int bmin = max(0, l - n0), bmax = min(nb - 1, l - 1);
             const int j = S.col[b];
             const double w00 = S.v[0][b], w01 = S.v[1][b], w02 = S.v[2][b];
             const double w11 = S.v[3][b], w12 = S.v[4][b], w22 = S.v[5][b];
@@ -1289,7 +1470,7 @@ __global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tr
         double tau = P.delta + (double)m * P.dt;
         int qp = front_class(P.cP * tau, d - F.maxL, 2.0 * F.maxL) + 1;
         int qs = front_class(P.cS * tau, d - F.maxL, 2.0 * F.maxL) + 1;
-        int k = qp * kFrontClasses + qs + kFrontKeys;
+        int k = qp * kFrontClasses + qs + kNearKeys;
         if (ka && m >= ka[q] && m <= kb[q]) k = (m == ka[q]) ? kKpKey : kKpRestKey;
         if (sa && m >= sa[q] && m <= sb[q]) k = (m == sa[q]) ? bf_head_key(sa[q], sb[q]) : kBfRestKey;
         key[it] = (uint8_t)k;

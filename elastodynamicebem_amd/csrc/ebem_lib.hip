@@ -479,6 +479,14 @@ struct ebem_handle {
     bool have_dn = false;
     bool sort_items = true;
     bool kp_split = true;  // kP ranges through k_levels_kp (EBEM_KP_SPLIT=0 turns it off)
+    bool node_tasks = true;  // FP64 per-level items by outer node (EBEM_NODE_TASKS=0: k_levels, item per lane)
+    struct {
+        DevBuf<uint8_t> key;
+        DevBuf<int32_t> perm;
+        DevBuf<double> part;
+        DevBuf<unsigned int> hist;
+        DevBuf<unsigned long long> cur;
+    } nw;  // node-task scratch of ebem_assemble (grow-only)
     bool k_by_field = true;  // regular K part by (S, F) pairs + gather (EBEM_K_BY_FIELD=0: by (S, s) slots)
     bool ppd_by_field = true;  // direct PP double layer by (pair, F) + gather (EBEM_PPD_BY_FIELD=0: by vertex)
     // kP and between-front ranges of the last assembly / post-processing:
@@ -749,6 +757,72 @@ void launch_levels_split(ebem_handle* h, AsmParams P, int r0, int64_t nitems, in
     }
 }
 
+// FP64 per-level items perm[0..n) by outer node: keys, counting sort of the
+// node tasks, one node per lane, reduction per item (k_node_* in
+// ebem_kernels.hpp).  n * NGH < 2^31 (task ids are int32; the chunk budget
+// keeps it so).
+template <int NGH>
+int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t n, const int32_t* item_pair, const int64_t* off,
+                 const int16_t* lo, const int32_t* perm, double* W) {
+    if (n <= 0) return 0;
+    const int64_t ntask = n * NGH;
+    if (ntask > INT32_MAX) return fail(EBEM_EINVAL, "too many node tasks in one chunk");
+    auto& nw = h->nw;
+    const int G = h->CUs * 8;
+    HIPC(nw.key.grow((size_t)ntask));
+    HIPC(nw.perm.grow((size_t)ntask));
+    HIPC(nw.part.grow((size_t)ntask * 8));
+    HIPC(nw.hist.grow((size_t)kNodeKeys * G));
+    HIPC(nw.cur.grow((size_t)kNodeKeys * G));
+    const int64_t per = (n + G - 1) / G;
+    hipLaunchKernelGGL(k_node_keys<NGH>, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, r0, n, per,
+                       item_pair, off, lo, perm, nw.key.p, nw.hist.p);
+    HIPC(hipGetLastError());
+    std::vector<unsigned int> hist((size_t)kNodeKeys * G);
+    std::vector<unsigned long long> cur((size_t)kNodeKeys * G);
+    HIPC(hipMemcpyAsync(hist.data(), nw.hist.p, sizeof(unsigned int) * hist.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    unsigned long long run = 0;
+    for (size_t k = 0; k < hist.size(); k++) {  // key-major, block-minor
+        cur[k] = run;
+        run += hist[k];
+    }
+    HIPC(hipMemcpyAsync(nw.cur.p, cur.data(), sizeof(unsigned long long) * cur.size(), hipMemcpyHostToDevice,
+                        h->stream));
+    // block b of the scatter takes the tasks of the items block b keyed
+    hipLaunchKernelGGL(k_node_scatter, dim3(G), dim3(256), 0, h->stream, ntask, per * NGH, nw.key.p, nw.cur.p,
+                       nw.perm.p);
+    HIPC(hipGetLastError());
+    if (run > 0) {
+        int64_t blocks = std::min<int64_t>(((int64_t)run + 255) / 256, (int64_t)h->CUs * 16);
+        hipLaunchKernelGGL(k_node_eval<NGH>, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0, h->stream,
+                           h->d_tri.p, h->d_gh.p, P, r0, (int64_t)run, item_pair, off, lo, perm, nw.perm.p,
+                           nw.part.p);
+        HIPC(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_node_reduce<NGH>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->d_tri.p, P,
+                       n, item_pair, perm, nw.key.p, nw.part.p, W);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+// The per-level items perm[0..nitems) of a chunk, in key order: far-field
+// items (mixed precision, nfar) by k_levels in R_far, coplanar pairs (ncop)
+// by k_levels in FP64, the rest by outer node.  Unsorted items (perm null) or
+// EBEM_NODE_TASKS=0: all by k_levels.
+template <int NGH>
+int launch_per_level(ebem_handle* h, AsmParams P, int r0, int64_t nitems, int64_t nfar, int64_t ncop,
+                     const int32_t* item_pair, const int64_t* off, const int16_t* lo, const int32_t* perm,
+                     double* W) {
+    if (!(perm && h->node_tasks)) {
+        launch_levels_split<NGH>(h, P, r0, nitems, nfar, item_pair, off, lo, perm, W);
+        return 0;
+    }
+    if (nfar > 0) launch_levels<NGH, float>(h, P, r0, nfar, item_pair, off, lo, perm, W);
+    launch_levels<NGH, double>(h, P, r0, ncop, item_pair, off, lo, perm + nfar, W);
+    return launch_nodes<NGH>(h, P, r0, nitems - nfar - ncop, item_pair, off, lo, perm + nfar + ncop, W);
+}
+
 int assemble(ebem_handle* h, int precision) {
     if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set before ebem_assemble");
     if (precision < 0 || precision > (EBEM_PREC_MIXED | EBEM_PREC_SURROGATE | EBEM_PREC_GHC))
@@ -854,7 +928,11 @@ int assemble(ebem_handle* h, int precision) {
     // row chunks bounded by the W scratch budget
     size_t freeb = 0, totb = 0;
     HIPC(hipMemGetInfo(&freeb, &totb));
-    int64_t budget_items = (int64_t)std::min<double>(256.0e6, 0.35 * (double)freeb / (6 * 8 + 4 + 4 + 1));
+    // per item: W, pair index, permutation, key; with node tasks also a
+    // 64-byte slot, a key and a task index per outer node
+    const double item_bytes = 6 * 8 + 4 + 4 + 1 + (h->node_tasks ? h->nGH * (64.0 + 1 + 4) : 0.0);
+    int64_t budget_items = (int64_t)std::min<double>(256.0e6, 0.35 * (double)freeb / item_bytes);
+    if (h->node_tasks) budget_items = std::min<int64_t>(budget_items, (int64_t)INT32_MAX / h->nGH);
     if (budget_items < 1) return fail(EBEM_ENOMEM, "not enough device memory for assembly scratch");
     int64_t maxrow_items = *std::max_element(rowitems.begin(), rowitems.end());
     if (maxrow_items > budget_items) budget_items = maxrow_items;
@@ -931,8 +1009,9 @@ int assemble(ebem_handle* h, int precision) {
                                d_perm.p);
             HIPC(hipGetLastError());
             const int32_t* perm = (h->sort_items || mixed || sur) ? d_perm.p : nullptr;
-            int64_t nfar = 0, nsur = 0, nkp = 0, nkrest = 0, nbf = 0, nbrest = 0;
+            int64_t nfar = 0, ncop = 0, nsur = 0, nkp = 0, nkrest = 0, nbf = 0, nbrest = 0;
             for (size_t k = 0; k < (size_t)kFrontKeys * G; k++) nfar += hist[k];
+            for (size_t k = (size_t)kCopKeys * G; k < (size_t)kNearKeys * G; k++) ncop += hist[k];
             for (int b = 0; b < G; b++) {
                 nsur += hist[(size_t)kSurKey * G + b];
                 nkp += hist[(size_t)kKpKey * G + b];
@@ -970,11 +1049,17 @@ int assemble(ebem_handle* h, int precision) {
                     default: GHC_LAUNCH(19); break;
                 }
 #undef GHC_LAUNCH
-            } else switch (h->nGH) {
-                case 3: launch_levels_split<3>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
-                case 7: launch_levels_split<7>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
-                case 12: launch_levels_split<12>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
-                default: launch_levels_split<19>(h, P, r0, nlev, nfar, d_ip.p, d_off.p, lo, perm, d_W.p); break;
+            } else {
+                int e = 0;
+#define PL_LAUNCH(NG) launch_per_level<NG>(h, P, r0, nlev, nfar, ncop, d_ip.p, d_off.p, lo, perm, d_W.p)
+                switch (h->nGH) {
+                    case 3: e = PL_LAUNCH(3); break;
+                    case 7: e = PL_LAUNCH(7); break;
+                    case 12: e = PL_LAUNCH(12); break;
+                    default: e = PL_LAUNCH(19); break;
+                }
+#undef PL_LAUNCH
+                if (e) return e;
             }
             HIPC(hipGetLastError());
             tlev.stop(h->stream);
@@ -1800,6 +1885,8 @@ int ebem_create(ebem_handle** out, int device) {
     h->sort_items = !(order && std::strcmp(order, "pair") == 0);
     const char* kps = std::getenv("EBEM_KP_SPLIT");
     h->kp_split = !(kps && std::strcmp(kps, "0") == 0);
+    const char* nts = std::getenv("EBEM_NODE_TASKS");
+    h->node_tasks = !(nts && std::strcmp(nts, "0") == 0);
     const char* kbf = std::getenv("EBEM_K_BY_FIELD");
     h->k_by_field = !(kbf && std::strcmp(kbf, "0") == 0);
     const char* pbf = std::getenv("EBEM_PPD_BY_FIELD");

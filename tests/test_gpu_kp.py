@@ -4,7 +4,8 @@ assembly with every level through k_levels (EBEM_KP_SPLIT=0), on the
 1280-triangle sphere, T = 4, 128 steps.  Every lag matrix agrees to 1e-12
 (Frobenius-relative, absolute against ||E_0|| for lags below 1e-6 ||E_0||);
 the ranges cover a large share of the items and no range falls back to the
-level-by-level evaluation.  Parity of the default path with the oracle is in
+level-by-level evaluation.  Also the per-level items by outer node (default)
+against one item per lane.  Parity of the default path with the oracle is in
 test_gpu_parity.py (the screens there take the coplanar branch)."""
 import os
 
@@ -16,22 +17,64 @@ from conftest import ROOT
 pytestmark = pytest.mark.gpu
 
 
-def _assemble(split):
+def _assemble(split, node_tasks=True):
     from elastodynamicebem_amd import api
     from elastodynamicebem_amd.mesh import read_mesh
     m = read_mesh(os.path.join(ROOT, "tests", "golden", "meshes", "sphereUniform_3.mesh"))
     pb = api.PbParam(rho=1.0, mu=1.0, lambda_=1.0, Tfin=4.0, nT=128)
-    old = os.environ.get("EBEM_KP_SPLIT")
-    os.environ["EBEM_KP_SPLIT"] = "1" if split else "0"
+    env = {"EBEM_KP_SPLIT": "1" if split else "0", "EBEM_NODE_TASKS": "1" if node_tasks else "0"}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         s = api.Solver(m, pb, 7)
     finally:
-        if old is None:
-            del os.environ["EBEM_KP_SPLIT"]
-        else:
-            os.environ["EBEM_KP_SPLIT"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
     s.assemble(api.PREC_FP64)
     return s
+
+
+def _worst_lag_difference(a, b):
+    """Largest per-lag difference of two assemblies: Frobenius-relative, and
+    absolute against ||E_0|| for lags below 1e-6 ||E_0||."""
+    e0, worst = None, (0.0, -1)
+    for lag in range(a.num_blocks()):
+        G, R = a.get_block(lag), b.get_block(lag)
+        nr = np.linalg.norm(R)
+        if e0 is None:
+            e0 = nr
+        e = np.linalg.norm(G - R) / (nr if nr >= 1e-6 * e0 else e0)
+        worst = max(worst, (e, lag))
+    return worst
+
+
+def test_node_tasks_match_item_per_lane():
+    """The per-level items evaluated by outer node (node tasks sorted by front
+    state, default) against one item per lane (EBEM_NODE_TASKS=0), kP and
+    between-front ranges off so that every level goes through them: the two
+    add an item's node integrals in a different order only, so every lag
+    matrix agrees to 1e-12 and the solved densities to 1e-12."""
+    from elastodynamicebem_amd import api
+    a = _assemble(False, node_tasks=True)
+    b = _assemble(False, node_tasks=False)
+    try:
+        assert a.store_info() == b.store_info()
+        ta, tb = a.timings()["levels_ms"], b.timings()["levels_ms"]
+        e, lag = _worst_lag_difference(a, b)
+        print(f"per-level integrals by node {ta:.1f} ms vs by item {tb:.1f} ms; worst lag difference {e:.2e} (lag {lag})")
+        assert e < 1e-12, (lag, e)
+        a.rhs(api.DATUM_SPHERE_T2EXP)
+        b.rhs(api.DATUM_SPHERE_T2EXP)
+        da, db = a.mot_solve(None), b.mot_solve(None)
+        ed = np.linalg.norm(da - db) / np.linalg.norm(db)
+        print(f"density difference {ed:.2e}")
+        assert ed < 1e-12
+    finally:
+        a.close()
+        b.close()
 
 
 def test_kp_split_matches_per_level_assembly():
