@@ -419,7 +419,8 @@ class Solver:
     def timings(self) -> dict:
         keys = ["levels_ms", "assembly_ms", "factor_ms", "history_ms", "apply_ms", "mot_ms", "pp_ms",
                 "history_launches", "levels_kp_ms", "pp_kp_ms", "levels_bf_ms", "pp_bf_ms", "k_assembly_ms",
-                "beta_k_ms", "pp_direct_ms", "mot_neumann_ms"]
+                "beta_k_ms", "pp_direct_ms", "mot_neumann_ms", "node_tasks", "node_tasks_reused",
+                "node_reuse_fallbacks"]
         out = np.zeros(len(keys))
         _lib.check(self.lib.ebem_timings(self.h, _lib.dptr(out), len(keys)))
         return dict(zip(keys, out.tolist()))

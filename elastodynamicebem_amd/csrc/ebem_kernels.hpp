@@ -414,11 +414,28 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_L
 // the field triangle), tasks whose node no front reaches are dropped, a
 // counting sort by key orders the rest, k_node_eval integrates one node per
 // lane into its own slot, and k_node_reduce adds an item's node results in
-// node order and applies the rotation.  Task k * NGH + q is node q of the
-// k-th item of perm; its slot is part[(k * NGH + q) * 8 ..] (7 values and
-// one pad, a 64-byte line per task).
+// node order and applies the rotation.  Task it * NGH + q is node q of item
+// it (the chunk's item index); its slot is part[(it * NGH + q) * 8 ..]
+// (7 values and one pad, a 64-byte line per task).
+//
+// Static nodes: once both circles of a node lie beyond every child's edge
+// (node_static_pattern), node_eval returns the same bits at every later level
+// of the pair.  A task whose node is static with a margin (node_static_margin)
+// at its level and at the level before is not evaluated: its key says how
+// many levels back the node's first static level is, whose slot the
+// reduction takes.  k_node_eval checks the exact pattern at every first
+// static level whose value is reused; a failed check raises *bad, and the
+// caller then evaluates the chunk again without reuse.
 constexpr int kNodeKeys = 48;
-constexpr uint8_t kNodeSkip = 255;
+// post-processing items keyed by point state: keys 0..kNodeKeys, below the range keys
+static_assert(kNodeKeys < kKpKey, "point-state keys must precede the range keys");
+constexpr uint8_t kNodeSkip = 255;  // no front reaches the node: nothing to add
+// Reused task: key kNodeReuse + d, its value is the slot of the item d levels
+// before it (1 <= d <= kNodeReuseMax); a task further than kNodeReuseMax
+// levels from the node's first static level is evaluated again.
+constexpr int kNodeReuse = 64;
+constexpr int kNodeReuseMax = 254 - kNodeReuse;
+static_assert(kNodeKeys <= kNodeReuse, "reuse keys follow the state keys");
 
 __device__ inline int node_task_key(const NodeGeo& G, const Level& K) {
     double rhoP, rhoS;
@@ -437,15 +454,44 @@ __device__ inline int node_task_key(const NodeGeo& G, const Level& K) {
     return (c - 1) * 16 + bin(rhoP) * 4 + bin(rhoS);
 }
 
+// Level m of item it of pair p, and the pair's first level.
+__device__ inline int item_level(const int16_t* __restrict__ lo, const int64_t* __restrict__ off, int32_t p,
+                                 int64_t it, int& mfirst) {
+    const int l0 = lo[p];
+    mfirst = l0 - 1 < 1 ? 1 : l0 - 1;
+    return mfirst + (int)(it - off[p]);
+}
+
+// First level in [mfirst, mlast] at which node_static_margin holds (it holds
+// at every later level: it only grows with t), or mlast + 1 if none: the
+// level where c_S t clears the threshold, then corrected by evaluating the
+// predicate itself on either side.
+__device__ inline int node_first_static(const NodeGeo& G, int mfirst, int mlast, const AsmParams& P) {
+    double dmax2 = 0.0;
+    for (int k = 0; k < 3; k++) dmax2 = fmax(dmax2, G.t[k][0] * G.t[k][0] + G.t[k][1] * G.t[k][1]);
+    const double rs = sqrt(dmax2 * 1.002 + 1.0e-5 + G.zeta * G.zeta);
+    double est = ceil(rs / (P.cS * P.dt));
+    int m = est < (double)mfirst ? mfirst : (est > (double)(mlast + 1) ? mlast + 1 : (int)est);
+    auto stat = [&](int l) { return node_static_margin(G, make_level((double)l * P.dt, P.cP, P.cS)); };
+    while (m > mfirst && stat(m - 1)) m--;
+    while (m <= mlast && !stat(m)) m++;
+    return m;
+}
+
+// perm[0..nitems): the chunk's per-level items evaluated by node; block b
+// keys the tasks of the items perm[b per .. (b + 1) per) and counts them per
+// key in bh[key * gridDim + b] (key kNodeKeys: the reused tasks).
 template <int NGH>
 __global__ void __launch_bounds__(256) k_node_keys(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
                                                    AsmParams P, int r0, int64_t nitems, int64_t per,
                                                    const int32_t* __restrict__ item_pair,
                                                    const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
-                                                   const int32_t* __restrict__ perm, uint8_t* __restrict__ tkey,
+                                                   const int16_t* __restrict__ hi, const int32_t* __restrict__ perm,
+                                                   const uint8_t* __restrict__ sflag, int reuse,
+                                                   uint8_t* __restrict__ tkey,
                                                    unsigned int* __restrict__ bh) {
-    __shared__ unsigned int sh[kNodeKeys];
-    for (int k = threadIdx.x; k < kNodeKeys; k += blockDim.x) sh[k] = 0;
+    __shared__ unsigned int sh[kNodeKeys + 1];  // + reused tasks
+    for (int k = threadIdx.x; k <= kNodeKeys; k += blockDim.x) sh[k] = 0;
     __syncthreads();
     const int64_t a = blockIdx.x * per;
     const int64_t e = a + per < nitems ? a + per : nitems;
@@ -453,35 +499,50 @@ __global__ void __launch_bounds__(256) k_node_keys(const FieldTri* __restrict__ 
         const int64_t it = perm[k];
         const int32_t p = item_pair[it];
         const int i = r0 + p / P.N, j = p % P.N;
-        const int l0 = lo[p];
-        const int m = (l0 - 1 < 1 ? 1 : l0 - 1) + (int)(it - off[p]);
+        int mfirst;
+        const int m = item_level(lo, off, p, it, mfirst);
+        const int mlast = hi[p] + 1;
+        const bool can_reuse = reuse && m > mfirst && !(sflag && sflag[p]);
         const FieldTri& F = tri[j];
         const double* g = gh + (int64_t)i * 4 * 32;
         const Level K = make_level((double)m * P.dt, P.cP, P.cS);
         for (int q = 0; q < NGH; q++) {
             const double x[3] = {g[q], g[32 + q], g[64 + q]};
-            const int key = node_task_key(node_geo(x, F), K);
-            tkey[k * NGH + q] = key < 0 ? kNodeSkip : (uint8_t)key;
-            if (key >= 0) atomicAdd(&sh[key], 1u);
+            const NodeGeo G = node_geo(x, F);
+            const int key = node_task_key(G, K);
+            int tk = key < 0 ? kNodeSkip : key;
+            if (key >= 0 && can_reuse) {
+                const int d = m - node_first_static(G, mfirst, mlast, P);
+                if (d >= 1 && d <= kNodeReuseMax) tk = kNodeReuse + d;
+            }
+            tkey[it * NGH + q] = (uint8_t)tk;
+            if (tk < kNodeKeys) atomicAdd(&sh[tk], 1u);
+            else if (tk != kNodeSkip) atomicAdd(&sh[kNodeKeys], 1u);
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < kNodeKeys; k += blockDim.x) bh[(int64_t)k * gridDim.x + blockIdx.x] = sh[k];
+    for (int k = threadIdx.x; k <= kNodeKeys; k += blockDim.x) bh[(int64_t)k * gridDim.x + blockIdx.x] = sh[k];
 }
 
-__global__ void __launch_bounds__(256) k_node_scatter(int64_t ntask, int64_t per, const uint8_t* __restrict__ tkey,
+// Block b places the tasks k_node_keys' block b keyed (same partition).
+template <int NGH>
+__global__ void __launch_bounds__(256) k_node_scatter(int64_t nitems, int64_t per, const int32_t* __restrict__ perm,
+                                                      const uint8_t* __restrict__ tkey,
                                                       const unsigned long long* __restrict__ bo,
                                                       int32_t* __restrict__ tperm) {
     __shared__ unsigned long long cur[kNodeKeys];
     for (int k = threadIdx.x; k < kNodeKeys; k += blockDim.x) cur[k] = bo[(int64_t)k * gridDim.x + blockIdx.x];
     __syncthreads();
     const int64_t a = blockIdx.x * per;
-    const int64_t e = a + per < ntask ? a + per : ntask;
-    for (int64_t t = a + threadIdx.x; t < e; t += blockDim.x) {
-        const uint8_t key = tkey[t];
-        if (key == kNodeSkip) continue;
-        const unsigned long long pos = atomicAdd(&cur[key], 1ull);
-        tperm[pos] = (int32_t)t;
+    const int64_t e = a + per < nitems ? a + per : nitems;
+    for (int64_t k = a + threadIdx.x; k < e; k += blockDim.x) {
+        const int64_t it = perm[k];
+        for (int q = 0; q < NGH; q++) {
+            const uint8_t key = tkey[it * NGH + q];
+            if (key >= kNodeKeys) continue;
+            const unsigned long long pos = atomicAdd(&cur[key], 1ull);
+            tperm[pos] = (int32_t)(it * NGH + q);
+        }
     }
 }
 
@@ -489,17 +550,17 @@ template <int NGH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_LEVELS_WAVES)))
 k_node_eval(const FieldTri* __restrict__ tri, const double* __restrict__ gh, AsmParams P, int r0, int64_t ntask,
             const int32_t* __restrict__ item_pair, const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
-            const int32_t* __restrict__ perm, const int32_t* __restrict__ tperm, double* __restrict__ part) {
+            const int16_t* __restrict__ hi, const int32_t* __restrict__ tperm, int reuse,
+            double* __restrict__ part, unsigned int* __restrict__ bad) {
     for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < ntask;
          s += (int64_t)gridDim.x * blockDim.x) {
         const int32_t task = tperm[s];
-        const int64_t k = task / NGH;
-        const int q = task - (int)k * NGH;
-        const int64_t it = perm[k];
+        const int64_t it = task / NGH;
+        const int q = task - (int)it * NGH;
         const int32_t p = item_pair[it];
         const int i = r0 + p / P.N, j = p % P.N;
-        const int l0 = lo[p];
-        const int m = (l0 - 1 < 1 ? 1 : l0 - 1) + (int)(it - off[p]);
+        int mfirst;
+        const int m = item_level(lo, off, p, it, mfirst);
         const FieldTri& F = tri[j];
         const double* g = gh + (int64_t)i * 4 * 32;
         const Level K = make_level((double)m * P.dt, P.cP, P.cS);
@@ -514,6 +575,12 @@ k_node_eval(const FieldTri* __restrict__ tri, const double* __restrict__ gh, Asm
 #pragma unroll
         for (int e = 0; e < 6; e++) o[1 + e] = acc.m[e];
         o[7] = 0.0;
+        // the first static level of a node, whose value the next levels
+        // reuse: the exact pattern must hold there
+        if (reuse && m < hi[p] + 1 && node_static_margin(G, K) &&
+            (m == mfirst || !node_static_margin(G, make_level((double)(m - 1) * P.dt, P.cP, P.cS))) &&
+            !node_static_pattern(G, K))
+            atomicOr(bad, 1u);
     }
 }
 
@@ -526,13 +593,17 @@ __global__ void __launch_bounds__(256) k_node_reduce(const FieldTri* __restrict_
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= nitems) return;
     const int64_t it = perm[k];
-    const int j = item_pair[it] % P.N;
+    const int32_t p = item_pair[it];
+    const int j = p % P.N;
     NodeAcc acc;
     acc.s = 0.0;
     for (int e = 0; e < 6; e++) acc.m[e] = 0.0;
     for (int q = 0; q < NGH; q++) {
-        if (tkey[k * NGH + q] == kNodeSkip) continue;
-        const double* o = part + (k * NGH + q) * 8;
+        const int key = tkey[it * NGH + q];
+        if (key == kNodeSkip) continue;
+        // a reused task takes the slot of its node's first static level
+        const int64_t src = key >= kNodeReuse ? it - (key - kNodeReuse) : it;
+        const double* o = part + (src * NGH + q) * 8;
         acc.s += o[0];
 #pragma unroll
         for (int e = 0; e < 6; e++) acc.m[e] += o[1 + e];
@@ -1465,14 +1536,24 @@ __global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tr
         const int p = p0 + q / P.N, j = q % P.N;
         const int m = ms[q] + (int)(it - off[q]);
         const FieldTri& F = tri[j];
-        double d0 = xs[3 * p] - F.center[0], d1 = xs[3 * p + 1] - F.center[1], d2 = xs[3 * p + 2] - F.center[2];
-        double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-        double tau = P.delta + (double)m * P.dt;
-        int qp = front_class(P.cP * tau, d - F.maxL, 2.0 * F.maxL) + 1;
-        int qs = front_class(P.cS * tau, d - F.maxL, 2.0 * F.maxL) + 1;
-        int k = qp * kFrontClasses + qs + kNearKeys;
-        if (ka && m >= ka[q] && m <= kb[q]) k = (m == ka[q]) ? kKpKey : kKpRestKey;
-        if (sa && m >= sa[q] && m <= sb[q]) k = (m == sa[q]) ? bf_head_key(sa[q], sb[q]) : kBfRestKey;
+        int k;
+        if (ka && m >= ka[q] && m <= kb[q]) {
+            k = (m == ka[q]) ? kKpKey : kKpRestKey;
+        } else if (sa && m >= sa[q] && m <= sb[q]) {
+            k = (m == sa[q]) ? bf_head_key(sa[q], sb[q]) : kBfRestKey;
+        } else {
+            // an item of k_pp_M is one point at one level: keyed by that
+            // point's front state (node_task_key), so that a wavefront's
+            // lanes take the same region branches; points no front reaches
+            // (nothing to integrate) get key kNodeKeys
+            const double tau = pp_tau(P, m);
+            k = kNodeKeys;
+            if (tau > 0.0) {
+                const double x[3] = {xs[3 * p], xs[3 * p + 1], xs[3 * p + 2]};
+                const int nk = node_task_key(node_geo(x, F), make_level(tau, P.cP, P.cS));
+                if (nk >= 0) k = nk;
+            }
+        }
         key[it] = (uint8_t)k;
         atomicAdd(&sh[k], 1u);
     }

@@ -486,7 +486,12 @@ struct ebem_handle {
         DevBuf<double> part;
         DevBuf<unsigned int> hist;
         DevBuf<unsigned long long> cur;
+        DevBuf<unsigned int> bad;
     } nw;  // node-task scratch of ebem_assemble (grow-only)
+    bool node_reuse = true;  // static node values reused across levels (EBEM_NODE_REUSE=0: evaluated per level)
+    // node tasks of the last assembly: evaluated, reused (static), chunks
+    // evaluated again without reuse after a failed check of the reused values
+    int64_t node_stats[3] = {0, 0, 0};
     bool k_by_field = true;  // regular K part by (S, F) pairs + gather (EBEM_K_BY_FIELD=0: by (S, s) slots)
     bool ppd_by_field = true;  // direct PP double layer by (pair, F) + gather (EBEM_PPD_BY_FIELD=0: by vertex)
     // kP and between-front ranges of the last assembly / post-processing:
@@ -757,70 +762,89 @@ void launch_levels_split(ebem_handle* h, AsmParams P, int r0, int64_t nitems, in
     }
 }
 
-// FP64 per-level items perm[0..n) by outer node: keys, counting sort of the
-// node tasks, one node per lane, reduction per item (k_node_* in
-// ebem_kernels.hpp).  n * NGH < 2^31 (task ids are int32; the chunk budget
-// keeps it so).
+// FP64 per-level items perm[0..n) of a chunk of nchunk items by outer node:
+// keys, counting sort of the node tasks, one node per lane, reduction per
+// item (k_node_* in ebem_kernels.hpp).  Task ids it * NGH + q (it < nchunk)
+// must stay below 2^31 (the chunk budget keeps it so).  With static-node
+// reuse on, a failed check of the reused values (k_node_eval's *bad) makes
+// the chunk run again without reuse.
 template <int NGH>
-int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t n, const int32_t* item_pair, const int64_t* off,
-                 const int16_t* lo, const int32_t* perm, double* W) {
+int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t nchunk, int64_t n, const int32_t* item_pair,
+                 const int64_t* off, const int16_t* lo, const int16_t* hi, const uint8_t* sflag,
+                 const int32_t* perm, double* W) {
     if (n <= 0) return 0;
-    const int64_t ntask = n * NGH;
+    const int64_t ntask = nchunk * NGH;
     if (ntask > INT32_MAX) return fail(EBEM_EINVAL, "too many node tasks in one chunk");
     auto& nw = h->nw;
     const int G = h->CUs * 8;
     HIPC(nw.key.grow((size_t)ntask));
-    HIPC(nw.perm.grow((size_t)ntask));
+    HIPC(nw.perm.grow((size_t)n * NGH));
     HIPC(nw.part.grow((size_t)ntask * 8));
-    HIPC(nw.hist.grow((size_t)kNodeKeys * G));
+    HIPC(nw.hist.grow((size_t)(kNodeKeys + 1) * G));
     HIPC(nw.cur.grow((size_t)kNodeKeys * G));
+    HIPC(nw.bad.grow(1));
     const int64_t per = (n + G - 1) / G;
-    hipLaunchKernelGGL(k_node_keys<NGH>, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, r0, n, per,
-                       item_pair, off, lo, perm, nw.key.p, nw.hist.p);
-    HIPC(hipGetLastError());
-    std::vector<unsigned int> hist((size_t)kNodeKeys * G);
-    std::vector<unsigned long long> cur((size_t)kNodeKeys * G);
-    HIPC(hipMemcpyAsync(hist.data(), nw.hist.p, sizeof(unsigned int) * hist.size(), hipMemcpyDeviceToHost, h->stream));
-    HIPC(hipStreamSynchronize(h->stream));
-    unsigned long long run = 0;
-    for (size_t k = 0; k < hist.size(); k++) {  // key-major, block-minor
-        cur[k] = run;
-        run += hist[k];
-    }
-    HIPC(hipMemcpyAsync(nw.cur.p, cur.data(), sizeof(unsigned long long) * cur.size(), hipMemcpyHostToDevice,
-                        h->stream));
-    // block b of the scatter takes the tasks of the items block b keyed
-    hipLaunchKernelGGL(k_node_scatter, dim3(G), dim3(256), 0, h->stream, ntask, per * NGH, nw.key.p, nw.cur.p,
-                       nw.perm.p);
-    HIPC(hipGetLastError());
-    if (run > 0) {
-        int64_t blocks = std::min<int64_t>(((int64_t)run + 255) / 256, (int64_t)h->CUs * 16);
-        hipLaunchKernelGGL(k_node_eval<NGH>, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0, h->stream,
-                           h->d_tri.p, h->d_gh.p, P, r0, (int64_t)run, item_pair, off, lo, perm, nw.perm.p,
-                           nw.part.p);
+    for (int reuse = h->node_reuse ? 1 : 0; reuse >= 0; reuse--) {
+        hipLaunchKernelGGL(k_node_keys<NGH>, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, r0, n, per,
+                           item_pair, off, lo, hi, perm, sflag, reuse, nw.key.p, nw.hist.p);
         HIPC(hipGetLastError());
+        std::vector<unsigned int> hist((size_t)(kNodeKeys + 1) * G);
+        std::vector<unsigned long long> cur((size_t)kNodeKeys * G);
+        HIPC(hipMemcpyAsync(hist.data(), nw.hist.p, sizeof(unsigned int) * hist.size(), hipMemcpyDeviceToHost,
+                            h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        unsigned long long run = 0;
+        for (size_t k = 0; k < cur.size(); k++) {  // key-major, block-minor
+            cur[k] = run;
+            run += hist[k];
+        }
+        int64_t reused = 0;
+        for (int b = 0; b < G; b++) reused += hist[(size_t)kNodeKeys * G + b];
+        HIPC(hipMemcpyAsync(nw.cur.p, cur.data(), sizeof(unsigned long long) * cur.size(), hipMemcpyHostToDevice,
+                            h->stream));
+        // block b of the scatter takes the tasks of the items block b keyed
+        hipLaunchKernelGGL(k_node_scatter<NGH>, dim3(G), dim3(256), 0, h->stream, n, per, perm, nw.key.p, nw.cur.p,
+                           nw.perm.p);
+        HIPC(hipGetLastError());
+        HIPC(hipMemsetAsync(nw.bad.p, 0, sizeof(unsigned int), h->stream));
+        if (run > 0) {
+            int64_t blocks = std::min<int64_t>(((int64_t)run + 255) / 256, (int64_t)h->CUs * 16);
+            hipLaunchKernelGGL(k_node_eval<NGH>, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0,
+                               h->stream, h->d_tri.p, h->d_gh.p, P, r0, (int64_t)run, item_pair, off, lo, hi,
+                               nw.perm.p, reuse, nw.part.p, nw.bad.p);
+            HIPC(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_node_reduce<NGH>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                           h->d_tri.p, P, n, item_pair, perm, nw.key.p, nw.part.p, W);
+        HIPC(hipGetLastError());
+        h->node_stats[0] += (int64_t)run;
+        h->node_stats[1] += reused;
+        if (!reuse) break;
+        unsigned int bad = 0;
+        HIPC(hipMemcpyAsync(&bad, nw.bad.p, sizeof bad, hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        if (!bad) break;
+        h->node_stats[2]++;
     }
-    hipLaunchKernelGGL(k_node_reduce<NGH>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->d_tri.p, P,
-                       n, item_pair, perm, nw.key.p, nw.part.p, W);
-    HIPC(hipGetLastError());
     return 0;
 }
 
-// The per-level items perm[0..nitems) of a chunk, in key order: far-field
-// items (mixed precision, nfar) by k_levels in R_far, coplanar pairs (ncop)
-// by k_levels in FP64, the rest by outer node.  Unsorted items (perm null) or
-// EBEM_NODE_TASKS=0: all by k_levels.
+// The per-level items perm[0..nitems) of a chunk of nchunk items, in key
+// order: far-field items (mixed precision, nfar) by k_levels in R_far,
+// coplanar pairs (ncop) by k_levels in FP64, the rest by outer node.
+// Unsorted items (perm null) or EBEM_NODE_TASKS=0: all by k_levels.
 template <int NGH>
-int launch_per_level(ebem_handle* h, AsmParams P, int r0, int64_t nitems, int64_t nfar, int64_t ncop,
-                     const int32_t* item_pair, const int64_t* off, const int16_t* lo, const int32_t* perm,
-                     double* W) {
+int launch_per_level(ebem_handle* h, AsmParams P, int r0, int64_t nchunk, int64_t nitems, int64_t nfar,
+                     int64_t ncop, const int32_t* item_pair, const int64_t* off, const int16_t* lo,
+                     const int16_t* hi, const uint8_t* sflag, const int32_t* perm, double* W) {
     if (!(perm && h->node_tasks)) {
         launch_levels_split<NGH>(h, P, r0, nitems, nfar, item_pair, off, lo, perm, W);
         return 0;
     }
     if (nfar > 0) launch_levels<NGH, float>(h, P, r0, nfar, item_pair, off, lo, perm, W);
     launch_levels<NGH, double>(h, P, r0, ncop, item_pair, off, lo, perm + nfar, W);
-    return launch_nodes<NGH>(h, P, r0, nitems - nfar - ncop, item_pair, off, lo, perm + nfar + ncop, W);
+    return launch_nodes<NGH>(h, P, r0, nchunk, nitems - nfar - ncop, item_pair, off, lo, hi, sflag,
+                             perm + nfar + ncop, W);
 }
 
 int assemble(ebem_handle* h, int precision) {
@@ -862,6 +886,7 @@ int assemble(ebem_handle* h, int precision) {
     P.kp = kp ? 1 : 0;
     P.sur_project = h->sur_project ? 1 : 0;
     for (int k = 0; k < 5; k++) h->kp_stats[k] = 0;
+    for (int k = 0; k < 3; k++) h->node_stats[k] = 0;
     const double near2 = (3.0 * h->hmax) * (3.0 * h->hmax);  // surrogate pairs: centroid distance <= 3h
     Timer tall, tlev, tkp, tbf;
     double lev_ms = 0, kp_ms = 0, bf_ms = 0;
@@ -1051,7 +1076,8 @@ int assemble(ebem_handle* h, int precision) {
 #undef GHC_LAUNCH
             } else {
                 int e = 0;
-#define PL_LAUNCH(NG) launch_per_level<NG>(h, P, r0, nlev, nfar, ncop, d_ip.p, d_off.p, lo, perm, d_W.p)
+#define PL_LAUNCH(NG) \
+    launch_per_level<NG>(h, P, r0, acc, nlev, nfar, ncop, d_ip.p, d_off.p, lo, hi, sflag, perm, d_W.p)
                 switch (h->nGH) {
                     case 3: e = PL_LAUNCH(3); break;
                     case 7: e = PL_LAUNCH(7); break;
@@ -1115,6 +1141,7 @@ int assemble(ebem_handle* h, int precision) {
     tall.stop(h->stream);
     h->tim[1] = tall.ms();
     h->tim[0] = lev_ms;
+    for (int k = 0; k < 3; k++) h->tim[16 + k] = (double)h->node_stats[k];
     h->tim[8] = kp_ms;
     h->tim[10] = bf_ms;
     {
@@ -1887,6 +1914,8 @@ int ebem_create(ebem_handle** out, int device) {
     h->kp_split = !(kps && std::strcmp(kps, "0") == 0);
     const char* nts = std::getenv("EBEM_NODE_TASKS");
     h->node_tasks = !(nts && std::strcmp(nts, "0") == 0);
+    const char* nru = std::getenv("EBEM_NODE_REUSE");
+    h->node_reuse = !(nru && std::strcmp(nru, "0") == 0);
     const char* kbf = std::getenv("EBEM_K_BY_FIELD");
     h->k_by_field = !(kbf && std::strcmp(kbf, "0") == 0);
     const char* pbf = std::getenv("EBEM_PPD_BY_FIELD");

@@ -515,13 +515,16 @@ __host__ __device__ inline ChildGeo child_geo(double v2x, double v2y, double v3x
     return g;
 }
 
-// calcIntgInt_childTriangle.m:9-234 for one child, result folded into A.
-template <typename R>
-__host__ __device__ inline void child_regions(const ChildGeo& g, double zeta, double RP, double rhoP, double RS,
-                                     double rhoS, int type, const Level& Kd, ChildAcc& A) {
+// calcIntgInt_childTriangle.m:27-50: the angles TH2..TH5 that bound the
+// regions of a child (TH1 = 0, TH6 = gamma), from the in-plane front radii;
+// front cutting is always FP64.  Rotated child: V2 = (a,0), V3 = (b cg, b sg).
+struct FrontAngles {
+    double TH2, TH3, TH4, TH5;
+};
+
+__host__ __device__ __attribute__((always_inline)) inline FrontAngles front_angles(const ChildGeo& g, double rhoP,
+                                                                                   double rhoS, int type) {
     const double a = g.a;
-    // childTriangle.m:27-50 (front cutting, always FP64).  Rotated child:
-    // V2 = (a,0), V3 = (b cg, b sg).
     double vt0 = (g.b * g.cg - a) / g.c;
     double Bm = a * vt0;
     double DeltaP = Bm * Bm - (a * a - rhoP * rhoP);
@@ -533,25 +536,34 @@ __host__ __device__ inline void child_regions(const ChildGeo& g, double zeta, do
         DeltaS = 0.0;
     }
     const double gamma = g.ga, beta = g.be;
-    double TH1 = 0.0, TH2, TH3, TH4, TH5, TH6 = gamma;
+    FrontAngles f;
     if (DeltaP <= 1.0e-6) {
-        TH2 = TH3 = TH4 = TH5 = gamma;
+        f.TH2 = f.TH3 = f.TH4 = f.TH5 = gamma;
     } else if (DeltaS <= 1.0e-6) {
         double as = asin(clamp1(rP));
         double O1 = as - beta, O2 = kPi - as - beta;
         double th1 = fmin(O1, O2), th2 = fmax(O1, O2);
-        TH2 = fmin(fmax(0.0, th1), gamma);
-        TH3 = fmax(0.0, fmin(th2, gamma));
-        TH4 = TH3;
-        TH5 = TH3;
+        f.TH2 = fmin(fmax(0.0, th1), gamma);
+        f.TH3 = fmax(0.0, fmin(th2, gamma));
+        f.TH4 = f.TH3;
+        f.TH5 = f.TH3;
     } else {
         double as = asin(clamp1(rP)), bs = asin(clamp1(rS));
         double O1 = as - beta, O2 = kPi - as - beta, O3 = bs - beta, O4 = kPi - bs - beta;
-        TH2 = fmin(fmax(0.0, fmin(O1, O2)), gamma);
-        TH3 = fmin(fmax(0.0, fmin(O3, O4)), gamma);
-        TH4 = fmax(0.0, fmin(fmax(O3, O4), gamma));
-        TH5 = fmax(0.0, fmin(fmax(O1, O2), gamma));
+        f.TH2 = fmin(fmax(0.0, fmin(O1, O2)), gamma);
+        f.TH3 = fmin(fmax(0.0, fmin(O3, O4)), gamma);
+        f.TH4 = fmax(0.0, fmin(fmax(O3, O4), gamma));
+        f.TH5 = fmax(0.0, fmin(fmax(O1, O2), gamma));
     }
+    return f;
+}
+
+// calcIntgInt_childTriangle.m:9-234 for one child, result folded into A.
+template <typename R>
+__host__ __device__ inline void child_regions(const ChildGeo& g, double zeta, double RP, double rhoP, double RS,
+                                     double rhoS, int type, const Level& Kd, ChildAcc& A) {
+    const FrontAngles fa = front_angles(g, rhoP, rhoS, type);
+    const double TH1 = 0.0, TH2 = fa.TH2, TH3 = fa.TH3, TH4 = fa.TH4, TH5 = fa.TH5, TH6 = g.ga;
     const bool sp = (type == kSPSP);
     const bool zpos = zeta > 1.0e-6;
     // Closed-form region integrals in R (FP64 for the plain form, FP32 for the
@@ -760,6 +772,40 @@ __host__ __device__ __attribute__((always_inline)) inline int node_eval(const No
         add_child(A, g, v2x, v2y, zeta * sgn, w * msign(C), acc);
     }
     return type;
+}
+
+// Static outer point: class kSPSP with both circles beyond the edge of every
+// child of the fan (TH2 = TH3 = 0, TH4 = TH5 = gamma), so that child_regions
+// evaluates the single region E3 over [0, gamma] (coeffG_TR_SP.m) with the
+// level weights k5 and k6, which do not depend on t: node_eval then returns
+// the same bits at every level where this holds.  Children node_eval skips
+// (degenerate) are skipped here too.
+__host__ __device__ inline bool node_static_pattern(const NodeGeo& G, const Level& K) {
+    double rhoP, rhoS;
+    if (classify(G.zeta, K.rP, K.rS, G.dMIN, rhoP, rhoS) != kSPSP) return false;
+#pragma unroll 1
+    for (int k = 0; k < 3; k++) {
+        double C, v2x, v2y, v3x, v3y;
+        if (!fan_child(G, k, C, v2x, v2y, v3x, v3y)) continue;
+        const ChildGeo g = child_geo(v2x, v2y, v3x, v3y);
+        const FrontAngles f = front_angles(g, rhoP, rhoS, kSPSP);
+        if (!(f.TH2 == 0.0 && f.TH3 == 0.0 && f.TH4 == g.ga && f.TH5 == g.ga && g.ga > 0.0)) return false;
+    }
+    return true;
+}
+
+// Sufficient condition for node_static_pattern with a margin, from the
+// projection's distances alone: class kSPSP and rho_S^2 above the largest
+// squared vertex distance by 0.2% plus 1e-5.  Every child's vertices then lie
+// strictly inside the S circle (and the P circle, which is larger), so the
+// circle meets the child's edge line outside the edge on both sides and both
+// discriminants of the front cutting exceed 1e-6.  It only grows with t.
+__host__ __device__ inline bool node_static_margin(const NodeGeo& G, const Level& K) {
+    double rhoP, rhoS;
+    if (classify(G.zeta, K.rP, K.rS, G.dMIN, rhoP, rhoS) != kSPSP) return false;
+    double dmax2 = 0.0;
+    for (int k = 0; k < 3; k++) dmax2 = fmax(dmax2, G.t[k][0] * G.t[k][0] + G.t[k][1] * G.t[k][1]);
+    return rhoS * rhoS > dmax2 * 1.002 + 1.0e-5;
 }
 
 // The same sum over the fan for an outer point that stays, over the levels

@@ -336,8 +336,12 @@ int ebem_set_option(ebem_handle *h, int option, double value);
  * kernel, out[10] assembly between-front kernel, out[11] post-processing
  * between-front kernel, out[12] ebem_assemble_k total, out[13] beta_K kernel
  * of the last ebem_beta_k / ebem_rhs_dirichlet, out[14] ebem_pp_eval_direct
- * total, out[15] ebem_mot_solve_neumann total. */
-#define EBEM_NUM_TIMINGS 16
+ * total, out[15] ebem_mot_solve_neumann total; counts of the last
+ * ebem_assemble: out[16] outer-node tasks evaluated, out[17] tasks whose
+ * static value was reused from an earlier level, out[18] row chunks
+ * evaluated again without reuse after a failed check of the reused values
+ * (0 expected). */
+#define EBEM_NUM_TIMINGS 19
 int ebem_timings(ebem_handle *h, double *out, int n);
 
 /* ------------------------------------------------------------------------

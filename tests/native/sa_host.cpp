@@ -109,3 +109,26 @@ int sa_item_kp(const double* rec, const double* gh, int nGH, double ta, double t
     return 1;
 }
 }
+
+extern "C" {
+// One outer point x at level t as k_node_eval evaluates it: the node sum
+// before the rotation (s, m00, m01, m02, m11, m12, m22) into acc7, weight w.
+// flags[0] = node_static_margin, flags[1] = node_static_pattern at t.
+int sa_node_static(const double* rec, const double* x, double t, double cP, double cS, double w, double* acc7,
+                   int* flags) {
+    FieldTri F;
+    std::memcpy(&F, rec, sizeof(FieldTri));
+    const Level K = make_level(t, cP, cS);
+    double xx[3] = {x[0], x[1], x[2]};
+    const NodeGeo G = node_geo(xx, F);
+    NodeAcc acc;
+    acc.s = 0.0;
+    for (int q = 0; q < 6; q++) acc.m[q] = 0.0;
+    const int type = node_eval<double>(G, K, w, acc);
+    acc7[0] = acc.s;
+    for (int q = 0; q < 6; q++) acc7[1 + q] = acc.m[q];
+    flags[0] = node_static_margin(G, K) ? 1 : 0;
+    flags[1] = node_static_pattern(G, K) ? 1 : 0;
+    return type;
+}
+}

@@ -100,13 +100,13 @@ def test_stats_and_timings_need_a_handle():
     import numpy as np
     from elastodynamicebem_amd import _lib
     lib = _lib.load()
-    out = np.zeros(16)
+    out = np.zeros(19)
     st = np.zeros(5, np.int64)
-    assert lib.ebem_timings(None, out.ctypes.data_as(_lib._dp), 16) < 0
+    assert lib.ebem_timings(None, out.ctypes.data_as(_lib._dp), 19) < 0
     assert lib.ebem_assembly_kp_stats(None, st.ctypes.data_as(_lib._i64p)) < 0
     assert lib.ebem_pp_kp_stats(None, st.ctypes.data_as(_lib._i64p)) < 0
     src = open(os.path.join(ROOT, "include", "ebem.h")).read()
-    assert "#define EBEM_NUM_TIMINGS 16" in src
+    assert "#define EBEM_NUM_TIMINGS 19" in src
     from elastodynamicebem_amd import api
     assert len(api.Solver._KP_KEYS) == 5
     # the double-layer entry points reject a null handle too

@@ -17,12 +17,13 @@ from conftest import ROOT
 pytestmark = pytest.mark.gpu
 
 
-def _assemble(split, node_tasks=True):
+def _assemble(split, node_tasks=True, reuse=True):
     from elastodynamicebem_amd import api
     from elastodynamicebem_amd.mesh import read_mesh
     m = read_mesh(os.path.join(ROOT, "tests", "golden", "meshes", "sphereUniform_3.mesh"))
     pb = api.PbParam(rho=1.0, mu=1.0, lambda_=1.0, Tfin=4.0, nT=128)
-    env = {"EBEM_KP_SPLIT": "1" if split else "0", "EBEM_NODE_TASKS": "1" if node_tasks else "0"}
+    env = {"EBEM_KP_SPLIT": "1" if split else "0", "EBEM_NODE_TASKS": "1" if node_tasks else "0",
+           "EBEM_NODE_REUSE": "1" if reuse else "0"}
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
@@ -72,6 +73,31 @@ def test_node_tasks_match_item_per_lane():
         ed = np.linalg.norm(da - db) / np.linalg.norm(db)
         print(f"density difference {ed:.2e}")
         assert ed < 1e-12
+    finally:
+        a.close()
+        b.close()
+
+
+def test_static_node_reuse_is_bit_identical():
+    """Static outer nodes (both fronts beyond every child's edge) take the
+    value of their first static level at the later levels of a pair
+    (default) instead of being evaluated again (EBEM_NODE_REUSE=0): the two
+    assemblies of the 1280-triangle sphere are equal bit for bit, a large
+    share of the node tasks is reused, and no chunk failed the device-side
+    check of the reused values."""
+    a = _assemble(True, reuse=True)
+    b = _assemble(True, reuse=False)
+    try:
+        assert a.store_info() == b.store_info()
+        ta, tb = a.timings(), b.timings()
+        print(f"per-level integrals {ta['levels_ms']:.1f} ms with reuse ({ta['node_tasks']:.0f} tasks evaluated, "
+              f"{ta['node_tasks_reused']:.0f} reused) vs {tb['levels_ms']:.1f} ms ({tb['node_tasks']:.0f} tasks)")
+        assert ta["node_reuse_fallbacks"] == 0
+        assert ta["node_tasks_reused"] > 0.1 * tb["node_tasks"]
+        assert ta["node_tasks"] + ta["node_tasks_reused"] == tb["node_tasks"]
+        for lag in range(a.num_blocks()):
+            G, R = a.get_block(lag), b.get_block(lag)
+            assert np.array_equal(G, R), lag
     finally:
         a.close()
         b.close()
