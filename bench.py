@@ -11,8 +11,9 @@ factorisation, the 128-step marching-on-in-time solve and the post-processed
 displacement at 4096 points of the shell 1.2 <= |x| <= 2 (seeded) at every step.
 value = culled pair.lag integrals / step time.
 
-Also reported: a roofline object for the dominant kernel (the assembly
-integral kernel k_levels) and the history-sum kernel, and the CPU oracle
+Also reported: a roofline object for the dominant kernel (the assembly's
+per-level integral stage, whose k_node_eval evaluates one outer node per lane)
+and the history-sum kernel, and the CPU oracle
 (FP64 C/OpenMP restatement of the reference SA path) timed on the host on a
 bounded sample of the same assembly.
 """
@@ -29,20 +30,25 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic FP64 operations per (pair, level) item of k_levels on this
-# workload (7 outer nodes): mean over the items outside the kP and
+# Algorithmic FP64 operations per (pair, level) item of the per-level stage on
+# this workload (7 outer nodes): mean over the items outside the kP and
 # between-front ranges (which k_levels_kp and k_levels_bf evaluate) in a
 # uniform sample of 200000 of the 43938820 items of the pruned lag windows,
-# counted by the op-counting host build of the device math
-# (python tests/opcount.py 200000 -> 8103.8 +- 14.4; kP share 0.2585,
-# between-front share 0.2634); each + - * / (a fused multiply-add as 2), sqrt
-# and libm call counts once.
-ALG_FLOP_PER_ITEM = 8103.8
-# HBM bytes per launch of k_levels on this workload from the rocprofv3 --pmc
-# passes FETCH_SIZE (x2.0: the counter reads 512 MiB for a 1 GiB device copy)
-# and WRITE_SIZE (x1.0): 136.7 GB + 86.0 GB, almost all register-spill scratch.
-K_LEVELS_TRAFFIC_BYTES = 222.8e9
-K_LEVELS_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this bench's k_levels "
+# counted by the op-counting host build of the device math, with the static
+# outer nodes whose value an earlier level of the pair holds left out as the
+# node tasks leave them out (python tests/opcount.py 200000 -> 6866.8 +- 16.4;
+# 8030.1 +- 14.2 with every node evaluated; kP share 0.2585, between-front
+# share 0.2634); each + - * / (a fused multiply-add as 2), sqrt and libm call
+# counts once.
+ALG_FLOP_PER_ITEM = 6866.8
+# HBM bytes per call of the per-level integral stage on this workload (the
+# kernels k_node_keys, k_node_scatter, k_node_eval, k_node_reduce and k_levels
+# for the coplanar pairs) from the rocprofv3 --pmc passes FETCH_SIZE (x2.0:
+# the counter reads 512 MiB for a 1 GiB device copy) and WRITE_SIZE (x1.0):
+# 109.8 GB + 77.0 GB, of which k_node_eval 93.4 GB + 75.0 GB, almost all
+# register-spill scratch (the node slots are 5.5 GB written, 7 GB read).
+K_LEVELS_TRAFFIC_BYTES = 186.8e9
+K_LEVELS_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this bench's per-level stage "
                            "(profiles/pmc_counters_sphere3_fp64.md), not measured in this run")
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
@@ -478,7 +484,9 @@ def main():
     roof = None
     if ALG_FLOP_PER_ITEM:
         ach = ALG_FLOP_PER_ITEM * level_items / (lev_ms * 1e-3) / 1e12
-        roof = {"bound": "valu", "pipe": "fp64-valu", "kernel": "k_levels", "achieved": ach,
+        roof = {"bound": "valu", "pipe": "fp64-valu",
+                "kernel": "per-level integral stage (k_node_keys, k_node_scatter, k_node_eval, k_node_reduce; "
+                          "k_levels for coplanar pairs)", "achieved": ach,
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
                 "traffic": K_LEVELS_TRAFFIC_BYTES, "traffic_source": K_LEVELS_TRAFFIC_SOURCE}
     hist_gbs = hb_streamed / (hist_ms * 1e-3) / 1e9

@@ -81,6 +81,40 @@ long long opcount_item(const double* rec, const double* gh, int nGH, double t, d
     return g_ops - setup + setup;
 }
 
+// The same item as the node-task path evaluates it: an outer node whose
+// static value the previous level of the pair already holds (tprev >= 0 and
+// node_static_margin at t and tprev) is not evaluated and costs nothing
+// (the predicate is bookkeeping, not counted).
+long long opcount_item_reuse(const double* rec, const double* gh, int nGH, double t, double tprev, double cP,
+                             double cS) {
+    FieldTri F;
+    std::memcpy((void*)&F, rec, sizeof(FieldTri));
+    g_ops = 0;
+    Level K = make_level(cdouble(t), cdouble(cP), cdouble(cS));
+    NodeAcc acc;
+    acc.s = 0.0;
+    for (int q = 0; q < 6; q++) acc.m[q] = 0.0;
+    for (int q = 0; q < nGH; q++) {
+        const long long before = g_ops;
+        cdouble x[3] = {gh[q], gh[32 + q], gh[64 + q]};
+        NodeGeo G = node_geo(x, F);
+        if (tprev >= 0.0) {
+            const long long keep = g_ops;
+            const bool reused = node_static_margin(G, K) &&
+                                node_static_margin(G, make_level(cdouble(tprev), cdouble(cP), cdouble(cS)));
+            g_ops = keep;
+            if (reused) {
+                g_ops = before;
+                continue;
+            }
+        }
+        node_eval(G, K, cdouble(gh[96 + q]), acc);
+    }
+    cdouble W[6];
+    finish_rotation(acc, F.C, W);
+    return g_ops;
+}
+
 // Flops of one kP range [ta, tb] of k_levels_kp: node_eval_kp over the nGH
 // nodes and the two rotations (the writes W_m = WP + t_m^2 WQ add 13 flops per
 // level, counted by the caller).  -1 if a node fails the state check.

@@ -1,5 +1,7 @@
 """Algorithmic FP64 operation count per (pair, level) item of the assembly
-kernel k_levels (items outside the kP and between-front ranges), per kP range
+(items outside the kP and between-front ranges; also with the static outer
+nodes whose value an earlier level holds left out, as the node-task path
+evaluates them), per kP range
 of k_levels_kp and per between-front range of k_levels_bf, from the op-counting host build of the device math
 (tests/native/opcount.cpp), on a uniform random sample of the items of a
 workload.  Run:  python tests/opcount.py [nsample]   (sphereUniform_3, T=4, nT=128).
@@ -38,6 +40,8 @@ def count(mesh, Tfin, nT, L, nsample, seed=0, nGH=7):
     lib = C.CDLL(os.path.join(HERE, "native", "libopcount.so"))
     lib.opcount_item.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double]
     lib.opcount_item.restype = C.c_longlong
+    lib.opcount_item_reuse.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+    lib.opcount_item_reuse.restype = C.c_longlong
     lib.opcount_kp.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
     lib.opcount_kp.restype = C.c_longlong
     lib.opcount_bf.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
@@ -57,7 +61,7 @@ def count(mesh, Tfin, nT, L, nsample, seed=0, nGH=7):
     pair = np.searchsorted(off, picks, side="right") - 1
     N = mesh.numberTriangles
     recs = {}
-    lev_ops, kp_ops, kp_w, bf_ops, bf_w = [], [], [], [], []
+    lev_ops, lev_reuse, kp_ops, kp_w, bf_ops, bf_w = [], [], [], [], [], []
     for it, p in zip(picks, pair):
         i, j = divmod(int(p), N)
         m = int(mlo.ravel()[p] + (it - off[p]))
@@ -89,11 +93,16 @@ def count(mesh, Tfin, nT, L, nsample, seed=0, nGH=7):
         else:
             lev_ops.append(lib.opcount_item(recs[j].ctypes.data_as(_dp), gh.ctypes.data_as(_dp), nGH, m * o.dt,
                                             o.velP, o.velS))
+            tprev = (m - 1) * o.dt if m > mlo.ravel()[p] else -1.0
+            lev_reuse.append(lib.opcount_item_reuse(recs[j].ctypes.data_as(_dp), gh.ctypes.data_as(_dp), nGH,
+                                                    m * o.dt, tprev, o.velP, o.velS))
     lev = np.array(lev_ops, float)
+    levr = np.array(lev_reuse, float)
     kpo, kpw = np.array(kp_ops, float), np.array(kp_w)
     bfo, bfw = np.array(bf_ops, float), np.array(bf_w)
     return dict(items=total, kp_share=len(kpo) / nsample, bf_share=len(bfo) / nsample, level_flops=lev.mean(),
-                level_flops_se=lev.std() / np.sqrt(len(lev)),
+                level_flops_se=lev.std() / np.sqrt(len(lev)), level_flops_reuse=levr.mean(),
+                level_flops_reuse_se=levr.std() / np.sqrt(len(levr)),
                 kp_range_flops=float((kpo * kpw).sum() / kpw.sum()) if len(kpo) else 0.0,
                 bf_level_flops=float(bfo.sum() / (bfw > 0).sum() / (1.0 / bfw).mean()) if len(bfo) else 0.0,
                 bf_range_flops=float((bfo * bfw).sum() / bfw.sum()) if len(bfo) else 0.0)
@@ -105,5 +114,6 @@ if __name__ == "__main__":
     m = read_mesh(os.path.join(HERE, "golden", "meshes", "sphereUniform_3.mesh"))
     r = count(m, 4.0, 128, 66, ns)
     print(f"items {r['items']}  kP share {r['kp_share']:.4f}  between-front share {r['bf_share']:.4f}  "
-          f"per-level flops/item {r['level_flops']:.1f} +- {r['level_flops_se']:.1f}  flops per kP range "
+          f"per-level flops/item {r['level_flops']:.1f} +- {r['level_flops_se']:.1f} (static nodes reused: "
+          f"{r['level_flops_reuse']:.1f} +- {r['level_flops_reuse_se']:.1f})  flops per kP range "
           f"{r['kp_range_flops']:.0f}, per between-front range {r['bf_range_flops']:.0f} (n={ns})")
