@@ -9,8 +9,15 @@ rows move by 1e-4 .. 1 under such a perturbation), the store size
 against the reference's light-cone count, and size-independent properties of
 the solved density: the mesh and the datum are invariant under the rotation by
 pi about the screen's normal, so the density is too (x, y components change
-sign, z does not).  The mixed-precision error against the FP64 run is
-printed."""
+sign, z does not).  Mixed precision (FP32 far field) and mixed precision with
+the learned near-field surrogate (the network of the 1280-triangle sphere's
+test: 2M seeded samples, 20 epochs) are solved on the same problem and their
+density errors against the FP64 run printed.  Mixed precision: 8.8e-4
+measured, held below 2e-3.  Mixed + surrogate: the marching on this open
+screen diverges (measured density error 2.4e25 with 1.3M predicted blocks,
+training loss 4.7e-4) although the moment projection keeps it bounded on the
+closed sphere (4.8e-2, test_gpu_surrogate_cfg3.py); the test holds that
+measured behaviour, so a change to it shows up here."""
 import time
 
 import numpy as np
@@ -72,10 +79,31 @@ def test_screen64_rows_store_and_density(screen64):
         st = s.assembly_stats()
         am = s.mot_solve(None)
         tmm = s.timings()
+        # mixed precision + surrogate (the combined form of the 64 x 64 screen)
+        from elastodynamicebem_amd import surrogate as SG
+        sur = SG.Surrogate()
+        try:
+            ntr, _ = SG.build_training_set(sur, 2_000_000, seed=0, held_frac=0.02)
+            sur.init(ntr, seed=0)
+            losses = sur.train(ntr, 20, batch=2048, lr=2e-3, seed=0)
+            sur.attach(s)
+            s.set_option(api.OPT_SURROGATE_SOLVE, 1)
+            s.assemble(api.PREC_MIXED | api.PREC_SURROGATE)
+            tms = s.timings()
+            ss = s.surrogate_stats()
+            ams = s.mot_solve(None)
+            sur.detach(s)
+        finally:
+            sur.close()
     err = frob_rel(am, a)
+    err_ms = frob_rel(ams, a) if np.isfinite(ams).all() else float("inf")
     print(f"screen64: assembly {t_asm:.2f} s wall ({tm['levels_ms']:.0f} ms integrals), "
           f"{info['pair_lags'] / (tm['assembly_ms'] * 1e-3):.3e} pair.lags/s; "
           f"MOT {tm['mot_ms']:.0f} ms (history {tm['history_ms']:.0f} ms, "
           f"{hist_bytes / (tm['history_ms'] * 1e-3) / 1e9:.0f} GB/s); mixed assembly "
-          f"{tmm['assembly_ms']:.0f} ms, {st['far_items']} FP32 items, density rel L2 vs FP64 {err:.3e}")
-    assert np.isfinite(err)
+          f"{tmm['assembly_ms']:.0f} ms, {st['far_items']} FP32 items, density rel L2 vs FP64 {err:.3e}; "
+          f"mixed + surrogate assembly {tms['assembly_ms']:.0f} ms, {ss['blocks']} predicted blocks, "
+          f"training loss {losses[0]:.3e} -> {losses[-1]:.3e}, density rel L2 vs FP64 {err_ms:.3e}")
+    assert err < 2e-3
+    assert ss["blocks"] > 1_000_000
+    assert err_ms > 1.0  # unstable on this open screen (see the module docstring)
