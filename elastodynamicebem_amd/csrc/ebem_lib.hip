@@ -1394,8 +1394,12 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     for (int k = 0; k < nt; k++)
         if (nhat[k] >= 0) order.push_back(k);
     std::sort(order.begin(), order.end(), [&](int a, int b) { return delta[a] < delta[b]; });
-    const int P0 = std::min(npts, 1024);  // points per chunk
-    const int G = h->CUs * 8;             // counting-sort blocks
+    // points per chunk: as many as keep every chunk's (point, triangle, m)
+    // items below 2^31 even if every window held all nT + 1 levels (at least
+    // 1024); fewer chunks, fewer launches and host round trips
+    const int64_t pmax = (int64_t)INT32_MAX / ((int64_t)N * (nT + 1));
+    const int P0 = (int)std::min<int64_t>(npts, std::max<int64_t>(1024, pmax));
+    const int G = h->CUs * 8;  // counting-sort blocks
     // kP ranges of m through k_pp_M_kp (analytic form with key-sorted items)
     const bool kp = !numeric && h->sort_items && h->kp_split;
     auto &d_ms = h->pw.ms, &d_me = h->pw.me, &d_iq = h->pw.iq, &d_nh = h->pw.nh, &d_kidx = h->pw.kidx;
