@@ -478,43 +478,46 @@ __device__ inline int node_first_static(const NodeGeo& G, int mfirst, int mlast,
     return m;
 }
 
-// perm[0..nitems): the chunk's per-level items evaluated by node; block b
-// keys the tasks of the items perm[b per .. (b + 1) per) and counts them per
-// key in bh[key * gridDim + b] (key kNodeKeys: the reused tasks).
+// An item k_item_keys put among the per-level items evaluated by node.
+__device__ inline bool node_path_item(uint8_t ikey) { return ikey >= kNearKeys && ikey < kNearKeys + kFrontKeys; }
+
+// (pair, node) tasks of the chunk's npairs pairs: block b takes pairs
+// [b per, (b + 1) per) with all their nodes; a thread computes its node's
+// projection and first static level once and keys the node at each of the
+// pair's levels that are evaluated by node (ikey), counting the tasks per key
+// in bh[key * gridDim + b] (key kNodeKeys: the reused tasks).
 template <int NGH>
 __global__ void __launch_bounds__(256) k_node_keys(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
-                                                   AsmParams P, int r0, int64_t nitems, int64_t per,
-                                                   const int32_t* __restrict__ item_pair,
+                                                   AsmParams P, int r0, int64_t npairs, int64_t per,
                                                    const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
-                                                   const int16_t* __restrict__ hi, const int32_t* __restrict__ perm,
+                                                   const int16_t* __restrict__ hi, const uint8_t* __restrict__ ikey,
                                                    const uint8_t* __restrict__ sflag, int reuse,
-                                                   uint8_t* __restrict__ tkey,
-                                                   unsigned int* __restrict__ bh) {
+                                                   uint8_t* __restrict__ tkey, unsigned int* __restrict__ bh) {
     __shared__ unsigned int sh[kNodeKeys + 1];  // + reused tasks
     for (int k = threadIdx.x; k <= kNodeKeys; k += blockDim.x) sh[k] = 0;
     __syncthreads();
     const int64_t a = blockIdx.x * per;
-    const int64_t e = a + per < nitems ? a + per : nitems;
-    for (int64_t k = a + threadIdx.x; k < e; k += blockDim.x) {
-        const int64_t it = perm[k];
-        const int32_t p = item_pair[it];
-        const int i = r0 + p / P.N, j = p % P.N;
-        int mfirst;
-        const int m = item_level(lo, off, p, it, mfirst);
-        const int mlast = hi[p] + 1;
-        const bool can_reuse = reuse && m > mfirst && !(sflag && sflag[p]);
-        const FieldTri& F = tri[j];
+    const int64_t e = a + per < npairs ? a + per : npairs;
+    for (int64_t k = a * NGH + threadIdx.x; k < e * NGH; k += blockDim.x) {
+        const int64_t p = k / NGH;
+        const int q = (int)(k - p * NGH);
+        const int l0 = lo[p], l1 = hi[p];
+        if (l0 > l1) continue;
+        const int mfirst = l0 - 1 < 1 ? 1 : l0 - 1, mlast = l1 + 1;
+        const int64_t it0 = off[p];
+        const int i = r0 + (int)(p / P.N), j = (int)(p % P.N);
         const double* g = gh + (int64_t)i * 4 * 32;
-        const Level K = make_level((double)m * P.dt, P.cP, P.cS);
-        for (int q = 0; q < NGH; q++) {
-            const double x[3] = {g[q], g[32 + q], g[64 + q]};
-            const NodeGeo G = node_geo(x, F);
-            const int key = node_task_key(G, K);
+        const double x[3] = {g[q], g[32 + q], g[64 + q]};
+        const NodeGeo G = node_geo(x, tri[j]);
+        const bool can_reuse = reuse && !(sflag && sflag[p]);
+        const int ms = can_reuse ? node_first_static(G, mfirst, mlast, P) : mlast + 1;
+        for (int m = mfirst; m <= mlast; m++) {
+            const int64_t it = it0 + (m - mfirst);
+            if (!node_path_item(ikey[it])) continue;
+            const int key = node_task_key(G, make_level((double)m * P.dt, P.cP, P.cS));
             int tk = key < 0 ? kNodeSkip : key;
-            if (key >= 0 && can_reuse) {
-                const int d = m - node_first_static(G, mfirst, mlast, P);
-                if (d >= 1 && d <= kNodeReuseMax) tk = kNodeReuse + d;
-            }
+            const int d = m - ms;
+            if (key >= 0 && m > mfirst && d >= 1 && d <= kNodeReuseMax) tk = kNodeReuse + d;
             tkey[it * NGH + q] = (uint8_t)tk;
             if (tk < kNodeKeys) atomicAdd(&sh[tk], 1u);
             else if (tk != kNodeSkip) atomicAdd(&sh[kNodeKeys], 1u);
@@ -526,7 +529,10 @@ __global__ void __launch_bounds__(256) k_node_keys(const FieldTri* __restrict__ 
 
 // Block b places the tasks k_node_keys' block b keyed (same partition).
 template <int NGH>
-__global__ void __launch_bounds__(256) k_node_scatter(int64_t nitems, int64_t per, const int32_t* __restrict__ perm,
+__global__ void __launch_bounds__(256) k_node_scatter(int64_t npairs, int64_t per, const int64_t* __restrict__ off,
+                                                      const int16_t* __restrict__ lo,
+                                                      const int16_t* __restrict__ hi,
+                                                      const uint8_t* __restrict__ ikey,
                                                       const uint8_t* __restrict__ tkey,
                                                       const unsigned long long* __restrict__ bo,
                                                       int32_t* __restrict__ tperm) {
@@ -534,10 +540,17 @@ __global__ void __launch_bounds__(256) k_node_scatter(int64_t nitems, int64_t pe
     for (int k = threadIdx.x; k < kNodeKeys; k += blockDim.x) cur[k] = bo[(int64_t)k * gridDim.x + blockIdx.x];
     __syncthreads();
     const int64_t a = blockIdx.x * per;
-    const int64_t e = a + per < nitems ? a + per : nitems;
-    for (int64_t k = a + threadIdx.x; k < e; k += blockDim.x) {
-        const int64_t it = perm[k];
-        for (int q = 0; q < NGH; q++) {
+    const int64_t e = a + per < npairs ? a + per : npairs;
+    for (int64_t k = a * NGH + threadIdx.x; k < e * NGH; k += blockDim.x) {
+        const int64_t p = k / NGH;
+        const int q = (int)(k - p * NGH);
+        const int l0 = lo[p], l1 = hi[p];
+        if (l0 > l1) continue;
+        const int mfirst = l0 - 1 < 1 ? 1 : l0 - 1, mlast = l1 + 1;
+        const int64_t it0 = off[p];
+        for (int m = mfirst; m <= mlast; m++) {
+            const int64_t it = it0 + (m - mfirst);
+            if (!node_path_item(ikey[it])) continue;
             const uint8_t key = tkey[it * NGH + q];
             if (key >= kNodeKeys) continue;
             const unsigned long long pos = atomicAdd(&cur[key], 1ull);

@@ -769,9 +769,9 @@ void launch_levels_split(ebem_handle* h, AsmParams P, int r0, int64_t nitems, in
 // reuse on, a failed check of the reused values (k_node_eval's *bad) makes
 // the chunk run again without reuse.
 template <int NGH>
-int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t nchunk, int64_t n, const int32_t* item_pair,
-                 const int64_t* off, const int16_t* lo, const int16_t* hi, const uint8_t* sflag,
-                 const int32_t* perm, double* W) {
+int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t npairs, int64_t nchunk, int64_t n,
+                 const int32_t* item_pair, const int64_t* off, const int16_t* lo, const int16_t* hi,
+                 const uint8_t* ikey, const uint8_t* sflag, const int32_t* perm, double* W) {
     if (n <= 0) return 0;
     const int64_t ntask = nchunk * NGH;
     if (ntask > INT32_MAX) return fail(EBEM_EINVAL, "too many node tasks in one chunk");
@@ -783,10 +783,10 @@ int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t nchunk, int64_t n,
     HIPC(nw.hist.grow((size_t)(kNodeKeys + 1) * G));
     HIPC(nw.cur.grow((size_t)kNodeKeys * G));
     HIPC(nw.bad.grow(1));
-    const int64_t per = (n + G - 1) / G;
+    const int64_t per = (npairs + G - 1) / G;  // pairs per sort block
     for (int reuse = h->node_reuse ? 1 : 0; reuse >= 0; reuse--) {
-        hipLaunchKernelGGL(k_node_keys<NGH>, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, r0, n, per,
-                           item_pair, off, lo, hi, perm, sflag, reuse, nw.key.p, nw.hist.p);
+        hipLaunchKernelGGL(k_node_keys<NGH>, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, r0, npairs,
+                           per, off, lo, hi, ikey, sflag, reuse, nw.key.p, nw.hist.p);
         HIPC(hipGetLastError());
         std::vector<unsigned int> hist((size_t)(kNodeKeys + 1) * G);
         std::vector<unsigned long long> cur((size_t)kNodeKeys * G);
@@ -803,8 +803,8 @@ int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t nchunk, int64_t n,
         HIPC(hipMemcpyAsync(nw.cur.p, cur.data(), sizeof(unsigned long long) * cur.size(), hipMemcpyHostToDevice,
                             h->stream));
         // block b of the scatter takes the tasks of the items block b keyed
-        hipLaunchKernelGGL(k_node_scatter<NGH>, dim3(G), dim3(256), 0, h->stream, n, per, perm, nw.key.p, nw.cur.p,
-                           nw.perm.p);
+        hipLaunchKernelGGL(k_node_scatter<NGH>, dim3(G), dim3(256), 0, h->stream, npairs, per, off, lo, hi, ikey,
+                           nw.key.p, nw.cur.p, nw.perm.p);
         HIPC(hipGetLastError());
         HIPC(hipMemsetAsync(nw.bad.p, 0, sizeof(unsigned int), h->stream));
         if (run > 0) {
@@ -834,16 +834,16 @@ int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t nchunk, int64_t n,
 // coplanar pairs (ncop) by k_levels in FP64, the rest by outer node.
 // Unsorted items (perm null) or EBEM_NODE_TASKS=0: all by k_levels.
 template <int NGH>
-int launch_per_level(ebem_handle* h, AsmParams P, int r0, int64_t nchunk, int64_t nitems, int64_t nfar,
-                     int64_t ncop, const int32_t* item_pair, const int64_t* off, const int16_t* lo,
-                     const int16_t* hi, const uint8_t* sflag, const int32_t* perm, double* W) {
+int launch_per_level(ebem_handle* h, AsmParams P, int r0, int64_t npairs, int64_t nchunk, int64_t nitems,
+                     int64_t nfar, int64_t ncop, const int32_t* item_pair, const int64_t* off, const int16_t* lo,
+                     const int16_t* hi, const uint8_t* ikey, const uint8_t* sflag, const int32_t* perm, double* W) {
     if (!(perm && h->node_tasks)) {
         launch_levels_split<NGH>(h, P, r0, nitems, nfar, item_pair, off, lo, perm, W);
         return 0;
     }
     if (nfar > 0) launch_levels<NGH, float>(h, P, r0, nfar, item_pair, off, lo, perm, W);
     launch_levels<NGH, double>(h, P, r0, ncop, item_pair, off, lo, perm + nfar, W);
-    return launch_nodes<NGH>(h, P, r0, nchunk, nitems - nfar - ncop, item_pair, off, lo, hi, sflag,
+    return launch_nodes<NGH>(h, P, r0, npairs, nchunk, nitems - nfar - ncop, item_pair, off, lo, hi, ikey, sflag,
                              perm + nfar + ncop, W);
 }
 
@@ -1076,8 +1076,9 @@ int assemble(ebem_handle* h, int precision) {
 #undef GHC_LAUNCH
             } else {
                 int e = 0;
-#define PL_LAUNCH(NG) \
-    launch_per_level<NG>(h, P, r0, acc, nlev, nfar, ncop, d_ip.p, d_off.p, lo, hi, sflag, perm, d_W.p)
+#define PL_LAUNCH(NG)                                                                                     \
+    launch_per_level<NG>(h, P, r0, (int64_t)rows * N, acc, nlev, nfar, ncop, d_ip.p, d_off.p, lo, hi, d_key.p, \
+                         sflag, perm, d_W.p)
                 switch (h->nGH) {
                     case 3: e = PL_LAUNCH(3); break;
                     case 7: e = PL_LAUNCH(7); break;
