@@ -45,9 +45,9 @@ ALG_FLOP_PER_ITEM = 6866.8
 # kernels k_node_keys, k_node_scatter, k_node_eval, k_node_reduce and k_levels
 # for the coplanar pairs) from the rocprofv3 --pmc passes FETCH_SIZE (x2.0:
 # the counter reads 512 MiB for a 1 GiB device copy) and WRITE_SIZE (x1.0):
-# 109.8 GB + 77.0 GB, of which k_node_eval 93.4 GB + 75.0 GB, almost all
+# 101.5 GB + 76.4 GB, of which k_node_eval 86.8 GB + 74.7 GB, almost all
 # register-spill scratch (the node slots are 5.5 GB written, 7 GB read).
-K_LEVELS_TRAFFIC_BYTES = 186.8e9
+K_LEVELS_TRAFFIC_BYTES = 177.9e9
 K_LEVELS_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this bench's per-level stage "
                            "(profiles/pmc_counters_sphere3_fp64.md), not measured in this run")
 PEAK_FP64_TFLOPS = 78.6
