@@ -478,8 +478,9 @@ __device__ inline int node_first_static(const NodeGeo& G, int mfirst, int mlast,
     return m;
 }
 
-// An item k_item_keys put among the per-level items evaluated by node.
-__device__ inline bool node_path_item(uint8_t ikey) { return ikey >= kNearKeys && ikey < kNearKeys + kFrontKeys; }
+// An item k_item_keys put among the per-level items evaluated by node: item
+// keys [klo, khi) (the FP64 near field, or the far field of the mixed form).
+__device__ inline bool node_path_item(uint8_t ikey, int klo, int khi) { return ikey >= klo && ikey < khi; }
 
 // (pair, node) tasks of the chunk's npairs pairs: block b takes pairs
 // [b per, (b + 1) per) with all their nodes; a thread computes its node's
@@ -491,7 +492,7 @@ __global__ void __launch_bounds__(256) k_node_keys(const FieldTri* __restrict__ 
                                                    AsmParams P, int r0, int64_t npairs, int64_t per,
                                                    const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
                                                    const int16_t* __restrict__ hi, const uint8_t* __restrict__ ikey,
-                                                   const uint8_t* __restrict__ sflag, int reuse,
+                                                   int klo, int khi, const uint8_t* __restrict__ sflag, int reuse,
                                                    uint8_t* __restrict__ tkey, unsigned int* __restrict__ bh) {
     __shared__ unsigned int sh[kNodeKeys + 1];  // + reused tasks
     for (int k = threadIdx.x; k <= kNodeKeys; k += blockDim.x) sh[k] = 0;
@@ -513,7 +514,7 @@ __global__ void __launch_bounds__(256) k_node_keys(const FieldTri* __restrict__ 
         const int ms = can_reuse ? node_first_static(G, mfirst, mlast, P) : mlast + 1;
         for (int m = mfirst; m <= mlast; m++) {
             const int64_t it = it0 + (m - mfirst);
-            if (!node_path_item(ikey[it])) continue;
+            if (!node_path_item(ikey[it], klo, khi)) continue;
             const int key = node_task_key(G, make_level((double)m * P.dt, P.cP, P.cS));
             int tk = key < 0 ? kNodeSkip : key;
             const int d = m - ms;
@@ -532,7 +533,7 @@ template <int NGH>
 __global__ void __launch_bounds__(256) k_node_scatter(int64_t npairs, int64_t per, const int64_t* __restrict__ off,
                                                       const int16_t* __restrict__ lo,
                                                       const int16_t* __restrict__ hi,
-                                                      const uint8_t* __restrict__ ikey,
+                                                      const uint8_t* __restrict__ ikey, int klo, int khi,
                                                       const uint8_t* __restrict__ tkey,
                                                       const unsigned long long* __restrict__ bo,
                                                       int32_t* __restrict__ tperm) {
@@ -550,7 +551,7 @@ __global__ void __launch_bounds__(256) k_node_scatter(int64_t npairs, int64_t pe
         const int64_t it0 = off[p];
         for (int m = mfirst; m <= mlast; m++) {
             const int64_t it = it0 + (m - mfirst);
-            if (!node_path_item(ikey[it])) continue;
+            if (!node_path_item(ikey[it], klo, khi)) continue;
             const uint8_t key = tkey[it * NGH + q];
             if (key >= kNodeKeys) continue;
             const unsigned long long pos = atomicAdd(&cur[key], 1ull);
@@ -559,7 +560,9 @@ __global__ void __launch_bounds__(256) k_node_scatter(int64_t npairs, int64_t pe
     }
 }
 
-template <int NGH>
+// R: the arithmetic of the closed forms (float for the far field of the
+// mixed form; the front cutting stays FP64).
+template <int NGH, typename R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_LEVELS_WAVES)))
 k_node_eval(const FieldTri* __restrict__ tri, const double* __restrict__ gh, AsmParams P, int r0, int64_t ntask,
             const int32_t* __restrict__ item_pair, const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
@@ -582,7 +585,7 @@ k_node_eval(const FieldTri* __restrict__ tri, const double* __restrict__ gh, Asm
         NodeAcc acc;
         acc.s = 0.0;
         for (int e = 0; e < 6; e++) acc.m[e] = 0.0;
-        node_eval<double>(G, K, g[96 + q], acc);
+        node_eval<R>(G, K, g[96 + q], acc);
         double* o = part + (int64_t)task * 8;
         o[0] = acc.s;
 #pragma unroll

@@ -768,10 +768,10 @@ void launch_levels_split(ebem_handle* h, AsmParams P, int r0, int64_t nitems, in
 // must stay below 2^31 (the chunk budget keeps it so).  With static-node
 // reuse on, a failed check of the reused values (k_node_eval's *bad) makes
 // the chunk run again without reuse.
-template <int NGH>
+template <int NGH, typename R>
 int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t npairs, int64_t nchunk, int64_t n,
                  const int32_t* item_pair, const int64_t* off, const int16_t* lo, const int16_t* hi,
-                 const uint8_t* ikey, const uint8_t* sflag, const int32_t* perm, double* W) {
+                 const uint8_t* ikey, int klo, int khi, const uint8_t* sflag, const int32_t* perm, double* W) {
     if (n <= 0) return 0;
     const int64_t ntask = nchunk * NGH;
     if (ntask > INT32_MAX) return fail(EBEM_EINVAL, "too many node tasks in one chunk");
@@ -786,7 +786,7 @@ int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t npairs, int64_t nc
     const int64_t per = (npairs + G - 1) / G;  // pairs per sort block
     for (int reuse = h->node_reuse ? 1 : 0; reuse >= 0; reuse--) {
         hipLaunchKernelGGL(k_node_keys<NGH>, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, r0, npairs,
-                           per, off, lo, hi, ikey, sflag, reuse, nw.key.p, nw.hist.p);
+                           per, off, lo, hi, ikey, klo, khi, sflag, reuse, nw.key.p, nw.hist.p);
         HIPC(hipGetLastError());
         std::vector<unsigned int> hist((size_t)(kNodeKeys + 1) * G);
         std::vector<unsigned long long> cur((size_t)kNodeKeys * G);
@@ -804,12 +804,12 @@ int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t npairs, int64_t nc
                             h->stream));
         // block b of the scatter takes the tasks of the items block b keyed
         hipLaunchKernelGGL(k_node_scatter<NGH>, dim3(G), dim3(256), 0, h->stream, npairs, per, off, lo, hi, ikey,
-                           nw.key.p, nw.cur.p, nw.perm.p);
+                           klo, khi, nw.key.p, nw.cur.p, nw.perm.p);
         HIPC(hipGetLastError());
         HIPC(hipMemsetAsync(nw.bad.p, 0, sizeof(unsigned int), h->stream));
         if (run > 0) {
             int64_t blocks = std::min<int64_t>(((int64_t)run + 255) / 256, (int64_t)h->CUs * 16);
-            hipLaunchKernelGGL(k_node_eval<NGH>, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0,
+            hipLaunchKernelGGL((k_node_eval<NGH, R>), dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0,
                                h->stream, h->d_tri.p, h->d_gh.p, P, r0, (int64_t)run, item_pair, off, lo, hi,
                                nw.perm.p, reuse, nw.part.p, nw.bad.p);
             HIPC(hipGetLastError());
@@ -830,8 +830,8 @@ int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t npairs, int64_t nc
 }
 
 // The per-level items perm[0..nitems) of a chunk of nchunk items, in key
-// order: far-field items (mixed precision, nfar) by k_levels in R_far,
-// coplanar pairs (ncop) by k_levels in FP64, the rest by outer node.
+// order: far-field items (mixed precision, nfar) by outer node in FP32,
+// coplanar pairs (ncop) by k_levels in FP64, the rest by outer node in FP64.
 // Unsorted items (perm null) or EBEM_NODE_TASKS=0: all by k_levels.
 template <int NGH>
 int launch_per_level(ebem_handle* h, AsmParams P, int r0, int64_t npairs, int64_t nchunk, int64_t nitems,
@@ -841,10 +841,14 @@ int launch_per_level(ebem_handle* h, AsmParams P, int r0, int64_t npairs, int64_
         launch_levels_split<NGH>(h, P, r0, nitems, nfar, item_pair, off, lo, perm, W);
         return 0;
     }
-    if (nfar > 0) launch_levels<NGH, float>(h, P, r0, nfar, item_pair, off, lo, perm, W);
+    if (nfar > 0) {
+        if (int e = launch_nodes<NGH, float>(h, P, r0, npairs, nchunk, nfar, item_pair, off, lo, hi, ikey, 0,
+                                             kFrontKeys, sflag, perm, W))
+            return e;
+    }
     launch_levels<NGH, double>(h, P, r0, ncop, item_pair, off, lo, perm + nfar, W);
-    return launch_nodes<NGH>(h, P, r0, npairs, nchunk, nitems - nfar - ncop, item_pair, off, lo, hi, ikey, sflag,
-                             perm + nfar + ncop, W);
+    return launch_nodes<NGH, double>(h, P, r0, npairs, nchunk, nitems - nfar - ncop, item_pair, off, lo, hi, ikey,
+                                     kNearKeys, kNearKeys + kFrontKeys, sflag, perm + nfar + ncop, W);
 }
 
 int assemble(ebem_handle* h, int precision) {
