@@ -64,6 +64,12 @@ PEAK_HBM_GBS = 8000.0
 # 585 per level and evaluation point (three children x nDiag nodes per outer
 # node).
 K_SF_POINT_BASE, K_SF_LEVEL = 242, 488
+# FP64 flops the hardware executes for the regular part (k_ksf) on the dd
+# workload (barH1 level 2, FN 19 64 3 256): 64 lanes x (2 FMA + ADD + MUL +
+# TRANS) wave instructions from the rocprofv3 --pmc pass, lane activity 1.00
+# (profiles/pmc_counters_barH1_2_dd.md); the source-level count above gives
+# 1.90e13 for the same work.
+K_SF_EXEC_FLOPS = 1.673e13
 K_SING_POINT_LEVEL = 585
 K_SF_CHUNK = 2
 
@@ -314,7 +320,7 @@ def k_windows_and_flops(m, pb, LK, nGH, nInt, nDiag, rows=None):
     nlev = np.where(b > a, b - np.maximum(a - 2, 1) + 1, 0)
     nchunk = (nlev + K_SF_CHUNK - 1) // K_SF_CHUNK
     reg_flops = float((nchunk * K_SF_POINT_BASE + nlev * K_SF_LEVEL).sum())
-    return slots, reg_flops * nGH * nInt + sing_flops
+    return slots, reg_flops * nGH * nInt, sing_flops
 
 
 def dd_main(args):
@@ -357,10 +363,13 @@ def dd_main(args):
     neu_wall = 1e3 * (time.perf_counter() - t0)
     neu_ms = s.timings()["mot_neumann_ms"]
     s.close()
-    slots, flops = k_windows_and_flops(m, pb, LK, nGH, nSub * nSing, nDiag)
+    slots, reg_flops, sing_flops = k_windows_and_flops(m, pb, LK, nGH, nSub * nSing, nDiag)
     assert slots == info["slots"], (slots, info["slots"])
     k_ms = float(np.mean([t["k_assembly_ms"] for t in tims]))
-    ach = flops / (k_ms * 1e-3) / 1e12
+    # the regular part at the FP64 work the hardware executes for it (the
+    # source-level count repeats level-independent products the compiler
+    # evaluates once per point), the singular part at its source-level count
+    ach = (K_SF_EXEC_FLOPS + sing_flops) / (k_ms * 1e-3) / 1e12
     out = {
         "metric": "double-layer K blocks (test triangle, vertex, lag) per second, direct Dirichlet driver",
         "value": slots / (k_ms * 1e-3), "unit": "K blocks/s",
@@ -383,7 +392,11 @@ def dd_main(args):
         "roofline": {"bound": "valu", "pipe": "fp64-valu", "kernel": "K assembly (k_ksf + k_kgather + k_ksing)",
                      "achieved": ach,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
-                     "traffic": None},
+                     "traffic": None,
+                     "flops": {"regular_executed": K_SF_EXEC_FLOPS, "regular_source_count": reg_flops,
+                               "singular_source_count": sing_flops,
+                               "source": "regular: rocprofv3 --pmc FP64 instruction counts of k_ksf "
+                                         "(profiles/pmc_counters_barH1_2_dd.md)"}},
         "cpu_baseline": None,
     }
     if not args.no_cpu:
@@ -397,7 +410,7 @@ def dd_main(args):
         for lag in range(LK):
             o.k_matrix(lag, rows, nthreads=threads)
         el = time.perf_counter() - t0
-        sub_slots, _ = k_windows_and_flops(m, pb, LK, nGH, nSub * nSing, nDiag, rows)
+        sub_slots, _, _ = k_windows_and_flops(m, pb, LK, nGH, nSub * nSing, nDiag, rows)
         out["cpu_baseline"] = {"value": sub_slots / el, "unit": "K blocks/s", "cores": threads, "kind": "port",
                                "sample": f"oracle (C/OpenMP FP64) K blocks of test row {rows[0]} for all "
                                          f"{LK} lags ({sub_slots} blocks, {el:.1f} s)"}
