@@ -359,15 +359,19 @@ __global__ void __launch_bounds__(256) k_item_scatter(int64_t nitems, int64_t pe
     }
 }
 
+// Waves per SIMD the per-level, node-task and post-processing integral
+// kernels are built for.  The node evaluation needs about 380 registers per
+// lane: at one wave per SIMD it runs without spills (256 VGPRs + AGPRs), at
+// two it spills ~450 B per lane to scratch.  One wave is measured faster on
+// gfx950 (1280-triangle sphere, 128 steps: per-level stage 46.6 vs 47.7 ms,
+// post-processing 52.0 vs 54.0 ms).
 #ifndef EBEM_LEVELS_WAVES
-#define EBEM_LEVELS_WAVES 2
+#define EBEM_LEVELS_WAVES 1
 #endif
 
 // Persistent grid-stride kernel over (pair, level) items of a row chunk.
 // The whole node evaluation is inlined (an out-of-line call spills ~1 KB of
-// stack per lane); a 256-VGPR budget (2 waves per SIMD) is the measured optimum
-// on gfx950: 209 ms vs 427 ms at 128 VGPRs and 605 ms out of line (1280-triangle
-// sphere, 5.9e7 items).
+// stack per lane; out-of-line region routines are 16% slower).
 // W[item*6 + q]: symmetric 3x3 in the global frame, NOT yet divided by 4 pi rho.
 template <int NGH, typename R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_LEVELS_WAVES))) k_levels(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
@@ -423,9 +427,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_L
 // of the pair.  A task whose node is static with a margin (node_static_margin)
 // at its level and at the level before is not evaluated: its key says how
 // many levels back the node's first static level is, whose slot the
-// reduction takes.  k_node_eval checks the exact pattern at every first
-// static level whose value is reused; a failed check raises *bad, and the
-// caller then evaluates the chunk again without reuse.
+// reduction takes.  k_node_keys checks the exact pattern at each (pair,
+// node)'s first static level; a failed check raises *bad, and the caller
+// then keys the chunk again without reuse.
 constexpr int kNodeKeys = 48;
 // post-processing items keyed by point state: keys 0..kNodeKeys, below the range keys
 static_assert(kNodeKeys < kKpKey, "point-state keys must precede the range keys");
@@ -486,14 +490,17 @@ __device__ inline bool node_path_item(uint8_t ikey, int klo, int khi) { return i
 // [b per, (b + 1) per) with all their nodes; a thread computes its node's
 // projection and first static level once and keys the node at each of the
 // pair's levels that are evaluated by node (ikey), counting the tasks per key
-// in bh[key * gridDim + b] (key kNodeKeys: the reused tasks).
+// in bh[key * gridDim + b] (key kNodeKeys: the reused tasks).  reuse: 0 none,
+// 1 static reuse with the exact check at the first static level, 2 the same
+// with every check reported failed (the caller's fallback, forced).
 template <int NGH>
 __global__ void __launch_bounds__(256) k_node_keys(const FieldTri* __restrict__ tri, const double* __restrict__ gh,
                                                    AsmParams P, int r0, int64_t npairs, int64_t per,
                                                    const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
                                                    const int16_t* __restrict__ hi, const uint8_t* __restrict__ ikey,
                                                    int klo, int khi, const uint8_t* __restrict__ sflag, int reuse,
-                                                   uint8_t* __restrict__ tkey, unsigned int* __restrict__ bh) {
+                                                   uint8_t* __restrict__ tkey, unsigned int* __restrict__ bh,
+                                                   unsigned int* __restrict__ bad) {
     __shared__ unsigned int sh[kNodeKeys + 1];  // + reused tasks
     for (int k = threadIdx.x; k <= kNodeKeys; k += blockDim.x) sh[k] = 0;
     __syncthreads();
@@ -512,6 +519,11 @@ __global__ void __launch_bounds__(256) k_node_keys(const FieldTri* __restrict__ 
         const NodeGeo G = node_geo(x, tri[j]);
         const bool can_reuse = reuse && !(sflag && sflag[p]);
         const int ms = can_reuse ? node_first_static(G, mfirst, mlast, P) : mlast + 1;
+        // the levels after ms take the value evaluated at ms: the exact
+        // pattern must hold there
+        if (ms < mlast && node_path_item(ikey[it0 + (ms - mfirst)], klo, khi) &&
+            (reuse == 2 || !node_static_pattern(G, make_level((double)ms * P.dt, P.cP, P.cS))))
+            atomicOr(bad, 1u);
         for (int m = mfirst; m <= mlast; m++) {
             const int64_t it = it0 + (m - mfirst);
             if (!node_path_item(ikey[it], klo, khi)) continue;
@@ -566,8 +578,7 @@ template <int NGH, typename R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_LEVELS_WAVES)))
 k_node_eval(const FieldTri* __restrict__ tri, const double* __restrict__ gh, AsmParams P, int r0, int64_t ntask,
             const int32_t* __restrict__ item_pair, const int64_t* __restrict__ off, const int16_t* __restrict__ lo,
-            const int16_t* __restrict__ hi, const int32_t* __restrict__ tperm, int reuse,
-            double* __restrict__ part, unsigned int* __restrict__ bad) {
+            const int32_t* __restrict__ tperm, double* __restrict__ part) {
     for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < ntask;
          s += (int64_t)gridDim.x * blockDim.x) {
         const int32_t task = tperm[s];
@@ -591,12 +602,6 @@ k_node_eval(const FieldTri* __restrict__ tri, const double* __restrict__ gh, Asm
 #pragma unroll
         for (int e = 0; e < 6; e++) o[1 + e] = acc.m[e];
         o[7] = 0.0;
-        // the first static level of a node, whose value the next levels
-        // reuse: the exact pattern must hold there
-        if (reuse && m < hi[p] + 1 && node_static_margin(G, K) &&
-            (m == mfirst || !node_static_margin(G, make_level((double)(m - 1) * P.dt, P.cP, P.cS))) &&
-            !node_static_pattern(G, K))
-            atomicOr(bad, 1u);
     }
 }
 

@@ -488,7 +488,10 @@ struct ebem_handle {
         DevBuf<unsigned long long> cur;
         DevBuf<unsigned int> bad;
     } nw;  // node-task scratch of ebem_assemble (grow-only)
-    bool node_reuse = true;  // static node values reused across levels (EBEM_NODE_REUSE=0: evaluated per level)
+    // static node values reused across levels: 1 (default), 0 evaluated per
+    // level (EBEM_NODE_REUSE=0), 2 every chunk's check reported failed so that
+    // it falls back to 0 (EBEM_NODE_REUSE=fallback, for tests of that path)
+    int node_reuse = 1;
     // node tasks of the last assembly: evaluated, reused (static), chunks
     // evaluated again without reuse after a failed check of the reused values
     int64_t node_stats[3] = {0, 0, 0};
@@ -766,8 +769,8 @@ void launch_levels_split(ebem_handle* h, AsmParams P, int r0, int64_t nitems, in
 // keys, counting sort of the node tasks, one node per lane, reduction per
 // item (k_node_* in ebem_kernels.hpp).  Task ids it * NGH + q (it < nchunk)
 // must stay below 2^31 (the chunk budget keeps it so).  With static-node
-// reuse on, a failed check of the reused values (k_node_eval's *bad) makes
-// the chunk run again without reuse.
+// reuse on, a failed check of the reused values (k_node_keys' *bad) makes
+// the chunk be keyed again without reuse.
 template <int NGH, typename R>
 int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t npairs, int64_t nchunk, int64_t n,
                  const int32_t* item_pair, const int64_t* off, const int16_t* lo, const int16_t* hi,
@@ -784,15 +787,22 @@ int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t npairs, int64_t nc
     HIPC(nw.cur.grow((size_t)kNodeKeys * G));
     HIPC(nw.bad.grow(1));
     const int64_t per = (npairs + G - 1) / G;  // pairs per sort block
-    for (int reuse = h->node_reuse ? 1 : 0; reuse >= 0; reuse--) {
+    for (int reuse = h->node_reuse;; reuse = 0) {
+        HIPC(hipMemsetAsync(nw.bad.p, 0, sizeof(unsigned int), h->stream));
         hipLaunchKernelGGL(k_node_keys<NGH>, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, r0, npairs,
-                           per, off, lo, hi, ikey, klo, khi, sflag, reuse, nw.key.p, nw.hist.p);
+                           per, off, lo, hi, ikey, klo, khi, sflag, reuse, nw.key.p, nw.hist.p, nw.bad.p);
         HIPC(hipGetLastError());
         std::vector<unsigned int> hist((size_t)(kNodeKeys + 1) * G);
         std::vector<unsigned long long> cur((size_t)kNodeKeys * G);
+        unsigned int bad = 0;
         HIPC(hipMemcpyAsync(hist.data(), nw.hist.p, sizeof(unsigned int) * hist.size(), hipMemcpyDeviceToHost,
                             h->stream));
+        HIPC(hipMemcpyAsync(&bad, nw.bad.p, sizeof bad, hipMemcpyDeviceToHost, h->stream));
         HIPC(hipStreamSynchronize(h->stream));
+        if (reuse && bad) {
+            h->node_stats[2]++;
+            continue;
+        }
         unsigned long long run = 0;
         for (size_t k = 0; k < cur.size(); k++) {  // key-major, block-minor
             cur[k] = run;
@@ -806,12 +816,11 @@ int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t npairs, int64_t nc
         hipLaunchKernelGGL(k_node_scatter<NGH>, dim3(G), dim3(256), 0, h->stream, npairs, per, off, lo, hi, ikey,
                            klo, khi, nw.key.p, nw.cur.p, nw.perm.p);
         HIPC(hipGetLastError());
-        HIPC(hipMemsetAsync(nw.bad.p, 0, sizeof(unsigned int), h->stream));
         if (run > 0) {
             int64_t blocks = std::min<int64_t>(((int64_t)run + 255) / 256, (int64_t)h->CUs * 16);
             hipLaunchKernelGGL((k_node_eval<NGH, R>), dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0,
-                               h->stream, h->d_tri.p, h->d_gh.p, P, r0, (int64_t)run, item_pair, off, lo, hi,
-                               nw.perm.p, reuse, nw.part.p, nw.bad.p);
+                               h->stream, h->d_tri.p, h->d_gh.p, P, r0, (int64_t)run, item_pair, off, lo,
+                               nw.perm.p, nw.part.p);
             HIPC(hipGetLastError());
         }
         hipLaunchKernelGGL(k_node_reduce<NGH>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
@@ -819,14 +828,8 @@ int launch_nodes(ebem_handle* h, AsmParams P, int r0, int64_t npairs, int64_t nc
         HIPC(hipGetLastError());
         h->node_stats[0] += (int64_t)run;
         h->node_stats[1] += reused;
-        if (!reuse) break;
-        unsigned int bad = 0;
-        HIPC(hipMemcpyAsync(&bad, nw.bad.p, sizeof bad, hipMemcpyDeviceToHost, h->stream));
-        HIPC(hipStreamSynchronize(h->stream));
-        if (!bad) break;
-        h->node_stats[2]++;
+        return 0;
     }
-    return 0;
 }
 
 // The per-level items perm[0..nitems) of a chunk of nchunk items, in key
@@ -1924,7 +1927,7 @@ int ebem_create(ebem_handle** out, int device) {
     const char* nts = std::getenv("EBEM_NODE_TASKS");
     h->node_tasks = !(nts && std::strcmp(nts, "0") == 0);
     const char* nru = std::getenv("EBEM_NODE_REUSE");
-    h->node_reuse = !(nru && std::strcmp(nru, "0") == 0);
+    h->node_reuse = (nru && std::strcmp(nru, "0") == 0) ? 0 : (nru && std::strcmp(nru, "fallback") == 0) ? 2 : 1;
     const char* kbf = std::getenv("EBEM_K_BY_FIELD");
     h->k_by_field = !(kbf && std::strcmp(kbf, "0") == 0);
     const char* pbf = std::getenv("EBEM_PPD_BY_FIELD");

@@ -23,7 +23,7 @@ def _assemble(split, node_tasks=True, reuse=True):
     m = read_mesh(os.path.join(ROOT, "tests", "golden", "meshes", "sphereUniform_3.mesh"))
     pb = api.PbParam(rho=1.0, mu=1.0, lambda_=1.0, Tfin=4.0, nT=128)
     env = {"EBEM_KP_SPLIT": "1" if split else "0", "EBEM_NODE_TASKS": "1" if node_tasks else "0",
-           "EBEM_NODE_REUSE": "1" if reuse else "0"}
+           "EBEM_NODE_REUSE": reuse if isinstance(reuse, str) else ("1" if reuse else "0")}
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
@@ -102,6 +102,26 @@ def test_static_node_reuse_is_bit_identical():
         a.close()
         b.close()
 
+
+
+def test_static_node_reuse_fallback_path():
+    """The fallback after a failed check of the reused static values, forced
+    from the host (EBEM_NODE_REUSE=fallback: every chunk's check reports a
+    failure): each chunk is keyed again without reuse, so the assembly equals
+    the one without reuse bit for bit and no task is reused."""
+    a = _assemble(True, reuse="fallback")
+    b = _assemble(True, reuse=False)
+    try:
+        assert a.store_info() == b.store_info()
+        ta, tb = a.timings(), b.timings()
+        print(f"fallbacks {ta['node_reuse_fallbacks']:.0f}; tasks {ta['node_tasks']:.0f} vs {tb['node_tasks']:.0f}")
+        assert ta["node_reuse_fallbacks"] >= 1
+        assert ta["node_tasks_reused"] == 0 and ta["node_tasks"] == tb["node_tasks"]
+        for lag in range(a.num_blocks()):
+            assert np.array_equal(a.get_block(lag), b.get_block(lag)), lag
+    finally:
+        a.close()
+        b.close()
 
 def test_kp_split_matches_per_level_assembly():
     a = _assemble(True)
