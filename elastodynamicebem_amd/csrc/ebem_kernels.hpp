@@ -268,8 +268,18 @@ constexpr int kBfRestKey = kBfKey + kBfHeadKeys;
 constexpr int kSurKey = kBfRestKey + 1;
 constexpr int kNumKeys = kSurKey + 1;
 
+// The post-processing's between-front ranges (k_pp_M_bf) in chunks of 8
+// levels (8.2 ms against 9.0 with 6 on 4096 points x 1280 triangles).
+#ifndef EBEM_PP_BF_CHUNK
+#define EBEM_PP_BF_CHUNK 8
+#endif
+constexpr int kPpBfChunk = EBEM_PP_BF_CHUNK;
+
+// Range key by the number of chunks of C levels (waves of ranges with equal
+// chunk counts run convergent).
+template <int C>
 __device__ inline int bf_head_key(int a, int b) {
-    const int chunks = (b - a + kBfChunk) / kBfChunk;
+    const int chunks = (b - a + C) / C;
     return kBfKey + (chunks < kBfHeadKeys ? chunks : kBfHeadKeys) - 1;
 }
 static_assert(kNumKeys <= 256, "keys are stored as uint8");
@@ -329,7 +339,7 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
         const int near_key = qp * kFrontClasses + qs + (coplanar_pair(S, F) ? kCopKeys : kNearKeys);
         int k = (P.far2 >= 0.0 && d2 > P.far2) ? qp * kFrontClasses + qs : near_key;
         if (ka && m >= ka[p] && m <= kb[p]) k = (m == ka[p]) ? kKpKey : kKpRestKey;
-        if (sa && m >= sa[p] && m <= sb[p]) k = (m == sa[p]) ? bf_head_key(sa[p], sb[p]) : kBfRestKey;
+        if (sa && m >= sa[p] && m <= sb[p]) k = (m == sa[p]) ? bf_head_key<kBfChunk>(sa[p], sb[p]) : kBfRestKey;
         // Surrogate pairs: the network predicts the lags max(lo,1)..hi-1; the
         // analytic routine (FP64, level by level) keeps E_0 = W_1 / (4 pi rho),
         // which the marching inverts, and the static level hi+1, which the
@@ -359,12 +369,13 @@ __global__ void __launch_bounds__(256) k_item_scatter(int64_t nitems, int64_t pe
     }
 }
 
-// Waves per SIMD the per-level, node-task and post-processing integral
-// kernels are built for.  The node evaluation needs about 380 registers per
-// lane: at one wave per SIMD it runs without spills (256 VGPRs + AGPRs), at
-// two it spills ~450 B per lane to scratch.  One wave is measured faster on
-// gfx950 (1280-triangle sphere, 128 steps: per-level stage 46.6 vs 47.7 ms,
-// post-processing 52.0 vs 54.0 ms).
+// Waves per SIMD the per-level, node-task, kP-range and post-processing
+// integral kernels are built for.  The node evaluation needs about 380
+// registers per lane: at one wave per SIMD it runs without spills (256 VGPRs
+// + AGPRs), at two it spills ~450 B per lane to scratch (~980 B for the kP
+// ranges).  One wave is measured faster on gfx950 (1280-triangle sphere, 128
+// steps: per-level stage 46.6 vs 47.7 ms, post-processing 52.0 vs 54.0 ms,
+// kP ranges 3.2 vs 4.6 ms and 1.8 vs 3.1 ms).
 #ifndef EBEM_LEVELS_WAVES
 #define EBEM_LEVELS_WAVES 1
 #endif
@@ -642,7 +653,7 @@ __global__ void __launch_bounds__(256) k_node_reduce(const FieldTri* __restrict_
 // items of the pair).  If some node is not in the assumed state, the range is
 // evaluated level by level as in k_levels.
 template <int NGH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_levels_kp(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_LEVELS_WAVES))) k_levels_kp(
     const FieldTri* __restrict__ tri, const double* __restrict__ gh, AsmParams P, int r0, int64_t nitems,
     const int32_t* __restrict__ item_pair, const int16_t* __restrict__ ka, const int16_t* __restrict__ kb,
     const int32_t* __restrict__ perm, double* __restrict__ W, unsigned int* __restrict__ nfallback) {
@@ -1317,7 +1328,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_L
 // M_m = (WP + tau_m^2 WQ) / (4 pi rho) with the 1e-14 cleanup for m = ka..kb
 // (consecutive items of the point-triangle pair).  A range whose state check
 // fails is evaluated per m as in k_pp_M.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_pp_M_kp(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_LEVELS_WAVES))) k_pp_M_kp(
     const FieldTri* __restrict__ tri, const double* __restrict__ xs, int p0, PPParams P, int64_t nheads,
     const int32_t* __restrict__ item_q, const int32_t* __restrict__ ka, const int32_t* __restrict__ kb,
     const int32_t* __restrict__ perm, double* __restrict__ M, unsigned int* __restrict__ nfallback) {
@@ -1370,7 +1381,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 
 // One thread per between-front range of m (the items of key kBfKey, m = sa):
 // M_m for m = sa..sb through node_eval_bf (one outer point, weight 1) in
-// chunks of kBfChunk, the same values as k_pp_M; from a chunk whose state
+// chunks of kPpBfChunk, the same values as k_pp_M; from a chunk whose state
 // check fails, per m as in k_pp_M.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_BF_WAVES))) k_pp_M_bf(
     const FieldTri* __restrict__ tri, const double* __restrict__ xs, int p0, PPParams P, int64_t nheads,
@@ -1386,20 +1397,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_B
         double x[3] = {xs[3 * p], xs[3 * p + 1], xs[3 * p + 2]};
         const NodeGeo G = node_geo(x, F);
         int m = ma;
-        for (; m <= mb; m += kBfChunk) {
-            const int n = mb - m + 1 < kBfChunk ? mb - m + 1 : kBfChunk;
-            double t[kBfChunk];
+        for (; m <= mb; m += kPpBfChunk) {
+            const int n = mb - m + 1 < kPpBfChunk ? mb - m + 1 : kPpBfChunk;
+            double t[kPpBfChunk];
 #pragma unroll
-            for (int l = 0; l < kBfChunk; l++) t[l] = pp_tau(P, m + (l < n ? l : 0));
-            NodeAcc acc[kBfChunk];
+            for (int l = 0; l < kPpBfChunk; l++) t[l] = pp_tau(P, m + (l < n ? l : 0));
+            NodeAcc acc[kPpBfChunk];
 #pragma unroll
-            for (int l = 0; l < kBfChunk; l++) {
+            for (int l = 0; l < kPpBfChunk; l++) {
                 acc[l].s = 0.0;
                 for (int k = 0; k < 6; k++) acc[l].m[k] = 0.0;
             }
-            if (!node_eval_bf<kBfChunk>(G, t, n, P.cP, P.cS, 1.0, acc)) break;
+            if (!node_eval_bf<kPpBfChunk>(G, t, n, P.cP, P.cS, 1.0, acc)) break;
 #pragma unroll
-            for (int l = 0; l < kBfChunk; l++) {
+            for (int l = 0; l < kPpBfChunk; l++) {
                 if (l < n) {
                     double W[6];
                     finish_rotation(acc[l], F.C, W);
@@ -1561,7 +1572,7 @@ __global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tr
         if (ka && m >= ka[q] && m <= kb[q]) {
             k = (m == ka[q]) ? kKpKey : kKpRestKey;
         } else if (sa && m >= sa[q] && m <= sb[q]) {
-            k = (m == sa[q]) ? bf_head_key(sa[q], sb[q]) : kBfRestKey;
+            k = (m == sa[q]) ? bf_head_key<kPpBfChunk>(sa[q], sb[q]) : kBfRestKey;
         } else {
             // an item of k_pp_M is one point at one level: keyed by that
             // point's front state (node_task_key), so that a wavefront's

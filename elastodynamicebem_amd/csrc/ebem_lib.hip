@@ -1440,6 +1440,9 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     while (g0 < order.size()) {
         size_t g1 = g0 + 1;
         while (g1 < order.size() && delta[order[g1]] - delta[order[g0]] <= 1e-12 * dt) g1++;
+        // a group's times by step, so that k_pp_conv's lanes read consecutive
+        // density steps
+        std::sort(order.begin() + g0, order.begin() + g1, [&](int a, int b) { return nhat[a] < nhat[b]; });
         std::vector<int32_t> nh, kid;
         int mmax = -1, krep = order[g0];
         for (size_t q = g0; q < g1; q++) {

@@ -870,9 +870,11 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_kp(cons
 }
 
 // Levels sharing one evaluation of the level-independent region terms in
-// node_eval_bf.
+// node_eval_bf for the assembly's between-front ranges (k_levels_bf: 6 at one
+// wave per SIMD runs without the spills of 8, 9.3 against 10.3 ms on the
+// 1280-triangle sphere; 4 is slower, 10.7 ms).
 #ifndef EBEM_BF_CHUNK
-#define EBEM_BF_CHUNK 8
+#define EBEM_BF_CHUNK 6
 #endif
 constexpr int kBfChunk = EBEM_BF_CHUNK;
 
