@@ -1069,11 +1069,14 @@ __global__ void __launch_bounds__(256) k_history(Store S, int N, int L, int n, i
 }
 
 // y[row] = sum_k A[row][k] x[k]; A row-major n x n; one wave per row, 4 rows per block.
-// If yT is set, the value also goes to yT[((row / 3) nT + step) 3 + row % 3]
-// (the density's triangle-major copy).
+// If yT is set, the value also goes to yT[((row / 3) (nT + 1) + 1 + step) 3 +
+// row % 3] (the density's triangle-major copy, each triangle's row led by one
+// zero step) and the same place of yD receives the step difference
+// y[row] - (y - n)[row] (the previous step's value; 0 at step 0), both read by
+// k_pp_conv.
 __global__ void __launch_bounds__(256) k_gemv_rows(const double* __restrict__ A, const double* __restrict__ x,
-                                                   double* __restrict__ y, int n, double* __restrict__ yT, int nT,
-                                                   int step) {
+                                                   double* __restrict__ y, int n, double* __restrict__ yT,
+                                                   double* __restrict__ yD, int nT, int step) {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
@@ -1089,7 +1092,11 @@ __global__ void __launch_bounds__(256) k_gemv_rows(const double* __restrict__ A,
     for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
     if (lane == 0) {
         y[row] = s;
-        if (yT) yT[((int64_t)(row / 3) * nT + step) * 3 + row % 3] = s;
+        if (yT) {
+            const int64_t o = ((int64_t)(row / 3) * (nT + 1) + 1 + step) * 3 + row % 3;
+            yT[o] = s;
+            yD[o] = s - (step > 0 ? y[(int64_t)row - n] : 0.0);
+        }
     }
 }
 
@@ -1444,91 +1451,154 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_B
     }
 }
 
-// One workgroup per point, one wave per slice of triangles (j = s, s + S,
-// s + 2S, ...); lane l of a wave takes the requested times kb + l + 64 i,
-// i < kConvPerLane, so one pass covers 64 kConvPerLane times.  The window and
-// the M values of a (point, triangle) pair are wave-uniform (scalar loads, one
-// difference D_m per wave); only the density reads and the accumulation are
-// per lane.  The slice partials are reduced through LDS.  aT: density as
-// [triangle][step][3].
+// One workgroup per group of kConvGroup points of the chunk, one wave per
+// slice of triangles (j = s, s + S, s + 2S, ...); lane l of a wave takes the
+// requested times kb + l + 64 i, i < kConvPerLane, so one pass covers
+// 64 kConvPerLane times.  The time sum of postProcA_singlePoint.m,
+// sum_m (M_m - M_{m-1}) alpha[n - m] over a (point, triangle) window [a, b]
+// (M_{a-1} = 0; beyond b the differences vanish), is taken by parts:
+//     sum_{m=a..b-1} M_m dalpha[n - m] + M_b alpha[n - b],
+// dalpha[k] = alpha[k] - alpha[k-1], alpha at negative steps 0.  The density
+// differences a lane loads at step n - m serve every point of the group whose
+// window holds m (the points are ordered along a space-filling curve, so the
+// windows of a group overlap), and each (point, m) costs the 9 FMAs of the
+// symmetric 3x3 product with M in scalar registers.  Window bounds and M
+// values are wave-uniform (scalar loads); only the density reads and the
+// accumulation are per lane.  The slice partials are reduced through LDS,
+// point by point.  aT, dT: alpha and dalpha as [triangle][1 + nT][3], each
+// triangle's row led by one zero step, which a lane reads whenever its step
+// n - m (or n - b) is negative: the reads need no branch, and the idle lanes
+// of a wave all read that one zero step.  The point of chunk row r is
+// written to column pout[p0 + r] of u.
 #ifndef EBEM_CONV_SLICES
 #define EBEM_CONV_SLICES 8
 #endif
 constexpr int kConvSlices = EBEM_CONV_SLICES;
 constexpr int kConvPerLane = 2;
+#ifndef EBEM_CONV_GROUP
+#define EBEM_CONV_GROUP 2
+#endif
+constexpr int kConvGroup = EBEM_CONV_GROUP;
 
 __global__ void __launch_bounds__(64 * kConvSlices) k_pp_conv(const double* __restrict__ M,
                                                             const int64_t* __restrict__ off,
                                                             const int32_t* __restrict__ ms,
                                                             const int32_t* __restrict__ me,
-                                                            const double* __restrict__ aT, int N, int nT, int p0,
+                                                            const double* __restrict__ aT,
+                                                            const double* __restrict__ dT, int N, int nT, int p0,
+                                                            int np, const int32_t* __restrict__ pout,
                                                             const int32_t* __restrict__ nhat,
                                                             const int32_t* __restrict__ kidx, int nk, int npts,
                                                             double* __restrict__ u) {
     constexpr int KP = 64 * kConvPerLane;  // times per pass
+    constexpr int GP = kConvGroup;
     __shared__ double red[kConvSlices][KP][3];
-    const int r = blockIdx.x;
-    const int p = p0 + r;
+    const int r0 = blockIdx.x * GP;  // chunk row of the group's first point
+    const int ng = np - r0 < GP ? np - r0 : GP;
     const int lane = threadIdx.x % 64;
     const int slice = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));  // blockDim = 64 * kConvSlices
     for (int kb = 0; kb < nk; kb += KP) {
         int nh[kConvPerLane];
-        double acc[kConvPerLane][3];
+        double acc[GP][kConvPerLane][3];
 #pragma unroll
         for (int i = 0; i < kConvPerLane; i++) {
             const int kk = kb + i * 64 + lane;
             nh[i] = kk < nk ? nhat[kk] : -1;
-            acc[i][0] = acc[i][1] = acc[i][2] = 0.0;
         }
+#pragma unroll
+        for (int g = 0; g < GP; g++)
+#pragma unroll
+            for (int i = 0; i < kConvPerLane; i++) acc[g][i][0] = acc[g][i][1] = acc[g][i][2] = 0.0;
         for (int j = slice; j < N; j += kConvSlices) {
-            const int64_t q = (int64_t)r * N + j;
-            const int a = ms[q], b = me[q];
-            if (a > b) continue;
-            const int64_t o = off[q];
-            const double* aj = aT + (int64_t)j * nT * 3;
-            double pv[6] = {0, 0, 0, 0, 0, 0};
-            for (int m = a; m <= b; m++) {
-                const double* Mm = M + (o + (m - a)) * 6;
-                double d[6];
+            int a[GP], b[GP];
+            int64_t o[GP];
+            int A = nT, B = -1;  // union of the group's windows
 #pragma unroll
-                for (int k = 0; k < 6; k++) {
-                    d[k] = Mm[k] - pv[k];
-                    pv[k] = Mm[k];
-                }
-#pragma unroll
-                for (int i = 0; i < kConvPerLane; i++) {
-                    if (m <= nh[i]) {
-                        const double* al = aj + (int64_t)(nh[i] - m) * 3;
-                        const double x0 = al[0], x1 = al[1], x2 = al[2];
-                        acc[i][0] += d[0] * x0 + d[1] * x1 + d[2] * x2;
-                        acc[i][1] += d[1] * x0 + d[3] * x1 + d[4] * x2;
-                        acc[i][2] += d[2] * x0 + d[4] * x1 + d[5] * x2;
+            for (int g = 0; g < GP; g++) {
+                a[g] = 1;
+                b[g] = 0;
+                o[g] = 0;
+                if (g < ng) {
+                    const int64_t q = (int64_t)(r0 + g) * N + j;
+                    a[g] = ms[q];
+                    b[g] = me[q];
+                    o[g] = off[q];
+                    if (a[g] <= b[g]) {
+                        A = a[g] < A ? a[g] : A;
+                        B = b[g] > B ? b[g] : B;
                     }
                 }
             }
+            if (A > B) continue;
+            const double* dj = dT + ((int64_t)j * (nT + 1) + 1) * 3;  // step 0 of row j
+            for (int m = A; m < B; m++) {
+                double x[kConvPerLane][3];
+#pragma unroll
+                for (int i = 0; i < kConvPerLane; i++) {
+                    const int k = nh[i] - m < -1 ? -1 : nh[i] - m;
+                    const double* al = dj + k * 3;
+                    x[i][0] = al[0];
+                    x[i][1] = al[1];
+                    x[i][2] = al[2];
+                }
+#pragma unroll
+                for (int g = 0; g < GP; g++) {
+                    if (m < a[g] || m >= b[g]) continue;
+                    const double* Mm = M + (o[g] + (m - a[g])) * 6;
+                    const double d0 = Mm[0], d1 = Mm[1], d2 = Mm[2], d3 = Mm[3], d4 = Mm[4], d5 = Mm[5];
+#pragma unroll
+                    for (int i = 0; i < kConvPerLane; i++) {
+                        acc[g][i][0] = fma(d2, x[i][2], fma(d1, x[i][1], fma(d0, x[i][0], acc[g][i][0])));
+                        acc[g][i][1] = fma(d4, x[i][2], fma(d3, x[i][1], fma(d1, x[i][0], acc[g][i][1])));
+                        acc[g][i][2] = fma(d5, x[i][2], fma(d4, x[i][1], fma(d2, x[i][0], acc[g][i][2])));
+                    }
+                }
+            }
+            // M_b alpha[n - b]
+            const double* aj = aT + ((int64_t)j * (nT + 1) + 1) * 3;
+#pragma unroll
+            for (int g = 0; g < GP; g++) {
+                if (a[g] > b[g]) continue;
+                const double* Mb = M + (o[g] + (b[g] - a[g])) * 6;
+                const double d0 = Mb[0], d1 = Mb[1], d2 = Mb[2], d3 = Mb[3], d4 = Mb[4], d5 = Mb[5];
+#pragma unroll
+                for (int i = 0; i < kConvPerLane; i++) {
+                    const int k = nh[i] - b[g] < -1 ? -1 : nh[i] - b[g];
+                    const double* al = aj + k * 3;
+                    const double x0 = al[0], x1 = al[1], x2 = al[2];
+                    acc[g][i][0] = fma(d2, x2, fma(d1, x1, fma(d0, x0, acc[g][i][0])));
+                    acc[g][i][1] = fma(d4, x2, fma(d3, x1, fma(d1, x0, acc[g][i][1])));
+                    acc[g][i][2] = fma(d5, x2, fma(d4, x1, fma(d2, x0, acc[g][i][2])));
+                }
+            }
         }
 #pragma unroll
-        for (int i = 0; i < kConvPerLane; i++) {
-            red[slice][i * 64 + lane][0] = acc[i][0];
-            red[slice][i * 64 + lane][1] = acc[i][1];
-            red[slice][i * 64 + lane][2] = acc[i][2];
-        }
-        __syncthreads();
-        for (int t = threadIdx.x; t < KP; t += blockDim.x) {
-            const int kk = kb + t;
-            if (kk >= nk) continue;
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-            for (int sl = 0; sl < kConvSlices; sl++) {
-                s0 += red[sl][t][0];
-                s1 += red[sl][t][1];
-                s2 += red[sl][t][2];
+        for (int g = 0; g < GP; g++) {
+            if (g >= ng) break;
+#pragma unroll
+            for (int i = 0; i < kConvPerLane; i++) {
+                red[slice][i * 64 + lane][0] = acc[g][i][0];
+                red[slice][i * 64 + lane][1] = acc[g][i][1];
+                red[slice][i * 64 + lane][2] = acc[g][i][2];
             }
-            const int64_t k = kidx[kk];
-            u[3 * ((int64_t)p + (int64_t)npts * k) + 0] = s0;
-            u[3 * ((int64_t)p + (int64_t)npts * k) + 1] = s1;
-            u[3 * ((int64_t)p + (int64_t)npts * k) + 2] = s2;
+            __syncthreads();
+            const int64_t col = pout[p0 + r0 + g];
+            for (int t = threadIdx.x; t < KP; t += blockDim.x) {
+                const int kk = kb + t;
+                if (kk >= nk) continue;
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+                for (int sl = 0; sl < kConvSlices; sl++) {
+                    s0 += red[sl][t][0];
+                    s1 += red[sl][t][1];
+                    s2 += red[sl][t][2];
+                }
+                const int64_t k = kidx[kk];
+                u[3 * (col + (int64_t)npts * k) + 0] = s0;
+                u[3 * (col + (int64_t)npts * k) + 1] = s1;
+                u[3 * (col + (int64_t)npts * k) + 2] = s2;
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -1593,15 +1663,21 @@ __global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tr
     for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) bh[(int64_t)k * gridDim.x + blockIdx.x] = sh[k];
 }
 
-// aT[(j*nT + n)*3 + c] = alpha[n*3N + 3j + c]
-__global__ void k_alpha_T(const double* __restrict__ alpha, int N, int nT, double* __restrict__ aT) {
+// aT[(j*(nT+1) + 1 + n)*3 + c] = alpha[n*3N + 3j + c] and dT at the same
+// place alpha[n*3N + 3j + c] - alpha[(n-1)*3N + 3j + c] (0 for n = 0); the
+// leading zero step of each row is cleared by the caller.
+__global__ void k_alpha_T(const double* __restrict__ alpha, int N, int nT, double* __restrict__ aT,
+                          double* __restrict__ dT) {
     int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)N * nT * 3;
     if (id >= tot) return;
     int c = (int)(id % 3);
     int64_t r = id / 3;
     int n = (int)(r % nT), j = (int)(r / nT);
-    aT[id] = alpha[(int64_t)n * 3 * N + 3 * j + c];
+    const double v = alpha[(int64_t)n * 3 * N + 3 * j + c];
+    const int64_t o = ((int64_t)j * (nT + 1) + 1 + n) * 3 + c;
+    aT[o] = v;
+    dT[o] = v - (n > 0 ? alpha[(int64_t)(n - 1) * 3 * N + 3 * j + c] : 0.0);
 }
 
 // ------------------------------------------------------------------ RHS --

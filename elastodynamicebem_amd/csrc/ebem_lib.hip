@@ -436,7 +436,7 @@ struct ebem_handle {
     DevBuf<int32_t> d_col;
     DevBuf<double> d_val;  // 6 * nnzb
     // MOT
-    DevBuf<double> d_beta, d_alpha, d_alphaT, d_r, d_A, d_Ainv, d_part;  // alphaT: [triangle][step][3]
+    DevBuf<double> d_beta, d_alpha, d_alphaT, d_dalphaT, d_r, d_A, d_Ainv, d_part;  // alphaT, dalphaT: k_pp_conv
     DevBuf<rocblas_int> d_ipiv, d_info;
     bool have_beta = false, have_alpha = false;
     bool e0_pivoted = false;  // the last factorisation needed partial pivoting
@@ -453,7 +453,7 @@ struct ebem_handle {
     } aw;
     struct {
         DevBuf<double> x, aT, u, dens, M;
-        DevBuf<int32_t> ms, me, iq, nh, kidx, perm, ka, kb, sa, sb;
+        DevBuf<int32_t> ms, me, iq, nh, kidx, perm, ka, kb, sa, sb, pout;
         DevBuf<unsigned int> nfb, bh;
         DevBuf<int64_t> off, rb, cnt;
         DevBuf<uint8_t> key;
@@ -1249,7 +1249,7 @@ int factor_e0(ebem_handle* h) {
         HIPC(dv.alloc(2 * (size_t)n3));
         HIPC(hipMemcpyAsync(dv.p, v.data(), sizeof(double) * n3, hipMemcpyHostToDevice, h->stream));
         hipLaunchKernelGGL(k_gemv_rows, dim3((n3 + 3) / 4), dim3(256), 0, h->stream, h->d_Ainv.p, dv.p, dv.p + n3, n3,
-                           nullptr, 0, 0);
+                           nullptr, nullptr, 0, 0);
         HIPC(hipMemsetAsync(h->d_A.p, 0, sizeof(double) * (size_t)n3 * n3, h->stream));
         hipLaunchKernelGGL(k_e0_dense, dim3(4, h->N), dim3(64), 0, h->stream, h->store(), h->N, L, h->d_A.p);
         // d_A holds E_0 column-major: E_0 y by rocBLAS dgemv
@@ -1288,7 +1288,12 @@ int mot(ebem_handle* h, const double* beta, double* alpha_out) {
         return fail(EBEM_EINVAL, "no right-hand side: pass beta or call ebem_rhs first");
     }
     HIPC(h->d_alpha.alloc((size_t)n3 * nT));
-    HIPC(h->d_alphaT.alloc((size_t)n3 * nT));
+    // triangle-major copies of the density and of its step differences for
+    // the post-processing, each triangle's row led by one zero step
+    HIPC(h->d_alphaT.alloc((size_t)n3 * (nT + 1)));
+    HIPC(h->d_dalphaT.alloc((size_t)n3 * (nT + 1)));
+    HIPC(hipMemsetAsync(h->d_alphaT.p, 0, sizeof(double) * h->d_alphaT.n, h->stream));
+    HIPC(hipMemsetAsync(h->d_dalphaT.p, 0, sizeof(double) * h->d_dalphaT.n, h->stream));
     HIPC(h->d_r.alloc(n3));
     Timer tall, tf;
     tall.start(h->stream);
@@ -1324,7 +1329,7 @@ int mot(ebem_handle* h, const double* beta, double* alpha_out) {
             launches++;
             HIPC(hipEventRecord(ev[3 * n + 1], h->stream));
             hipLaunchKernelGGL(k_gemv_rows, dim3((n3 + 3) / 4), dim3(256), 0, h->stream, h->d_Ainv.p, h->d_r.p,
-                               h->d_alpha.p + (size_t)n * n3, n3, h->d_alphaT.p, nT, n);
+                               h->d_alpha.p + (size_t)n * n3, n3, h->d_alphaT.p, h->d_dalphaT.p, nT, n);
             HIPC(hipEventRecord(ev[3 * n + 2], h->stream));
         }
     }
@@ -1377,8 +1382,38 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     double ppk_ms = 0, ppb_ms = 0;
     tall.start(h->stream);
     auto &d_x = h->pw.x, &d_aT = h->pw.aT, &d_u = h->pw.u, &d_dens = h->pw.dens, &d_M = h->pw.M;
+    auto& d_pout = h->pw.pout;
+    // points in the order of a Morton curve over their bounding box, so that
+    // the point groups of k_pp_conv are neighbours; pout maps back
+    std::vector<int32_t> pout(npts);
+    {
+        double lo[3], hi[3];
+        for (int c = 0; c < 3; c++) lo[c] = hi[c] = x[c];
+        for (int p = 0; p < npts; p++)
+            for (int c = 0; c < 3; c++) {
+                lo[c] = std::min(lo[c], x[3 * p + c]);
+                hi[c] = std::max(hi[c], x[3 * p + c]);
+            }
+        std::vector<uint32_t> code(npts);
+        for (int p = 0; p < npts; p++) {
+            uint32_t z = 0;
+            for (int c = 0; c < 3; c++) {
+                const double e = hi[c] - lo[c];
+                const uint32_t qv = e > 0 ? (uint32_t)std::min(1023.0, std::floor((x[3 * p + c] - lo[c]) / e * 1024.0)) : 0u;
+                for (int bit = 0; bit < 10; bit++) z |= ((qv >> bit) & 1u) << (3 * bit + c);
+            }
+            code[p] = z;
+            pout[p] = p;
+        }
+        std::stable_sort(pout.begin(), pout.end(), [&](int32_t a, int32_t b) { return code[a] < code[b]; });
+    }
+    std::vector<double> xs((size_t)3 * npts);
+    for (int r = 0; r < npts; r++)
+        for (int c = 0; c < 3; c++) xs[3 * (size_t)r + c] = x[3 * (size_t)pout[r] + c];
     HIPC(d_x.grow(3 * (size_t)npts));
-    HIPC(hipMemcpyAsync(d_x.p, x, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(d_x.p, xs.data(), sizeof(double) * 3 * npts, hipMemcpyHostToDevice, h->stream));
+    HIPC(d_pout.grow(npts));
+    HIPC(hipMemcpyAsync(d_pout.p, pout.data(), sizeof(int32_t) * npts, hipMemcpyHostToDevice, h->stream));
     const double* dens = h->d_alpha.p;
     if (density) {
         HIPC(d_dens.grow((size_t)3 * N * nT));
@@ -1387,13 +1422,16 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     }
     // the density triangle-major: the MOT's copy, or a transpose of the one passed in
     const double* aT = h->d_alphaT.p;
+    const double* dT = h->d_dalphaT.p;
     if (density) {
-        HIPC(d_aT.grow((size_t)3 * N * nT));
-        int64_t tot = (int64_t)3 * N * nT;
+        const int64_t tot = (int64_t)3 * N * nT, row = (int64_t)3 * N * (nT + 1);
+        HIPC(d_aT.grow((size_t)2 * row));
+        HIPC(hipMemsetAsync(d_aT.p, 0, sizeof(double) * 2 * row, h->stream));
         hipLaunchKernelGGL(k_alpha_T, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, dens, N, nT,
-                           d_aT.p);
+                           d_aT.p, d_aT.p + row);
         HIPC(hipGetLastError());
         aT = d_aT.p;
+        dT = d_aT.p + row;
     }
     HIPC(d_u.grow((size_t)3 * npts * nt));
     HIPC(hipMemsetAsync(d_u.p, 0, sizeof(double) * 3 * npts * nt, h->stream));
@@ -1558,8 +1596,9 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
                 tppb.stop(h->stream);
                 ppb_ms += tppb.ms();
             }
-            hipLaunchKernelGGL(k_pp_conv, dim3(np), dim3(64 * kConvSlices), 0, h->stream, d_M.p, d_off.p, d_ms.p,
-                               d_me.p, aT, N, nT, p0, d_nh.p, d_kidx.p, (int)nh.size(), npts, d_u.p);
+            hipLaunchKernelGGL(k_pp_conv, dim3((np + kConvGroup - 1) / kConvGroup), dim3(64 * kConvSlices), 0,
+                               h->stream, d_M.p, d_off.p, d_ms.p, d_me.p, aT, dT, N, nT, p0, np, d_pout.p, d_nh.p,
+                               d_kidx.p, (int)nh.size(), npts, d_u.p);
             HIPC(hipGetLastError());
         }
         g0 = g1;
@@ -1959,6 +1998,7 @@ int ebem_destroy(ebem_handle* h) {
         h->d_beta.reset();
         h->d_alpha.reset();
         h->d_alphaT.reset();
+        h->d_dalphaT.reset();
         h->d_r.reset();
         h->d_A.reset();
         h->d_Ainv.reset();
