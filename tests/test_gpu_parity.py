@@ -167,6 +167,26 @@ def test_pp_vs_oracle(screen2, sphere2):
         assert frob_rel(u, ref) < max(1e-10, 3 * noise), (frob_rel(u, ref), noise)
 
 
+def test_pp_point_order_does_not_matter(sphere2):
+    """The post-processing takes the points in its own (Morton) order and
+    pairs neighbours in the convolution; each point's sum is its own, so any
+    order of the caller's points gives the same field bit for bit, written
+    back to the caller's columns (an odd count leaves one point alone)."""
+    rng = np.random.default_rng(11)
+    Tfin, nT = 4.0, 32
+    s = _solver(sphere2, Tfin, nT)
+    dens = rng.normal(size=(3 * sphere2.numberTriangles, nT))
+    v = rng.normal(size=(37, 3))
+    pts = v / np.linalg.norm(v, axis=1)[:, None] * rng.uniform(1.1, 2.0, size=(37, 1))
+    times = Tfin / nT * np.arange(1, nT + 1)
+    u = s.pp_eval(pts, times, dens)
+    perm = rng.permutation(pts.shape[0])
+    up = s.pp_eval(pts[perm], times, dens)
+    assert np.array_equal(up, u[:, perm, :])
+    assert np.abs(u).max() > 0
+    s.close()
+
+
 def test_pp_after_mot_uses_device_density(screen2):
     s = _solver(screen2, 1.0, 16)
     s.assemble()
