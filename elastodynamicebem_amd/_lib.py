@@ -52,6 +52,7 @@ SIGNATURES = {
     "ebem_mot_solve_neumann": (C.c_int, [_vp, C.c_int, _dp]),
     "ebem_mot_solve": (C.c_int, [_vp, _dp, _dp]),
     "ebem_pp_eval": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
+    "ebem_pp_prepare": (C.c_int, [_vp, C.c_int, _dp, C.c_int, _dp]),
     "ebem_node_integral": (C.c_int, [_vp, _dp, C.c_int, C.c_double, _dp]),
     "ebem_set_pp_rule": (C.c_int, [_vp, C.c_int, C.c_int]),
     "ebem_pp_eval_numeric": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),

@@ -378,6 +378,13 @@ class Solver:
                                          _lib.dptr(x), t.size, _lib.dptr(t), _lib.dptr(out)))
         return out.transpose(2, 1, 0).copy()
 
+    def pp_prepare(self, x: np.ndarray, t: np.ndarray) -> None:
+        """Start the density-independent part of pp_eval(x, t) on the library's
+        side stream (ebem_pp_prepare), to overlap the next mot_solve."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(t, np.float64).ravel()
+        _lib.check(self.lib.ebem_pp_prepare(self.h, x.shape[0], _lib.dptr(x), t.size, _lib.dptr(t)))
+
     def set_pp_rule(self, nSub: int, nSing: int) -> None:
         """Rule of the numeric post-processing (pp_eval_numeric)."""
         _lib.check(self.lib.ebem_set_pp_rule(self.h, nSub, nSing))
@@ -420,7 +427,7 @@ class Solver:
         keys = ["levels_ms", "assembly_ms", "factor_ms", "history_ms", "apply_ms", "mot_ms", "pp_ms",
                 "history_launches", "levels_kp_ms", "pp_kp_ms", "levels_bf_ms", "pp_bf_ms", "k_assembly_ms",
                 "beta_k_ms", "pp_direct_ms", "mot_neumann_ms", "node_tasks", "node_tasks_reused",
-                "node_reuse_fallbacks"]
+                "node_reuse_fallbacks", "pp_prepare_ms"]
         out = np.zeros(len(keys))
         _lib.check(self.lib.ebem_timings(self.h, _lib.dptr(out), len(keys)))
         return dict(zip(keys, out.tolist()))

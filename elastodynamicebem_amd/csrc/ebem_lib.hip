@@ -15,7 +15,9 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -441,6 +443,25 @@ struct ebem_handle {
     bool have_beta = false, have_alpha = false;
     bool e0_pivoted = false;  // the last factorisation needed partial pivoting
     double tim[EBEM_NUM_TIMINGS] = {};
+    // ebem_pp_prepare: the density-independent part of one post-processing
+    // request, run by a worker thread on a stream of its own so that it
+    // overlaps whatever the caller runs next on the library's stream
+    struct PPPrep {
+        std::thread worker;
+        bool active = false;  // worker started and not yet joined
+        bool ready = false;   // the pw buffers hold the prepared part of (x, t)
+        int err = 0;
+        std::string msg;
+        std::vector<double> x, t;
+        hipStream_t stream = nullptr;
+        double ms = 0;  // span of the prepared part on its stream
+        // the worker starts its kernels once E_0 is factorised (the
+        // factorisation's many small rocSOLVER kernels slow down 4x beside
+        // them) or when a call joins it
+        std::mutex go_m;
+        std::condition_variable go_cv;
+        bool go = false;
+    } prep;
     // scratch of ebem_assemble and ebem_pp_eval, kept across calls (grow-only)
     struct {
         DevBuf<uint8_t> flag, key;
@@ -458,6 +479,7 @@ struct ebem_handle {
         DevBuf<int64_t> off, rb, cnt;
         DevBuf<uint8_t> key;
         DevBuf<unsigned long long> bo;
+        int64_t nitems = 0;  // items of the last chunk
     } pw;
     std::vector<hipEvent_t> mot_ev;  // per-step events of ebem_mot_solve
     // double-layer K of the direct Dirichlet driver (ebem_assemble_k):
@@ -1273,6 +1295,8 @@ int factor_e0(ebem_handle* h) {
     return fail(EBEM_ESINGULAR, "E_0 factorisation failed");
 }
 
+void pp_release(ebem_handle* h);
+
 int mot(ebem_handle* h, const double* beta, double* alpha_out) {
     if (!h->assembled) return fail(EBEM_EINVAL, "ebem_assemble must run before ebem_mot_solve");
     if ((h->precision & EBEM_PREC_SURROGATE) && !h->allow_sur_solve)
@@ -1300,7 +1324,8 @@ int mot(ebem_handle* h, const double* beta, double* alpha_out) {
     tf.start(h->stream);
     if (int e = factor_e0(h)) return e;
     tf.stop(h->stream);
-    h->tim[2] = tf.ms();
+    h->tim[2] = tf.ms();  // waits for the factorisation
+    if (h->prep.active) pp_release(h);  // a prepared post-processing may start beside the marching
     Store S = h->store();
     const int L = h->store_L;
     HIPC(h->d_part.alloc((size_t)kHistBlock * n3));
@@ -1355,15 +1380,32 @@ int mot(ebem_handle* h, const double* beta, double* alpha_out) {
     return 0;
 }
 
+// Points per chunk of the post-processing: as many as keep every chunk's
+// (point, triangle, m) items below 2^31 even if every window held all nT + 1
+// levels (at least 1024); fewer chunks, fewer launches and host round trips.
+int pp_chunk_points(const ebem_handle* h, int npts) {
+    const int64_t pmax = (int64_t)INT32_MAX / ((int64_t)h->N * (h->nT + 1));
+    return (int)std::min<int64_t>(npts, std::max<int64_t>(1024, pmax));
+}
+
 // Post-processing: times grouped by delta = t - nhat dt (shared M_m), points
-// chunked so that the M scratch stays within budget.
+// chunked so that the M scratch stays within budget.  stage 0: the whole
+// evaluation on the library's stream.  stage 1 (ebem_pp_prepare, a request of
+// one time group and one chunk of points, analytic form): the part that does
+// not depend on the density -- points, windows, items, keys, the M integrals
+// -- on stream st, complete when it returns; u_out and density are not used.
+// stage 2: the rest of such a request after stage 1 (density copies and the
+// convolution) on the library's stream.
 int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, int nt, const double* t,
-            double* u_out, bool numeric = false) {
+            double* u_out, bool numeric = false, int stage = 0, hipStream_t st = nullptr) {
 #pragma clang fp contract(off)
     if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
-    if (npts <= 0 || nt <= 0 || !x || !t || !u_out) return fail(EBEM_EINVAL, "bad post-processing arguments");
-    if (!density && !h->have_alpha) return fail(EBEM_EINVAL, "no density: pass one or run ebem_mot_solve first");
+    if (npts <= 0 || nt <= 0 || !x || !t || (!u_out && stage != 1))
+        return fail(EBEM_EINVAL, "bad post-processing arguments");
+    if (stage != 1 && !density && !h->have_alpha)
+        return fail(EBEM_EINVAL, "no density: pass one or run ebem_mot_solve first");
     if (numeric && !h->pp_nInt) return fail(EBEM_EINVAL, "numeric post-processing needs ebem_set_pp_rule first");
+    hipStream_t s0 = stage == 1 ? st : h->stream;
     const int N = h->N, nT = h->nT;
     const double dt = h->dt;
     std::vector<int> nhat(nt);
@@ -1380,13 +1422,14 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     }
     Timer tall, tppk, tppb;
     double ppk_ms = 0, ppb_ms = 0;
-    tall.start(h->stream);
+    tall.start(s0);
     auto &d_x = h->pw.x, &d_aT = h->pw.aT, &d_u = h->pw.u, &d_dens = h->pw.dens, &d_M = h->pw.M;
     auto& d_pout = h->pw.pout;
     // points in the order of a Morton curve over their bounding box, so that
     // the point groups of k_pp_conv are neighbours; pout maps back
     std::vector<int32_t> pout(npts);
-    {
+    std::vector<double> xs((size_t)3 * npts);
+    if (stage != 2) {
         double lo[3], hi[3];
         for (int c = 0; c < 3; c++) lo[c] = hi[c] = x[c];
         for (int p = 0; p < npts; p++)
@@ -1406,16 +1449,15 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             pout[p] = p;
         }
         std::stable_sort(pout.begin(), pout.end(), [&](int32_t a, int32_t b) { return code[a] < code[b]; });
+        for (int r = 0; r < npts; r++)
+            for (int c = 0; c < 3; c++) xs[3 * (size_t)r + c] = x[3 * (size_t)pout[r] + c];
+        HIPC(d_x.grow(3 * (size_t)npts));
+        HIPC(hipMemcpyAsync(d_x.p, xs.data(), sizeof(double) * 3 * npts, hipMemcpyHostToDevice, s0));
+        HIPC(d_pout.grow(npts));
+        HIPC(hipMemcpyAsync(d_pout.p, pout.data(), sizeof(int32_t) * npts, hipMemcpyHostToDevice, s0));
     }
-    std::vector<double> xs((size_t)3 * npts);
-    for (int r = 0; r < npts; r++)
-        for (int c = 0; c < 3; c++) xs[3 * (size_t)r + c] = x[3 * (size_t)pout[r] + c];
-    HIPC(d_x.grow(3 * (size_t)npts));
-    HIPC(hipMemcpyAsync(d_x.p, xs.data(), sizeof(double) * 3 * npts, hipMemcpyHostToDevice, h->stream));
-    HIPC(d_pout.grow(npts));
-    HIPC(hipMemcpyAsync(d_pout.p, pout.data(), sizeof(int32_t) * npts, hipMemcpyHostToDevice, h->stream));
     const double* dens = h->d_alpha.p;
-    if (density) {
+    if (density && stage != 1) {
         HIPC(d_dens.grow((size_t)3 * N * nT));
         HIPC(hipMemcpyAsync(d_dens.p, density, sizeof(double) * 3 * N * nT, hipMemcpyHostToDevice, h->stream));
         dens = d_dens.p;
@@ -1423,7 +1465,7 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     // the density triangle-major: the MOT's copy, or a transpose of the one passed in
     const double* aT = h->d_alphaT.p;
     const double* dT = h->d_dalphaT.p;
-    if (density) {
+    if (density && stage != 1) {
         const int64_t tot = (int64_t)3 * N * nT, row = (int64_t)3 * N * (nT + 1);
         HIPC(d_aT.grow((size_t)2 * row));
         HIPC(hipMemsetAsync(d_aT.p, 0, sizeof(double) * 2 * row, h->stream));
@@ -1433,18 +1475,16 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
         aT = d_aT.p;
         dT = d_aT.p + row;
     }
-    HIPC(d_u.grow((size_t)3 * npts * nt));
-    HIPC(hipMemsetAsync(d_u.p, 0, sizeof(double) * 3 * npts * nt, h->stream));
+    if (stage != 1) {
+        HIPC(d_u.grow((size_t)3 * npts * nt));
+        HIPC(hipMemsetAsync(d_u.p, 0, sizeof(double) * 3 * npts * nt, h->stream));
+    }
     // group times by delta
     std::vector<int> order;
     for (int k = 0; k < nt; k++)
         if (nhat[k] >= 0) order.push_back(k);
     std::sort(order.begin(), order.end(), [&](int a, int b) { return delta[a] < delta[b]; });
-    // points per chunk: as many as keep every chunk's (point, triangle, m)
-    // items below 2^31 even if every window held all nT + 1 levels (at least
-    // 1024); fewer chunks, fewer launches and host round trips
-    const int64_t pmax = (int64_t)INT32_MAX / ((int64_t)N * (nT + 1));
-    const int P0 = (int)std::min<int64_t>(npts, std::max<int64_t>(1024, pmax));
+    const int P0 = pp_chunk_points(h, npts);
     const int G = h->CUs * 8;  // counting-sort blocks
     // kP ranges of m through k_pp_M_kp (analytic form with key-sorted items)
     const bool kp = !numeric && h->sort_items && h->kp_split;
@@ -1452,7 +1492,7 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
     auto &d_perm = h->pw.perm, &d_ka = h->pw.ka, &d_kb = h->pw.kb, &d_sa = h->pw.sa, &d_sb = h->pw.sb;
     auto& d_nfb = h->pw.nfb;
     HIPC(d_nfb.grow(1));
-    HIPC(hipMemsetAsync(d_nfb.p, 0, sizeof(unsigned int), h->stream));
+    if (stage != 2) HIPC(hipMemsetAsync(d_nfb.p, 0, sizeof(unsigned int), s0));
     auto &d_off = h->pw.off, &d_rb = h->pw.rb, &d_cnt = h->pw.cnt;
     auto& d_key = h->pw.key;
     auto& d_bh = h->pw.bh;
@@ -1465,7 +1505,8 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
         HIPC(d_sa.grow((size_t)P0 * N));
         HIPC(d_sb.grow((size_t)P0 * N));
     }
-    for (int k = 0; k < 5; k++) h->pp_kp_stats[k] = 0;
+    if (stage != 2)
+        for (int k = 0; k < 5; k++) h->pp_kp_stats[k] = 0;
     HIPC(d_off.grow((size_t)P0 * N));
     HIPC(d_rb.grow(P0));
     HIPC(d_cnt.grow(P0));
@@ -1503,54 +1544,67 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
         P.trep = t[krep];
         P.nrep = mmax;
         P.kp = kp ? 1 : 0;
-        HIPC(d_nh.grow(nh.size()));
-        HIPC(d_kidx.grow(kid.size()));
-        HIPC(hipMemcpyAsync(d_nh.p, nh.data(), sizeof(int32_t) * nh.size(), hipMemcpyHostToDevice, h->stream));
-        HIPC(hipMemcpyAsync(d_kidx.p, kid.data(), sizeof(int32_t) * kid.size(), hipMemcpyHostToDevice, h->stream));
+        if (stage == 1 && (g1 != order.size() || P0 < npts || numeric))
+            return fail(EBEM_EINVAL, "a prepared post-processing request must be one time group and one chunk");
+        if (stage != 2) {
+            HIPC(d_nh.grow(nh.size()));
+            HIPC(d_kidx.grow(kid.size()));
+            HIPC(hipMemcpyAsync(d_nh.p, nh.data(), sizeof(int32_t) * nh.size(), hipMemcpyHostToDevice, s0));
+            HIPC(hipMemcpyAsync(d_kidx.p, kid.data(), sizeof(int32_t) * kid.size(), hipMemcpyHostToDevice, s0));
+        }
         for (int p0 = 0; p0 < npts; p0 += P0) {
             const int np = std::min(P0, npts - p0);
+            if (stage == 2) {  // the prepared items and M of this (only) chunk
+                if (h->pw.nitems > 0)
+                    hipLaunchKernelGGL(k_pp_conv, dim3((np + kConvGroup - 1) / kConvGroup), dim3(64 * kConvSlices), 0,
+                                       h->stream, d_M.p, h->pw.off.p, h->pw.ms.p, h->pw.me.p, aT, dT, N, nT, p0, np,
+                                       d_pout.p, d_nh.p, d_kidx.p, (int)nh.size(), npts, d_u.p);
+                HIPC(hipGetLastError());
+                continue;
+            }
             int64_t tot = (int64_t)np * N;
             int64_t blocks = std::min<int64_t>((tot + 255) / 256, (int64_t)h->CUs * 64);
-            hipLaunchKernelGGL(k_pp_windows, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p,
+            hipLaunchKernelGGL(k_pp_windows, dim3((unsigned)blocks), dim3(256), 0, s0, h->d_tri.p,
                                h->d_verts.p, d_x.p, p0, np, P, d_ms.p, d_me.p, d_ka.p, d_kb.p, d_sa.p, d_sb.p);
             HIPC(hipGetLastError());
-            hipLaunchKernelGGL(k_pp_counts, dim3(np), dim3(256), 0, h->stream, d_ms.p, d_me.p, N, d_cnt.p);
+            hipLaunchKernelGGL(k_pp_counts, dim3(np), dim3(256), 0, s0, d_ms.p, d_me.p, N, d_cnt.p);
             HIPC(hipGetLastError());
-            HIPC(hipMemcpyAsync(cnt.data(), d_cnt.p, sizeof(int64_t) * np, hipMemcpyDeviceToHost, h->stream));
-            HIPC(hipStreamSynchronize(h->stream));
+            HIPC(hipMemcpyAsync(cnt.data(), d_cnt.p, sizeof(int64_t) * np, hipMemcpyDeviceToHost, s0));
+            HIPC(hipStreamSynchronize(s0));
             int64_t run = 0;
             for (int r = 0; r < np; r++) {
                 rb[r] = run;
                 run += cnt[r];
             }
             const int64_t nitems = run;
+            h->pw.nitems = nitems;
             if (nitems == 0) continue;
             if (nitems > INT32_MAX) return fail(EBEM_EINVAL, "too many post-processing items in one chunk");
-            HIPC(hipMemcpyAsync(d_rb.p, rb.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice, h->stream));
+            HIPC(hipMemcpyAsync(d_rb.p, rb.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice, s0));
             HIPC(d_iq.grow((size_t)nitems));
             HIPC(d_M.grow((size_t)nitems * 6));
             HIPC(d_key.grow((size_t)nitems));
             HIPC(d_perm.grow((size_t)nitems));
-            hipLaunchKernelGGL(k_pp_items, dim3(np), dim3(256), 0, h->stream, d_ms.p, d_me.p, N, d_rb.p, d_off.p,
+            hipLaunchKernelGGL(k_pp_items, dim3(np), dim3(256), 0, s0, d_ms.p, d_me.p, N, d_rb.p, d_off.p,
                                d_iq.p);
             HIPC(hipGetLastError());
             // bucket items by front-state key (two-pass counting sort)
             const int64_t per = (nitems + G - 1) / G;
-            hipLaunchKernelGGL(k_pp_keys, dim3(G), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P, nitems, per,
+            hipLaunchKernelGGL(k_pp_keys, dim3(G), dim3(256), 0, s0, h->d_tri.p, d_x.p, p0, P, nitems, per,
                                d_iq.p, d_off.p, d_ms.p, kp ? d_ka.p : nullptr, kp ? d_kb.p : nullptr,
                                kp ? d_sa.p : nullptr, kp ? d_sb.p : nullptr, d_key.p, d_bh.p);
             HIPC(hipGetLastError());
             HIPC(hipMemcpyAsync(bh.data(), d_bh.p, sizeof(unsigned int) * bh.size(), hipMemcpyDeviceToHost,
-                                h->stream));
-            HIPC(hipStreamSynchronize(h->stream));
+                                s0));
+            HIPC(hipStreamSynchronize(s0));
             unsigned long long runk = 0;
             for (size_t k = 0; k < bh.size(); k++) {
                 bo[k] = runk;
                 runk += bh[k];
             }
             HIPC(hipMemcpyAsync(d_bo.p, bo.data(), sizeof(unsigned long long) * bo.size(), hipMemcpyHostToDevice,
-                                h->stream));
-            hipLaunchKernelGGL(k_item_scatter, dim3(G), dim3(256), 0, h->stream, nitems, per, d_key.p, d_bo.p,
+                                s0));
+            hipLaunchKernelGGL(k_item_scatter, dim3(G), dim3(256), 0, s0, nitems, per, d_key.p, d_bo.p,
                                d_perm.p);
             HIPC(hipGetLastError());
             // perm: [per-m items | kP range heads | other kP items |
@@ -1571,47 +1625,104 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             int64_t mb = std::min<int64_t>(std::max<int64_t>(1, (nper + 255) / 256), (int64_t)h->CUs * 16);
             if (numeric)
                 hipLaunchKernelGGL(k_pp_M_num, dim3((unsigned)mb), dim3(256), sizeof(double) * 4 * h->pp_nInt,
-                                   h->stream, h->d_tri.p, h->d_verts.p, d_x.p, p0, P, h->d_pp_rule.p, h->pp_nInt,
+                                   s0, h->d_tri.p, h->d_verts.p, d_x.p, p0, P, h->d_pp_rule.p, h->pp_nInt,
                                    nitems, d_iq.p, d_off.p, d_ms.p, 4 * kPi * h->rho, d_M.p);
             else
-                hipLaunchKernelGGL(k_pp_M, dim3((unsigned)mb), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P,
+                hipLaunchKernelGGL(k_pp_M, dim3((unsigned)mb), dim3(256), 0, s0, h->d_tri.p, d_x.p, p0, P,
                                    nper, d_iq.p, d_off.p, d_ms.p, h->sort_items ? d_perm.p : nullptr, d_M.p);
             HIPC(hipGetLastError());
             if (nkp > 0) {
-                tppk.start(h->stream);
+                tppk.start(s0);
                 const int64_t kblocks = std::min<int64_t>((nkp + 255) / 256, (int64_t)h->CUs * 16);
-                hipLaunchKernelGGL(k_pp_M_kp, dim3((unsigned)kblocks), dim3(256), 0, h->stream, h->d_tri.p, d_x.p, p0, P,
+                hipLaunchKernelGGL(k_pp_M_kp, dim3((unsigned)kblocks), dim3(256), 0, s0, h->d_tri.p, d_x.p, p0, P,
                                    nkp, d_iq.p, d_ka.p, d_kb.p, d_perm.p + nper, d_M.p, d_nfb.p);
                 HIPC(hipGetLastError());
-                tppk.stop(h->stream);
+                tppk.stop(s0);
                 ppk_ms += tppk.ms();
             }
             if (nbf > 0) {
-                tppb.start(h->stream);
+                tppb.start(s0);
                 const int64_t bblocks = std::min<int64_t>((nbf + 255) / 256, (int64_t)h->CUs * 16);
-                hipLaunchKernelGGL(k_pp_M_bf, dim3((unsigned)bblocks), dim3(256), 0, h->stream, h->d_tri.p, d_x.p,
+                hipLaunchKernelGGL(k_pp_M_bf, dim3((unsigned)bblocks), dim3(256), 0, s0, h->d_tri.p, d_x.p,
                                    p0, P, nbf, d_iq.p, d_sa.p, d_sb.p, d_perm.p + nper + nkp + nkrest, d_M.p,
                                    d_nfb.p);
                 HIPC(hipGetLastError());
-                tppb.stop(h->stream);
+                tppb.stop(s0);
                 ppb_ms += tppb.ms();
             }
+            if (stage == 1) continue;
             hipLaunchKernelGGL(k_pp_conv, dim3((np + kConvGroup - 1) / kConvGroup), dim3(64 * kConvSlices), 0,
-                               h->stream, d_M.p, d_off.p, d_ms.p, d_me.p, aT, dT, N, nT, p0, np, d_pout.p, d_nh.p,
+                               s0, d_M.p, d_off.p, d_ms.p, d_me.p, aT, dT, N, nT, p0, np, d_pout.p, d_nh.p,
                                d_kidx.p, (int)nh.size(), npts, d_u.p);
             HIPC(hipGetLastError());
         }
         g0 = g1;
     }
-    tall.stop(h->stream);
+    tall.stop(s0);
+    if (stage == 1) {  // complete on return: the caller joins and stage 2 runs on another stream
+        HIPC(hipStreamSynchronize(s0));
+        h->prep.ms = tall.ms();
+        h->tim[9] = ppk_ms;
+        h->tim[11] = ppb_ms;
+        return 0;
+    }
     HIPC(hipMemcpyAsync(u_out, d_u.p, sizeof(double) * 3 * npts * nt, hipMemcpyDeviceToHost, h->stream));
     unsigned int nfb = 0;
     HIPC(hipMemcpyAsync(&nfb, d_nfb.p, sizeof nfb, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
     h->pp_kp_stats[4] = nfb;
     h->tim[6] = tall.ms();
-    h->tim[9] = ppk_ms;
-    h->tim[11] = ppb_ms;
+    if (stage == 0) {
+        h->tim[9] = ppk_ms;
+        h->tim[11] = ppb_ms;
+        h->tim[19] = 0;
+    } else {
+        h->tim[19] = h->prep.ms;
+    }
+    return 0;
+}
+
+// Whether a post-processing request can be prepared by ebem_pp_prepare: the
+// analytic form, one group of times (one delta = t - nhat dt) and one chunk
+// of points.
+bool pp_preparable(const ebem_handle* h, int npts, int nt, const double* t) {
+    if (npts <= 0 || nt <= 0 || pp_chunk_points(h, npts) < npts) return false;
+    double dmin = 0, dmax = 0;
+    int nv = 0;
+    for (int k = 0; k < nt; k++) {
+        if (!(t[k] > 0)) continue;
+        const int nh = (int)std::ceil(t[k] / h->dt) - 1;
+        if (nh > h->nT - 1) return false;
+        const double d = t[k] - nh * h->dt;
+        dmin = nv ? std::min(dmin, d) : d;
+        dmax = nv ? std::max(dmax, d) : d;
+        nv++;
+    }
+    return nv > 0 && dmax - dmin <= 1e-12 * h->dt;
+}
+
+// Waits for the worker of ebem_pp_prepare, if one runs, and reports its
+// error; keep = false also drops the prepared request (state it depends on
+// may change, or its buffers may be reused).
+void pp_release(ebem_handle* h) {
+    std::lock_guard<std::mutex> g(h->prep.go_m);
+    h->prep.go = true;
+    h->prep.go_cv.notify_all();
+}
+
+int pp_join(ebem_handle* h, bool keep = false) {
+    if (h->prep.active) {
+        pp_release(h);
+        h->prep.worker.join();
+        h->prep.active = false;
+    }
+    if (!keep) h->prep.ready = false;
+    if (h->prep.err) {
+        const int e = h->prep.err;
+        h->prep.err = 0;
+        h->prep.ready = false;
+        return fail(e, "%s", h->prep.msg.c_str());
+    }
     return 0;
 }
 
@@ -1982,8 +2093,10 @@ int ebem_destroy(ebem_handle* h) {
     if (!h) return 0;
     {
         std::lock_guard<std::mutex> lk(h->lock);
+        (void)pp_join(h);
         (void)hipSetDevice(h->device);
         (void)hipStreamSynchronize(h->stream);
+        if (h->prep.stream) (void)hipStreamDestroy(h->prep.stream);
         if (h->blas) rocblas_destroy_handle(h->blas);
         if (h->sur) {
             std::lock_guard<std::mutex> lk2(h->sur->lock);
@@ -2016,6 +2129,7 @@ int ebem_destroy(ebem_handle* h) {
 int ebem_set_mesh(ebem_handle* h, int N, int Nv, const double* coords, const int32_t* tri, int one_based) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (N <= 0 || Nv <= 0 || !coords || !tri) return fail(EBEM_EINVAL, "invalid mesh (N=%d, Nv=%d)", N, Nv);
     if ((int64_t)N * N > (int64_t)1 << 31) return fail(EBEM_EINVAL, "mesh too large (N=%d)", N);
     HIPC(hipSetDevice(h->device));
@@ -2056,6 +2170,7 @@ int ebem_set_mesh(ebem_handle* h, int N, int Nv, const double* coords, const int
 int ebem_set_problem(ebem_handle* h, double rho, double mu, double lambda, double Tfin, int nT) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (!(rho > 0) || !(mu > 0) || !(lambda + 2 * mu > 0) || !(Tfin > 0) || nT <= 0)
         return fail(EBEM_EINVAL, "invalid problem parameters");
     h->rho = rho;
@@ -2077,6 +2192,7 @@ int ebem_set_problem(ebem_handle* h, double rho, double mu, double lambda, doubl
 int ebem_set_rule(ebem_handle* h, int nGH) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (nGH != 3 && nGH != 7 && nGH != 12 && nGH != 19)
         return fail(EBEM_EINVAL, "Valori nodi esterni non validi: %d (allowed 3, 7, 12, 19)", nGH);
     HIPC(hipSetDevice(h->device));
@@ -2090,6 +2206,7 @@ int ebem_set_rule(ebem_handle* h, int nGH) {
 int ebem_set_inner_rule(ebem_handle* h, int nSub, int nSing, int nDiag) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     // BEMenerg_setupCore.m:58-93 (method families GPU / FN)
     if (!(nSub == 1 || nSub == 4 || nSub == 16 || nSub == 64) ||
         !(nSing == 1 || nSing == 3 || nSing == 7 || nSing == 12 || nSing == 19))
@@ -2123,6 +2240,7 @@ int ebem_inner_rule_tables(int nSub, int nSing, int nDiag, double* inner, double
 int ebem_num_blocks(ebem_handle* h, int* L) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     if (!h->have_mesh || !h->have_problem || !L) return fail(EBEM_EINVAL, "mesh and problem must be set");
     compute_L(h);
     *L = h->L;
@@ -2132,6 +2250,7 @@ int ebem_num_blocks(ebem_handle* h, int* L) {
 int ebem_assemble(ebem_handle* h, int precision) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     HIPC(hipSetDevice(h->device));
     return assemble(h, precision);
 }
@@ -2139,6 +2258,7 @@ int ebem_assemble(ebem_handle* h, int precision) {
 int ebem_store_info(ebem_handle* h, int64_t* nnzb, int64_t* pair_lags, int64_t* items, int64_t* bytes) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
     if (nnzb) *nnzb = h->nnzb;
     if (pair_lags) *pair_lags = h->pair_lags;
@@ -2150,6 +2270,7 @@ int ebem_store_info(ebem_handle* h, int64_t* nnzb, int64_t* pair_lags, int64_t* 
 int ebem_store_lag_blocks(ebem_handle* h, int64_t* counts) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
     if (!counts) return fail(EBEM_EINVAL, "null output");
     const int L = h->store_L;
@@ -2162,6 +2283,7 @@ int ebem_store_lag_blocks(ebem_handle* h, int64_t* counts) {
 int ebem_assembly_kp_stats(ebem_handle* h, int64_t* out5) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
     if (!out5) return fail(EBEM_EINVAL, "null output");
     for (int k = 0; k < 5; k++) out5[k] = h->kp_stats[k];
@@ -2171,6 +2293,7 @@ int ebem_assembly_kp_stats(ebem_handle* h, int64_t* out5) {
 int ebem_pp_kp_stats(ebem_handle* h, int64_t* out5) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     if (!out5) return fail(EBEM_EINVAL, "null output");
     for (int k = 0; k < 5; k++) out5[k] = h->pp_kp_stats[k];
     return 0;
@@ -2179,6 +2302,7 @@ int ebem_pp_kp_stats(ebem_handle* h, int64_t* out5) {
 int ebem_assembly_stats(ebem_handle* h, int64_t* far_items, int64_t* fp64_items) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
     if (far_items) *far_items = h->far_items;
     if (fp64_items) *fp64_items = h->items - h->far_items - h->sur_items;
@@ -2188,6 +2312,7 @@ int ebem_assembly_stats(ebem_handle* h, int64_t* far_items, int64_t* fp64_items)
 int ebem_get_block_rows(ebem_handle* h, int lag, int r0, int r1, double* out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
     if (!out) return fail(EBEM_EINVAL, "null output");
     if (lag < 0) return fail(EBEM_EINVAL, "negative lag");
@@ -2214,6 +2339,7 @@ int ebem_get_block(ebem_handle* h, int lag, double* out) {
     {
         if (int e = check_handle(h)) return e;
         std::lock_guard<std::mutex> lk(h->lock);
+        if (int e = pp_join(h, true)) return e;
         N = h->N;
     }
     return ebem_get_block_rows(h, lag, 0, N, out);
@@ -2241,6 +2367,7 @@ int ebem_rhs(ebem_handle* h, int datum_kind, double* beta_out) {
 int ebem_k_num_blocks(ebem_handle* h, int* LK) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
     if (!LK) return fail(EBEM_EINVAL, "bad argument");
     *LK = k_num_blocks(h);
@@ -2250,6 +2377,7 @@ int ebem_k_num_blocks(ebem_handle* h, int* LK) {
 int ebem_assemble_k(ebem_handle* h) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     HIPC(hipSetDevice(h->device));
     return assemble_k(h);
 }
@@ -2257,6 +2385,7 @@ int ebem_assemble_k(ebem_handle* h) {
 int ebem_k_store_info(ebem_handle* h, int64_t* pairs, int64_t* slots, int64_t* bytes) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     if (!h->k_ready) return fail(EBEM_EINVAL, "ebem_assemble_k has not run");
     if (pairs) *pairs = h->k_pairs;
     if (slots) *slots = h->k_slots;
@@ -2267,6 +2396,7 @@ int ebem_k_store_info(ebem_handle* h, int64_t* pairs, int64_t* slots, int64_t* b
 int ebem_get_k_block_rows(ebem_handle* h, int lag, int r0, int r1, double* out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     if (!h->k_ready) return fail(EBEM_EINVAL, "ebem_assemble_k has not run");
     if (!out || r0 < 0 || r1 > h->N || r0 >= r1) return fail(EBEM_EINVAL, "bad argument");
     if (lag < 0 || lag >= h->nT) return fail(EBEM_EINVAL, "lag %d out of range [0, %d)", lag, h->nT);
@@ -2300,6 +2430,7 @@ int ebem_get_k_block_rows(ebem_handle* h, int lag, int r0, int r1, double* out) 
 int ebem_dirichlet_datum(ebem_handle* h, int kind, double* g_out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
     if (kind < EBEM_DATUM_REF_BAR_H1 || kind > EBEM_DATUM_MAX)
         return fail(EBEM_EINVAL, "Problema non codificato (datum %d has no Dirichlet trace)", kind);
@@ -2320,6 +2451,7 @@ int ebem_dirichlet_datum(ebem_handle* h, int kind, double* g_out) {
 int ebem_beta_k(ebem_handle* h, const double* g, double* beta_k_out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (!h->k_ready) return fail(EBEM_EINVAL, "ebem_assemble_k has not run");
     if (!g || !beta_k_out) return fail(EBEM_EINVAL, "bad argument");
     HIPC(hipSetDevice(h->device));
@@ -2342,6 +2474,7 @@ int ebem_beta_k(ebem_handle* h, const double* g, double* beta_k_out) {
 int ebem_rhs_dirichlet(ebem_handle* h, int kind, double* beta_out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (!h->k_ready) return fail(EBEM_EINVAL, "ebem_assemble_k has not run");
     if (kind < EBEM_DATUM_REF_BAR_H1 || kind > EBEM_DATUM_MAX)
         return fail(EBEM_EINVAL, "Problema non codificato (datum %d has no Dirichlet trace)", kind);
@@ -2370,6 +2503,7 @@ int ebem_pp_eval_direct(ebem_handle* h, const double* density, const double* g, 
                         const double* t, double* u_out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
     if (!h->pp_nInt) return fail(EBEM_EINVAL, "the direct post-processing needs ebem_set_pp_rule first");
     if (npts < 0 || nt < 0 || (npts > 0 && !x) || (nt > 0 && !t) || !u_out) return fail(EBEM_EINVAL, "bad argument");
@@ -2531,6 +2665,7 @@ int ebem_pp_eval_direct(ebem_handle* h, const double* density, const double* g, 
 int ebem_neumann_datum(ebem_handle* h, int datum, double* gV_out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
     if (datum != EBEM_DATUM_REF_BAR_H1 && datum != EBEM_DATUM_REF_BAR_H3)
         return fail(EBEM_EINVAL, "Problema non codificato (datum %d has no Neumann trace)", datum);
@@ -2551,6 +2686,7 @@ int ebem_neumann_datum(ebem_handle* h, int datum, double* gV_out) {
 int ebem_beta_v(ebem_handle* h, const double* gV, double* beta_v_out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (!h->assembled) return fail(EBEM_EINVAL, "ebem_assemble must run before ebem_beta_v");
     if (!gV || !beta_v_out) return fail(EBEM_EINVAL, "bad argument");
     HIPC(hipSetDevice(h->device));
@@ -2571,6 +2707,7 @@ int ebem_beta_v(ebem_handle* h, const double* gV, double* beta_v_out) {
 int ebem_mot_solve_neumann(ebem_handle* h, int datum, double* density_out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (!h->assembled) return fail(EBEM_EINVAL, "ebem_assemble must run before ebem_mot_solve_neumann");
     if (!h->k_ready) return fail(EBEM_EINVAL, "ebem_assemble_k must run before ebem_mot_solve_neumann");
     if (datum != EBEM_DATUM_REF_BAR_H1 && datum != EBEM_DATUM_REF_BAR_H3)
@@ -2595,6 +2732,7 @@ int ebem_mot_solve(ebem_handle* h, const double* beta, double* alpha_out) {
 int ebem_node_integral(ebem_handle* h, const double* x, int tri, double t, double* out9) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
     if (tri < 0 || tri >= h->N || !x || !out9) return fail(EBEM_EINVAL, "bad argument");
     HIPC(hipSetDevice(h->device));
@@ -2616,6 +2754,7 @@ int ebem_analitic_intg(ebem_handle* h, int n, const int32_t* kind, const double*
                        const double* B, double* ris) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (n < 0 || (n > 0 && (!kind || !args || !info || !B || !ris))) return fail(EBEM_EINVAL, "bad argument");
     if (n > (1 << 24)) return fail(EBEM_EINVAL, "too many region calls in one batch (%d > %d)", n, 1 << 24);
     for (int k = 0; k < n; k++)
@@ -2643,6 +2782,7 @@ int ebem_analitic_intg_noalloc(ebem_handle* h, int n, const int32_t* kind, const
                                const double* B, double* ris) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (n < 0 || (n > 0 && (!kind || !args || !info || !B || !ris))) return fail(EBEM_EINVAL, "bad argument");
     if (n > kAiScratchCalls)
         return fail(EBEM_EINVAL, "ebem_analitic_intg_noalloc takes at most %d calls per batch (got %d)",
@@ -2713,6 +2853,7 @@ int ebem_analiticIntg_TR_SP(ebem_handle* h, double zeta, double THETA1, double T
 int ebem_set_option(ebem_handle* h, int option, double value) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     switch (option) {
         case EBEM_OPT_FAR_FACTOR:
             if (!std::isfinite(value)) return fail(EBEM_EINVAL, "far-field factor must be finite");
@@ -2736,13 +2877,58 @@ int ebem_pp_eval(ebem_handle* h, const double* density, int npts, const double* 
                  double* u_out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     HIPC(hipSetDevice(h->device));
-    return pp_eval(h, density, npts, x, nt, t, u_out);
+    // the request ebem_pp_prepare prepared (same points and times, bit for
+    // bit): only its density-dependent rest
+    auto& P = h->prep;
+    const bool prepared = P.ready && x && t && npts > 0 && nt > 0 && P.x.size() == 3 * (size_t)npts &&
+                          P.t.size() == (size_t)nt &&
+                          std::memcmp(P.x.data(), x, sizeof(double) * 3 * (size_t)npts) == 0 &&
+                          std::memcmp(P.t.data(), t, sizeof(double) * (size_t)nt) == 0;
+    P.ready = false;
+    return pp_eval(h, density, npts, x, nt, t, u_out, false, prepared ? 2 : 0);
+}
+
+int ebem_pp_prepare(ebem_handle* h, int npts, const double* x, int nt, const double* t) {
+    if (int e = check_handle(h)) return e;
+    std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
+    if (!h->have_mesh || !h->have_problem) return fail(EBEM_EINVAL, "mesh and problem must be set");
+    if (npts <= 0 || nt <= 0 || !x || !t) return fail(EBEM_EINVAL, "bad post-processing arguments");
+    HIPC(hipSetDevice(h->device));
+    if (!pp_preparable(h, npts, nt, t)) return 0;  // ebem_pp_eval then does all of it
+    auto& P = h->prep;
+    if (!P.stream) HIPC(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+    P.x.assign(x, x + 3 * (size_t)npts);
+    P.t.assign(t, t + (size_t)nt);
+    P.err = 0;
+    P.go = false;
+    P.active = true;
+    P.worker = std::thread([h, npts, nt] {
+        auto& Q = h->prep;
+        {
+            std::unique_lock<std::mutex> g(Q.go_m);
+            Q.go_cv.wait(g, [&Q] { return Q.go; });
+        }
+        hipError_t e0 = hipSetDevice(h->device);
+        const int e = e0 != hipSuccess ? fail(EBEM_EHIP, "hipSetDevice failed: %s", hipGetErrorString(e0))
+                                       : pp_eval(h, nullptr, npts, Q.x.data(), nt, Q.t.data(), nullptr, false, 1,
+                                                 Q.stream);
+        if (e) {
+            Q.err = e;
+            Q.msg = g_err;
+        } else {
+            Q.ready = true;
+        }
+    });
+    return 0;
 }
 
 int ebem_set_pp_rule(ebem_handle* h, int nSub, int nSing) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (!(nSub == 1 || nSub == 4 || nSub == 16 || nSub == 64) ||
         !(nSing == 1 || nSing == 3 || nSing == 7 || nSing == 12 || nSing == 19))
         return fail(EBEM_EINVAL, "Valori nodi interni non validi (%d sub-regions x %d nodes)", nSub, nSing);
@@ -2760,6 +2946,7 @@ int ebem_pp_eval_numeric(ebem_handle* h, const double* density, int npts, const 
                          double* u_out) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     HIPC(hipSetDevice(h->device));
     return pp_eval(h, density, npts, x, nt, t, u_out, true);
 }
@@ -2767,6 +2954,7 @@ int ebem_pp_eval_numeric(ebem_handle* h, const double* density, int npts, const 
 int ebem_timings(ebem_handle* h, double* out, int n) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     if (!out || n < 0) return fail(EBEM_EINVAL, "null output or negative count");
     for (int k = 0; k < n && k < EBEM_NUM_TIMINGS; k++) out[k] = h->tim[k];
     return 0;
@@ -3129,6 +3317,7 @@ int ebem_sur_params(ebem_sur* s, int set, float* params, float* norm) {
 int ebem_attach_surrogate(ebem_handle* h, ebem_sur* s) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h)) return e;
     if (s && s->device != h->device) return fail(EBEM_EINVAL, "surrogate and handle live on different devices");
     if (h->sur) {
         std::lock_guard<std::mutex> lk2(h->sur->lock);
@@ -3145,6 +3334,7 @@ int ebem_attach_surrogate(ebem_handle* h, ebem_sur* s) {
 int ebem_surrogate_stats(ebem_handle* h, int64_t* items, int64_t* blocks, double* ms) {
     if (int e = check_handle(h)) return e;
     std::lock_guard<std::mutex> lk(h->lock);
+    if (int e = pp_join(h, true)) return e;
     if (!h->assembled) return fail(EBEM_EINVAL, "not assembled");
     if (items) *items = h->sur_items;
     if (blocks) *blocks = h->sur_pairs_blocks;

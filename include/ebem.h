@@ -230,6 +230,21 @@ int ebem_mot_solve(ebem_handle *h, const double *beta, double *alpha_out);
 int ebem_pp_eval(ebem_handle *h, const double *density, int npts, const double *x, int nt, const double *t,
                  double *u_out);
 
+/* No counterpart in the reference (its post-processing runs after the
+ * marching).  For callers that post-process at points and times known before
+ * the density: starts the part of ebem_pp_eval(x, t) that does not depend on
+ * the density (point windows, items, the M_m integrals) on a stream and host
+ * thread of its own and returns at once, so that it runs while the caller's
+ * next calls -- ebem_rhs, ebem_mot_solve -- run on the library's stream.
+ * The next ebem_pp_eval with the same points and times (bit for bit) does
+ * only the rest; any other call on the handle waits for the prepared part,
+ * and every call but ebem_rhs, ebem_mot_solve, ebem_pp_eval and the
+ * statistics and timing queries discards it.  Requests of more than one
+ * group of times (t - ceil(t/dt - 1) dt differing by more than 1e-12 dt) or
+ * more than one chunk of points are not prepared (ebem_pp_eval then does all
+ * of it).  The results are the same bit for bit either way. */
+int ebem_pp_prepare(ebem_handle *h, int npts, const double *x, int nt, const double *t);
+
 /* Numeric post-processing of the reference's GPU path
  * (BEMenerg_postProcGPU_setupLayer.m / _calcLayer.m, CUDAkernels/kernelLayer.cu;
  * kernelSP.cu is the same sum for one point): the M(x, tau) of each triangle
@@ -340,8 +355,9 @@ int ebem_set_option(ebem_handle *h, int option, double value);
  * ebem_assemble: out[16] outer-node tasks evaluated, out[17] tasks whose
  * static value was reused from an earlier level, out[18] row chunks
  * evaluated again without reuse after a failed check of the reused values
- * (0 expected). */
-#define EBEM_NUM_TIMINGS 19
+ * (0 expected); out[19] the part of the last ebem_pp_eval prepared by
+ * ebem_pp_prepare, on its own stream (0 if none; out[6] is then the rest). */
+#define EBEM_NUM_TIMINGS 20
 int ebem_timings(ebem_handle *h, double *out, int n);
 
 /* ------------------------------------------------------------------------

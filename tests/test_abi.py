@@ -100,13 +100,13 @@ def test_stats_and_timings_need_a_handle():
     import numpy as np
     from elastodynamicebem_amd import _lib
     lib = _lib.load()
-    out = np.zeros(19)
+    out = np.zeros(20)
     st = np.zeros(5, np.int64)
-    assert lib.ebem_timings(None, out.ctypes.data_as(_lib._dp), 19) < 0
+    assert lib.ebem_timings(None, out.ctypes.data_as(_lib._dp), 20) < 0
     assert lib.ebem_assembly_kp_stats(None, st.ctypes.data_as(_lib._i64p)) < 0
     assert lib.ebem_pp_kp_stats(None, st.ctypes.data_as(_lib._i64p)) < 0
     src = open(os.path.join(ROOT, "include", "ebem.h")).read()
-    assert "#define EBEM_NUM_TIMINGS 19" in src
+    assert "#define EBEM_NUM_TIMINGS 20" in src
     from elastodynamicebem_amd import api
     assert len(api.Solver._KP_KEYS) == 5
     # the double-layer entry points reject a null handle too
@@ -120,6 +120,7 @@ def test_stats_and_timings_need_a_handle():
     assert lib.ebem_beta_k(None, out.ctypes.data_as(_lib._dp), out.ctypes.data_as(_lib._dp)) < 0
     assert lib.ebem_rhs_dirichlet(None, 4, out.ctypes.data_as(_lib._dp)) < 0
     assert lib.ebem_pp_eval_direct(None, None, None, 0, None, 0, None, out.ctypes.data_as(_lib._dp)) < 0
+    assert lib.ebem_pp_prepare(None, 0, None, 0, None) < 0
     assert lib.ebem_neumann_datum(None, 4, out.ctypes.data_as(_lib._dp)) < 0
     assert lib.ebem_beta_v(None, out.ctypes.data_as(_lib._dp), out.ctypes.data_as(_lib._dp)) < 0
     assert lib.ebem_mot_solve_neumann(None, 4, out.ctypes.data_as(_lib._dp)) < 0
