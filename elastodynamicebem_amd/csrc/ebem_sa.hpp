@@ -700,26 +700,46 @@ __host__ __device__ inline int classify(double zeta, double rP, double rS, doubl
     return kNone;
 }
 
-// Child k of the fan around the projection (O, V_k, V_{k+1 mod 3}) in
-// counter-clockwise order (fieldTriangle.m:22-33); false for a degenerate
-// child (|doubled area| < 1e-6), which the reference skips.  Selects keep the
-// fan in registers.
-__host__ __device__ __attribute__((always_inline)) inline bool fan_child(const NodeGeo& G, int k, double& C,
-                                                                         double& v2x, double& v2y, double& v3x,
-                                                                         double& v3y) {
-    C = (k == 0) ? G.c[0] : ((k == 1) ? G.c[1] : G.c[2]);
-    if (fabs(C) < 1.0e-06) return false;
-    v2x = (k == 0) ? G.t[0][0] : ((k == 1) ? G.t[1][0] : G.t[2][0]);
-    v2y = (k == 0) ? G.t[0][1] : ((k == 1) ? G.t[1][1] : G.t[2][1]);
-    v3x = (k == 0) ? G.t[1][0] : ((k == 1) ? G.t[2][0] : G.t[0][0]);
-    v3y = (k == 0) ? G.t[1][1] : ((k == 1) ? G.t[2][1] : G.t[0][1]);
-    if (C < 0.0) {
-        double tx = v2x, ty = v2y;
-        v2x = v3x; v2y = v3y;
-        v3x = tx; v3y = ty;
+// The fan of three children around the projection, child k = (O, V_k,
+// V_{k+1 mod 3}) in counter-clockwise order (fieldTriangle.m:22-33), walked
+// child by child with the vertices and doubled areas rotated in registers, so
+// that a loop over k indexes no array (an indexed fan is kept in scratch
+// memory).  next() gives the child false if it is degenerate (|doubled area|
+// < 1e-6), which the reference skips, and always moves on to the next.
+struct Fan {
+    double ax, ay, bx, by, cx, cy;  // V_k, V_{k+1}, V_{k+2}
+    double ca, cb, cc;              // their doubled child areas c[k], c[k+1], c[k+2]
+    __host__ __device__ __attribute__((always_inline)) explicit Fan(const NodeGeo& G)
+        : ax(G.t[0][0]), ay(G.t[0][1]), bx(G.t[1][0]), by(G.t[1][1]), cx(G.t[2][0]), cy(G.t[2][1]),
+          ca(G.c[0]), cb(G.c[1]), cc(G.c[2]) {}
+    __host__ __device__ __attribute__((always_inline)) bool next(double& C, double& v2x, double& v2y, double& v3x,
+                                                                 double& v3y) {
+        C = ca;
+        v2x = ax;
+        v2y = ay;
+        v3x = bx;
+        v3y = by;
+        const double tx = ax, ty = ay, tc = ca;
+        ax = bx;
+        ay = by;
+        ca = cb;
+        bx = cx;
+        by = cy;
+        cb = cc;
+        cx = tx;
+        cy = ty;
+        cc = tc;
+        if (fabs(C) < 1.0e-06) return false;
+        if (C < 0.0) {
+            const double sx = v2x, sy = v2y;
+            v2x = v3x;
+            v2y = v3y;
+            v3x = sx;
+            v3y = sy;
+        }
+        return true;
     }
-    return true;
-}
+};
 
 // Folds one child's coefficients into the node accumulator: child matrix
 // sum_i b_i B_i (coeffB.m:20-42, symmetric, child frame) rotated by
@@ -760,10 +780,11 @@ __host__ __device__ __attribute__((always_inline)) inline int node_eval(const No
     if (type == kNone) return kNone;
     const double zeta = G.zeta;
     const double sgn = G.sgn;
+    Fan fan(G);
 #pragma unroll 1
     for (int k = 0; k < 3; k++) {
         double C, v2x, v2y, v3x, v3y;
-        if (!fan_child(G, k, C, v2x, v2y, v3x, v3y)) continue;
+        if (!fan.next(C, v2x, v2y, v3x, v3y)) continue;
         ChildGeo g = child_geo(v2x, v2y, v3x, v3y);
         ChildAcc A;
         A.s = 0.0;
@@ -783,10 +804,11 @@ __host__ __device__ __attribute__((always_inline)) inline int node_eval(const No
 __host__ __device__ inline bool node_static_pattern(const NodeGeo& G, const Level& K) {
     double rhoP, rhoS;
     if (classify(G.zeta, K.rP, K.rS, G.dMIN, rhoP, rhoS) != kSPSP) return false;
+    Fan fan(G);
 #pragma unroll 1
     for (int k = 0; k < 3; k++) {
         double C, v2x, v2y, v3x, v3y;
-        if (!fan_child(G, k, C, v2x, v2y, v3x, v3y)) continue;
+        if (!fan.next(C, v2x, v2y, v3x, v3y)) continue;
         const ChildGeo g = child_geo(v2x, v2y, v3x, v3y);
         const FrontAngles f = front_angles(g, rhoP, rhoS, kSPSP);
         if (!(f.TH2 == 0.0 && f.TH3 == 0.0 && f.TH4 == g.ga && f.TH5 == g.ga && g.ga > 0.0)) return false;
@@ -836,10 +858,11 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_kp(cons
     NodeAcc nP, nQ;
     nP.s = nQ.s = 0.0;
     for (int q = 0; q < 6; q++) nP.m[q] = nQ.m[q] = 0.0;
+    Fan fan(G);
 #pragma unroll 1
     for (int k = 0; k < 3; k++) {
         double C, v2x, v2y, v3x, v3y;
-        if (!fan_child(G, k, C, v2x, v2y, v3x, v3y)) continue;
+        if (!fan.next(C, v2x, v2y, v3x, v3y)) continue;
         ChildGeo g = child_geo(v2x, v2y, v3x, v3y);
         // front cutting of child_regions for class kP at KA
         const double a = g.a;
@@ -911,10 +934,11 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_bf(cons
     const double zeta = G.zeta;
     const double zs = zeta * G.sgn;
     const bool zpos = zeta > 1.0e-6;
+    Fan fan(G);
 #pragma unroll 1
     for (int k = 0; k < 3; k++) {
         double Cs, v2x, v2y, v3x, v3y;
-        if (!fan_child(G, k, Cs, v2x, v2y, v3x, v3y)) continue;
+        if (!fan.next(Cs, v2x, v2y, v3x, v3y)) continue;
         const ChildGeo g = child_geo(v2x, v2y, v3x, v3y);
         // front cutting of child_regions by the P circle at t[0]
         const double a = g.a;
