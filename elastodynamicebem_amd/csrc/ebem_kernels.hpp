@@ -1618,49 +1618,81 @@ __global__ void __launch_bounds__(256) k_pp_counts(const int32_t* __restrict__ m
     if (threadIdx.x == 0) cnt[r] = part[0];
 }
 
-// Front-state keys of post-processing items (same 5 x 5 classes as the
-// assembly: P and S front against the span [d - maxL, d + maxL] of the
-// point-triangle distance), pass 1 of the counting sort.
+// Keys of the post-processing items, pass 1 of the counting sort: block b
+// takes the (point, triangle) pairs [b per, (b + 1) per) -- items
+// [off[b per], off[(b + 1) per]) -- and a thread walks one pair's window of m.
+// Items of a kP range get kKpKey (its head) or kKpRestKey, items of a
+// between-front range their head key or kBfRestKey; an item of k_pp_M is one
+// point at one level, keyed by that point's front state (node_task_key, the
+// key of the assembly's node tasks, with the point's projection computed once
+// per pair), so that a wavefront's lanes take the same region branches;
+// points no front reaches (nothing to integrate) get key kNodeKeys.
 __global__ void __launch_bounds__(256) k_pp_keys(const FieldTri* __restrict__ tri, const double* __restrict__ xs,
-                                                 int p0, PPParams P, int64_t nitems, int64_t per,
-                                                 const int32_t* __restrict__ item_q, const int64_t* __restrict__ off,
-                                                 const int32_t* __restrict__ ms, const int32_t* __restrict__ ka,
+                                                 int p0, PPParams P, int64_t npairs, int64_t per,
+                                                 const int64_t* __restrict__ off, const int32_t* __restrict__ ms,
+                                                 const int32_t* __restrict__ me, const int32_t* __restrict__ ka,
                                                  const int32_t* __restrict__ kb, const int32_t* __restrict__ sa,
                                                  const int32_t* __restrict__ sb, uint8_t* __restrict__ key,
                                                  unsigned int* __restrict__ bh) {
     __shared__ unsigned int sh[kNumKeys];
     for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) sh[k] = 0;
     __syncthreads();
-    const int64_t a = blockIdx.x * per;
-    const int64_t e = a + per < nitems ? a + per : nitems;
-    for (int64_t it = a + threadIdx.x; it < e; it += blockDim.x) {
-        const int32_t q = item_q[it];
-        const int p = p0 + q / P.N, j = q % P.N;
-        const int m = ms[q] + (int)(it - off[q]);
-        const FieldTri& F = tri[j];
-        int k;
-        if (ka && m >= ka[q] && m <= kb[q]) {
-            k = (m == ka[q]) ? kKpKey : kKpRestKey;
-        } else if (sa && m >= sa[q] && m <= sb[q]) {
-            k = (m == sa[q]) ? bf_head_key<kPpBfChunk>(sa[q], sb[q]) : kBfRestKey;
-        } else {
-            // an item of k_pp_M is one point at one level: keyed by that
-            // point's front state (node_task_key), so that a wavefront's
-            // lanes take the same region branches; points no front reaches
-            // (nothing to integrate) get key kNodeKeys
-            const double tau = pp_tau(P, m);
-            k = kNodeKeys;
-            if (tau > 0.0) {
-                const double x[3] = {xs[3 * p], xs[3 * p + 1], xs[3 * p + 2]};
-                const int nk = node_task_key(node_geo(x, F), make_level(tau, P.cP, P.cS));
-                if (nk >= 0) k = nk;
+    const int64_t qa = blockIdx.x * per;
+    const int64_t qe = qa + per < npairs ? qa + per : npairs;
+    for (int64_t q = qa + threadIdx.x; q < qe; q += blockDim.x) {
+        const int a = ms[q], b = me[q];
+        if (a > b) continue;
+        const int p = p0 + (int)(q / P.N), j = (int)(q % P.N);
+        const int64_t it0 = off[q];
+        const int kpa = ka ? ka[q] : 1, kpb = ka ? kb[q] : 0;
+        const int bfa = sa ? sa[q] : 1, bfb = sa ? sb[q] : 0;
+        bool have = false;
+        NodeGeo G;
+        for (int m = a; m <= b; m++) {
+            int k;
+            if (m >= kpa && m <= kpb) {
+                k = (m == kpa) ? kKpKey : kKpRestKey;
+            } else if (m >= bfa && m <= bfb) {
+                k = (m == bfa) ? bf_head_key<kPpBfChunk>(bfa, bfb) : kBfRestKey;
+            } else {
+                const double tau = pp_tau(P, m);
+                k = kNodeKeys;
+                if (tau > 0.0) {
+                    if (!have) {
+                        const double x[3] = {xs[3 * p], xs[3 * p + 1], xs[3 * p + 2]};
+                        G = node_geo(x, tri[j]);
+                        have = true;
+                    }
+                    const int nk = node_task_key(G, make_level(tau, P.cP, P.cS));
+                    if (nk >= 0) k = nk;
+                }
             }
+            key[it0 + (m - a)] = (uint8_t)k;
+            atomicAdd(&sh[k], 1u);
         }
-        key[it] = (uint8_t)k;
-        atomicAdd(&sh[k], 1u);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) bh[(int64_t)k * gridDim.x + blockIdx.x] = sh[k];
+}
+
+// Pass 2 for k_pp_keys' partition: block b places the items of its pairs
+// [b per, (b + 1) per), starting from the exclusive offsets bo[key*G + b].
+__global__ void __launch_bounds__(256) k_pp_scatter(int64_t npairs, int64_t per, const int64_t* __restrict__ off,
+                                                    int64_t nitems, const uint8_t* __restrict__ key,
+                                                    const unsigned long long* __restrict__ bo,
+                                                    int32_t* __restrict__ perm) {
+    __shared__ unsigned long long cur[kNumKeys];
+    for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) cur[k] = bo[(int64_t)k * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const int64_t qa = blockIdx.x * per;
+    if (qa >= npairs) return;
+    const int64_t qe = qa + per < npairs ? qa + per : npairs;
+    const int64_t a = off[qa];
+    const int64_t e = qe < npairs ? off[qe] : nitems;
+    for (int64_t it = a + threadIdx.x; it < e; it += blockDim.x) {
+        unsigned long long pos = atomicAdd(&cur[key[it]], 1ull);
+        perm[pos] = (int32_t)it;
+    }
 }
 
 // aT[(j*(nT+1) + 1 + n)*3 + c] = alpha[n*3N + 3j + c] and dT at the same

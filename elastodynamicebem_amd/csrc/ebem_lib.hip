@@ -1588,10 +1588,11 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             hipLaunchKernelGGL(k_pp_items, dim3(np), dim3(256), 0, s0, d_ms.p, d_me.p, N, d_rb.p, d_off.p,
                                d_iq.p);
             HIPC(hipGetLastError());
-            // bucket items by front-state key (two-pass counting sort)
-            const int64_t per = (nitems + G - 1) / G;
-            hipLaunchKernelGGL(k_pp_keys, dim3(G), dim3(256), 0, s0, h->d_tri.p, d_x.p, p0, P, nitems, per,
-                               d_iq.p, d_off.p, d_ms.p, kp ? d_ka.p : nullptr, kp ? d_kb.p : nullptr,
+            // bucket items by front-state key (two-pass counting sort; blocks
+            // take ranges of (point, triangle) pairs)
+            const int64_t per = (tot + G - 1) / G;
+            hipLaunchKernelGGL(k_pp_keys, dim3(G), dim3(256), 0, s0, h->d_tri.p, d_x.p, p0, P, tot, per, d_off.p,
+                               d_ms.p, d_me.p, kp ? d_ka.p : nullptr, kp ? d_kb.p : nullptr,
                                kp ? d_sa.p : nullptr, kp ? d_sb.p : nullptr, d_key.p, d_bh.p);
             HIPC(hipGetLastError());
             HIPC(hipMemcpyAsync(bh.data(), d_bh.p, sizeof(unsigned int) * bh.size(), hipMemcpyDeviceToHost,
@@ -1604,7 +1605,7 @@ int pp_eval(ebem_handle* h, const double* density, int npts, const double* x, in
             }
             HIPC(hipMemcpyAsync(d_bo.p, bo.data(), sizeof(unsigned long long) * bo.size(), hipMemcpyHostToDevice,
                                 s0));
-            hipLaunchKernelGGL(k_item_scatter, dim3(G), dim3(256), 0, s0, nitems, per, d_key.p, d_bo.p,
+            hipLaunchKernelGGL(k_pp_scatter, dim3(G), dim3(256), 0, s0, tot, per, d_off.p, nitems, d_key.p, d_bo.p,
                                d_perm.p);
             HIPC(hipGetLastError());
             // perm: [per-m items | kP range heads | other kP items |
