@@ -45,9 +45,9 @@ ALG_FLOP_PER_ITEM = 6866.8
 # kernels k_node_keys, k_node_scatter, k_node_eval, k_node_reduce and k_levels
 # for the coplanar pairs) from the rocprofv3 --pmc passes FETCH_SIZE (x2.0:
 # the counter reads 512 MiB for a 1 GiB device copy) and WRITE_SIZE (x1.0):
-# 17.0 GB + 9.0 GB, of which k_node_eval 2.3 GB + 6.5 GB (the node slots are
-# 5.5 GB written) and k_node_reduce 14.2 GB + 1.1 GB (the slots read back).
-K_LEVELS_TRAFFIC_BYTES = 26.0e9
+# 17.0 GB + 7.2 GB, of which k_node_eval 2.3 GB + 5.5 GB (the node slots) and
+# k_node_reduce 14.2 GB + 1.1 GB (the slots read back).
+K_LEVELS_TRAFFIC_BYTES = 24.1e9
 K_LEVELS_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this bench's per-level stage "
                            "(profiles/pmc_counters_sphere3_fp64.md), not measured in this run")
 PEAK_FP64_TFLOPS = 78.6
