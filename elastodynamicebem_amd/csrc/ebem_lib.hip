@@ -2905,24 +2905,28 @@ int ebem_pp_prepare(ebem_handle* h, int npts, const double* x, int nt, const dou
     P.t.assign(t, t + (size_t)nt);
     P.err = 0;
     P.go = false;
+    try {
+        P.worker = std::thread([h, npts, nt] {
+            auto& Q = h->prep;
+            {
+                std::unique_lock<std::mutex> g(Q.go_m);
+                Q.go_cv.wait(g, [&Q] { return Q.go; });
+            }
+            hipError_t e0 = hipSetDevice(h->device);
+            const int e = e0 != hipSuccess ? fail(EBEM_EHIP, "hipSetDevice failed: %s", hipGetErrorString(e0))
+                                           : pp_eval(h, nullptr, npts, Q.x.data(), nt, Q.t.data(), nullptr, false,
+                                                     1, Q.stream);
+            if (e) {
+                Q.err = e;
+                Q.msg = g_err;
+            } else {
+                Q.ready = true;
+            }
+        });
+    } catch (const std::exception& ex) {  // no worker: ebem_pp_eval does all of it
+        return fail(EBEM_EINVAL, "ebem_pp_prepare: cannot start its worker thread (%s)", ex.what());
+    }
     P.active = true;
-    P.worker = std::thread([h, npts, nt] {
-        auto& Q = h->prep;
-        {
-            std::unique_lock<std::mutex> g(Q.go_m);
-            Q.go_cv.wait(g, [&Q] { return Q.go; });
-        }
-        hipError_t e0 = hipSetDevice(h->device);
-        const int e = e0 != hipSuccess ? fail(EBEM_EHIP, "hipSetDevice failed: %s", hipGetErrorString(e0))
-                                       : pp_eval(h, nullptr, npts, Q.x.data(), nt, Q.t.data(), nullptr, false, 1,
-                                                 Q.stream);
-        if (e) {
-            Q.err = e;
-            Q.msg = g_err;
-        } else {
-            Q.ready = true;
-        }
-    });
     return 0;
 }
 
