@@ -893,9 +893,10 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_kp(cons
 }
 
 // Levels sharing one evaluation of the level-independent region terms in
-// node_eval_bf for the assembly's between-front ranges (k_levels_bf: 6 at one
-// wave per SIMD runs without the spills of 8, 9.3 against 10.3 ms on the
-// 1280-triangle sphere; 4 is slower, 10.7 ms).
+// node_eval_bf for the assembly's between-front ranges (k_levels_bf at one wave
+// per SIMD, 1280-triangle sphere: 6 levels 9.0 ms with 292 B/lane of spills,
+// 8 levels 10.3 ms with 704 B/lane, 5 levels 9.6 ms, 4 levels without spills
+// 10.7 ms: the shared level-independent terms against the spills).
 #ifndef EBEM_BF_CHUNK
 #define EBEM_BF_CHUNK 6
 #endif
