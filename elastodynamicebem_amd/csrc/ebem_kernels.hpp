@@ -260,11 +260,17 @@ constexpr int kFrontKeys = kFrontClasses * kFrontClasses;
 // ranges take similar time), surrogate pairs last.
 constexpr int kCopKeys = kFrontKeys;     // kFrontKeys .. 2 kFrontKeys - 1
 constexpr int kNearKeys = 2 * kFrontKeys;  // .. 3 kFrontKeys - 1
-constexpr int kKpKey = 3 * kFrontKeys;
-constexpr int kKpRestKey = kKpKey + 1;
+// Range heads of far-field pairs (mixed precision, region integrals in FP32)
+// precede those of the other pairs, and so do their other levels.
+constexpr int kKpFarKey = 3 * kFrontKeys;
+constexpr int kKpKey = kKpFarKey + 1;
+constexpr int kKpFarRestKey = kKpKey + 1;
+constexpr int kKpRestKey = kKpFarRestKey + 1;
 constexpr int kBfHeadKeys = 4;
-constexpr int kBfKey = kKpKey + 2;  // .. kBfKey + kBfHeadKeys - 1
-constexpr int kBfRestKey = kBfKey + kBfHeadKeys;
+constexpr int kBfFarKey = kKpRestKey + 1;  // .. kBfFarKey + kBfHeadKeys - 1
+constexpr int kBfKey = kBfFarKey + kBfHeadKeys;  // .. kBfKey + kBfHeadKeys - 1
+constexpr int kBfFarRestKey = kBfKey + kBfHeadKeys;
+constexpr int kBfRestKey = kBfFarRestKey + 1;
 constexpr int kSurKey = kBfRestKey + 1;
 constexpr int kNumKeys = kSurKey + 1;
 
@@ -278,9 +284,9 @@ constexpr int kPpBfChunk = EBEM_PP_BF_CHUNK;
 // Range key by the number of chunks of C levels (waves of ranges with equal
 // chunk counts run convergent).
 template <int C>
-__device__ inline int bf_head_key(int a, int b) {
+__device__ inline int bf_head_key(int a, int b, int base = kBfKey) {
     const int chunks = (b - a + C) / C;
-    return kBfKey + (chunks < kBfHeadKeys ? chunks : kBfHeadKeys) - 1;
+    return base + (chunks < kBfHeadKeys ? chunks : kBfHeadKeys) - 1;
 }
 static_assert(kNumKeys <= 256, "keys are stored as uint8");
 
@@ -337,9 +343,13 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
         int qp = front_class(P.cP * t, d - ex, 2.0 * ex) + 1;
         int qs = front_class(P.cS * t, d - ex, 2.0 * ex) + 1;
         const int near_key = qp * kFrontClasses + qs + (coplanar_pair(S, F) ? kCopKeys : kNearKeys);
-        int k = (P.far2 >= 0.0 && d2 > P.far2) ? qp * kFrontClasses + qs : near_key;
-        if (ka && m >= ka[p] && m <= kb[p]) k = (m == ka[p]) ? kKpKey : kKpRestKey;
-        if (sa && m >= sa[p] && m <= sb[p]) k = (m == sa[p]) ? bf_head_key<kBfChunk>(sa[p], sb[p]) : kBfRestKey;
+        const bool far = P.far2 >= 0.0 && d2 > P.far2;
+        int k = far ? qp * kFrontClasses + qs : near_key;
+        if (ka && m >= ka[p] && m <= kb[p])
+            k = (m == ka[p]) ? (far ? kKpFarKey : kKpKey) : (far ? kKpFarRestKey : kKpRestKey);
+        if (sa && m >= sa[p] && m <= sb[p])
+            k = (m == sa[p]) ? bf_head_key<kBfChunk>(sa[p], sb[p], far ? kBfFarKey : kBfKey)
+                             : (far ? kBfFarRestKey : kBfRestKey);
         // Surrogate pairs: the network predicts the lags max(lo,1)..hi-1; the
         // analytic routine (FP64, level by level) keeps E_0 = W_1 / (4 pi rho),
         // which the marching inverts, and the static level hi+1, which the
@@ -443,7 +453,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_L
 // then keys the chunk again without reuse.
 constexpr int kNodeKeys = 48;
 // post-processing items keyed by point state: keys 0..kNodeKeys, below the range keys
-static_assert(kNodeKeys < kKpKey, "point-state keys must precede the range keys");
+static_assert(kNodeKeys < kKpFarKey, "point-state keys must precede the range keys");
 constexpr uint8_t kNodeSkip = 255;  // no front reaches the node: nothing to add
 // Reused task: key kNodeReuse + d, its value is the slot of the item d levels
 // before it (1 <= d <= kNodeReuseMax); a task further than kNodeReuseMax
@@ -647,12 +657,13 @@ __global__ void __launch_bounds__(256) k_node_reduce(const FieldTri* __restrict_
     for (int e = 0; e < 6; e++) w[e] = Wv[e];
 }
 
-// One thread per kP range (the items of key kKpKey, the first level ka of the
+// One thread per kP range (the items of key kKpFarKey or kKpKey, the first level ka of the
 // range): WP and WQ of node_eval_kp at the ends ka, kb, then
 // W_m = WP + t_m^2 WQ for m = ka..kb into the range's W slots (consecutive
 // items of the pair).  If some node is not in the assumed state, the range is
-// evaluated level by level as in k_levels.
-template <int NGH>
+// evaluated level by level as in k_levels.  Region integrals in R (FP32 for
+// the far-field pairs of the mixed-precision form).
+template <int NGH, typename R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_LEVELS_WAVES))) k_levels_kp(
     const FieldTri* __restrict__ tri, const double* __restrict__ gh, AsmParams P, int r0, int64_t nitems,
     const int32_t* __restrict__ item_pair, const int16_t* __restrict__ ka, const int16_t* __restrict__ kb,
@@ -675,7 +686,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_L
         for (int q = 0; q < NGH && ok; q++) {
             double x[3] = {g[q], g[32 + q], g[64 + q]};
             NodeGeo G = node_geo(x, F);
-            ok = node_eval_kp(G, KA, KB, g[96 + q], aP, aQ);
+            ok = node_eval_kp<R>(G, KA, KB, g[96 + q], aP, aQ);
         }
         if (ok) {
             double WP[6], WQ[6];
@@ -699,7 +710,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_L
                 for (int q = 0; q < NGH; q++) {
                     double x[3] = {g[q], g[32 + q], g[64 + q]};
                     NodeGeo G = node_geo(x, F);
-                    node_eval<double>(G, K, g[96 + q], acc);
+                    node_eval<R>(G, K, g[96 + q], acc);
                 }
                 double Wv[6];
                 finish_rotation(acc, F.C, Wv);
@@ -719,12 +730,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_L
 #define EBEM_BF_WAVES 1
 #endif
 
-// One thread per between-front range (the items of key kBfKey, the first
+// One thread per between-front range (the head items of kBfFarKey or kBfKey, the first
 // level sa of the range): levels sa..sb in chunks of kBfChunk through
 // node_eval_bf, which yields k_levels' values with the level-independent
 // region terms evaluated once per chunk.  From a chunk whose state check
 // fails, the rest of the range is evaluated level by level as in k_levels.
-template <int NGH>
+// Region integrals in R, as in k_levels_kp.
+template <int NGH, typename R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_BF_WAVES))) k_levels_bf(
     const FieldTri* __restrict__ tri, const double* __restrict__ gh, AsmParams P, int r0, int64_t nitems,
     const int32_t* __restrict__ item_pair, const int16_t* __restrict__ sa, const int16_t* __restrict__ sb,
@@ -754,7 +766,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_B
             for (int q = 0; q < NGH && ok; q++) {
                 double x[3] = {g[q], g[32 + q], g[64 + q]};
                 NodeGeo G = node_geo(x, F);
-                ok = node_eval_bf<kBfChunk>(G, t, n, P.cP, P.cS, g[96 + q], acc);
+                ok = node_eval_bf<kBfChunk, R>(G, t, n, P.cP, P.cS, g[96 + q], acc);
             }
             if (!ok) break;
 #pragma unroll
@@ -778,7 +790,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_B
                 for (int q = 0; q < NGH; q++) {
                     double x[3] = {g[q], g[32 + q], g[64 + q]};
                     NodeGeo G = node_geo(x, F);
-                    node_eval<double>(G, K, g[96 + q], acc);
+                    node_eval<R>(G, K, g[96 + q], acc);
                 }
                 double Wv[6];
                 finish_rotation(acc, F.C, Wv);

@@ -1064,23 +1064,30 @@ int assemble(ebem_handle* h, int precision) {
             HIPC(hipGetLastError());
             const int32_t* perm = (h->sort_items || mixed || sur) ? d_perm.p : nullptr;
             int64_t nfar = 0, ncop = 0, nsur = 0, nkp = 0, nkrest = 0, nbf = 0, nbrest = 0;
+            int64_t nkpfar = 0, nkfrest = 0, nbffar = 0, nbfrest = 0;  // far-field ranges (FP32)
             for (size_t k = 0; k < (size_t)kFrontKeys * G; k++) nfar += hist[k];
             for (size_t k = (size_t)kCopKeys * G; k < (size_t)kNearKeys * G; k++) ncop += hist[k];
             for (int b = 0; b < G; b++) {
                 nsur += hist[(size_t)kSurKey * G + b];
-                nkp += hist[(size_t)kKpKey * G + b];
-                nkrest += hist[(size_t)kKpRestKey * G + b];
-                for (int c = 0; c < kBfHeadKeys; c++) nbf += hist[(size_t)(kBfKey + c) * G + b];
-                nbrest += hist[(size_t)kBfRestKey * G + b];
+                nkpfar += hist[(size_t)kKpFarKey * G + b];
+                nkfrest += hist[(size_t)kKpFarRestKey * G + b];
+                nkp += hist[(size_t)kKpFarKey * G + b] + hist[(size_t)kKpKey * G + b];
+                nkrest += hist[(size_t)kKpFarRestKey * G + b] + hist[(size_t)kKpRestKey * G + b];
+                for (int c = 0; c < kBfHeadKeys; c++) {
+                    nbffar += hist[(size_t)(kBfFarKey + c) * G + b];
+                    nbf += hist[(size_t)(kBfFarKey + c) * G + b] + hist[(size_t)(kBfKey + c) * G + b];
+                }
+                nbfrest += hist[(size_t)kBfFarRestKey * G + b];
+                nbrest += hist[(size_t)kBfFarRestKey * G + b] + hist[(size_t)kBfRestKey * G + b];
             }
-            h->far_items += nfar;
+            h->far_items += nfar + nkpfar + nkfrest + nbffar + nbfrest;
             h->sur_items += nsur;
             h->kp_stats[0] += nkp;
             h->kp_stats[1] += nkp + nkrest;
             h->kp_stats[2] += nbf;
             h->kp_stats[3] += nbf + nbrest;
-            // perm: [per-level items | kP range heads | other kP levels |
-            //        between-front range heads | their other levels | surrogate items]
+            // perm: [per-level items | kP range heads (far ones first) | other kP levels |
+            //        between-front range heads (far ones first) | their other levels | surrogate items]
             const int64_t nlev = acc - nsur - nkp - nkrest - nbf - nbrest;
             tlev.start(h->stream);
             if (ghc) {
@@ -1122,36 +1129,60 @@ int assemble(ebem_handle* h, int precision) {
             lev_ms += tlev.ms();
             if (nkp > 0) {
                 tkp.start(h->stream);
-                const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((nkp + 255) / 256, (int64_t)h->CUs * 16));
-#define KP_LAUNCH(NG)                                                                                              \
-    hipLaunchKernelGGL(k_levels_kp<NG>, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, \
-                       r0, nkp, d_ip.p, ka, kb, d_perm.p + nlev, d_W.p, d_nfb.p)
-                switch (h->nGH) {
-                    case 3: KP_LAUNCH(3); break;
-                    case 7: KP_LAUNCH(7); break;
-                    case 12: KP_LAUNCH(12); break;
-                    default: KP_LAUNCH(19); break;
-                }
+                // far-field ranges (mixed precision) in FP32, then the rest in FP64
+                for (int part = 0; part < 2; part++) {
+                    const int64_t n = part ? nkp - nkpfar : nkpfar;
+                    if (n == 0) continue;
+                    const int32_t* pk = d_perm.p + nlev + (part ? nkpfar : 0);
+                    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)h->CUs * 16));
+#define KP_LAUNCH(NG, R)                                                                                              \
+    hipLaunchKernelGGL((k_levels_kp<NG, R>), dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, \
+                       P, r0, n, d_ip.p, ka, kb, pk, d_W.p, d_nfb.p)
+#define KP_LAUNCH_R(R)                    \
+    switch (h->nGH) {                     \
+        case 3: KP_LAUNCH(3, R); break;   \
+        case 7: KP_LAUNCH(7, R); break;   \
+        case 12: KP_LAUNCH(12, R); break; \
+        default: KP_LAUNCH(19, R); break; \
+    }
+                    if (part) {
+                        KP_LAUNCH_R(double);
+                    } else {
+                        KP_LAUNCH_R(float);
+                    }
+#undef KP_LAUNCH_R
 #undef KP_LAUNCH
-                HIPC(hipGetLastError());
+                    HIPC(hipGetLastError());
+                }
                 tkp.stop(h->stream);
                 kp_ms += tkp.ms();
             }
             if (nbf > 0) {
                 tbf.start(h->stream);
-                const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((nbf + 255) / 256, (int64_t)h->CUs * 16));
-                const int32_t* pb = d_perm.p + nlev + nkp + nkrest;
-#define BF_LAUNCH(NG)                                                                                              \
-    hipLaunchKernelGGL(k_levels_bf<NG>, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, P, \
-                       r0, nbf, d_ip.p, sa, sb, pb, d_W.p, d_nfb.p)
-                switch (h->nGH) {
-                    case 3: BF_LAUNCH(3); break;
-                    case 7: BF_LAUNCH(7); break;
-                    case 12: BF_LAUNCH(12); break;
-                    default: BF_LAUNCH(19); break;
-                }
+                for (int part = 0; part < 2; part++) {
+                    const int64_t n = part ? nbf - nbffar : nbffar;
+                    if (n == 0) continue;
+                    const int32_t* pb = d_perm.p + nlev + nkp + nkrest + (part ? nbffar : 0);
+                    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)h->CUs * 16));
+#define BF_LAUNCH(NG, R)                                                                                              \
+    hipLaunchKernelGGL((k_levels_bf<NG, R>), dim3((unsigned)blocks), dim3(256), 0, h->stream, h->d_tri.p, h->d_gh.p, \
+                       P, r0, n, d_ip.p, sa, sb, pb, d_W.p, d_nfb.p)
+#define BF_LAUNCH_R(R)                    \
+    switch (h->nGH) {                     \
+        case 3: BF_LAUNCH(3, R); break;   \
+        case 7: BF_LAUNCH(7, R); break;   \
+        case 12: BF_LAUNCH(12, R); break; \
+        default: BF_LAUNCH(19, R); break; \
+    }
+                    if (part) {
+                        BF_LAUNCH_R(double);
+                    } else {
+                        BF_LAUNCH_R(float);
+                    }
+#undef BF_LAUNCH_R
 #undef BF_LAUNCH
-                HIPC(hipGetLastError());
+                    HIPC(hipGetLastError());
+                }
                 tbf.stop(h->stream);
                 bf_ms += tbf.ms();
             }

@@ -88,6 +88,11 @@ __host__ __device__ inline ChildGeoT<R> cast_geo(const ChildGeo& g) {
                         R(g.al), R(g.be), R(g.ga)};
 }
 
+template <typename R>
+__host__ __device__ inline LevelT<R> cast_level(const Level& K) {
+    return LevelT<R>{R(K.t), R(K.rP), R(K.rS), R(K.k1), R(K.k3), R(K.k5), R(K.k2), R(K.k4), R(K.k6)};
+}
+
 __host__ __device__ inline double clamp1(double x) { return fmin(1.0, fmax(-1.0, x)); }
 __host__ __device__ inline double sqrt0(double x) { return x < 0.0 ? 0.0 : sqrt(x); }
 __host__ __device__ inline double msign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
@@ -840,6 +845,8 @@ __host__ __device__ inline bool node_static_margin(const NodeGeo& G, const Level
 // and k3 = k4 = 0, accQ with k1 = k2 = 0 and (k3, k4) / t^2 = (-1/2, 3/2).
 // The class is checked at both ends and the covering at KA (it only grows
 // with t); returns false, leaving accP and accQ untouched, if either fails.
+// The region integrals run in R (the front cutting and the sums stay FP64).
+template <typename R = double>
 __host__ __device__ __attribute__((always_inline)) inline bool node_eval_kp(const NodeGeo& G, const Level& KA,
                                                                             const Level& KB, double w,
                                                                             NodeAcc& accP, NodeAcc& accQ) {
@@ -873,12 +880,13 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_kp(cons
         const double as = asin(clamp1(a * g.sb / rhoP));
         const double O1 = as - g.be, O2 = kPi - as - g.be;
         if (fmin(fmax(0.0, fmin(O1, O2)), g.ga) != 0.0 || fmax(0.0, fmin(fmax(O1, O2), g.ga)) != g.ga) return false;
-        const Phi<double> P = make_phi<double>(0.0, g.ga, g);
+        const ChildGeoT<R> gr = cast_geo<R>(g);
+        const Phi<R> P = make_phi<R>(R(0.0), gr.ga, gr);
         ChildAcc AP, AQ;
         AP.s = AQ.s = 0.0;
         for (int i = 0; i < 6; i++) AP.b[i] = AQ.b[i] = 0.0;
-        region_edgeP<double>(false, KA.rS, -1.0, zeta, 0.0, g.ga, g, P, KPw, AP);
-        region_edgeP<double>(false, KA.rS, -1.0, zeta, 0.0, g.ga, g, P, KQw, AQ);
+        region_edgeP<R>(false, R(KA.rS), R(-1.0), R(zeta), R(0.0), gr.ga, gr, P, cast_level<R>(KPw), AP);
+        region_edgeP<R>(false, R(KA.rS), R(-1.0), R(zeta), R(0.0), gr.ga, gr, P, cast_level<R>(KQw), AQ);
         const double ws = w * msign(C);
         add_child(AP, g, v2x, v2y, zs, ws, nP);
         add_child(AQ, g, v2x, v2y, zs, ws, nQ);
@@ -912,8 +920,9 @@ constexpr int kBfChunk = EBEM_BF_CHUNK;
 // geometry, angular integrals, edge functions) are evaluated once per child
 // and finished per level with node_eval's own expressions, so acc[l] receives
 // exactly what node_eval adds at t[l].  Returns false (acc then partly
-// updated) if a level's class or the covering check does not hold.
-template <int C>
+// updated) if a level's class or the covering check does not hold.  The
+// region integrals run in R (the front cutting and the sums stay FP64).
+template <int C, typename R = double>
 __host__ __device__ __attribute__((always_inline)) inline bool node_eval_bf(const NodeGeo& G, const double* t, int n,
                                                                             double cP, double cS, double w,
                                                                             NodeAcc* acc) {
@@ -950,10 +959,11 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_bf(cons
         const double as = asin(clamp1(a * g.sb / rhoP0));
         const double O1 = as - g.be, O2 = kPi - as - g.be;
         if (fmin(fmax(0.0, fmin(O1, O2)), g.ga) != 0.0 || fmax(0.0, fmin(fmax(O1, O2), g.ga)) != g.ga) return false;
-        const Phi<double> P = make_phi<double>(0.0, g.ga, g);
+        const ChildGeoT<R> gr = cast_geo<R>(g);
+        const Phi<R> P = make_phi<R>(R(0.0), gr.ga, gr);
         const double ws = w * msign(Cs);
         if (zpos) {
-            const EdgePre<double> X = edge_pre<double>(zeta, 0.0, g.ga, g);
+            const EdgePre<R> X = edge_pre<R>(R(zeta), R(0.0), gr.ga, gr);
 #pragma unroll
             for (int l = 0; l < C; l++) {
                 if (l < n) {
@@ -961,12 +971,12 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_bf(cons
                     ChildAcc A;
                     A.s = 0.0;
                     for (int i = 0; i < 6; i++) A.b[i] = 0.0;
-                    edge_fin<double>(type[l] != kP, K.rS, rhoS[l], zeta, X, g, P, K, A);
+                    edge_fin<R>(type[l] != kP, R(K.rS), R(rhoS[l]), R(zeta), X, gr, P, cast_level<R>(K), A);
                     add_child(A, g, v2x, v2y, zs, ws, acc[l]);
                 }
             }
         } else {
-            const CopPre<double> X = cop_pre<double>(0.0, g.ga, g);
+            const CopPre<R> X = cop_pre<R>(R(0.0), gr.ga, gr);
 #pragma unroll
             for (int l = 0; l < C; l++) {
                 if (l < n) {
@@ -974,7 +984,7 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_bf(cons
                     ChildAcc A;
                     A.s = 0.0;
                     for (int i = 0; i < 6; i++) A.b[i] = 0.0;
-                    cop_fin<double>(K.rS, X, P, K, A);
+                    cop_fin<R>(R(K.rS), X, P, cast_level<R>(K), A);
                     add_child(A, g, v2x, v2y, zs, ws, acc[l]);
                 }
             }

@@ -1,9 +1,10 @@
 """Mixed-precision form (EBEM_PREC_MIXED) on the device, 320-triangle sphere,
-T = 4, 64 steps: only front-crossing items of far-field pairs (centroid
-distance > 3 h, h the longest edge) run in FP32.  So every 3x3 block of a
-near pair is bit-identical to the FP64 assembly, far blocks differ at FP32
-level, and the density stays within 1e-3 of the FP64 one (measured 1.5e-4
-here, 1e-3 on the 1280-triangle sphere: the closed forms cancel in FP32)."""
+T = 4, 64 steps: only the items of far-field pairs (centroid distance > 3 h,
+h the longest edge) have their region integrals in FP32, per level and in
+kP and between-front ranges.  So every 3x3 block of a near pair is
+bit-identical to the FP64 assembly, far blocks differ at FP32 level, and the
+density stays within 1e-3 of the FP64 one (measured 1.6e-4 here, 1.4e-3 on
+the 1280-triangle sphere: the closed forms cancel in FP32)."""
 import os
 
 import numpy as np

@@ -1,7 +1,8 @@
 """The mixed-precision form on its own workload: the 1280-triangle sphere
 (sphereUniform_3), T = 4, 128 steps, datum n t^2 e^{-4t}, post-processing at
 the 4096 seeded shell points at every step, far field = centroid distance
-above 3h in FP32.
+above 3h: its per-level items, kP ranges and between-front ranges with the
+region integrals in FP32.
 
 The form does not reach 1e-6 in the density, and these tests state why with
 measured numbers (each printed):
@@ -13,7 +14,7 @@ measured numbers (each printed):
 * the FP32 closed forms themselves lose 3-4 digits in W (the level
   integrals whose second difference is E_l): their differences of nearly
   equal terms are ill-conditioned, as the reference's FP64 results' own
-  one-ulp sensitivity shows.  The density error is ~1e-3 at 3h and falls only
+  one-ulp sensitivity shows.  The density error is ~1.4e-3 at 3h and falls only
   slowly when fewer pairs go to FP32 (the Pareto points asserted below).
 Near-field blocks are bit-identical to the plain form at every threshold."""
 import os
@@ -78,8 +79,9 @@ def test_mixed_cfg2_errors_and_pareto():
             assert np.array_equal(rows[k][:, :, near], rows64[k][:, :, near]), (f, r)
     # no FP32: the plain form
     assert errs[0.0][0] == 0.0 and errs[0.0][2] == 0
-    # the default (config) threshold: measured 9.84e-4 / 3.96e-5
-    assert errs[3.0][0] < 1.5e-3 and errs[3.0][1] < 1e-4
+    # the default (config) threshold: measured 1.44e-3 / 5.42e-5 (9.84e-4 /
+    # 3.96e-5 with the far pairs' kP and between-front ranges in FP64)
+    assert errs[3.0][0] < 2e-3 and errs[3.0][1] < 1e-4
     # Pareto: fewer FP32 pairs, smaller error -- but never 1e-6
     assert errs[3.0][2] > errs[6.0][2] > errs[10.0][2] > 0
     assert errs[10.0][0] < errs[3.0][0]
