@@ -64,10 +64,10 @@ void sa_kp_range(const double* rec, const double* gh, const double* vF, int nGH,
 }
 
 // W (6 values per level, as sa_item) of the levels m0 .. m0+n-1 by
-// node_eval_bf in chunks of kBfChunk, as k_levels_bf computes them; returns
-// 0 if a chunk fails the state check.
+// node_eval_bf in chunks of kBfChunk, as k_levels_bf computes them (region
+// integrals in FP32 if fp32); returns 0 if a chunk fails the state check.
 int sa_item_bf(const double* rec, const double* gh, int nGH, int m0, int n, double dt, double cP, double cS,
-               double* W) {
+               double* W, int fp32) {
     FieldTri F;
     std::memcpy(&F, rec, sizeof(FieldTri));
     for (int c0 = 0; c0 < n; c0 += kBfChunk) {
@@ -82,17 +82,20 @@ int sa_item_bf(const double* rec, const double* gh, int nGH, int m0, int n, doub
         for (int q = 0; q < nGH; q++) {
             double x[3] = {gh[q], gh[32 + q], gh[64 + q]};
             NodeGeo G = node_geo(x, F);
-            if (!node_eval_bf<kBfChunk>(G, t, nc, cP, cS, gh[96 + q], acc)) return 0;
+            const bool ok = fp32 ? node_eval_bf<kBfChunk, float>(G, t, nc, cP, cS, gh[96 + q], acc)
+                                 : node_eval_bf<kBfChunk, double>(G, t, nc, cP, cS, gh[96 + q], acc);
+            if (!ok) return 0;
         }
         for (int l = 0; l < nc; l++) finish_rotation(acc[l], F.C, W + 6 * (c0 + l));
     }
     return 1;
 }
 
-// WP, WQ of a kP range [ta, tb] as k_levels_kp computes them; returns 0 if a
-// node fails the state check (k_levels_kp then evaluates level by level).
+// WP, WQ of a kP range [ta, tb] as k_levels_kp computes them (region
+// integrals in FP32 if fp32); returns 0 if a node fails the state check
+// (k_levels_kp then evaluates level by level).
 int sa_item_kp(const double* rec, const double* gh, int nGH, double ta, double tb, double cP, double cS,
-               double* WP6, double* WQ6) {
+               double* WP6, double* WQ6, int fp32) {
     FieldTri F;
     std::memcpy(&F, rec, sizeof(FieldTri));
     Level KA = make_level(ta, cP, cS), KB = make_level(tb, cP, cS);
@@ -102,7 +105,9 @@ int sa_item_kp(const double* rec, const double* gh, int nGH, double ta, double t
     for (int q = 0; q < nGH; q++) {
         double x[3] = {gh[q], gh[32 + q], gh[64 + q]};
         NodeGeo G = node_geo(x, F);
-        if (!node_eval_kp(G, KA, KB, gh[96 + q], aP, aQ)) return 0;
+        const bool ok = fp32 ? node_eval_kp<float>(G, KA, KB, gh[96 + q], aP, aQ)
+                             : node_eval_kp<double>(G, KA, KB, gh[96 + q], aP, aQ);
+        if (!ok) return 0;
     }
     finish_rotation(aP, F.C, WP6);
     finish_rotation(aQ, F.C, WQ6);
