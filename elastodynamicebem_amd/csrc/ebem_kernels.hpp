@@ -348,7 +348,8 @@ __global__ void __launch_bounds__(256) k_item_keys(const FieldTri* __restrict__ 
         if (ka && m >= ka[p] && m <= kb[p])
             k = (m == ka[p]) ? (far ? kKpFarKey : kKpKey) : (far ? kKpFarRestKey : kKpRestKey);
         if (sa && m >= sa[p] && m <= sb[p])
-            k = (m == sa[p]) ? bf_head_key<kBfChunk>(sa[p], sb[p], far ? kBfFarKey : kBfKey)
+            k = (m == sa[p]) ? (far ? bf_head_key<kBfChunkF32>(sa[p], sb[p], kBfFarKey)
+                                    : bf_head_key<kBfChunk>(sa[p], sb[p], kBfKey))
                              : (far ? kBfFarRestKey : kBfRestKey);
         // Surrogate pairs: the network predicts the lags max(lo,1)..hi-1; the
         // analytic routine (FP64, level by level) keeps E_0 = W_1 / (4 pi rho),
@@ -749,15 +750,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_B
         const int ma = sa[p], mb = sb[p];
         const FieldTri& F = tri[j];
         const double* g = gh + (int64_t)i * 4 * 32;
+        constexpr int CH = sizeof(R) == 4 ? kBfChunkF32 : kBfChunk;
         int m = ma;
-        for (; m <= mb; m += kBfChunk) {
-            const int n = mb - m + 1 < kBfChunk ? mb - m + 1 : kBfChunk;
-            double t[kBfChunk];
+        for (; m <= mb; m += CH) {
+            const int n = mb - m + 1 < CH ? mb - m + 1 : CH;
+            double t[CH];
 #pragma unroll
-            for (int l = 0; l < kBfChunk; l++) t[l] = (double)(m + (l < n ? l : 0)) * P.dt;
-            NodeAcc acc[kBfChunk];
+            for (int l = 0; l < CH; l++) t[l] = (double)(m + (l < n ? l : 0)) * P.dt;
+            NodeAcc acc[CH];
 #pragma unroll
-            for (int l = 0; l < kBfChunk; l++) {
+            for (int l = 0; l < CH; l++) {
                 acc[l].s = 0.0;
                 for (int q = 0; q < 6; q++) acc[l].m[q] = 0.0;
             }
@@ -766,11 +768,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EBEM_B
             for (int q = 0; q < NGH && ok; q++) {
                 double x[3] = {g[q], g[32 + q], g[64 + q]};
                 NodeGeo G = node_geo(x, F);
-                ok = node_eval_bf<kBfChunk, R>(G, t, n, P.cP, P.cS, g[96 + q], acc);
+                ok = node_eval_bf<CH, R>(G, t, n, P.cP, P.cS, g[96 + q], acc);
             }
             if (!ok) break;
 #pragma unroll
-            for (int l = 0; l < kBfChunk; l++) {
+            for (int l = 0; l < CH; l++) {
                 if (l < n) {
                     double Wv[6];
                     finish_rotation(acc[l], F.C, Wv);

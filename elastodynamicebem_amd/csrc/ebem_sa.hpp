@@ -909,6 +909,14 @@ __host__ __device__ __attribute__((always_inline)) inline bool node_eval_kp(cons
 #define EBEM_BF_CHUNK 6
 #endif
 constexpr int kBfChunk = EBEM_BF_CHUNK;
+// The same for the FP32 ranges of the mixed-precision form's far field
+// (k_levels_bf<7, float>, 1280-triangle sphere: 10 levels 5.5 ms without
+// spills, 6 levels 6.2 ms, 8 levels 6.2 ms with 100 B/lane of spills, 12
+// levels 6.7 ms with 128 B/lane).
+#ifndef EBEM_BF_CHUNK_F32
+#define EBEM_BF_CHUNK_F32 10
+#endif
+constexpr int kBfChunkF32 = EBEM_BF_CHUNK_F32;
 
 // Outer point between the fronts at each of the n <= C levels t[0..n) (in
 // increasing order): the P circle covers each child's far edge (TH2 = 0 and

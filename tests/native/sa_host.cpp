@@ -64,25 +64,28 @@ void sa_kp_range(const double* rec, const double* gh, const double* vF, int nGH,
 }
 
 // W (6 values per level, as sa_item) of the levels m0 .. m0+n-1 by
-// node_eval_bf in chunks of kBfChunk, as k_levels_bf computes them (region
+// node_eval_bf in chunks of levels, as k_levels_bf computes them (region
 // integrals in FP32 if fp32); returns 0 if a chunk fails the state check.
 int sa_item_bf(const double* rec, const double* gh, int nGH, int m0, int n, double dt, double cP, double cS,
                double* W, int fp32) {
     FieldTri F;
     std::memcpy(&F, rec, sizeof(FieldTri));
-    for (int c0 = 0; c0 < n; c0 += kBfChunk) {
-        const int nc = n - c0 < kBfChunk ? n - c0 : kBfChunk;
-        double t[kBfChunk];
-        for (int l = 0; l < kBfChunk; l++) t[l] = (double)(m0 + c0 + (l < nc ? l : 0)) * dt;
-        NodeAcc acc[kBfChunk];
-        for (int l = 0; l < kBfChunk; l++) {
+    // chunks of kBfChunk levels in FP64, of kBfChunkF32 in FP32 (both <= kMaxChunk)
+    constexpr int kMaxChunk = kBfChunk > kBfChunkF32 ? kBfChunk : kBfChunkF32;
+    const int CH = fp32 ? kBfChunkF32 : kBfChunk;
+    for (int c0 = 0; c0 < n; c0 += CH) {
+        const int nc = n - c0 < CH ? n - c0 : CH;
+        double t[kMaxChunk];
+        for (int l = 0; l < kMaxChunk; l++) t[l] = (double)(m0 + c0 + (l < nc ? l : 0)) * dt;
+        NodeAcc acc[kMaxChunk];
+        for (int l = 0; l < kMaxChunk; l++) {
             acc[l].s = 0.0;
             for (int q = 0; q < 6; q++) acc[l].m[q] = 0.0;
         }
         for (int q = 0; q < nGH; q++) {
             double x[3] = {gh[q], gh[32 + q], gh[64 + q]};
             NodeGeo G = node_geo(x, F);
-            const bool ok = fp32 ? node_eval_bf<kBfChunk, float>(G, t, nc, cP, cS, gh[96 + q], acc)
+            const bool ok = fp32 ? node_eval_bf<kBfChunkF32, float>(G, t, nc, cP, cS, gh[96 + q], acc)
                                  : node_eval_bf<kBfChunk, double>(G, t, nc, cP, cS, gh[96 + q], acc);
             if (!ok) return 0;
         }
